@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Throughput of the batched (omega,k)-solve at block 192.
+
+Workload (one GPU): Si|Ge interface film, 4x2 conventional-cell cross-section
+(192 dof per principal layer), 40 device layers, transverse k on an 8x8 grid,
+1024 frequencies in (0, 16] THz.  One step = the full hot path (four lead
+decimations' worth of surface GFs needed for T, both RGF sweeps and Caroli T)
+for all 1024 frequencies of one k-point, inputs resident in HBM.
+
+Prints one JSON line (see the repository's measurement contract).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6, "mixed": 157.3}     # MI355X dense MFMA TFLOP/s (FP64, FP32)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kgrid(n):
+    """Monkhorst-Pack style grid centred like negf/config.f90:156-175."""
+    ks = []
+    for i in range(n):
+        for j in range(n):
+            ks.append((i / n - 0.5 * (n - 1) / n, j / n - 0.5 * (n - 1) / n))
+    return ks
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=1024, help="frequencies per step (one k-point)")
+    ap.add_argument("--layers", type=int, default=40)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--cpu-sample", type=int, default=16, help="solves in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the (omega,k) points shard across GPUs with no "
+              "inter-GPU traffic but a final gather, and multi-GPU runs are not part of this build; "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from phinfortran_amd import lattice as lt
+    from phinfortran_amd import transport as T
+    from phinfortran_amd import _native as N
+
+    dev = torch.device("cuda", 0)
+    L = args.layers
+    ks = kgrid(8)
+    nsteps = args.warmup + args.steps
+    t0 = time.time()
+    blocks = [lt.si_ge_interface_film(nx=4, ny=2, L=L, kfrac=ks[i % len(ks)]) for i in range(nsteps)]
+    log(f"built {nsteps} k-point block sets in {time.time() - t0:.1f}s (b={blocks[0].b}, L={L})")
+    omega = 16.0 * np.arange(1, args.points + 1) / args.points
+    E = torch.as_tensor(omega ** 2, dtype=torch.float64, device=dev)
+    p = T.params("phonon", precision=args.precision)
+
+    eng = T.Engine(0, batch_limit=args.points)
+    dblocks = [eng.upload_blocks(b) for b in blocks]
+    torch.cuda.synchronize()
+
+    for s in range(args.warmup):
+        r = eng.solve_tdb(dblocks[s], E, p)
+        log(f"warmup {s}: status ok={int((r['status'] == 0).sum())}/{args.points}, "
+            f"mean iters {eng.last_mean_iters():.1f}")
+    eng.profile_reset()
+    eng.profile(True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    iters = []
+    for s in range(args.steps):
+        r = eng.solve_tdb(dblocks[args.warmup + s], E, p, stream_sync=False)
+        iters.append(eng.last_mean_iters())
+    torch.cuda.synchronize()
+    t_el = time.perf_counter() - t_start
+    eng.profile(False)
+    ok = int((r["status"] == 0).sum())
+    solves = args.steps * args.points
+    value = solves / t_el
+    nbar = float(np.mean(iters))
+    b = blocks[0].b
+    flop_solve = b ** 3 * (112 * nbar + 40 * L + 104)
+
+    gm = eng.profile_get(N.NT_K_GEMM)
+    inv = eng.profile_get(N.NT_K_INV)
+    el = eng.profile_get(N.NT_K_ELEM)
+    kern = {"gemm": gm, "inverse": inv, "elementwise": el}
+    dom_name, dom = max(((k, v) for k, v in kern.items() if k != "elementwise"), key=lambda kv: kv[1]["ms"])
+    avg_ms = dom["ms"] / max(1, dom["launches"])
+    avg_flops = dom["flops"] / max(1, dom["launches"])
+    achieved = avg_flops / (avg_ms * 1e-3) / 1e12
+    peak = PEAK[args.precision] if dom_name == "gemm" else PEAK["fp64"]
+    log(f"kernels: " + ", ".join(f"{k}: {v['launches']} launches {v['ms']:.1f} ms {v['flops'] / max(1e-9, v['ms'] * 1e-3) / 1e12:.2f} TF/s"
+                                for k, v in kern.items()))
+
+    cpu = None
+    if args.cpu_sample > 0:
+        import oracle as O
+        nth = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ns = args.cpu_sample
+        sample_E = (omega ** 2)[:: max(1, len(omega) // ns)][:ns]
+        op = O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
+        t0 = time.perf_counter()
+        tl, tr, st = O.solve_tdb_batch(blocks[0].as_dict(), sample_E, op, nthreads=nth)
+        dt = time.perf_counter() - t0
+        cpu = {"value": ns / dt, "unit": "(omega,k)-solves/s", "cores": nth, "kind": "port",
+               "sample": f"{ns} solves at b={b}, L={L} (one k-point, {ns} frequencies), "
+                         f"FP64 C oracle, OpenMP over points, {dt:.1f} s"}
+        gpu_T = r["T_LR"].cpu().numpy()
+        log(f"cpu baseline: {ns} solves in {dt:.1f}s on {nth} threads")
+
+    out = {
+        "metric": "RGF (omega,k)-solves/s at block=192",
+        "value": value,
+        "unit": "(omega,k)-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if args.precision == "fp64" else "c64+c128",
+        "data": "synthetic",
+        "config": {"workload": "Si|Ge interface film 4x2 cells, b=192, L=%d, 8x8 k-grid, "
+                               "%d frequencies in (0,16] THz; one k-point per step" % (L, args.points),
+                   "block": b, "layers": L, "points_per_step": args.points, "precision": args.precision,
+                   "parallelism": "single GPU, batched over frequencies"},
+        "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None},
+        "solve_model": {"mean_decimation_iters": nbar, "flop_per_solve": flop_solve,
+                        "solve_tflops": flop_solve * value / 1e12,
+                        "status_ok_last_step": ok},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

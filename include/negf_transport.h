@@ -1,0 +1,205 @@
+/*
+ * negf_transport.h -- C ABI of the MI355X transport engine.
+ *
+ * Drop-in boundary for the per-(omega,k) solve of phinfortran.  Every entry
+ * point takes plain pointers and sizes; complex matrices are column-major
+ * complex(kind=8) (nt_c128 = {re, im}, same bytes as Fortran
+ * complex(c_double_complex)), leading dimension = number of rows.
+ *
+ * Status codes (all entry points):
+ *    0  success
+ *   >0  exact zero pivot while inverting a block (LAPACK-style column, 1-based);
+ *       the reference stops with 'Matrix is numerically singular!'
+ *       (negf/util.f90:44-58)
+ *   -1  NaN in a block that is inverted (negf/util.f90:29-32)
+ *   -2  surface decimation did not converge within max_iter (the reference
+ *       loops forever, negf/surface.f90:488)
+ *   -3  invalid argument (sizes, null pointers, unsupported block size)
+ *   -4  HIP runtime error (see nt_last_error)
+ *   -5  out of device memory
+ */
+#ifndef NEGF_TRANSPORT_H
+#define NEGF_TRANSPORT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } nt_c128;
+typedef struct nt_ctx nt_ctx;
+
+enum {
+    NT_OK = 0,
+    NT_ERR_NAN = -1,
+    NT_ERR_NOT_CONVERGED = -2,
+    NT_ERR_ARG = -3,
+    NT_ERR_HIP = -4,
+    NT_ERR_NOMEM = -5
+};
+
+/* Broadening conventions. */
+enum {
+    NT_ETA_RELATIVE = 0,   /* eta(E) = E*eta + eta0  (negf/surface.f90:479) */
+    NT_ETA_ABSOLUTE = 1    /* eta(E) = eta           (tb/surface.f90:218)   */
+};
+
+/* Arithmetic of the RGF sweep. */
+enum {
+    NT_PREC_FP64 = 0,      /* complex FP64 throughout (the reference's arithmetic) */
+    NT_PREC_MIXED = 1      /* complex FP32 MFMA sweep + FP64 refinement; FP64 decimation */
+};
+
+/* Scalar parameters that the reference keeps as module globals
+ * (negf/config.f90:58-68, negf/surface.f90:5-7). */
+typedef struct {
+    double eta;               /* &convergelist eta  */
+    double eta0;              /* &convergelist eta0 */
+    double convergence;       /* &convergelist convergence: stop when max|alpha| <= convergence */
+    double device_eta_factor; /* 10 in negf (lesser.f90:556, device.f90:348), 1 in tb */
+    double lambda_eta;        /* &convergelist lambda_eta (propagating-mode test) */
+    int eta_mode;             /* NT_ETA_RELATIVE (negf) or NT_ETA_ABSOLUTE (tb) */
+    int max_iter;             /* decimation iteration cap, >= 1 */
+    int precision;            /* NT_PREC_FP64 or NT_PREC_MIXED */
+    int reserved;
+} nt_params;
+
+/* Fills the negf defaults: eta = eta0 = 1e-6 (config.f90:88-89), device x10,
+ * convergence 1e-12, lambda_eta 5e-3, max_iter 300, FP64. */
+void nt_params_default(nt_params* p);
+
+/* ------------------------------------------------------------ context */
+int nt_create(nt_ctx** ctx, int device_ordinal);
+int nt_destroy(nt_ctx* ctx);
+int nt_sync(nt_ctx* ctx);
+/* HIP stream the context launches on (hipStream_t as void*). */
+void* nt_stream(nt_ctx* ctx);
+/* Use an external stream instead of the context's own one (NULL restores it). */
+int nt_set_stream(nt_ctx* ctx, void* stream);
+/* Upper bound on (omega,k) points per device batch (default 512). */
+int nt_set_batch_limit(nt_ctx* ctx, int max_points);
+const char* nt_last_error(nt_ctx* ctx);
+const char* nt_status_string(int status);
+
+/* ------------------------------------------- per-point host-buffer ABI
+ * These mirror the reference subroutines one-to-one so a Fortran caller can
+ * swap a call for a bind(C) call.  Inputs/outputs are host arrays. */
+
+/* surface_green_tdb (negf/surface.f90:385-605) and tb surface_green
+ * (tb/surface.f90:129-343, with params->eta_mode = NT_ETA_ABSOLUTE).
+ *   Haml, Hamlv  nl x nl: left lead H00 and coupling V_{-1,0}
+ *   Hamr, Hamrv  nr x nr: right lead H00 and coupling V_{N+2,N+1}
+ * Outputs (nl x nl / nr x nr), any may be NULL except as noted:
+ *   gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, sl_adv, sr_ret.
+ *   gl_plus/gr_minus are only computed when non-NULL (they feed the
+ *   mode-resolved solve only).
+ * iters (int[4], may be NULL): decimation iterations of gl_adv, gl_plus,
+ * gr_ret, gr_minus. */
+int nt_surface_green_tdb(nt_ctx* ctx, const nt_params* p, int nl, int nr, double E,
+                         const nt_c128* Haml, const nt_c128* Hamr,
+                         const nt_c128* Hamlv, const nt_c128* Hamrv,
+                         nt_c128* gl_adv, nt_c128* gr_ret, nt_c128* gl_plus, nt_c128* gr_minus,
+                         nt_c128* gml_adv, nt_c128* gmr_ret, nt_c128* sl_adv, nt_c128* sr_ret,
+                         int* iters);
+
+/* get_G_full_tdb (negf/lesser.f90:475-809) as called by get_G_full_tdb_mpi
+ * (lesser.f90:141-154): gl, gml, sl are the RETARDED left quantities
+ * (gl_adv^+, -gml_adv, sl_adv^+), gr, gmr, sr the right ones.
+ *   hamd   b x b x L  (layer i at hamd + i*b*b)
+ *   hamdv  b x b x (L-1), V_{i,i+1}
+ *   hamld  nl x b (V_{0,1}), hamrd nr x b (V_{N+1,N}), haml nl x nl, hamr nr x nr
+ *   first_point != 0 reproduces the reference's T = 0 at ie = 1
+ *   (lesser.f90:802-807).
+ * Outputs: transl (T_LR), transr (T_RL); G_D_RL (nr x nl), G_D_LR (nl x nr),
+ * G_L (nl x nl), G_R (nr x nr) may be NULL.  The Buttiker-probe bookkeeping
+ * (GLeft, G_full, G_q_qp, ...) is not produced. */
+int nt_get_G_full_tdb(nt_ctx* ctx, const nt_params* p, int first_point,
+                      int nl, int nr, int b, int L, double E,
+                      const nt_c128* gl, const nt_c128* gr, const nt_c128* gml, const nt_c128* gmr,
+                      const nt_c128* sl, const nt_c128* sr,
+                      const nt_c128* hamd, const nt_c128* hamdv, const nt_c128* hamld,
+                      const nt_c128* hamrd, const nt_c128* haml, const nt_c128* hamr,
+                      double* transl, double* transr,
+                      nt_c128* G_D_RL, nt_c128* G_D_LR, nt_c128* G_L, nt_c128* G_R);
+
+/* G_block (negf/device.f90:289-560, tb/device.f90:291-575): the same sweep on
+ * variable layer sizes layer_list[L] cut from a dense Hamiltonian.
+ *   Hd[i]       layer_list[i]^2 diagonal blocks
+ *   Hdv[i]      layer_list[i] x layer_list[i+1], V_{i,i+1} (left-to-right sweep)
+ *   Hdv_back[i] layer_list[i+1] x layer_list[i], V_{i+1,i} (right-to-left), or NULL
+ *               to use Hdv[i]^+
+ *   Hld         nl x layer_list[0], V_{0,1}
+ *   Hrd         nr x layer_list[L-1], V_{N+1,N}
+ *   Hrd_fwd     layer_list[L-1] x nr, V_{N,N+1}, or NULL to use Hrd^+
+ *   Haml, Hamr  lead H00 of the lead layers next to the device. */
+int nt_G_block(nt_ctx* ctx, const nt_params* p, int nl, int nr, int L, const int* layer_list, double E,
+               const nt_c128* gl, const nt_c128* gr, const nt_c128* gml, const nt_c128* gmr,
+               const nt_c128* sl, const nt_c128* sr,
+               const nt_c128* const* Hd, const nt_c128* const* Hdv, const nt_c128* const* Hdv_back,
+               const nt_c128* Hld, const nt_c128* Hrd, const nt_c128* Hrd_fwd,
+               const nt_c128* Haml, const nt_c128* Hamr,
+               double* transl, double* transr,
+               nt_c128* G_D_RL, nt_c128* G_D_LR, nt_c128* G_L, nt_c128* G_R);
+
+/* ----------------------------------------------- batched (omega,k) solve
+ * The hot path: for npts points (E[i], k-point kidx[i]) it runs the lead
+ * decimations, the adv->ret conversion, both RGF sweeps and Caroli T, all on
+ * the GPU.  This replaces the body of the driver's ie loop
+ * (negf/phinfortran.f90:1196-1251) for every energy of one or more k-points.
+ *
+ * Block arrays hold nk k-points back to back:
+ *   Haml, Hamlv  nk x (nl x nl)          Hamr, Hamrv  nk x (nr x nr)
+ *   Hamd         nk x L x (b x b)        Hamdv        nk x (L-1) x (b x b)
+ *   Hamld        nk x (nl x b)           Hamrd        nk x (nr x b)
+ * kidx (npts ints, may be NULL when nk == 1) selects the k-point of each
+ * point.  first_point_mask (may be NULL): points with a nonzero entry get
+ * T = 0 (the reference's ie = 1 quirk).
+ * Outputs: T_LR[npts], T_RL[npts]; status[npts] and iters[npts] (mean of the
+ * two decimations, may be NULL).  Returns the first nonzero point status. */
+typedef struct {
+    /* optional per-point outputs (npts x n x n each); NULL to skip */
+    nt_c128* gl_adv;  nt_c128* gr_ret;  nt_c128* gl_plus; nt_c128* gr_minus;
+    nt_c128* gml_adv; nt_c128* gmr_ret; nt_c128* sl_adv;  nt_c128* sr_ret;
+    nt_c128* G_D_LR;  nt_c128* G_D_RL;  nt_c128* G_L;     nt_c128* G_R;
+    double* sdos_l;   double* sdos_r;   /* surface_ldos_mpi (negf/surface.f90:608-630) */
+} nt_solve_extras;
+
+/* Device-pointer variant: every array is device memory (e.g. torch tensors). */
+int nt_solve_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b, int L,
+                             int npts, const double* E, const int* kidx, int nk,
+                             const nt_c128* Haml, const nt_c128* Hamlv,
+                             const nt_c128* Hamr, const nt_c128* Hamrv,
+                             const nt_c128* Hamd, const nt_c128* Hamdv,
+                             const nt_c128* Hamld, const nt_c128* Hamrd,
+                             const int* first_point_mask,
+                             double* T_LR, double* T_RL, int* status, int* iters,
+                             const nt_solve_extras* extras);
+
+/* Host-pointer variant (copies in and out; same layout). */
+int nt_solve_tdb_batched(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b, int L,
+                         int npts, const double* E, const int* kidx, int nk,
+                         const nt_c128* Haml, const nt_c128* Hamlv,
+                         const nt_c128* Hamr, const nt_c128* Hamrv,
+                         const nt_c128* Hamd, const nt_c128* Hamdv,
+                         const nt_c128* Hamld, const nt_c128* Hamrd,
+                         const int* first_point_mask,
+                         double* T_LR, double* T_RL, int* status, int* iters,
+                         const nt_solve_extras* extras);
+
+/* ------------------------------------------------------- measurement */
+/* Kernel classes for nt_profile_get. */
+enum { NT_K_GEMM = 0, NT_K_INV = 1, NT_K_ELEM = 2, NT_K_COUNT = 3 };
+/* While enabled, each launch is bracketed by HIP events on the context's
+ * stream; nt_profile_get returns launches, summed milliseconds and the summed
+ * algorithmic flops (GEMM 8mnk, inverse 8n^3 per active item) of a class. */
+int nt_profile_enable(nt_ctx* ctx, int on);
+int nt_profile_get(nt_ctx* ctx, int kind, long long* launches, double* total_ms, double* flops);
+int nt_profile_reset(nt_ctx* ctx);
+/* Mean decimation iterations of the last batched solve. */
+double nt_last_mean_iters(nt_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

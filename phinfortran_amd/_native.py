@@ -1,0 +1,110 @@
+"""ctypes binding of libnegf_transport.so (the C ABI in include/negf_transport.h).
+
+The library is required: importing the solver API without it raises, there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnegf_transport.so")
+
+NT_OK = 0
+NT_ERR_NAN = -1
+NT_ERR_NOT_CONVERGED = -2
+NT_ERR_ARG = -3
+NT_ERR_HIP = -4
+NT_ERR_NOMEM = -5
+
+NT_ETA_RELATIVE = 0
+NT_ETA_ABSOLUTE = 1
+NT_PREC_FP64 = 0
+NT_PREC_MIXED = 1
+
+NT_K_GEMM = 0
+NT_K_INV = 1
+NT_K_ELEM = 2
+
+# every symbol the header declares
+EXPORTS = [
+    "nt_params_default", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
+    "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
+    "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
+    "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
+]
+
+
+class NtParams(C.Structure):
+    _fields_ = [
+        ("eta", C.c_double),
+        ("eta0", C.c_double),
+        ("convergence", C.c_double),
+        ("device_eta_factor", C.c_double),
+        ("lambda_eta", C.c_double),
+        ("eta_mode", C.c_int),
+        ("max_iter", C.c_int),
+        ("precision", C.c_int),
+        ("reserved", C.c_int),
+    ]
+
+
+class NtSolveExtras(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in (
+        "gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "sl_adv", "sr_ret",
+        "G_D_LR", "G_D_RL", "G_L", "G_R", "sdos_l", "sdos_r")]
+
+
+class TransportError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m phinfortran_amd.build` "
+                          "(the transport path has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    I = C.c_int
+    D = C.c_double
+    pp = C.POINTER(NtParams)
+    L.nt_params_default.argtypes = [pp]
+    L.nt_params_default.restype = None
+    L.nt_create.argtypes = [C.POINTER(P), I]
+    L.nt_destroy.argtypes = [P]
+    L.nt_sync.argtypes = [P]
+    L.nt_stream.argtypes = [P]
+    L.nt_stream.restype = P
+    L.nt_set_stream.argtypes = [P, P]
+    L.nt_set_batch_limit.argtypes = [P, I]
+    L.nt_last_error.argtypes = [P]
+    L.nt_last_error.restype = C.c_char_p
+    L.nt_status_string.argtypes = [I]
+    L.nt_status_string.restype = C.c_char_p
+    L.nt_surface_green_tdb.argtypes = [P, pp, I, I, D, P, P, P, P, P, P, P, P, P, P, P, P, P]
+    L.nt_get_G_full_tdb.argtypes = [P, pp, I, I, I, I, I, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]
+    L.nt_G_block.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]
+    solve_args = [P, pp, I, I, I, I, I, P, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, C.POINTER(NtSolveExtras)]
+    L.nt_solve_tdb_batched_dev.argtypes = solve_args
+    L.nt_solve_tdb_batched.argtypes = solve_args
+    L.nt_profile_enable.argtypes = [P, I]
+    L.nt_profile_get.argtypes = [P, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D)]
+    L.nt_profile_reset.argtypes = [P]
+    L.nt_last_mean_iters.argtypes = [P]
+    L.nt_last_mean_iters.restype = D
+    _lib = L
+    return L
+
+
+def status_string(status: int) -> str:
+    return lib().nt_status_string(int(status)).decode()

@@ -1,0 +1,295 @@
+// nt_gemm.hip -- batched complex GEMM on CDNA4 MFMA (FP64 and FP32).
+//
+// One workgroup (4 wave64) computes a 64x64 complex tile of one batch item.
+// Operand tiles of depth 16 are staged through LDS as separate real and
+// imaginary planes; each wave owns a 32x32 quadrant as 2x2 MFMA 16x16 tiles
+// and issues four real MFMAs per complex 16x16x4 step.
+//
+// The product is formed transposed (C^T = B^T A^T): the MFMA output layout
+// then maps the 16 consecutive lanes of a row group to 16 consecutive rows of
+// the column-major C, so every store is a 256-byte (FP64) contiguous run.
+#include "nt_kernels.h"
+
+namespace nt {
+
+namespace {
+
+constexpr int BM = 64;
+constexpr int BN = 64;
+constexpr int BK = 16;
+constexpr int NT = 256;
+
+template <typename T>
+struct GemmSmem {
+    T ar[BK][BM];
+    T ai[BK][BM];
+    T br[BK][BN];
+    T bi[BK][BN];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_a_regs(const T* __restrict__ A, int lda, int opA, int M, int K,
+                                            int m0, int k0, T (&r)[4], T (&i)[4])
+{
+    const int t = threadIdx.x;
+    typedef typename vec2<T>::type V2;
+    if (opA == OP_N) {
+        const int m = t & 63, kq = t >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kq + 4 * j;
+            const int gm = m0 + m, gk = k0 + k;
+            V2 v = {0, 0};
+            if (gm < M && gk < K) v = *reinterpret_cast<const V2*>(A + 2 * ((long)gm + (long)gk * lda));
+            r[j] = v.x;
+            i[j] = v.y;
+        }
+    } else {
+        const int k = t & 15, mq = t >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = mq + 16 * j;
+            const int gm = m0 + m, gk = k0 + k;
+            V2 v = {0, 0};
+            if (gm < M && gk < K) v = *reinterpret_cast<const V2*>(A + 2 * ((long)gk + (long)gm * lda));
+            r[j] = v.x;
+            i[j] = -v.y;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_a_smem(GemmSmem<T>& s, int opA, const T (&r)[4], const T (&i)[4])
+{
+    const int t = threadIdx.x;
+    if (opA == OP_N) {
+        const int m = t & 63, kq = t >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s.ar[kq + 4 * j][m] = r[j];
+            s.ai[kq + 4 * j][m] = i[j];
+        }
+    } else {
+        const int k = t & 15, mq = t >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s.ar[k][mq + 16 * j] = r[j];
+            s.ai[k][mq + 16 * j] = i[j];
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, int opB, int K, int N,
+                                            int k0, int n0, T (&r)[4], T (&i)[4])
+{
+    const int t = threadIdx.x;
+    typedef typename vec2<T>::type V2;
+    if (opB == OP_N) {
+        const int k = t & 15, nq = t >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = nq + 16 * j;
+            const int gk = k0 + k, gn = n0 + n;
+            V2 v = {0, 0};
+            if (gk < K && gn < N) v = *reinterpret_cast<const V2*>(B + 2 * ((long)gk + (long)gn * ldb));
+            r[j] = v.x;
+            i[j] = v.y;
+        }
+    } else {
+        const int n = t & 63, kq = t >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = kq + 4 * j;
+            const int gk = k0 + k, gn = n0 + n;
+            V2 v = {0, 0};
+            if (gk < K && gn < N) v = *reinterpret_cast<const V2*>(B + 2 * ((long)gn + (long)gk * ldb));
+            r[j] = v.x;
+            i[j] = -v.y;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_b_smem(GemmSmem<T>& s, int opB, const T (&r)[4], const T (&i)[4])
+{
+    const int t = threadIdx.x;
+    if (opB == OP_N) {
+        const int k = t & 15, nq = t >> 4;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s.br[k][nq + 16 * j] = r[j];
+            s.bi[k][nq + 16 * j] = i[j];
+        }
+    } else {
+        const int n = t & 63, kq = t >> 6;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            s.br[kq + 4 * j][n] = r[j];
+            s.bi[kq + 4 * j][n] = i[j];
+        }
+    }
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v)
+{
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
+{
+    typedef typename acc4<T>::type A4;
+    const int tiles_m = (a.M + BM - 1) / BM;
+    const int tiles_n = (a.N + BN - 1) / BN;
+    const int tiles = tiles_m * tiles_n;
+    const long bid = blockIdx.x;
+    const int tile = (int)(bid % tiles);
+    const long item = bid / tiles;
+    const int p = (int)(item / a.groups);
+    const int g = (int)(item % a.groups);
+    if (a.active && !a.active[p]) return;
+    const int m0 = (tile % tiles_m) * BM;
+    const int n0 = (tile / tiles_m) * BN;
+
+    __shared__ GemmSmem<T> s;
+    const T* __restrict__ A = bptr(a.A, p, g);
+    const T* __restrict__ B = bptr(a.B, p, g);
+
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    A4 accR[2][2], accI[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            accR[x][y] = A4{0, 0, 0, 0};
+            accI[x][y] = A4{0, 0, 0, 0};
+        }
+
+    T ra[4], ia[4], rb[4], ib[4];
+    const int nst = (a.K + BK - 1) / BK;
+    load_a_regs(A, a.A.ld, a.opA, a.M, a.K, m0, 0, ra, ia);
+    load_b_regs(B, a.B.ld, a.opB, a.K, a.N, 0, n0, rb, ib);
+    store_a_smem(s, a.opA, ra, ia);
+    store_b_smem(s, a.opB, rb, ib);
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const bool more = st + 1 < nst;
+        if (more) {
+            load_a_regs(A, a.A.ld, a.opA, a.M, a.K, m0, (st + 1) * BK, ra, ia);
+            load_b_regs(B, a.B.ld, a.opB, a.K, a.N, (st + 1) * BK, n0, rb, ib);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int k = kk + lk;
+            T xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                xr[x] = s.ar[k][wm + 16 * x + lr];
+                xi[x] = s.ai[k][wm + 16 * x + lr];
+            }
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                yr[y] = s.br[k][wn + 16 * y + lr];
+                yi[y] = s.bi[k][wn + 16 * y + lr];
+            }
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    // (C^T) = B^T A^T: MFMA "A" operand is B, "B" operand is A.
+                    accR[x][y] = mfma16x16x4(yr[y], xr[x], accR[x][y]);
+                    accR[x][y] = mfma16x16x4(-yi[y], xi[x], accR[x][y]);
+                    accI[x][y] = mfma16x16x4(yi[y], xr[x], accI[x][y]);
+                    accI[x][y] = mfma16x16x4(yr[y], xi[x], accI[x][y]);
+                }
+        }
+        __syncthreads();
+        if (more) {
+            store_a_smem(s, a.opA, ra, ia);
+            store_b_smem(s, a.opB, rb, ib);
+            __syncthreads();
+        }
+    }
+
+    // Epilogue.  Lane holds D[n = 4*lk + r][m = lr] of each 16x16 sub-tile.
+    T* __restrict__ C = bptr(a.C, p, g);
+    const T* __restrict__ Cin = a.Cin.base ? bptr(a.Cin, p, g) : nullptr;
+    T* __restrict__ C2 = a.C2.base ? bptr(a.C2, p, g) : nullptr;
+    T dr = 0, di = 0;
+    if (a.diag) {
+        dr = a.diag[2 * p];
+        di = a.diag[2 * p + 1];
+    }
+    double vmax = 0.0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + 16 * x + lr;
+                const int n = n0 + wn + 16 * y + 4 * lk + r;
+                if (m < a.M && n < a.N) {
+                    const T pr = accR[x][y][r], pi = accI[x][y][r];
+                    T vr = a.alpha_re * pr - a.alpha_im * pi;
+                    T vi = a.alpha_re * pi + a.alpha_im * pr;
+                    T cr = vr, ci = vi;
+                    if (Cin) {
+                        const T* c = Cin + 2 * ((long)m + (long)n * a.Cin.ld);
+                        const T qr = c[0], qi = c[1];
+                        cr += a.beta_re * qr - a.beta_im * qi;
+                        ci += a.beta_re * qi + a.beta_im * qr;
+                    }
+                    if (m == n) {
+                        cr += dr;
+                        ci += di;
+                    }
+                    T* o = C + 2 * ((long)m + (long)n * a.C.ld);
+                    o[0] = cr;
+                    o[1] = ci;
+                    if (C2) {
+                        T* o2 = C2 + 2 * ((long)m + (long)n * a.C2.ld);
+                        o2[0] += vr;
+                        o2[1] += vi;
+                    }
+                    if (a.maxabs) {
+                        const double ab = sqrt((double)cr * (double)cr + (double)ci * (double)ci);
+                        vmax = (ab > vmax || ab != ab) ? ab : vmax;
+                    }
+                }
+            }
+    if (a.maxabs && g == 0) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double o = __shfl_xor(vmax, off);
+            vmax = (o > vmax || o != o) ? o : vmax;
+        }
+        if (lane == 0) {
+            // NaN is encoded as +inf so that it always wins and fails the convergence test.
+            atomic_max_nonneg(&a.maxabs[p], vmax != vmax ? __longlong_as_double(0x7ff0000000000000LL) : vmax);
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s)
+{
+    if (a.M <= 0 || a.N <= 0 || a.batch <= 0 || a.groups <= 0) return hipSuccess;
+    const long tiles = (long)((a.M + BM - 1) / BM) * ((a.N + BN - 1) / BN);
+    const long blocks = tiles * (long)a.batch * a.groups;
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gemm<T>, dim3((unsigned)blocks), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_gemm<double>(const GemmArgs<double>&, hipStream_t);
+template hipError_t launch_gemm<float>(const GemmArgs<float>&, hipStream_t);
+
+}  // namespace nt

@@ -1,0 +1,419 @@
+// nt_inv.hip -- batched complex matrix inverse (blocked Gauss-Jordan with
+// partial row pivoting), one workgroup of 256 threads per matrix.
+//
+// Per panel of NB columns J:
+//   1. the n x NB panel lives in registers, one row per thread (RPT rows for
+//      n > 256); every column is pivoted (max |re|+|im| over the rows not yet
+//      used as pivots, first index on ties, as LAPACK's IZAMAX) and swept
+//      (pivot row scaled, all other rows eliminated).  Rows are exchanged by
+//      ownership, so the row swap costs nothing.  Two barriers per column.
+//   2. the swept panel W (rows J hold M_JJ^-1, others -M_OJ M_JJ^-1) is
+//      written back and kept in LDS; the row exchanges of the panel are folded
+//      into a gather permutation pi.
+//   3. every other column strip of 16 is updated with MFMA:
+//         new[:, q] = old[pi(:), q] (rows J zeroed) + W * old[pi(J), q].
+// At the end, X[:, j] = W[:, sigma(j)] undoes the column order (sigma is the
+// reverse composition of the row exchanges).
+//
+// LDS: W planes NB x npad (npad = n rounded up to 16), a 16-column strip of
+// old pivot rows, the permutations and small exchange buffers.
+#include "nt_kernels.h"
+
+namespace nt {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int QW = 16;
+
+template <typename T>
+struct Cx {
+    T r, i;
+};
+
+template <typename T, int NB>
+struct InvSmemFixed {
+    T btr[NB][QW];
+    T bti[NB][QW];
+    T xr[2][2][NB];        // [buf][0 = pivot row, 1 = old row c][col]
+    T xi[2][2][NB];
+    double redv[2][4];
+    int redi[2][4];
+    int nan_flag;
+    int info;
+};
+
+__device__ __forceinline__ int wave_of() { return threadIdx.x >> 6; }
+
+template <typename T>
+__device__ __forceinline__ T cabs1(T r, T i) { return fabs(r) + fabs(i); }
+
+template <typename T, int NB, int RPT>
+__global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
+{
+    typedef typename acc4<T>::type A4;
+    typedef typename vec2<T>::type V2;
+    const int p = blockIdx.x;
+    if (a.active && !a.active[p]) return;
+    const int n = a.n;
+    const int npad = (n + 15) & ~15;
+    const int t = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = wave_of();
+
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    InvSmemFixed<T, NB>& s = *reinterpret_cast<InvSmemFixed<T, NB>*>(smem_raw);
+    T* wr = reinterpret_cast<T*>(smem_raw + sizeof(InvSmemFixed<T, NB>));   // [NB][npad]
+    T* wi = wr + (long)NB * npad;
+    int* pi = reinterpret_cast<int*>(wi + (long)NB * npad);                  // [n]
+    int* ipiv = pi + n;                                                       // [n]
+    int* sigma = ipiv + n;                                                    // [n]
+
+    T* __restrict__ W = bptr(a.W, p, 0);
+    const int ld = a.W.ld;
+
+    if (t == 0) {
+        s.nan_flag = 0;
+        s.info = 0;
+    }
+    // Optional prologue: W = z I - src
+    if (a.src.base) {
+        const T* __restrict__ S = bptr(a.src, p, 0);
+        const T zr = a.z[2 * p], zi = a.z[2 * p + 1];
+        for (long e = t; e < (long)n * n; e += NT) {
+            const int r = (int)(e % n), c = (int)(e / n);
+            V2 v = *reinterpret_cast<const V2*>(S + 2 * ((long)r + (long)c * a.src.ld));
+            V2 o;
+            o.x = (r == c ? zr : (T)0) - v.x;
+            o.y = (r == c ? zi : (T)0) - v.y;
+            *reinterpret_cast<V2*>(W + 2 * ((long)r + (long)c * ld)) = o;
+        }
+    }
+    // zero the padded rows of the W planes once
+    for (int e = t; e < NB * (npad - n); e += NT) {
+        const int c = e / (npad - n), r = n + e % (npad - n);
+        wr[c * npad + r] = 0;
+        wi[c * npad + r] = 0;
+    }
+    __syncthreads();
+
+    Cx<T> x[RPT][NB];
+    int step = 0;   // parity of the exchange buffers
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nbe = min(NB, n - j0);
+        // ---- load panel rows into registers; reset pi on [j0, n)
+        for (int r = j0 + t; r < n; r += NT) pi[r] = r;
+#pragma unroll
+        for (int sl = 0; sl < RPT; ++sl) {
+            const int r = t + sl * NT;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                Cx<T> v = {0, 0};
+                if (r < n && c < nbe) {
+                    V2 q = *reinterpret_cast<const V2*>(W + 2 * ((long)r + (long)(j0 + c) * ld));
+                    v.r = q.x;
+                    v.i = q.y;
+                    if (q.x != q.x || q.y != q.y) s.nan_flag = 1;
+                }
+                x[sl][c] = v;
+            }
+        }
+        __syncthreads();
+        // ---- unblocked sweep of the panel
+        for (int cl = 0; cl < nbe; ++cl) {
+            const int c = j0 + cl;
+            // local argmax over owned rows r >= c
+            T bv = (T)-1;
+            int bi = n;
+#pragma unroll
+            for (int sl = 0; sl < RPT; ++sl) {
+                const int r = t + sl * NT;
+                T v = (T)-1;
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc)
+                    if (cc == cl) v = cabs1(x[sl][cc].r, x[sl][cc].i);
+                if (r < n && r >= c && v > bv) {
+                    bv = v;
+                    bi = r;
+                }
+            }
+            double dv = (double)bv;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(dv, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ov > dv || (ov == dv && oi < bi)) {
+                    dv = ov;
+                    bi = oi;
+                }
+            }
+            if (lane == 0) {
+                s.redv[step][wave] = dv;
+                s.redi[step][wave] = bi;
+            }
+            __syncthreads();
+            double best = s.redv[step][0];
+            int piv = s.redi[step][0];
+#pragma unroll
+            for (int w = 1; w < NT / 64; ++w) {
+                const double ov = s.redv[step][w];
+                const int oi = s.redi[step][w];
+                if (ov > best || (ov == best && oi < piv)) {
+                    best = ov;
+                    piv = oi;
+                }
+            }
+            if (piv >= n) piv = c;   // all candidates NaN: keep the diagonal
+            // The pivot-row owner publishes the scaled row s_c' = pv_c' / pv_cl
+            // (s_cl = 1 / pv_cl); the owner of row c publishes its old row.
+#pragma unroll
+            for (int sl = 0; sl < RPT; ++sl) {
+                const int r = t + sl * NT;
+                if (r == piv) {
+                    T pr = 0, pim = 0;
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc)
+                        if (cc == cl) {
+                            pr = x[sl][cc].r;
+                            pim = x[sl][cc].i;
+                        }
+                    const T den = pr * pr + pim * pim;
+                    T dr = 0, di = 0;
+                    if (den != (T)0) {
+                        dr = pr / den;
+                        di = -pim / den;
+                    } else if (s.info == 0) {
+                        s.info = c + 1;
+                    }
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc) {
+                        const T vr = x[sl][cc].r, vi = x[sl][cc].i;
+                        s.xr[step][0][cc] = (cc == cl) ? dr : vr * dr - vi * di;
+                        s.xi[step][0][cc] = (cc == cl) ? di : vr * di + vi * dr;
+                    }
+                }
+                if (r == c) {
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc) {
+                        s.xr[step][1][cc] = x[sl][cc].r;
+                        s.xi[step][1][cc] = x[sl][cc].i;
+                    }
+                }
+            }
+            if (t == 0) {
+                ipiv[c] = piv;
+                if (piv != c) {
+                    const int tmp = pi[c];
+                    pi[c] = pi[piv];
+                    pi[piv] = tmp;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int sl = 0; sl < RPT; ++sl) {
+                const int r = t + sl * NT;
+                if (r >= n) continue;
+                if (r == c) {
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc) {
+                        x[sl][cc].r = s.xr[step][0][cc];
+                        x[sl][cc].i = s.xi[step][0][cc];
+                    }
+                } else {
+                    if (r == piv) {
+#pragma unroll
+                        for (int cc = 0; cc < NB; ++cc) {
+                            x[sl][cc].r = s.xr[step][1][cc];
+                            x[sl][cc].i = s.xi[step][1][cc];
+                        }
+                    }
+                    T fr = 0, fi = 0;
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc)
+                        if (cc == cl) {
+                            fr = x[sl][cc].r;
+                            fi = x[sl][cc].i;
+                        }
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc) {
+                        const T sr = s.xr[step][0][cc], si = s.xi[step][0][cc];
+                        const T br = (cc == cl) ? (T)0 : x[sl][cc].r;
+                        const T bim = (cc == cl) ? (T)0 : x[sl][cc].i;
+                        x[sl][cc].r = br - (fr * sr - fi * si);
+                        x[sl][cc].i = bim - (fr * si + fi * sr);
+                    }
+                }
+            }
+            step ^= 1;
+        }
+        // ---- write panel back (global + LDS W planes)
+#pragma unroll
+        for (int sl = 0; sl < RPT; ++sl) {
+            const int r = t + sl * NT;
+            if (r < n) {
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc) {
+                    if (cc < nbe) {
+                        V2 q;
+                        q.x = x[sl][cc].r;
+                        q.y = x[sl][cc].i;
+                        *reinterpret_cast<V2*>(W + 2 * ((long)r + (long)(j0 + cc) * ld)) = q;
+                    }
+                    wr[cc * npad + r] = (cc < nbe) ? x[sl][cc].r : (T)0;
+                    wi[cc * npad + r] = (cc < nbe) ? x[sl][cc].i : (T)0;
+                }
+            }
+        }
+        // ---- trailing update of all other columns, strip by strip
+        for (int q0 = 0; q0 < n; q0 += QW) {
+            if (q0 >= j0 && q0 + QW <= j0 + nbe) continue;   // strip is inside J
+            __syncthreads();
+            // old pivot rows of this strip -> LDS (zero beyond nbe / n)
+            for (int e = t; e < NB * QW; e += NT) {
+                const int k = e / QW, qq = e % QW;
+                const int col = q0 + qq;
+                T vr = 0, vi = 0;
+                if (k < nbe && col < n && !(col >= j0 && col < j0 + nbe)) {
+                    const int row = pi[j0 + k];
+                    V2 q = *reinterpret_cast<const V2*>(W + 2 * ((long)row + (long)col * ld));
+                    vr = q.x;
+                    vi = q.y;
+                }
+                s.btr[k][qq] = vr;
+                s.bti[k][qq] = vi;
+            }
+            // accumulator init: old[pi(row)][col] for rows outside J
+            constexpr int MAXRT = (RPT * NT / 16 + 3) / 4;
+            const int nrt = npad / 16;
+            A4 accR[MAXRT], accI[MAXRT];
+#pragma unroll
+            for (int u = 0; u < MAXRT; ++u) {
+                const int rt = wave + 4 * u;
+                const int row = rt * 16 + (lane & 15);
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int col = q0 + 4 * (lane >> 4) + r4;
+                    T vr = 0, vi = 0;
+                    if (rt < nrt && row < n && col < n && !(row >= j0 && row < j0 + nbe)) {
+                        const int src = pi[row];
+                        V2 q = *reinterpret_cast<const V2*>(W + 2 * ((long)src + (long)col * ld));
+                        vr = q.x;
+                        vi = q.y;
+                    }
+                    accR[u][r4] = vr;
+                    accI[u][r4] = vi;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < MAXRT; ++u) {
+                const int rt = wave + 4 * u;
+                if (rt >= nrt) continue;
+                const int row = rt * 16 + (lane & 15);
+#pragma unroll
+                for (int kk = 0; kk < NB; kk += 4) {
+                    const int k = kk + (lane >> 4);
+                    const T ar = s.btr[k][lane & 15], ai = s.bti[k][lane & 15];
+                    const T br = wr[k * npad + row], bim = wi[k * npad + row];
+                    accR[u] = mfma16x16x4(ar, br, accR[u]);
+                    accR[u] = mfma16x16x4(-ai, bim, accR[u]);
+                    accI[u] = mfma16x16x4(ai, br, accI[u]);
+                    accI[u] = mfma16x16x4(ar, bim, accI[u]);
+                }
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int col = q0 + 4 * (lane >> 4) + r4;
+                    if (row < n && col < n && !(col >= j0 && col < j0 + nbe)) {
+                        V2 q;
+                        q.x = accR[u][r4];
+                        q.y = accI[u][r4];
+                        *reinterpret_cast<V2*>(W + 2 * ((long)row + (long)col * ld)) = q;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- column order: sigma = reverse composition of the row exchanges
+    if (t == 0) {
+        for (int j = 0; j < n; ++j) sigma[j] = j;
+        for (int c = n - 1; c >= 0; --c) {
+            const int q = ipiv[c];
+            if (q != c) {
+                const int tmp = sigma[c];
+                sigma[c] = sigma[q];
+                sigma[q] = tmp;
+            }
+        }
+        // merge into the caller's status: the first error of an item wins
+        const int v = s.nan_flag ? -1 : s.info;
+        if (a.info && v != 0 && a.info[p] == 0) a.info[p] = v;
+    }
+    __syncthreads();
+    T* __restrict__ X = bptr(a.X, p, 0);
+    for (long e = t; e < (long)n * n; e += NT) {
+        const int r = (int)(e % n), j = (int)(e / n);
+        V2 q = *reinterpret_cast<const V2*>(W + 2 * ((long)r + (long)sigma[j] * ld));
+        *reinterpret_cast<V2*>(X + 2 * ((long)r + (long)j * a.X.ld)) = q;
+    }
+}
+
+template <typename T, int NB>
+size_t inv_smem_bytes(int n)
+{
+    const int npad = (n + 15) & ~15;
+    return sizeof(InvSmemFixed<T, NB>) + 2 * sizeof(T) * (size_t)NB * npad + 3 * sizeof(int) * (size_t)n;
+}
+
+template <typename T, int NB, int RPT>
+hipError_t launch_inv_t(const InvArgs<T>& a, hipStream_t s)
+{
+    const size_t sm = inv_smem_bytes<T, NB>(a.n);
+    static bool configured = false;
+    if (!configured) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inv_gj<T, NB, RPT>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        configured = true;
+    }
+    hipLaunchKernelGGL((k_inv_gj<T, NB, RPT>), dim3(a.batch), dim3(NT), sm, s, a);
+    return hipGetLastError();
+}
+
+// Panel width NB is chosen so the LDS copy of the panel stays under ~100 KB;
+// RPT (rows per thread) = ceil(n / 256) must not exceed MAXRPT for that NB.
+template <typename T, int NB, int MAXRPT>
+hipError_t launch_inv_nb(const InvArgs<T>& a, hipStream_t s)
+{
+    const int rpt = (a.n + NT - 1) / NT;
+    if (rpt <= 1) return launch_inv_t<T, NB, 1>(a, s);
+    if constexpr (MAXRPT >= 2) if (rpt == 2) return launch_inv_t<T, NB, 2>(a, s);
+    if constexpr (MAXRPT >= 3) if (rpt == 3) return launch_inv_t<T, NB, 3>(a, s);
+    if constexpr (MAXRPT >= 4) if (rpt == 4) return launch_inv_t<T, NB, 4>(a, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+int inv_max_n() { return 1024; }
+
+template <>
+hipError_t launch_inv<double>(const InvArgs<double>& a, hipStream_t s)
+{
+    if (a.batch <= 0 || a.n <= 0) return hipSuccess;
+    if (a.n > inv_max_n()) return hipErrorInvalidValue;
+    if (a.n <= 192) return launch_inv_nb<double, 32, 1>(a, s);
+    if (a.n <= 384) return launch_inv_nb<double, 16, 2>(a, s);
+    return launch_inv_nb<double, 8, 4>(a, s);
+}
+
+template <>
+hipError_t launch_inv<float>(const InvArgs<float>& a, hipStream_t s)
+{
+    if (a.batch <= 0 || a.n <= 0) return hipSuccess;
+    if (a.n > inv_max_n()) return hipErrorInvalidValue;
+    if (a.n <= 256) return launch_inv_nb<float, 32, 1>(a, s);
+    if (a.n <= 512) return launch_inv_nb<float, 16, 2>(a, s);
+    return launch_inv_nb<float, 8, 4>(a, s);
+}
+
+}  // namespace nt

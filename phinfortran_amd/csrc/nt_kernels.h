@@ -1,0 +1,86 @@
+// nt_kernels.h -- launch interfaces of the batched transport kernels.
+#pragma once
+
+#include "nt_device.h"
+
+namespace nt {
+
+enum Op { OP_N = 0, OP_C = 1 };
+
+// C[p,g] = alpha * op(A[p,g]) * op(B[p,g]) + beta * Cin[p,g] + diag[p] * I
+// C2[p,g] += alpha * op(A) * op(B)          (optional second accumulator)
+// maxabs[p] = max(maxabs[p], max |C[p,0]|)  (optional, group 0 only)
+template <typename T> struct GemmArgs {
+    int M, N, K;
+    int opA, opB;
+    BMat<T> A, B, Cin;
+    BMatW<T> C, C2;
+    T alpha_re, alpha_im, beta_re, beta_im;
+    const T* diag;          // interleaved complex per item p (stride 2), or null
+    const int* active;      // per item p, or null
+    unsigned long long* maxabs;  // per item p (bits of a non-negative double), or null
+    int batch, groups;
+};
+
+template <typename T> hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s);
+
+// Batched in-place Gauss-Jordan inverse with partial (row) pivoting.
+//   W[p]  n x n work matrix, destroyed.  If src.base is set, the kernel first
+//         forms W = z[p] * I - src[p].
+//   X[p]  receives inv(W).  Must not alias W.
+//   info[p]: left untouched on success; if it is 0 and the inverse hits an exact
+//   zero pivot at column k it becomes k (1-based), -1 for NaN input.
+template <typename T> struct InvArgs {
+    int n;
+    BMatW<T> W;
+    BMat<T> src;
+    const T* z;             // interleaved complex per item, used with src
+    BMatW<T> X;
+    const int* active;
+    int* info;
+    int batch;
+};
+
+template <typename T> hipError_t launch_inv(const InvArgs<T>& a, hipStream_t s);
+int inv_max_n();
+
+// Elementwise helpers (nt_elem.hip).
+// out[p] = sign * (dagger ? M^+ : M) for an n x n matrix
+template <typename T>
+hipError_t launch_copy_op(int m, int n, BMat<T> in, BMatW<T> out, int dagger, T sign,
+                          const int* active, int batch, hipStream_t s);
+// Gamma[p] = scale * i (S - S^+)
+template <typename T>
+hipError_t launch_gamma(int n, BMat<T> S, BMatW<T> G, T scale, const int* active, int batch, hipStream_t s);
+// out[p] = z[p] I - A[p] - B[p]   (B optional)
+template <typename T>
+hipError_t launch_shift_sub(int n, const T* z, BMat<T> A, BMat<T> B, BMatW<T> out, const int* active,
+                            int batch, hipStream_t s);
+// Decimation start: e1 = es = H00, alpha = opA(V), beta = opB(V)
+template <typename T>
+hipError_t launch_dec_init(int n, BMat<T> H00, BMat<T> V, int alpha_dagger, BMatW<T> e1, BMatW<T> es,
+                           BMatW<T> ab, int batch, hipStream_t s);
+// T[p] = Re sum_ij X_ij Y_ji over n1 x n2 X and n2 x n1 Y (into out[p] * out_stride)
+template <typename T>
+hipError_t launch_trace_prod(int n1, int n2, BMat<T> X, BMat<T> Y, double* out, int out_stride,
+                             const int* active, int batch, hipStream_t s);
+// Im trace of g (for surface DOS), out[p] = coef * sqrt(E[p]) * Im Tr g / pi
+template <typename T>
+hipError_t launch_trace_im(int n, BMat<T> g, const double* E, double coef, double* out, int batch,
+                           hipStream_t s);
+
+// Shifts of one (omega,k) point: z arrays interleaved (re, im), length 2*batch.
+//   zl  = E - i eta(E)     (left advanced / right reflected)
+//   zr  = E + i eta(E)     (right retarded / left reflected)
+//   zd  = E + i f eta(E)   (device and closing lead layers)
+hipError_t launch_make_shifts(const double* E, int batch, double eta, double eta0, int eta_mode,
+                              double dev_factor, double* zl, double* zr, double* zd, float* zl32,
+                              float* zr32, float* zd32, hipStream_t s);
+// active[p] &= (maxabs[p] > conv); iters[p] += was_active; maxabs[p] = 0; counts active into *n_active
+hipError_t launch_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv,
+                                  int* n_active, int batch, hipStream_t s);
+hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
+// Final transmission clean-up: clamp < 1e-60 to 0, zero when flagged, status merge.
+hipError_t launch_finish_T(double* T, int n, int zero_all, hipStream_t s);
+
+}  // namespace nt

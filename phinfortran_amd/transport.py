@@ -1,0 +1,252 @@
+"""Host-side mirror of phinfortran's per-(omega,k) transport interface on the
+MI355X engine.
+
+The functions keep the reference's names, argument meaning and error
+behaviour (a nonzero status raises, where the reference would `stop`):
+
+    Engine.surface_green_tdb   negf/surface.f90:385-605   (tb/surface.f90:129-343)
+    Engine.get_G_full_tdb      negf/lesser.f90:475-809
+    Engine.G_block             negf/device.f90:289-560    (tb/device.f90:291-575)
+    Engine.solve_tdb           the driver's ie loop, negf/phinfortran.f90:1196-1251,
+                               batched over energies and k-points on the GPU
+
+Matrices are numpy / torch complex128 in the natural (row-major) orientation;
+the conversion to the column-major C ABI happens here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+def _f(a):
+    """complex128, Fortran-ordered (the C ABI layout of one matrix)."""
+    return np.asfortranarray(np.asarray(a, dtype=np.complex128))
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _stack_f(blocks):
+    """(..., m, n) array -> contiguous buffer of column-major matrices."""
+    a = np.asarray(blocks, dtype=np.complex128)
+    return np.ascontiguousarray(np.swapaxes(a, -1, -2))
+
+
+def params(kind: str = "phonon", **kw) -> N.NtParams:
+    """Scalar parameters of the reference's module globals.
+
+    kind='phonon': negf conventions, eta(E) = E*eta + eta0, device eta x10.
+    kind='tb':     tb conventions, constant eta in leads and device."""
+    p = N.NtParams()
+    N.lib().nt_params_default(C.byref(p))
+    if kind == "tb":
+        p.eta_mode = N.NT_ETA_ABSOLUTE
+        p.device_eta_factor = 1.0
+        p.eta0 = 0.0
+    elif kind != "phonon":
+        raise ValueError(kind)
+    for k, v in kw.items():
+        if k == "precision" and isinstance(v, str):
+            v = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[v]
+        setattr(p, k, v)
+    return p
+
+
+class Engine:
+    """One GPU context (HIP stream + device workspace)."""
+
+    def __init__(self, device: int = 0, batch_limit: Optional[int] = None, use_torch_stream: bool = True):
+        self._L = N.lib()
+        ctx = C.c_void_p()
+        st = self._L.nt_create(C.byref(ctx), int(device))
+        if st != 0:
+            raise N.TransportError(st, f"nt_create failed: {N.status_string(st)}")
+        self.ctx = ctx
+        self.device = device
+        if batch_limit is not None:
+            self._check(self._L.nt_set_batch_limit(self.ctx, int(batch_limit)), "nt_set_batch_limit")
+        self._torch_stream = use_torch_stream
+
+    def close(self):
+        if self.ctx:
+            self._L.nt_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != 0:
+            err = self._L.nt_last_error(self.ctx).decode() if self.ctx else ""
+            raise N.TransportError(st, f"{what}: {N.status_string(st)} ({st}) {err}".strip())
+
+    def sync(self):
+        self._check(self._L.nt_sync(self.ctx), "nt_sync")
+
+    # ------------------------------------------------------------ per point
+    def surface_green_tdb(self, Haml, Hamr, Hamlv, Hamrv, E, p: N.NtParams, want_plus_minus=True):
+        """surface_green_tdb: returns gl_adv, gr_ret, gl_plus, gr_minus, gml_adv,
+        gmr_ret, sl_adv, sr_ret (numpy, row-major) and the iteration counts."""
+        Haml, Hamr, Hamlv, Hamrv = map(_f, (Haml, Hamr, Hamlv, Hamrv))
+        nl, nr = Haml.shape[0], Hamr.shape[0]
+        out = {k: np.zeros((nl, nl), np.complex128, order="F") for k in ("gl_adv", "gl_plus", "gml_adv", "sl_adv")}
+        out.update({k: np.zeros((nr, nr), np.complex128, order="F") for k in ("gr_ret", "gr_minus", "gmr_ret", "sr_ret")})
+        iters = (C.c_int * 4)()
+        st = self._L.nt_surface_green_tdb(
+            self.ctx, C.byref(p), nl, nr, float(E), _p(Haml), _p(Hamr), _p(Hamlv), _p(Hamrv),
+            _p(out["gl_adv"]), _p(out["gr_ret"]), _p(out["gl_plus"]) if want_plus_minus else None,
+            _p(out["gr_minus"]) if want_plus_minus else None, _p(out["gml_adv"]), _p(out["gmr_ret"]),
+            _p(out["sl_adv"]), _p(out["sr_ret"]), C.cast(iters, C.c_void_p))
+        self._check(st, "surface_green_tdb")
+        res = {k: np.ascontiguousarray(v) for k, v in out.items()}
+        res["iters"] = list(iters)
+        return res
+
+    def get_G_full_tdb(self, gl, gr, gml, gmr, sl, sr, E, hamd, hamdv, hamld, hamrd, haml, hamr,
+                       p: N.NtParams, first_point=False):
+        """get_G_full_tdb with the retarded lead quantities of get_G_full_tdb_mpi
+        (gl = gl_adv^+, gml = -gml_adv, sl = sl_adv^+).  hamd: (L, b, b)."""
+        hamd = np.asarray(hamd, np.complex128)
+        L, b = hamd.shape[0], hamd.shape[1]
+        hd = _stack_f(hamd)
+        hdv = _stack_f(hamdv) if L > 1 else np.zeros((1, b, b), np.complex128)
+        gl, gr, gml, gmr, sl, sr, hamld, hamrd, haml, hamr = map(_f, (gl, gr, gml, gmr, sl, sr, hamld, hamrd, haml, hamr))
+        nl, nr = haml.shape[0], hamr.shape[0]
+        G_D_RL = np.zeros((nr, nl), np.complex128, order="F")
+        G_D_LR = np.zeros((nl, nr), np.complex128, order="F")
+        G_L = np.zeros((nl, nl), np.complex128, order="F")
+        G_R = np.zeros((nr, nr), np.complex128, order="F")
+        tl, tr = C.c_double(), C.c_double()
+        st = self._L.nt_get_G_full_tdb(
+            self.ctx, C.byref(p), int(bool(first_point)), nl, nr, b, L, float(E),
+            _p(gl), _p(gr), _p(gml), _p(gmr), _p(sl), _p(sr), _p(hd), _p(hdv), _p(hamld), _p(hamrd),
+            _p(haml), _p(hamr), C.cast(C.byref(tl), C.c_void_p), C.cast(C.byref(tr), C.c_void_p),
+            _p(G_D_RL), _p(G_D_LR), _p(G_L), _p(G_R))
+        self._check(st, "get_G_full_tdb")
+        return dict(transl=tl.value, transr=tr.value, G_D_RL=np.ascontiguousarray(G_D_RL),
+                    G_D_LR=np.ascontiguousarray(G_D_LR), G_L=np.ascontiguousarray(G_L),
+                    G_R=np.ascontiguousarray(G_R))
+
+    def G_block(self, gl, gr, gml, gmr, sl, sr, E, Hd: Sequence, Hdv: Sequence, Hld, Hrd, Haml, Hamr,
+                p: N.NtParams, Hdv_back: Optional[Sequence] = None, Hrd_fwd=None):
+        """G_block on variable layer sizes (layer_list = [h.shape[0] for h in Hd])."""
+        L = len(Hd)
+        Hd = [_f(h) for h in Hd]
+        Hdv = [_f(h) for h in Hdv]
+        layer_list = (C.c_int * L)(*[h.shape[0] for h in Hd])
+        pHd = (C.c_void_p * L)(*[h.ctypes.data for h in Hd])
+        pHdv = (C.c_void_p * max(L - 1, 1))(*[h.ctypes.data for h in Hdv])
+        pback = None
+        if Hdv_back is not None:
+            Hdv_back = [_f(h) for h in Hdv_back]
+            pback = (C.c_void_p * max(L - 1, 1))(*[h.ctypes.data for h in Hdv_back])
+        gl, gr, gml, gmr, sl, sr, Hld, Hrd, Haml, Hamr = map(_f, (gl, gr, gml, gmr, sl, sr, Hld, Hrd, Haml, Hamr))
+        Hrd_fwd = None if Hrd_fwd is None else _f(Hrd_fwd)
+        nl, nr = Haml.shape[0], Hamr.shape[0]
+        G_D_RL = np.zeros((nr, nl), np.complex128, order="F")
+        G_D_LR = np.zeros((nl, nr), np.complex128, order="F")
+        G_L = np.zeros((nl, nl), np.complex128, order="F")
+        G_R = np.zeros((nr, nr), np.complex128, order="F")
+        tl, tr = C.c_double(), C.c_double()
+        st = self._L.nt_G_block(
+            self.ctx, C.byref(p), nl, nr, L, C.cast(layer_list, C.c_void_p), float(E),
+            _p(gl), _p(gr), _p(gml), _p(gmr), _p(sl), _p(sr), C.cast(pHd, C.c_void_p), C.cast(pHdv, C.c_void_p),
+            None if pback is None else C.cast(pback, C.c_void_p), _p(Hld), _p(Hrd), _p(Hrd_fwd), _p(Haml), _p(Hamr),
+            C.cast(C.byref(tl), C.c_void_p), C.cast(C.byref(tr), C.c_void_p),
+            _p(G_D_RL), _p(G_D_LR), _p(G_L), _p(G_R))
+        self._check(st, "G_block")
+        return dict(transl=tl.value, transr=tr.value, G_D_RL=np.ascontiguousarray(G_D_RL),
+                    G_D_LR=np.ascontiguousarray(G_D_LR), G_L=np.ascontiguousarray(G_L),
+                    G_R=np.ascontiguousarray(G_R))
+
+    # -------------------------------------------------------------- batched
+    def upload_blocks(self, blocks_per_k):
+        """Stack tdb block sets of nk k-points into device tensors (ABI layout)."""
+        import torch
+        if not isinstance(blocks_per_k, (list, tuple)):
+            blocks_per_k = [blocks_per_k]
+        dev = torch.device("cuda", self.device)
+        out = {}
+        for name in ("Haml", "Hamlv", "Hamr", "Hamrv", "Hamd", "Hamdv", "Hamld", "Hamrd"):
+            arr = np.stack([np.asarray(getattr(b, name) if not isinstance(b, dict) else b[name]) for b in blocks_per_k])
+            if name == "Hamdv" and arr.shape[1] == 0:
+                arr = np.zeros((arr.shape[0], 1) + arr.shape[2:], np.complex128)
+            out[name] = torch.from_numpy(_stack_f(arr)).to(dev)
+        b0 = blocks_per_k[0]
+        get = (lambda k: b0[k]) if isinstance(b0, dict) else (lambda k: getattr(b0, k))
+        out["nl"] = get("Haml").shape[0]
+        out["nr"] = get("Hamr").shape[0]
+        out["b"] = get("Hamd").shape[1]
+        out["L"] = get("Hamd").shape[0]
+        out["nk"] = len(blocks_per_k)
+        return out
+
+    def solve_tdb(self, dblocks, E, p: N.NtParams, kidx=None, first_point_mask=None, extras=(), stream_sync=True):
+        """Batched (omega,k)-solve on device-resident blocks (see upload_blocks).
+        E, kidx: torch tensors (float64 / int32) on the GPU or array-likes.
+        Returns dict(T_LR, T_RL, status, iters[, extras...]) of torch tensors."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
+        npts = E.numel()
+        nl, nr, b, L, nk = dblocks["nl"], dblocks["nr"], dblocks["b"], dblocks["L"], dblocks["nk"]
+        if kidx is not None:
+            kidx = torch.as_tensor(kidx, dtype=torch.int32, device=dev).contiguous()
+        if first_point_mask is not None:
+            first_point_mask = torch.as_tensor(first_point_mask, dtype=torch.int32, device=dev).contiguous()
+        T_LR = torch.empty(npts, dtype=torch.float64, device=dev)
+        T_RL = torch.empty(npts, dtype=torch.float64, device=dev)
+        status = torch.zeros(npts, dtype=torch.int32, device=dev)
+        iters = torch.zeros(npts, dtype=torch.int32, device=dev)
+        shapes = dict(gl_adv=(nl, nl), gr_ret=(nr, nr), gl_plus=(nl, nl), gr_minus=(nr, nr), gml_adv=(nl, nl),
+                      gmr_ret=(nr, nr), sl_adv=(nl, nl), sr_ret=(nr, nr), G_D_LR=(nr, nl), G_D_RL=(nl, nr),
+                      G_L=(nl, nl), G_R=(nr, nr))
+        ex = N.NtSolveExtras()
+        res = {}
+        for name in extras:
+            if name in ("sdos_l", "sdos_r"):
+                t = torch.zeros(npts, dtype=torch.float64, device=dev)
+            else:
+                # stored column-major: tensor [p, col, row]
+                t = torch.zeros((npts,) + shapes[name], dtype=torch.complex128, device=dev)
+            res[name] = t
+            setattr(ex, name, t.data_ptr())
+        if self._torch_stream:
+            self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        st = self._L.nt_solve_tdb_batched_dev(
+            self.ctx, C.byref(p), nl, nr, b, L, npts, ptr(E), ptr(kidx), nk,
+            ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
+            ptr(dblocks["Hamd"]), ptr(dblocks["Hamdv"]), ptr(dblocks["Hamld"]), ptr(dblocks["Hamrd"]),
+            ptr(first_point_mask), ptr(T_LR), ptr(T_RL), ptr(status), ptr(iters), C.byref(ex))
+        self._check(st, "solve_tdb")
+        if stream_sync:
+            torch.cuda.synchronize(dev)
+        out = dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters)
+        for name, t in res.items():
+            out[name] = t if t.dim() == 1 else t.transpose(-1, -2)
+        return out
+
+    # ---------------------------------------------------------- measurement
+    def profile(self, on: bool):
+        self._check(self._L.nt_profile_enable(self.ctx, int(on)), "nt_profile_enable")
+
+    def profile_reset(self):
+        self._check(self._L.nt_profile_reset(self.ctx), "nt_profile_reset")
+
+    def profile_get(self, kind: int):
+        n, ms, fl = C.c_longlong(), C.c_double(), C.c_double()
+        self._check(self._L.nt_profile_get(self.ctx, int(kind), C.byref(n), C.byref(ms), C.byref(fl)), "nt_profile_get")
+        return dict(launches=n.value, ms=ms.value, flops=fl.value)
+
+    def last_mean_iters(self) -> float:
+        return float(self._L.nt_last_mean_iters(self.ctx))

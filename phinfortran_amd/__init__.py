@@ -4,16 +4,14 @@
     transport   host mirror of the reference interface over the HIP C ABI
     build       in-tree build of libnegf_transport.so (gfx950)
 """
+import importlib
+
 from . import lattice  # noqa: F401
 
 __all__ = ["lattice", "transport", "build"]
 
 
 def __getattr__(name):
-    if name == "transport":
-        from . import transport
-        return transport
-    if name == "build":
-        from . import build
-        return build
+    if name in ("transport", "build", "_native"):
+        return importlib.import_module(f"{__name__}.{name}")
     raise AttributeError(name)

@@ -187,6 +187,23 @@ int nt_solve_tdb_batched(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b,
                          double* T_LR, double* T_RL, int* status, int* iters,
                          const nt_solve_extras* extras);
 
+/* ------------------------------------------------ batched building blocks
+ * The two kernels every solve is made of, on device memory (column-major,
+ * leading dimension = rows, item p at base + p*stride elements; stride 0
+ * shares one matrix across the batch).  No counterpart in the reference,
+ * whose equivalents are Fortran matmul and inv (negf/util.f90:16-67). */
+
+/* C[p] = alpha op(A[p]) op(B[p]) + beta C[p], op = 'N' or 'C' (conjugate transpose). */
+int nt_zgemm_batched_dev(nt_ctx* ctx, char opA, char opB, int m, int n, int k, nt_c128 alpha,
+                         const nt_c128* A, int lda, long long strideA,
+                         const nt_c128* B, int ldb, long long strideB, nt_c128 beta,
+                         nt_c128* C, int ldc, long long strideC, int batch);
+
+/* X[p] = inv(A[p]) by Gauss-Jordan with partial pivoting; A is overwritten.
+ * info[p] (may be NULL): 0, k > 0 for an exact zero pivot, -1 for NaN. */
+int nt_zinv_batched_dev(nt_ctx* ctx, int n, nt_c128* A, long long strideA, nt_c128* X, long long strideX,
+                        int* info, int batch);
+
 /* ------------------------------------------------------- measurement */
 /* Kernel classes for nt_profile_get. */
 enum { NT_K_GEMM = 0, NT_K_INV = 1, NT_K_ELEM = 2, NT_K_COUNT = 3 };

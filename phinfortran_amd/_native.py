@@ -33,7 +33,12 @@ EXPORTS = [
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
+    "nt_zgemm_batched_dev", "nt_zinv_batched_dev",
 ]
+
+
+class NtC128(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
 
 
 class NtParams(C.Structure):
@@ -102,6 +107,10 @@ def lib():
     L.nt_profile_reset.argtypes = [P]
     L.nt_last_mean_iters.argtypes = [P]
     L.nt_last_mean_iters.restype = D
+    LL = C.c_longlong
+    L.nt_zgemm_batched_dev.argtypes = [P, C.c_char, C.c_char, I, I, I, NtC128, P, I, LL, P, I, LL, NtC128,
+                                       P, I, LL, I]
+    L.nt_zinv_batched_dev.argtypes = [P, I, P, LL, P, LL, P, I]
     _lib = L
     return L
 

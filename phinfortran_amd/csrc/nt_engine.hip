@@ -48,6 +48,9 @@ struct nt_ctx {
     double ms[NT_K_COUNT] = {0, 0, 0};
     double flops[NT_K_COUNT] = {0, 0, 0};
     double last_mean_iters = 0.0;
+    // device address ranges a launch may touch: the workspace and the caller's
+    // buffers of the current call (sizes derived from the call's arguments)
+    std::vector<std::pair<uintptr_t, uintptr_t>> regions;
 };
 
 namespace {
@@ -126,11 +129,71 @@ struct Arena {
     }
 };
 
+
+
+template <typename T>
+BMat<T> bm(const T* base, long stride, int ld, const int* idx = nullptr, long gstride = 0, int idx_n = 0)
+{
+    return BMat<T>{base, stride, gstride, idx, ld, idx_n};
+}
+template <typename T>
+BMatW<T> bw(T* base, long stride, int ld, const int* idx = nullptr, long gstride = 0, int idx_n = 0)
+{
+    return BMatW<T>{base, stride, gstride, idx, ld, idx_n};
+}
+template <typename T>
+BMat<T> ro(const BMatW<T>& w)
+{
+    return BMat<T>{w.base, w.stride, w.gstride, w.idx, w.ld, w.idx_n};
+}
+template <typename T>
+BMat<T> none()
+{
+    return BMat<T>{nullptr, 0, 0, nullptr, 0, 0};
+}
+template <typename T>
+BMatW<T> nonew()
+{
+    return BMatW<T>{nullptr, 0, 0, nullptr, 0, 0};
+}
+
+// ---------------------------------------------------------- bounds audit
+// Every launch checks, on the host, that each operand's whole address range
+// (all items, groups, rows and columns) lies inside one registered region.
+// An addressing mistake then becomes NT_ERR_ARG instead of a GPU fault.
+void reg_region(nt_ctx* c, const void* p, size_t bytes)
+{
+    if (p && bytes) c->regions.push_back({(uintptr_t)p, (uintptr_t)p + bytes});
+}
+
+void unreg_region(nt_ctx* c, const void* p)
+{
+    for (size_t i = 0; i < c->regions.size(); ++i)
+        if (c->regions[i].first == (uintptr_t)p) {
+            c->regions.erase(c->regions.begin() + i);
+            return;
+        }
+}
+
+void check_range(nt_ctx* c, const void* p, size_t bytes, const char* what)
+{
+    if (!p || !bytes) return;
+    const uintptr_t b = (uintptr_t)p, e = b + bytes;
+    for (auto& r : c->regions)
+        if (b >= r.first && e <= r.second) return;
+    char buf[256];
+    snprintf(buf, sizeof(buf), "bounds audit: %s touches [%p, +%zu) outside every registered buffer", what, p,
+             bytes);
+    c->err = buf;
+    throw Fail{NT_ERR_ARG};
+}
+
 void ensure_ws(nt_ctx* c, size_t bytes)
 {
     if (c->ws_bytes >= bytes) return;
     if (c->ws) {
         hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        unreg_region(c, c->ws);
         hipcheck(c, hipFree(c->ws), "hipFree");
         c->ws = nullptr;
         c->ws_bytes = 0;
@@ -138,32 +201,20 @@ void ensure_ws(nt_ctx* c, size_t bytes)
     size_t want = bytes + bytes / 8;
     hipcheck(c, hipMalloc(&c->ws, want), "hipMalloc workspace");
     c->ws_bytes = want;
+    reg_region(c, c->ws, c->ws_bytes);
 }
 
-template <typename T>
-BMat<T> bm(const T* base, long stride, int ld, const int* idx = nullptr, long gstride = 0)
+template <typename T, typename M>
+void audit(nt_ctx* c, const M& m, int rows, int cols, int batch, int groups, const char* what)
 {
-    return BMat<T>{base, stride, gstride, idx, ld};
-}
-template <typename T>
-BMatW<T> bw(T* base, long stride, int ld, const int* idx = nullptr, long gstride = 0)
-{
-    return BMatW<T>{base, stride, gstride, idx, ld};
-}
-template <typename T>
-BMat<T> ro(const BMatW<T>& w)
-{
-    return BMat<T>{w.base, w.stride, w.gstride, w.idx, w.ld};
-}
-template <typename T>
-BMat<T> none()
-{
-    return BMat<T>{nullptr, 0, 0, nullptr, 0};
-}
-template <typename T>
-BMatW<T> nonew()
-{
-    return BMatW<T>{nullptr, 0, 0, nullptr, 0};
+    if (!m.base || rows <= 0 || cols <= 0 || batch <= 0) return;
+    if (m.stride < 0 || m.gstride < 0 || m.ld < rows || (m.idx && m.idx_n <= 0)) {
+        c->err = std::string("bounds audit: bad descriptor for ") + what;
+        throw Fail{NT_ERR_ARG};
+    }
+    const long items = m.idx ? m.idx_n : batch;
+    const long last = (items - 1) * m.stride + (long)(groups - 1) * m.gstride + (rows - 1) + (long)(cols - 1) * m.ld;
+    check_range(c, m.base, (size_t)(last + 1) * 2 * sizeof(T), what);
 }
 
 // C = alpha op(A) op(B) + beta Cin + diag
@@ -194,6 +245,14 @@ void gemm(nt_ctx* c, int M, int N, int K, int opA, BMat<T> A, int opB, BMat<T> B
     a.batch = batch;
     a.groups = groups;
     const double items = (nitems < 0 ? (double)batch : nitems) * groups;
+    audit<T>(c, A, opA == OP_N ? M : K, opA == OP_N ? K : M, batch, groups, "gemm A");
+    audit<T>(c, B, opB == OP_N ? K : N, opB == OP_N ? N : K, batch, groups, "gemm B");
+    audit<T>(c, C, M, N, batch, groups, "gemm C");
+    audit<T>(c, Cin, M, N, batch, groups, "gemm Cin");
+    audit<T>(c, C2, M, N, batch, groups, "gemm C2");
+    check_range(c, diag, sizeof(T) * 2 * batch, "gemm diag");
+    check_range(c, active, sizeof(int) * batch, "gemm active");
+    check_range(c, maxabs, sizeof(unsigned long long) * batch, "gemm maxabs");
     launch(c, NT_K_GEMM, 8.0 * M * N * K * items, "gemm", [&] { return launch_gemm<T>(a, c->stream); });
 }
 
@@ -211,7 +270,55 @@ void inverse(nt_ctx* c, int n, BMatW<T> W, BMat<T> src, const T* z, BMatW<T> X, 
     a.info = info;
     a.batch = batch;
     const double items = nitems < 0 ? (double)batch : nitems;
+    audit<T>(c, W, n, n, batch, 1, "inverse W");
+    audit<T>(c, src, n, n, batch, 1, "inverse src");
+    audit<T>(c, X, n, n, batch, 1, "inverse X");
+    if (src.base) check_range(c, z, sizeof(T) * 2 * batch, "inverse z");
+    check_range(c, info, sizeof(int) * batch, "inverse info");
+    check_range(c, active, sizeof(int) * batch, "inverse active");
+    if (W.base == X.base) {
+        c->err = "inverse: work and output alias";
+        throw Fail{NT_ERR_ARG};
+    }
     launch(c, NT_K_INV, 8.0 * n * (double)n * n * items, "inverse", [&] { return launch_inv<T>(a, c->stream); });
+}
+
+template <typename T>
+void copy_op(nt_ctx* c, int m, int n, BMat<T> in, BMatW<T> out, int dagger, T sign, int batch)
+{
+    audit<T>(c, in, m, n, batch, 1, "copy in");
+    audit<T>(c, out, dagger ? n : m, dagger ? m : n, batch, 1, "copy out");
+    launch(c, NT_K_ELEM, 0, "copy",
+           [&] { return launch_copy_op<T>(m, n, in, out, dagger, sign, nullptr, batch, c->stream); });
+}
+
+template <typename T>
+void gamma_op(nt_ctx* c, int n, BMat<T> S, BMatW<T> G, int batch)
+{
+    audit<T>(c, S, n, n, batch, 1, "gamma in");
+    audit<T>(c, G, n, n, batch, 1, "gamma out");
+    launch(c, NT_K_ELEM, 0, "gamma", [&] { return launch_gamma<T>(n, S, G, (T)1, nullptr, batch, c->stream); });
+}
+
+template <typename T>
+void shift_sub(nt_ctx* c, int n, const T* z, BMat<T> A, BMat<T> B, BMatW<T> out, int batch)
+{
+    check_range(c, z, sizeof(T) * 2 * batch, "shift z");
+    audit<T>(c, A, n, n, batch, 1, "shift A");
+    audit<T>(c, B, n, n, batch, 1, "shift B");
+    audit<T>(c, out, n, n, batch, 1, "shift out");
+    launch(c, NT_K_ELEM, 0, "shift_sub",
+           [&] { return launch_shift_sub<T>(n, z, A, B, out, nullptr, batch, c->stream); });
+}
+
+template <typename T>
+void trace_prod(nt_ctx* c, int n1, int n2, BMat<T> X, BMat<T> Y, double* out, int batch)
+{
+    audit<T>(c, X, n1, n2, batch, 1, "trace X");
+    audit<T>(c, Y, n2, n1, batch, 1, "trace Y");
+    check_range(c, out, sizeof(double) * batch, "trace out");
+    launch(c, NT_K_ELEM, 0, "trace",
+           [&] { return launch_trace_prod<T>(n1, n2, X, Y, out, 1, nullptr, batch, c->stream); });
 }
 
 // ------------------------------------------------------- engine kernels
@@ -312,9 +419,15 @@ void decimate(nt_ctx* c, int n, int Q, BMat<T> H00, BMat<T> V, int alpha_dagger,
     int* nact = sa.ptr<int>(sa.take(sizeof(int)));
 
     hipStream_t s = c->stream;
+    audit<T>(c, H00, n, n, Q, 1, "dec H00");
+    audit<T>(c, V, n, n, Q, 1, "dec V");
+    audit<T>(c, bw(ab0, 2 * nn, n, nullptr, nn), n, n, Q, 2, "dec ab");
     launch(c, NT_K_ELEM, 0, "dec_init",
            [&] { return launch_dec_init<T>(n, H00, V, alpha_dagger, bw(e1, nn, n), bw(es, nn, n),
                                            bw(ab0, 2 * nn, n, nullptr, nn), Q, s); });
+    check_range(c, z, sizeof(T) * 2 * Q, "dec z");
+    check_range(c, status, sizeof(int) * Q, "dec status");
+    check_range(c, iters, sizeof(int) * Q, "dec iters");
     hipcheck(c, hipMemsetAsync(iters, 0, sizeof(int) * Q, s), "memset");
     hipcheck(c, hipMemsetAsync(maxabs, 0, sizeof(unsigned long long) * Q, s), "memset");
     hipcheck(c, hipMemsetAsync(nact, 0, sizeof(int), s), "memset");
@@ -369,9 +482,7 @@ void decimate(nt_ctx* c, int n, int Q, BMat<T> H00, BMat<T> V, int alpha_dagger,
                 nullptr, Q);
         gemm<T>(c, n, n, n, OP_N, bm<T>(t12, 2 * nn, n), alpha_dagger ? OP_N : OP_C, V, out.sigma, 1.0, none<T>(),
                 0.0, nullptr, Q);
-        if (out.gamma.base)
-            launch(c, NT_K_ELEM, 0, "gamma",
-                   [&] { return launch_gamma<T>(n, ro(out.sigma), out.gamma, (T)1, nullptr, Q, s); });
+        if (out.gamma.base) gamma_op<T>(c, n, ro(out.sigma), out.gamma, Q);
     }
 }
 
@@ -480,8 +591,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         const int opVcH = fwd ? OP_C : OP_N;     // op(.) giving Vc^+    (nr x bN)
         gemm<T>(c, nr, bN, bN, opVcH, Vc, OP_N, R(G, bN), W(T1, nr), 1.0, none<T>(), 0.0, nullptr, P);
         // Aw = (z - Hamr - sr) - T1 Vc   (lesser.f90:616-633)
-        launch(c, NT_K_ELEM, 0, "shift_sub",
-               [&] { return launch_shift_sub<T>(nr, zd, in.Hamr, in.sr, W(X, nr), nullptr, P, s); });
+        shift_sub<T>(c, nr, zd, in.Hamr, in.sr, W(X, nr), P);
         gemm<T>(c, nr, nr, bN, OP_N, R(T1, nr), opVc, Vc, W(Aw, nr), -1.0, R(X, nr), 1.0, nullptr, P);
         inverse<T>(c, nr, W(Aw, nr), none<T>(), nullptr, gRo, st, P);
         gemm<T>(c, nl, nr, bN, OP_N, R(G0, nl), opVc, Vc, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
@@ -489,8 +599,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         // Caroli T_LR = Re Tr[(G_LR gmr)(G_LR^+ gml)]
         gemm<T>(c, nl, nr, nr, OP_N, ro(gdlr), OP_N, in.gmr, W(X, nl), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, nr, nl, nl, OP_C, ro(gdlr), OP_N, in.gml, W(Y, nr), 1.0, none<T>(), 0.0, nullptr, P);
-        launch(c, NT_K_ELEM, 0, "trace",
-               [&] { return launch_trace_prod<T>(nl, nr, R(X, nl), R(Y, nr), out.T_LR, 1, nullptr, P, s); });
+        trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P);
     }
     // ---------------- right to left
     {
@@ -519,8 +628,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         const int b1 = in.bsz[0];
         // closing lead layer 0: V = Hld (nl x b1); (V G) V^+  (lesser.f90:757-771)
         gemm<T>(c, nl, b1, b1, OP_N, in.Hld, OP_N, R(G, b1), W(T1, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        launch(c, NT_K_ELEM, 0, "shift_sub",
-               [&] { return launch_shift_sub<T>(nl, zd, in.Haml, in.sl, W(X, nl), nullptr, P, s); });
+        shift_sub<T>(c, nl, zd, in.Haml, in.sl, W(X, nl), P);
         gemm<T>(c, nl, nl, b1, OP_N, R(T1, nl), OP_C, in.Hld, W(Aw, nl), -1.0, R(X, nl), 1.0, nullptr, P);
         BMatW<T> gL = out.G_L.base ? out.G_L : W(Gn, nl);
         inverse<T>(c, nl, W(Aw, nl), none<T>(), nullptr, gL, st, P);
@@ -529,8 +637,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         // Caroli T_RL = Re Tr[(G_RL gml)(G_RL^+ gmr)]
         gemm<T>(c, nr, nl, nl, OP_N, ro(gdrl), OP_N, in.gml, W(X, nr), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, nl, nr, nr, OP_C, ro(gdrl), OP_N, in.gmr, W(Y, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        launch(c, NT_K_ELEM, 0, "trace",
-               [&] { return launch_trace_prod<T>(nr, nl, R(X, nr), R(Y, nl), out.T_RL, 1, nullptr, P, s); });
+        trace_prod<T>(c, nr, nl, R(X, nr), R(Y, nl), out.T_RL, P);
     }
     (void)gLo;
 }
@@ -623,6 +730,10 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         // lead stack [left k-points..., right k-points...] so one decimation
         // batch serves both leads
         const long n_a = nll * a.nk;
+        check_range(c, Haml, sizeof(T) * 2 * n_a, "Haml");
+        check_range(c, Hamr, sizeof(T) * 2 * n_a, "Hamr");
+        check_range(c, Hamlv, sizeof(T) * 2 * n_a, "Hamlv");
+        check_range(c, Hamrv, sizeof(T) * 2 * n_a, "Hamrv");
         hipLaunchKernelGGL(k_copy_lead_stack, dim3(nblk(n_a)), dim3(256), 0, s,
                            reinterpret_cast<const double2*>(Haml), reinterpret_cast<const double2*>(Hamr),
                            reinterpret_cast<double2*>(lh), n_a);
@@ -649,12 +760,12 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         if (same) {
             const long n2 = nll;
             DecimationOut<T> o{bw(gbuf, n2, nl), bw(sigbuf, n2, nl), bw(gambuf, n2, nl)};
-            decimate<T>(c, nl, 2 * P, bm<T>(lh, n2, nl, idx), bm<T>(lv, n2, nl, idx), 1, zdec, prm->convergence,
-                        prm->max_iter, o, stdec, itdec, ar, o_scr);
+            decimate<T>(c, nl, 2 * P, bm<T>(lh, n2, nl, idx, 0, 2 * a.nk), bm<T>(lv, n2, nl, idx, 0, 2 * a.nk), 1,
+                        zdec, prm->convergence, prm->max_iter, o, stdec, itdec, ar, o_scr);
             if (pm) {
                 DecimationOut<T> o2{bw(gpmbuf, n2, nl), nonew<T>(), nonew<T>()};
-                decimate<T>(c, nl, 2 * P, bm<T>(lh, n2, nl, idx), bm<T>(lv, n2, nl, idx), 0, zpm, prm->convergence,
-                            prm->max_iter, o2, stpm, itpm, ar, o_scr);
+                decimate<T>(c, nl, 2 * P, bm<T>(lh, n2, nl, idx, 0, 2 * a.nk), bm<T>(lv, n2, nl, idx, 0, 2 * a.nk), 0,
+                            zpm, prm->convergence, prm->max_iter, o2, stpm, itpm, ar, o_scr);
             }
         } else {
             // separate batches per lead: item strides 2*nmm keep [left, right] pairs
@@ -672,27 +783,25 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             (void)orr;
         }
         // ---- adv -> ret (lesser.f90:141-144): gl = gl_adv^+, sl = sl_adv^+, gml = -gml_adv
-        launch(c, NT_K_ELEM, 0, "dagger",
-               [&] { return launch_copy_op<T>(nl, nl, bm<T>(gbuf, 2 * nll, nl), bw(glr, nll, nl), 1, (T)1, nullptr,
-                                              P, s); });
-        launch(c, NT_K_ELEM, 0, "dagger",
-               [&] { return launch_copy_op<T>(nl, nl, bm<T>(sigbuf, 2 * nll, nl), bw(slr, nll, nl), 1, (T)1,
-                                              nullptr, P, s); });
-        launch(c, NT_K_ELEM, 0, "neg",
-               [&] { return launch_copy_op<T>(nl, nl, bm<T>(gambuf, 2 * nll, nl), bw(gmlr, nll, nl), 0, (T)-1,
-                                              nullptr, P, s); });
+        copy_op<T>(c, nl, nl, bm<T>(gbuf, 2 * nll, nl), bw(glr, nll, nl), 1, (T)1, P);
+        copy_op<T>(c, nl, nl, bm<T>(sigbuf, 2 * nll, nl), bw(slr, nll, nl), 1, (T)1, P);
+        copy_op<T>(c, nl, nl, bm<T>(gambuf, 2 * nll, nl), bw(gmlr, nll, nl), 0, (T)-1, P);
         // ---- RGF + Caroli
+        // Per-k operands: item p reads k-point kidx[p]; without a k map there is
+        // one k-point and every item shares it (stride 0).
+        auto kst = [&](long per_k) { return kidx ? per_k : 0L; };
+        const int kn = a.nk;
         RgfInput<T> in;
         in.L = L;
         in.bsz.assign(L, b);
-        for (int i = 0; i < L; ++i) in.Hd.push_back(bm<T>(Hamd + 2 * i * bb, (long)L * bb, b, kidx));
+        for (int i = 0; i < L; ++i) in.Hd.push_back(bm<T>(Hamd + 2 * i * bb, kst((long)L * bb), b, kidx, 0, kn));
         for (int i = 0; i + 1 < L; ++i)
-            in.Hdv.push_back(bm<T>(Hamdv + 2 * i * bb, (long)std::max(L - 1, 1) * bb, b, kidx));
-        in.Hld = bm<T>(Hamld, (long)nl * b, nl, kidx);
-        in.Hrd = bm<T>(Hamrd, (long)nr * b, nr, kidx);
+            in.Hdv.push_back(bm<T>(Hamdv + 2 * i * bb, kst((long)std::max(L - 1, 1) * bb), b, kidx, 0, kn));
+        in.Hld = bm<T>(Hamld, kst((long)nl * b), nl, kidx, 0, kn);
+        in.Hrd = bm<T>(Hamrd, kst((long)nr * b), nr, kidx, 0, kn);
         in.Hrd_fwd = none<T>();
-        in.Haml = bm<T>(Haml, nll, nl, kidx);
-        in.Hamr = bm<T>(Hamr, nrr, nr, kidx);
+        in.Haml = bm<T>(Haml, kst(nll), nl, kidx, 0, kn);
+        in.Hamr = bm<T>(Hamr, kst(nrr), nr, kidx, 0, kn);
         in.gl = bm<T>(glr, nll, nl);
         in.sl = bm<T>(slr, nll, nl);
         in.gml = bm<T>(gmlr, nll, nl);
@@ -715,11 +824,8 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         // ---- optional lead outputs
         auto put = [&](nt_c128* dst, const T* srcbase, long stride, int n) {
             if (!dst) return;
-            launch(c, NT_K_ELEM, 0, "copy", [&] {
-                return launch_copy_op<T>(n, n, bm<T>(srcbase, stride, n),
-                                         bw(reinterpret_cast<T*>(dst) + 2 * (long)p0 * n * n, (long)n * n, n), 0,
-                                         (T)1, nullptr, P, s);
-            });
+            copy_op<T>(c, n, n, bm<T>(srcbase, stride, n),
+                       bw(reinterpret_cast<T*>(dst) + 2 * (long)p0 * n * n, (long)n * n, n), 0, (T)1, P);
         };
         put(a.ex.gl_adv, gbuf, 2 * nll, nl);
         put(a.ex.gr_ret, gbuf + 2 * nrr, 2 * nrr, nr);
@@ -731,6 +837,8 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             put(a.ex.gl_plus, gpmbuf, 2 * nll, nl);
             put(a.ex.gr_minus, gpmbuf + 2 * nrr, 2 * nrr, nr);
         }
+        if (a.ex.sdos_l) check_range(c, a.ex.sdos_l + p0, sizeof(double) * P, "sdos_l");
+        if (a.ex.sdos_r) check_range(c, a.ex.sdos_r + p0, sizeof(double) * P, "sdos_r");
         if (a.ex.sdos_l)
             launch(c, NT_K_ELEM, 0, "sdos", [&] {
                 return launch_trace_im<T>(nl, bm<T>(gbuf, 2 * nll, nl), E, 2.0, a.ex.sdos_l + p0, P, s);
@@ -766,6 +874,8 @@ int guarded(nt_ctx* c, F&& f)
     if (!c) return NT_ERR_ARG;
     try {
         hipcheck(c, hipSetDevice(c->device), "hipSetDevice");
+        c->regions.clear();
+        reg_region(c, c->ws, c->ws_bytes);
         int r = f();
         if (c->prof) harvest_profile(c);
         return r;
@@ -784,10 +894,14 @@ struct DevBuf {
     DevBuf(nt_ctx* c_, size_t bytes) : c(c_)
     {
         if (bytes) hipcheck(c, hipMalloc(&p, bytes), "hipMalloc");
+        reg_region(c, p, bytes);
     }
     ~DevBuf()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            unreg_region(c, p);
+            (void)hipFree(p);
+        }
     }
     template <typename P>
     P* as()
@@ -946,8 +1060,52 @@ int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int 
         return NT_ERR_ARG;
     if (npts == 0) return NT_OK;
     return guarded(c, [&] {
+        // caller buffers, sized from the arguments, for the bounds audit
+        const size_t z = sizeof(nt_c128);
+        const size_t nll = (size_t)nl * nl, nrr = (size_t)nr * nr, bb = (size_t)b * b;
+        reg_region(c, E, sizeof(double) * npts);
+        reg_region(c, kidx, sizeof(int) * npts);
+        reg_region(c, Haml, z * nll * nk);
+        reg_region(c, Hamlv, z * nll * nk);
+        reg_region(c, Hamr, z * nrr * nk);
+        reg_region(c, Hamrv, z * nrr * nk);
+        reg_region(c, Hamd, z * bb * L * nk);
+        reg_region(c, Hamdv, z * bb * std::max(L - 1, 1) * nk);
+        reg_region(c, Hamld, z * nl * b * nk);
+        reg_region(c, Hamrd, z * nr * b * nk);
+        reg_region(c, first_point_mask, sizeof(int) * npts);
+        reg_region(c, T_LR, sizeof(double) * npts);
+        reg_region(c, T_RL, sizeof(double) * npts);
+        reg_region(c, status, sizeof(int) * npts);
+        reg_region(c, iters, sizeof(int) * npts);
+        nt_solve_extras ex = extras ? *extras : nt_solve_extras{};
+        reg_region(c, ex.gl_adv, z * nll * npts);
+        reg_region(c, ex.gr_ret, z * nrr * npts);
+        reg_region(c, ex.gl_plus, z * nll * npts);
+        reg_region(c, ex.gr_minus, z * nrr * npts);
+        reg_region(c, ex.gml_adv, z * nll * npts);
+        reg_region(c, ex.gmr_ret, z * nrr * npts);
+        reg_region(c, ex.sl_adv, z * nll * npts);
+        reg_region(c, ex.sr_ret, z * nrr * npts);
+        reg_region(c, ex.G_D_LR, z * nl * nr * npts);
+        reg_region(c, ex.G_D_RL, z * nl * nr * npts);
+        reg_region(c, ex.G_L, z * nll * npts);
+        reg_region(c, ex.G_R, z * nrr * npts);
+        reg_region(c, ex.sdos_l, sizeof(double) * npts);
+        reg_region(c, ex.sdos_r, sizeof(double) * npts);
+        if (kidx) {
+            // every k index must name one of the nk block sets
+            std::vector<int> hk(npts);
+            hipcheck(c, hipMemcpyAsync(hk.data(), kidx, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream), "D2H");
+            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+            for (int v : hk)
+                if (v < 0 || v >= nk) {
+                    c->err = "kidx out of range [0, nk)";
+                    return (int)NT_ERR_ARG;
+                }
+        }
         TdbArgs a{nl, nr, b, L, npts, nk, E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
-                  first_point_mask, T_LR, T_RL, status, iters, extras ? *extras : nt_solve_extras{}};
+                  first_point_mask, T_LR, T_RL, status, iters, ex};
         solve_tdb_dev(c, p, a);
         return 0;
     });
@@ -965,6 +1123,9 @@ int nt_solve_tdb_batched(nt_ctx* c, const nt_params* p, int nl, int nr, int b, i
     if (!E || !Haml || !Hamlv || !Hamr || !Hamrv || !Hamd || (L > 1 && !Hamdv) || !Hamld || !Hamrd || !T_LR || !T_RL)
         return NT_ERR_ARG;
     if (npts == 0) return NT_OK;
+    if (kidx)
+        for (int i = 0; i < npts; ++i)
+            if (kidx[i] < 0 || kidx[i] >= nk) return NT_ERR_ARG;
     return guarded(c, [&] {
         const size_t z = sizeof(nt_c128);
         const size_t nll = (size_t)nl * nl, nrr = (size_t)nr * nr, bb = (size_t)b * b;
@@ -1249,6 +1410,69 @@ int nt_G_block(nt_ctx* c, const nt_params* p, int nl, int nr, int L, const int* 
     return guarded(c, [&] {
         return rgf_host_common(c, p, 0, nl, nr, L, layer_list, E, gl, gr, gml, gmr, sl, sr, Hd, Hdv, Hdv_back, Hld,
                                Hrd, Hrd_fwd, Haml, Hamr, transl, transr, G_D_RL, G_D_LR, G_L, G_R);
+    });
+}
+
+int nt_zgemm_batched_dev(nt_ctx* c, char opA, char opB, int m, int n, int k, nt_c128 alpha, const nt_c128* A,
+                         int lda, long long strideA, const nt_c128* B, int ldb, long long strideB, nt_c128 beta,
+                         nt_c128* C, int ldc, long long strideC, int batch)
+{
+    if (!c) return NT_ERR_ARG;
+    auto opv = [](char o) { return o == 'N' || o == 'n' ? OP_N : (o == 'C' || o == 'c' ? OP_C : -1); };
+    const int oa = opv(opA), ob = opv(opB);
+    if (oa < 0 || ob < 0 || m < 0 || n < 0 || k < 0 || batch < 0 || !C || (k > 0 && (!A || !B))) return NT_ERR_ARG;
+    if (ldc < std::max(1, m) || lda < 1 || ldb < 1) return NT_ERR_ARG;
+    return guarded(c, [&] {
+        typedef double T;
+        // the caller vouches for the ranges its arguments describe
+        auto span = [](long long stride, int ld, int rows, int cols, int nb) {
+            return (size_t)((nb - 1) * std::max(0LL, stride) + (long long)(cols - 1) * ld + rows) * sizeof(nt_c128);
+        };
+        if (k > 0) {
+            reg_region(c, A, span(strideA, lda, oa == OP_N ? m : k, oa == OP_N ? k : m, batch));
+            reg_region(c, B, span(strideB, ldb, ob == OP_N ? k : n, ob == OP_N ? n : k, batch));
+        }
+        if (m > 0 && n > 0) reg_region(c, C, span(strideC, ldc, m, n, batch));
+        GemmArgs<T> a;
+        a.M = m;
+        a.N = n;
+        a.K = k;
+        a.opA = oa;
+        a.opB = ob;
+        a.A = bm<T>(reinterpret_cast<const T*>(A), strideA, lda);
+        a.B = bm<T>(reinterpret_cast<const T*>(B), strideB, ldb);
+        a.C = bw<T>(reinterpret_cast<T*>(C), strideC, ldc);
+        a.Cin = (beta.re != 0.0 || beta.im != 0.0) ? bm<T>(reinterpret_cast<const T*>(C), strideC, ldc) : none<T>();
+        a.C2 = nonew<T>();
+        a.alpha_re = alpha.re;
+        a.alpha_im = alpha.im;
+        a.beta_re = beta.re;
+        a.beta_im = beta.im;
+        a.diag = nullptr;
+        a.active = nullptr;
+        a.maxabs = nullptr;
+        a.batch = batch;
+        a.groups = 1;
+        launch(c, NT_K_GEMM, 8.0 * m * n * k * (double)batch, "gemm", [&] { return launch_gemm<T>(a, c->stream); });
+        return 0;
+    });
+}
+
+int nt_zinv_batched_dev(nt_ctx* c, int n, nt_c128* A, long long strideA, nt_c128* X, long long strideX, int* info,
+                        int batch)
+{
+    if (!c) return NT_ERR_ARG;
+    if (n <= 0 || n > inv_max_n() || batch < 0 || !A || !X || A == X) return NT_ERR_ARG;
+    return guarded(c, [&] {
+        typedef double T;
+        const size_t span = (size_t)((batch - 1) * std::max(0LL, strideA) + (long long)n * n) * sizeof(nt_c128);
+        const size_t spanx = (size_t)((batch - 1) * std::max(0LL, strideX) + (long long)n * n) * sizeof(nt_c128);
+        reg_region(c, A, span);
+        reg_region(c, X, spanx);
+        reg_region(c, info, sizeof(int) * batch);
+        inverse<T>(c, n, bw<T>(reinterpret_cast<T*>(A), strideA, n), none<T>(), nullptr,
+                   bw<T>(reinterpret_cast<T*>(X), strideX, n), info, batch);
+        return 0;
     });
 }
 

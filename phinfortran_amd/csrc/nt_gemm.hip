@@ -6,8 +6,8 @@
 // and issues four real MFMAs per complex 16x16x4 step.
 //
 // The product is formed transposed (C^T = B^T A^T): the MFMA output layout
-// then maps the 16 consecutive lanes of a row group to 16 consecutive rows of
-// the column-major C, so every store is a 256-byte (FP64) contiguous run.
+// then maps 16 consecutive lanes to 16 consecutive rows of the column-major C,
+// so every store is a 256-byte (FP64) contiguous run.
 #include "nt_kernels.h"
 
 namespace nt {
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
         }
     }
 
-    // Epilogue.  Lane holds D[n = 4*lk + r][m = lr] of each 16x16 sub-tile.
+    // Epilogue.  Lane holds D[n = mfma_out_row(lane, r)][m = lr] of each sub-tile.
     T* __restrict__ C = bptr(a.C, p, g);
     const T* __restrict__ Cin = a.Cin.base ? bptr(a.Cin, p, g) : nullptr;
     T* __restrict__ C2 = a.C2.base ? bptr(a.C2, p, g) : nullptr;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm + 16 * x + lr;
-                const int n = n0 + wn + 16 * y + 4 * lk + r;
+                const int n = n0 + wn + 16 * y + mfma_out_row<T>(lane, r);
                 if (m < a.M && n < a.N) {
                     const T pr = accR[x][y][r], pi = accI[x][y][r];
                     T vr = a.alpha_re * pr - a.alpha_im * pi;

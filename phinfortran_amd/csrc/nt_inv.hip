@@ -101,8 +101,9 @@ __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
     int step = 0;   // parity of the exchange buffers
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nbe = min(NB, n - j0);
-        // ---- load panel rows into registers; reset pi on [j0, n)
-        for (int r = j0 + t; r < n; r += NT) pi[r] = r;
+        // ---- load panel rows into registers; pi restarts as the identity
+        // (the previous panel's exchanges are already applied in memory)
+        for (int r = t; r < n; r += NT) pi[r] = r;
 #pragma unroll
         for (int sl = 0; sl < RPT; ++sl) {
             const int r = t + sl * NT;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
                 const int row = rt * 16 + (lane & 15);
 #pragma unroll
                 for (int r4 = 0; r4 < 4; ++r4) {
-                    const int col = q0 + 4 * (lane >> 4) + r4;
+                    const int col = q0 + mfma_out_row<T>(lane, r4);
                     T vr = 0, vi = 0;
                     if (rt < nrt && row < n && col < n && !(row >= j0 && row < j0 + nbe)) {
                         const int src = pi[row];
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
                 }
 #pragma unroll
                 for (int r4 = 0; r4 < 4; ++r4) {
-                    const int col = q0 + 4 * (lane >> 4) + r4;
+                    const int col = q0 + mfma_out_row<T>(lane, r4);
                     if (row < n && col < n && !(col >= j0 && col < j0 + nbe)) {
                         V2 q;
                         q.x = accR[u][r4];

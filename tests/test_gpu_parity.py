@@ -2,8 +2,8 @@
 
 FP64 tolerance: the engine inverts with blocked Gauss-Jordan and sums GEMMs in
 MFMA order while the oracle uses LU and sequential sums, so results agree to
-rounding: |dT| <= 1e-10 and matrix outputs to 1e-10 relative on the well
-conditioned cases used here.
+rounding: matrix outputs to max(1e-10, 64 eps cond(G)) relative (ctol), T to
+1e-9 relative on the solve-level cases.
 """
 import numpy as np
 import pytest
@@ -14,10 +14,18 @@ from phinfortran_amd import lattice as lt
 pytestmark = pytest.mark.gpu
 
 TOL_T = 1e-10
+EPS = 2.220446049250313e-16
 
 
 def rel(a, b):
     return np.abs(a - b).max() / max(1e-300, np.abs(b).max())
+
+
+def ctol(*mats, floor=1e-10, ulps=64):
+    """Rounding bound for results that pass through inverses: two correct FP64
+    algorithms differ by ~eps * cond of the inverted matrix, i.e. cond(G)."""
+    k = max(np.linalg.cond(m) for m in mats)
+    return max(floor, ulps * EPS * k)
 
 
 def oparams(p):
@@ -31,16 +39,33 @@ def T():
     return transport
 
 
+def dyson_residual(g, H, alpha, beta, z):
+    n = g.shape[0]
+    return np.abs(g - np.linalg.inv(z * np.eye(n) - H - alpha @ g @ beta)).max() / np.abs(g).max()
+
+
 @pytest.mark.parametrize("n,seed", [(1, 0), (5, 1), (24, 2), (37, 3), (48, 4), (96, 5)])
 def test_surface_green_tdb(engine, T, n, seed):
+    """Each surface GF must (a) satisfy its Dyson fixed point as tightly as the
+    oracle's own result does and (b) agree with the oracle to 1e-8; the
+    decimation iterates through matrices far worse conditioned than g, so the
+    two FP64 algorithms differ by more than eps*cond(g)."""
     blk = lt.random_blocks(n, n, 1, seed=seed)
     p = T.params("phonon", eta=1e-4, eta0=1e-5, convergence=1e-13)
     E = 0.37
+    eta = E * p.eta + p.eta0
     print(f"surface_green_tdb n={n} seed={seed}", flush=True)
     g = engine.surface_green_tdb(blk.Haml, blk.Hamr, blk.Hamlv, blk.Hamrv, E, p)
     o = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, E, oparams(p))
+    Vl, Vr = blk.Hamlv, blk.Hamrv
+    cases = {"gl_adv": (blk.Haml, Vl.conj().T, Vl, E - 1j * eta), "gl_plus": (blk.Haml, Vl, Vl.conj().T, E + 1j * eta),
+             "gr_ret": (blk.Hamr, Vr.conj().T, Vr, E + 1j * eta), "gr_minus": (blk.Hamr, Vr, Vr.conj().T, E - 1j * eta)}
+    for k, (H, a, b, z) in cases.items():
+        r_gpu = dyson_residual(g[k], H, a, b, z)
+        r_ora = dyson_residual(o[k], H, a, b, z)
+        assert r_gpu < 10 * r_ora + 1e-12, (k, r_gpu, r_ora)
     for k in ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "sl_adv", "sr_ret"):
-        assert rel(g[k], o[k]) < 1e-10, k
+        assert rel(g[k], o[k]) < 1e-8, k
     assert all(abs(a - b) <= 1 for a, b in zip(g["iters"], o["iters"]))
 
 
@@ -63,10 +88,11 @@ def test_get_G_full_tdb(engine, T, nl, b, L, seed):
                               blk.Hamd, blk.Hamdv, blk.Hamld, blk.Hamrd, blk.Haml, blk.Hamr, p)
     o = O.rgf(list(blk.Hamd), list(blk.Hamdv), blk.Hamld, blk.Hamrd, blk.Haml, blk.Hamr,
               lq["gl"], lq["gr"], lq["gml"], lq["gmr"], lq["sl"], lq["sr"], E, op)
-    assert abs(g["transl"] - o["T_LR"]) < TOL_T * max(1, o["T_LR"])
-    assert abs(g["transr"] - o["T_RL"]) < TOL_T * max(1, o["T_RL"])
+    tol = ctol(o["G_L"], o["G_R"])
+    assert abs(g["transl"] - o["T_LR"]) < tol * max(1, o["T_LR"])
+    assert abs(g["transr"] - o["T_RL"]) < tol * max(1, o["T_RL"])
     for k in ("G_D_LR", "G_D_RL", "G_L", "G_R"):
-        assert rel(g[k], o[k]) < 1e-10, k
+        assert rel(g[k], o[k]) < tol, (k, tol)
     z = engine.get_G_full_tdb(lq["gl"], lq["gr"], lq["gml"], lq["gmr"], lq["sl"], lq["sr"], E,
                               blk.Hamd, blk.Hamdv, blk.Hamld, blk.Hamrd, blk.Haml, blk.Hamr, p, first_point=True)
     assert z["transl"] == 0.0 and z["transr"] == 0.0
@@ -94,10 +120,11 @@ def test_G_block_variable_layers(engine, T):
                        blk.Haml, blk.Hamr, p, Hdv_back=back, Hrd_fwd=Hrd.conj().T)
     o = O.rgf(Hd, Hdv, Hld, Hrd, blk.Haml, blk.Hamr, lq["gl"], lq["gr"], lq["gml"], lq["gmr"], lq["sl"], lq["sr"],
               E, op, Hdv_back=back, Hrd_fwd=Hrd.conj().T)
-    assert abs(g["transl"] - o["T_LR"]) < TOL_T
-    assert abs(g["transr"] - o["T_RL"]) < TOL_T
+    tol = ctol(o["G_L"], o["G_R"])
+    assert abs(g["transl"] - o["T_LR"]) < tol
+    assert abs(g["transr"] - o["T_RL"]) < tol
     for k in ("G_D_LR", "G_D_RL", "G_L", "G_R"):
-        assert rel(g[k], o[k]) < 1e-10, k
+        assert rel(g[k], o[k]) < tol, (k, tol)
 
 
 def _batched(engine, T, blocks, E, p, kidx=None, **kw):
@@ -148,11 +175,12 @@ def test_batched_random_multi_k(engine, T, nl, b, L, nk):
     for i in range(1, npts):
         blk = blocks[kidx[i]] if nk > 1 else blocks[0]
         otl, otr, it = O.solve_tdb(blk.as_dict(), E[i], op)
-        assert abs(tl[i] - otl) < TOL_T * max(1, otl), (i, tl[i], otl)
-        assert abs(tr[i] - otr) < TOL_T * max(1, otr), (i, tr[i], otr)
+        assert abs(tl[i] - otl) < 1e-9 * max(1, otl), (i, tl[i], otl)
+        assert abs(tr[i] - otr) < 1e-9 * max(1, otr), (i, tr[i], otr)
         sg = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, E[i], op, want_plus_minus=False)
-        assert rel(r["gl_adv"][i].cpu().numpy(), sg["gl_adv"]) < 1e-10
-        assert rel(r["gmr_ret"][i].cpu().numpy(), sg["gmr_ret"]) < 1e-10
+        tol = ctol(sg["gl_adv"], sg["gr_ret"])
+        assert rel(r["gl_adv"][i].cpu().numpy(), sg["gl_adv"]) < tol
+        assert rel(r["gmr_ret"][i].cpu().numpy(), sg["gmr_ret"]) < tol
         sdos = 2.0 * np.sqrt(E[i]) * np.trace(sg["gl_adv"]).imag / np.pi
         assert abs(r["sdos_l"][i].item() - sdos) < 1e-9 * max(1, abs(sdos))
 

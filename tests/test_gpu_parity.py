@@ -51,7 +51,7 @@ def test_surface_green_tdb(engine, T, n, seed):
     decimation iterates through matrices far worse conditioned than g, so the
     two FP64 algorithms differ by more than eps*cond(g)."""
     blk = lt.random_blocks(n, n, 1, seed=seed)
-    p = T.params("phonon", eta=1e-4, eta0=1e-5, convergence=1e-13)
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
     E = 0.37
     eta = E * p.eta + p.eta0
     print(f"surface_green_tdb n={n} seed={seed}", flush=True)
@@ -65,7 +65,7 @@ def test_surface_green_tdb(engine, T, n, seed):
         r_ora = dyson_residual(o[k], H, a, b, z)
         assert r_gpu < 10 * r_ora + 1e-12, (k, r_gpu, r_ora)
     for k in ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "sl_adv", "sr_ret"):
-        assert rel(g[k], o[k]) < 1e-8, k
+        assert rel(g[k], o[k]) < 1e-8, (k, rel(g[k], o[k]))
     assert all(abs(a - b) <= 1 for a, b in zip(g["iters"], o["iters"]))
 
 
@@ -133,15 +133,23 @@ def _batched(engine, T, blocks, E, p, kidx=None, **kw):
 
 
 def test_batched_chain_unit_transmission(engine, T):
+    """T = 1 inside the band 0 < w^2 < 4k/m; the eta broadening pulls T below 1
+    near the band edges, so the band interior is checked against 1 and every
+    point against the oracle."""
     blk = lt.chain_blocks(L=6)
     p = T.params("phonon", eta=1e-8, eta0=1e-9, convergence=1e-14, max_iter=500)
+    op = oparams(p)
     E = np.linspace(0.05, 3.95, 64)
     print("batched chain", flush=True)
     r = _batched(engine, T, blk, E, p)
     tl = r["T_LR"].cpu().numpy()
     tr = r["T_RL"].cpu().numpy()
     assert (r["status"].cpu().numpy() == 0).all()
-    assert np.abs(tl - 1).max() < 1e-5 and np.abs(tr - 1).max() < 1e-5
+    inner = (E > 0.2) & (E < 3.8)
+    assert np.abs(tl[inner] - 1).max() < 1e-5 and np.abs(tr[inner] - 1).max() < 1e-5
+    for i in range(0, 64, 7):
+        otl, otr, _ = O.solve_tdb(blk.as_dict(), E[i], op)
+        assert abs(tl[i] - otl) < 1e-9 and abs(tr[i] - otr) < 1e-9, (E[i], tl[i], otl)
 
 
 def test_batched_mass_defect_closed_form(engine, T):
@@ -157,7 +165,7 @@ def test_batched_mass_defect_closed_form(engine, T):
 @pytest.mark.parametrize("nl,b,L,nk", [(4, 4, 3, 1), (24, 24, 5, 3), (40, 40, 2, 2)])
 def test_batched_random_multi_k(engine, T, nl, b, L, nk):
     blocks = [lt.random_blocks(nl, b, L, seed=10 + i) for i in range(nk)]
-    p = T.params("phonon", eta=1e-4, eta0=1e-5, convergence=1e-13)
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
     op = oparams(p)
     rng = np.random.default_rng(1)
     npts = 13
@@ -178,9 +186,9 @@ def test_batched_random_multi_k(engine, T, nl, b, L, nk):
         assert abs(tl[i] - otl) < 1e-9 * max(1, otl), (i, tl[i], otl)
         assert abs(tr[i] - otr) < 1e-9 * max(1, otr), (i, tr[i], otr)
         sg = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, E[i], op, want_plus_minus=False)
-        tol = ctol(sg["gl_adv"], sg["gr_ret"])
-        assert rel(r["gl_adv"][i].cpu().numpy(), sg["gl_adv"]) < tol
-        assert rel(r["gmr_ret"][i].cpu().numpy(), sg["gmr_ret"]) < tol
+        # decimation outputs: same policy as test_surface_green_tdb
+        assert rel(r["gl_adv"][i].cpu().numpy(), sg["gl_adv"]) < 1e-8
+        assert rel(r["gmr_ret"][i].cpu().numpy(), sg["gmr_ret"]) < 1e-8
         sdos = 2.0 * np.sqrt(E[i]) * np.trace(sg["gl_adv"]).imag / np.pi
         assert abs(r["sdos_l"][i].item() - sdos) < 1e-9 * max(1, abs(sdos))
 

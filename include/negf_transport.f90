@@ -1,0 +1,221 @@
+! negf_transport.f90 -- ISO_C_BINDING interface to libnegf_transport.so.
+!
+! Gives the Fortran driver of phinfortran a drop-in for its per-(omega,k)
+! solve.  The bind(C) interfaces map one-to-one onto include/negf_transport.h;
+! the nt_* Fortran wrappers below keep the reference's argument order and
+! meaning and its error behaviour (stop 'Matrix is numerically singular!',
+! negf/util.f90:44-58) so a call site changes only in its name:
+!
+!   call surface_green_tdb(mm, gl_adv, ...)       negf/surface.f90:385
+!     -> call nt_surface_green_tdb_f(ctx, p, E, Haml, Hamr, Hamlv, Hamrv, gl_adv, ...)
+!   call get_G_full_tdb(ie, mm, gl, gr, ...)       negf/lesser.f90:475
+!     -> call nt_get_G_full_tdb_f(ctx, p, ie == 1, E, gl, gr, ..., transl, transr, ...)
+!   the whole ie loop of one k-point                negf/phinfortran.f90:1196-1251
+!     -> call nt_solve_tdb_f(ctx, p, Egrid(:, nn), Haml, ..., transl(:, nn), transr(:, nn))
+module negf_transport
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: NT_OK = 0, NT_ERR_NAN = -1, NT_ERR_NOT_CONVERGED = -2, &
+                                 NT_ERR_ARG = -3, NT_ERR_HIP = -4, NT_ERR_NOMEM = -5
+    integer(c_int), parameter :: NT_ETA_RELATIVE = 0, NT_ETA_ABSOLUTE = 1
+    integer(c_int), parameter :: NT_PREC_FP64 = 0, NT_PREC_MIXED = 1
+
+    type, bind(C) :: nt_params
+        real(c_double) :: eta, eta0, convergence, device_eta_factor, lambda_eta
+        integer(c_int) :: eta_mode, max_iter, precision, reserved
+    end type nt_params
+
+    type, bind(C) :: nt_solve_extras
+        type(c_ptr) :: gl_adv = c_null_ptr, gr_ret = c_null_ptr, gl_plus = c_null_ptr, gr_minus = c_null_ptr
+        type(c_ptr) :: gml_adv = c_null_ptr, gmr_ret = c_null_ptr, sl_adv = c_null_ptr, sr_ret = c_null_ptr
+        type(c_ptr) :: G_D_LR = c_null_ptr, G_D_RL = c_null_ptr, G_L = c_null_ptr, G_R = c_null_ptr
+        type(c_ptr) :: sdos_l = c_null_ptr, sdos_r = c_null_ptr
+    end type nt_solve_extras
+
+    interface
+        subroutine nt_params_default(p) bind(C, name="nt_params_default")
+            import :: nt_params
+            type(nt_params), intent(out) :: p
+        end subroutine
+
+        integer(c_int) function nt_create(ctx, device) bind(C, name="nt_create")
+            import :: c_int, c_ptr
+            type(c_ptr), intent(out) :: ctx
+            integer(c_int), value :: device
+        end function
+
+        integer(c_int) function nt_destroy(ctx) bind(C, name="nt_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+
+        integer(c_int) function nt_set_batch_limit(ctx, n) bind(C, name="nt_set_batch_limit")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: n
+        end function
+
+        integer(c_int) function nt_surface_green_tdb(ctx, p, nl, nr, E, Haml, Hamr, Hamlv, Hamrv, &
+                gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, sl_adv, sr_ret, iters) &
+                bind(C, name="nt_surface_green_tdb")
+            import :: c_int, c_ptr, c_double, nt_params
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: nl, nr
+            real(c_double), value :: E
+            type(c_ptr), value :: Haml, Hamr, Hamlv, Hamrv
+            type(c_ptr), value :: gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, sl_adv, sr_ret, iters
+        end function
+
+        integer(c_int) function nt_get_G_full_tdb(ctx, p, first_point, nl, nr, b, L, E, gl, gr, gml, gmr, &
+                sl, sr, hamd, hamdv, hamld, hamrd, haml, hamr, transl, transr, G_D_RL, G_D_LR, G_L, G_R) &
+                bind(C, name="nt_get_G_full_tdb")
+            import :: c_int, c_ptr, c_double, nt_params
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: first_point, nl, nr, b, L
+            real(c_double), value :: E
+            type(c_ptr), value :: gl, gr, gml, gmr, sl, sr, hamd, hamdv, hamld, hamrd, haml, hamr
+            real(c_double), intent(out) :: transl, transr
+            type(c_ptr), value :: G_D_RL, G_D_LR, G_L, G_R
+        end function
+
+        integer(c_int) function nt_solve_tdb_batched(ctx, p, nl, nr, b, L, npts, E, kidx, nk, &
+                Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd, first_point_mask, &
+                T_LR, T_RL, status, iters, extras) bind(C, name="nt_solve_tdb_batched")
+            import :: c_int, c_ptr, nt_params, nt_solve_extras
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: nl, nr, b, L, npts, nk
+            type(c_ptr), value :: E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd
+            type(c_ptr), value :: first_point_mask, T_LR, T_RL, status, iters
+            type(nt_solve_extras), intent(in) :: extras
+        end function
+
+        function nt_status_string(status) bind(C, name="nt_status_string")
+            import :: c_int, c_ptr
+            integer(c_int), value :: status
+            type(c_ptr) :: nt_status_string
+        end function
+    end interface
+
+contains
+
+    ! The reference stops the program on a singular block (negf/util.f90:44-58)
+    ! and on NaN input (util.f90:29-32); the wrappers do the same.
+    subroutine nt_check(status)
+        integer(c_int), intent(in) :: status
+        if (status > 0) then
+            write(*,*) 'info'
+            write(*,*) status
+            stop 'Matrix is numerically singular!'
+        else if (status == NT_ERR_NAN) then
+            stop
+        else if (status /= NT_OK) then
+            write(*,*) 'negf_transport status', status
+            stop 'negf_transport failed'
+        end if
+    end subroutine nt_check
+
+    ! surface_green_tdb (negf/surface.f90:385-605) on the GPU.
+    subroutine nt_surface_green_tdb_f(ctx, p, E, Haml, Hamr, Hamlv, Hamrv, &
+                                      gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, sl_adv, sr_ret)
+        type(c_ptr), intent(in) :: ctx
+        type(nt_params), intent(in) :: p
+        real(c_double), intent(in) :: E
+        complex(c_double_complex), intent(in), target :: Haml(:,:), Hamr(:,:), Hamlv(:,:), Hamrv(:,:)
+        complex(c_double_complex), intent(out), target :: gl_adv(:,:), gr_ret(:,:), gl_plus(:,:), gr_minus(:,:)
+        complex(c_double_complex), intent(out), target :: gml_adv(:,:), gmr_ret(:,:), sl_adv(:,:), sr_ret(:,:)
+        complex(c_double_complex), allocatable, target :: hl(:,:), hr(:,:), vl(:,:), vr(:,:)
+        integer(c_int) :: st
+        hl = Haml
+        hr = Hamr
+        vl = Hamlv
+        vr = Hamrv
+        st = nt_surface_green_tdb(ctx, p, int(size(Haml, 1), c_int), int(size(Hamr, 1), c_int), E, &
+                c_loc(hl), c_loc(hr), c_loc(vl), c_loc(vr), c_loc(gl_adv), c_loc(gr_ret), &
+                c_loc(gl_plus), c_loc(gr_minus), c_loc(gml_adv), c_loc(gmr_ret), c_loc(sl_adv), &
+                c_loc(sr_ret), c_null_ptr)
+        call nt_check(st)
+    end subroutine nt_surface_green_tdb_f
+
+    ! get_G_full_tdb (negf/lesser.f90:475-809) on the GPU.  hamd(L,b,b) and
+    ! hamdv(L,b,b) are in the reference's layer-first layout and are regrouped
+    ! layer by layer here.  gl, gml, sl are the retarded left quantities that
+    ! get_G_full_tdb_mpi passes (transpose(dconjg(gl_adv)), -gml_adv, ...).
+    subroutine nt_get_G_full_tdb_f(ctx, p, first_point, E, gl, gr, gml, gmr, sl, sr, &
+                                   transl, transr, G_D_RL, G_D_LR, G_L, G_R, &
+                                   hamd, hamdv, hamld, hamrd, haml, hamr)
+        type(c_ptr), intent(in) :: ctx
+        type(nt_params), intent(in) :: p
+        logical, intent(in) :: first_point
+        real(c_double), intent(in) :: E
+        complex(c_double_complex), intent(in) :: gl(:,:), gr(:,:), gml(:,:), gmr(:,:), sl(:,:), sr(:,:)
+        real(c_double), intent(out) :: transl, transr
+        complex(c_double_complex), intent(out), target :: G_D_RL(:,:), G_D_LR(:,:), G_L(:,:), G_R(:,:)
+        complex(c_double_complex), intent(in) :: hamd(:,:,:), hamdv(:,:,:)
+        complex(c_double_complex), intent(in) :: hamld(:,:), hamrd(:,:), haml(:,:), hamr(:,:)
+        complex(c_double_complex), allocatable, target :: d(:,:,:), dv(:,:,:)
+        complex(c_double_complex), allocatable, target :: a1(:,:), a2(:,:), a3(:,:), a4(:,:), a5(:,:), a6(:,:)
+        complex(c_double_complex), allocatable, target :: b1(:,:), b2(:,:), b3(:,:), b4(:,:)
+        integer :: i, L, b
+        integer(c_int) :: st, fp
+        L = size(hamd, 1)
+        b = size(hamd, 2)
+        allocate(d(b, b, L), dv(b, b, max(L - 1, 1)))
+        do i = 1, L
+            d(:, :, i) = hamd(i, :, :)
+        end do
+        do i = 1, L - 1
+            dv(:, :, i) = hamdv(i, :, :)
+        end do
+        a1 = gl; a2 = gr; a3 = gml; a4 = gmr; a5 = sl; a6 = sr
+        b1 = hamld; b2 = hamrd; b3 = haml; b4 = hamr
+        fp = 0
+        if (first_point) fp = 1
+        st = nt_get_G_full_tdb(ctx, p, fp, int(size(haml, 1), c_int), int(size(hamr, 1), c_int), int(b, c_int), &
+                int(L, c_int), E, c_loc(a1), c_loc(a2), c_loc(a3), c_loc(a4), c_loc(a5), c_loc(a6), &
+                c_loc(d), c_loc(dv), c_loc(b1), c_loc(b2), c_loc(b3), c_loc(b4), transl, transr, &
+                c_loc(G_D_RL), c_loc(G_D_LR), c_loc(G_L), c_loc(G_R))
+        call nt_check(st)
+    end subroutine nt_get_G_full_tdb_f
+
+    ! All energies of one k-point in one batched call: the body of the driver's
+    ! ie loop (negf/phinfortran.f90:1196-1251) for surface GFs + RGF + Caroli.
+    ! E(ne) is Egrid(:, nn); point 1 gets the reference's T = 0 (lesser.f90:802).
+    subroutine nt_solve_tdb_f(ctx, p, E, Haml, Hamlv, Hamr, Hamrv, hamd, hamdv, hamld, hamrd, transl, transr)
+        type(c_ptr), intent(in) :: ctx
+        type(nt_params), intent(in) :: p
+        real(c_double), intent(in), target :: E(:)
+        complex(c_double_complex), intent(in) :: Haml(:,:), Hamlv(:,:), Hamr(:,:), Hamrv(:,:)
+        complex(c_double_complex), intent(in) :: hamd(:,:,:), hamdv(:,:,:), hamld(:,:), hamrd(:,:)
+        real(c_double), intent(out), target :: transl(:), transr(:)
+        complex(c_double_complex), allocatable, target :: d(:,:,:), dv(:,:,:)
+        complex(c_double_complex), allocatable, target :: hl(:,:), vl(:,:), hr(:,:), vr(:,:), ld(:,:), rd(:,:)
+        integer(c_int), allocatable, target :: mask(:), status(:)
+        real(c_double), allocatable, target :: ee(:)
+        type(nt_solve_extras) :: ex
+        integer :: i, L, b, ne
+        integer(c_int) :: st
+        L = size(hamd, 1)
+        b = size(hamd, 2)
+        ne = size(E)
+        allocate(d(b, b, L), dv(b, b, max(L - 1, 1)), mask(ne), status(ne))
+        do i = 1, L
+            d(:, :, i) = hamd(i, :, :)
+        end do
+        do i = 1, L - 1
+            dv(:, :, i) = hamdv(i, :, :)
+        end do
+        hl = Haml; vl = Hamlv; hr = Hamr; vr = Hamrv; ld = hamld; rd = hamrd; ee = E
+        mask = 0
+        mask(1) = 1
+        st = nt_solve_tdb_batched(ctx, p, int(size(Haml, 1), c_int), int(size(Hamr, 1), c_int), int(b, c_int), &
+                int(L, c_int), int(ne, c_int), c_loc(ee), c_null_ptr, 1_c_int, c_loc(hl), c_loc(vl), c_loc(hr), &
+                c_loc(vr), c_loc(d), c_loc(dv), c_loc(ld), c_loc(rd), c_loc(mask), c_loc(transl), c_loc(transr), &
+                c_loc(status), c_null_ptr, ex)
+        call nt_check(st)
+    end subroutine nt_solve_tdb_f
+
+end module negf_transport

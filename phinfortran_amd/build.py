@@ -59,5 +59,29 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the Fortran interface module and the Fortran caller check
+    (tests/fortran/fortran_abi_check.x) against the built library."""
+    root = os.path.dirname(HERE)
+    fc = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
+    if not os.path.exists(fc):
+        raise RuntimeError("flang not found")
+    mod_dir = os.path.join(BUILD_DIR, "fmod")
+    os.makedirs(mod_dir, exist_ok=True)
+    exe = os.path.join(root, "tests", "fortran", "fortran_abi_check.x")
+    cmd = [fc, "-O2", "-module-dir", mod_dir, os.path.join(root, "include", "negf_transport.f90"),
+           os.path.join(root, "tests", "fortran", "fortran_abi_check.f90"), "-o", exe,
+           "-L" + HERE, "-lnegf_transport", "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../phinfortran_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=BUILD_DIR)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang failed:\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}", file=sys.stderr)
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_fortran(verbose=True)

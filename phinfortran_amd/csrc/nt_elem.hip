@@ -44,6 +44,29 @@ __global__ __launch_bounds__(NT) void k_copy_op(int m, int n, BMat<T> in, BMatW<
     }
 }
 
+template <typename Ti, typename To>
+__global__ __launch_bounds__(NT) void k_convert(int m, int n, BMat<Ti> in, BMatW<To> out, int dagger, To sign, int batch)
+{
+    const int p = blockIdx.y;
+    const Ti* __restrict__ A = bptr(in, p, 0);
+    To* __restrict__ O = bptr(out, p, 0);
+    const long e = (long)blockIdx.x * NT + threadIdx.x;
+    if (e >= (long)m * n) return;
+    if (dagger) {
+        const int j = (int)(e % n), i = (int)(e / n);
+        const Ti* a = A + 2 * ((long)i + (long)j * in.ld);
+        To* o = O + 2 * ((long)j + (long)i * out.ld);
+        o[0] = sign * (To)a[0];
+        o[1] = -sign * (To)a[1];
+    } else {
+        const int i = (int)(e % m), j = (int)(e / m);
+        const Ti* a = A + 2 * ((long)i + (long)j * in.ld);
+        To* o = O + 2 * ((long)i + (long)j * out.ld);
+        o[0] = sign * (To)a[0];
+        o[1] = sign * (To)a[1];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(NT) void k_gamma(int n, BMat<T> S, BMatW<T> G, T scale, const int* active, int batch)
 {
@@ -216,6 +239,17 @@ hipError_t launch_copy_op(int m, int n, BMat<T> in, BMatW<T> out, int dagger, T 
                        sign, active, batch);
     return hipGetLastError();
 }
+
+template <typename Ti, typename To>
+hipError_t launch_convert(int m, int n, BMat<Ti> in, BMatW<To> out, int dagger, To sign, int batch, hipStream_t s)
+{
+    if (batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_convert<Ti, To>), dim3(blocks_for((long)m * n), batch), dim3(NT), 0, s, m, n, in, out,
+                       dagger, sign, batch);
+    return hipGetLastError();
+}
+template hipError_t launch_convert<double, float>(int, int, BMat<double>, BMatW<float>, int, float, int, hipStream_t);
+template hipError_t launch_convert<float, double>(int, int, BMat<float>, BMatW<double>, int, double, int, hipStream_t);
 
 template <typename T>
 hipError_t launch_gamma(int n, BMat<T> S, BMatW<T> G, T scale, const int* active, int batch, hipStream_t s)

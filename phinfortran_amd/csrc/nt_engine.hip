@@ -292,6 +292,15 @@ void copy_op(nt_ctx* c, int m, int n, BMat<T> in, BMatW<T> out, int dagger, T si
            [&] { return launch_copy_op<T>(m, n, in, out, dagger, sign, nullptr, batch, c->stream); });
 }
 
+template <typename Ti, typename To>
+void convert_op(nt_ctx* c, int m, int n, BMat<Ti> in, BMatW<To> out, int dagger, To sign, int batch)
+{
+    audit<Ti>(c, in, m, n, batch, 1, "convert in");
+    audit<To>(c, out, dagger ? n : m, dagger ? m : n, batch, 1, "convert out");
+    launch(c, NT_K_ELEM, 0, "convert",
+           [&] { return launch_convert<Ti, To>(m, n, in, out, dagger, sign, batch, c->stream); });
+}
+
 template <typename T>
 void gamma_op(nt_ctx* c, int n, BMat<T> S, BMatW<T> G, int batch)
 {
@@ -691,9 +700,33 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const size_t o_glr = ar.take(sizeof(nt_c128) * nll * Pmax);        // gl = gl_adv^+
     const size_t o_slr = ar.take(sizeof(nt_c128) * nll * Pmax);        // sl = sl_adv^+
     const size_t o_gmlr = ar.take(sizeof(nt_c128) * nll * Pmax);       // gml = -gml_adv
+    // mixed precision: FP32 copies of the H(k) blocks (once per call) and of
+    // the lead quantities and device shifts (per chunk)
+    const bool mixed = prm->precision == NT_PREC_MIXED;
+    const size_t c32 = 2 * sizeof(float);
+    const long Ldv = std::max(L - 1, 1);
+    size_t o_h32[6] = {0, 0, 0, 0, 0, 0};
+    if (mixed) {
+        o_h32[0] = ar.take(c32 * bb * L * a.nk);
+        o_h32[1] = ar.take(c32 * bb * Ldv * a.nk);
+        o_h32[2] = ar.take(c32 * nl * b * a.nk);
+        o_h32[3] = ar.take(c32 * nr * b * a.nk);
+        o_h32[4] = ar.take(c32 * nll * a.nk);
+        o_h32[5] = ar.take(c32 * nrr * a.nk);
+    }
+    const size_t o_zdev32 = ar.take(c32 * Pmax);
     const size_t o_scr = ar.off;
+    // per-chunk FP32 lead quantities + optional FP32 outputs, then the FP32 RGF scratch
+    Arena l32;
+    size_t o_l32[10];
+    for (int i = 0; i < 3; ++i) o_l32[i] = l32.take(c32 * nll * Pmax);       // gl, sl, gml
+    for (int i = 3; i < 6; ++i) o_l32[i] = l32.take(c32 * nrr * Pmax);       // gr, sr, gmr
+    o_l32[6] = l32.take(c32 * nl * nr * Pmax);                               // G_D_LR
+    o_l32[7] = l32.take(c32 * nl * nr * Pmax);                               // G_D_RL
+    o_l32[8] = l32.take(c32 * nll * Pmax);                                   // G_L
+    o_l32[9] = l32.take(c32 * nrr * Pmax);                                   // G_R
     const size_t need_dec = decimate_scratch<T>(nmax, 2 * Pmax);
-    const size_t need_rgf = rgf_scratch<T>(nl, nr, b, Pmax);
+    const size_t need_rgf = mixed ? l32.off + rgf_scratch<float>(nl, nr, b, Pmax) : rgf_scratch<T>(nl, nr, b, Pmax);
     ensure_ws(c, o_scr + std::max(need_dec, need_rgf));
     ar.base = static_cast<char*>(c->ws);
     hipStream_t s = c->stream;
@@ -742,6 +775,21 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                            reinterpret_cast<double2*>(lv), n_a);
         hipcheck(c, hipGetLastError(), "lead stack");
     }
+
+    float* H32[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (mixed) {
+        for (int i = 0; i < 6; ++i) H32[i] = ar.ptr<float>(o_h32[i]);
+        const float one = 1.0f;
+        convert_op<T, float>(c, b, b, bm<T>(Hamd, bb, b), bw<float>(H32[0], bb, b), 0, one, L * a.nk);
+        if (L > 1) convert_op<T, float>(c, b, b, bm<T>(Hamdv, bb, b), bw<float>(H32[1], bb, b), 0, one, Ldv * a.nk);
+        convert_op<T, float>(c, nl, b, bm<T>(Hamld, (long)nl * b, nl), bw<float>(H32[2], (long)nl * b, nl), 0, one,
+                             a.nk);
+        convert_op<T, float>(c, nr, b, bm<T>(Hamrd, (long)nr * b, nr), bw<float>(H32[3], (long)nr * b, nr), 0, one,
+                             a.nk);
+        convert_op<T, float>(c, nl, nl, bm<T>(Haml, nll, nl), bw<float>(H32[4], nll, nl), 0, one, a.nk);
+        convert_op<T, float>(c, nr, nr, bm<T>(Hamr, nrr, nr), bw<float>(H32[5], nrr, nr), 0, one, a.nk);
+    }
+    float* zdev32 = ar.ptr<float>(o_zdev32);
 
     double iters_sum = 0.0;
     std::vector<int> hit;
@@ -819,7 +867,60 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         out.G_D_RL = exw(a.ex.G_D_RL, nr, nl);
         out.G_L = exw(a.ex.G_L, nl, nl);
         out.G_R = exw(a.ex.G_R, nr, nr);
-        rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr);
+        if (!mixed) {
+            rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr);
+        } else {
+            // Mixed precision: the surface GFs above stay FP64 (decimation); the
+            // sweep runs in complex FP32 on MFMA with FP32 accumulation, and the
+            // Caroli traces accumulate in FP64.
+            Arena la = ar;
+            la.off = o_scr;
+            float* L32[10];
+            for (int i = 0; i < 10; ++i) L32[i] = ar.ptr<float>(o_scr + o_l32[i]);
+            const float one = 1.0f;
+            convert_op<T, float>(c, nl, nl, bm<T>(gbuf, 2 * nll, nl), bw<float>(L32[0], nll, nl), 1, one, P);
+            convert_op<T, float>(c, nl, nl, bm<T>(sigbuf, 2 * nll, nl), bw<float>(L32[1], nll, nl), 1, one, P);
+            convert_op<T, float>(c, nl, nl, bm<T>(gambuf, 2 * nll, nl), bw<float>(L32[2], nll, nl), 0, -one, P);
+            convert_op<T, float>(c, nr, nr, bm<T>(gbuf + 2 * nrr, 2 * nrr, nr), bw<float>(L32[3], nrr, nr), 0, one, P);
+            convert_op<T, float>(c, nr, nr, bm<T>(sigbuf + 2 * nrr, 2 * nrr, nr), bw<float>(L32[4], nrr, nr), 0, one,
+                                 P);
+            convert_op<T, float>(c, nr, nr, bm<T>(gambuf + 2 * nrr, 2 * nrr, nr), bw<float>(L32[5], nrr, nr), 0, one,
+                                 P);
+            convert_op<T, float>(c, 1, 1, bm<T>(zdev, 1, 1), bw<float>(zdev32, 1, 1), 0, one, P);
+            RgfInput<float> i32;
+            i32.L = L;
+            i32.bsz.assign(L, b);
+            for (int i = 0; i < L; ++i)
+                i32.Hd.push_back(bm<float>(H32[0] + 2 * i * bb, kst((long)L * bb), b, kidx, 0, kn));
+            for (int i = 0; i + 1 < L; ++i)
+                i32.Hdv.push_back(bm<float>(H32[1] + 2 * i * bb, kst(Ldv * bb), b, kidx, 0, kn));
+            i32.Hld = bm<float>(H32[2], kst((long)nl * b), nl, kidx, 0, kn);
+            i32.Hrd = bm<float>(H32[3], kst((long)nr * b), nr, kidx, 0, kn);
+            i32.Hrd_fwd = none<float>();
+            i32.Haml = bm<float>(H32[4], kst(nll), nl, kidx, 0, kn);
+            i32.Hamr = bm<float>(H32[5], kst(nrr), nr, kidx, 0, kn);
+            i32.gl = bm<float>(L32[0], nll, nl);
+            i32.sl = bm<float>(L32[1], nll, nl);
+            i32.gml = bm<float>(L32[2], nll, nl);
+            i32.gr = bm<float>(L32[3], nrr, nr);
+            i32.sr = bm<float>(L32[4], nrr, nr);
+            i32.gmr = bm<float>(L32[5], nrr, nr);
+            RgfOutput<float> o32;
+            o32.T_LR = a.T_LR + p0;
+            o32.T_RL = a.T_RL + p0;
+            o32.status = strgf;
+            const bool gx = a.ex.G_D_LR || a.ex.G_D_RL || a.ex.G_L || a.ex.G_R;
+            o32.G_D_LR = gx ? bw<float>(L32[6], (long)nl * nr, nl) : nonew<float>();
+            o32.G_D_RL = gx ? bw<float>(L32[7], (long)nl * nr, nr) : nonew<float>();
+            o32.G_L = gx ? bw<float>(L32[8], nll, nl) : nonew<float>();
+            o32.G_R = gx ? bw<float>(L32[9], nrr, nr) : nonew<float>();
+            rgf_run<float>(c, nl, nr, i32, zdev32, P, o32, ar, o_scr + l32.off);
+            const double d1 = 1.0;
+            if (out.G_D_LR.base) convert_op<float, T>(c, nl, nr, ro(o32.G_D_LR), out.G_D_LR, 0, d1, P);
+            if (out.G_D_RL.base) convert_op<float, T>(c, nr, nl, ro(o32.G_D_RL), out.G_D_RL, 0, d1, P);
+            if (out.G_L.base) convert_op<float, T>(c, nl, nl, ro(o32.G_L), out.G_L, 0, d1, P);
+            if (out.G_R.base) convert_op<float, T>(c, nr, nr, ro(o32.G_R), out.G_R, 0, d1, P);
+        }
 
         // ---- optional lead outputs
         auto put = [&](nt_c128* dst, const T* srcbase, long stride, int n) {

@@ -49,6 +49,9 @@ int inv_max_n();
 template <typename T>
 hipError_t launch_copy_op(int m, int n, BMat<T> in, BMatW<T> out, int dagger, T sign,
                           const int* active, int batch, hipStream_t s);
+// out[p] = sign * (dagger ? M^+ : M) converted between precisions (m x n input)
+template <typename Ti, typename To>
+hipError_t launch_convert(int m, int n, BMat<Ti> in, BMatW<To> out, int dagger, To sign, int batch, hipStream_t s);
 // Gamma[p] = scale * i (S - S^+)
 template <typename T>
 hipError_t launch_gamma(int n, BMat<T> S, BMatW<T> G, T scale, const int* active, int batch, hipStream_t s);

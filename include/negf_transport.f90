@@ -23,7 +23,7 @@ module negf_transport
 
     type, bind(C) :: nt_params
         real(c_double) :: eta, eta0, convergence, device_eta_factor, lambda_eta
-        integer(c_int) :: eta_mode, max_iter, precision, reserved
+        integer(c_int) :: eta_mode, max_iter, precision, circular, n_bloch_x, n_bloch_y
     end type nt_params
 
     type, bind(C) :: nt_solve_extras
@@ -38,6 +38,12 @@ module negf_transport
             import :: nt_params
             type(nt_params), intent(out) :: p
         end subroutine
+
+        integer(c_int) function nt_params_validate(p, nl, nr) bind(C, name="nt_params_validate")
+            import :: c_int, nt_params
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: nl, nr
+        end function
 
         integer(c_int) function nt_create(ctx, device) bind(C, name="nt_create")
             import :: c_int, c_ptr

@@ -62,12 +62,26 @@ typedef struct {
     int eta_mode;             /* NT_ETA_RELATIVE (negf) or NT_ETA_ABSOLUTE (tb) */
     int max_iter;             /* decimation iteration cap, >= 1 */
     int precision;            /* NT_PREC_FP64 or NT_PREC_MIXED */
-    int reserved;
+    int circular;             /* &unfolding circular_flag: block-circulant lead decimation */
+    int n_bloch_x;            /* &unfolding n_bloch_x: lead copies along x (inner) */
+    int n_bloch_y;            /* &unfolding n_bloch_y: lead copies along y (outer) */
 } nt_params;
 
 /* Fills the negf defaults: eta = eta0 = 1e-6 (config.f90:88-89), device x10,
- * convergence 1e-12, lambda_eta 5e-3, max_iter 300, FP64. */
+ * convergence 1e-12, lambda_eta 5e-3, max_iter 300, FP64, direct decimation
+ * (circular = 0, n_bloch_x = n_bloch_y = 1).
+ * With circular = 1 the lead blocks (n x n) must be block-circulant over
+ * n_bloch_y x n_bloch_x transverse copies (copy index y outer, x inner); each
+ * lead GF is then computed as in circular_g (negf/circular.f90:43-114): a
+ * block DFT of the first block row, n_bloch_x*n_bloch_y independent
+ * decimations of size n/(n_bloch_x*n_bloch_y), and the back transform. */
 void nt_params_default(nt_params* p);
+
+/* NT_OK if p is usable for leads of nl and nr orbitals, else NT_ERR_ARG; every
+ * entry point runs the same check first.  No reference counterpart: it stands
+ * for the reference's namelist checks (negf/config.f90, tb/config.f90), which
+ * stop the program instead of returning. */
+int nt_params_validate(const nt_params* p, int nl, int nr);
 
 /* ------------------------------------------------------------ context */
 int nt_create(nt_ctx** ctx, int device_ordinal);

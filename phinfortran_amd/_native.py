@@ -29,7 +29,7 @@ NT_K_ELEM = 2
 
 # every symbol the header declares
 EXPORTS = [
-    "nt_params_default", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
+    "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
@@ -51,7 +51,9 @@ class NtParams(C.Structure):
         ("eta_mode", C.c_int),
         ("max_iter", C.c_int),
         ("precision", C.c_int),
-        ("reserved", C.c_int),
+        ("circular", C.c_int),
+        ("n_bloch_x", C.c_int),
+        ("n_bloch_y", C.c_int),
     ]
 
 
@@ -85,6 +87,8 @@ def lib():
     pp = C.POINTER(NtParams)
     L.nt_params_default.argtypes = [pp]
     L.nt_params_default.restype = None
+    L.nt_params_validate.argtypes = [pp, C.c_int, C.c_int]
+    L.nt_params_validate.restype = C.c_int
     L.nt_create.argtypes = [C.POINTER(P), I]
     L.nt_destroy.argtypes = [P]
     L.nt_sync.argtypes = [P]

@@ -51,6 +51,12 @@ struct nt_ctx {
     // device address ranges a launch may touch: the workspace and the caller's
     // buffers of the current call (sizes derived from the call's arguments)
     std::vector<std::pair<uintptr_t, uintptr_t>> regions;
+    // unitary block DFT U = P1 (1 (x) P2) of circular decimation, per lead size
+    struct CircU {
+        int n, nbx, nby;
+        double* d;
+    };
+    std::vector<CircU> circ_u;
 };
 
 namespace {
@@ -495,6 +501,181 @@ void decimate(nt_ctx* c, int n, int Q, BMat<T> H00, BMat<T> V, int alpha_dagger,
     }
 }
 
+// ------------------------------------------------ circular decimation
+// Leads that are nby x nbx transverse copies of a smaller cell have block-
+// circulant H00 and V.  circular_g (negf/circular.f90:43-114) block-diagonalises
+// them with the nested block DFT of their first block row (util.f90:732-760),
+// decimates each nac x nac block on its own (nac = n / (nby nbx)) and returns
+// g = P1 (1 (x) P2) blockdiag(g_s) (1 (x) P2)^+ P1^+ with P of set_pmatrix
+// (circular.f90:9-40).  The small problems form one larger decimation batch.
+
+// out[m * NB + i * nbx + l] = sum_{j,q} M_m[0:nac, j na + q nac + :] e^{i thy i j / nby} e^{i thx l q / nbx}
+__global__ void k_circ_forward(int n, int nby, int nbx, double thy, double thx, const double2* M, int dag, double2* out,
+                               int nm)
+{
+    const int NB = nby * nbx, na = n / nby, nac = na / nbx;
+    const int o = blockIdx.y;
+    const int m = o / NB, sidx = o % NB, i = sidx / nbx, l = sidx % nbx;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nac * nac || m >= nm) return;
+    const int r = (int)(e % nac), cc = (int)(e / nac);
+    const double2* A = M + (long)m * n * n;
+    double ar = 0.0, ai = 0.0;
+    for (int j = 0; j < nby; ++j)
+        for (int q = 0; q < nbx; ++q) {
+            const int col = j * na + q * nac + cc;
+            double2 v = dag ? A[col + (long)r * n] : A[r + (long)col * n];
+            if (dag) v.y = -v.y;
+            const double ph = thy * (double)i / (double)nby * (double)j + thx * (double)l / (double)nbx * (double)q;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            ar += v.x * cs - v.y * sn;
+            ai += v.x * sn + v.y * cs;
+        }
+    out[(long)o * nac * nac + r + (long)cc * nac] = make_double2(ar, ai);
+}
+
+// item expansion: small problem q*NB + s solves block s of lead matrix idx[q]
+__global__ void k_circ_items(const int* idx, const double* z, int Q, int NB, int* idx_c, double* z_c)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Q * NB) return;
+    const int q = t / NB, sidx = t % NB;
+    idx_c[t] = (idx ? idx[q] : 0) * NB + sidx;
+    z_c[2 * t] = z[2 * q];
+    z_c[2 * t + 1] = z[2 * q + 1];
+}
+
+__global__ void k_circ_assemble(int n, int nac, int NB, const double2* gs, double2* D, int Q)
+{
+    const int q = blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n || q >= Q) return;
+    const int r = (int)(e % n), c = (int)(e / n);
+    const int sr = r / nac, sc = c / nac;
+    double2 v = make_double2(0.0, 0.0);
+    if (sr == sc) v = gs[((long)q * NB + sr) * nac * nac + (r % nac) + (long)(c % nac) * nac];
+    D[(long)q * n * n + e] = v;
+}
+
+__global__ void k_circ_merge(const int* st_c, const int* it_c, int Q, int NB, int* status, int* iters)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Q) return;
+    int st = 0, it = 0;
+    for (int sidx = 0; sidx < NB; ++sidx) {
+        const int v = st_c[q * NB + sidx];
+        if (!st && v) st = v;
+        it = max(it, it_c[q * NB + sidx]);
+    }
+    if (!status[q]) status[q] = st;
+    iters[q] = it;
+}
+
+const double* circ_matrix(nt_ctx* c, int n, int nbx, int nby)
+{
+    for (auto& u : c->circ_u)
+        if (u.n == n && u.nbx == nbx && u.nby == nby) return u.d;
+    // P of set_pmatrix: P[(i na + k), (j na + k)] = e^{i theta i j / nb} / sqrt(nb)
+    const double th = 6.283185307179586476925286766559;
+    const int na = n / nby, nac = na / nbx;
+    std::vector<double> P1((size_t)2 * n * n, 0.0), P2((size_t)2 * na * na, 0.0), U((size_t)2 * n * n, 0.0);
+    auto pm = [&](std::vector<double>& P, int N, int nb) {
+        const int blk = N / nb;
+        for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j) {
+                const double ph = th * (double)j / (double)nb * (double)i;
+                for (int k = 0; k < blk; ++k) {
+                    const size_t e = (size_t)(i * blk + k) + (size_t)(j * blk + k) * N;
+                    P[2 * e] = cos(ph) / sqrt((double)nb);
+                    P[2 * e + 1] = sin(ph) / sqrt((double)nb);
+                }
+            }
+    };
+    pm(P1, n, nby);
+    pm(P2, na, nbx);
+    // U = P1 * blockdiag(P2, ..., P2): U[:, i na + c] = P1[:, i na : (i+1) na] P2[:, c]
+    for (int i = 0; i < nby; ++i)
+        for (int cc = 0; cc < na; ++cc)
+            for (int r = 0; r < n; ++r) {
+                double sr = 0.0, si = 0.0;
+                for (int k = 0; k < na; ++k) {
+                    const size_t a = (size_t)r + (size_t)(i * na + k) * n, b = (size_t)k + (size_t)cc * na;
+                    sr += P1[2 * a] * P2[2 * b] - P1[2 * a + 1] * P2[2 * b + 1];
+                    si += P1[2 * a] * P2[2 * b + 1] + P1[2 * a + 1] * P2[2 * b];
+                }
+                const size_t o = (size_t)r + (size_t)(i * na + cc) * n;
+                U[2 * o] = sr;
+                U[2 * o + 1] = si;
+            }
+    (void)nac;
+    double* d = nullptr;
+    hipcheck(c, hipMalloc(&d, sizeof(double) * U.size()), "hipMalloc circ");
+    hipcheck(c, hipMemcpy(d, U.data(), sizeof(double) * U.size(), hipMemcpyHostToDevice), "H2D circ");
+    c->circ_u.push_back({n, nbx, nby, d});
+    return d;
+}
+
+size_t decimate_circular_scratch(int n, int Q, int nm, int NB);
+
+// Lead surface GFs of Q items through circular_g.  H00s/Vs hold nm dense lead
+// matrices; item q uses matrix idx[q] (idx may be null when nm == 1).
+void decimate_circular(nt_ctx* c, const nt_params* prm, int n, int Q, int nm, const double* H00s, const double* Vs,
+                       const int* idx, int alpha_dagger, const double* z, DecimationOut<double> out, int* status,
+                       int* iters, Arena& ar, size_t scratch_off)
+{
+    typedef double T;
+    const int nbx = prm->n_bloch_x, nby = prm->n_bloch_y, NB = nbx * nby;
+    const int nac = n / NB;
+    const long nn = (long)n * n, ns = (long)nac * nac;
+    Arena sa = ar;
+    sa.off = scratch_off;
+    T* Hs = sa.ptr<T>(sa.take(sizeof(T) * 2 * ns * nm * NB));
+    T* As = sa.ptr<T>(sa.take(sizeof(T) * 2 * ns * nm * NB));
+    int* idx_c = sa.ptr<int>(sa.take(sizeof(int) * Q * NB));
+    T* z_c = sa.ptr<T>(sa.take(sizeof(T) * 2 * Q * NB));
+    int* st_c = sa.ptr<int>(sa.take(sizeof(int) * Q * NB));
+    int* it_c = sa.ptr<int>(sa.take(sizeof(int) * Q * NB));
+    T* gs = sa.ptr<T>(sa.take(sizeof(T) * 2 * ns * Q * NB));
+    T* D = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * Q));
+    T* T1 = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * Q));
+    const size_t dec_off = sa.off;
+    hipStream_t s = c->stream;
+    const double th = 6.283185307179586476925286766559;   // G.R of a lattice and its reciprocal
+    check_range(c, H00s, sizeof(T) * 2 * nn * nm, "circ H00");
+    check_range(c, Vs, sizeof(T) * 2 * nn * nm, "circ V");
+    check_range(c, idx, sizeof(int) * Q, "circ idx");
+    check_range(c, z, sizeof(T) * 2 * Q, "circ z");
+    hipLaunchKernelGGL(k_circ_forward, dim3(nblk(ns), nm * NB), dim3(256), 0, s, n, nby, nbx, th, th,
+                       reinterpret_cast<const double2*>(H00s), 0, reinterpret_cast<double2*>(Hs), nm);
+    // alpha0 = V^+ or V is transformed itself (its first block row), beta = alpha^+
+    hipLaunchKernelGGL(k_circ_forward, dim3(nblk(ns), nm * NB), dim3(256), 0, s, n, nby, nbx, th, th,
+                       reinterpret_cast<const double2*>(Vs), alpha_dagger, reinterpret_cast<double2*>(As), nm);
+    hipLaunchKernelGGL(k_circ_items, dim3(nblk((long)Q * NB)), dim3(256), 0, s, idx, z, Q, NB, idx_c, z_c);
+    hipcheck(c, hipGetLastError(), "circ forward");
+    hipcheck(c, hipMemsetAsync(st_c, 0, sizeof(int) * Q * NB, s), "memset");
+    DecimationOut<T> os{bw(gs, ns, nac), nonew<T>(), nonew<T>()};
+    decimate<T>(c, nac, Q * NB, bm<T>(Hs, ns, nac, idx_c, 0, nm * NB), bm<T>(As, ns, nac, idx_c, 0, nm * NB), 0, z_c,
+                prm->convergence, prm->max_iter, os, st_c, it_c, ar, dec_off);
+    hipLaunchKernelGGL(k_circ_assemble, dim3(nblk(nn), Q), dim3(256), 0, s, n, nac, NB,
+                       reinterpret_cast<const double2*>(gs), reinterpret_cast<double2*>(D), Q);
+    hipLaunchKernelGGL(k_circ_merge, dim3(nblk(Q)), dim3(256), 0, s, st_c, it_c, Q, NB, status, iters);
+    hipcheck(c, hipGetLastError(), "circ assemble");
+    const T* U = circ_matrix(c, n, nbx, nby);
+    reg_region(c, U, sizeof(T) * 2 * nn);
+    gemm<T>(c, n, n, n, OP_N, bm<T>(U, 0, n), OP_N, bm<T>(D, nn, n), bw(T1, nn, n), 1.0, none<T>(), 0.0, nullptr, Q);
+    gemm<T>(c, n, n, n, OP_N, bm<T>(T1, nn, n), OP_C, bm<T>(U, 0, n), out.g, 1.0, none<T>(), 0.0, nullptr, Q);
+    if (out.sigma.base) {
+        // Sigma = alpha0 g beta0 with the full lead coupling (surface.f90:504, :565)
+        const BMat<T> V = bm<T>(Vs, nn, n, idx, 0, nm);
+        gemm<T>(c, n, n, n, alpha_dagger ? OP_C : OP_N, V, OP_N, ro(out.g), bw(D, nn, n), 1.0, none<T>(), 0.0, nullptr,
+                Q);
+        gemm<T>(c, n, n, n, OP_N, bm<T>(D, nn, n), alpha_dagger ? OP_N : OP_C, V, out.sigma, 1.0, none<T>(), 0.0,
+                nullptr, Q);
+        if (out.gamma.base) gamma_op<T>(c, n, ro(out.sigma), out.gamma, Q);
+    }
+}
+
 template <typename T>
 size_t decimate_scratch(int n, int Q)
 {
@@ -511,6 +692,45 @@ size_t decimate_scratch(int n, int Q)
     a.take(sizeof(unsigned long long) * Q);
     a.take(sizeof(int));
     return a.off;
+}
+
+size_t decimate_circular_scratch(int n, int Q, int nm, int NB)
+{
+    typedef double T;
+    const int nac = n / NB;
+    const size_t nn = (size_t)n * n, ns = (size_t)nac * nac;
+    Arena a;
+    a.take(sizeof(T) * 2 * ns * nm * NB);
+    a.take(sizeof(T) * 2 * ns * nm * NB);
+    a.take(sizeof(int) * Q * NB);
+    a.take(sizeof(T) * 2 * Q * NB);
+    a.take(sizeof(int) * Q * NB);
+    a.take(sizeof(int) * Q * NB);
+    a.take(sizeof(T) * 2 * ns * Q * NB);
+    a.take(sizeof(T) * 2 * nn * Q);
+    a.take(sizeof(T) * 2 * nn * Q);
+    return a.off + decimate_scratch<T>(nac, Q * NB);
+}
+
+// Surface GFs (and Sigma, Gamma) of Q lead items; direct or circular decimation.
+void lead_gfs(nt_ctx* c, const nt_params* prm, int n, int Q, int nm, const double* H00s, const double* Vs,
+              const int* idx, int alpha_dagger, const double* z, DecimationOut<double> out, int* status, int* iters,
+              Arena& ar, size_t scratch_off)
+{
+    if (prm->circular && prm->n_bloch_x * prm->n_bloch_y > 1) {
+        decimate_circular(c, prm, n, Q, nm, H00s, Vs, idx, alpha_dagger, z, out, status, iters, ar, scratch_off);
+    } else {
+        const long nn = (long)n * n;
+        decimate<double>(c, n, Q, bm<double>(H00s, nm > 1 || idx ? nn : 0, n, idx, 0, idx ? nm : 0),
+                         bm<double>(Vs, nm > 1 || idx ? nn : 0, n, idx, 0, idx ? nm : 0), alpha_dagger, z,
+                         prm->convergence, prm->max_iter, out, status, iters, ar, scratch_off);
+    }
+}
+
+size_t lead_gfs_scratch(const nt_params* prm, int n, int Q, int nm)
+{
+    const int NB = prm->circular ? prm->n_bloch_x * prm->n_bloch_y : 1;
+    return NB > 1 ? decimate_circular_scratch(n, Q, nm, NB) : decimate_scratch<double>(n, Q);
 }
 
 // ------------------------------------------------------------------ RGF
@@ -666,6 +886,17 @@ struct TdbArgs {
 int check_params(const nt_params* p)
 {
     if (!p || p->max_iter < 1 || !(p->convergence > 0.0) || (p->eta_mode != 0 && p->eta_mode != 1)) return NT_ERR_ARG;
+    if (p->precision != NT_PREC_FP64 && p->precision != NT_PREC_MIXED) return NT_ERR_ARG;
+    if (p->circular && (p->n_bloch_x < 1 || p->n_bloch_y < 1)) return NT_ERR_ARG;
+    return 0;
+}
+
+// circular decimation needs both lead sizes divisible by the number of copies
+int check_circular(const nt_params* p, int nl, int nr)
+{
+    if (!p->circular) return 0;
+    const int nb = p->n_bloch_x * p->n_bloch_y;
+    if (nl % nb || nr % nb || (nl / p->n_bloch_y) % p->n_bloch_x) return NT_ERR_ARG;
     return 0;
 }
 
@@ -725,7 +956,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     o_l32[7] = l32.take(c32 * nl * nr * Pmax);                               // G_D_RL
     o_l32[8] = l32.take(c32 * nll * Pmax);                                   // G_L
     o_l32[9] = l32.take(c32 * nrr * Pmax);                                   // G_R
-    const size_t need_dec = decimate_scratch<T>(nmax, 2 * Pmax);
+    const size_t need_dec = lead_gfs_scratch(prm, nmax, 2 * Pmax, 2 * a.nk);
     const size_t need_rgf = mixed ? l32.off + rgf_scratch<float>(nl, nr, b, Pmax) : rgf_scratch<T>(nl, nr, b, Pmax);
     ensure_ws(c, o_scr + std::max(need_dec, need_rgf));
     ar.base = static_cast<char*>(c->ws);
@@ -808,12 +1039,10 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         if (same) {
             const long n2 = nll;
             DecimationOut<T> o{bw(gbuf, n2, nl), bw(sigbuf, n2, nl), bw(gambuf, n2, nl)};
-            decimate<T>(c, nl, 2 * P, bm<T>(lh, n2, nl, idx, 0, 2 * a.nk), bm<T>(lv, n2, nl, idx, 0, 2 * a.nk), 1,
-                        zdec, prm->convergence, prm->max_iter, o, stdec, itdec, ar, o_scr);
+            lead_gfs(c, prm, nl, 2 * P, 2 * a.nk, lh, lv, idx, 1, zdec, o, stdec, itdec, ar, o_scr);
             if (pm) {
                 DecimationOut<T> o2{bw(gpmbuf, n2, nl), nonew<T>(), nonew<T>()};
-                decimate<T>(c, nl, 2 * P, bm<T>(lh, n2, nl, idx, 0, 2 * a.nk), bm<T>(lv, n2, nl, idx, 0, 2 * a.nk), 0,
-                            zpm, prm->convergence, prm->max_iter, o2, stpm, itpm, ar, o_scr);
+                lead_gfs(c, prm, nl, 2 * P, 2 * a.nk, lh, lv, idx, 0, zpm, o2, stpm, itpm, ar, o_scr);
             }
         } else {
             // separate batches per lead: item strides 2*nmm keep [left, right] pairs
@@ -1026,6 +1255,12 @@ void down(nt_ctx* c, void* h, const DevBuf& d, size_t bytes)
 // =================================================================== C ABI
 extern "C" {
 
+int nt_params_validate(const nt_params* p, int nl, int nr)
+{
+    if (!p || check_params(p) || nl <= 0 || nr <= 0 || check_circular(p, nl, nr)) return NT_ERR_ARG;
+    return NT_OK;
+}
+
 void nt_params_default(nt_params* p)
 {
     if (!p) return;
@@ -1037,7 +1272,9 @@ void nt_params_default(nt_params* p)
     p->eta_mode = NT_ETA_RELATIVE;
     p->max_iter = 300;
     p->precision = NT_PREC_FP64;
-    p->reserved = 0;
+    p->circular = 0;
+    p->n_bloch_x = 1;
+    p->n_bloch_y = 1;
 }
 
 int nt_create(nt_ctx** out, int device_ordinal)
@@ -1070,6 +1307,7 @@ int nt_destroy(nt_ctx* c)
     }
     for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->ws) (void)hipFree(c->ws);
+    for (auto& u : c->circ_u) (void)hipFree(u.d);
     if (c->host_flag) (void)hipHostFree(c->host_flag);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
@@ -1154,7 +1392,7 @@ int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int 
                              int* status, int* iters, const nt_solve_extras* extras)
 {
     if (!c) return NT_ERR_ARG;
-    if (check_params(p)) return NT_ERR_ARG;
+    if (check_params(p) || check_circular(p, nl, nr)) return NT_ERR_ARG;
     if (nl <= 0 || nr <= 0 || b <= 0 || L <= 0 || npts < 0 || nk <= 0 || (nk > 1 && !kidx)) return NT_ERR_ARG;
     if (std::max(std::max(nl, nr), b) > inv_max_n()) return NT_ERR_ARG;
     if (!E || !Haml || !Hamlv || !Hamr || !Hamrv || !Hamd || (L > 1 && !Hamdv) || !Hamld || !Hamrd || !T_LR || !T_RL)
@@ -1219,7 +1457,7 @@ int nt_solve_tdb_batched(nt_ctx* c, const nt_params* p, int nl, int nr, int b, i
                          int* iters, const nt_solve_extras* extras)
 {
     if (!c) return NT_ERR_ARG;
-    if (check_params(p)) return NT_ERR_ARG;
+    if (check_params(p) || check_circular(p, nl, nr)) return NT_ERR_ARG;
     if (nl <= 0 || nr <= 0 || b <= 0 || L <= 0 || npts < 0 || nk <= 0 || (nk > 1 && !kidx)) return NT_ERR_ARG;
     if (!E || !Haml || !Hamlv || !Hamr || !Hamrv || !Hamd || (L > 1 && !Hamdv) || !Hamld || !Hamrd || !T_LR || !T_RL)
         return NT_ERR_ARG;
@@ -1318,6 +1556,7 @@ int nt_surface_green_tdb(nt_ctx* c, const nt_params* p, int nl, int nr, double E
 {
     if (!c) return NT_ERR_ARG;
     if (check_params(p) || nl <= 0 || nr <= 0 || !Haml || !Hamr || !Hamlv || !Hamrv) return NT_ERR_ARG;
+    if (check_circular(p, nl, nr)) return NT_ERR_ARG;
     if (std::max(nl, nr) > inv_max_n()) return NT_ERR_ARG;
     return guarded(c, [&] {
         typedef double T;
@@ -1340,14 +1579,13 @@ int nt_surface_green_tdb(nt_ctx* c, const nt_params* p, int nl, int nr, double E
         const long nmm = (long)nmax * nmax;
         DevBuf dg(c, z * nmm * 4), dsig(c, z * nmm * 2), dgam(c, z * nmm * 2);
         Arena ar;
-        ensure_ws(c, decimate_scratch<T>(nmax, 1));
+        ensure_ws(c, lead_gfs_scratch(p, nmax, 1, 1));
         ar.base = static_cast<char*>(c->ws);
         const T* zz = dzz.as<T>();
         auto one = [&](int n, const T* H, const T* V, int adag, const T* zp, T* g, T* sig, T* gam, int* st, int* it) {
             DecimationOut<T> o{bw(g, (long)n * n, n), sig ? bw(sig, (long)n * n, n) : nonew<T>(),
                                gam ? bw(gam, (long)n * n, n) : nonew<T>()};
-            decimate<T>(c, n, 1, bm<T>(H, 0, n), bm<T>(V, 0, n), adag, zp, p->convergence, p->max_iter, o, st, it, ar,
-                        0);
+            lead_gfs(c, p, n, 1, 1, H, V, nullptr, adag, zp, o, st, it, ar, 0);
         };
         T* g = dg.as<T>();
         one(nl, dHl.as<T>(), dHlv.as<T>(), 1, zz + 0, g, dsig.as<T>(), dgam.as<T>(), dst.as<int>(), dit.as<int>());

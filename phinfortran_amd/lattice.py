@@ -344,3 +344,49 @@ def _agnr_topology(n):
                 if abs(d - acc) < 1e-6 and (shift > 0 or i < j):
                     lst.append((i, j))
     return intra, inter
+
+
+# ------------------------------------------------------------ block-circulant leads
+
+def circulant_matrix(cells, nbx, nby):
+    """Full matrix of an nby x nbx supercell from its translation blocks.
+
+    cells[(dx, dy)] couples cell (ix, iy) to cell ((ix+dx) % nbx, (iy+dy) % nby);
+    cells are ordered y-major (index iy*nbx + ix), which is the nesting that
+    negf/circular.f90 block-diagonalises (outer y, inner x)."""
+    na = next(iter(cells.values())).shape[0]
+    n = nbx * nby * na
+    M = np.zeros((n, n), np.complex128)
+    for iy in range(nby):
+        for ix in range(nbx):
+            for jy in range(nby):
+                for jx in range(nbx):
+                    d = ((jx - ix) % nbx, (jy - iy) % nby)
+                    r, c = (iy * nbx + ix) * na, (jy * nbx + jx) * na
+                    M[r:r + na, c:c + na] = cells[d]
+    return M
+
+
+def circulant_lead_blocks(na, nbx, nby, L, seed=0, scale=0.3, device_noise=0.1):
+    """Random Hermitian system whose leads are nby x nbx supercells with
+    block-circulant H00 and V (a lead at a transverse k commensurate with the
+    supercell); the device is the lead plus a random Hermitian perturbation."""
+    rng = np.random.default_rng(seed)
+
+    def c(m, n):
+        return (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))) * scale / math.sqrt(max(m, n))
+
+    keys = [(dx, dy) for dx in range(nbx) for dy in range(nby)]
+    K = circulant_matrix({d: c(na, na) for d in keys}, nbx, nby)
+    H = 0.5 * (K + K.conj().T)
+    V = circulant_matrix({d: c(na, na) for d in keys}, nbx, nby)
+    n = H.shape[0]
+
+    def pert():
+        x = c(n, n) * device_noise
+        return 0.5 * (x + x.conj().T)
+    return Blocks(Haml=H.copy(), Hamlv=V.copy(), Hamr=H.copy(), Hamrv=V.conj().T.copy(),
+                  Hamd=np.stack([H + pert() for _ in range(L)]),
+                  Hamdv=np.stack([V + c(n, n) * device_noise for _ in range(L - 1)]) if L > 1
+                  else np.zeros((0, n, n), np.complex128),
+                  Hamld=V.copy(), Hamrd=V.conj().T.copy())

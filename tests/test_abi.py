@@ -63,3 +63,21 @@ def test_argument_validation_without_gpu():
     assert L.nt_sync(None) == N.NT_ERR_ARG
     assert L.nt_status_string(N.NT_ERR_NOT_CONVERGED).decode().startswith("surface decimation")
     assert L.nt_status_string(3).decode() == "Matrix is numerically singular!"
+
+
+def test_circular_sizes_validated_without_gpu():
+    """circular decimation needs n divisible by n_bloch_x * n_bloch_y"""
+    from phinfortran_amd import _native as N
+    L = N.lib()
+    p = N.NtParams()
+    L.nt_params_default(ctypes.byref(p))
+    assert p.circular == 0 and p.n_bloch_x == 1 and p.n_bloch_y == 1
+    assert L.nt_params_validate(ctypes.byref(p), 12, 12) == N.NT_OK
+    p.circular, p.n_bloch_x, p.n_bloch_y = 1, 3, 2
+    assert L.nt_params_validate(ctypes.byref(p), 12, 12) == N.NT_OK
+    assert L.nt_params_validate(ctypes.byref(p), 12, 8) == N.NT_ERR_ARG
+    assert L.nt_params_validate(ctypes.byref(p), 9, 9) == N.NT_ERR_ARG
+    p.n_bloch_x = 0
+    assert L.nt_params_validate(ctypes.byref(p), 12, 12) == N.NT_ERR_ARG
+    p.n_bloch_x, p.precision = 1, 7
+    assert L.nt_params_validate(ctypes.byref(p), 12, 12) == N.NT_ERR_ARG

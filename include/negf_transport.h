@@ -201,6 +201,34 @@ int nt_solve_tdb_batched(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b,
                          double* T_LR, double* T_RL, int* status, int* iters,
                          const nt_solve_extras* extras);
 
+/* ------------------------------------------------ adjoint (gradient) solve
+ * Batched solve that also returns the gradient of W = sum_i weight[i] T_LR[i]
+ * with respect to the device blocks of layers lo..hi (0-based, 0 <= lo <= hi
+ * < L).  No counterpart in the reference, which has no derivatives; central
+ * finite differences of its transl (negf/lesser.f90:787-800) pin it.
+ *
+ * With G the Green's function of the left-to-right sweep and
+ * A = Gamma_R G_{0,N+1}^+ Gamma_L, dT_LR = 2 Re Tr[dH X] where
+ * X_{j,i} = G_{j,N+1} A G_{0,i}.  The outputs are ACCUMULATED (+=) per k-point,
+ * in the convention of torch's complex autograd (dW/dRe + i dW/dIm):
+ *   dW_dHamd   nk x (hi-lo+1) x (b x b):  2 sum_i w_i X_{l,l}^+
+ *   dW_dHamdv  nk x nc x (b x b), nc = min(hi, L-2)-lo+1 (may be NULL if nc<=0):
+ *              2 sum_i w_i (X_{l+1,l}^+ + X_{l,l+1})  for coupling l -> l+1,
+ *              whose dagger is the lower block of the tdb layout.
+ * Points with a nonzero status or a nonzero first_point_mask add nothing.
+ * The sweep runs in FP64 whatever p->precision says.  Other arguments as
+ * nt_solve_tdb_batched_dev (device pointers). */
+int nt_solve_tdb_adjoint_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b, int L,
+                                     int npts, const double* E, const int* kidx, int nk,
+                                     const nt_c128* Haml, const nt_c128* Hamlv,
+                                     const nt_c128* Hamr, const nt_c128* Hamrv,
+                                     const nt_c128* Hamd, const nt_c128* Hamdv,
+                                     const nt_c128* Hamld, const nt_c128* Hamrd,
+                                     const int* first_point_mask, const double* weight,
+                                     int lo, int hi,
+                                     double* T_LR, double* T_RL, int* status, int* iters,
+                                     nt_c128* dW_dHamd, nt_c128* dW_dHamdv);
+
 /* ------------------------------------------------ batched building blocks
  * The two kernels every solve is made of, on device memory (column-major,
  * leading dimension = rows, item p at base + p*stride elements; stride 0

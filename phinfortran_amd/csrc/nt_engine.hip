@@ -752,6 +752,102 @@ struct RgfOutput {
     BMatW<T> G_D_LR, G_D_RL, G_L, G_R;   // may be null (then scratch)
 };
 
+// Adjoint of T_LR (FP64 only).  Layers [lo, hi] of the device (0-based) form
+// the window whose blocks get gradients.  All buffers hold P points of
+// bmax^2 complex entries per slot (slot-major, then point).
+struct AdjSpec {
+    int lo, hi;
+    const double* w;     // per-point weight of T_LR in the objective
+    double* SL;          // left-connected G^L_i, i = lo..L-1           (L - lo slots)
+    double* S0;          // left-connected corner G^L_{0,i}, i = lo..L-1 (L - lo slots)
+    double* CW;          // column G_{j,N+1}, j = lo..hi1              (hi1 - lo + 1 slots)
+    double* RW;          // row G_{0,j}, j = lo..hi1                   (hi1 - lo + 1 slots)
+    double* CA;          // G_{j,N+1} A                                 (hi1 - lo + 1 slots)
+    double* Zd;          // X_{l,l}^+, l = lo..hi                       (hi - lo + 1 slots)
+    double* Zc;          // X_{l+1,l}^+ + X_{l,l+1}, l = lo..hc         (hc - lo + 1 slots)
+    double* A;           // w Gamma_R G_{0,N+1}^+ Gamma_L               (1 slot)
+    double* GR;          // G_{N+1,N+1}                                 (1 slot)
+    double* Cb[2];       // recursion ping-pong
+    double* Rb[2];
+};
+
+inline int adj_hi1(int hi, int L) { return std::min(hi + 1, L - 1); }
+inline int adj_hc(int hi, int L) { return std::min(hi, L - 2); }
+inline int adj_slots(int lo, int hi, int L)
+{
+    const int nw1 = adj_hi1(hi, L) - lo + 1;
+    return 2 * (L - lo) + 3 * nw1 + (hi - lo + 1) + std::max(0, adj_hc(hi, L) - lo + 1) + 6;
+}
+
+// carve the adjoint buffers (P points, slot size mm complex) from an arena
+inline AdjSpec adj_carve(Arena& a, int lo, int hi, int L, long mm, int P, const double* w)
+{
+    const size_t sb = sizeof(double) * 2 * (size_t)mm * P;
+    const int nw1 = adj_hi1(hi, L) - lo + 1, nc = std::max(0, adj_hc(hi, L) - lo + 1);
+    AdjSpec s;
+    s.lo = lo;
+    s.hi = hi;
+    s.w = w;
+    s.SL = a.ptr<double>(a.take(sb * (L - lo)));
+    s.S0 = a.ptr<double>(a.take(sb * (L - lo)));
+    s.CW = a.ptr<double>(a.take(sb * nw1));
+    s.RW = a.ptr<double>(a.take(sb * nw1));
+    s.CA = a.ptr<double>(a.take(sb * nw1));
+    s.Zd = a.ptr<double>(a.take(sb * (hi - lo + 1)));
+    s.Zc = a.ptr<double>(a.take(sb * std::max(nc, 1)));
+    s.A = a.ptr<double>(a.take(sb));
+    s.GR = a.ptr<double>(a.take(sb));
+    for (int i = 0; i < 2; ++i) {
+        s.Cb[i] = a.ptr<double>(a.take(sb));
+        s.Rb[i] = a.ptr<double>(a.take(sb));
+    }
+    return s;
+}
+
+__global__ void k_scale_points(double2* A, long n, long stride, const double* w, int P)
+{
+    const int p = blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || e >= n) return;
+    const double f = w[p];
+    double2 v = A[(long)p * stride + e];
+    A[(long)p * stride + e] = make_double2(v.x * f, v.y * f);
+}
+
+// out[k][blk][e] += 2 sum_{p : k(p) = k, point ok} Z[blk][p][e]
+__global__ void k_adj_reduce(const double2* Z, long mm, int P, long ne, const int* kidx, const int* stdec,
+                             const int* strgf, const int* first_mask, double2* out, int nblk)
+{
+    const int blk = blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne || blk >= nblk) return;
+    const double2* z = Z + (long)blk * mm * P + e;
+    int kc = -1;
+    double ar = 0.0, ai = 0.0;
+    for (int p = 0; p < P; ++p) {
+        const bool ok = !stdec[2 * p] && !stdec[2 * p + 1] && !strgf[p] && !(first_mask && first_mask[p]);
+        if (!ok) continue;
+        const int k = kidx ? kidx[p] : 0;
+        if (k != kc) {
+            if (kc >= 0) {
+                double2* o = out + ((long)kc * nblk + blk) * ne + e;
+                o->x += 2.0 * ar;
+                o->y += 2.0 * ai;
+            }
+            kc = k;
+            ar = ai = 0.0;
+        }
+        const double2 v = z[(long)p * mm];
+        ar += v.x;
+        ai += v.y;
+    }
+    if (kc >= 0) {
+        double2* o = out + ((long)kc * nblk + blk) * ne + e;
+        o->x += 2.0 * ar;
+        o->y += 2.0 * ai;
+    }
+}
+
 template <typename T>
 size_t rgf_scratch(int nl, int nr, int bmax, int P)
 {
@@ -762,9 +858,87 @@ size_t rgf_scratch(int nl, int nr, int bmax, int P)
     return a.off;
 }
 
+// Adjoint of T_LR = Re Tr[G_{0,N+1} Gamma_R G_{0,N+1}^+ Gamma_L] with respect to the
+// device blocks.  G is the Green's function of the system the left-to-right
+// sweep inverts (layer 0 closed by the left surface GF, layers 1..N+1 at the
+// device broadening, Sigma_R on N+1).  For a change dH of the device blocks
+// dG = G dH G, so dT = 2 Re Tr[dH X] with X = G Gamma_R G^+ Gamma_L G and
+//   X_{j,i} = G_{j,N+1} A G_{0,i},   A = Gamma_R G_{0,N+1}^+ Gamma_L.
+// The column and the row come from the stored left-connected blocks by the
+// Dyson identities G = g + g V G and G = g + G V g of the cut between layers
+// j and j+1:
+//   G_{j,N+1} = G^L_j V_{j,j+1} G_{j+1,N+1},
+//   G_{0,j}   = G^L_{0,j} + G_{0,j+1} V_{j+1,j} G^L_j.
+// The gradients (real objective, complex blocks, d/dRe + i d/dIm) are
+// 2 X_{l,l}^+ for Hamd_l and 2 (X_{l+1,l}^+ + X_{l,l+1}) for Hamdv_l, whose
+// dagger is the lower block.  The weight w multiplies A.
+template <typename T>
+void adjoint_backward(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, int P, long mm, BMatW<T> gdlr, BMatW<T> gRo,
+                      const BMat<T>& Vc, int opVc, int opVcH, const AdjSpec& adj, T* T1, T* T2)
+{
+    const int L = in.L;
+    const long slot = mm * P;
+    auto W = [&](T* p, int ld) { return bw(p, mm, ld); };
+    auto R = [&](T* p, int ld) { return bm<T>(p, mm, ld); };
+    auto at = [&](double* base, int k) { return reinterpret_cast<T*>(base) + 2 * (long)k * slot; };
+    const int lo = adj.lo, hi = adj.hi, hi1 = adj_hi1(hi, L), hc = adj_hc(hi, L);
+    T* A = reinterpret_cast<T*>(adj.A);
+    hipStream_t s = c->stream;
+    // A = w Gamma_R G_{0,N+1}^+ Gamma_L    (nr x nl)
+    gemm<T>(c, nr, nl, nr, OP_N, in.gmr, OP_C, ro(gdlr), W(T1, nr), 1.0, none<T>(), 0.0, nullptr, P);
+    gemm<T>(c, nr, nl, nl, OP_N, R(T1, nr), OP_N, in.gml, W(A, nr), 1.0, none<T>(), 0.0, nullptr, P);
+    check_range(c, adj.w, sizeof(double) * P, "adjoint weights");
+    hipLaunchKernelGGL(k_scale_points, dim3(nblk((long)nr * nl), P), dim3(256), 0, s, reinterpret_cast<double2*>(A),
+                       (long)nr * nl, mm, adj.w, P);
+    hipcheck(c, hipGetLastError(), "scale points");
+    // backward recursion from the closing layer N+1 to the window
+    BMat<T> Ccur = ro(gRo);      // G_{N+1,N+1}   (nr x nr)
+    BMat<T> Rcur = ro(gdlr);     // G_{0,N+1}     (nl x nr)
+    int pp = 0;
+    for (int j = L - 1; j >= lo; --j) {
+        const int bj = in.bsz[j];
+        const bool last = j == L - 1;
+        const int bn = last ? nr : in.bsz[j + 1];
+        const BMat<T> V = last ? Vc : in.Hdv[j];          // V_{j,j+1} = op(V)
+        const int opV = last ? opVc : OP_N, opVH = last ? opVcH : OP_C;
+        T* Cn = j <= hi1 ? at(adj.CW, j - lo) : reinterpret_cast<T*>(adj.Cb[pp]);
+        T* Rn = j <= hi1 ? at(adj.RW, j - lo) : reinterpret_cast<T*>(adj.Rb[pp]);
+        pp ^= 1;
+        // G_{j,N+1} = G^L_j (V_{j,j+1} G_{j+1,N+1})
+        gemm<T>(c, bj, nr, bn, opV, V, OP_N, Ccur, W(T1, bj), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, bj, nr, bj, OP_N, R(at(adj.SL, j - lo), bj), OP_N, R(T1, bj), W(Cn, bj), 1.0, none<T>(), 0.0,
+                nullptr, P);
+        // G_{0,j} = G^L_{0,j} + (G_{0,j+1} V_{j+1,j}) G^L_j
+        gemm<T>(c, nl, bj, bn, OP_N, Rcur, opVH, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, nl, bj, bj, OP_N, R(T2, nl), OP_N, R(at(adj.SL, j - lo), bj), W(Rn, nl), 1.0,
+                R(at(adj.S0, j - lo), nl), 1.0, nullptr, P);
+        Ccur = R(Cn, bj);
+        Rcur = R(Rn, nl);
+    }
+    // X blocks of the window (stored as the gradient operands)
+    for (int j = lo; j <= hi1; ++j)
+        gemm<T>(c, in.bsz[j], nl, nr, OP_N, R(at(adj.CW, j - lo), in.bsz[j]), OP_N, R(A, nr),
+                W(at(adj.CA, j - lo), in.bsz[j]), 1.0, none<T>(), 0.0, nullptr, P);
+    for (int l = lo; l <= hi; ++l) {
+        const int bl = in.bsz[l];
+        // X_{l,l}^+ = G_{0,l}^+ (G_{l,N+1} A)^+
+        gemm<T>(c, bl, bl, nl, OP_C, R(at(adj.RW, l - lo), nl), OP_C, R(at(adj.CA, l - lo), bl),
+                W(at(adj.Zd, l - lo), bl), 1.0, none<T>(), 0.0, nullptr, P);
+    }
+    for (int l = lo; l <= hc; ++l) {
+        const int bl = in.bsz[l], bn = in.bsz[l + 1];
+        T* Z = at(adj.Zc, l - lo);
+        // X_{l+1,l}^+ = G_{0,l}^+ (G_{l+1,N+1} A)^+ ; + X_{l,l+1} = (G_{l,N+1} A) G_{0,l+1}
+        gemm<T>(c, bl, bn, nl, OP_C, R(at(adj.RW, l - lo), nl), OP_C, R(at(adj.CA, l + 1 - lo), bn), W(Z, bl), 1.0,
+                none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, bl, bn, nl, OP_N, R(at(adj.CA, l - lo), bl), OP_N, R(at(adj.RW, l + 1 - lo), nl), W(Z, bl), 1.0,
+                R(Z, bl), 1.0, nullptr, P);
+    }
+}
+
 template <typename T>
 void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int P, RgfOutput<T> out, Arena& ar,
-             size_t scratch_off)
+             size_t scratch_off, const AdjSpec* adj = nullptr)
 {
     int bmax = std::max(nl, nr);
     for (int b : in.bsz) bmax = std::max(bmax, b);
@@ -793,9 +967,24 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     BMatW<T> gRo = out.G_R.base ? out.G_R : W(Gn, nr);
     int* st = out.status;
 
+    // adjoint storage slots of device layer i (null when layer i is not stored)
+    const long slot = mm * P;
+    auto slotL = [&](int i) -> T* {
+        return adj && i >= adj->lo ? reinterpret_cast<T*>(adj->SL) + 2 * (long)(i - adj->lo) * slot : nullptr;
+    };
+    auto slot0 = [&](int i) -> T* {
+        return adj && i >= adj->lo ? reinterpret_cast<T*>(adj->S0) + 2 * (long)(i - adj->lo) * slot : nullptr;
+    };
+    T* const gbufs[2] = {G, Gn};
+    T* const g0bufs[2] = {G0, G0n};
+    auto other = [](T* const* bufs, T* cur) { return cur == bufs[0] ? bufs[1] : bufs[0]; };
+    if (adj) gRo = W(reinterpret_cast<T*>(adj->GR), nr);
+
     // ---------------- left to right
     {
         const int b = in.bsz[0];
+        if (T* sl = slotL(0)) G = sl;
+        if (T* s0 = slot0(0)) G0 = s0;
         gemm<T>(c, b, nl, nl, OP_C, in.Hld, OP_N, in.gl, W(T1, b), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, in.Hld, W(Aw, b), -1.0, in.Hd[0], -1.0, zd, P);
         inverse<T>(c, b, W(Aw, b), none<T>(), nullptr, W(G, b), st, P);
@@ -804,14 +993,17 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         for (int i = 1; i < L; ++i) {
             const int bp = in.bsz[i - 1], bi = in.bsz[i];
             const BMat<T>& V = in.Hdv[i - 1];
+            T* Gi = slotL(i) ? slotL(i) : other(gbufs, G);
+            T* G0i = slot0(i) ? slot0(i) : other(g0bufs, G0);
             gemm<T>(c, bi, bp, bp, OP_C, V, OP_N, R(G, bp), W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P);
             gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, V, W(Aw, bi), -1.0, in.Hd[i], -1.0, zd, P);
-            inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gn, bi), st, P);
+            inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gi, bi), st, P);
             gemm<T>(c, nl, bi, bp, OP_N, R(G0, nl), OP_N, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
-            gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gn, bi), W(G0n, nl), 1.0, none<T>(), 0.0, nullptr, P);
-            std::swap(G, Gn);
-            std::swap(G0, G0n);
+            gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gi, bi), W(G0i, nl), 1.0, none<T>(), 0.0, nullptr, P);
+            G = Gi;
+            G0 = G0i;
         }
+        if (!out.G_R.base && !adj) gRo = W(other(gbufs, G), nr);
         const int bN = in.bsz[L - 1];
         // closing lead layer N+1: Vc = V_{N,N+1} = Hrd_fwd or Hrd^+; (Vc^+ G) Vc
         const bool fwd = in.Hrd_fwd.base != nullptr;
@@ -829,8 +1021,13 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         gemm<T>(c, nl, nr, nr, OP_N, ro(gdlr), OP_N, in.gmr, W(X, nl), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, nr, nl, nl, OP_C, ro(gdlr), OP_N, in.gml, W(Y, nr), 1.0, none<T>(), 0.0, nullptr, P);
         trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P);
+        if (adj) adjoint_backward<T>(c, nl, nr, in, P, mm, gdlr, gRo, Vc, opVc, opVcH, *adj, T1, T2);
     }
     // ---------------- right to left
+    G = gbufs[0];
+    Gn = gbufs[1];
+    G0 = g0bufs[0];
+    G0n = g0bufs[1];
     {
         const int bN = in.bsz[L - 1];
         gemm<T>(c, bN, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, P);
@@ -881,7 +1078,15 @@ struct TdbArgs {
     double *T_LR, *T_RL;
     int *status, *iters;
     nt_solve_extras ex;
+    // adjoint of sum_p w_p T_LR(p) (w null: no adjoint)
+    const double* adj_w = nullptr;
+    int adj_lo = 0, adj_hi = -1;
+    nt_c128* dHd = nullptr;    // (nk, hi - lo + 1, b, b), accumulated
+    nt_c128* dHdv = nullptr;   // (nk, min(hi, L-2) - lo + 1, b, b), accumulated
 };
+
+// adjoint storage is capped so that a chunk of points fits this budget
+constexpr double ADJ_BUDGET_BYTES = 24.0 * 1024 * 1024 * 1024;
 
 int check_params(const nt_params* p)
 {
@@ -907,9 +1112,15 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const long nll = (long)nl * nl, nrr = (long)nr * nr, bb = (long)b * b;
     const bool same = nl == nr;
     const bool pm = a.ex.gl_plus || a.ex.gr_minus;
-    const int Pmax = std::min(a.npts, c->batch_limit);
+    const bool adjoint = a.adj_w != nullptr;
     const int nmax = std::max(nl, nr);
     const long nmm = (long)nmax * nmax;
+    const long mmr = (long)std::max(nmax, b) * std::max(nmax, b);   // RGF slot size
+    int Pmax = std::min(a.npts, c->batch_limit);
+    if (adjoint) {
+        const double per_pt = (double)adj_slots(a.adj_lo, a.adj_hi, L) * mmr * sizeof(nt_c128);
+        Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
+    }
 
     // ---- plan the workspace for one chunk of Pmax points
     Arena ar;
@@ -957,8 +1168,17 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     o_l32[8] = l32.take(c32 * nll * Pmax);                                   // G_L
     o_l32[9] = l32.take(c32 * nrr * Pmax);                                   // G_R
     const size_t need_dec = lead_gfs_scratch(prm, nmax, 2 * Pmax, 2 * a.nk);
-    const size_t need_rgf = mixed ? l32.off + rgf_scratch<float>(nl, nr, b, Pmax) : rgf_scratch<T>(nl, nr, b, Pmax);
-    ensure_ws(c, o_scr + std::max(need_dec, need_rgf));
+    const bool sweep32 = mixed && !adjoint;   // the adjoint runs the FP64 sweep
+    const size_t need_rgf =
+        sweep32 ? l32.off + rgf_scratch<float>(nl, nr, b, Pmax) : rgf_scratch<T>(nl, nr, b, Pmax);
+    const size_t o_adj = o_scr + std::max(need_dec, need_rgf);
+    size_t need_adj = 0;
+    if (adjoint) {
+        Arena probe;
+        (void)adj_carve(probe, a.adj_lo, a.adj_hi, L, mmr, Pmax, nullptr);
+        need_adj = probe.off;
+    }
+    ensure_ws(c, o_adj + need_adj);
     ar.base = static_cast<char*>(c->ws);
     hipStream_t s = c->stream;
 
@@ -1096,8 +1316,30 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         out.G_D_RL = exw(a.ex.G_D_RL, nr, nl);
         out.G_L = exw(a.ex.G_L, nl, nl);
         out.G_R = exw(a.ex.G_R, nr, nr);
-        if (!mixed) {
-            rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr);
+        if (!sweep32) {
+            AdjSpec adj{};
+            if (adjoint) {
+                Arena aa = ar;
+                aa.off = o_adj;
+                adj = adj_carve(aa, a.adj_lo, a.adj_hi, L, mmr, P, a.adj_w + p0);
+            }
+            rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr, adjoint ? &adj : nullptr);
+            if (adjoint) {
+                const long ne = bb;
+                const int nw = a.adj_hi - a.adj_lo + 1, nc = adj_hc(a.adj_hi, L) - a.adj_lo + 1;
+                const int* fm = a.first_mask ? a.first_mask + p0 : nullptr;
+                check_range(c, a.dHd, sizeof(nt_c128) * ne * nw * a.nk, "dHd");
+                hipLaunchKernelGGL(k_adj_reduce, dim3(nblk(ne), nw), dim3(256), 0, s,
+                                   reinterpret_cast<const double2*>(adj.Zd), mmr, P, ne, kidx, stdec, strgf, fm,
+                                   reinterpret_cast<double2*>(a.dHd), nw);
+                if (nc > 0 && a.dHdv) {
+                    check_range(c, a.dHdv, sizeof(nt_c128) * ne * nc * a.nk, "dHdv");
+                    hipLaunchKernelGGL(k_adj_reduce, dim3(nblk(ne), nc), dim3(256), 0, s,
+                                       reinterpret_cast<const double2*>(adj.Zc), mmr, P, ne, kidx, stdec, strgf, fm,
+                                       reinterpret_cast<double2*>(a.dHdv), nc);
+                }
+                hipcheck(c, hipGetLastError(), "adjoint reduce");
+            }
         } else {
             // Mixed precision: the surface GFs above stay FP64 (decimation); the
             // sweep runs in complex FP32 on MFMA with FP32 accumulation, and the
@@ -1445,6 +1687,67 @@ int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int 
         }
         TdbArgs a{nl, nr, b, L, npts, nk, E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
                   first_point_mask, T_LR, T_RL, status, iters, ex};
+        solve_tdb_dev(c, p, a);
+        return 0;
+    });
+}
+
+int nt_solve_tdb_adjoint_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts,
+                                     const double* E, const int* kidx, int nk, const nt_c128* Haml,
+                                     const nt_c128* Hamlv, const nt_c128* Hamr, const nt_c128* Hamrv,
+                                     const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
+                                     const nt_c128* Hamrd, const int* first_point_mask, const double* weight, int lo,
+                                     int hi, double* T_LR, double* T_RL, int* status, int* iters, nt_c128* dW_dHamd,
+                                     nt_c128* dW_dHamdv)
+{
+    if (!c) return NT_ERR_ARG;
+    if (check_params(p) || check_circular(p, nl, nr)) return NT_ERR_ARG;
+    if (nl <= 0 || nr <= 0 || b <= 0 || L <= 0 || npts < 0 || nk <= 0 || (nk > 1 && !kidx)) return NT_ERR_ARG;
+    if (std::max(std::max(nl, nr), b) > inv_max_n()) return NT_ERR_ARG;
+    if (!E || !Haml || !Hamlv || !Hamr || !Hamrv || !Hamd || (L > 1 && !Hamdv) || !Hamld || !Hamrd || !T_LR || !T_RL)
+        return NT_ERR_ARG;
+    if (!weight || !dW_dHamd || lo < 0 || hi < lo || hi >= L) return NT_ERR_ARG;
+    const int nc = std::min(hi, L - 2) - lo + 1;
+    if (nc > 0 && !dW_dHamdv) return NT_ERR_ARG;
+    if (npts == 0) return NT_OK;
+    return guarded(c, [&] {
+        const size_t z = sizeof(nt_c128);
+        const size_t nll = (size_t)nl * nl, nrr = (size_t)nr * nr, bb = (size_t)b * b;
+        reg_region(c, E, sizeof(double) * npts);
+        reg_region(c, kidx, sizeof(int) * npts);
+        reg_region(c, Haml, z * nll * nk);
+        reg_region(c, Hamlv, z * nll * nk);
+        reg_region(c, Hamr, z * nrr * nk);
+        reg_region(c, Hamrv, z * nrr * nk);
+        reg_region(c, Hamd, z * bb * L * nk);
+        reg_region(c, Hamdv, z * bb * std::max(L - 1, 1) * nk);
+        reg_region(c, Hamld, z * nl * b * nk);
+        reg_region(c, Hamrd, z * nr * b * nk);
+        reg_region(c, first_point_mask, sizeof(int) * npts);
+        reg_region(c, weight, sizeof(double) * npts);
+        reg_region(c, T_LR, sizeof(double) * npts);
+        reg_region(c, T_RL, sizeof(double) * npts);
+        reg_region(c, status, sizeof(int) * npts);
+        reg_region(c, iters, sizeof(int) * npts);
+        reg_region(c, dW_dHamd, z * bb * (hi - lo + 1) * nk);
+        if (nc > 0) reg_region(c, dW_dHamdv, z * bb * nc * nk);
+        if (kidx) {
+            std::vector<int> hk(npts);
+            hipcheck(c, hipMemcpyAsync(hk.data(), kidx, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream), "D2H");
+            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+            for (int v : hk)
+                if (v < 0 || v >= nk) {
+                    c->err = "kidx out of range [0, nk)";
+                    return (int)NT_ERR_ARG;
+                }
+        }
+        TdbArgs a{nl, nr, b, L, npts, nk, E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
+                  first_point_mask, T_LR, T_RL, status, iters, nt_solve_extras{}};
+        a.adj_w = weight;
+        a.adj_lo = lo;
+        a.adj_hi = hi;
+        a.dHd = dW_dHamd;
+        a.dHdv = nc > 0 ? dW_dHamdv : nullptr;
         solve_tdb_dev(c, p, a);
         return 0;
     });

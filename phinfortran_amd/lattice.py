@@ -143,8 +143,25 @@ def _phase(R, kfrac):
     return np.exp(2j * np.pi * (R[0] * kfrac[0] + R[1] * kfrac[1]))
 
 
+def bond_owner(layer: DiamondLayer, L: int, l: int, bi: int):
+    """(owning device layer, owning bond index) of directed bond bi of layer l
+    (1-based layers), or None for a bond touching a lead layer.  A bond
+    between two device layers is owned by the lower layer (its dl=+1 entry),
+    an in-layer pair by the smaller of its two directed indices."""
+    dl = layer.bonds[bi][2]
+    l2 = l + dl
+    if min(l, l2) <= 0 or max(l, l2) >= L + 1:
+        return None
+    mb = _mirror(layer, bi)
+    if dl > 0:
+        return (l, bi)
+    if dl < 0:
+        return (l2, mb)
+    return (l, min(bi, mb))
+
+
 def phonon_blocks(layer: DiamondLayer, masses, springs, lead_mass_l, lead_mass_r,
-                  kfrac=(0.0, 0.0), alpha=48.5, beta=13.8):
+                  kfrac=(0.0, 0.0), alpha=48.5, beta=13.8, beta_bond=None):
     """Dynamical-matrix tdb blocks (THz^2) for a device of L layers.
 
     masses   (L, natom) device masses in amu
@@ -155,6 +172,8 @@ def phonon_blocks(layer: DiamondLayer, masses, springs, lead_mass_l, lead_mass_r
              Bonds touching a lead layer keep scale 1, so the lead layers 0 and
              N+1 are exactly the pristine lead principal layers.
     kfrac    transverse k in units of the supercell reciprocal vectors
+    beta_bond (L, nbond) optional per-bond angular constant (N/m) with the
+             same ownership rule, replacing beta for device bonds
     """
     masses = np.asarray(masses, float)
     springs = np.asarray(springs, float)
@@ -162,33 +181,30 @@ def phonon_blocks(layer: DiamondLayer, masses, springs, lead_mass_l, lead_mass_r
     n, nd = layer.natom, layer.ndof
     ml = np.broadcast_to(np.asarray(lead_mass_l, float), (n,))
     mr = np.broadcast_to(np.asarray(lead_mass_r, float), (n,))
-    mir = [_mirror(layer, bi) for bi in range(layer.nbond)]
     phis = [layer.phi(e, alpha, beta) for (_, _, _, _, _, e) in layer.bonds]
     phases = [_phase((Rx, Ry), kfrac) for (_, _, _, Rx, Ry, _) in layer.bonds]
+    beta_bond = None if beta_bond is None else np.asarray(beta_bond, float)
 
     def mass(l):
         return ml if l <= 0 else (mr if l >= L + 1 else masses[l - 1])
 
-    def scale(l, bi):
-        dl = layer.bonds[bi][2]
-        l2 = l + dl
-        if min(l, l2) <= 0 or max(l, l2) >= L + 1:
-            return 1.0
-        if dl > 0:
-            return springs[l - 1, bi]
-        if dl < 0:
-            return springs[l2 - 1, mir[bi]]
-        return springs[l - 1, min(bi, mir[bi])]
+    def phi_s(l, bi):
+        own = bond_owner(layer, L, l, bi)
+        if own is None:
+            return phis[bi]
+        lo, bo = own
+        ph = phis[bi] if beta_bond is None else layer.phi(layer.bonds[bi][5], alpha, beta_bond[lo - 1, bo])
+        return ph * springs[lo - 1, bo]
 
     def D(l, l2):
         out = np.zeros((nd, nd), np.complex128)
         dl = l2 - l
         for bi, (i, j, d, _, _, _) in enumerate(layer.bonds):
-            s = scale(l, bi)
+            ph = phi_s(l, bi)
             if l == l2:
-                out[3 * i:3 * i + 3, 3 * i:3 * i + 3] -= phis[bi] * s
+                out[3 * i:3 * i + 3, 3 * i:3 * i + 3] -= ph
             if d == dl:
-                out[3 * i:3 * i + 3, 3 * j:3 * j + 3] += phis[bi] * s * phases[bi]
+                out[3 * i:3 * i + 3, 3 * j:3 * j + 3] += ph * phases[bi]
         w1 = 1.0 / np.sqrt(np.repeat(mass(l), 3))
         w2 = 1.0 / np.sqrt(np.repeat(mass(l2), 3))
         return out * w1[:, None] * w2[None, :] * THZ2_PER_N_M_AMU

@@ -236,6 +236,51 @@ class Engine:
             out[name] = t if t.dim() == 1 else t.transpose(-1, -2)
         return out
 
+    def solve_tdb_adjoint(self, dblocks, E, p: N.NtParams, weight, lo: int, hi: int, kidx=None,
+                          first_point_mask=None, stream_sync=True):
+        """Batched solve plus the gradient of W = sum_i weight[i] T_LR[i] with
+        respect to the device blocks of layers lo..hi (0-based).
+
+        Returns dict(T_LR, T_RL, status, iters, dHamd, dHamdv): dHamd is
+        (nk, hi-lo+1, b, b) and dHamdv (nk, nc, b, b) for the couplings
+        l -> l+1, l = lo..min(hi, L-2), both row-major views with torch's
+        complex-gradient convention (dW/dRe + i dW/dIm)."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
+        npts = E.numel()
+        w = torch.as_tensor(weight, dtype=torch.float64, device=dev).contiguous()
+        if w.numel() != npts:
+            raise ValueError("weight must have one entry per point")
+        nl, nr, b, L, nk = dblocks["nl"], dblocks["nr"], dblocks["b"], dblocks["L"], dblocks["nk"]
+        if not (0 <= lo <= hi < L):
+            raise ValueError(f"layer window [{lo}, {hi}] outside [0, {L})")
+        nc = min(hi, L - 2) - lo + 1
+        if kidx is not None:
+            kidx = torch.as_tensor(kidx, dtype=torch.int32, device=dev).contiguous()
+        if first_point_mask is not None:
+            first_point_mask = torch.as_tensor(first_point_mask, dtype=torch.int32, device=dev).contiguous()
+        T_LR = torch.empty(npts, dtype=torch.float64, device=dev)
+        T_RL = torch.empty(npts, dtype=torch.float64, device=dev)
+        status = torch.zeros(npts, dtype=torch.int32, device=dev)
+        iters = torch.zeros(npts, dtype=torch.int32, device=dev)
+        gd = torch.zeros((nk, hi - lo + 1, b, b), dtype=torch.complex128, device=dev)
+        gv = torch.zeros((nk, max(nc, 0), b, b), dtype=torch.complex128, device=dev)
+        if self._torch_stream:
+            self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        st = self._L.nt_solve_tdb_adjoint_batched_dev(
+            self.ctx, C.byref(p), nl, nr, b, L, npts, ptr(E), ptr(kidx), nk,
+            ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
+            ptr(dblocks["Hamd"]), ptr(dblocks["Hamdv"]), ptr(dblocks["Hamld"]), ptr(dblocks["Hamrd"]),
+            ptr(first_point_mask), ptr(w), int(lo), int(hi), ptr(T_LR), ptr(T_RL), ptr(status), ptr(iters),
+            ptr(gd), ptr(gv) if nc > 0 else None)
+        self._check(st, "solve_tdb_adjoint")
+        if stream_sync:
+            torch.cuda.synchronize(dev)
+        return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters,
+                    dHamd=gd.transpose(-1, -2), dHamdv=gv.transpose(-1, -2))
+
     # ---------------------------------------------------------- measurement
     def profile(self, on: bool):
         self._check(self._L.nt_profile_enable(self.ctx, int(on)), "nt_profile_enable")

@@ -46,6 +46,7 @@ def main():
                     help="path reported as value: auto = mixed when it meets dT/T <= 1e-6 and is faster")
     ap.add_argument("--cpu-sample", type=int, default=16, help="solves in the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the (omega,k) points shard across GPUs with no "
@@ -102,6 +103,26 @@ def main():
 
     el64, T64, k64, nbar, ok64 = run("fp64")
     elmx, Tmx, kmx, _, okmx = run("mixed")
+
+    def run_adjoint():
+        """forward + adjoint of the conductance weights over the interface window"""
+        from phinfortran_amd.adjoint import conductance_weights
+        p = T.params("phonon")
+        w = torch.as_tensor(conductance_weights(np.sqrt(omega ** 2), 300.0, (4 * 2) * (lt.A_SI * 1e-10) ** 2),
+                            dtype=torch.float64, device=dev)
+        lo, hi = L // 2 - 1, L // 2 + 1
+        for s in range(args.warmup):
+            eng.solve_tdb_adjoint(dblocks[s], E, p, w, lo, hi)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(args.steps):
+            r = eng.solve_tdb_adjoint(dblocks[args.warmup + s], E, p, w, lo, hi, stream_sync=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        log(f"adjoint: {args.steps * args.points / el:.1f} (omega,k)-gradients/s")
+        return el, bool((r["status"] == 0).all())
+
+    eladj, okadj = run_adjoint() if args.adjoint else (None, True)
     t64 = np.concatenate(T64)
     tmx = np.concatenate(Tmx)
     sig = t64 > 1e-3 * max(1e-300, t64.max())
@@ -166,6 +187,11 @@ def main():
                         "solve_tflops": flop_solve * value / 1e12},
         "cpu_baseline": cpu,
     }
+    if eladj is not None:
+        out["adjoint"] = {"grads_per_s": solves / eladj, "unit": "(omega,k)-gradients/s",
+                          "what": "T_LR and dW/dH of the 3 device layers around the interface (bonds owned by "
+                                  "the 2 interfacial layers), W = conductance-weighted sum of T_LR, FP64",
+                          "cost_vs_fp64_forward": eladj / el64, "status_ok": okadj}
     print(json.dumps(out))
     eng.close()
 

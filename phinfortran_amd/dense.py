@@ -1,0 +1,192 @@
+"""Dense-Hamiltonian layout of the reference's non-tdb path.
+
+The binary-force-constant (negf) and tight-binding (tb) drivers hold one dense
+Ham(norb, norb) per (omega, k): n_buffer_l buffer orbitals, two left-lead
+principal layers (nl each), the device (nd), two right-lead layers (nr each)
+and n_buffer_r buffer orbitals.  The device is cut into variable-size layers
+by blockwise_setup (negf/device.f90:11-249), and the blocks the decimation and
+the RGF read are slices of Ham (negf/surface.f90:179-199,
+tb/surface.f90:190-206, negf/device.f90:341-523).  This module restates that
+layer detection and slicing on the host; the slices feed Engine.G_block and
+Engine.surface_green_tdb.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List
+
+import numpy as np
+
+TOL = 1.0e-7     # sparsity threshold of blockwise_setup (negf/device.f90:50,108)
+
+
+def _mx(a):
+    """maxval(abs(.)) with Fortran's value for an empty array (-huge)."""
+    return float(np.max(a)) if a.size else -np.inf
+
+
+def blockwise_setup(Ham, nl: int, nr: int, n_buffer_l: int = 0, n_buffer_r: int = 0, tol: float = TOL) -> List[int]:
+    """layer_list of the device region, following negf/device.f90:11-249 step
+    for step (1-based indices kept in the variable names; slices converted).
+
+    Quirks kept: every improving candidate of the first scan appends a layer,
+    the second scan continues its fixed j range with the updated start, and
+    the last block is merged into its neighbour while it is smaller than the
+    right-lead coupling range maxij3."""
+    A = np.abs(np.asarray(Ham))
+    norb = A.shape[0]
+    nd = norb - n_buffer_l - n_buffer_r - 2 * nl - 2 * nr
+    d0 = n_buffer_l + 2 * nl                       # 0-based first device orbital
+
+    def HD(r1, r2, c1, c2):                        # HD(r1:r2, c1:c2), 1-based inclusive
+        r1, c1 = max(r1, 1), max(c1, 1)
+        if r2 < r1 or c2 < c1:
+            return np.zeros((0,))
+        return A[d0 + r1 - 1:d0 + r2, d0 + c1 - 1:d0 + c2]
+
+    # coupling range of the left lead into the device (:44-55)
+    VLD = A[n_buffer_l + nl:n_buffer_l + 2 * nl, d0:d0 + nd]
+    maxij1 = 0
+    for i in range(nl):
+        for j in range(1, nd):
+            if VLD[i, j - 1] > tol and _mx(VLD[i, j:nd]) < tol and j >= maxij1:
+                maxij1 = j
+    # coupling range of the right lead into the device (:57-70)
+    r0 = norb - n_buffer_r - 2 * nr
+    VRD = A[r0:r0 + nr, r0 - nd:r0]
+    maxij3 = 0
+    for i in range(nr):
+        for j in range(2, nd + 1):
+            if VRD[i, j - 1] > tol and _mx(VRD[i, 0:j - 1]) < tol and nd - j + 1 > maxij3:
+                maxij3 = nd - j + 1
+
+    layers: List[int] = []
+    start, layer1, layer2, cont, start0 = 1, 1, 1, 0, 1
+    lay1 = lay2 = 0
+    while start < nd - 1 and cont < 2 and start >= start0:
+        maxij2 = 0
+        start0 = start
+        if cont == 0:
+            for i in range(maxij1, nd - 1):
+                for j in range(i + 1, nd):
+                    layer1 = i + 1 - start
+                    layer2 = j - i
+                    c0 = start + layer1 + layer2
+                    Vt = HD(start, start + layer1 - 1, c0, nd)
+                    Vl = HD(start, start + layer1 - 1, c0 - 1, c0 - 1)
+                    Vb = HD(start + layer1, start + layer1, c0, nd)
+                    width = nd - start - layer1 - layer2 + 1
+                    if _mx(Vl) > tol and _mx(Vb) > tol and _mx(Vt) < tol and width > maxij2:
+                        maxij2 = width
+                        lay1, lay2 = layer1, layer2
+                        layers.append(lay1)
+            cont += 1
+            start += lay1
+        else:
+            layer1 = lay2
+            cont = 2
+            for j in range(start + layer1, nd):        # bounds fixed at loop entry
+                layer2 = j + 1 - start - layer1
+                c0 = start + layer1 + layer2
+                Vt = HD(start, start + layer1 - 1, c0, nd)
+                Vl = HD(start, start + layer1 - 1, c0 - 1, c0 - 1)
+                width = nd - start - layer1 - layer2 + 1
+                if _mx(Vl) > tol and _mx(Vt) < tol and width > maxij2:
+                    maxij2 = width
+                    lay1, lay2 = layer1, layer2
+                    layers.append(lay1)
+                    cont = 1
+                    start += lay1
+                    if start + layer1 + layer2 == nd + 1:
+                        layers.append(lay1)
+    layers.append(nd - sum(layers))
+    # merge the last block while it is narrower than the right coupling (:199-212)
+    while len(layers) > 1 and layers[-1] < maxij3:
+        last = layers.pop()
+        layers[-1] += last
+    return layers
+
+
+@dataclass
+class DenseSlices:
+    """Blocks of one dense Ham, named as the engine's G_block arguments."""
+    Haml: np.ndarray      # lead H00 used by decimation and the closing layer
+    Hamlv: np.ndarray     # V (H_oi) of the left lead
+    Hamr: np.ndarray
+    Hamrv: np.ndarray
+    Hd: List[np.ndarray]
+    Hdv: List[np.ndarray]        # V_{i,i+1}
+    Hdv_back: List[np.ndarray]   # V_{i+1,i} (dense lower blocks)
+    Hld: np.ndarray              # V_{0,1}   (nl x b_1)
+    Hrd: np.ndarray              # V_{N+1,N} (nr x b_N)
+    Hrd_fwd: np.ndarray          # V_{N,N+1} (b_N x nr)
+
+
+def slices(Ham, layer_list, nl: int, nr: int, n_buffer_l: int = 0, n_buffer_r: int = 0,
+           variant: str = "negf") -> DenseSlices:
+    """Cut Ham into lead and device blocks.
+
+    variant='negf': the lead H00 is the lead layer next to the device
+    (Ham_l_o_o, negf/surface.f90:186-199); variant='tb': the outer layer
+    (Ham_l_o_o := Ham_l_i_i, tb/surface.f90:190-206).  Both use
+    Hamlv = Ham_l_o_i and Hamrv = Ham_r_i_o, i.e. the couplings whose adjoint
+    the decimation uses as alpha for the advanced/retarded surface GFs."""
+    H = np.asarray(Ham)
+    norb = H.shape[0]
+    bl, br = n_buffer_l, norb - n_buffer_r
+    li = slice(bl, bl + nl)                 # outer left layer
+    lo = slice(bl + nl, bl + 2 * nl)        # left layer next to the device
+    ro_ = slice(br - 2 * nr, br - nr)       # right layer next to the device
+    ri = slice(br - nr, br)                 # outer right layer
+    if variant == "negf":
+        Haml, Hamr = H[lo, lo], H[ro_, ro_]
+    elif variant == "tb":
+        Haml, Hamr = H[li, li], H[ri, ri]
+    else:
+        raise ValueError(variant)
+    Hamlv = H[li, lo]                       # Ham_l_o_i
+    Hamrv = H[ri, ro_]                      # Ham_r_i_o
+    d0 = bl + 2 * nl
+    starts = np.concatenate([[0], np.cumsum(layer_list)[:-1]]).astype(int)
+    sl = [slice(d0 + s, d0 + s + n) for s, n in zip(starts, layer_list)]
+    Hd = [H[s, s].copy() for s in sl]
+    Hdv = [H[sl[i], sl[i + 1]].copy() for i in range(len(sl) - 1)]
+    Hdv_back = [H[sl[i + 1], sl[i]].copy() for i in range(len(sl) - 1)]
+    return DenseSlices(Haml=Haml.copy(), Hamlv=Hamlv.copy(), Hamr=Hamr.copy(), Hamrv=Hamrv.copy(),
+                       Hd=Hd, Hdv=Hdv, Hdv_back=Hdv_back, Hld=H[lo, sl[0]].copy(),
+                       Hrd=H[ro_, sl[-1]].copy(), Hrd_fwd=H[sl[-1], ro_].copy())
+
+
+def assemble(blk, n_buffer: int = 0) -> np.ndarray:
+    """Dense Ham of a tdb block set with two periodic lead layers on each side
+    (and optional buffer layers of the lead): the layout the dense drivers
+    read.  Inverse of slices() for a Hermitian system."""
+    nl, nr, b, L = blk.nl, blk.nr, blk.b, blk.L
+    nb = n_buffer * nl                      # buffer layers of the left lead
+    n = nb + 2 * nl + L * b + 2 * nr
+    H = np.zeros((n, n), np.complex128)
+    # left: buffer layers, outer layer, inner layer; each coupled by V = Hamlv
+    nlay_l = n_buffer + 2
+    for m in range(nlay_l):
+        s = m * nl
+        H[s:s + nl, s:s + nl] = blk.Haml
+        if m + 1 < nlay_l:
+            H[s:s + nl, s + nl:s + 2 * nl] = blk.Hamlv
+            H[s + nl:s + 2 * nl, s:s + nl] = blk.Hamlv.conj().T
+    d0 = nb + 2 * nl
+    for i in range(L):
+        s = d0 + i * b
+        H[s:s + b, s:s + b] = blk.Hamd[i]
+        if i + 1 < L:
+            H[s:s + b, s + b:s + 2 * b] = blk.Hamdv[i]
+            H[s + b:s + 2 * b, s:s + b] = blk.Hamdv[i].conj().T
+    H[d0 - nl:d0, d0:d0 + b] = blk.Hamld
+    H[d0:d0 + b, d0 - nl:d0] = blk.Hamld.conj().T
+    r0 = d0 + L * b
+    H[r0:r0 + nr, r0:r0 + nr] = blk.Hamr
+    H[r0 + nr:r0 + 2 * nr, r0 + nr:r0 + 2 * nr] = blk.Hamr
+    H[r0:r0 + nr, r0 - b:r0] = blk.Hamrd
+    H[r0 - b:r0, r0:r0 + nr] = blk.Hamrd.conj().T
+    H[r0 + nr:r0 + 2 * nr, r0:r0 + nr] = blk.Hamrv
+    H[r0:r0 + nr, r0 + nr:r0 + 2 * nr] = blk.Hamrv.conj().T
+    return H

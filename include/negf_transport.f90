@@ -87,6 +87,35 @@ module negf_transport
             type(c_ptr), value :: G_D_RL, G_D_LR, G_L, G_R
         end function
 
+        ! G_block (negf/device.f90:289-560, tb/device.f90:291-575): layer_list is an
+        ! int array, Hd/Hdv/Hdv_back arrays of c_ptr (c_loc of each block copy)
+        integer(c_int) function nt_G_block(ctx, p, nl, nr, L, layer_list, E, gl, gr, gml, gmr, sl, sr, &
+                Hd, Hdv, Hdv_back, Hld, Hrd, Hrd_fwd, Haml, Hamr, transl, transr, G_D_RL, G_D_LR, G_L, G_R) &
+                bind(C, name="nt_G_block")
+            import :: c_int, c_ptr, c_double, nt_params
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: nl, nr, L
+            type(c_ptr), value :: layer_list
+            real(c_double), value :: E
+            type(c_ptr), value :: gl, gr, gml, gmr, sl, sr, Hd, Hdv, Hdv_back, Hld, Hrd, Hrd_fwd, Haml, Hamr
+            real(c_double), intent(out) :: transl, transr
+            type(c_ptr), value :: G_D_RL, G_D_LR, G_L, G_R
+        end function
+
+        ! get_ldos (tb/device.f90:610-855); ldos(L, maxlen) column-major
+        integer(c_int) function nt_get_ldos(ctx, p, nl, nr, L, layer_list, E, gl, gr, Hd, Hdv, Hdv_back, &
+                Hld, Hrd, Hrd_fwd, maxlen, ldos, ddos) bind(C, name="nt_get_ldos")
+            import :: c_int, c_ptr, c_double, nt_params
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: nl, nr, L, maxlen
+            type(c_ptr), value :: layer_list
+            real(c_double), value :: E
+            type(c_ptr), value :: gl, gr, Hd, Hdv, Hdv_back, Hld, Hrd, Hrd_fwd, ldos
+            real(c_double), intent(out) :: ddos
+        end function
+
         integer(c_int) function nt_solve_tdb_batched(ctx, p, nl, nr, b, L, npts, E, kidx, nk, &
                 Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd, first_point_mask, &
                 T_LR, T_RL, status, iters, extras) bind(C, name="nt_solve_tdb_batched")

@@ -156,6 +156,21 @@ int nt_G_block(nt_ctx* ctx, const nt_params* p, int nl, int nr, int L, const int
                double* transl, double* transr,
                nt_c128* G_D_RL, nt_c128* G_D_LR, nt_c128* G_L, nt_c128* G_R);
 
+/* get_ldos (tb/device.f90:610-855): layer-resolved LDOS of the device.
+ * Arguments as nt_G_block (gl = retarded left surface GF, gr = retarded
+ * right surface GF, layer blocks with optional dense lower blocks and
+ * optional V_{N,N+1}); the device broadening is factor*eta(E) (TB: eta).
+ * Outputs ldos (column-major L x maxlen, maxlen >= max layer size, zero
+ * beyond each layer's size): -Im GLR_i(j,j)/pi with
+ * GLR_i = (I - G^L_i V_{i,i+1} G^R_{i+1} V_{i,i+1}^+)^{-1} G^L_i, and
+ * ddos = sum of ldos (may be NULL).  The reference writes ldos with
+ * write_ldos and returns ddos. */
+int nt_get_ldos(nt_ctx* ctx, const nt_params* p, int nl, int nr, int L, const int* layer_list,
+                double E, const nt_c128* gl, const nt_c128* gr,
+                const nt_c128* const* Hd, const nt_c128* const* Hdv, const nt_c128* const* Hdv_back,
+                const nt_c128* Hld, const nt_c128* Hrd, const nt_c128* Hrd_fwd,
+                int maxlen, double* ldos, double* ddos);
+
 /* ----------------------------------------------- batched (omega,k) solve
  * The hot path: for npts points (E[i], k-point kidx[i]) it runs the lead
  * decimations, the adv->ret conversion, both RGF sweeps and Caroli T, all on

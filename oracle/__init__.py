@@ -77,6 +77,8 @@ def lib():
                                             C.c_int, P, C.POINTER(OrParams), C.c_int, P, P, P]
         _lib.or_circular_g.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, CDouble, C.c_double, C.c_int,
                                        P, C.POINTER(C.c_int)]
+        _lib.or_get_ldos.argtypes = [C.c_int, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_double,
+                                     C.POINTER(OrParams), C.c_int, P, P]
     return _lib
 
 
@@ -226,3 +228,30 @@ def _pack(blocks):
     hdv = np.asfortranarray(np.transpose(Hamdv[: max(L - 1, 0)], (1, 2, 0))) if L > 1 else np.zeros((b, b, 1), np.complex128, order="F")
     return [_f(blocks["Haml"]), _f(blocks["Hamlv"]), _f(blocks["Hamr"]), _f(blocks["Hamrv"]),
             hd, hdv, _f(blocks["Hamld"]), _f(blocks["Hamrd"])]
+
+
+def get_ldos(Hd, Hdv, Hld, Hrd, gl, gr, E, params, Hdv_back=None, Hrd_fwd=None, maxlen=None):
+    """get_ldos (tb/device.f90:610-855): (ldos (L, maxlen), ddos)."""
+    L = len(Hd)
+    Hd = [_f(h) for h in Hd]
+    Hdv = [_f(h) for h in Hdv]
+    bsz = (C.c_int * L)(*[h.shape[0] for h in Hd])
+    maxlen = maxlen or max(h.shape[0] for h in Hd)
+    pHd = (C.c_void_p * L)(*[h.ctypes.data for h in Hd])
+    pHdv = (C.c_void_p * max(L - 1, 1))(*[h.ctypes.data for h in Hdv])
+    pback = None
+    if Hdv_back is not None:
+        Hdv_back = [_f(h) for h in Hdv_back]
+        pback = (C.c_void_p * max(L - 1, 1))(*[h.ctypes.data for h in Hdv_back])
+    Hld, Hrd, gl, gr = map(_f, (Hld, Hrd, gl, gr))
+    Hrd_fwd = None if Hrd_fwd is None else _f(Hrd_fwd)
+    nl, nr = gl.shape[0], gr.shape[0]
+    ldos = np.zeros((L, maxlen), order="F")
+    ddos = C.c_double(0)
+    st = lib().or_get_ldos(L, C.cast(bsz, C.c_void_p), C.cast(pHd, C.c_void_p), C.cast(pHdv, C.c_void_p),
+                           None if pback is None else C.cast(pback, C.c_void_p), nl, nr,
+                           _p(Hld), _p(Hrd), _p(Hrd_fwd), _p(gl), _p(gr), float(E), C.byref(params),
+                           int(maxlen), ldos.ctypes.data_as(C.c_void_p), C.cast(C.byref(ddos), C.c_void_p))
+    if st:
+        raise OracleError(st, "or_get_ldos")
+    return np.ascontiguousarray(ldos), ddos.value

@@ -584,3 +584,92 @@ int or_circular_g(int n, int nby, int nbx, const ocplx* H00, const ocplx* alpha0
     const double twopi = 6.283185307179586476925286766559;
     return circular_g_theta(n, nby, nbx, twopi, twopi, H00, alpha0, z, conv, max_iter, g, iters);
 }
+
+/* ------------------------------------------------------------- layer LDOS */
+#define OR_PI 3.14159265358979323846
+/* get_ldos (tb/device.f90:610-855): left-connected G^L_i (L->R from gl),
+ * right-connected G^R_i (R->L from gr), then per layer
+ *   GLR_i = (I - G^L_i V_{i,i+1} G^R_{i+1} V_{i,i+1}^+)^{-1} G^L_i
+ * with the right lead closing the last layer (V = Hrd_fwd or Hrd^+, G^R = gr),
+ * ldos(i, j) = -Im GLR_i(j, j) / pi (zero beyond the layer size), ddos = sum.
+ * The R->L sweep couples layer i to i-1 with the stored lower block
+ * (Hdv_back) when given (:768-773), the GLR step with V_{i,i+1}^+ (:813-821). */
+int or_get_ldos(int L, const int* bsz, const ocplx* const* Hd, const ocplx* const* Hdv,
+                const ocplx* const* Hdv_back, int nl, int nr,
+                const ocplx* Hld, const ocplx* Hrd, const ocplx* Hrd_fwd,
+                const ocplx* gl, const ocplx* gr, double E, const or_params* p,
+                int maxlen, double* ldos, double* ddos)
+{
+    int bmax = nl > nr ? nl : nr;
+    for (int i = 0; i < L; ++i) if (bsz[i] > bmax) bmax = bsz[i];
+    size_t big = (size_t)bmax * bmax;
+    ocplx** GL = (ocplx**)malloc(sizeof(ocplx*) * L);
+    for (int i = 0; i < L; ++i) GL[i] = cmalloc(big);
+    ocplx *GR = cmalloc(big), *GRn = cmalloc(big), *A = cmalloc(big), *T1 = cmalloc(big), *M = cmalloc(big);
+    ocplx *Vt = cmalloc(big), *X = cmalloc(big);
+    int st = 0;
+    ocplx zd = MKC(E, p->device_eta_factor * or_lead_eta(p, E));
+    for (size_t k = 0; k < (size_t)L * maxlen; ++k) ldos[k] = 0.0;
+    /* L -> R */
+    mm('C', 'N', bsz[0], nl, nl, Hld, gl, T1);
+    mm('N', 'N', bsz[0], bsz[0], nl, T1, Hld, M);
+    dyson_lhs(bsz[0], zd, Hd[0], NULL, M, A);
+    if ((st = or_inv(bsz[0], A, GL[0]))) goto out;
+    for (int i = 1; i < L; ++i) {
+        int bp = bsz[i - 1], bi = bsz[i];
+        mm('C', 'N', bi, bp, bp, Hdv[i - 1], GL[i - 1], T1);
+        mm('N', 'N', bi, bi, bp, T1, Hdv[i - 1], M);
+        dyson_lhs(bi, zd, Hd[i], NULL, M, A);
+        if ((st = or_inv(bi, A, GL[i]))) goto out;
+    }
+    /* last layer: V = Vc (bN x nr), partner gr */
+    {
+        int bN = bsz[L - 1];
+        if (Hrd_fwd) memcpy(Vt, Hrd_fwd, sizeof(ocplx) * bN * nr);
+        else dagger(nr, bN, Hrd, Vt);
+        /* R->L start: G^R_N = (z - H_N - Hrd^+ gr Hrd)^-1, Hrd = V_{N+1,N} */
+        mm('C', 'N', bN, nr, nr, Hrd, gr, T1);
+        mm('N', 'N', bN, bN, nr, T1, Hrd, M);
+        dyson_lhs(bN, zd, Hd[L - 1], NULL, M, A);
+        if ((st = or_inv(bN, A, GR))) goto out;
+        /* GLR_N with the lead: I - G^L_N Vc gr Vc^+ */
+        mm('N', 'N', bN, nr, bN, GL[L - 1], Vt, T1);
+        mm('N', 'N', bN, nr, nr, T1, gr, M);
+        mm('N', 'C', bN, bN, nr, M, Vt, A);
+        for (size_t k = 0; k < (size_t)bN * bN; ++k) A[k] = -A[k];
+        for (int j = 0; j < bN; ++j) A[j + (size_t)j * bN] += 1.0;
+        if ((st = or_inv(bN, A, M))) goto out;
+        mm('N', 'N', bN, bN, bN, M, GL[L - 1], X);
+        for (int j = 0; j < bN && j < maxlen; ++j) ldos[(L - 1) + (size_t)j * L] = -IM(X[j + (size_t)j * bN]) / OR_PI;
+    }
+    for (int i = L - 1; i >= 1; --i) {
+        int bi = bsz[i], bm = bsz[i - 1];
+        /* V_{i,i-1}: stored lower block or Hdv^+ ; G^R_{i-1} */
+        if (Hdv_back && Hdv_back[i - 1]) memcpy(Vt, Hdv_back[i - 1], sizeof(ocplx) * bi * bm);
+        else dagger(bm, bi, Hdv[i - 1], Vt);
+        /* GLR_{i-1} uses G^R_i (current GR) and V_{i-1,i} = Hdv[i-1] */
+        mm('N', 'N', bm, bi, bm, GL[i - 1], Hdv[i - 1], T1);
+        mm('N', 'N', bm, bi, bi, T1, GR, M);
+        mm('N', 'C', bm, bm, bi, M, Hdv[i - 1], A);
+        for (size_t k = 0; k < (size_t)bm * bm; ++k) A[k] = -A[k];
+        for (int j = 0; j < bm; ++j) A[j + (size_t)j * bm] += 1.0;
+        if ((st = or_inv(bm, A, M))) goto out;
+        mm('N', 'N', bm, bm, bm, M, GL[i - 1], X);
+        for (int j = 0; j < bm && j < maxlen; ++j) ldos[(i - 1) + (size_t)j * L] = -IM(X[j + (size_t)j * bm]) / OR_PI;
+        /* G^R_{i-1} = (z - H_{i-1} - V^+ G^R_i V), V = V_{i,i-1} */
+        mm('C', 'N', bm, bi, bi, Vt, GR, T1);
+        mm('N', 'N', bm, bm, bi, T1, Vt, M);
+        dyson_lhs(bm, zd, Hd[i - 1], NULL, M, A);
+        if ((st = or_inv(bm, A, GRn))) goto out;
+        ocplx* t = GR; GR = GRn; GRn = t;
+    }
+    if (ddos) {
+        double acc = 0.0;
+        for (size_t k = 0; k < (size_t)L * maxlen; ++k) acc += ldos[k];
+        *ddos = acc;
+    }
+out:
+    for (int i = 0; i < L; ++i) free(GL[i]);
+    free(GL); free(GR); free(GRn); free(A); free(T1); free(M); free(Vt); free(X);
+    return st;
+}

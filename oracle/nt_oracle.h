@@ -127,6 +127,15 @@ int or_circular_g(int n, int nby, int nbx, const ocplx* H00, const ocplx* alpha0
                   const ocplx* beta0, ocplx z, double conv, int max_iter,
                   ocplx* g, int* iters);
 
+/* get_ldos (tb/device.f90:610-855): per-layer LDOS -Im GLR_i(j,j)/pi of the
+ * device at z = E + i*factor*eta(E), ldos column-major (L x maxlen), ddos the
+ * sum.  Pinned against the diagonal of the dense device Green's function. */
+int or_get_ldos(int L, const int* bsz, const ocplx* const* Hd, const ocplx* const* Hdv,
+                const ocplx* const* Hdv_back, int nl, int nr,
+                const ocplx* Hld, const ocplx* Hrd, const ocplx* Hrd_fwd,
+                const ocplx* gl, const ocplx* gr, double E, const or_params* p,
+                int maxlen, double* ldos, double* ddos);
+
 #ifdef __cplusplus
 }
 #endif

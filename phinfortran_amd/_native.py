@@ -32,7 +32,7 @@ EXPORTS = [
     "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
-    "nt_solve_tdb_adjoint_batched_dev",
+    "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev",
 ]
@@ -108,6 +108,7 @@ def lib():
     L.nt_solve_tdb_batched_dev.argtypes = solve_args
     L.nt_solve_tdb_batched.argtypes = solve_args
     L.nt_solve_tdb_adjoint_batched_dev.argtypes = solve_args[:20] + [P, I, I, P, P, P, P, P, P]
+    L.nt_get_ldos.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, I, P, P]
     L.nt_profile_enable.argtypes = [P, I]
     L.nt_profile_get.argtypes = [P, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D)]
     L.nt_profile_reset.argtypes = [P]

@@ -1927,6 +1927,147 @@ int nt_surface_green_tdb(nt_ctx* c, const nt_params* p, int nl, int nr, double E
     });
 }
 
+// layer LDOS of get_ldos (tb/device.f90:610-855):
+//   G^L_i left-connected from gl, G^R_i right-connected from gr,
+//   GLR_i = (I - G^L_i V_{i,i+1} G^R_{i+1} V_{i,i+1}^+)^{-1} G^L_i   (:800-852)
+// with the right lead (Vc, gr) closing the last layer; ldos(i,j) = -Im GLR_i(j,j)/pi.
+__global__ void k_ldos_diag(const double2* X, int n, int layer, int L, int maxlen, double* ldos)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= maxlen) return;
+    const double v = j < n ? -X[j + (long)j * n].y / 3.14159265358979323846 : 0.0;
+    ldos[layer + (long)j * L] = v;
+}
+
+static int ldos_host(nt_ctx* c, const nt_params* p, int nl, int nr, int L, const int* bsz, double E,
+                     const nt_c128* gl, const nt_c128* gr, const nt_c128* const* Hd, const nt_c128* const* Hdv,
+                     const nt_c128* const* Hdv_back, const nt_c128* Hld, const nt_c128* Hrd, const nt_c128* Hrd_fwd,
+                     int maxlen, double* ldos, double* ddos)
+{
+    typedef double T;
+    const size_t z = sizeof(nt_c128);
+    std::vector<DevBuf*> owned;
+    auto upm = [&](const nt_c128* h, long elems) -> const T* {
+        if (!h) return nullptr;
+        DevBuf* d = new DevBuf(c, z * elems);
+        owned.push_back(d);
+        up(c, *d, h, z * elems);
+        return d->as<T>();
+    };
+    auto buf = [&](long elems) -> T* {
+        DevBuf* d = new DevBuf(c, z * elems);
+        owned.push_back(d);
+        return d->as<T>();
+    };
+    int rc = 0;
+    try {
+        int bmax = std::max(nl, nr);
+        for (int i = 0; i < L; ++i) bmax = std::max(bmax, bsz[i]);
+        const long mm = (long)bmax * bmax;
+        std::vector<BMat<T>> hd, hv, hb;
+        for (int i = 0; i < L; ++i) hd.push_back(bm<T>(upm(Hd[i], (long)bsz[i] * bsz[i]), 0, bsz[i]));
+        for (int i = 0; i + 1 < L; ++i) {
+            hv.push_back(bm<T>(upm(Hdv[i], (long)bsz[i] * bsz[i + 1]), 0, bsz[i]));
+            const nt_c128* b = Hdv_back ? Hdv_back[i] : nullptr;
+            hb.push_back(b ? bm<T>(upm(b, (long)bsz[i] * bsz[i + 1]), 0, bsz[i + 1]) : none<T>());
+        }
+        const BMat<T> hld = bm<T>(upm(Hld, (long)nl * bsz[0]), 0, nl);
+        const BMat<T> hrd = bm<T>(upm(Hrd, (long)nr * bsz[L - 1]), 0, nr);
+        const BMat<T> hfw = Hrd_fwd ? bm<T>(upm(Hrd_fwd, (long)nr * bsz[L - 1]), 0, bsz[L - 1]) : none<T>();
+        const BMat<T> dgl = bm<T>(upm(gl, (long)nl * nl), 0, nl);
+        const BMat<T> dgr = bm<T>(upm(gr, (long)nr * nr), 0, nr);
+        T* GL = buf(mm * L);
+        T* GR = buf(mm);
+        T* GRn = buf(mm);
+        T* T1 = buf(mm);
+        T* M = buf(mm);
+        T* A = buf(mm);
+        T* X = buf(mm);
+        DevBuf dz(c, sizeof(double) * 4), dst(c, sizeof(int)), dl(c, sizeof(double) * (size_t)L * maxlen);
+        const double et = p->eta_mode == NT_ETA_RELATIVE ? E * p->eta + p->eta0 : p->eta;
+        const double zh[4] = {E, p->device_eta_factor * et, 1.0, 0.0};
+        up(c, dz, zh, sizeof(zh));
+        const T* zd = dz.as<T>();
+        const T* one = dz.as<T>() + 2;
+        int* st = dst.as<int>();
+        hipcheck(c, hipMemsetAsync(st, 0, sizeof(int), c->stream), "memset");
+        auto W = [&](T* q, int ld) { return bw(q, 0, ld); };
+        auto R = [&](T* q, int ld) { return bm<T>(q, 0, ld); };
+        auto GLi = [&](int i) { return GL + 2 * (long)i * mm; };
+        auto put = [&](int layer, int n) {
+            hipLaunchKernelGGL(k_ldos_diag, dim3(nblk(maxlen)), dim3(256), 0, c->stream,
+                               reinterpret_cast<const double2*>(X), n, layer, L, maxlen, dl.as<double>());
+            hipcheck(c, hipGetLastError(), "ldos diag");
+        };
+        // left to right
+        {
+            const int b = bsz[0];
+            gemm<T>(c, b, nl, nl, OP_C, hld, OP_N, dgl, W(T1, b), 1.0, none<T>(), 0.0, nullptr, 1);
+            gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, hld, W(A, b), -1.0, hd[0], -1.0, zd, 1);
+            inverse<T>(c, b, W(A, b), none<T>(), nullptr, W(GLi(0), b), st, 1);
+            for (int i = 1; i < L; ++i) {
+                const int bp = bsz[i - 1], bi = bsz[i];
+                gemm<T>(c, bi, bp, bp, OP_C, hv[i - 1], OP_N, R(GLi(i - 1), bp), W(T1, bi), 1.0, none<T>(), 0.0,
+                        nullptr, 1);
+                gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, hv[i - 1], W(A, bi), -1.0, hd[i], -1.0, zd, 1);
+                inverse<T>(c, bi, W(A, bi), none<T>(), nullptr, W(GLi(i), bi), st, 1);
+            }
+        }
+        // last layer: GLR_N with the right lead, Vc = Hrd_fwd or Hrd^+ ; R->L start G^R_N
+        {
+            const int bN = bsz[L - 1];
+            const bool fwd = hfw.base != nullptr;
+            const BMat<T> Vc = fwd ? hfw : hrd;
+            const int opVc = fwd ? OP_N : OP_C, opVcH = fwd ? OP_C : OP_N;
+            gemm<T>(c, bN, nr, bN, OP_N, R(GLi(L - 1), bN), opVc, Vc, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, 1);
+            gemm<T>(c, bN, nr, nr, OP_N, R(T1, bN), OP_N, dgr, W(M, bN), 1.0, none<T>(), 0.0, nullptr, 1);
+            gemm<T>(c, bN, bN, nr, OP_N, R(M, bN), opVcH, Vc, W(A, bN), -1.0, none<T>(), 0.0, one, 1);
+            inverse<T>(c, bN, W(A, bN), none<T>(), nullptr, W(M, bN), st, 1);
+            gemm<T>(c, bN, bN, bN, OP_N, R(M, bN), OP_N, R(GLi(L - 1), bN), W(X, bN), 1.0, none<T>(), 0.0, nullptr,
+                    1);
+            put(L - 1, bN);
+            gemm<T>(c, bN, nr, nr, OP_C, hrd, OP_N, dgr, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, 1);
+            gemm<T>(c, bN, bN, nr, OP_N, R(T1, bN), OP_N, hrd, W(A, bN), -1.0, hd[L - 1], -1.0, zd, 1);
+            inverse<T>(c, bN, W(A, bN), none<T>(), nullptr, W(GR, bN), st, 1);
+        }
+        for (int i = L - 1; i >= 1; --i) {
+            const int bi = bsz[i], bm_ = bsz[i - 1];
+            // GLR_{i-1} = (I - G^L_{i-1} V G^R_i V^+)^{-1} G^L_{i-1}, V = V_{i-1,i}
+            gemm<T>(c, bm_, bi, bm_, OP_N, R(GLi(i - 1), bm_), OP_N, hv[i - 1], W(T1, bm_), 1.0, none<T>(), 0.0,
+                    nullptr, 1);
+            gemm<T>(c, bm_, bi, bi, OP_N, R(T1, bm_), OP_N, R(GR, bi), W(M, bm_), 1.0, none<T>(), 0.0, nullptr, 1);
+            gemm<T>(c, bm_, bm_, bi, OP_N, R(M, bm_), OP_C, hv[i - 1], W(A, bm_), -1.0, none<T>(), 0.0, one, 1);
+            inverse<T>(c, bm_, W(A, bm_), none<T>(), nullptr, W(M, bm_), st, 1);
+            gemm<T>(c, bm_, bm_, bm_, OP_N, R(M, bm_), OP_N, R(GLi(i - 1), bm_), W(X, bm_), 1.0, none<T>(), 0.0,
+                    nullptr, 1);
+            put(i - 1, bm_);
+            // G^R_{i-1} = (z - H_{i-1} - V^+ G^R_i V)^{-1}, V = V_{i,i-1} (lower block or Hdv^+)
+            const bool back = hb[i - 1].base != nullptr;
+            const BMat<T> Vb = back ? hb[i - 1] : hv[i - 1];
+            const int opV = back ? OP_N : OP_C, opVH = back ? OP_C : OP_N;
+            gemm<T>(c, bm_, bi, bi, opVH, Vb, OP_N, R(GR, bi), W(T1, bm_), 1.0, none<T>(), 0.0, nullptr, 1);
+            gemm<T>(c, bm_, bm_, bi, OP_N, R(T1, bm_), opV, Vb, W(A, bm_), -1.0, hd[i - 1], -1.0, zd, 1);
+            inverse<T>(c, bm_, W(A, bm_), none<T>(), nullptr, W(GRn, bm_), st, 1);
+            std::swap(GR, GRn);
+        }
+        int hs = 0;
+        down(c, ldos, dl, sizeof(double) * (size_t)L * maxlen);
+        hipcheck(c, hipMemcpyAsync(&hs, dst.p, sizeof(int), hipMemcpyDeviceToHost, c->stream), "D2H");
+        hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+        if (ddos) {
+            double acc = 0.0;
+            for (size_t k = 0; k < (size_t)L * maxlen; ++k) acc += ldos[k];
+            *ddos = acc;
+        }
+        rc = hs;
+    } catch (...) {
+        for (auto* d : owned) delete d;
+        throw;
+    }
+    for (auto* d : owned) delete d;
+    return rc;
+}
+
 static int rgf_host_common(nt_ctx* c, const nt_params* p, int first_point, int nl, int nr, int L, const int* bsz,
                            double E, const nt_c128* gl, const nt_c128* gr, const nt_c128* gml, const nt_c128* gmr,
                            const nt_c128* sl, const nt_c128* sr, const nt_c128* const* Hd, const nt_c128* const* Hdv,
@@ -2053,6 +2194,25 @@ int nt_G_block(nt_ctx* c, const nt_params* p, int nl, int nr, int L, const int* 
         return rgf_host_common(c, p, 0, nl, nr, L, layer_list, E, gl, gr, gml, gmr, sl, sr, Hd, Hdv, Hdv_back, Hld,
                                Hrd, Hrd_fwd, Haml, Hamr, transl, transr, G_D_RL, G_D_LR, G_L, G_R);
     });
+}
+
+int nt_get_ldos(nt_ctx* c, const nt_params* p, int nl, int nr, int L, const int* layer_list, double E,
+                const nt_c128* gl, const nt_c128* gr, const nt_c128* const* Hd, const nt_c128* const* Hdv,
+                const nt_c128* const* Hdv_back, const nt_c128* Hld, const nt_c128* Hrd, const nt_c128* Hrd_fwd,
+                int maxlen, double* ldos, double* ddos)
+{
+    if (!c) return NT_ERR_ARG;
+    if (check_params(p) || nl <= 0 || nr <= 0 || L <= 0 || !layer_list || !Hd || (L > 1 && !Hdv)) return NT_ERR_ARG;
+    if (!gl || !gr || !Hld || !Hrd || !ldos) return NT_ERR_ARG;
+    int mx = std::max(nl, nr), bl = 0;
+    for (int i = 0; i < L; ++i) {
+        if (layer_list[i] <= 0 || !Hd[i] || (i + 1 < L && !Hdv[i])) return NT_ERR_ARG;
+        mx = std::max(mx, layer_list[i]);
+        bl = std::max(bl, layer_list[i]);
+    }
+    if (mx > inv_max_n() || maxlen < bl) return NT_ERR_ARG;
+    return guarded(c, [&] { return ldos_host(c, p, nl, nr, L, layer_list, E, gl, gr, Hd, Hdv, Hdv_back, Hld, Hrd,
+                                             Hrd_fwd, maxlen, ldos, ddos); });
 }
 
 int nt_zgemm_batched_dev(nt_ctx* c, char opA, char opB, int m, int n, int k, nt_c128 alpha, const nt_c128* A,

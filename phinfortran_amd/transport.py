@@ -168,6 +168,33 @@ class Engine:
                     G_D_LR=np.ascontiguousarray(G_D_LR), G_L=np.ascontiguousarray(G_L),
                     G_R=np.ascontiguousarray(G_R))
 
+    def get_ldos(self, gl, gr, E, Hd: Sequence, Hdv: Sequence, Hld, Hrd, p: N.NtParams,
+                 Hdv_back: Optional[Sequence] = None, Hrd_fwd=None, maxlen: Optional[int] = None):
+        """get_ldos: (ldos (L, maxlen), ddos) of the device layers."""
+        L = len(Hd)
+        Hd = [_f(h) for h in Hd]
+        Hdv = [_f(h) for h in Hdv]
+        maxlen = maxlen or max(h.shape[0] for h in Hd)
+        layer_list = (C.c_int * L)(*[h.shape[0] for h in Hd])
+        pHd = (C.c_void_p * L)(*[h.ctypes.data for h in Hd])
+        pHdv = (C.c_void_p * max(L - 1, 1))(*[h.ctypes.data for h in Hdv])
+        pback = None
+        if Hdv_back is not None:
+            Hdv_back = [_f(h) for h in Hdv_back]
+            pback = (C.c_void_p * max(L - 1, 1))(*[h.ctypes.data for h in Hdv_back])
+        gl, gr, Hld, Hrd = map(_f, (gl, gr, Hld, Hrd))
+        Hrd_fwd = None if Hrd_fwd is None else _f(Hrd_fwd)
+        nl, nr = gl.shape[0], gr.shape[0]
+        ldos = np.zeros((L, maxlen), order="F")
+        ddos = C.c_double()
+        st = self._L.nt_get_ldos(self.ctx, C.byref(p), nl, nr, L, C.cast(layer_list, C.c_void_p), float(E),
+                                 _p(gl), _p(gr), C.cast(pHd, C.c_void_p), C.cast(pHdv, C.c_void_p),
+                                 None if pback is None else C.cast(pback, C.c_void_p), _p(Hld), _p(Hrd),
+                                 _p(Hrd_fwd), int(maxlen), ldos.ctypes.data_as(C.c_void_p),
+                                 C.cast(C.byref(ddos), C.c_void_p))
+        self._check(st, "get_ldos")
+        return np.ascontiguousarray(ldos), ddos.value
+
     # -------------------------------------------------------------- batched
     def upload_blocks(self, blocks_per_k):
         """Stack tdb block sets of nk k-points into device tensors (ABI layout)."""

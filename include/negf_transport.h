@@ -244,6 +244,16 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, in
                                      double* T_LR, double* T_RL, int* status, int* iters,
                                      nt_c128* dW_dHamd, nt_c128* dW_dHamdv);
 
+/* ------------------------------------------------ autotuning
+ * The batched inverse runs blocked Gauss-Jordan with a panel width of 8, 16 or
+ * 32 columns; the widest that fits in LDS is the default.  nt_autotune_inverse
+ * times every supported width on `batch` n x n matrices and keeps the fastest
+ * for this context (precision NT_PREC_FP64: complex double, NT_PREC_MIXED:
+ * complex float); best_nb / best_ms may be NULL.  nt_set_inverse_panel sets a
+ * width directly (0 restores the default).  No reference counterpart. */
+int nt_autotune_inverse(nt_ctx* ctx, int n, int batch, int precision, int* best_nb, double* best_ms);
+int nt_set_inverse_panel(nt_ctx* ctx, int n, int precision, int nb);
+
 /* ------------------------------------------------ batched building blocks
  * The two kernels every solve is made of, on device memory (column-major,
  * leading dimension = rows, item p at base + p*stride elements; stride 0
@@ -261,9 +271,23 @@ int nt_zgemm_batched_dev(nt_ctx* ctx, char opA, char opB, int m, int n, int k, n
 int nt_zinv_batched_dev(nt_ctx* ctx, int n, nt_c128* A, long long strideA, nt_c128* X, long long strideX,
                         int* info, int batch);
 
+/* Eigen-decomposition of general complex matrices, as eigen (negf/util.f90:
+ * 226-262, LAPACK ZGEEV with right eigenvectors): A[p] (n x n, preserved)
+ * -> eigenvalues w[p] (n) and eigenvectors V[p] (columns, unit 2-norm with
+ * the component of largest modulus real and positive).  Hessenberg reduction
+ * + single-shift complex QR to the Schur form, then back substitution.
+ * status[p] (may be NULL): 0, or -3 when the QR iteration did not converge. */
+int nt_zgeev_batched_dev(nt_ctx* ctx, int n, const nt_c128* A, long long strideA, nt_c128* w,
+                         nt_c128* V, long long strideV, int* status, int batch);
+/* Principal square root of general complex matrices as sqrtm (negf/util.f90:
+ * 359-454): complex Schur form A = Z T Z^H, the upper triangular square root
+ * of T (zero where the two diagonal roots sum to exactly zero), S = Z U Z^H. */
+int nt_zsqrtm_batched_dev(nt_ctx* ctx, int n, const nt_c128* A, long long strideA, nt_c128* S,
+                          long long strideS, int* status, int batch);
+
 /* ------------------------------------------------------- measurement */
 /* Kernel classes for nt_profile_get. */
-enum { NT_K_GEMM = 0, NT_K_INV = 1, NT_K_ELEM = 2, NT_K_COUNT = 3 };
+enum { NT_K_GEMM = 0, NT_K_INV = 1, NT_K_ELEM = 2, NT_K_EIG = 3, NT_K_COUNT = 4 };
 /* While enabled, each launch is bracketed by HIP events on the context's
  * stream; nt_profile_get returns launches, summed milliseconds and the summed
  * algorithmic flops (GEMM 8mnk, inverse 8n^3 per active item) of a class. */

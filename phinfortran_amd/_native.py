@@ -26,15 +26,16 @@ NT_PREC_MIXED = 1
 NT_K_GEMM = 0
 NT_K_INV = 1
 NT_K_ELEM = 2
+NT_K_EIG = 3
 
 # every symbol the header declares
 EXPORTS = [
     "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
-    "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos",
+    "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos", "nt_autotune_inverse", "nt_set_inverse_panel",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
-    "nt_zgemm_batched_dev", "nt_zinv_batched_dev",
+    "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
 ]
 
 
@@ -107,8 +108,10 @@ def lib():
     solve_args = [P, pp, I, I, I, I, I, P, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, C.POINTER(NtSolveExtras)]
     L.nt_solve_tdb_batched_dev.argtypes = solve_args
     L.nt_solve_tdb_batched.argtypes = solve_args
-    L.nt_solve_tdb_adjoint_batched_dev.argtypes = solve_args[:20] + [P, I, I, P, P, P, P, P, P]
+    L.nt_solve_tdb_adjoint_batched_dev.argtypes = solve_args[:19] + [P, I, I, P, P, P, P, P, P]
     L.nt_get_ldos.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, I, P, P]
+    L.nt_autotune_inverse.argtypes = [P, I, I, I, C.POINTER(I), C.POINTER(D)]
+    L.nt_set_inverse_panel.argtypes = [P, I, I, I]
     L.nt_profile_enable.argtypes = [P, I]
     L.nt_profile_get.argtypes = [P, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D)]
     L.nt_profile_reset.argtypes = [P]
@@ -118,6 +121,8 @@ def lib():
     L.nt_zgemm_batched_dev.argtypes = [P, C.c_char, C.c_char, I, I, I, NtC128, P, I, LL, P, I, LL, NtC128,
                                        P, I, LL, I]
     L.nt_zinv_batched_dev.argtypes = [P, I, P, LL, P, LL, P, I]
+    L.nt_zgeev_batched_dev.argtypes = [P, I, P, LL, P, P, LL, P, I]
+    L.nt_zsqrtm_batched_dev.argtypes = [P, I, P, LL, P, LL, P, I]
     _lib = L
     return L
 

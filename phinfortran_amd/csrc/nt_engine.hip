@@ -19,6 +19,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/negf_transport.h"
@@ -44,9 +45,9 @@ struct nt_ctx {
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
-    long long launches[NT_K_COUNT] = {0, 0, 0};
-    double ms[NT_K_COUNT] = {0, 0, 0};
-    double flops[NT_K_COUNT] = {0, 0, 0};
+    long long launches[NT_K_COUNT] = {};
+    double ms[NT_K_COUNT] = {};
+    double flops[NT_K_COUNT] = {};
     double last_mean_iters = 0.0;
     // device address ranges a launch may touch: the workspace and the caller's
     // buffers of the current call (sizes derived from the call's arguments)
@@ -57,6 +58,8 @@ struct nt_ctx {
         double* d;
     };
     std::vector<CircU> circ_u;
+    // autotuned inverse panel width per (element bytes, n); absent: size rule
+    std::map<std::pair<int, int>, int> inv_nb;
 };
 
 namespace {
@@ -275,6 +278,8 @@ void inverse(nt_ctx* c, int n, BMatW<T> W, BMat<T> src, const T* z, BMatW<T> X, 
     a.active = active;
     a.info = info;
     a.batch = batch;
+    auto tuned = c->inv_nb.find({(int)sizeof(T), n});
+    a.nb = tuned == c->inv_nb.end() ? 0 : tuned->second;
     const double items = nitems < 0 ? (double)batch : nitems;
     audit<T>(c, W, n, n, batch, 1, "inverse W");
     audit<T>(c, src, n, n, batch, 1, "inverse src");
@@ -1495,6 +1500,148 @@ void down(nt_ctx* c, void* h, const DevBuf& d, size_t bytes)
 }  // namespace
 
 // =================================================================== C ABI
+// ------------------------------------------------------- eigen / sqrtm
+// Scratch: T and Z (and X) per item, n^2 complex each.
+size_t eig_scratch(int n, int batch) { return sizeof(double) * 2 * (size_t)n * n * batch * 3; }
+
+// w, V from A (A is copied, not modified).  The eigenvector matrix is Z X
+// with X the triangular eigenvectors of T.
+void eig_run(nt_ctx* c, int n, BMat<double> A, double* w, BMatW<double> V, int* status, int batch, Arena& ar,
+             size_t off)
+{
+    typedef double T;
+    const long nn = (long)n * n;
+    Arena sa = ar;
+    sa.off = off;
+    T* Tm = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * batch));
+    T* Zm = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * batch));
+    T* Xm = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * batch));
+    copy_op<T>(c, n, n, A, bw(Tm, nn, n), 0, (T)1, batch);
+    EigArgs e;
+    e.n = n;
+    e.A = bw(Tm, nn, n);
+    e.Z = bw(Zm, nn, n);
+    e.X = bw(Xm, nn, n);
+    e.V = V;
+    e.w = w;
+    e.status = status;
+    e.batch = batch;
+    audit<T>(c, e.A, n, n, batch, 1, "schur T");
+    audit<T>(c, e.Z, n, n, batch, 1, "schur Z");
+    audit<T>(c, e.X, n, n, batch, 1, "trevc X");
+    audit<T>(c, V, n, n, batch, 1, "eig V");
+    check_range(c, w, sizeof(T) * 2 * (size_t)n * batch, "eig w");
+    check_range(c, status, sizeof(int) * batch, "eig status");
+    launch(c, NT_K_EIG, 0.0, "schur", [&] { return launch_schur(e, c->stream); });
+    launch(c, NT_K_EIG, 0.0, "trevc", [&] { return launch_trevc(e, c->stream); });
+    gemm<T>(c, n, n, n, OP_N, ro(e.Z), OP_N, ro(e.X), V, 1.0, none<T>(), 0.0, nullptr, batch);
+    launch(c, NT_K_EIG, 0.0, "ev normalize", [&] { return launch_ev_normalize(e, c->stream); });
+}
+
+// S = sqrtm(A) (A copied)
+void sqrtm_run(nt_ctx* c, int n, BMat<double> A, BMatW<double> S, int* status, int batch, Arena& ar, size_t off)
+{
+    typedef double T;
+    const long nn = (long)n * n;
+    Arena sa = ar;
+    sa.off = off;
+    T* Tm = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * batch));
+    T* Zm = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * batch));
+    T* Xm = sa.ptr<T>(sa.take(sizeof(T) * 2 * nn * batch));
+    copy_op<T>(c, n, n, A, bw(Tm, nn, n), 0, (T)1, batch);
+    EigArgs e;
+    e.n = n;
+    e.A = bw(Tm, nn, n);
+    e.Z = bw(Zm, nn, n);
+    e.X = bw(Xm, nn, n);
+    e.V = bw(Xm, nn, n);
+    e.w = nullptr;
+    e.status = status;
+    e.batch = batch;
+    audit<T>(c, e.A, n, n, batch, 1, "schur T");
+    audit<T>(c, e.Z, n, n, batch, 1, "schur Z");
+    audit<T>(c, S, n, n, batch, 1, "sqrtm S");
+    check_range(c, status, sizeof(int) * batch, "sqrtm status");
+    launch(c, NT_K_EIG, 0.0, "schur", [&] { return launch_schur(e, c->stream); });
+    launch(c, NT_K_EIG, 0.0, "sqrt tri", [&] { return launch_sqrt_tri(e, c->stream); });
+    // S = (Z U) Z^H ; Z U into the T buffer
+    gemm<T>(c, n, n, n, OP_N, ro(e.Z), OP_N, ro(e.X), e.A, 1.0, none<T>(), 0.0, nullptr, batch);
+    gemm<T>(c, n, n, n, OP_N, ro(e.A), OP_C, ro(e.Z), S, 1.0, none<T>(), 0.0, nullptr, batch);
+}
+
+// ------------------------------------------------------------- autotuner
+// Deterministic diagonally dominant test matrices: entry hash in [-0.5, 0.5)
+// plus n on the diagonal.
+template <typename T>
+__global__ void k_fill_tune(T* A, int n, int batch)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)n * n * batch;
+    if (e >= total) return;
+    unsigned long long h = (unsigned long long)e * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    h ^= h >> 31;
+    h *= 0xD6E8FEB86659FD93ull;
+    h ^= h >> 29;
+    const double u = (double)(h & 0xFFFFF) / (double)0x100000 - 0.5;
+    const double v = (double)((h >> 20) & 0xFFFFF) / (double)0x100000 - 0.5;
+    const long m = e % ((long)n * n);
+    const bool diag = (m % n) == (m / n);
+    A[2 * e] = (T)(u + (diag ? n : 0));
+    A[2 * e + 1] = (T)v;
+}
+
+// Times every panel width the inverse supports for n x n matrices in batches
+// of `batch` and records the fastest for this context.
+template <typename T>
+void tune_inverse(nt_ctx* c, int n, int batch, int* best_nb, double* best_ms)
+{
+    const long nn = (long)n * n;
+    DevBuf A(c, sizeof(T) * 2 * nn * batch), W(c, sizeof(T) * 2 * nn * batch), X(c, sizeof(T) * 2 * nn * batch),
+        info(c, sizeof(int) * batch), z(c, sizeof(T) * 2 * batch);
+    hipLaunchKernelGGL(k_fill_tune<T>, dim3(nblk(nn * batch)), dim3(256), 0, c->stream, A.as<T>(), n, batch);
+    hipcheck(c, hipMemsetAsync(z.p, 0, sizeof(T) * 2 * batch, c->stream), "memset");
+    hipcheck(c, hipGetLastError(), "fill tune");
+    reg_region(c, A.p, sizeof(T) * 2 * nn * batch);
+    reg_region(c, W.p, sizeof(T) * 2 * nn * batch);
+    reg_region(c, X.p, sizeof(T) * 2 * nn * batch);
+    reg_region(c, info.p, sizeof(int) * batch);
+    reg_region(c, z.p, sizeof(T) * 2 * batch);
+    int widths[3];
+    const int k = inv_panel_widths<T>(n, widths);
+    hipEvent_t e0, e1;
+    hipcheck(c, hipEventCreate(&e0), "event");
+    hipcheck(c, hipEventCreate(&e1), "event");
+    double best = 1e300;
+    int bnb = widths[0];
+    const bool prof = c->prof;
+    c->prof = false;
+    for (int i = 0; i < k; ++i) {
+        c->inv_nb[{(int)sizeof(T), n}] = widths[i];
+        auto run = [&] {
+            inverse<T>(c, n, bw(W.as<T>(), nn, n), bm<T>(A.as<T>(), nn, n), z.as<T>(), bw(X.as<T>(), nn, n),
+                       info.as<int>(), batch);
+        };
+        run();   // warm-up (first launch configures the kernel)
+        hipcheck(c, hipEventRecord(e0, c->stream), "event");
+        const int reps = 3;
+        for (int r = 0; r < reps; ++r) run();
+        hipcheck(c, hipEventRecord(e1, c->stream), "event");
+        hipcheck(c, hipEventSynchronize(e1), "event sync");
+        float ms = 0.f;
+        hipcheck(c, hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (ms / reps < best) {
+            best = ms / reps;
+            bnb = widths[i];
+        }
+    }
+    c->prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->inv_nb[{(int)sizeof(T), n}] = bnb;
+    if (best_nb) *best_nb = bnb;
+    if (best_ms) *best_ms = best;
+}
+
 extern "C" {
 
 int nt_params_validate(const nt_params* p, int nl, int nr)
@@ -2213,6 +2360,79 @@ int nt_get_ldos(nt_ctx* c, const nt_params* p, int nl, int nr, int L, const int*
     if (mx > inv_max_n() || maxlen < bl) return NT_ERR_ARG;
     return guarded(c, [&] { return ldos_host(c, p, nl, nr, L, layer_list, E, gl, gr, Hd, Hdv, Hdv_back, Hld, Hrd,
                                              Hrd_fwd, maxlen, ldos, ddos); });
+}
+
+int nt_autotune_inverse(nt_ctx* c, int n, int batch, int precision, int* best_nb, double* best_ms)
+{
+    if (!c || n <= 0 || n > inv_max_n() || batch <= 0) return NT_ERR_ARG;
+    if (precision != NT_PREC_FP64 && precision != NT_PREC_MIXED) return NT_ERR_ARG;
+    return guarded(c, [&] {
+        if (precision == NT_PREC_FP64)
+            tune_inverse<double>(c, n, batch, best_nb, best_ms);
+        else
+            tune_inverse<float>(c, n, batch, best_nb, best_ms);
+        return 0;
+    });
+}
+
+int nt_set_inverse_panel(nt_ctx* c, int n, int precision, int nb)
+{
+    if (!c || n <= 0 || n > inv_max_n()) return NT_ERR_ARG;
+    if (precision != NT_PREC_FP64 && precision != NT_PREC_MIXED) return NT_ERR_ARG;
+    const int bytes = precision == NT_PREC_FP64 ? 8 : 4;
+    if (nb == 0) {
+        c->inv_nb.erase({bytes, n});
+        return NT_OK;
+    }
+    int w[3];
+    const int k = precision == NT_PREC_FP64 ? inv_panel_widths<double>(n, w) : inv_panel_widths<float>(n, w);
+    for (int i = 0; i < k; ++i)
+        if (w[i] == nb) {
+            c->inv_nb[{bytes, n}] = nb;
+            return NT_OK;
+        }
+    return NT_ERR_ARG;
+}
+
+int nt_zgeev_batched_dev(nt_ctx* c, int n, const nt_c128* A, long long strideA, nt_c128* w, nt_c128* V,
+                         long long strideV, int* status, int batch)
+{
+    if (!c || n <= 0 || batch < 0 || !A || !w || !V || strideA < (long long)n * n || strideV < (long long)n * n)
+        return NT_ERR_ARG;
+    if (batch == 0) return NT_OK;
+    return guarded(c, [&] {
+        const size_t z = sizeof(nt_c128);
+        reg_region(c, A, z * ((size_t)strideA * (batch - 1) + (size_t)n * n));
+        reg_region(c, V, z * ((size_t)strideV * (batch - 1) + (size_t)n * n));
+        reg_region(c, w, z * (size_t)n * batch);
+        reg_region(c, status, sizeof(int) * batch);
+        ensure_ws(c, eig_scratch(n, batch));
+        Arena ar;
+        ar.base = static_cast<char*>(c->ws);
+        eig_run(c, n, bm<double>(reinterpret_cast<const double*>(A), strideA, n), reinterpret_cast<double*>(w),
+                bw(reinterpret_cast<double*>(V), strideV, n), status, batch, ar, 0);
+        return 0;
+    });
+}
+
+int nt_zsqrtm_batched_dev(nt_ctx* c, int n, const nt_c128* A, long long strideA, nt_c128* S, long long strideS,
+                          int* status, int batch)
+{
+    if (!c || n <= 0 || batch < 0 || !A || !S || strideA < (long long)n * n || strideS < (long long)n * n)
+        return NT_ERR_ARG;
+    if (batch == 0) return NT_OK;
+    return guarded(c, [&] {
+        const size_t z = sizeof(nt_c128);
+        reg_region(c, A, z * ((size_t)strideA * (batch - 1) + (size_t)n * n));
+        reg_region(c, S, z * ((size_t)strideS * (batch - 1) + (size_t)n * n));
+        reg_region(c, status, sizeof(int) * batch);
+        ensure_ws(c, eig_scratch(n, batch));
+        Arena ar;
+        ar.base = static_cast<char*>(c->ws);
+        sqrtm_run(c, n, bm<double>(reinterpret_cast<const double*>(A), strideA, n),
+                  bw(reinterpret_cast<double*>(S), strideS, n), status, batch, ar, 0);
+        return 0;
+    });
 }
 
 int nt_zgemm_batched_dev(nt_ctx* c, char opA, char opB, int m, int n, int k, nt_c128 alpha, const nt_c128* A,

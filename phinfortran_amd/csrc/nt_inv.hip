@@ -397,24 +397,46 @@ hipError_t launch_inv_nb(const InvArgs<T>& a, hipStream_t s)
 
 int inv_max_n() { return 1024; }
 
-template <>
-hipError_t launch_inv<double>(const InvArgs<double>& a, hipStream_t s)
+// Largest n each panel width supports (LDS copy of the panel <= ~130 KB and
+// rows per thread within the kernel's instantiation).
+template <typename T>
+int inv_panel_widths(int n, int* out)
+{
+    const int lim32 = sizeof(T) == 8 ? 192 : 256, lim16 = sizeof(T) == 8 ? 384 : 512;
+    int k = 0;
+    if (n <= lim32) out[k++] = 32;
+    if (n <= lim16) out[k++] = 16;
+    if (n <= inv_max_n()) out[k++] = 8;
+    return k;
+}
+template int inv_panel_widths<double>(int, int*);
+template int inv_panel_widths<float>(int, int*);
+
+template <typename T>
+hipError_t launch_inv_any(const InvArgs<T>& a, hipStream_t s)
 {
     if (a.batch <= 0 || a.n <= 0) return hipSuccess;
     if (a.n > inv_max_n()) return hipErrorInvalidValue;
-    if (a.n <= 192) return launch_inv_nb<double, 32, 1>(a, s);
-    if (a.n <= 384) return launch_inv_nb<double, 16, 2>(a, s);
-    return launch_inv_nb<double, 8, 4>(a, s);
+    int w[3];
+    const int k = inv_panel_widths<T>(a.n, w);
+    int nb = w[0];                       // size rule: widest panel that fits
+    for (int i = 0; i < k; ++i)
+        if (a.nb == w[i]) nb = a.nb;
+    if (nb == 32) return launch_inv_nb<T, 32, 1>(a, s);
+    if (nb == 16) return launch_inv_nb<T, 16, 2>(a, s);
+    return launch_inv_nb<T, 8, 4>(a, s);
+}
+
+template <>
+hipError_t launch_inv<double>(const InvArgs<double>& a, hipStream_t s)
+{
+    return launch_inv_any<double>(a, s);
 }
 
 template <>
 hipError_t launch_inv<float>(const InvArgs<float>& a, hipStream_t s)
 {
-    if (a.batch <= 0 || a.n <= 0) return hipSuccess;
-    if (a.n > inv_max_n()) return hipErrorInvalidValue;
-    if (a.n <= 256) return launch_inv_nb<float, 32, 1>(a, s);
-    if (a.n <= 512) return launch_inv_nb<float, 16, 2>(a, s);
-    return launch_inv_nb<float, 8, 4>(a, s);
+    return launch_inv_any<float>(a, s);
 }
 
 }  // namespace nt

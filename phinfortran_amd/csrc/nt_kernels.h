@@ -39,10 +39,29 @@ template <typename T> struct InvArgs {
     const int* active;
     int* info;
     int batch;
+    int nb = 0;             // panel width (8, 16, 32); 0 or unsupported for n: size rule
 };
 
 template <typename T> hipError_t launch_inv(const InvArgs<T>& a, hipStream_t s);
 int inv_max_n();
+// panel widths launch_inv accepts for an n x n matrix of type T (largest first)
+template <typename T> int inv_panel_widths(int n, int* out);
+
+// Batched complex Schur / eigen / square-root kernels (nt_eig.hip), FP64.
+struct EigArgs {
+    int n;
+    BMatW<double> A;     // in: matrix; out: Schur form T (upper triangular)
+    BMatW<double> Z;     // Schur vectors (A = Z T Z^H)
+    BMatW<double> X;     // trevc: upper triangular eigenvectors of T; sqrt_tri: sqrt(T)
+    BMatW<double> V;     // ev_normalize: eigenvectors normalised in place
+    double* w;           // optional eigenvalues (n complex per item)
+    int* status;         // merged: first error of an item wins (-3: QR did not converge)
+    int batch;
+};
+hipError_t launch_schur(const EigArgs& a, hipStream_t s);
+hipError_t launch_trevc(const EigArgs& a, hipStream_t s);
+hipError_t launch_ev_normalize(const EigArgs& a, hipStream_t s);
+hipError_t launch_sqrt_tri(const EigArgs& a, hipStream_t s);
 
 // Elementwise helpers (nt_elem.hip).
 // out[p] = sign * (dagger ? M^+ : M) for an n x n matrix

@@ -308,6 +308,20 @@ class Engine:
         return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters,
                     dHamd=gd.transpose(-1, -2), dHamdv=gv.transpose(-1, -2))
 
+    # ------------------------------------------------------------ tuning
+    def autotune_inverse(self, n: int, batch: int, precision: str = "fp64"):
+        """Time every inverse panel width for n x n batches; keeps the fastest.
+        Returns (panel width, ms per batched launch)."""
+        prec = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[precision]
+        nb, ms = C.c_int(), C.c_double()
+        self._check(self._L.nt_autotune_inverse(self.ctx, int(n), int(batch), prec, C.byref(nb), C.byref(ms)),
+                    "nt_autotune_inverse")
+        return nb.value, ms.value
+
+    def set_inverse_panel(self, n: int, nb: int, precision: str = "fp64"):
+        prec = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[precision]
+        self._check(self._L.nt_set_inverse_panel(self.ctx, int(n), prec, int(nb)), "nt_set_inverse_panel")
+
     # ---------------------------------------------------------- measurement
     def profile(self, on: bool):
         self._check(self._L.nt_profile_enable(self.ctx, int(on)), "nt_profile_enable")

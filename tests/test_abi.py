@@ -81,3 +81,38 @@ def test_circular_sizes_validated_without_gpu():
     assert L.nt_params_validate(ctypes.byref(p), 12, 12) == N.NT_ERR_ARG
     p.n_bloch_x, p.precision = 1, 7
     assert L.nt_params_validate(ctypes.byref(p), 12, 12) == N.NT_ERR_ARG
+
+
+def header_param_counts():
+    """name -> number of parameters of every function declared in the header"""
+    src = open(HDR).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(nt_\w+)\s*\(([^;]*?)\)\s*;", src, flags=re.M | re.S):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_python_argtypes_match_header():
+    """every ctypes binding declares exactly the header's number of parameters"""
+    from phinfortran_amd import _native
+    L = _native.lib()
+    counts = header_param_counts()
+    assert counts, "no declarations parsed"
+    for name, k in counts.items():
+        f = getattr(L, name)
+        if f.argtypes is not None:
+            assert len(f.argtypes) == k, (name, len(f.argtypes), k)
+
+
+def test_fortran_interfaces_match_header():
+    """every bind(C) interface of the Fortran module has the header's parameter count"""
+    src = open(FMOD).read()
+    counts = header_param_counts()
+    found = list(re.finditer(r"(?:function|subroutine)\s+\w+\s*\(((?:[^()]|&\s*\n)*?)\)\s*(?:&\s*\n\s*)?"
+                             r"bind\(C,\s*name=\"(nt_\w+)\"\)", src, flags=re.S))
+    assert len(found) == len(re.findall(r'bind\(C,\s*name="nt_', src))
+    for m in found:
+        args = [a for a in re.sub(r"&\s*\n", " ", m.group(1)).replace("&", " ").split(",") if a.strip()]
+        assert len(args) == counts[m.group(2)], (m.group(2), len(args), counts[m.group(2)])

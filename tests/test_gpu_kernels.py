@@ -124,3 +124,34 @@ def test_zinv_singular_and_nan(engine, torch_dev):
     inf = info.cpu().numpy()
     assert inf[0] == 8
     assert inf[1] == -1
+
+
+@pytest.mark.parametrize("n", [24, 96, 192, 200, 384, 500])
+def test_zinv_every_panel_width(engine, torch_dev, n):
+    """Each panel width the autotuner may pick inverts to the same accuracy,
+    and the autotuner returns one of them."""
+    from phinfortran_amd import _native as N
+    torch, dev = torch_dev
+    rng = np.random.default_rng(100 + n)
+    batch = 2
+    A = [crand(rng, n, n) + 0.5 * n ** 0.5 * np.eye(n) for _ in range(batch)]
+    A[0][0, 0] = 0.0
+    widths = [w for w, lim in ((32, 192), (16, 384), (8, 1024)) if n <= lim]
+    print(f"zinv panels n={n} widths={widths}", flush=True)
+    for w in widths:
+        engine.set_inverse_panel(n, w)
+        dA = to_dev(torch, dev, A)
+        dX = torch.zeros_like(dA)
+        info = torch.zeros(batch, dtype=torch.int32, device=dev)
+        st = N.lib().nt_zinv_batched_dev(engine.ctx, n, C.c_void_p(dA.data_ptr()), n * n,
+                                         C.c_void_p(dX.data_ptr()), n * n, C.c_void_p(info.data_ptr()), batch)
+        assert st == 0
+        engine.sync()
+        X = from_dev(dX)
+        for i in range(batch):
+            ref = np.linalg.inv(A[i])
+            assert np.abs(X[i] - ref).max() < 1e-14 * np.linalg.cond(A[i]) * np.abs(ref).max() * n, (w, i)
+    engine.set_inverse_panel(n, 0)
+    nb, ms = engine.autotune_inverse(n, 64)
+    assert nb in widths and ms > 0
+    engine.set_inverse_panel(n, 0)

@@ -1,0 +1,143 @@
+"""CPU restatement of the mode-resolved transmission (test infrastructure only).
+
+Follows vel_mat_tdb (negf/velocity_mat.f90:1692-3150) for path_mode != 1 (the
+branch the reference's example runs: Gram-Schmidt on the propagating modes,
+no degenerate-subspace rotation), without the Brillouin-zone unfolding that
+labels modes as specular or not (:2020-2130 and its three mirrors).  The
+LAPACK calls of the reference are the LAPACK calls here: ZGEEV through
+numpy.linalg.eig (eigen, negf/util.f90:226-262) and ZGEES through
+scipy.linalg.schur (sqrtm, negf/util.f90:359-454); the triangular square root,
+Gram-Schmidt and log_fbz are restated line by line.
+
+Pinned by: the si-si example's committed per-mode tables
+(examples/negf/si-si/output/trl_specular_param_*.dat: incident k_z and
+per-mode transmission of every (omega, k)), the closed form of the 1-D chain
+(one mode, |t|^2 = T), sum_n T_n = Caroli T, and unitarity of the scattering
+matrix of pristine leads (tests/test_modes_oracle.py).  Per-mode rows inside
+a degenerate subspace depend on the eigenvector basis ZGEEV returns and are
+compared only through basis-invariant sums.
+"""
+import numpy as np
+import scipy.linalg as sl
+
+NOT_PROP = 100.0     # k_z of an evanescent mode (velocity_mat.f90:1856)
+
+
+def log_fbz(lam, az):
+    """negf/util.f90:825-839"""
+    gz = 2.0 * np.pi / az
+    kz = np.mod(np.real(np.log(lam) / (1j * az)), gz)
+    return kz - gz if kz > 0.5 * gz else kz
+
+
+def gram_schmidt(U1):
+    """negf/util.f90:800-823, including its coefficient
+    dot_product(U1_i, U2_j) / dot_product(U2_j, U2_j) (conjugate on U1_i)."""
+    U2 = np.array(U1, dtype=complex, copy=True)
+    n = U1.shape[1]
+    for i in range(1, n):
+        for j in range(i):
+            U2[:, i] = U2[:, i] - np.vdot(U1[:, i], U2[:, j]) / np.vdot(U2[:, j], U2[:, j]) * U2[:, j]
+    for i in range(n):
+        U2[:, i] = U2[:, i] / np.sqrt(np.vdot(U2[:, i], U2[:, i]))
+    return U2
+
+
+def sqrtm(A):
+    """negf/util.f90:359-454: complex Schur form, triangular square root with
+    U_ij = 0 where U_ii + U_jj is exactly zero, Q U Q^H."""
+    S, Q = sl.schur(np.asarray(A, dtype=complex), output="complex")
+    n = S.shape[0]
+    U = np.zeros_like(S)
+    for i in range(n):
+        U[i, i] = np.sqrt(S[i, i])
+    for sd in range(1, n):
+        for i in range(n - sd):
+            j = i + sd
+            v = S[i, j] - U[i, i + 1:j] @ U[i + 1:j, j]
+            den = U[i, i] + U[j, j]
+            U[i, j] = v / den if den != 0 else 0.0
+    return Q @ U @ Q.conj().T
+
+
+def _modes(F, lambda_eta, az, sign):
+    """eigen + propagating selection + Gram-Schmidt of one lead direction
+    (velocity_mat.f90:1839-1882 and mirrors): (lambda, U, k_z, n_prop)."""
+    lam, U = np.linalg.eig(F)
+    kz = np.full(lam.shape[0], NOT_PROP)
+    prop = [i for i in range(lam.shape[0]) if abs(abs(lam[i]) - 1.0) < lambda_eta]
+    for i in prop:
+        kz[i] = sign * log_fbz(lam[i], az)
+    if prop:
+        UL = gram_schmidt(U[:, prop])
+        U = U.copy()
+        U[:, prop] = UL
+    return lam, U, kz, len(prop)
+
+
+def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R,
+                Haml, Hamr, Hamlv, Hamrv, eta, eta0, lambda_eta, a_z_L, a_z_R, first_point=False):
+    """Mode-resolved transmission and reflection of one (omega, k) point.
+
+    Returns dict with trl (nr x nl, left -> right), tlr, rrr, rll, the four
+    k_z arrays (NOT_PROP for evanescent modes), the propagating counts, the
+    per-mode sums t_mode_L[i] = sum_j |trl(j,i)|^2 and t_mode_R[j]
+    (velocity_mat.f90:3036-3041)."""
+    c = 1j * 0.5 / np.sqrt(E)
+    # left advanced, incident from the left (:1835-2012)
+    lam_al, U_al, kz_al, n_al = _modes(gl_adv @ Hamlv, lambda_eta, a_z_L, -1.0)
+    V_al = -(U_al.conj().T @ gml_adv @ U_al) * c * a_z_L
+    inv_U_al = np.linalg.inv(U_al.conj().T)
+    Vh_al = sqrtm(V_al)
+    # left retarded, leaving to the left (:2131-2303)
+    lam_rl, U_rl, kz_rl, n_rl = _modes(gl_adv.conj().T @ Hamlv, lambda_eta, a_z_L, -1.0)
+    V_rl = -(U_rl.conj().T @ gml_adv.conj().T @ U_rl) * c * a_z_L
+    inv_U_rl = np.linalg.inv(U_rl)
+    Vh_rl = sqrtm(V_rl)
+    # right retarded, leaving to the right (:2394-2576)
+    lam_rr, U_rr, kz_rr, n_rr = _modes(gr_ret @ Hamrv, lambda_eta, a_z_R, 1.0)
+    V_rr = (U_rr.conj().T @ gmr_ret @ U_rr) * c * a_z_R
+    inv_U_rr = np.linalg.inv(U_rr)
+    Vh_rr = sqrtm(V_rr)
+    # right advanced, incident from the right (:2657-2824)
+    lam_ar, U_ar, kz_ar, n_ar = _modes(gr_ret.conj().T @ Hamrv, lambda_eta, a_z_R, 1.0)
+    V_ar = (U_ar.conj().T @ gmr_ret.conj().T @ U_ar) * c * a_z_R
+    inv_U_ar = np.linalg.inv(U_ar.conj().T)
+    Vh_ar = sqrtm(V_ar)
+    # bulk GFs of the leads (:2893-2938), lead eta
+    z = E + 1j * (E * eta + eta0)
+    nl, nr = Haml.shape[0], Hamr.shape[0]
+    Q_L = z * np.eye(nl) - Haml - Hamlv.conj().T @ gl_adv.conj().T @ Hamlv - Hamlv @ gl_plus @ Hamlv.conj().T
+    Q_R = z * np.eye(nr) - Hamr - Hamrv.conj().T @ gr_ret @ Hamrv - Hamrv @ gr_minus.conj().T @ Hamrv.conj().T
+    # t and r (:2945-2963)
+    trl = 1j * (Vh_rr @ (inv_U_rr @ (G_D_RL @ (inv_U_al @ Vh_al)))) * 2.0 * np.sqrt(E / a_z_R / a_z_L)
+    tlr = 1j * (Vh_rl @ (inv_U_rl @ (G_D_LR @ (inv_U_ar @ Vh_ar)))) * 2.0 * np.sqrt(E / a_z_L / a_z_R)
+    rrr = 1j * (Vh_rr @ (inv_U_rr @ ((G_R - np.linalg.inv(Q_R)) @ (inv_U_ar @ Vh_ar)))) * 2.0 * np.sqrt(E / a_z_R ** 2)
+    rll = 1j * (Vh_rl @ (inv_U_rl @ ((G_L - np.linalg.inv(Q_L)) @ (inv_U_al @ Vh_al)))) * 2.0 * np.sqrt(E / a_z_L ** 2)
+    if first_point:                         # :2965-2970
+        trl, tlr, rrr, rll = (np.zeros_like(x) for x in (trl, tlr, rrr, rll))
+    a2 = lambda x: np.abs(x) ** 2
+    return dict(trl=trl, tlr=tlr, rrr=rrr, rll=rll,
+                k_z_L_adv=kz_al, k_z_L_ret=kz_rl, k_z_R_ret=kz_rr, k_z_R_adv=kz_ar,
+                lam_L_adv=lam_al, lam_L_ret=lam_rl, lam_R_ret=lam_rr, lam_R_adv=lam_ar,
+                n_prop=dict(L_adv=n_al, L_ret=n_rl, R_ret=n_rr, R_adv=n_ar),
+                t_mode_L=a2(trl).sum(axis=0), t_mode_R=a2(trl).sum(axis=1),
+                T_modes_RL=float(a2(trl).sum()), T_modes_LR=float(a2(tlr).sum()),
+                R_modes_R=float(a2(rrr).sum()), R_modes_L=float(a2(rll).sum()))
+
+
+def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=False):
+    """Full oracle chain for one point: surface GFs and get_G_full_tdb of the
+    C oracle, then vel_mat_tdb with the driver's argument mapping
+    (negf/phinfortran.f90:1238-1250)."""
+    import oracle as O
+    d = blk.as_dict() if hasattr(blk, "as_dict") else blk
+    sg = O.surface_green(d["Haml"], d["Hamlv"], d["Hamr"], d["Hamrv"], E, params, want_plus_minus=True)
+    r = O.rgf(list(d["Hamd"]), list(d["Hamdv"]), d["Hamld"], d["Hamrd"], d["Haml"], d["Hamr"],
+              sg["gl_adv"].conj().T, sg["gr_ret"], -sg["gml_adv"], sg["gmr_ret"], sg["sl_adv"].conj().T,
+              sg["sr_ret"], E, params)
+    m = vel_mat_tdb(E, sg["gl_adv"], sg["gr_ret"], sg["gl_plus"], sg["gr_minus"], sg["gml_adv"], sg["gmr_ret"],
+                    r["G_D_RL"], r["G_D_LR"], r["G_L"], r["G_R"], d["Haml"], d["Hamr"], d["Hamlv"], d["Hamrv"],
+                    params.eta, params.eta0, lambda_eta, a_z_L, a_z_R, first_point)
+    m["T_LR"], m["T_RL"] = r["T_LR"], r["T_RL"]
+    return m

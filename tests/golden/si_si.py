@@ -45,6 +45,14 @@ def make_fixture():
         shutil.copy(os.path.join(REF_EXAMPLE, f), os.path.join(FIXTURE, f))
     for f in ("transl_tot.dat", "transr_tot.dat"):
         shutil.copy(os.path.join(REF_EXAMPLE, "output", f), os.path.join(FIXTURE, f))
+    # per-mode transmission of left-incident modes at a few k-points
+    for ik in MODE_KPOINTS:
+        f = "trl_specular_param_%06d.dat" % (ik + 1)
+        shutil.copy(os.path.join(REF_EXAMPLE, "output", f), os.path.join(FIXTURE, f))
+
+
+# k-points (0-based) whose per-mode tables are kept as fixtures
+MODE_KPOINTS = (0, 10, 24, 40)
 
 
 def read_config(path=None):
@@ -251,6 +259,17 @@ def golden_T(which="transl"):
     """(ne, nk) table of the committed reference output."""
     tab = np.loadtxt(os.path.join(FIXTURE, f"{which}_tot.dat"))
     return tab[:, 0], tab[:, 1:]
+
+
+def golden_modes(ik):
+    """Per-mode rows of trl_specular_param (negf/write_data.f90:3-75) at
+    k-point ik: dict omega -> array of (k_z of the incident left mode,
+    transmission summed over propagating right modes), rows in file order."""
+    tab = np.loadtxt(os.path.join(FIXTURE, "trl_specular_param_%06d.dat" % (ik + 1)))
+    out = {}
+    for row in np.atleast_2d(tab):
+        out.setdefault(round(row[1], 8), []).append((row[4], row[12]))
+    return {k: np.array(v) for k, v in out.items()}
 
 
 if __name__ == "__main__":

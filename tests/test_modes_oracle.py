@@ -1,0 +1,123 @@
+"""The mode-resolved oracle (oracle/modes.py, vel_mat_tdb) pinned on the CPU:
+the 1-D chain's closed form, sum of per-mode T = Caroli T, unitarity for a
+pristine lead, and the reference's committed per-mode tables of the si-si
+example (k_z and transmission of every incident mode)."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import modes as M
+from phinfortran_amd import lattice as lt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import si_si as S  # noqa: E402
+
+
+def test_log_fbz_and_gram_schmidt():
+    az = 2.0
+    for q in (-1.4, -0.3, 0.2, 1.5):
+        assert abs(M.log_fbz(np.exp(1j * q * az), az) - q) < 1e-14
+    # the reference's coefficient conjugates the projection: for real vectors
+    # it is classical Gram-Schmidt
+    rng = np.random.default_rng(0)
+    U = rng.standard_normal((6, 3)) + 0j
+    Q = M.gram_schmidt(U)
+    assert np.abs(Q.conj().T @ Q - np.eye(3)).max() < 1e-12
+    # ... and for complex vectors it is not orthogonalising (a kept quirk)
+    Uc = U + 1j * rng.standard_normal((6, 3))
+    Qc = M.gram_schmidt(Uc)
+    assert abs(np.vdot(Qc[:, 0], Qc[:, 1])) > 1e-3
+
+
+def test_sqrtm_matches_scipy():
+    import scipy.linalg as sl
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((7, 7)) + 1j * rng.standard_normal((7, 7)) + 3 * np.eye(7)
+    assert np.abs(M.sqrtm(A) - sl.sqrtm(A)).max() < 1e-12
+
+
+@pytest.mark.parametrize("E", [0.3, 1.7, 3.5])
+def test_chain_one_mode(E):
+    """monatomic chain (k = m = 1): one mode each way with k_z = acos(1 - E/2),
+    |t|^2 = T = 1 inside the band, no reflection"""
+    blk = lt.chain_blocks(L=4)
+    p = O.OrParams(1e-9, 1e-10, 1e-14, 10.0, 0, 2000)
+    m = M.solve_point_modes(blk, E, p, 5e-3, 1.0, 1.0)
+    q = math.acos(1 - E / 2)
+    assert m["n_prop"] == dict(L_adv=1, L_ret=1, R_ret=1, R_adv=1)
+    kz = m["k_z_L_adv"][m["k_z_L_adv"] < 99]
+    assert abs(abs(kz[0]) - q) < 1e-9
+    assert abs(abs(m["trl"][0, 0]) ** 2 - 1) < 1e-6 and abs(m["T_modes_RL"] - m["T_RL"]) < 1e-6
+    assert abs(m["rll"][0, 0]) < 1e-3
+
+
+def test_mass_defect_transmission_and_unitarity():
+    """isotope defect: |t|^2 + |r|^2 = 1 and |t|^2 = Caroli T"""
+    blk = lt.chain_blocks(L=1, masses=[1.7])
+    p = O.OrParams(1e-10, 1e-11, 1e-14, 10.0, 0, 4000)
+    m = M.solve_point_modes(blk, 1.5, p, 5e-3, 1.0, 1.0)
+    t2 = abs(m["trl"][0, 0]) ** 2
+    r2 = abs(m["rll"][0, 0]) ** 2
+    assert abs(t2 - m["T_RL"]) < 1e-7
+    assert abs(t2 + r2 - 1) < 1e-6
+
+
+def test_modes_sum_to_caroli_multichannel():
+    """pristine multi-orbital lead (device = lead): every open channel
+    transmits, sum of per-mode T = Caroli T = channel count"""
+    rng = np.random.default_rng(3)
+    n = 6
+    H = rng.standard_normal((n, n)) * 0.3
+    H = H + H.T + 0j
+    V = rng.standard_normal((n, n)) * 0.3 + 0j
+    blk = lt.Blocks(Haml=H, Hamlv=V, Hamr=H.copy(), Hamrv=V.conj().T.copy(), Hamd=np.stack([H] * 3),
+                    Hamdv=np.stack([V] * 2), Hamld=V.copy(), Hamrd=V.conj().T.copy())
+    p = O.OrParams(1e-9, 1e-10, 1e-14, 10.0, 0, 4000)
+    for E in (0.1, 0.5, 1.2):
+        m = M.solve_point_modes(blk, E, p, 5e-3, 1.0, 1.0)
+        npr = m["n_prop"]["L_adv"]
+        assert npr == m["n_prop"]["R_ret"]
+        assert abs(m["T_RL"] - round(m["T_RL"])) < 1e-4
+        assert abs(m["T_modes_RL"] - m["T_RL"]) < 1e-4
+        assert m["R_modes_L"] < 1e-4
+
+
+@pytest.fixture(scope="module")
+def si():
+    cfg = S.read_config()
+    qe = S.read_qe_fc(cfg)
+    return cfg, S.TdbAssembler(qe, cfg), S.k_grid(cfg), S.e_grid(cfg), qe["cell"][2, 2]
+
+
+@pytest.mark.parametrize("ik", list(S.MODE_KPOINTS))
+def test_si_si_per_mode_tables(si, ik):
+    """Every incident propagating mode of the committed trl table: its k_z
+    and its transmission summed over propagating outgoing modes (rows of one
+    omega compared after sorting by k_z)."""
+    cfg, asm, ks, E, az = si
+    gold = S.golden_modes(ik)
+    blk = asm.blocks(ks[ik])
+    p = O.OrParams(cfg["eta"], cfg["eta0"], cfg["convergence"], 10.0, O.ETA_RELATIVE, 500)
+    checked = 0
+    for ie in range(0, len(E), 4):
+        w = math.sqrt(E[ie])
+        rows = gold.get(round(w, 8))
+        m = M.solve_point_modes(blk, E[ie], p, 5e-3, az, az, first_point=(ie == 0))
+        inc = np.where(m["k_z_L_adv"] < 99.9)[0]
+        out = np.where(m["k_z_R_ret"] < 99.9)[0]
+        if rows is None:
+            assert len(inc) == 0
+            continue
+        assert len(inc) == len(rows), (ik, ie)
+        rt = (np.abs(m["trl"][np.ix_(out, inc)]) ** 2).sum(axis=0)
+        order = np.argsort(m["k_z_L_adv"][inc])
+        g = rows[np.argsort(rows[:, 0])]
+        assert np.abs(m["k_z_L_adv"][inc][order] - g[:, 0]).max() < 1e-7, (ik, ie)
+        assert np.abs(rt[order] - g[:, 1]).max() < 2e-6, (ik, ie, rt[order], g[:, 1])
+        checked += 1
+    assert checked > 5

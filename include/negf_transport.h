@@ -244,6 +244,44 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, in
                                      double* T_LR, double* T_RL, int* status, int* iters,
                                      nt_c128* dW_dHamd, nt_c128* dW_dHamdv);
 
+/* ------------------------------------------------ mode-resolved transmission
+ * vel_mat_tdb (negf/velocity_mat.f90:1692-3150; tb/velocity_mat.f90:222-1560)
+ * for npts points, without the Brillouin-zone unfolding that splits specular
+ * and non-specular parts.  Inputs are the outputs of the batched solve for
+ * the same points (nt_solve_extras: gl_adv, gr_ret, gl_plus, gr_minus,
+ * gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R; npts x n x n each), the lead
+ * blocks per k-point (kidx as in the batched solve) and the lead periods
+ * a_z_L, a_z_R.  tb = 0: negf conventions (velocity i a/(2 sqrt E), t prefactor
+ * 2 i sqrt(E/(a a'))); tb = 1: tight-binding conventions (no factors).
+ * Every output may be NULL; per point:
+ *   trl nr x nl (t_RL, left -> right), tlr nl x nr, rrr nr x nr, rll nl x nl;
+ *   k_z of the four mode sets (L_adv, L_ret: nl; R_ret, R_adv: nr), 100 for
+ *   evanescent modes; n_prop (4 ints: L_adv, L_ret, R_ret, R_adv);
+ *   t_mode_L (nl): for each propagating incident left mode, its transmission
+ *   summed over propagating outgoing right modes (the r_t column of the
+ *   reference's trl tables); t_mode_R (nr) the same for t_LR;
+ *   sums (4 doubles): T_RL, T_LR, R_RR, R_LL summed over propagating modes.
+ * Points flagged in first_point_mask get zero t and r (velocity_mat.f90:2965). */
+typedef struct {
+    nt_c128* trl; nt_c128* tlr; nt_c128* rrr; nt_c128* rll;
+    double* k_z_L_adv; double* k_z_L_ret; double* k_z_R_ret; double* k_z_R_adv;
+    int* n_prop;
+    double* t_mode_L; double* t_mode_R;
+    double* sums;
+} nt_modes_out;
+
+int nt_modes_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int tb, int nl, int nr, int npts,
+                             const double* E, const int* kidx, int nk,
+                             const nt_c128* Haml, const nt_c128* Hamlv,
+                             const nt_c128* Hamr, const nt_c128* Hamrv,
+                             const nt_c128* gl_adv, const nt_c128* gr_ret,
+                             const nt_c128* gl_plus, const nt_c128* gr_minus,
+                             const nt_c128* gml_adv, const nt_c128* gmr_ret,
+                             const nt_c128* G_D_RL, const nt_c128* G_D_LR,
+                             const nt_c128* G_L, const nt_c128* G_R,
+                             double a_z_L, double a_z_R, const int* first_point_mask,
+                             const nt_modes_out* out, int* status);
+
 /* ------------------------------------------------ autotuning
  * The batched inverse runs blocked Gauss-Jordan with a panel width of 8, 16 or
  * 32 columns; the widest that fits in LDS is the default.  nt_autotune_inverse

@@ -83,6 +83,15 @@ __device__ double block_sum(double v, double* red)
     return red[0] + red[1] + red[2] + red[3];
 }
 
+// reduction scratch of k_modes_select: after prop[n], nrm2[n], coef[n]
+__device__ __forceinline__ double* coef_red(unsigned char* base, int n)
+{
+    int* prop = reinterpret_cast<int*>(base);
+    double* nrm2 = reinterpret_cast<double*>(prop + ((n + 1) & ~1));
+    C2* coef = reinterpret_cast<C2*>(nrm2 + n);
+    return reinterpret_cast<double*>(coef + n);
+}
+
 struct SchurShared {
     double red[4];
     int ired[4];
@@ -468,6 +477,154 @@ hipError_t launch_sqrt_tri(const EigArgs& a, hipStream_t s)
 {
     if (a.batch <= 0 || a.n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sqrt_tri, dim3(a.batch), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace nt
+
+namespace nt {
+
+// ------------------------------------------------------ mode bookkeeping
+// Propagating selection, k_z and Gram-Schmidt of one lead direction
+// (negf/velocity_mat.f90:1855-1882 and its three mirrors), one workgroup per
+// matrix.  Mode i propagates when ||lambda_i| - 1| < lambda_eta; its
+// k_z = sign * log_fbz(lambda_i, a_z) (negf/util.f90:825-839), others get 100.
+// The propagating columns of U are orthogonalised in index order with the
+// reference's coefficient dot_product(U1_i, U2_j) / dot_product(U2_j, U2_j)
+// (conjugate on the original column U1_i, negf/util.f90:800-823) and
+// normalised.
+__global__ __launch_bounds__(NT) void k_modes_select(ModesArgs a)
+{
+    const int p = blockIdx.x;
+    if (p >= a.batch) return;
+    const int n = a.n;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int* prop = reinterpret_cast<int*>(smem_raw);                        // [n] propagating indices
+    double* nrm2 = reinterpret_cast<double*>(prop + ((n + 1) & ~1));     // [n] |U2_j|^2
+    C2* coef = reinterpret_cast<C2*>(nrm2 + n);                           // [n]
+    __shared__ int s_np;
+    const double* w = a.w + 2 * (long)p * n;
+    double* kz = a.kz + (long)p * n;
+    double* U = bptr(a.U, p, 0);
+    const int lu = a.U.ld;
+    const double az = a.az, gz = 2.0 * 3.14159265358979323846 / az;
+    if (t == 0) {
+        int np = 0;
+        for (int i = 0; i < n; ++i) {
+            const double lr = w[2 * i], li = w[2 * i + 1];
+            const double mod = hypot(lr, li);
+            if (fabs(mod - 1.0) < a.lambda_eta) {
+                prop[np++] = i;
+                // Re(log(lambda) / (i az)) = arg(lambda) / az, folded into [-gz/2, gz/2]
+                double k = fmod(atan2(li, lr) / az, gz);
+                if (k < 0.0) k += gz;
+                if (k > 0.5 * gz) k -= gz;
+                kz[i] = a.sign * k;
+            } else {
+                kz[i] = 100.0;
+            }
+        }
+        s_np = np;
+        if (a.nprop) a.nprop[p] = np;
+    }
+    __syncthreads();
+    const int np = s_np;
+    for (int q = 0; q < np; ++q) {
+        const int ci = prop[q];
+        // coefficients against the finished columns j < q (one wave per j)
+        for (int r = wave; r < q; r += NT / 64) {
+            const int cj = prop[r];
+            double cr = 0.0, cim = 0.0;
+            for (int i = lane; i < n; i += 64) {
+                const C2 x = ld(U, lu, i, ci), y = ld(U, lu, i, cj);
+                cr += x.r * y.r + x.i * y.i;        // conj(x) y
+                cim += x.r * y.i - x.i * y.r;
+            }
+            cr = wave_sum(cr);
+            cim = wave_sum(cim);
+            if (lane == 0) coef[r] = C2{cr / nrm2[r], cim / nrm2[r]};
+        }
+        __syncthreads();
+        double s = 0.0;
+        for (int i = t; i < n; i += NT) {
+            C2 x = ld(U, lu, i, ci);
+            for (int r = 0; r < q; ++r) x = csub(x, cmul(coef[r], ld(U, lu, i, prop[r])));
+            st(U, lu, i, ci, x);
+            s += x.r * x.r + x.i * x.i;
+        }
+        s = block_sum(s, coef_red(smem_raw, n));
+        if (t == 0) nrm2[q] = s;
+        __syncthreads();
+    }
+    for (int q = 0; q < np; ++q) {
+        const double f = 1.0 / sqrt(nrm2[q]);
+        for (int i = t; i < n; i += NT) st(U, lu, i, prop[q], cscale(ld(U, lu, i, prop[q]), f));
+    }
+}
+
+// M[p] *= s[p] (complex per item)
+__global__ void k_scale_items(double* M, long stride, int ld_, int m, int n, const double* s, int batch)
+{
+    const int p = blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= batch || e >= (long)m * n) return;
+    const int r = (int)(e % m), c = (int)(e / m);
+    double* x = M + (long)p * stride * 2 + 2 * ((long)r + (long)c * ld_);
+    const double xr = x[0], xi = x[1], sr = s[2 * p], si = s[2 * p + 1];
+    x[0] = xr * sr - xi * si;
+    x[1] = xr * si + xi * sr;
+}
+
+// per-mode sums of |S(j,i)|^2 over propagating rows j (kz_row < 99.9) for
+// propagating columns i (kz_col < 99.9): col_sum[p][i] and total[p]
+__global__ void k_mode_sums(const double* S, long stride, int ld_, int m, int n, const double* kz_row,
+                            const double* kz_col, double* col_sum, double* total, int batch)
+{
+    const int p = blockIdx.x;
+    if (p >= batch) return;
+    const int t = threadIdx.x;
+    __shared__ double red[4];
+    double tot = 0.0;
+    for (int i = t; i < n; i += NT) {
+        double acc = 0.0;
+        if (kz_col[(long)p * n + i] < 99.9)
+            for (int j = 0; j < m; ++j)
+                if (kz_row[(long)p * m + j] < 99.9) {
+                    const double* x = S + (long)p * stride * 2 + 2 * ((long)j + (long)i * ld_);
+                    acc += x[0] * x[0] + x[1] * x[1];
+                }
+        if (col_sum) col_sum[(long)p * n + i] = acc;
+        tot += acc;
+    }
+    tot = block_sum(tot, red);
+    if (t == 0 && total) total[p] = tot;
+}
+
+hipError_t launch_modes_select(const ModesArgs& a, hipStream_t s)
+{
+    if (a.batch <= 0 || a.n <= 0) return hipSuccess;
+    const size_t sm = sizeof(int) * (size_t)((a.n + 1) & ~1) + sizeof(double) * a.n + sizeof(C2) * a.n +
+                      sizeof(double) * 4;
+    hipLaunchKernelGGL(k_modes_select, dim3(a.batch), dim3(NT), sm, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale_items(double* M, long stride, int ld_, int m, int n, const double* sc, int batch,
+                              hipStream_t s)
+{
+    if (batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_items, dim3(((long)m * n + NT - 1) / NT, batch), dim3(NT), 0, s, M, stride, ld_, m, n,
+                       sc, batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_mode_sums(const double* S, long stride, int ld_, int m, int n, const double* kz_row,
+                            const double* kz_col, double* col_sum, double* total, int batch, hipStream_t s)
+{
+    if (batch <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_mode_sums, dim3(batch), dim3(NT), 0, s, S, stride, ld_, m, n, kz_row, kz_col, col_sum, total,
+                       batch);
     return hipGetLastError();
 }
 

@@ -1502,7 +1502,12 @@ void down(nt_ctx* c, void* h, const DevBuf& d, size_t bytes)
 // =================================================================== C ABI
 // ------------------------------------------------------- eigen / sqrtm
 // Scratch: T and Z (and X) per item, n^2 complex each.
-size_t eig_scratch(int n, int batch) { return sizeof(double) * 2 * (size_t)n * n * batch * 3; }
+size_t eig_scratch(int n, int batch)
+{
+    Arena a;
+    for (int i = 0; i < 3; ++i) a.take(sizeof(double) * 2 * (size_t)n * n * batch);
+    return a.off;
+}
 
 // w, V from A (A is copied, not modified).  The eigenvector matrix is Z X
 // with X the triangular eigenvectors of T.
@@ -1567,6 +1572,228 @@ void sqrtm_run(nt_ctx* c, int n, BMat<double> A, BMatW<double> S, int* status, i
     // S = (Z U) Z^H ; Z U into the T buffer
     gemm<T>(c, n, n, n, OP_N, ro(e.Z), OP_N, ro(e.X), e.A, 1.0, none<T>(), 0.0, nullptr, batch);
     gemm<T>(c, n, n, n, OP_N, ro(e.A), OP_C, ro(e.Z), S, 1.0, none<T>(), 0.0, nullptr, batch);
+}
+
+// ------------------------------------------------------ mode-resolved T
+// vel_mat_tdb (negf/velocity_mat.f90:1692-3150, tb/velocity_mat.f90:222-1560)
+// for a batch of points, without the Brillouin-zone unfolding.  Per point and
+// lead direction: F = g V, eigen-decomposition, propagating selection + k_z +
+// Gram-Schmidt, velocity matrix V = s U^+ Gamma' U, its square root, inv(U^+)
+// (incident) or inv(U) (outgoing); then the bulk GFs Q_L, Q_R and
+//   t_RL = f Vh_Rret iU_Rret G_D_RL iU_Ladv Vh_Ladv,  r_LL = f Vh_Lret iU_Lret (G_L - Q_L^-1) iU_Ladv Vh_Ladv
+// and the mirrored t_LR, r_RR (negf/velocity_mat.f90:2945-2963).  negf
+// conventions: s = -/+ i a_z / (2 sqrt(E)), f = 2 i sqrt(E / (a a')); tb: s = -/+1,
+// f = i (tb/velocity_mat.f90:467-468, 1352-1370).
+
+struct ModeOut {
+    double* trl;      // nr x nl per point (scratch when the caller passes none)
+    double* tlr;      // nl x nr
+    double* rrr;      // nr x nr
+    double* rll;      // nl x nl
+    double* kz[4];    // L_adv, L_ret, R_ret, R_adv: n per point
+    int* nprop[4];    // counts per point
+    double* t_mode_L; // nl per point: sum over propagating outgoing right modes of |t_RL(j,i)|^2
+    double* t_mode_R; // nr per point: same for t_LR
+    double* sums[4];  // T_modes_RL, T_modes_LR, R_modes_R, R_modes_L per point
+};
+
+// per-point complex scalars, one array of P per slot (slot q at out + 2 q P):
+// 0-3 the V scales of the four directions, 4-7 the prefactors of t_RL, t_LR,
+// r_RR, r_LL (zero at the reference's ie = 1 point), 8 the lead shift
+// z = E + i eta(E)
+__global__ void k_mode_scalars(const double* E, int P, double az_l, double az_r, int tb, double eta, double eta0,
+                               int eta_mode, const int* first_mask, double* out)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const double e = E[p];
+    auto put = [&](int q, double re, double im) {
+        out[2 * ((long)q * P + p)] = re;
+        out[2 * ((long)q * P + p) + 1] = im;
+    };
+    const double sq = sqrt(e);
+    const double vl = tb ? 1.0 : az_l / (2.0 * sq), vr = tb ? 1.0 : az_r / (2.0 * sq);
+    // V: L_adv, L_ret: -s ; R_ret, R_adv: +s ; s = i a/(2 sqrt E) (negf) or 1 (tb)
+    const double sv[4] = {-vl, -vl, vr, vr};
+    for (int q = 0; q < 4; ++q) put(q, tb ? sv[q] : 0.0, tb ? 0.0 : sv[q]);
+    const bool zero = first_mask && first_mask[p];
+    const double f[4] = {tb ? 1.0 : 2.0 * sqrt(e / (az_r * az_l)), tb ? 1.0 : 2.0 * sqrt(e / (az_l * az_r)),
+                         tb ? 1.0 : 2.0 * sqrt(e / (az_r * az_r)), tb ? 1.0 : 2.0 * sqrt(e / (az_l * az_l))};
+    for (int q = 0; q < 4; ++q) put(4 + q, 0.0, zero ? 0.0 : f[q]);
+    put(8, e, eta_mode == 0 ? e * eta + eta0 : eta);
+}
+
+// out[p * rows + q] = in[q * cols + p]   (rows x cols slot-major -> point-major)
+template <typename T>
+__global__ void k_transpose_small(const T* in, int rows, int cols, T* out)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)rows * cols) return;
+    const int q = (int)(e / cols), p = (int)(e % cols);
+    out[(long)p * rows + q] = in[e];
+}
+
+size_t modes_scratch(int nl, int nr, int P)
+{
+    const int n = std::max(nl, nr);
+    const size_t mb = sizeof(double) * 2 * (size_t)n * n * P;
+    Arena a;
+    for (int i = 0; i < 4; ++i) {
+        a.take(mb);                                    // U
+        a.take(mb);                                    // Vh
+        a.take(mb);                                    // invU
+        a.take(sizeof(double) * 2 * (size_t)n * P);   // w
+        a.take(sizeof(double) * (size_t)n * P);       // kz (when not output)
+    }
+    a.take(mb);  // QinvL / scratch
+    a.take(mb);  // QinvR
+    a.take(mb);  // T1
+    a.take(mb);  // T2
+    a.take(mb);  // V / W
+    for (int i = 0; i < 4; ++i) a.take(mb);   // t/r scratch
+    a.take(sizeof(double) * 18 * P);
+    return a.off + eig_scratch(n, P);
+}
+
+void modes_run(nt_ctx* c, const nt_params* prm, int tb, int nl, int nr, int P, const double* E, const int* kidx,
+               int nk, BMat<double> Haml, BMat<double> Hamlv, BMat<double> Hamr, BMat<double> Hamrv,
+               BMat<double> gl_adv, BMat<double> gr_ret, BMat<double> gl_plus, BMat<double> gr_minus,
+               BMat<double> gml_adv, BMat<double> gmr_ret, BMat<double> G_D_RL, BMat<double> G_D_LR,
+               BMat<double> G_L, BMat<double> G_R, double az_l, double az_r, const int* first_mask, ModeOut o,
+               int* status, Arena& ar, size_t off)
+{
+    typedef double T;
+    const int n = std::max(nl, nr);
+    const long mm = (long)n * n;
+    const size_t mb = sizeof(T) * 2 * (size_t)mm * P;
+    Arena sa = ar;
+    sa.off = off;
+    T *U[4], *Vh[4], *iU[4], *w[4], *kzs[4];
+    for (int i = 0; i < 4; ++i) {
+        U[i] = sa.ptr<T>(sa.take(mb));
+        Vh[i] = sa.ptr<T>(sa.take(mb));
+        iU[i] = sa.ptr<T>(sa.take(mb));
+        w[i] = sa.ptr<T>(sa.take(sizeof(T) * 2 * (size_t)n * P));
+        kzs[i] = sa.ptr<T>(sa.take(sizeof(T) * (size_t)n * P));
+    }
+    T* QL = sa.ptr<T>(sa.take(mb));
+    T* QR = sa.ptr<T>(sa.take(mb));
+    T* T1 = sa.ptr<T>(sa.take(mb));
+    T* T2 = sa.ptr<T>(sa.take(mb));
+    T* Vw = sa.ptr<T>(sa.take(mb));
+    T* tr[4];
+    for (int i = 0; i < 4; ++i) tr[i] = sa.ptr<T>(sa.take(mb));
+    T* sc = sa.ptr<T>(sa.take(sizeof(T) * 18 * P));
+    const size_t eig_off = sa.off;
+    hipStream_t s = c->stream;
+    auto W = [&](T* q, int ld) { return bw(q, mm, ld); };
+    auto R = [&](T* q, int ld) { return bm<T>(q, mm, ld); };
+    check_range(c, E, sizeof(double) * P, "modes E");
+    check_range(c, first_mask, sizeof(int) * P, "modes first mask");
+    hipLaunchKernelGGL(k_mode_scalars, dim3(nblk(P)), dim3(256), 0, s, E, P, az_l, az_r, tb, prm->eta, prm->eta0,
+                       prm->eta_mode, first_mask, sc);
+    hipcheck(c, hipGetLastError(), "mode scalars");
+    // the four directions: (n, g, op(g), V lead coupling, Gamma', op(Gamma'), k_z sign, incident?)
+    struct Dir {
+        int n;
+        BMat<T> g;
+        int opg;
+        BMat<T> V;
+        BMat<T> gam;
+        int opgam;
+        double sign;
+        bool incident;
+        double az;
+    };
+    const Dir dirs[4] = {{nl, gl_adv, OP_N, Hamlv, gml_adv, OP_N, -1.0, true, az_l},
+                         {nl, gl_adv, OP_C, Hamlv, gml_adv, OP_C, -1.0, false, az_l},
+                         {nr, gr_ret, OP_N, Hamrv, gmr_ret, OP_N, 1.0, false, az_r},
+                         {nr, gr_ret, OP_C, Hamrv, gmr_ret, OP_C, 1.0, true, az_r}};
+    for (int d = 0; d < 4; ++d) {
+        const Dir& D = dirs[d];
+        const int m = D.n;
+        double* kz = o.kz[d] ? o.kz[d] : kzs[d];
+        // F = op(g) V  -> U (eigen input, overwritten by the eigenvectors)
+        gemm<T>(c, m, m, m, D.opg, D.g, OP_N, D.V, W(U[d], m), 1.0, none<T>(), 0.0, nullptr, P);
+        eig_run(c, m, R(U[d], m), w[d], W(U[d], m), status, P, ar, eig_off);
+        ModesArgs ma;
+        ma.n = m;
+        ma.w = w[d];
+        ma.U = W(U[d], m);
+        ma.kz = kz;
+        ma.nprop = o.nprop[d];
+        ma.lambda_eta = prm->lambda_eta;
+        ma.az = D.az;
+        ma.sign = D.sign;
+        ma.batch = P;
+        audit<T>(c, ma.U, m, m, P, 1, "modes U");
+        check_range(c, kz, sizeof(double) * (size_t)m * P, "modes kz");
+        check_range(c, o.nprop[d], sizeof(int) * P, "modes nprop");
+        launch(c, NT_K_EIG, 0.0, "modes select", [&] { return launch_modes_select(ma, s); });
+        // V = s (U^+ Gamma' U)
+        gemm<T>(c, m, m, m, OP_C, R(U[d], m), D.opgam, D.gam, W(T1, m), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, m, m, m, OP_N, R(T1, m), OP_N, R(U[d], m), W(Vw, m), 1.0, none<T>(), 0.0, nullptr, P);
+        launch(c, NT_K_ELEM, 0.0, "scale V",
+               [&] { return launch_scale_items(Vw, mm, m, m, m, sc + 2 * (long)d * P, P, s); });
+        sqrtm_run(c, m, R(Vw, m), W(Vh[d], m), status, P, ar, eig_off);
+        // inv(U^+) for incident directions, inv(U) for outgoing ones
+        copy_op<T>(c, m, m, R(U[d], m), W(T2, m), D.incident ? 1 : 0, (T)1, P);
+        inverse<T>(c, m, W(T2, m), none<T>(), nullptr, W(iU[d], m), status, P);
+    }
+    // bulk GFs: Q = z - H00 - V^+ op(g) V - V op(g') V^+, inverted
+    {
+        // left: z - Haml - Hamlv^+ gl_adv^+ Hamlv - Hamlv gl_plus Hamlv^+
+        gemm<T>(c, nl, nl, nl, OP_C, Hamlv, OP_C, gl_adv, W(T1, nl), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, nl, nl, nl, OP_N, R(T1, nl), OP_N, Hamlv, W(T2, nl), -1.0, Haml, -1.0, sc + 16 * (long)P, P);
+        gemm<T>(c, nl, nl, nl, OP_N, Hamlv, OP_N, gl_plus, W(T1, nl), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, nl, nl, nl, OP_N, R(T1, nl), OP_C, Hamlv, W(T2, nl), -1.0, R(T2, nl), 1.0, nullptr, P);
+        inverse<T>(c, nl, W(T2, nl), none<T>(), nullptr, W(QL, nl), status, P);
+        // right: z - Hamr - Hamrv^+ gr_ret Hamrv - Hamrv gr_minus^+ Hamrv^+
+        gemm<T>(c, nr, nr, nr, OP_C, Hamrv, OP_N, gr_ret, W(T1, nr), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, nr, nr, nr, OP_N, R(T1, nr), OP_N, Hamrv, W(T2, nr), -1.0, Hamr, -1.0, sc + 16 * (long)P, P);
+        gemm<T>(c, nr, nr, nr, OP_N, Hamrv, OP_C, gr_minus, W(T1, nr), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, nr, nr, nr, OP_N, R(T1, nr), OP_C, Hamrv, W(T2, nr), -1.0, R(T2, nr), 1.0, nullptr, P);
+        inverse<T>(c, nr, W(T2, nr), none<T>(), nullptr, W(QR, nr), status, P);
+    }
+    // t and r: out = f Vh_o iU_o M iU_i Vh_i   (o: outgoing, i: incident)
+    struct Chain {
+        int o, i;        // direction indices
+        BMat<T> M;       // middle matrix (G block)
+        T* Qinv;         // subtracted bulk inverse (reflections) or null
+        double* out;
+        int q;           // prefactor index
+    };
+    const Chain ch[4] = {{2, 0, G_D_RL, nullptr, o.trl ? o.trl : tr[0], 0},
+                         {1, 3, G_D_LR, nullptr, o.tlr ? o.tlr : tr[1], 1},
+                         {2, 3, G_R, QR, o.rrr ? o.rrr : tr[2], 2},
+                         {1, 0, G_L, QL, o.rll ? o.rll : tr[3], 3}};
+    for (const Chain& C : ch) {
+        const int mo = dirs[C.o].n, mi = dirs[C.i].n;
+        const long so = (long)mo * mi;
+        BMatW<T> outw = bw(C.out, so, mo);
+        audit<T>(c, outw, mo, mi, P, 1, "modes t/r");
+        gemm<T>(c, mi, mi, mi, OP_N, R(iU[C.i], mi), OP_N, R(Vh[C.i], mi), W(T1, mi), 1.0, none<T>(), 0.0, nullptr,
+                P);
+        if (C.Qinv)   // M - Q^-1 into T2 (square: mo == mi)
+            copy_op<T>(c, mo, mi, R(C.Qinv, mo), W(T2, mo), 0, (T)-1, P);
+        gemm<T>(c, mo, mi, mi, OP_N, C.M, OP_N, R(T1, mi), W(Vw, mo), 1.0, none<T>(), 0.0, nullptr, P);
+        if (C.Qinv) {
+            // (M - Q^-1) T1 = M T1 + (-Q^-1) T1
+            gemm<T>(c, mo, mi, mi, OP_N, R(T2, mo), OP_N, R(T1, mi), W(Vw, mo), 1.0, R(Vw, mo), 1.0, nullptr, P);
+        }
+        gemm<T>(c, mo, mi, mo, OP_N, R(iU[C.o], mo), OP_N, R(Vw, mo), W(T1, mo), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, mo, mi, mo, OP_N, R(Vh[C.o], mo), OP_N, R(T1, mo), outw, 1.0, none<T>(), 0.0, nullptr, P);
+        launch(c, NT_K_ELEM, 0.0, "scale t",
+               [&] { return launch_scale_items(C.out, so, mo, mo, mi, sc + 2 * (long)(4 + C.q) * P, P, s); });
+        double* kzo = o.kz[C.o] ? o.kz[C.o] : kzs[C.o];
+        double* kzi = o.kz[C.i] ? o.kz[C.i] : kzs[C.i];
+        double* colsum = C.q == 0 ? o.t_mode_L : (C.q == 1 ? o.t_mode_R : nullptr);
+        check_range(c, colsum, sizeof(double) * (size_t)mi * P, "modes t_mode");
+        check_range(c, o.sums[C.q], sizeof(double) * P, "modes sums");
+        launch(c, NT_K_ELEM, 0.0, "mode sums", [&] {
+            return launch_mode_sums(C.out, so, mo, mo, mi, kzo, kzi, colsum, o.sums[C.q], P, s);
+        });
+    }
 }
 
 // ------------------------------------------------------------- autotuner
@@ -2392,6 +2619,130 @@ int nt_set_inverse_panel(nt_ctx* c, int n, int precision, int nb)
             return NT_OK;
         }
     return NT_ERR_ARG;
+}
+
+int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, int npts, const double* E,
+                             const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv,
+                             const nt_c128* Hamr, const nt_c128* Hamrv, const nt_c128* gl_adv,
+                             const nt_c128* gr_ret, const nt_c128* gl_plus, const nt_c128* gr_minus,
+                             const nt_c128* gml_adv, const nt_c128* gmr_ret, const nt_c128* G_D_RL,
+                             const nt_c128* G_D_LR, const nt_c128* G_L, const nt_c128* G_R, double a_z_L,
+                             double a_z_R, const int* first_point_mask, const nt_modes_out* out, int* status)
+{
+    if (!c || check_params(p) || (tb != 0 && tb != 1)) return NT_ERR_ARG;
+    if (nl <= 0 || nr <= 0 || npts < 0 || nk <= 0 || (nk > 1 && !kidx) || !(a_z_L > 0.0) || !(a_z_R > 0.0))
+        return NT_ERR_ARG;
+    if (!E || !Haml || !Hamlv || !Hamr || !Hamrv || !gl_adv || !gr_ret || !gl_plus || !gr_minus || !gml_adv ||
+        !gmr_ret || !G_D_RL || !G_D_LR || !G_L || !G_R)
+        return NT_ERR_ARG;
+    if (std::max(nl, nr) > inv_max_n()) return NT_ERR_ARG;
+    if (npts == 0) return NT_OK;
+    return guarded(c, [&] {
+        const size_t z = sizeof(nt_c128);
+        const long nll = (long)nl * nl, nrr = (long)nr * nr, nlr = (long)nl * nr;
+        reg_region(c, E, sizeof(double) * npts);
+        reg_region(c, kidx, sizeof(int) * npts);
+        reg_region(c, first_point_mask, sizeof(int) * npts);
+        reg_region(c, status, sizeof(int) * npts);
+        reg_region(c, Haml, z * nll * nk);
+        reg_region(c, Hamlv, z * nll * nk);
+        reg_region(c, Hamr, z * nrr * nk);
+        reg_region(c, Hamrv, z * nrr * nk);
+        const nt_c128* pl[] = {gl_adv, gl_plus, gml_adv, G_L};
+        const nt_c128* pr[] = {gr_ret, gr_minus, gmr_ret, G_R};
+        for (auto q : pl) reg_region(c, q, z * nll * npts);
+        for (auto q : pr) reg_region(c, q, z * nrr * npts);
+        reg_region(c, G_D_RL, z * nlr * npts);
+        reg_region(c, G_D_LR, z * nlr * npts);
+        nt_modes_out o = out ? *out : nt_modes_out{};
+        reg_region(c, o.trl, z * nlr * npts);
+        reg_region(c, o.tlr, z * nlr * npts);
+        reg_region(c, o.rrr, z * nrr * npts);
+        reg_region(c, o.rll, z * nll * npts);
+        reg_region(c, o.k_z_L_adv, sizeof(double) * nl * npts);
+        reg_region(c, o.k_z_L_ret, sizeof(double) * nl * npts);
+        reg_region(c, o.k_z_R_ret, sizeof(double) * nr * npts);
+        reg_region(c, o.k_z_R_adv, sizeof(double) * nr * npts);
+        reg_region(c, o.n_prop, sizeof(int) * 4 * npts);
+        reg_region(c, o.t_mode_L, sizeof(double) * nl * npts);
+        reg_region(c, o.t_mode_R, sizeof(double) * nr * npts);
+        reg_region(c, o.sums, sizeof(double) * 4 * npts);
+        if (kidx) {
+            std::vector<int> hk(npts);
+            hipcheck(c, hipMemcpyAsync(hk.data(), kidx, sizeof(int) * npts, hipMemcpyDeviceToHost, c->stream), "D2H");
+            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+            for (int v : hk)
+                if (v < 0 || v >= nk) {
+                    c->err = "kidx out of range [0, nk)";
+                    return (int)NT_ERR_ARG;
+                }
+        }
+        // per-point scratch for the counts and sums of a chunk (outputs are point-major)
+        const int n = std::max(nl, nr);
+        const double per_pt = (double)modes_scratch(nl, nr, 1) + 64.0 * n;
+        const int Pmax = std::max(1, std::min(std::min(npts, c->batch_limit), (int)std::min(1.0e9, 16.0e9 / per_pt)));
+        Arena probe;
+        const size_t o_np = probe.take(sizeof(int) * 4 * Pmax);
+        const size_t o_sum = probe.take(sizeof(double) * 4 * Pmax);
+        const size_t o_st = probe.take(sizeof(int) * Pmax);
+        const size_t o_run = probe.off;
+        ensure_ws(c, o_run + modes_scratch(nl, nr, Pmax));
+        Arena ar;
+        ar.base = static_cast<char*>(c->ws);
+        int* np4 = ar.ptr<int>(o_np);
+        double* sum4 = ar.ptr<double>(o_sum);
+        int* stc = ar.ptr<int>(o_st);
+        typedef double T;
+        auto kst = [&](long per_k) { return kidx ? per_k : 0L; };
+        for (int p0 = 0; p0 < npts; p0 += Pmax) {
+            const int P = std::min(Pmax, npts - p0);
+            const int* kk = kidx ? kidx + p0 : nullptr;
+            auto lead = [&](const nt_c128* b, long per_k, int ld) {
+                return bm<T>(reinterpret_cast<const T*>(b), kst(per_k), ld, kk, 0, nk);
+            };
+            auto pt = [&](const nt_c128* b, long per, int ld) {
+                return bm<T>(reinterpret_cast<const T*>(b) + 2 * (long)p0 * per, per, ld);
+            };
+            auto ptw = [&](nt_c128* b, long per) {
+                return b ? reinterpret_cast<double*>(b) + 2 * (long)p0 * per : nullptr;
+            };
+            auto ptd = [&](double* b, long per) { return b ? b + (long)p0 * per : nullptr; };
+            ModeOut mo;
+            mo.trl = ptw(o.trl, nlr);
+            mo.tlr = ptw(o.tlr, nlr);
+            mo.rrr = ptw(o.rrr, nrr);
+            mo.rll = ptw(o.rll, nll);
+            mo.kz[0] = ptd(o.k_z_L_adv, nl);
+            mo.kz[1] = ptd(o.k_z_L_ret, nl);
+            mo.kz[2] = ptd(o.k_z_R_ret, nr);
+            mo.kz[3] = ptd(o.k_z_R_adv, nr);
+            for (int d = 0; d < 4; ++d) mo.nprop[d] = o.n_prop ? np4 + d * P : nullptr;
+            mo.t_mode_L = ptd(o.t_mode_L, nl);
+            mo.t_mode_R = ptd(o.t_mode_R, nr);
+            for (int q = 0; q < 4; ++q) mo.sums[q] = o.sums ? sum4 + q * P : nullptr;
+            hipcheck(c, hipMemsetAsync(stc, 0, sizeof(int) * P, c->stream), "memset");
+            modes_run(c, p, tb, nl, nr, P, E + p0, kk, nk, lead(Haml, nll, nl), lead(Hamlv, nll, nl),
+                      lead(Hamr, nrr, nr), lead(Hamrv, nrr, nr), pt(gl_adv, nll, nl), pt(gr_ret, nrr, nr),
+                      pt(gl_plus, nll, nl), pt(gr_minus, nrr, nr), pt(gml_adv, nll, nl), pt(gmr_ret, nrr, nr),
+                      pt(G_D_RL, nlr, nr), pt(G_D_LR, nlr, nl), pt(G_L, nll, nl), pt(G_R, nrr, nr), a_z_L, a_z_R,
+                      first_point_mask ? first_point_mask + p0 : nullptr, mo, stc, ar, o_run);
+            // counts and sums: slot-major scratch -> point-major outputs
+            if (o.n_prop) {
+                hipLaunchKernelGGL(k_transpose_small<int>, dim3(nblk(4L * P)), dim3(256), 0, c->stream, np4, 4, P,
+                                   o.n_prop + 4L * p0);
+            }
+            if (o.sums) {
+                hipLaunchKernelGGL(k_transpose_small<double>, dim3(nblk(4L * P)), dim3(256), 0, c->stream, sum4, 4,
+                                   P, o.sums + 4L * p0);
+            }
+            if (status)
+                hipcheck(c, hipMemcpyAsync(status + p0, stc, sizeof(int) * P, hipMemcpyDeviceToDevice, c->stream),
+                         "status copy");
+            hipcheck(c, hipGetLastError(), "modes outputs");
+        }
+        hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+        return 0;
+    });
 }
 
 int nt_zgeev_batched_dev(nt_ctx* c, int n, const nt_c128* A, long long strideA, nt_c128* w, nt_c128* V,

@@ -62,6 +62,22 @@ hipError_t launch_schur(const EigArgs& a, hipStream_t s);
 hipError_t launch_trevc(const EigArgs& a, hipStream_t s);
 hipError_t launch_ev_normalize(const EigArgs& a, hipStream_t s);
 hipError_t launch_sqrt_tri(const EigArgs& a, hipStream_t s);
+// propagating-mode selection, k_z and Gram-Schmidt (nt_eig.hip)
+struct ModesArgs {
+    int n;
+    const double* w;       // eigenvalues, n complex per item
+    BMatW<double> U;       // eigenvectors, propagating columns orthonormalised in place
+    double* kz;            // n per item (100 for evanescent modes)
+    int* nprop;            // optional count per item
+    double lambda_eta, az, sign;
+    int batch;
+};
+hipError_t launch_modes_select(const ModesArgs& a, hipStream_t s);
+// M[p] (m x n, leading dim ld, item stride in complex elements) *= sc[p]
+hipError_t launch_scale_items(double* M, long stride, int ld, int m, int n, const double* sc, int batch,
+                              hipStream_t s);
+hipError_t launch_mode_sums(const double* S, long stride, int ld, int m, int n, const double* kz_row,
+                            const double* kz_col, double* col_sum, double* total, int batch, hipStream_t s);
 
 // Elementwise helpers (nt_elem.hip).
 // out[p] = sign * (dagger ? M^+ : M) for an n x n matrix

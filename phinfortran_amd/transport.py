@@ -308,6 +308,57 @@ class Engine:
         return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters,
                     dHamd=gd.transpose(-1, -2), dHamdv=gv.transpose(-1, -2))
 
+    MODE_FIELDS = ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "G_D_RL", "G_D_LR", "G_L", "G_R")
+
+    def solve_modes(self, dblocks, E, p: N.NtParams, a_z_L: float, a_z_R: float, kidx=None, first_point_mask=None,
+                    tb: bool = False, want=("trl", "tlr", "rrr", "rll")):
+        """Batched solve followed by the mode-resolved transmission (vel_mat_tdb).
+
+        Returns the solve's dict plus k_z_* (npts, n), n_prop (npts, 4),
+        t_mode_L (npts, nl), t_mode_R (npts, nr), sums (npts, 4: T_RL, T_LR,
+        R_RR, R_LL over propagating modes) and the requested t/r matrices as
+        row-major (npts, rows, cols) views."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        r = self.solve_tdb(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=self.MODE_FIELDS)
+        E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
+        npts = E.numel()
+        nl, nr, nk = dblocks["nl"], dblocks["nr"], dblocks["nk"]
+        if kidx is not None:
+            kidx = torch.as_tensor(kidx, dtype=torch.int32, device=dev).contiguous()
+        if first_point_mask is not None:
+            first_point_mask = torch.as_tensor(first_point_mask, dtype=torch.int32, device=dev).contiguous()
+        shapes = dict(trl=(nl, nr), tlr=(nr, nl), rrr=(nr, nr), rll=(nl, nl))    # storage [p, col, row]
+        out = N.NtModesOut()
+        res = {}
+        for name in want:
+            t = torch.zeros((npts,) + shapes[name], dtype=torch.complex128, device=dev)
+            res[name] = t
+            setattr(out, name, t.data_ptr())
+        for name, n in (("k_z_L_adv", nl), ("k_z_L_ret", nl), ("k_z_R_ret", nr), ("k_z_R_adv", nr),
+                        ("t_mode_L", nl), ("t_mode_R", nr), ("sums", 4)):
+            t = torch.zeros((npts, n), dtype=torch.float64, device=dev)
+            res[name] = t
+            setattr(out, name, t.data_ptr())
+        res["n_prop"] = torch.zeros((npts, 4), dtype=torch.int32, device=dev)
+        out.n_prop = res["n_prop"].data_ptr()
+        status = torch.zeros(npts, dtype=torch.int32, device=dev)
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        # extras come back transposed (row-major views); the ABI wants the storage
+        g = {k: r[k].transpose(-1, -2) for k in self.MODE_FIELDS}
+        st = self._L.nt_modes_tdb_batched_dev(
+            self.ctx, C.byref(p), int(tb), nl, nr, npts, ptr(E), ptr(kidx), nk,
+            ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
+            *[ptr(g[k]) for k in self.MODE_FIELDS], float(a_z_L), float(a_z_R), ptr(first_point_mask),
+            C.byref(out), ptr(status))
+        self._check(st, "modes_tdb")
+        torch.cuda.synchronize(dev)
+        for name in want:
+            res[name] = res[name].transpose(-1, -2)
+        res["modes_status"] = status
+        r.update(res)
+        return r
+
     # ------------------------------------------------------------ tuning
     def autotune_inverse(self, n: int, batch: int, precision: str = "fp64"):
         """Time every inverse panel width for n x n batches; keeps the fastest.

@@ -1,0 +1,126 @@
+"""Mode-resolved transmission on the GPU (nt_modes_tdb_batched_dev) against
+the oracle (oracle/modes.py, itself pinned to the reference's per-mode
+tables) and against those tables directly.
+
+Mode order inside the propagating set follows the eigensolver's deflation
+order, which is not LAPACK's; the Gram-Schmidt step then mixes
+non-orthogonal modes differently, so per-mode values are compared where they
+are basis independent (pristine leads, single-mode cases) and the sums over
+modes everywhere."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import modes as M
+from phinfortran_amd import lattice as lt
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import si_si as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def T():
+    from phinfortran_amd import transport
+    return transport
+
+
+def oparams(p):
+    return O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
+
+
+def test_chain_single_mode(engine, T):
+    blk = lt.chain_blocks(L=4)
+    p = T.params("phonon", eta=1e-9, eta0=1e-10, convergence=1e-14, max_iter=2000)
+    E = np.array([0.3, 1.7, 3.5])
+    d = engine.upload_blocks(blk)
+    print("modes chain", flush=True)
+    r = engine.solve_modes(d, E, p, 1.0, 1.0)
+    assert (r["modes_status"].cpu().numpy() == 0).all()
+    npr = r["n_prop"].cpu().numpy()
+    assert (npr == 1).all()
+    kz = r["k_z_L_adv"].cpu().numpy()[:, 0]
+    assert np.abs(np.abs(kz) - np.arccos(1 - E / 2)).max() < 1e-9
+    assert np.abs(r["t_mode_L"].cpu().numpy()[:, 0] - 1).max() < 1e-6
+
+
+@pytest.mark.parametrize("n,L,nk", [(6, 3, 1), (8, 4, 2)])
+def test_random_system_vs_oracle(engine, T, n, L, nk):
+    blocks = [lt.random_blocks(n, n, L, seed=70 + i, scale=0.6) for i in range(nk)]
+    for b in blocks:      # Hermitian leads with Hamrv = Hamlv^+ mirrored, device as in random_blocks
+        b.Hamr[...] = b.Haml
+        b.Hamrv[...] = b.Hamlv.conj().T
+    p = T.params("phonon", eta=1e-7, eta0=1e-8, convergence=1e-13, max_iter=3000)
+    op = oparams(p)
+    rng = np.random.default_rng(2)
+    npts = 8
+    E = rng.uniform(0.05, 0.6, npts)
+    kidx = rng.integers(0, nk, npts).astype(np.int32)
+    az = 1.3
+    d = engine.upload_blocks(blocks)
+    print(f"modes random n={n} L={L} nk={nk}", flush=True)
+    r = engine.solve_modes(d, E, p, az, az, kidx=kidx if nk > 1 else None)
+    assert (r["modes_status"].cpu().numpy() == 0).all()
+    sums = r["sums"].cpu().numpy()
+    npr = r["n_prop"].cpu().numpy()
+    kz = {k: r[k].cpu().numpy() for k in ("k_z_L_adv", "k_z_L_ret", "k_z_R_ret", "k_z_R_adv")}
+    for i in range(npts):
+        m = M.solve_point_modes(blocks[kidx[i] if nk > 1 else 0], E[i], op, p.lambda_eta, az, az)
+        assert list(npr[i]) == [m["n_prop"][k] for k in ("L_adv", "L_ret", "R_ret", "R_adv")]
+        for k in kz:
+            a = np.sort(kz[k][i][kz[k][i] < 99])
+            b = np.sort(m[k][m[k] < 99])
+            assert np.abs(a - b).max(initial=0) < 1e-8, (i, k)
+        inc, out = m["k_z_L_adv"] < 99, m["k_z_R_ret"] < 99
+        ref = [(np.abs(m["trl"][np.ix_(out, inc)]) ** 2).sum(),
+               (np.abs(m["tlr"][np.ix_(m["k_z_L_ret"] < 99, m["k_z_R_adv"] < 99)]) ** 2).sum(),
+               (np.abs(m["rrr"][np.ix_(out, m["k_z_R_adv"] < 99)]) ** 2).sum(),
+               (np.abs(m["rll"][np.ix_(m["k_z_L_ret"] < 99, inc)]) ** 2).sum()]
+        assert np.abs(sums[i] - ref).max() < 1e-6 * max(1.0, max(ref)), (i, sums[i], ref)
+        # mode sums against Caroli
+        assert abs(sums[i][0] - r["T_RL"][i].item()) < 1e-4 * max(1.0, sums[i][0])
+
+
+def test_si_si_per_mode_tables_gpu(engine, T):
+    cfg = S.read_config()
+    qe = S.read_qe_fc(cfg)
+    asm = S.TdbAssembler(qe, cfg)
+    ks, E = S.k_grid(cfg), S.e_grid(cfg)
+    az = qe["cell"][2, 2]
+    kp = list(S.MODE_KPOINTS)
+    d = engine.upload_blocks([asm.blocks(ks[ik]) for ik in kp])
+    ne = len(E)
+    kidx = np.repeat(np.arange(len(kp), dtype=np.int32), ne)
+    mask = np.tile((np.arange(ne) == 0).astype(np.int32), len(kp))
+    p = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500,
+                 lambda_eta=5e-3)
+    print("modes si-si", flush=True)
+    r = engine.solve_modes(d, np.tile(E, len(kp)), p, az, az, kidx=kidx, first_point_mask=mask, want=())
+    assert (r["modes_status"].cpu().numpy() == 0).all()
+    kz = r["k_z_L_adv"].cpu().numpy()
+    tm = r["t_mode_L"].cpu().numpy()
+    dk = dt = 0.0
+    nmodes = 0
+    for j, ik in enumerate(kp):
+        gold = S.golden_modes(ik)
+        for ie in range(ne):
+            q = j * ne + ie
+            inc = np.where(kz[q] < 99.9)[0]
+            rows = gold.get(round(math.sqrt(E[ie]), 8))
+            if rows is None:
+                assert len(inc) == 0
+                continue
+            assert len(inc) == len(rows), (ik, ie)
+            o = np.argsort(kz[q][inc])
+            g = rows[np.argsort(rows[:, 0])]
+            dk = max(dk, np.abs(kz[q][inc][o] - g[:, 0]).max())
+            dt = max(dt, np.abs(tm[q][inc][o] - g[:, 1]).max())
+            nmodes += len(inc)
+    print(f"si-si per-mode: {nmodes} modes, max |dk_z| {dk:.2e}, max |dT_n| {dt:.2e}")
+    assert dk < 1e-7 and dt < 2e-6

@@ -47,6 +47,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16, help="solves in the CPU baseline sample (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint measurement")
+    ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="skip the mixed-precision run")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="before timing, run one device-to-device copy of this many MiB (HBM-counter calibration "
+                         "under rocprofv3 --pmc)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the (omega,k) points shard across GPUs with no "
@@ -73,6 +78,17 @@ def main():
     eng = T.Engine(0, batch_limit=args.points)
     dblocks = [eng.upload_blocks(b) for b in blocks]
     torch.cuda.synchronize()
+    if args.copy_probe_mib:
+        src = torch.empty(args.copy_probe_mib * (1 << 20) // 8, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
+    tuned = {}
+    if args.autotune:
+        for prec in ("fp64", "mixed"):
+            tuned[prec] = eng.autotune_inverse(blocks[0].b, args.points, prec)
+            log(f"inverse panel width for b={blocks[0].b} {prec}: {tuned[prec][0]} ({tuned[prec][1]:.2f} ms)")
 
     def run(prec):
         """warmup + timed steps for one precision; returns (s, per-step T, kernel stats, mean iters)"""
@@ -102,7 +118,7 @@ def main():
         return el, [r["T_LR"].cpu().numpy() for r in outs], kern, float(np.mean(iters)), ok
 
     el64, T64, k64, nbar, ok64 = run("fp64")
-    elmx, Tmx, kmx, _, okmx = run("mixed")
+    elmx, Tmx, kmx, _, okmx = run("mixed") if args.mixed else (el64, T64, k64, nbar, ok64)
 
     def run_adjoint():
         """forward + adjoint of the conductance weights over the interface window"""
@@ -182,7 +198,8 @@ def main():
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,
                               "mixed_speedup": vmx / v64, "mixed_max_rel_dT": rel_err,
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",
-                              "status_ok": bool(ok64 and okmx)},
+                              "status_ok": bool(ok64 and okmx)} if args.mixed else None,
+        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps_on_two_streams": True},
         "solve_model": {"mean_decimation_iters": nbar, "flop_per_solve": flop_solve,
                         "solve_tflops": flop_solve * value / 1e12},
         "cpu_baseline": cpu,

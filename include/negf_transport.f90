@@ -33,6 +33,12 @@ module negf_transport
         type(c_ptr) :: sdos_l = c_null_ptr, sdos_r = c_null_ptr
     end type nt_solve_extras
 
+    type, bind(C) :: nt_modes_out
+        type(c_ptr) :: trl = c_null_ptr, tlr = c_null_ptr, rrr = c_null_ptr, rll = c_null_ptr
+        type(c_ptr) :: k_z_L_adv = c_null_ptr, k_z_L_ret = c_null_ptr, k_z_R_ret = c_null_ptr, k_z_R_adv = c_null_ptr
+        type(c_ptr) :: n_prop = c_null_ptr, t_mode_L = c_null_ptr, t_mode_R = c_null_ptr, sums = c_null_ptr
+    end type nt_modes_out
+
     interface
         subroutine nt_params_default(p) bind(C, name="nt_params_default")
             import :: nt_params
@@ -114,6 +120,22 @@ module negf_transport
             real(c_double), value :: E
             type(c_ptr), value :: gl, gr, Hd, Hdv, Hdv_back, Hld, Hrd, Hrd_fwd, ldos
             real(c_double), intent(out) :: ddos
+        end function
+
+        ! vel_mat_tdb (negf/velocity_mat.f90:1692-3150) on device memory
+        integer(c_int) function nt_modes_tdb_batched_dev(ctx, p, tb, nl, nr, npts, E, kidx, nk, Haml, Hamlv, &
+                Hamr, Hamrv, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R, &
+                a_z_L, a_z_R, first_point_mask, out, status) bind(C, name="nt_modes_tdb_batched_dev")
+            import :: c_int, c_ptr, c_double, nt_params, nt_modes_out
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: tb, nl, nr, npts, nk
+            type(c_ptr), value :: E, kidx, Haml, Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus, gr_minus
+            type(c_ptr), value :: gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R
+            real(c_double), value :: a_z_L, a_z_R
+            type(c_ptr), value :: first_point_mask
+            type(nt_modes_out), intent(in) :: out
+            type(c_ptr), value :: status
         end function
 
         integer(c_int) function nt_solve_tdb_batched(ctx, p, nl, nr, b, L, npts, E, kidx, nk, &

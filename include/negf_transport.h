@@ -291,6 +291,9 @@ int nt_modes_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int tb, int nl, in
  * width directly (0 restores the default).  No reference counterpart. */
 int nt_autotune_inverse(nt_ctx* ctx, int n, int batch, int precision, int* best_nb, double* best_ms);
 int nt_set_inverse_panel(nt_ctx* ctx, int n, int precision, int nb);
+/* on = 1 (default): the right-to-left RGF sweep runs on a second stream of
+ * the context beside the left-to-right one; on = 0: both on the main stream. */
+int nt_set_overlap(nt_ctx* ctx, int on);
 
 /* ------------------------------------------------ batched building blocks
  * The two kernels every solve is made of, on device memory (column-major,

@@ -30,6 +30,9 @@ using namespace nt;
 struct nt_ctx {
     int device = 0;
     hipStream_t own = nullptr;
+    hipStream_t aux = nullptr;     // second stream: the right-to-left sweep runs beside the left-to-right one
+    hipEvent_t fork = nullptr, join = nullptr;
+    bool overlap = true;
     hipStream_t stream = nullptr;
     std::string err;
     int batch_limit = 512;
@@ -809,6 +812,12 @@ inline AdjSpec adj_carve(Arena& a, int lo, int hi, int L, long mm, int P, const 
     return s;
 }
 
+__global__ void k_merge_status(const int* src, int* dst, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P && dst[p] == 0 && src[p] != 0) dst[p] = src[p];
+}
+
 __global__ void k_scale_points(double2* A, long n, long stride, const double* w, int P)
 {
     const int p = blockIdx.y;
@@ -860,6 +869,9 @@ size_t rgf_scratch(int nl, int nr, int bmax, int P)
     const size_t mb = sizeof(T) * 2 * (size_t)m * m;
     Arena a;
     for (int i = 0; i < 11; ++i) a.take(mb * P);
+    // the right-to-left sweep's own buffers and status (concurrent sweeps)
+    for (int i = 0; i < 10; ++i) a.take(mb * P);
+    a.take(sizeof(int) * P);
     return a.off;
 }
 
@@ -962,6 +974,19 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     T* GRL = sa.ptr<T>(sa.take(mb * P));
     T* X = sa.ptr<T>(sa.take(mb * P));
     T* Y = sa.ptr<T>(sa.take(mb * P));
+    // buffers of the right-to-left sweep
+    T* G_b = sa.ptr<T>(sa.take(mb * P));
+    T* Gn_b = sa.ptr<T>(sa.take(mb * P));
+    T* Aw_b = sa.ptr<T>(sa.take(mb * P));
+    T* T1_b = sa.ptr<T>(sa.take(mb * P));
+    T* G0_b = sa.ptr<T>(sa.take(mb * P));
+    T* G0n_b = sa.ptr<T>(sa.take(mb * P));
+    T* T2_b = sa.ptr<T>(sa.take(mb * P));
+    T* GRL_b = sa.ptr<T>(sa.take(mb * P));
+    T* X_b = sa.ptr<T>(sa.take(mb * P));
+    T* Y_b = sa.ptr<T>(sa.take(mb * P));
+    int* st_b = sa.ptr<int>(sa.take(sizeof(int) * P));
+    (void)GRL;
     hipStream_t s = c->stream;
     const int L = in.L;
     auto W = [&](T* p, int ld) { return bw(p, mm, ld); };
@@ -971,6 +996,10 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     BMatW<T> gLo = out.G_L.base ? out.G_L : W(Gn, nl);
     BMatW<T> gRo = out.G_R.base ? out.G_R : W(Gn, nr);
     int* st = out.status;
+    if (c->overlap && c->aux) {
+        hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
+        hipcheck(c, hipStreamWaitEvent(c->aux, c->fork, 0), "wait fork");
+    }
 
     // adjoint storage slots of device layer i (null when layer i is not stored)
     const long slot = mm * P;
@@ -1028,11 +1057,29 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P);
         if (adj) adjoint_backward<T>(c, nl, nr, in, P, mm, gdlr, gRo, Vc, opVc, opVcH, *adj, T1, T2);
     }
-    // ---------------- right to left
-    G = gbufs[0];
-    Gn = gbufs[1];
-    G0 = g0bufs[0];
-    G0n = g0bufs[1];
+    // ---------------- right to left, on the auxiliary stream when enabled: it
+    // needs only the inputs, so it runs beside the left-to-right sweep and
+    // fills the CUs the latency-bound inverses of either sweep leave idle.
+    // (The left-to-right part above was only enqueued; the fork event is
+    // recorded before it on the main stream.)
+    G = G_b;
+    Gn = Gn_b;
+    G0 = G0_b;
+    G0n = G0n_b;
+    Aw = Aw_b;
+    T1 = T1_b;
+    T2 = T2_b;
+    X = X_b;
+    Y = Y_b;
+    hipStream_t main_stream = c->stream;
+    const bool two = c->overlap && c->aux;
+    if (two) {
+        c->stream = c->aux;
+        s = c->aux;
+    }
+    hipcheck(c, hipMemsetAsync(st_b, 0, sizeof(int) * P, s), "memset");
+    st = st_b;
+    gdrl = out.G_D_RL.base ? out.G_D_RL : W(GRL_b, nr);
     {
         const int bN = in.bsz[L - 1];
         gemm<T>(c, bN, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, P);
@@ -1070,6 +1117,14 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         gemm<T>(c, nl, nr, nr, OP_C, ro(gdrl), OP_N, in.gmr, W(Y, nl), 1.0, none<T>(), 0.0, nullptr, P);
         trace_prod<T>(c, nr, nl, R(X, nr), R(Y, nl), out.T_RL, P);
     }
+    if (two) {
+        hipcheck(c, hipEventRecord(c->join, c->aux), "event join");
+        c->stream = main_stream;
+        hipcheck(c, hipStreamWaitEvent(main_stream, c->join, 0), "wait join");
+    }
+    // merge the right-to-left statuses (first error of a point wins)
+    hipLaunchKernelGGL(k_merge_status, dim3(nblk(P)), dim3(256), 0, main_stream, st_b, out.status, P);
+    hipcheck(c, hipGetLastError(), "merge status");
     (void)gLo;
 }
 
@@ -1903,6 +1958,9 @@ int nt_create(nt_ctx** out, int device_ordinal)
     nt_ctx* c = new nt_ctx();
     c->device = device_ordinal;
     if (hipSetDevice(device_ordinal) != hipSuccess || hipStreamCreate(&c->own) != hipSuccess ||
+        hipStreamCreate(&c->aux) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&c->host_flag), sizeof(int)) != hipSuccess) {
         delete c;
         return NT_ERR_HIP;
@@ -1926,6 +1984,9 @@ int nt_destroy(nt_ctx* c)
     for (auto& u : c->circ_u) (void)hipFree(u.d);
     if (c->host_flag) (void)hipHostFree(c->host_flag);
     if (c->own) (void)hipStreamDestroy(c->own);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
     delete c;
     return NT_OK;
 }
@@ -2600,6 +2661,13 @@ int nt_autotune_inverse(nt_ctx* c, int n, int batch, int precision, int* best_nb
             tune_inverse<float>(c, n, batch, best_nb, best_ms);
         return 0;
     });
+}
+
+int nt_set_overlap(nt_ctx* c, int on)
+{
+    if (!c || (on != 0 && on != 1)) return NT_ERR_ARG;
+    c->overlap = on == 1;
+    return NT_OK;
 }
 
 int nt_set_inverse_panel(nt_ctx* c, int n, int precision, int nb)

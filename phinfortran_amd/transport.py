@@ -369,6 +369,10 @@ class Engine:
                     "nt_autotune_inverse")
         return nb.value, ms.value
 
+    def set_overlap(self, on: bool):
+        """Run the two RGF sweeps on two streams (default) or serially."""
+        self._check(self._L.nt_set_overlap(self.ctx, int(bool(on))), "nt_set_overlap")
+
     def set_inverse_panel(self, n: int, nb: int, precision: str = "fp64"):
         prec = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[precision]
         self._check(self._L.nt_set_inverse_panel(self.ctx, int(n), prec, int(nb)), "nt_set_inverse_panel")

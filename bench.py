@@ -52,6 +52,9 @@ def main():
                     help="before timing, run one device-to-device copy of this many MiB (HBM-counter calibration "
                          "under rocprofv3 --pmc)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--c3-devices", type=int, default=4,
+                    help="uneven-batching sample: Si/Ge superlattice devices (one per block size 24/96/216/384 "
+                         "in turn), 64 frequencies x 2x2 k each, solved in one multi-device call (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the (omega,k) points shard across GPUs with no "
@@ -139,6 +142,44 @@ def main():
         return el, bool((r["status"] == 0).all())
 
     eladj, okadj = run_adjoint() if args.adjoint else (None, True)
+
+    def run_c3():
+        """uneven devices in one call, FP64 and mixed; per-device mixed error"""
+        specs = lt.superlattice_set(args.c3_devices, seed=3)
+        for i, sp in enumerate(specs):           # one of each block size in turn
+            sp["n"] = (1, 2, 3, 4)[i % 4]
+            sp["b"] = 24 * sp["n"] ** 2
+        ks = lt.mp_grid(2, 2)
+        f = 16.0 * np.arange(1, 65) / 64
+        devs, npts = [], 0
+        t0 = time.time()
+        for sp in specs:
+            blk = [lt.si_ge_superlattice_film(sp["n"], sp["period"], sp["L"], kf, seed=sp["seed"]) for kf in ks]
+            E = np.tile(f ** 2, len(ks))
+            kidx = np.repeat(np.arange(len(ks), dtype=np.int32), len(f))
+            devs.append(dict(blocks=eng.upload_blocks(blk), E=E, kidx=kidx))
+            npts += E.size
+        log(f"c3 sample: {len(devs)} devices built in {time.time() - t0:.1f}s")
+        res = {}
+        for prec in ("fp64", "mixed"):
+            p = T.params("phonon", precision=prec)
+            eng.solve_tdb_multi(devs, p)                      # warm-up (tunes the inverse per block size)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            outs = eng.solve_tdb_multi(devs, p)
+            el = time.perf_counter() - t0
+            res[prec] = (npts / el, [o["T_LR"].cpu().numpy() for o in outs],
+                         all(bool((o["status"] == 0).all()) for o in outs))
+            log(f"c3 {prec}: {npts / el:.1f} solves/s")
+        errs = []
+        for a, b in zip(res["fp64"][1], res["mixed"][1]):
+            sig = a > 1e-3 * max(1e-300, a.max())
+            errs.append(float(np.max(np.abs(b[sig] - a[sig]) / a[sig])) if sig.any() else 0.0)
+        return {"devices": [dict(b=sp["b"], L=sp["L"], period=sp["period"]) for sp in specs],
+                "points": npts, "fp64_solves_per_s": res["fp64"][0], "mixed_solves_per_s": res["mixed"][0],
+                "mixed_max_rel_dT_per_device": errs, "status_ok": res["fp64"][2] and res["mixed"][2]}
+
+    c3 = run_c3() if args.c3_devices > 0 else None
     t64 = np.concatenate(T64)
     tmx = np.concatenate(Tmx)
     sig = t64 > 1e-3 * max(1e-300, t64.max())
@@ -204,6 +245,8 @@ def main():
                         "solve_tflops": flop_solve * value / 1e12},
         "cpu_baseline": cpu,
     }
+    if c3 is not None:
+        out["uneven_batching"] = c3
     if eladj is not None:
         out["adjoint"] = {"grads_per_s": solves / eladj, "unit": "(omega,k)-gradients/s",
                           "what": "T_LR and dW/dH of the 3 device layers around the interface (bonds owned by "

@@ -205,6 +205,27 @@ int nt_solve_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, int nr, in
                              double* T_LR, double* T_RL, int* status, int* iters,
                              const nt_solve_extras* extras);
 
+/* Several devices of uneven block size b, lead sizes and layer count L in
+ * one call (every pointer of a descriptor is device memory, laid out as for
+ * nt_solve_tdb_batched_dev).  Devices are solved largest-first (cost
+ * b^3 (L + 2 N_dec) per point), each as one batched solve over its points;
+ * the inverse panel width is autotuned once per distinct block size.  Each
+ * descriptor's status array (may be NULL) receives its points' statuses;
+ * returns NT_OK or the first argument/HIP error.  No reference counterpart:
+ * the reference solves one device per run. */
+typedef struct {
+    int nl, nr, b, L, nk, npts;
+    const nt_c128 *Haml, *Hamlv, *Hamr, *Hamrv, *Hamd, *Hamdv, *Hamld, *Hamrd;
+    const double* E;
+    const int* kidx;
+    const int* first_point_mask;
+    double* T_LR;
+    double* T_RL;
+    int* status;
+} nt_device_desc;
+
+int nt_solve_tdb_multi_dev(nt_ctx* ctx, const nt_params* p, int ndev, const nt_device_desc* devs);
+
 /* Host-pointer variant (copies in and out; same layout). */
 int nt_solve_tdb_batched(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b, int L,
                          int npts, const double* E, const int* kidx, int nk,

@@ -2188,6 +2188,49 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* c, const nt_params* p, int nl, int 
     });
 }
 
+int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_device_desc* devs)
+{
+    if (!c || ndev < 0 || (ndev > 0 && !devs) || check_params(p)) return NT_ERR_ARG;
+    for (int d = 0; d < ndev; ++d) {
+        const nt_device_desc& D = devs[d];
+        if (check_circular(p, D.nl, D.nr)) return NT_ERR_ARG;
+        if (D.nl <= 0 || D.nr <= 0 || D.b <= 0 || D.L <= 0 || D.npts < 0 || D.nk <= 0 || (D.nk > 1 && !D.kidx))
+            return NT_ERR_ARG;
+        if (std::max(std::max(D.nl, D.nr), D.b) > inv_max_n()) return NT_ERR_ARG;
+        if (D.npts > 0 && (!D.E || !D.Haml || !D.Hamlv || !D.Hamr || !D.Hamrv || !D.Hamd || (D.L > 1 && !D.Hamdv) ||
+                           !D.Hamld || !D.Hamrd || !D.T_LR || !D.T_RL))
+            return NT_ERR_ARG;
+    }
+    // largest first: b^3 (L + 2 N_dec) per point, N_dec ~ 15
+    std::vector<int> order(ndev);
+    for (int d = 0; d < ndev; ++d) order[d] = d;
+    auto cost = [&](int d) {
+        const double b = devs[d].b;
+        return b * b * b * (devs[d].L + 30.0) * devs[d].npts;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::vector<std::pair<int, int>> tuned;   // (bytes, n) already tuned this call
+    for (int d : order) {
+        const nt_device_desc& D = devs[d];
+        if (D.npts == 0) continue;
+        const int bytes = p->precision == NT_PREC_FP64 ? 8 : 4;
+        for (int n : {D.b, std::max(D.nl, D.nr)}) {
+            if (c->inv_nb.count({bytes, n}) || std::find(tuned.begin(), tuned.end(), std::make_pair(bytes, n)) !=
+                                                    tuned.end())
+                continue;
+            const int tb = std::min(D.npts, c->batch_limit);
+            const int st = nt_autotune_inverse(c, n, std::max(1, std::min(tb, 256)), p->precision, nullptr, nullptr);
+            if (st) return st;
+            tuned.push_back({bytes, n});
+        }
+        const int st = nt_solve_tdb_batched_dev(c, p, D.nl, D.nr, D.b, D.L, D.npts, D.E, D.kidx, D.nk, D.Haml, D.Hamlv,
+                                                D.Hamr, D.Hamrv, D.Hamd, D.Hamdv, D.Hamld, D.Hamrd,
+                                                D.first_point_mask, D.T_LR, D.T_RL, D.status, nullptr, nullptr);
+        if (st < 0 && st != NT_ERR_NAN && st != NT_ERR_NOT_CONVERGED) return st;
+    }
+    return NT_OK;
+}
+
 int nt_solve_tdb_batched(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,
                          const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
                          const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,

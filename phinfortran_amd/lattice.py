@@ -406,3 +406,43 @@ def circulant_lead_blocks(na, nbx, nby, L, seed=0, scale=0.3, device_noise=0.1):
                   Hamdv=np.stack([V + c(n, n) * device_noise for _ in range(L - 1)]) if L > 1
                   else np.zeros((0, n, n), np.complex128),
                   Hamld=V.copy(), Hamrd=V.conj().T.copy())
+
+
+def si_ge_superlattice_film(n=1, period=2, L=10, kfrac=(0.0, 0.0), seed=0, bond_noise=0.02,
+                            alpha=48.5, beta=13.8):
+    """Si/Ge superlattice film: n x n conventional cells in-plane (b = 24 n^2),
+    periodic in-plane, device layers alternating `period` Si and `period` Ge
+    layers, seeded per-bond spring noise, pristine Si leads."""
+    rng = np.random.default_rng(seed)
+    layer = _layer_cache(n, n)
+    masses = np.array([[MASS_SI if (l // period) % 2 == 0 else MASS_GE] * layer.natom for l in range(L)])
+    springs = rng.uniform(1.0 - bond_noise, 1.0 + bond_noise, (L, layer.nbond))
+    return phonon_blocks(layer, masses, springs, MASS_SI, MASS_SI, kfrac, alpha, beta)
+
+
+_LAYERS = {}
+
+
+def _layer_cache(nx, ny):
+    key = (nx, ny)
+    if key not in _LAYERS:
+        _LAYERS[key] = DiamondLayer(nx, ny, periodic=True)
+    return _LAYERS[key]
+
+
+def superlattice_set(ndev=256, seed=0, sizes=(1, 2, 3, 4), periods=(2, 8), lengths=(10, 80)):
+    """Seeded device descriptions of the uneven-batching workload:
+    list of dict(n, b, period, L, seed)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for d in range(ndev):
+        n = int(rng.choice(sizes))
+        out.append(dict(n=n, b=24 * n * n, period=int(rng.integers(periods[0], periods[1] + 1)),
+                        L=int(rng.integers(lengths[0], lengths[1] + 1)), seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def mp_grid(nk1, nk2):
+    """Monkhorst-Pack transverse grid centred as negf/config.f90:156-175."""
+    return [(i / nk1 - 0.5 * (nk1 - 1) / nk1, j / nk2 - 0.5 * (nk2 - 1) / nk2)
+            for i in range(nk1) for j in range(nk2)]

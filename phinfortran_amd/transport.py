@@ -308,6 +308,41 @@ class Engine:
         return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters,
                     dHamd=gd.transpose(-1, -2), dHamdv=gv.transpose(-1, -2))
 
+    def solve_tdb_multi(self, devices, p: N.NtParams):
+        """Several devices (uneven b, L, leads) in one call.  devices: list of
+        dicts with 'blocks' (from upload_blocks), 'E' and optional 'kidx',
+        'first_point_mask'.  Returns a list of dict(T_LR, T_RL, status)."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        descs = (N.NtDeviceDesc * max(1, len(devices)))()
+        outs, keep = [], []
+        ptr = lambda t: None if t is None else t.data_ptr()
+        for i, d in enumerate(devices):
+            blk = d["blocks"]
+            E = torch.as_tensor(d["E"], dtype=torch.float64, device=dev).contiguous()
+            kidx = d.get("kidx")
+            kidx = None if kidx is None else torch.as_tensor(kidx, dtype=torch.int32, device=dev).contiguous()
+            mask = d.get("first_point_mask")
+            mask = None if mask is None else torch.as_tensor(mask, dtype=torch.int32, device=dev).contiguous()
+            n = E.numel()
+            o = dict(T_LR=torch.empty(n, dtype=torch.float64, device=dev),
+                     T_RL=torch.empty(n, dtype=torch.float64, device=dev),
+                     status=torch.zeros(n, dtype=torch.int32, device=dev))
+            outs.append(o)
+            keep += [E, kidx, mask]
+            D = descs[i]
+            D.nl, D.nr, D.b, D.L, D.nk, D.npts = blk["nl"], blk["nr"], blk["b"], blk["L"], blk["nk"], n
+            for name in ("Haml", "Hamlv", "Hamr", "Hamrv", "Hamd", "Hamdv", "Hamld", "Hamrd"):
+                setattr(D, name, ptr(blk[name]))
+            D.E, D.kidx, D.first_point_mask = ptr(E), ptr(kidx), ptr(mask)
+            D.T_LR, D.T_RL, D.status = ptr(o["T_LR"]), ptr(o["T_RL"]), ptr(o["status"])
+        if self._torch_stream:
+            self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+        self._check(self._L.nt_solve_tdb_multi_dev(self.ctx, C.byref(p), len(devices), C.cast(descs, C.c_void_p)),
+                    "solve_tdb_multi")
+        torch.cuda.synchronize(dev)
+        return outs
+
     MODE_FIELDS = ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "G_D_RL", "G_D_LR", "G_L", "G_R")
 
     def solve_modes(self, dblocks, E, p: N.NtParams, a_z_L: float, a_z_R: float, kidx=None, first_point_mask=None,

@@ -303,6 +303,20 @@ int nt_modes_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int tb, int nl, in
                              double a_z_L, double a_z_R, const int* first_point_mask,
                              const nt_modes_out* out, int* status);
 
+/* ------------------------------------------------ H(k) assembly
+ * gen_dyn_qe_tdb_k (negf/dyn.f90:1482-2080) on the GPU: one block for nk
+ * k-points.  The host tabulates the k-independent part as nseg segments (one
+ * per 3x3 atom-pair sub-block at atom row seg_rc[2s], atom column
+ * seg_rc[2s+1]) of entries seg_off[s] .. seg_off[s+1]-1, each with a 3x3
+ * real coefficient (row-major, coef[9 q + 3 alpha + beta]) = w Phi/sqrt(m m')
+ * and its image vector dr[3 q] (Angstrom).  For every k (kcart: nk x 3,
+ * 1/Angstrom) out[k] = sum coef exp(i k.dr), a rows x cols complex block with
+ * leading dimension ld at out + k * kstride (complex elements); entries of no
+ * segment are zero.  All pointers are device memory. */
+int nt_hk_assemble_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const int* seg_off, const int* seg_rc,
+                       int nent, const double* coef, const double* dr, int rows, int cols, nt_c128* out, int ld,
+                       long long kstride);
+
 /* ------------------------------------------------ autotuning
  * The batched inverse runs blocked Gauss-Jordan with a panel width of 8, 16 or
  * 32 columns; the widest that fits in LDS is the default.  nt_autotune_inverse

@@ -2856,6 +2856,50 @@ int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int 
     });
 }
 
+int nt_hk_assemble_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const int* seg_off, const int* seg_rc,
+                       int nent, const double* coef, const double* dr, int rows, int cols, nt_c128* out, int ld,
+                       long long kstride)
+{
+    if (!c || nk < 0 || nseg < 0 || nent < 0 || rows <= 0 || cols <= 0 || rows % 3 || cols % 3 || ld < rows ||
+        !out || (nk > 1 && kstride < (long long)ld * cols) || (nseg > 0 && (!seg_off || !seg_rc || !coef || !dr)) ||
+        (nk > 0 && !kcart))
+        return NT_ERR_ARG;
+    if (nk == 0) return NT_OK;
+    return guarded(c, [&] {
+        // host check of the segment table: offsets monotone within [0, nent],
+        // atom indices inside the block
+        if (nseg > 0) {
+            std::vector<int> off(nseg + 1), rc(2 * (size_t)nseg);
+            hipcheck(c, hipMemcpy(off.data(), seg_off, sizeof(int) * (nseg + 1), hipMemcpyDeviceToHost), "D2H");
+            hipcheck(c, hipMemcpy(rc.data(), seg_rc, sizeof(int) * 2 * nseg, hipMemcpyDeviceToHost), "D2H");
+            if (off[0] != 0 || off[nseg] != nent) {
+                c->err = "hk: segment offsets must run from 0 to nent";
+                return (int)NT_ERR_ARG;
+            }
+            for (int q = 0; q < nseg; ++q)
+                if (off[q + 1] < off[q] || rc[2 * q] < 0 || 3 * rc[2 * q] + 3 > rows || rc[2 * q + 1] < 0 ||
+                    3 * rc[2 * q + 1] + 3 > cols) {
+                    c->err = "hk: segment outside the block";
+                    return (int)NT_ERR_ARG;
+                }
+        }
+        const size_t z = sizeof(nt_c128);
+        reg_region(c, kcart, sizeof(double) * 3 * nk);
+        reg_region(c, seg_off, sizeof(int) * (nseg + 1));
+        reg_region(c, seg_rc, sizeof(int) * 2 * nseg);
+        reg_region(c, coef, sizeof(double) * 9 * (size_t)nent);
+        reg_region(c, dr, sizeof(double) * 3 * (size_t)nent);
+        const size_t span = z * ((size_t)(nk - 1) * kstride + (size_t)ld * (cols - 1) + rows);
+        reg_region(c, out, span);
+        check_range(c, kcart, sizeof(double) * 3 * nk, "hk k");
+        launch(c, NT_K_ELEM, 0.0, "hk assemble", [&] {
+            return launch_hk_assemble(nk, kcart, nseg, seg_off, seg_rc, coef, dr, rows, cols,
+                                      reinterpret_cast<double*>(out), ld, (long)kstride, c->stream);
+        });
+        return 0;
+    });
+}
+
 int nt_zgeev_batched_dev(nt_ctx* c, int n, const nt_c128* A, long long strideA, nt_c128* w, nt_c128* V,
                          long long strideV, int* status, int batch)
 {

@@ -79,6 +79,13 @@ hipError_t launch_scale_items(double* M, long stride, int ld, int m, int n, cons
 hipError_t launch_mode_sums(const double* S, long stride, int ld, int m, int n, const double* kz_row,
                             const double* kz_col, double* col_sum, double* total, int batch, hipStream_t s);
 
+// H(k) phase sums of the QE force-constant path (nt_hk.hip): out[k] (rows x cols,
+// leading dimension ld, k stride kstride complex elements) = sum over the
+// entries of each 3x3 segment of coef * exp(i dr . k); zero where no segment.
+hipError_t launch_hk_assemble(int nk, const double* kc, int nseg, const int* seg_off, const int* seg_rc,
+                              const double* coef, const double* dr, int rows, int cols, double* out, int ld,
+                              long kstride, hipStream_t s);
+
 // Elementwise helpers (nt_elem.hip).
 // out[p] = sign * (dagger ? M^+ : M) for an n x n matrix
 template <typename T>

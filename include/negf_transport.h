@@ -329,6 +329,10 @@ int nt_set_inverse_panel(nt_ctx* ctx, int n, int precision, int nb);
 /* on = 1 (default): the right-to-left RGF sweep runs on a second stream of
  * the context beside the left-to-right one; on = 0: both on the main stream. */
 int nt_set_overlap(nt_ctx* ctx, int on);
+/* on = 1: FP64 complex GEMMs form each complex product from three real MFMA
+ * products ((Ar+Ai)(Br+Bi) - ArBr - AiBi for the imaginary part) instead of
+ * four; on = 0 (default): four. */
+int nt_set_gemm_3m(nt_ctx* ctx, int on);
 
 /* ------------------------------------------------ batched building blocks
  * The two kernels every solve is made of, on device memory (column-major,

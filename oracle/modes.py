@@ -76,44 +76,53 @@ def _modes(F, lambda_eta, az, sign):
 
 
 def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R,
-                Haml, Hamr, Hamlv, Hamrv, eta, eta0, lambda_eta, a_z_L, a_z_R, first_point=False):
+                Haml, Hamr, Hamlv, Hamrv, eta, eta0, lambda_eta, a_z_L, a_z_R, first_point=False, tb=False):
     """Mode-resolved transmission and reflection of one (omega, k) point.
+
+    tb=True follows tb/velocity_mat.f90: velocity matrices without the
+    i a/(2 sqrt E) factor (:467-468), t and r with prefactor i (:1352-1370),
+    constant lead broadening eta in Q (:1315).
 
     Returns dict with trl (nr x nl, left -> right), tlr, rrr, rll, the four
     k_z arrays (NOT_PROP for evanescent modes), the propagating counts, the
     per-mode sums t_mode_L[i] = sum_j |trl(j,i)|^2 and t_mode_R[j]
     (velocity_mat.f90:3036-3041)."""
-    c = 1j * 0.5 / np.sqrt(E)
+    # velocity factors s_L, s_R and t/r prefactors (negf: i a/(2 sqrt E) and
+    # 2 i sqrt(E/(a a')); tb: 1 and i)
+    sL = 1.0 if tb else 1j * a_z_L / (2.0 * np.sqrt(E))
+    sR = 1.0 if tb else 1j * a_z_R / (2.0 * np.sqrt(E))
+    f = (lambda a, b: 1j) if tb else (lambda a, b: 1j * 2.0 * np.sqrt(E / a / b))
+
     # left advanced, incident from the left (:1835-2012)
     lam_al, U_al, kz_al, n_al = _modes(gl_adv @ Hamlv, lambda_eta, a_z_L, -1.0)
-    V_al = -(U_al.conj().T @ gml_adv @ U_al) * c * a_z_L
+    V_al = -(U_al.conj().T @ gml_adv @ U_al) * sL
     inv_U_al = np.linalg.inv(U_al.conj().T)
     Vh_al = sqrtm(V_al)
     # left retarded, leaving to the left (:2131-2303)
     lam_rl, U_rl, kz_rl, n_rl = _modes(gl_adv.conj().T @ Hamlv, lambda_eta, a_z_L, -1.0)
-    V_rl = -(U_rl.conj().T @ gml_adv.conj().T @ U_rl) * c * a_z_L
+    V_rl = -(U_rl.conj().T @ gml_adv.conj().T @ U_rl) * sL
     inv_U_rl = np.linalg.inv(U_rl)
     Vh_rl = sqrtm(V_rl)
     # right retarded, leaving to the right (:2394-2576)
     lam_rr, U_rr, kz_rr, n_rr = _modes(gr_ret @ Hamrv, lambda_eta, a_z_R, 1.0)
-    V_rr = (U_rr.conj().T @ gmr_ret @ U_rr) * c * a_z_R
+    V_rr = (U_rr.conj().T @ gmr_ret @ U_rr) * sR
     inv_U_rr = np.linalg.inv(U_rr)
     Vh_rr = sqrtm(V_rr)
     # right advanced, incident from the right (:2657-2824)
     lam_ar, U_ar, kz_ar, n_ar = _modes(gr_ret.conj().T @ Hamrv, lambda_eta, a_z_R, 1.0)
-    V_ar = (U_ar.conj().T @ gmr_ret.conj().T @ U_ar) * c * a_z_R
+    V_ar = (U_ar.conj().T @ gmr_ret.conj().T @ U_ar) * sR
     inv_U_ar = np.linalg.inv(U_ar.conj().T)
     Vh_ar = sqrtm(V_ar)
     # bulk GFs of the leads (:2893-2938), lead eta
-    z = E + 1j * (E * eta + eta0)
+    z = E + 1j * (eta if tb else E * eta + eta0)
     nl, nr = Haml.shape[0], Hamr.shape[0]
     Q_L = z * np.eye(nl) - Haml - Hamlv.conj().T @ gl_adv.conj().T @ Hamlv - Hamlv @ gl_plus @ Hamlv.conj().T
     Q_R = z * np.eye(nr) - Hamr - Hamrv.conj().T @ gr_ret @ Hamrv - Hamrv @ gr_minus.conj().T @ Hamrv.conj().T
     # t and r (:2945-2963)
-    trl = 1j * (Vh_rr @ (inv_U_rr @ (G_D_RL @ (inv_U_al @ Vh_al)))) * 2.0 * np.sqrt(E / a_z_R / a_z_L)
-    tlr = 1j * (Vh_rl @ (inv_U_rl @ (G_D_LR @ (inv_U_ar @ Vh_ar)))) * 2.0 * np.sqrt(E / a_z_L / a_z_R)
-    rrr = 1j * (Vh_rr @ (inv_U_rr @ ((G_R - np.linalg.inv(Q_R)) @ (inv_U_ar @ Vh_ar)))) * 2.0 * np.sqrt(E / a_z_R ** 2)
-    rll = 1j * (Vh_rl @ (inv_U_rl @ ((G_L - np.linalg.inv(Q_L)) @ (inv_U_al @ Vh_al)))) * 2.0 * np.sqrt(E / a_z_L ** 2)
+    trl = f(a_z_R, a_z_L) * (Vh_rr @ (inv_U_rr @ (G_D_RL @ (inv_U_al @ Vh_al))))
+    tlr = f(a_z_L, a_z_R) * (Vh_rl @ (inv_U_rl @ (G_D_LR @ (inv_U_ar @ Vh_ar))))
+    rrr = f(a_z_R, a_z_R) * (Vh_rr @ (inv_U_rr @ ((G_R - np.linalg.inv(Q_R)) @ (inv_U_ar @ Vh_ar))))
+    rll = f(a_z_L, a_z_L) * (Vh_rl @ (inv_U_rl @ ((G_L - np.linalg.inv(Q_L)) @ (inv_U_al @ Vh_al))))
     if first_point:                         # :2965-2970
         trl, tlr, rrr, rll = (np.zeros_like(x) for x in (trl, tlr, rrr, rll))
     a2 = lambda x: np.abs(x) ** 2
@@ -126,7 +135,7 @@ def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, 
                 R_modes_R=float(a2(rrr).sum()), R_modes_L=float(a2(rll).sum()))
 
 
-def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=False):
+def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=False, tb=False):
     """Full oracle chain for one point: surface GFs and get_G_full_tdb of the
     C oracle, then vel_mat_tdb with the driver's argument mapping
     (negf/phinfortran.f90:1238-1250)."""
@@ -138,6 +147,6 @@ def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=Fals
               sg["sr_ret"], E, params)
     m = vel_mat_tdb(E, sg["gl_adv"], sg["gr_ret"], sg["gl_plus"], sg["gr_minus"], sg["gml_adv"], sg["gmr_ret"],
                     r["G_D_RL"], r["G_D_LR"], r["G_L"], r["G_R"], d["Haml"], d["Hamr"], d["Hamlv"], d["Hamrv"],
-                    params.eta, params.eta0, lambda_eta, a_z_L, a_z_R, first_point)
+                    params.eta, params.eta0, lambda_eta, a_z_L, a_z_R, first_point, tb)
     m["T_LR"], m["T_RL"] = r["T_LR"], r["T_RL"]
     return m

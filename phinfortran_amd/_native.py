@@ -34,7 +34,7 @@ EXPORTS = [
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
     "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
-    "nt_set_overlap",
+    "nt_set_overlap", "nt_set_gemm_3m",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
 ]
@@ -129,6 +129,7 @@ def lib():
     L.nt_autotune_inverse.argtypes = [P, I, I, I, C.POINTER(I), C.POINTER(D)]
     L.nt_set_inverse_panel.argtypes = [P, I, I, I]
     L.nt_set_overlap.argtypes = [P, I]
+    L.nt_set_gemm_3m.argtypes = [P, I]
     L.nt_profile_enable.argtypes = [P, I]
     L.nt_profile_get.argtypes = [P, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D)]
     L.nt_profile_reset.argtypes = [P]

@@ -33,6 +33,7 @@ struct nt_ctx {
     hipStream_t aux = nullptr;     // second stream: the right-to-left sweep runs beside the left-to-right one
     hipEvent_t fork = nullptr, join = nullptr;
     bool overlap = true;
+    bool gemm_m3 = false;          // FP64 GEMMs with three real products per complex product
     hipStream_t stream = nullptr;
     std::string err;
     int batch_limit = 512;
@@ -256,6 +257,7 @@ void gemm(nt_ctx* c, int M, int N, int K, int opA, BMat<T> A, int opB, BMat<T> B
     a.maxabs = maxabs;
     a.batch = batch;
     a.groups = groups;
+    a.m3 = (sizeof(T) == 8 && c->gemm_m3) ? 1 : 0;
     const double items = (nitems < 0 ? (double)batch : nitems) * groups;
     audit<T>(c, A, opA == OP_N ? M : K, opA == OP_N ? K : M, batch, groups, "gemm A");
     audit<T>(c, B, opB == OP_N ? K : N, opB == OP_N ? N : K, batch, groups, "gemm B");
@@ -2706,6 +2708,13 @@ int nt_autotune_inverse(nt_ctx* c, int n, int batch, int precision, int* best_nb
     });
 }
 
+int nt_set_gemm_3m(nt_ctx* c, int on)
+{
+    if (!c || (on != 0 && on != 1)) return NT_ERR_ARG;
+    c->gemm_m3 = on == 1;
+    return NT_OK;
+}
+
 int nt_set_overlap(nt_ctx* c, int on)
 {
     if (!c || (on != 0 && on != 1)) return NT_ERR_ARG;
@@ -2981,6 +2990,7 @@ int nt_zgemm_batched_dev(nt_ctx* c, char opA, char opB, int m, int n, int k, nt_
         a.maxabs = nullptr;
         a.batch = batch;
         a.groups = 1;
+        a.m3 = c->gemm_m3 ? 1 : 0;
         launch(c, NT_K_GEMM, 8.0 * m * n * k * (double)batch, "gemm", [&] { return launch_gemm<T>(a, c->stream); });
         return 0;
     });

@@ -136,8 +136,10 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, doub
     atomicMax(addr, (unsigned long long)__double_as_longlong(v));
 }
 
-template <typename T>
-__global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
+// M3: complex products by three real products (P1 = Ar Br, P2 = Ai Bi,
+// P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2) instead of four.
+template <typename T, bool M3>
+__global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
 {
     typedef typename acc4<T>::type A4;
     const int tiles_m = (a.M + BM - 1) / BM;
@@ -161,13 +163,14 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
 
-    A4 accR[2][2], accI[2][2];
+    A4 accR[2][2], accI[2][2], accS[2][2];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
             accR[x][y] = A4{0, 0, 0, 0};
             accI[x][y] = A4{0, 0, 0, 0};
+            accS[x][y] = A4{0, 0, 0, 0};
         }
 
     T ra[4], ia[4], rb[4], ib[4];
@@ -197,16 +200,33 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
                 yr[y] = s.br[k][wn + 16 * y + lr];
                 yi[y] = s.bi[k][wn + 16 * y + lr];
             }
+            if constexpr (M3) {
+                T xs[2], ys[2];
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+                for (int x = 0; x < 2; ++x) xs[x] = xr[x] + xi[x];
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
-                    // (C^T) = B^T A^T: MFMA "A" operand is B, "B" operand is A.
-                    accR[x][y] = mfma16x16x4(yr[y], xr[x], accR[x][y]);
-                    accR[x][y] = mfma16x16x4(-yi[y], xi[x], accR[x][y]);
-                    accI[x][y] = mfma16x16x4(yi[y], xr[x], accI[x][y]);
-                    accI[x][y] = mfma16x16x4(yr[y], xi[x], accI[x][y]);
-                }
+                for (int y = 0; y < 2; ++y) ys[y] = yr[y] + yi[y];
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+                        // accR = P1, accI = P2, accS = P3
+                        accR[x][y] = mfma16x16x4(yr[y], xr[x], accR[x][y]);
+                        accI[x][y] = mfma16x16x4(yi[y], xi[x], accI[x][y]);
+                        accS[x][y] = mfma16x16x4(ys[y], xs[x], accS[x][y]);
+                    }
+            } else {
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+                        // (C^T) = B^T A^T: MFMA "A" operand is B, "B" operand is A.
+                        accR[x][y] = mfma16x16x4(yr[y], xr[x], accR[x][y]);
+                        accR[x][y] = mfma16x16x4(-yi[y], xi[x], accR[x][y]);
+                        accI[x][y] = mfma16x16x4(yi[y], xr[x], accI[x][y]);
+                        accI[x][y] = mfma16x16x4(yr[y], xi[x], accI[x][y]);
+                    }
+            }
         }
         __syncthreads();
         if (more) {
@@ -235,7 +255,15 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<T> a)
                 const int m = m0 + wm + 16 * x + lr;
                 const int n = n0 + wn + 16 * y + mfma_out_row<T>(lane, r);
                 if (m < a.M && n < a.N) {
-                    const T pr = accR[x][y][r], pi = accI[x][y][r];
+                    T pr, pi;
+                    if constexpr (M3) {
+                        const T p1 = accR[x][y][r], p2 = accI[x][y][r], p3 = accS[x][y][r];
+                        pr = p1 - p2;
+                        pi = p3 - p1 - p2;
+                    } else {
+                        pr = accR[x][y][r];
+                        pi = accI[x][y][r];
+                    }
                     T vr = a.alpha_re * pr - a.alpha_im * pi;
                     T vi = a.alpha_re * pi + a.alpha_im * pr;
                     T cr = vr, ci = vi;
@@ -285,7 +313,10 @@ hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s)
     const long tiles = (long)((a.M + BM - 1) / BM) * ((a.N + BN - 1) / BN);
     const long blocks = tiles * (long)a.batch * a.groups;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gemm<T>, dim3((unsigned)blocks), dim3(NT), 0, s, a);
+    if (a.m3)
+        hipLaunchKernelGGL((k_gemm<T, true>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_gemm<T, false>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
 

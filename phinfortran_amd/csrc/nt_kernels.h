@@ -20,6 +20,7 @@ template <typename T> struct GemmArgs {
     const int* active;      // per item p, or null
     unsigned long long* maxabs;  // per item p (bits of a non-negative double), or null
     int batch, groups;
+    int m3 = 0;             // 1: three real products per complex product
 };
 
 template <typename T> hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s);

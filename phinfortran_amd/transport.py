@@ -404,6 +404,10 @@ class Engine:
                     "nt_autotune_inverse")
         return nb.value, ms.value
 
+    def set_gemm_3m(self, on: bool):
+        """FP64 GEMMs with three real products per complex product."""
+        self._check(self._L.nt_set_gemm_3m(self.ctx, int(bool(on))), "nt_set_gemm_3m")
+
     def set_overlap(self, on: bool):
         """Run the two RGF sweeps on two streams (default) or serially."""
         self._check(self._L.nt_set_overlap(self.ctx, int(bool(on))), "nt_set_overlap")

@@ -155,3 +155,35 @@ def test_zinv_every_panel_width(engine, torch_dev, n):
     nb, ms = engine.autotune_inverse(n, 64)
     assert nb in widths and ms > 0
     engine.set_inverse_panel(n, 0)
+
+
+@pytest.mark.parametrize("m,n,k", [(5, 5, 5), (37, 37, 37), (192, 192, 192), (250, 130, 70)])
+@pytest.mark.parametrize("ops", ["NN", "CC", "NC"])
+def test_zgemm_three_products(engine, torch_dev, m, n, k, ops):
+    """the 3-multiplication complex product agrees with numpy to FP64 rounding"""
+    from phinfortran_amd import _native as N
+    torch, dev = torch_dev
+    rng = np.random.default_rng(m + 7 * n + 13 * k)
+    batch = 2
+    A = [crand(rng, *((m, k) if ops[0] == "N" else (k, m))) for _ in range(batch)]
+    B = [crand(rng, *((k, n) if ops[1] == "N" else (n, k))) for _ in range(batch)]
+    C0 = [crand(rng, m, n) for _ in range(batch)]
+    alpha, beta = 0.7 - 0.2j, -0.4 + 1.1j
+    dA, dB, dC = to_dev(torch, dev, A), to_dev(torch, dev, B), to_dev(torch, dev, C0)
+    engine.set_gemm_3m(True)
+    try:
+        st = N.lib().nt_zgemm_batched_dev(engine.ctx, ops[0].encode(), ops[1].encode(), m, n, k,
+                                          N.NtC128(alpha.real, alpha.imag), C.c_void_p(dA.data_ptr()),
+                                          A[0].shape[0], A[0].size, C.c_void_p(dB.data_ptr()), B[0].shape[0],
+                                          B[0].size, N.NtC128(beta.real, beta.imag), C.c_void_p(dC.data_ptr()), m,
+                                          m * n, batch)
+        engine.sync()
+    finally:
+        engine.set_gemm_3m(False)
+    assert st == 0
+    got = from_dev(dC)
+    for i in range(batch):
+        a = A[i] if ops[0] == "N" else A[i].conj().T
+        b = B[i] if ops[1] == "N" else B[i].conj().T
+        ref = alpha * a @ b + beta * C0[i]
+        assert np.abs(got[i] - ref).max() < 2e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5

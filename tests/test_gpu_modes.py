@@ -124,3 +124,24 @@ def test_si_si_per_mode_tables_gpu(engine, T):
             nmodes += len(inc)
     print(f"si-si per-mode: {nmodes} modes, max |dk_z| {dk:.2e}, max |dT_n| {dt:.2e}")
     assert dk < 1e-7 and dt < 2e-6
+
+
+def test_tb_ribbon_vs_oracle(engine, T):
+    """tight-binding conventions (tb=1) on a disordered armchair ribbon"""
+    blk = lt.agnr_blocks(n_dimer=4, L=3, seed=2)
+    p = T.params("tb", eta=1e-7, convergence=1e-13, max_iter=3000)
+    op = oparams(p)
+    E = np.array([-1.3, -0.4, 0.45, 1.1])
+    d = engine.upload_blocks(blk)
+    print("modes tb ribbon", flush=True)
+    r = engine.solve_modes(d, E, p, 1.0, 1.0, tb=True)
+    assert (r["modes_status"].cpu().numpy() == 0).all()
+    sums = r["sums"].cpu().numpy()
+    npr = r["n_prop"].cpu().numpy()
+    for i in range(len(E)):
+        m = M.solve_point_modes(blk, E[i], op, p.lambda_eta, 1.0, 1.0, tb=True)
+        assert list(npr[i]) == [m["n_prop"][k] for k in ("L_adv", "L_ret", "R_ret", "R_adv")]
+        inc, out = m["k_z_L_adv"] < 99, m["k_z_R_ret"] < 99
+        ref = (np.abs(m["trl"][np.ix_(out, inc)]) ** 2).sum()
+        assert abs(sums[i][0] - ref) < 1e-6 * max(1.0, ref), (i, sums[i][0], ref)
+        assert abs(sums[i][0] - r["T_RL"][i].item()) < 1e-4 * max(1.0, ref)

@@ -121,3 +121,20 @@ def test_si_si_per_mode_tables(si, ik):
         assert np.abs(rt[order] - g[:, 1]).max() < 2e-6, (ik, ie, rt[order], g[:, 1])
         checked += 1
     assert checked > 5
+
+
+@pytest.mark.parametrize("E", [-1.5, 0.3, 1.9])
+def test_tb_chain_one_mode(E):
+    """single-orbital TB chain (hopping t = -1): one mode, k_z = acos(-E/2),
+    |t|^2 = T = 1 in the band [-2, 2] (tight-binding conventions)"""
+    t = -1.0
+    blk = lt.Blocks(Haml=np.zeros((1, 1), complex), Hamlv=np.full((1, 1), t, complex),
+                    Hamr=np.zeros((1, 1), complex), Hamrv=np.full((1, 1), t, complex),
+                    Hamd=np.zeros((3, 1, 1), complex), Hamdv=np.full((2, 1, 1), t, complex),
+                    Hamld=np.full((1, 1), t, complex), Hamrd=np.full((1, 1), t, complex))
+    p = O.OrParams(1e-9, 0.0, 1e-14, 1.0, 1, 4000)
+    m = M.solve_point_modes(blk, E, p, 5e-3, 1.0, 1.0, tb=True)
+    assert m["n_prop"]["L_adv"] == 1 and m["n_prop"]["R_ret"] == 1
+    kz = m["k_z_L_adv"][m["k_z_L_adv"] < 99]
+    assert abs(abs(kz[0]) - math.acos(-E / 2)) < 1e-8
+    assert abs(abs(m["trl"][0, 0]) ** 2 - 1) < 1e-6 and abs(m["T_RL"] - 1) < 1e-6

@@ -18,6 +18,7 @@ constexpr int BM = 64;
 constexpr int BN = 64;
 constexpr int BK = 16;
 constexpr int NT = 256;
+constexpr int NXCD = 8;       // accelerator complex dies of the MI355X
 
 template <typename T>
 struct GemmSmem {
@@ -145,7 +146,15 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
     const int tiles_m = (a.M + BM - 1) / BM;
     const int tiles_n = (a.N + BN - 1) / BN;
     const int tiles = tiles_m * tiles_n;
-    const long bid = blockIdx.x;
+    // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+    // L2.  When the item count divides by 8, renumber so that XCD x runs a
+    // contiguous range of logical blocks: all tiles of one item then share
+    // that XCD's L2 and its A and B panels are fetched from HBM once.
+    long bid = blockIdx.x;
+    if (((long)a.batch * a.groups) % NXCD == 0) {
+        const long per = (long)gridDim.x / NXCD;
+        bid = (bid % NXCD) * per + bid / NXCD;
+    }
     const int tile = (int)(bid % tiles);
     const long item = bid / tiles;
     const int p = (int)(item / a.groups);

@@ -187,3 +187,29 @@ def test_zgemm_three_products(engine, torch_dev, m, n, k, ops):
         b = B[i] if ops[1] == "N" else B[i].conj().T
         ref = alpha * a @ b + beta * C0[i]
         assert np.abs(got[i] - ref).max() < 2e-12 * max(1.0, np.abs(ref).max()) * max(1, k) ** 0.5
+
+
+@pytest.mark.parametrize("batch,groups_like", [(8, 1), (16, 1), (24, 1)])
+def test_zgemm_batch_multiple_of_xcds(engine, torch_dev, batch, groups_like):
+    """batches divisible by the XCD count take the renumbered tile order"""
+    from phinfortran_amd import _native as N
+    torch, dev = torch_dev
+    rng = np.random.default_rng(batch)
+    m, n, k = 130, 150, 70
+    A = [crand(rng, m, k) for _ in range(batch)]
+    B = [crand(rng, k, n) for _ in range(batch)]
+    C0 = [crand(rng, m, n) for _ in range(batch)]
+    dA, dB, dC = to_dev(torch, dev, A), to_dev(torch, dev, B), to_dev(torch, dev, C0)
+    for m3 in (False, True):
+        engine.set_gemm_3m(m3)
+        dC.copy_(to_dev(torch, dev, C0))
+        st = N.lib().nt_zgemm_batched_dev(engine.ctx, b"N", b"N", m, n, k, N.NtC128(1.0, 0.0),
+                                          C.c_void_p(dA.data_ptr()), m, m * k, C.c_void_p(dB.data_ptr()), k, k * n,
+                                          N.NtC128(1.0, 0.0), C.c_void_p(dC.data_ptr()), m, m * n, batch)
+        engine.sync()
+        engine.set_gemm_3m(False)
+        assert st == 0
+        got = from_dev(dC)
+        for i in range(batch):
+            ref = A[i] @ B[i] + C0[i]
+            assert np.abs(got[i] - ref).max() < 2e-12 * np.abs(ref).max() * k ** 0.5, (m3, i)

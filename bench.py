@@ -26,6 +26,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def traffic_per_launch(kernel, prec, b, points):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 --pmc summary of the same workload (FETCH_SIZE and WRITE_SIZE
+    per dispatch, each scaled by the counter's reading for a copy of known
+    size); None when no summary covers this kernel and shape."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_b{b}_{prec}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", txt)
+    p = re.search(r"points_per_launch:\s*(\d+)", txt)
+    if not m or not p or int(p.group(1)) != points:
+        return None
+    return float(m.group(1))
+
+
 def kgrid(n):
     """Monkhorst-Pack style grid centred like negf/config.f90:156-175."""
     ks = []
@@ -235,7 +252,8 @@ def main():
                    "block": b, "layers": L, "points_per_step": args.points, "precision": prec_used,
                    "parallelism": "single GPU, batched over frequencies"},
         "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": traffic_per_launch(dom_name, prec_used, b, args.points)},
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,
                               "mixed_speedup": vmx / v64, "mixed_max_rel_dT": rel_err,
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",

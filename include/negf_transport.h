@@ -319,7 +319,7 @@ int nt_hk_assemble_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const
 
 /* ------------------------------------------------ autotuning
  * The batched inverse runs blocked Gauss-Jordan with a panel width of 8, 16 or
- * 32 columns; the widest that fits in LDS is the default.  nt_autotune_inverse
+ * 32 columns; 16 is the default where it fits (else 8).  nt_autotune_inverse
  * times every supported width on `batch` n x n matrices and keeps the fastest
  * for this context (precision NT_PREC_FP64: complex double, NT_PREC_MIXED:
  * complex float); best_nb / best_ms may be NULL.  nt_set_inverse_panel sets a

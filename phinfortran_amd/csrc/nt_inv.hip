@@ -419,7 +419,11 @@ hipError_t launch_inv_any(const InvArgs<T>& a, hipStream_t s)
     if (a.n > inv_max_n()) return hipErrorInvalidValue;
     int w[3];
     const int k = inv_panel_widths<T>(a.n, w);
-    int nb = w[0];                       // size rule: widest panel that fits
+    // default: 16 columns where it fits (measured fastest at b = 192: two
+    // workgroups per CU against one with 32), else the widest that fits
+    int nb = w[0];
+    for (int i = 0; i < k; ++i)
+        if (w[i] == 16) nb = 16;
     for (int i = 0; i < k; ++i)
         if (a.nb == w[i]) nb = a.nb;
     if (nb == 32) return launch_inv_nb<T, 32, 1>(a, s);

@@ -253,7 +253,11 @@ def main():
                    "parallelism": "single GPU, batched over frequencies"},
         "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": traffic_per_launch(dom_name, prec_used, b, args.points)},
+                     "traffic": traffic_per_launch(dom_name, prec_used, b, args.points),
+                     "launches": dom["launches"], "avg_launch_ms": avg_ms,
+                     "timing": "HIP events around every launch on its own stream; the two RGF sweeps run "
+                               "concurrently on two streams, so a launch's duration includes the time it "
+                               "shares the GPU with the other sweep's kernels"},
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,
                               "mixed_speedup": vmx / v64, "mixed_max_rel_dT": rel_err,
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",

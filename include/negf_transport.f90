@@ -39,6 +39,29 @@ module negf_transport
         type(c_ptr) :: n_prop = c_null_ptr, t_mode_L = c_null_ptr, t_mode_R = c_null_ptr, sums = c_null_ptr
     end type nt_modes_out
 
+    ! Brillouin-zone unfolding inputs (nt_unfold_desc): the reciprocal
+    ! vectors row-major (pass reshape(transpose(recivec), [9]) of the
+    ! reference's recivec(3,3)), device pointers to ksc (transpose(k_grid)),
+    ! the orbital positions (transpose(pos_l(:,1:3))) and the cell-atom
+    ! positions positions(L_list(pidxL(o)),:) of the first image's orbitals.
+    type, bind(C) :: nt_unfold_desc
+        integer(c_int) :: n_bloch_x = 1, n_bloch_y = 1
+        real(c_double) :: recivec(9) = 0.0d0, reci_uc_l(9) = 0.0d0, reci_uc_r(9) = 0.0d0
+        type(c_ptr) :: ksc = c_null_ptr, pos_l = c_null_ptr, pos_uc_l = c_null_ptr
+        type(c_ptr) :: pos_r = c_null_ptr, pos_uc_r = c_null_ptr
+    end type nt_unfold_desc
+
+    ! Unfolding outputs (device pointers, any may be null): bz_idx_*, k_xyz_*
+    ! (n x 3 per point, the reference's k_xyz_* layout), the eight split
+    ! sums per point and the (total, specular) sums per incident mode.
+    type, bind(C) :: nt_unfold_out
+        type(c_ptr) :: bz_L_adv = c_null_ptr, bz_L_ret = c_null_ptr, bz_R_ret = c_null_ptr, bz_R_adv = c_null_ptr
+        type(c_ptr) :: k_xyz_L_adv = c_null_ptr, k_xyz_L_ret = c_null_ptr
+        type(c_ptr) :: k_xyz_R_ret = c_null_ptr, k_xyz_R_adv = c_null_ptr
+        type(c_ptr) :: split = c_null_ptr
+        type(c_ptr) :: spec_trl = c_null_ptr, spec_tlr = c_null_ptr, spec_rrr = c_null_ptr, spec_rll = c_null_ptr
+    end type nt_unfold_out
+
     interface
         subroutine nt_params_default(p) bind(C, name="nt_params_default")
             import :: nt_params
@@ -135,6 +158,25 @@ module negf_transport
             real(c_double), value :: a_z_L, a_z_R
             type(c_ptr), value :: first_point_mask
             type(nt_modes_out), intent(in) :: out
+            type(c_ptr), value :: status
+        end function
+
+        ! vel_mat_tdb with the Brillouin-zone unfolding (velocity_mat.f90:2020-2130, :3032-3099)
+        integer(c_int) function nt_modes_unfold_tdb_batched_dev(ctx, p, tb, nl, nr, npts, E, kidx, nk, Haml, &
+                Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, &
+                G_R, a_z_L, a_z_R, first_point_mask, out, uf, uo, status) &
+                bind(C, name="nt_modes_unfold_tdb_batched_dev")
+            import :: c_int, c_ptr, c_double, nt_params, nt_modes_out, nt_unfold_desc, nt_unfold_out
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            integer(c_int), value :: tb, nl, nr, npts, nk
+            type(c_ptr), value :: E, kidx, Haml, Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus, gr_minus
+            type(c_ptr), value :: gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R
+            real(c_double), value :: a_z_L, a_z_R
+            type(c_ptr), value :: first_point_mask
+            type(nt_modes_out), intent(in) :: out
+            type(nt_unfold_desc), intent(in) :: uf
+            type(nt_unfold_out), intent(in) :: uo
             type(c_ptr), value :: status
         end function
 

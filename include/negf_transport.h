@@ -267,8 +267,8 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, in
 
 /* ------------------------------------------------ mode-resolved transmission
  * vel_mat_tdb (negf/velocity_mat.f90:1692-3150; tb/velocity_mat.f90:222-1560)
- * for npts points, without the Brillouin-zone unfolding that splits specular
- * and non-specular parts.  Inputs are the outputs of the batched solve for
+ * for npts points (the Brillouin-zone unfolding that splits specular and
+ * non-specular parts: nt_modes_unfold_tdb_batched_dev below).  Inputs are the outputs of the batched solve for
  * the same points (nt_solve_extras: gl_adv, gr_ret, gl_plus, gr_minus,
  * gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R; npts x n x n each), the lead
  * blocks per k-point (kidx as in the batched solve) and the lead periods
@@ -302,6 +302,76 @@ int nt_modes_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int tb, int nl, in
                              const nt_c128* G_L, const nt_c128* G_R,
                              double a_z_L, double a_z_R, const int* first_point_mask,
                              const nt_modes_out* out, int* status);
+
+/* ------------------------------------------------ Brillouin-zone unfolding
+ * The specular / non-specular labelling of vel_mat_tdb: every propagating
+ * mode gets the in-plane k-shift j (1 .. n_bloch_x n_bloch_y, numbered as
+ * k_shift in negf/config.f90:176-183) of largest weight
+ *   W(i,j) = sum_o |sum_ir exp(-i(-kc.r + kp_j.(r - r_uc(o)))) U(o + ir nl/nb, i) lambda_i|^2 / nb
+ * (negf/velocity_mat.f90:2035-2076, mirrored at :2321-2355, :2578-2614,
+ * :2826-2862), and the unfolded k = move_in_ws2((ksc + shift_j) . recivec)
+ * (negf/util.f90:675-706) with k_z as third component.  The lead geometry is
+ * what transverse_uc_setup (velocity_mat.f90:14-179) derives from the atom
+ * positions of the second lead period:
+ *   pos_l      nl x 3 (row-major): position of every left-lead orbital (pos_l(:,1:3))
+ *   pos_uc_l   (nl / nb) x 3: for orbital o of the first transverse image, the
+ *              position of the transverse-cell atom it maps to
+ *              (positions(L_list(pidxL(o)), :))
+ *   pos_r, pos_uc_r the same for the right lead;
+ *   recivec    the device supercell's reciprocal vectors (rows, row-major);
+ *   reci_uc_l/r the lead unit cells' (rows, row-major);
+ *   ksc        nk x 3: each k-point in crystal coordinates of recivec (k_grid).
+ * n_bloch_x * n_bloch_y must divide nl and nr and be at most NT_MAX_BLOCH.
+ * All pointers are device memory. */
+#define NT_MAX_BLOCH 256
+typedef struct {
+    int n_bloch_x, n_bloch_y;
+    double recivec[9];
+    double reci_uc_l[9];
+    double reci_uc_r[9];
+    const double* ksc;
+    const double* pos_l;
+    const double* pos_uc_l;
+    const double* pos_r;
+    const double* pos_uc_r;
+} nt_unfold_desc;
+
+/* Outputs (device memory, each may be NULL), per point:
+ *   bz_*       n ints: the label (0 for evanescent modes)            (bz_idx_*)
+ *   k_xyz_*    n x 3 column-major: unfolded k, 100 for evanescent modes (k_xyz_*)
+ *   split      8 doubles: transl_s, transl_ns, transr_s, transr_ns, refll_s,
+ *              refll_ns, reflr_s, reflr_ns (velocity_mat.f90:3032-3099)
+ *   spec_trl   nl x 2 column-major: for every incident left mode (L_adv) the
+ *              sum of |t_RL(j,i)|^2 over propagating outgoing right modes and
+ *              its part with equal in-plane k (|dk|^2 < 1e-5): the r_t and r_s
+ *              columns of write_specular (negf/write_data.f90:3-76) for 'trl';
+ *   spec_tlr   nr x 2: the same for t_LR (incident R_adv, outgoing L_ret),
+ *   spec_rrr   nr x 2: r_RR (incident R_adv, outgoing R_ret),
+ *   spec_rll   nl x 2: r_LL (incident L_adv, outgoing L_ret); write_md's
+ *              columns are spec_rll + spec_trl ('mdl') and spec_rrr + spec_tlr
+ *              ('mdr').
+ * The primitive-cell columns of those tables (find_k_pc, negf/dyn.f90:770-889)
+ * are not produced. */
+typedef struct {
+    int* bz_L_adv; int* bz_L_ret; int* bz_R_ret; int* bz_R_adv;
+    double* k_xyz_L_adv; double* k_xyz_L_ret; double* k_xyz_R_ret; double* k_xyz_R_adv;
+    double* split;
+    double* spec_trl; double* spec_tlr; double* spec_rrr; double* spec_rll;
+} nt_unfold_out;
+
+/* nt_modes_tdb_batched_dev plus the unfolding of every point (uf required). */
+int nt_modes_unfold_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int tb, int nl, int nr, int npts,
+                                    const double* E, const int* kidx, int nk,
+                                    const nt_c128* Haml, const nt_c128* Hamlv,
+                                    const nt_c128* Hamr, const nt_c128* Hamrv,
+                                    const nt_c128* gl_adv, const nt_c128* gr_ret,
+                                    const nt_c128* gl_plus, const nt_c128* gr_minus,
+                                    const nt_c128* gml_adv, const nt_c128* gmr_ret,
+                                    const nt_c128* G_D_RL, const nt_c128* G_D_LR,
+                                    const nt_c128* G_L, const nt_c128* G_R,
+                                    double a_z_L, double a_z_R, const int* first_point_mask,
+                                    const nt_modes_out* out, const nt_unfold_desc* uf,
+                                    const nt_unfold_out* uo, int* status);
 
 /* ------------------------------------------------ H(k) assembly
  * gen_dyn_qe_tdb_k (negf/dyn.f90:1482-2080) on the GPU: one block for nk

@@ -129,6 +129,7 @@ def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, 
     return dict(trl=trl, tlr=tlr, rrr=rrr, rll=rll,
                 k_z_L_adv=kz_al, k_z_L_ret=kz_rl, k_z_R_ret=kz_rr, k_z_R_adv=kz_ar,
                 lam_L_adv=lam_al, lam_L_ret=lam_rl, lam_R_ret=lam_rr, lam_R_adv=lam_ar,
+                U_L_adv=U_al, U_L_ret=U_rl, U_R_ret=U_rr, U_R_adv=U_ar,
                 n_prop=dict(L_adv=n_al, L_ret=n_rl, R_ret=n_rr, R_adv=n_ar),
                 t_mode_L=a2(trl).sum(axis=0), t_mode_R=a2(trl).sum(axis=1),
                 T_modes_RL=float(a2(trl).sum()), T_modes_LR=float(a2(tlr).sum()),

@@ -33,7 +33,7 @@ EXPORTS = [
     "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
-    "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
+    "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
     "nt_set_overlap", "nt_set_gemm_3m",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
@@ -47,6 +47,20 @@ class NtC128(C.Structure):
 class NtModesOut(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("trl", "tlr", "rrr", "rll", "k_z_L_adv", "k_z_L_ret", "k_z_R_ret",
                                           "k_z_R_adv", "n_prop", "t_mode_L", "t_mode_R", "sums")]
+
+
+class NtUnfoldDesc(C.Structure):
+    _fields_ = [("n_bloch_x", C.c_int), ("n_bloch_y", C.c_int), ("recivec", C.c_double * 9),
+                ("reci_uc_l", C.c_double * 9), ("reci_uc_r", C.c_double * 9)] + \
+               [(n, C.c_void_p) for n in ("ksc", "pos_l", "pos_uc_l", "pos_r", "pos_uc_r")]
+
+
+UNFOLD_OUT_FIELDS = ("bz_L_adv", "bz_L_ret", "bz_R_ret", "bz_R_adv", "k_xyz_L_adv", "k_xyz_L_ret", "k_xyz_R_ret",
+                     "k_xyz_R_adv", "split", "spec_trl", "spec_tlr", "spec_rrr", "spec_rll")
+
+
+class NtUnfoldOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in UNFOLD_OUT_FIELDS]
 
 
 class NtDeviceDesc(C.Structure):
@@ -126,6 +140,8 @@ def lib():
     L.nt_get_ldos.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, I, P, P]
     L.nt_modes_tdb_batched_dev.argtypes = [P, pp, I, I, I, I, P, P, I] + [P] * 14 + [D, D, P,
                                                                               C.POINTER(NtModesOut), P]
+    L.nt_modes_unfold_tdb_batched_dev.argtypes = [P, pp, I, I, I, I, P, P, I] + [P] * 14 + [
+        D, D, P, C.POINTER(NtModesOut), C.POINTER(NtUnfoldDesc), C.POINTER(NtUnfoldOut), P]
     L.nt_autotune_inverse.argtypes = [P, I, I, I, C.POINTER(I), C.POINTER(D)]
     L.nt_set_inverse_panel.argtypes = [P, I, I, I]
     L.nt_set_overlap.argtypes = [P, I]

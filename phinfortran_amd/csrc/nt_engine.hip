@@ -1709,15 +1709,29 @@ size_t modes_scratch(int nl, int nr, int P)
     a.take(mb);  // V / W
     for (int i = 0; i < 4; ++i) a.take(mb);   // t/r scratch
     a.take(sizeof(double) * 18 * P);
+    for (int i = 0; i < 4; ++i) {
+        a.take(sizeof(int) * (size_t)n * P);       // unfolding labels
+        a.take(sizeof(double) * 3 * (size_t)n * P);  // unfolded k
+    }
     return a.off + eig_scratch(n, P);
 }
+
+// Unfolding request of one chunk of points (outputs already offset to it;
+// null outputs use scratch).
+struct UnfoldRun {
+    const nt_unfold_desc* d;
+    int* bz[4];          // L_adv, L_ret, R_ret, R_adv
+    double* kxyz[4];
+    double* split;       // 8 per point
+    double* rows[4];     // t_RL, t_LR, r_RR, r_LL: (total, specular) per incident mode
+};
 
 void modes_run(nt_ctx* c, const nt_params* prm, int tb, int nl, int nr, int P, const double* E, const int* kidx,
                int nk, BMat<double> Haml, BMat<double> Hamlv, BMat<double> Hamr, BMat<double> Hamrv,
                BMat<double> gl_adv, BMat<double> gr_ret, BMat<double> gl_plus, BMat<double> gr_minus,
                BMat<double> gml_adv, BMat<double> gmr_ret, BMat<double> G_D_RL, BMat<double> G_D_LR,
                BMat<double> G_L, BMat<double> G_R, double az_l, double az_r, const int* first_mask, ModeOut o,
-               int* status, Arena& ar, size_t off)
+               int* status, Arena& ar, size_t off, const UnfoldRun* uf = nullptr)
 {
     typedef double T;
     const int n = std::max(nl, nr);
@@ -1741,6 +1755,12 @@ void modes_run(nt_ctx* c, const nt_params* prm, int tb, int nl, int nr, int P, c
     T* tr[4];
     for (int i = 0; i < 4; ++i) tr[i] = sa.ptr<T>(sa.take(mb));
     T* sc = sa.ptr<T>(sa.take(sizeof(T) * 18 * P));
+    int* bzs[4];
+    double* kxs[4];
+    for (int i = 0; i < 4; ++i) {
+        bzs[i] = sa.ptr<int>(sa.take(sizeof(int) * (size_t)n * P));
+        kxs[i] = sa.ptr<double>(sa.take(sizeof(double) * 3 * (size_t)n * P));
+    }
     const size_t eig_off = sa.off;
     hipStream_t s = c->stream;
     auto W = [&](T* q, int ld) { return bw(q, mm, ld); };
@@ -1851,6 +1871,60 @@ void modes_run(nt_ctx* c, const nt_params* prm, int tb, int nl, int nr, int P, c
             return launch_mode_sums(C.out, so, mo, mo, mi, kzo, kzi, colsum, o.sums[C.q], P, s);
         });
     }
+    if (!uf) return;
+    // Brillouin-zone unfolding (negf/velocity_mat.f90:2020-2130 and mirrors, :3032-3099)
+    const nt_unfold_desc& D = *uf->d;
+    const int nb = D.n_bloch_x * D.n_bloch_y;
+    for (int d = 0; d < 4; ++d) {
+        const int m = dirs[d].n;
+        const bool left = d < 2;
+        UnfoldDirArgs ua;
+        ua.n = m;
+        ua.nuc = m / nb;
+        ua.nbx = D.n_bloch_x;
+        ua.nby = D.n_bloch_y;
+        ua.U = U[d];
+        ua.ustride = mm;
+        ua.ldu = m;
+        ua.w = w[d];
+        ua.kz = o.kz[d] ? o.kz[d] : kzs[d];
+        ua.ksc = D.ksc;
+        ua.kidx = kidx;
+        for (int q = 0; q < 9; ++q) {
+            ua.recivec[q] = D.recivec[q];
+            ua.reci_uc[q] = left ? D.reci_uc_l[q] : D.reci_uc_r[q];
+        }
+        ua.pos = left ? D.pos_l : D.pos_r;
+        ua.pos_uc = left ? D.pos_uc_l : D.pos_uc_r;
+        ua.bz = uf->bz[d] ? uf->bz[d] : bzs[d];
+        ua.kxyz = uf->kxyz[d] ? uf->kxyz[d] : kxs[d];
+        ua.batch = P;
+        bzs[d] = ua.bz;
+        kxs[d] = ua.kxyz;
+        check_range(c, ua.bz, sizeof(int) * (size_t)m * P, "unfold bz");
+        check_range(c, ua.kxyz, sizeof(double) * 3 * (size_t)m * P, "unfold k");
+        launch(c, NT_K_ELEM, 0.0, "unfold weights", [&] { return launch_unfold_weights(ua, s); });
+    }
+    UnfoldSplitArgs sp;
+    const int slot[4] = {1, 0, 3, 2};   // t_RL -> transr, t_LR -> transl, r_RR -> reflr, r_LL -> refll
+    for (int q = 0; q < 4; ++q) {
+        const Chain& C = ch[q];
+        UnfoldChain& u = sp.ch[q];
+        u.M = C.out;
+        u.mo = dirs[C.o].n;
+        u.mi = dirs[C.i].n;
+        u.bz_row = bzs[C.o];
+        u.bz_col = bzs[C.i];
+        u.k_row = kxs[C.o];
+        u.k_col = kxs[C.i];
+        u.rows = uf->rows[q];
+        u.slot = slot[q];
+        check_range(c, u.rows, sizeof(double) * 2 * (size_t)u.mi * P, "unfold rows");
+    }
+    sp.split = uf->split;
+    sp.batch = P;
+    check_range(c, sp.split, sizeof(double) * 8 * (size_t)P, "unfold split");
+    launch(c, NT_K_ELEM, 0.0, "unfold split", [&] { return launch_unfold_split(sp, s); });
 }
 
 // ------------------------------------------------------------- autotuner
@@ -2741,13 +2815,13 @@ int nt_set_inverse_panel(nt_ctx* c, int n, int precision, int nb)
     return NT_ERR_ARG;
 }
 
-int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, int npts, const double* E,
-                             const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv,
-                             const nt_c128* Hamr, const nt_c128* Hamrv, const nt_c128* gl_adv,
-                             const nt_c128* gr_ret, const nt_c128* gl_plus, const nt_c128* gr_minus,
-                             const nt_c128* gml_adv, const nt_c128* gmr_ret, const nt_c128* G_D_RL,
-                             const nt_c128* G_D_LR, const nt_c128* G_L, const nt_c128* G_R, double a_z_L,
-                             double a_z_R, const int* first_point_mask, const nt_modes_out* out, int* status)
+static int modes_entry(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, int npts, const double* E,
+                       const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
+                       const nt_c128* Hamrv, const nt_c128* gl_adv, const nt_c128* gr_ret, const nt_c128* gl_plus,
+                       const nt_c128* gr_minus, const nt_c128* gml_adv, const nt_c128* gmr_ret,
+                       const nt_c128* G_D_RL, const nt_c128* G_D_LR, const nt_c128* G_L, const nt_c128* G_R,
+                       double a_z_L, double a_z_R, const int* first_point_mask, const nt_modes_out* out,
+                       const nt_unfold_desc* uf, const nt_unfold_out* uo, int* status)
 {
     if (!c || check_params(p) || (tb != 0 && tb != 1)) return NT_ERR_ARG;
     if (nl <= 0 || nr <= 0 || npts < 0 || nk <= 0 || (nk > 1 && !kidx) || !(a_z_L > 0.0) || !(a_z_R > 0.0))
@@ -2756,10 +2830,37 @@ int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int 
         !gmr_ret || !G_D_RL || !G_D_LR || !G_L || !G_R)
         return NT_ERR_ARG;
     if (std::max(nl, nr) > inv_max_n()) return NT_ERR_ARG;
+    if (uf) {
+        const int nb = uf->n_bloch_x * uf->n_bloch_y;
+        if (uf->n_bloch_x <= 0 || uf->n_bloch_y <= 0 || nb > NT_MAX_BLOCH || nl % nb || nr % nb || !uf->ksc ||
+            !uf->pos_l || !uf->pos_uc_l || !uf->pos_r || !uf->pos_uc_r)
+            return NT_ERR_ARG;
+    }
     if (npts == 0) return NT_OK;
     return guarded(c, [&] {
         const size_t z = sizeof(nt_c128);
         const long nll = (long)nl * nl, nrr = (long)nr * nr, nlr = (long)nl * nr;
+        nt_unfold_out u = uo ? *uo : nt_unfold_out{};
+        if (uf) {
+            const int nb = uf->n_bloch_x * uf->n_bloch_y;
+            reg_region(c, uf->ksc, sizeof(double) * 3 * nk);
+            reg_region(c, uf->pos_l, sizeof(double) * 3 * nl);
+            reg_region(c, uf->pos_r, sizeof(double) * 3 * nr);
+            reg_region(c, uf->pos_uc_l, sizeof(double) * 3 * (nl / nb));
+            reg_region(c, uf->pos_uc_r, sizeof(double) * 3 * (nr / nb));
+            const int nn[4] = {nl, nl, nr, nr};
+            int* bz[4] = {u.bz_L_adv, u.bz_L_ret, u.bz_R_ret, u.bz_R_adv};
+            double* kx[4] = {u.k_xyz_L_adv, u.k_xyz_L_ret, u.k_xyz_R_ret, u.k_xyz_R_adv};
+            for (int d = 0; d < 4; ++d) {
+                reg_region(c, bz[d], sizeof(int) * (size_t)nn[d] * npts);
+                reg_region(c, kx[d], sizeof(double) * 3 * (size_t)nn[d] * npts);
+            }
+            reg_region(c, u.split, sizeof(double) * 8 * npts);
+            reg_region(c, u.spec_trl, sizeof(double) * 2 * (size_t)nl * npts);
+            reg_region(c, u.spec_tlr, sizeof(double) * 2 * (size_t)nr * npts);
+            reg_region(c, u.spec_rrr, sizeof(double) * 2 * (size_t)nr * npts);
+            reg_region(c, u.spec_rll, sizeof(double) * 2 * (size_t)nl * npts);
+        }
         reg_region(c, E, sizeof(double) * npts);
         reg_region(c, kidx, sizeof(int) * npts);
         reg_region(c, first_point_mask, sizeof(int) * npts);
@@ -2841,11 +2942,29 @@ int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int 
             mo.t_mode_R = ptd(o.t_mode_R, nr);
             for (int q = 0; q < 4; ++q) mo.sums[q] = o.sums ? sum4 + q * P : nullptr;
             hipcheck(c, hipMemsetAsync(stc, 0, sizeof(int) * P, c->stream), "memset");
+            UnfoldRun ur;
+            if (uf) {
+                auto pti = [&](int* b, long per) { return b ? b + (long)p0 * per : nullptr; };
+                ur.d = uf;
+                ur.bz[0] = pti(u.bz_L_adv, nl);
+                ur.bz[1] = pti(u.bz_L_ret, nl);
+                ur.bz[2] = pti(u.bz_R_ret, nr);
+                ur.bz[3] = pti(u.bz_R_adv, nr);
+                ur.kxyz[0] = ptd(u.k_xyz_L_adv, 3L * nl);
+                ur.kxyz[1] = ptd(u.k_xyz_L_ret, 3L * nl);
+                ur.kxyz[2] = ptd(u.k_xyz_R_ret, 3L * nr);
+                ur.kxyz[3] = ptd(u.k_xyz_R_adv, 3L * nr);
+                ur.split = ptd(u.split, 8);
+                ur.rows[0] = ptd(u.spec_trl, 2L * nl);
+                ur.rows[1] = ptd(u.spec_tlr, 2L * nr);
+                ur.rows[2] = ptd(u.spec_rrr, 2L * nr);
+                ur.rows[3] = ptd(u.spec_rll, 2L * nl);
+            }
             modes_run(c, p, tb, nl, nr, P, E + p0, kk, nk, lead(Haml, nll, nl), lead(Hamlv, nll, nl),
                       lead(Hamr, nrr, nr), lead(Hamrv, nrr, nr), pt(gl_adv, nll, nl), pt(gr_ret, nrr, nr),
                       pt(gl_plus, nll, nl), pt(gr_minus, nrr, nr), pt(gml_adv, nll, nl), pt(gmr_ret, nrr, nr),
                       pt(G_D_RL, nlr, nr), pt(G_D_LR, nlr, nl), pt(G_L, nll, nl), pt(G_R, nrr, nr), a_z_L, a_z_R,
-                      first_point_mask ? first_point_mask + p0 : nullptr, mo, stc, ar, o_run);
+                      first_point_mask ? first_point_mask + p0 : nullptr, mo, stc, ar, o_run, uf ? &ur : nullptr);
             // counts and sums: slot-major scratch -> point-major outputs
             if (o.n_prop) {
                 hipLaunchKernelGGL(k_transpose_small<int>, dim3(nblk(4L * P)), dim3(256), 0, c->stream, np4, 4, P,
@@ -2863,6 +2982,35 @@ int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int 
         hipcheck(c, hipStreamSynchronize(c->stream), "sync");
         return 0;
     });
+}
+
+int nt_modes_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, int npts, const double* E,
+                             const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv,
+                             const nt_c128* Hamr, const nt_c128* Hamrv, const nt_c128* gl_adv,
+                             const nt_c128* gr_ret, const nt_c128* gl_plus, const nt_c128* gr_minus,
+                             const nt_c128* gml_adv, const nt_c128* gmr_ret, const nt_c128* G_D_RL,
+                             const nt_c128* G_D_LR, const nt_c128* G_L, const nt_c128* G_R, double a_z_L,
+                             double a_z_R, const int* first_point_mask, const nt_modes_out* out, int* status)
+{
+    return modes_entry(c, p, tb, nl, nr, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus,
+                       gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R, a_z_L, a_z_R, first_point_mask, out,
+                       nullptr, nullptr, status);
+}
+
+int nt_modes_unfold_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, int npts,
+                                    const double* E, const int* kidx, int nk, const nt_c128* Haml,
+                                    const nt_c128* Hamlv, const nt_c128* Hamr, const nt_c128* Hamrv,
+                                    const nt_c128* gl_adv, const nt_c128* gr_ret, const nt_c128* gl_plus,
+                                    const nt_c128* gr_minus, const nt_c128* gml_adv, const nt_c128* gmr_ret,
+                                    const nt_c128* G_D_RL, const nt_c128* G_D_LR, const nt_c128* G_L,
+                                    const nt_c128* G_R, double a_z_L, double a_z_R, const int* first_point_mask,
+                                    const nt_modes_out* out, const nt_unfold_desc* uf, const nt_unfold_out* uo,
+                                    int* status)
+{
+    if (!uf) return NT_ERR_ARG;
+    return modes_entry(c, p, tb, nl, nr, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus,
+                       gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R, a_z_L, a_z_R, first_point_mask, out,
+                       uf, uo, status);
 }
 
 int nt_hk_assemble_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const int* seg_off, const int* seg_rc,

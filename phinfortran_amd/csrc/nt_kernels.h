@@ -80,6 +80,45 @@ hipError_t launch_scale_items(double* M, long stride, int ld, int m, int n, cons
 hipError_t launch_mode_sums(const double* S, long stride, int ld, int m, int n, const double* kz_row,
                             const double* kz_col, double* col_sum, double* total, int batch, hipStream_t s);
 
+// Brillouin-zone unfolding of lead modes (nt_unfold.hip).  Weights of one
+// mode set: label bz (1-based k-shift of largest weight, 0 for evanescent
+// modes) and unfolded k (column-major n x 3, 100 for evanescent modes).
+struct UnfoldDirArgs {
+    int n, nuc, nbx, nby;     // modes, orbitals per transverse cell, k-shift grid
+    const double* U;          // eigenvectors (complex), item stride ustride, leading dim ldu
+    long ustride;
+    int ldu;
+    const double* w;          // eigenvalues, n complex per item
+    const double* kz;         // n per item (>= 100: evanescent)
+    const double* ksc;        // 3 per k-point, crystal coordinates of recivec
+    const int* kidx;          // k-point of each item, or null (k-point 0)
+    double recivec[9], reci_uc[9];   // row-major reciprocal vectors
+    const double* pos;        // n x 3 orbital positions
+    const double* pos_uc;     // nuc x 3 transverse-cell atom of each first-image orbital
+    int* bz;                  // n per item
+    double* kxyz;             // 3 n per item
+    int batch;
+};
+// One t or r matrix (outgoing x incident) with the labels and unfolded k of
+// its two mode sets; rows: optional (total, specular) per incident mode.
+struct UnfoldChain {
+    const double* M;
+    int mo, mi;
+    const int* bz_row;
+    const int* bz_col;
+    const double* k_row;
+    const double* k_col;
+    double* rows;
+    int slot;                 // split pair index (0..3)
+};
+struct UnfoldSplitArgs {
+    UnfoldChain ch[4];
+    double* split;            // 8 per item, or null
+    int batch;
+};
+hipError_t launch_unfold_weights(const UnfoldDirArgs& a, hipStream_t s);
+hipError_t launch_unfold_split(const UnfoldSplitArgs& a, hipStream_t s);
+
 // H(k) phase sums of the QE force-constant path (nt_hk.hip): out[k] (rows x cols,
 // leading dimension ld, k stride kstride complex elements) = sum over the
 // entries of each 3x3 segment of coef * exp(i dr . k); zero where no segment.

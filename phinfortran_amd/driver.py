@@ -1,12 +1,15 @@
-"""Caroli run of the reference's QE force-constant (tdb) path on one GPU.
+"""The reference's QE force-constant (tdb) run on one GPU.
 
-Replaces the k/omega loop of negf/phinfortran.f90:1188-1275 for the Caroli
-outputs: the namelist config and the QE force constants are read as the
-reference reads them (phinfortran_amd/qe.py), H(k) of every transverse
-k-point is assembled on the GPU, every (omega, k) point is solved in one
-batched call with the reference's ie = 1 zeroing, and transl_tot.dat /
-transr_tot.dat are written in the reference's format (write_trans,
-negf/write_data.f90:340-497).
+Replaces the k/omega loop of negf/phinfortran.f90:1188-1275: the namelist
+config and the QE force constants are read as the reference reads them
+(phinfortran_amd/qe.py), H(k) of every transverse k-point is assembled on the
+GPU, every (omega, k) point is solved in one batched call with the
+reference's ie = 1 zeroing, and the tables are written in the reference's
+formats: transl_tot.dat / transr_tot.dat (write_trans,
+negf/write_data.f90:340-497) and, with modes=True, the mode-resolved run's
+specular / non-specular tables and the six per-mode files of every k-point
+(vel_mat_tdb with its Brillouin-zone unfolding, negf/velocity_mat.f90:1692-3150;
+write_specular / write_md, negf/write_data.f90:3-175).
 """
 from __future__ import annotations
 
@@ -20,8 +23,10 @@ from . import qe as Q
 
 
 def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None, engine=None,
-               mass_profile: Optional[str] = None, precision: str = "fp64"):
-    """Returns dict(E (ne), k (nk x 3), transl, transr (ne x nk), status)."""
+               mass_profile: Optional[str] = None, precision: str = "fp64", modes: bool = False):
+    """Returns dict(E (ne), k (nk x 3), transl, transr (ne x nk), status);
+    with modes=True also the eight split tables (ne x nk each, names of
+    unfold.SPLIT_NAMES) and the mode-resolved status."""
     import torch
     from . import transport as T
     cfg = Q.read_config(config_path)
@@ -36,12 +41,35 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
     mask = np.tile((np.arange(ne) == 0).astype(np.int32), nk)
     p = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500,
                  precision=precision)
-    r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, first_point_mask=mask)
+    out = {}
+    if modes:
+        from .unfold import SPLIT_NAMES, UnfoldGeometry
+        p.lambda_eta = cfg["lambda_eta"]
+        geom = UnfoldGeometry.from_qe(qe, cfg)
+        az = qe["cell"][2, 2]
+        r = eng.solve_modes(d, np.tile(E, nk), p, az, az, kidx=kidx, first_point_mask=mask,
+                            want=(), unfold=geom, ksc=ks)
+        split = r["split"].cpu().numpy().reshape(nk, ne, 8)
+        for i, name in enumerate(SPLIT_NAMES):
+            out[name] = split[:, :, i].T
+        out["modes_status"] = r["modes_status"].cpu().numpy().reshape(nk, ne).T
+        if output_dir:
+            host = {k: r[k].cpu().numpy() for k in ("k_xyz_L_adv", "k_xyz_R_adv", "spec_trl", "spec_tlr",
+                                                    "spec_rrr", "spec_rll")}
+            for ik in range(nk):
+                for ie in range(ne):
+                    q = ik * ne + ie
+                    OUT.write_mode_tables(output_dir, ik + 1, float(np.sqrt(E[ie])),
+                                          {k: v[q] for k, v in host.items()})
+    else:
+        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, first_point_mask=mask)
     tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
     tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
     st = r["status"].cpu().numpy().reshape(nk, ne).T
     if output_dir:
-        OUT.write_trans(output_dir, np.repeat(E[:, None], nk, axis=1), tl, tr)
+        OUT.write_trans(output_dir, np.repeat(E[:, None], nk, axis=1), tl, tr,
+                        **{k: out[k] for k in out if k != "modes_status"})
     if engine is None:
         eng.close()
-    return dict(E=E, k=ks, transl=tl, transr=tr, status=st, blocks=d)
+    out.update(E=E, k=ks, transl=tl, transr=tr, status=st, blocks=d)
+    return out

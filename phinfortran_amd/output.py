@@ -61,3 +61,75 @@ def write_trans(output_dir: str, E, transl, transr, transl_s: Optional[np.ndarra
             _table(os.path.join(output_dir, name + ".dat"), E1, T, "")
     _table(os.path.join(output_dir, "transl_tot.dat"), E1, transl, "")
     _table(os.path.join(output_dir, "transr_tot.dat"), E1, transr, "")
+
+
+# ------------------------------------------------------------ per-mode tables
+def _mode_row(count: int, vals) -> str:
+    """One record of '(1I5,13e20.10)' / '(1I5,15e20.10)'."""
+    return "%5d" % count + "".join(fortran_e(v, 20, 10) for v in vals)
+
+
+def _clamp(x: float) -> float:
+    return 1.0e-60 if 0.0 < x < 1.0e-60 else x
+
+
+def specular_rows(omega, k_b, total, spec, kp_b=None, v_b=None):
+    """write_specular's records for one (omega, k) point
+    (negf/write_data.f90:36-73): one per incident propagating mode (k_b(i,3)
+    < 99.9, numbered from 1): omega, k_b(i,1:3), kp_b(i,1:3), v_b(i,1:3),
+    r_s, r_t, r_s / r_t (0 when r_t = 0); r_s and r_t below 1e-60 become
+    1e-60.  total / spec: the (total, specular) sums of the mode
+    (nt_unfold_out spec_*).  kp_b / v_b (the primitive-cell k and velocity of
+    find_k_pc) default to zero."""
+    k_b = np.asarray(k_b, float)
+    n = len(k_b)
+    kp_b = np.zeros((n, 3)) if kp_b is None else np.asarray(kp_b, float)
+    v_b = np.zeros((n, 3)) if v_b is None else np.asarray(v_b, float)
+    out, count = [], 0
+    for i in range(n):
+        if not k_b[i, 2] < 99.9:
+            continue
+        count += 1
+        rt, rs = _clamp(float(total[i])), _clamp(float(spec[i]))
+        out.append(_mode_row(count, [omega, *k_b[i], *kp_b[i], *v_b[i], rs, rt, rs / rt if rt != 0.0 else 0.0]))
+    return out
+
+
+def md_rows(omega, k_b, r_total, r_spec, t_total, t_spec, kp_b=None, v_b=None):
+    """write_md's records (negf/write_data.f90:113-171): omega, k_b(1:3),
+    kp_b(1:3), v_b(1:3), r_s, r_t, t_s, t_t per incident propagating mode."""
+    k_b = np.asarray(k_b, float)
+    n = len(k_b)
+    kp_b = np.zeros((n, 3)) if kp_b is None else np.asarray(kp_b, float)
+    v_b = np.zeros((n, 3)) if v_b is None else np.asarray(v_b, float)
+    out, count = [], 0
+    for i in range(n):
+        if not k_b[i, 2] < 99.9:
+            continue
+        count += 1
+        vals = [_clamp(float(x)) for x in (r_spec[i], r_total[i], t_spec[i], t_total[i])]
+        out.append(_mode_row(count, [omega, *k_b[i], *kp_b[i], *v_b[i], *vals]))
+    return out
+
+
+def write_mode_tables(output_dir: str, nn: int, omega: float, point: dict):
+    """Append one (omega, k) point to the six per-mode files of vel_mat_tdb
+    (negf/velocity_mat.f90:3101-3130): {tlr,trl,right,left,mdl,mdr}_specular_param_NNNNNN.dat,
+    NNNNNN the 1-based k index nn.  point: host arrays of one point from
+    Engine.solve_modes(..., unfold=...): k_xyz_L_adv / k_xyz_R_adv (n x 3)
+    and spec_trl, spec_tlr, spec_rrr, spec_rll (n x 2: total, specular)."""
+    os.makedirs(output_dir, exist_ok=True)
+    kl, kr = point["k_xyz_L_adv"], point["k_xyz_R_adv"]
+    s = {k: np.asarray(point["spec_" + k]) for k in ("trl", "tlr", "rrr", "rll")}
+    tables = {
+        "tlr": specular_rows(omega, kr, s["tlr"][:, 0], s["tlr"][:, 1]),
+        "trl": specular_rows(omega, kl, s["trl"][:, 0], s["trl"][:, 1]),
+        "right": specular_rows(omega, kr, s["rrr"][:, 0], s["rrr"][:, 1]),
+        "left": specular_rows(omega, kl, s["rll"][:, 0], s["rll"][:, 1]),
+        "mdl": md_rows(omega, kl, s["rll"][:, 0], s["rll"][:, 1], s["trl"][:, 0], s["trl"][:, 1]),
+        "mdr": md_rows(omega, kr, s["rrr"][:, 0], s["rrr"][:, 1], s["tlr"][:, 0], s["tlr"][:, 1]),
+    }
+    for name, rows in tables.items():
+        with open(os.path.join(output_dir, "%s_specular_param_%06d.dat" % (name, nn)), "a") as f:
+            for r in rows:
+                f.write(r + "\n")

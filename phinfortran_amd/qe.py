@@ -43,7 +43,8 @@ def read_config(path):
     return dict(nk1=mesh[0], nk2=mesh[1], ne=int(vals["ne"]), emin=g("emin"), emax=g("emax"),
                 eta=g("eta"), eta0=g("eta0"), convergence=g("convergence"), r_cutoff=g("r_cutoff"),
                 nz=int(vals["nz"]), n_bloch_x=int(vals["n_bloch_x"]), n_bloch_y=int(vals["n_bloch_y"]),
-                period=int(vals["period_left"]), buffer=int(vals["buffer_left"]))
+                period=int(vals["period_left"]), buffer=int(vals["buffer_left"]),
+                lambda_eta=g("lambda_eta") if "lambda_eta" in vals else 5.0e-3)
 
 
 def read_qe_fc(cfg, path, mass_profile=None):

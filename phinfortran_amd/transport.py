@@ -346,13 +346,20 @@ class Engine:
     MODE_FIELDS = ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "G_D_RL", "G_D_LR", "G_L", "G_R")
 
     def solve_modes(self, dblocks, E, p: N.NtParams, a_z_L: float, a_z_R: float, kidx=None, first_point_mask=None,
-                    tb: bool = False, want=("trl", "tlr", "rrr", "rll")):
+                    tb: bool = False, want=("trl", "tlr", "rrr", "rll"), unfold=None, ksc=None):
         """Batched solve followed by the mode-resolved transmission (vel_mat_tdb).
 
         Returns the solve's dict plus k_z_* (npts, n), n_prop (npts, 4),
         t_mode_L (npts, nl), t_mode_R (npts, nr), sums (npts, 4: T_RL, T_LR,
         R_RR, R_LL over propagating modes) and the requested t/r matrices as
-        row-major (npts, rows, cols) views."""
+        row-major (npts, rows, cols) views.
+
+        unfold (phinfortran_amd.unfold.UnfoldGeometry) with ksc (nk x 3, the
+        k-points in crystal coordinates of its recivec) adds the Brillouin-zone
+        unfolding: bz_* (npts, n) labels, k_xyz_* (npts, n, 3) unfolded k,
+        split (npts, 8) in the order of unfold.SPLIT_NAMES, and spec_trl,
+        spec_tlr, spec_rrr, spec_rll (npts, n_incident, 2): (total, specular)
+        per incident mode."""
         import torch
         dev = torch.device("cuda", self.device)
         r = self.solve_tdb(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=self.MODE_FIELDS)
@@ -381,15 +388,36 @@ class Engine:
         ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
         # extras come back transposed (row-major views); the ABI wants the storage
         g = {k: r[k].transpose(-1, -2) for k in self.MODE_FIELDS}
-        st = self._L.nt_modes_tdb_batched_dev(
-            self.ctx, C.byref(p), int(tb), nl, nr, npts, ptr(E), ptr(kidx), nk,
-            ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
-            *[ptr(g[k]) for k in self.MODE_FIELDS], float(a_z_L), float(a_z_R), ptr(first_point_mask),
-            C.byref(out), ptr(status))
+        args = (self.ctx, C.byref(p), int(tb), nl, nr, npts, ptr(E), ptr(kidx), nk,
+                ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
+                *[ptr(g[k]) for k in self.MODE_FIELDS], float(a_z_L), float(a_z_R), ptr(first_point_mask),
+                C.byref(out))
+        if unfold is None:
+            st = self._L.nt_modes_tdb_batched_dev(*args, ptr(status))
+        else:
+            if ksc is None:
+                raise ValueError("unfolding needs the k-points (ksc)")
+            desc, keep = unfold.desc(torch, dev, ksc)
+            if keep["ksc"].shape[0] != nk:
+                raise ValueError(f"ksc has {keep['ksc'].shape[0]} k-points, the blocks {nk}")
+            uo = N.NtUnfoldOut()
+            nn = dict(L_adv=nl, L_ret=nl, R_ret=nr, R_adv=nr)
+            for key, n in nn.items():
+                res["bz_" + key] = torch.zeros((npts, n), dtype=torch.int32, device=dev)
+                res["k_xyz_" + key] = torch.zeros((npts, 3, n), dtype=torch.float64, device=dev)
+            for key, n in (("trl", nl), ("tlr", nr), ("rrr", nr), ("rll", nl)):
+                res["spec_" + key] = torch.zeros((npts, 2, n), dtype=torch.float64, device=dev)
+            res["split"] = torch.zeros((npts, 8), dtype=torch.float64, device=dev)
+            for name in N.UNFOLD_OUT_FIELDS:
+                setattr(uo, name, res[name].data_ptr())
+            st = self._L.nt_modes_unfold_tdb_batched_dev(*args, C.byref(desc), C.byref(uo), ptr(status))
         self._check(st, "modes_tdb")
         torch.cuda.synchronize(dev)
         for name in want:
             res[name] = res[name].transpose(-1, -2)
+        if unfold is not None:
+            for name in [k for k in res if k.startswith("k_xyz_") or k.startswith("spec_")]:
+                res[name] = res[name].transpose(-1, -2)
         res["modes_status"] = status
         r.update(res)
         return r

@@ -35,10 +35,23 @@ def make_fixture():
     for ik in MODE_KPOINTS:
         f = "trl_specular_param_%06d.dat" % (ik + 1)
         shutil.copy(os.path.join(REF_EXAMPLE, "output", f), os.path.join(FIXTURE, f))
+    # write_md tables (both incident sides) at a generic k-point and at Gamma
+    for ik in MD_KPOINTS:
+        for d in ("mdl", "mdr"):
+            f = "%s_specular_param_%06d.dat" % (d, ik + 1)
+            shutil.copy(os.path.join(REF_EXAMPLE, "output", f), os.path.join(FIXTURE, f))
+    # specular / non-specular tables: the MODE_KPOINTS columns only
+    cols = [np.loadtxt(os.path.join(REF_EXAMPLE, "output", n + ".dat")) for n in SPLIT_TABLES]
+    out = [cols[0][:, 0]] + [c[:, 1 + ik] for c in cols for ik in MODE_KPOINTS]
+    np.savetxt(os.path.join(FIXTURE, "split_tables.dat"), np.stack(out, axis=1), fmt="%.16e",
+               header="E (omega^2), then for each table in " + " ".join(SPLIT_TABLES) +
+                      " the columns of k-points (0-based) " + " ".join(map(str, MODE_KPOINTS)))
 
 
 # k-points (0-based) whose per-mode tables are kept as fixtures
 MODE_KPOINTS = (0, 10, 24, 40)
+MD_KPOINTS = (10, 24)
+SPLIT_TABLES = ("transl_s", "transl_ns", "transr_s", "transr_ns", "refll_s", "refll_ns", "reflr_s", "reflr_ns")
 
 
 def read_config(path=None):
@@ -63,6 +76,27 @@ def golden_modes(ik):
     out = {}
     for row in np.atleast_2d(tab):
         out.setdefault(round(row[1], 8), []).append((row[4], row[12]))
+    return {k: np.array(v) for k, v in out.items()}
+
+
+def golden_split():
+    """dict table name -> (ne, len(MODE_KPOINTS)) columns of the committed
+    specular / non-specular tables (negf/phinfortran.f90 writes them with
+    write_trans), and the omega column."""
+    tab = np.loadtxt(os.path.join(FIXTURE, "split_tables.dat"))
+    nk = len(MODE_KPOINTS)
+    out = {n: tab[:, 1 + i * nk:1 + (i + 1) * nk] for i, n in enumerate(SPLIT_TABLES)}
+    return tab[:, 0], out
+
+
+def golden_rows(name, ik):
+    """Rows of a per-mode table (write_specular / write_md, negf/write_data.f90)
+    at k-point ik: dict omega -> 2-D array of the row's numbers after the
+    mode counter (omega, k_b(1:3), kp_b(1:3), v_b(1:3), sums...)."""
+    tab = np.loadtxt(os.path.join(FIXTURE, "%s_specular_param_%06d.dat" % (name, ik + 1)))
+    out = {}
+    for row in np.atleast_2d(tab):
+        out.setdefault(round(row[1], 8), []).append(row[1:])
     return {k: np.array(v) for k, v in out.items()}
 
 

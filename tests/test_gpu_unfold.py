@@ -1,0 +1,174 @@
+"""Brillouin-zone unfolding on the GPU (nt_modes_unfold_tdb_batched_dev)
+against the reference's committed si-si specular / non-specular tables and
+per-mode tables, and against the oracle (oracle/unfold.py, itself pinned to
+those tables and to the closed form of transverse Bloch states).
+
+Inside a degenerate set of propagating modes (equal k_z) the labels follow
+the eigenvector basis, which the GPU eigensolver and LAPACK's ZGEEV choose
+differently: such points are compared through s + ns only.  Mode order also
+differs between the two eigensolvers, so per-mode values are compared after
+sorting by (k_z, k_x, k_y)."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import modes as M
+from oracle import unfold as UF
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import si_si as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+SETS = ("L_adv", "L_ret", "R_ret", "R_adv")
+
+
+@pytest.fixture(scope="module")
+def si_gpu(engine):
+    from phinfortran_amd import transport as T
+    from phinfortran_amd.unfold import UnfoldGeometry
+    cfg = S.read_config()
+    qe = S.read_qe_fc(cfg)
+    asm = S.TdbAssembler(qe, cfg)
+    ks, E = S.k_grid(cfg), S.e_grid(cfg)
+    az = qe["cell"][2, 2]
+    kp = list(S.MODE_KPOINTS)
+    geom = UnfoldGeometry.from_qe(qe, cfg)
+    d = engine.upload_blocks([asm.blocks(ks[ik]) for ik in kp])
+    ne = len(E)
+    kidx = np.repeat(np.arange(len(kp), dtype=np.int32), ne)
+    mask = np.tile((np.arange(ne) == 0).astype(np.int32), len(kp))
+    p = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500,
+                 lambda_eta=5e-3)
+    print("unfold si-si", flush=True)
+    r = engine.solve_modes(d, np.tile(E, len(kp)), p, az, az, kidx=kidx, first_point_mask=mask, unfold=geom,
+                           ksc=ks[kp])
+    assert (r["modes_status"].cpu().numpy() == 0).all()
+    host = {k: v.cpu().numpy() for k, v in r.items() if hasattr(v, "cpu")}
+    return dict(cfg=cfg, qe=qe, asm=asm, ks=ks, E=E, az=az, kp=kp, r=host, ne=ne)
+
+
+def degenerate(kz_sets, tol=1e-6):
+    for kz in kz_sets:
+        kz = np.sort(kz[kz < 99.9])
+        if len(kz) > 1 and (np.diff(kz) < tol).any():
+            return True
+    return False
+
+
+def rows_of(kxyz, spec):
+    """(k_x, k_y, k_z, total, specular) of the propagating incident modes, sorted."""
+    inc = np.where(kxyz[:, 2] < 99.9)[0]
+    a = np.concatenate([kxyz[inc], spec[inc]], axis=1)
+    return a[np.lexsort((np.round(a[:, 1], 6), np.round(a[:, 0], 6), np.round(a[:, 2], 6)))]
+
+
+def test_split_tables_against_reference(si_gpu):
+    """transl_s/ns, transr_s/ns, refll_s/ns, reflr_s/ns of the committed tables
+    at every frequency of the four fixture k-points."""
+    g, r = si_gpu, si_gpu["r"]
+    Eg, split = S.golden_split()
+    names = ("transl_s", "transl_ns", "transr_s", "transr_ns", "refll_s", "refll_ns", "reflr_s", "reflr_ns")
+    n_split = n_all = 0
+    for j, ik in enumerate(g["kp"]):
+        for ie in range(g["ne"]):
+            q = j * g["ne"] + ie
+            got = dict(zip(names, r["split"][q]))
+            deg = degenerate([r["k_z_" + k][q] for k in SETS])
+            for base in ("transl", "transr", "refll", "reflr"):
+                gs, gns = split[base + "_s"][ie, j], split[base + "_ns"][ie, j]
+                s, ns = got[base + "_s"], got[base + "_ns"]
+                scale = max(1.0, gs + gns)
+                assert abs((s + ns) - (gs + gns)) < 2e-6 * scale, (ik, ie, base, s, ns, gs, gns)
+                if not deg:
+                    assert abs(s - gs) < 2e-6 * scale and abs(ns - gns) < 2e-6 * scale, (ik, ie, base, s, ns, gs, gns)
+            n_all += 1
+            n_split += not deg
+    print(f"split tables: {n_all} points, {n_split} compared term by term")
+    assert n_split > 0.9 * n_all
+
+
+def test_mode_tables_against_reference(si_gpu):
+    """k_xyz and the (r_t, r_s) columns of every row of the trl tables and the
+    four sums of the mdl / mdr tables (write_specular / write_md)."""
+    g, r = si_gpu, si_gpu["r"]
+    nrow = 0
+    for j, ik in enumerate(g["kp"]):
+        trl = S.golden_rows("trl", ik)
+        md = {d: S.golden_rows(d, ik) for d in ("mdl", "mdr")} if ik in S.MD_KPOINTS else None
+        for ie in range(g["ne"]):
+            q = j * g["ne"] + ie
+            w = round(math.sqrt(g["E"][ie]), 8)
+            if degenerate([r["k_z_" + k][q] for k in SETS]):
+                continue
+            ours = rows_of(r["k_xyz_L_adv"][q], r["spec_trl"][q])
+            gold = trl.get(w)
+            if gold is None:
+                assert len(ours) == 0
+                continue
+            gg = gold[:, [1, 2, 3, 11, 10]]
+            gg = gg[np.lexsort((np.round(gg[:, 1], 6), np.round(gg[:, 0], 6), np.round(gg[:, 2], 6)))]
+            assert ours.shape == gg.shape, (ik, ie)
+            assert np.abs(ours[:, :3] - gg[:, :3]).max() < 1e-7, (ik, ie)
+            assert np.abs(ours[:, 3:] - gg[:, 3:]).max() < 2e-6, (ik, ie)
+            nrow += len(ours)
+            if md:
+                for d, kb, rr, tt in (("mdl", "L_adv", "spec_rll", "spec_trl"), ("mdr", "R_adv", "spec_rrr", "spec_tlr")):
+                    a = np.concatenate([r["k_xyz_" + kb][q], r[rr][q][:, ::-1], r[tt][q][:, ::-1]], axis=1)
+                    a = a[a[:, 2] < 99.9]
+                    a = a[np.lexsort((np.round(a[:, 1], 6), np.round(a[:, 0], 6), np.round(a[:, 2], 6)))]
+                    b = md[d][w][:, [1, 2, 3, 10, 11, 12, 13]]
+                    b = b[np.lexsort((np.round(b[:, 1], 6), np.round(b[:, 0], 6), np.round(b[:, 2], 6)))]
+                    assert a.shape == b.shape, (d, ik, ie)
+                    assert np.abs(a[:, :3] - b[:, :3]).max() < 1e-7, (d, ik, ie)
+                    assert np.abs(a[:, 3:] - b[:, 3:]).max() < 2e-6, (d, ik, ie)
+    print(f"mode tables: {nrow} trl rows compared")
+    assert nrow > 300
+
+
+def test_against_oracle(si_gpu):
+    """labels, unfolded k, split sums and per-mode sums against the oracle's
+    chain (oracle.modes + oracle.unfold) on a sample of points."""
+    g, r = si_gpu, si_gpu["r"]
+    cfg, qe = g["cfg"], g["qe"]
+    pos = qe["pos_all"]
+    N, per, buf = len(pos), cfg["period"], cfg["buffer"]
+    nbx, nby = cfg["n_bloch_x"], cfg["n_bloch_y"]
+    setl = UF.transverse_uc_setup(pos, qe["cell"], buf + per, per, nbx, nby, 1e-3)
+    setr = UF.transverse_uc_setup(pos, qe["cell"], N - buf - 2 * per, per, nbx, nby, 1e-5)
+    recivec = 2 * np.pi * np.linalg.inv(qe["cell_all"]).T
+    reci_uc = 2 * np.pi * np.linalg.inv(qe["cell"]).T
+    p = O.OrParams(cfg["eta"], cfg["eta0"], cfg["convergence"], 10.0, O.ETA_RELATIVE, 500)
+    names = ("transl_s", "transl_ns", "transr_s", "transr_ns", "refll_s", "refll_ns", "reflr_s", "reflr_ns")
+    checked = 0
+    for j, ik in enumerate(g["kp"]):
+        blk = g["asm"].blocks(g["ks"][ik])
+        for ie in range(3, g["ne"], 9):
+            q = j * g["ne"] + ie
+            m = M.solve_point_modes(blk, g["E"][ie], p, 5e-3, g["az"], g["az"])
+            U = {k: m["U_" + k] for k in SETS}
+            lam = {k: m["lam_" + k] for k in SETS}
+            bz, kx, sp = UF.unfold_point(m, U, lam, g["ks"][ik], recivec, reci_uc, reci_uc, nbx, nby, pos, setl, setr)
+            got = dict(zip(names, r["split"][q]))
+            for n in names:
+                assert abs(got[n] - sp[n]) < 1e-7 * max(1.0, sp[n]) or degenerate([m["k_z_" + k] for k in SETS]), \
+                    (ik, ie, n, got[n], sp[n])
+            if degenerate([m["k_z_" + k] for k in SETS]):
+                continue
+            for key in SETS:
+                a = np.sort(np.round(r["k_xyz_" + key][q][r["bz_" + key][q] > 0], 9), axis=0)
+                b = np.sort(np.round(kx[key][bz[key] > 0], 9), axis=0)
+                assert a.shape == b.shape and (len(a) == 0 or np.abs(a - b).max() < 1e-8), (ik, ie, key)
+                assert sorted(r["bz_" + key][q][r["bz_" + key][q] > 0]) == sorted(bz[key][bz[key] > 0])
+            ours = rows_of(r["k_xyz_R_adv"][q], r["spec_tlr"][q])
+            ref = [(list(kx["R_adv"][i]) + [t, s]) for i, t, s in UF.specular_rows(m["tlr"], kx["L_ret"], kx["R_adv"])]
+            ref = np.array(ref).reshape(-1, 5)
+            ref = ref[np.lexsort((np.round(ref[:, 1], 6), np.round(ref[:, 0], 6), np.round(ref[:, 2], 6)))]
+            assert ours.shape == ref.shape and (len(ref) == 0 or np.abs(ours - ref).max() < 1e-8), (ik, ie)
+            checked += 1
+    print(f"oracle: {checked} points compared")
+    assert checked >= 10

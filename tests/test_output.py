@@ -30,3 +30,42 @@ def test_write_trans_reproduces_committed_tables(tmp_path):
     for name in ("transl_tot", "transr_tot"):
         assert open(os.path.join(tmp_path, name + ".dat"), "rb").read() == \
             open(os.path.join(FIX, name + ".dat"), "rb").read(), name
+
+
+def test_mode_records_reproduce_committed_tables():
+    """'(1I5,13e20.10)' and '(1I5,15e20.10)' records of write_specular /
+    write_md (negf/write_data.f90:3-175) rebuilt byte for byte from their own
+    numbers."""
+    n = 0
+    for f in sorted(os.listdir(FIX)):
+        if "_specular_param_" not in f:
+            continue
+        for ln in open(os.path.join(FIX, f)).read().splitlines():
+            t = ln.split()
+            assert OUT._mode_row(int(t[0]), [float(x) for x in t[1:]]) == ln, (f, ln)
+            n += 1
+    assert n > 1000
+
+
+def test_specular_and_md_row_rules(tmp_path):
+    k = np.array([[0.1, 0.2, 0.3], [0.0, 0.0, 100.0], [0.5, 0.0, -0.2], [0.0, 0.4, 0.1]])
+    tot = np.array([0.9, 0.0, 0.0, 5e-61])
+    spec = np.array([0.6, 0.0, 0.0, 0.0])
+    rows = OUT.specular_rows(2.5, k, tot, spec)
+    assert len(rows) == 3                                   # the evanescent mode has no record
+    vals = [[float(x) for x in r.split()] for r in rows]
+    assert [int(v[0]) for v in vals] == [1, 2, 3]           # numbered over propagating modes
+    assert vals[0][1] == 2.5 and vals[0][2:5] == [0.1, 0.2, 0.3]
+    assert abs(vals[0][-1] - 0.6 / 0.9) < 1e-10 and vals[0][-3:-1] == [0.6, 0.9]
+    assert vals[1][-3:] == [0.0, 0.0, 0.0]                  # r_t = 0: ratio written as 0
+    assert vals[2][-2] == 1e-60                             # 0 < r_t < 1e-60 -> 1e-60
+    md = OUT.md_rows(2.5, k, tot, spec, tot * 0.5, spec * 0.5)
+    assert len(md) == 3 and len(md[0].split()) == 15     # counter + 14 numbers
+    point = {"k_xyz_L_adv": k, "k_xyz_R_adv": k[::-1].copy()}
+    for name in ("trl", "tlr", "rrr", "rll"):
+        point["spec_" + name] = np.stack([tot, spec], axis=1)
+    OUT.write_mode_tables(str(tmp_path), 7, 2.5, point)
+    OUT.write_mode_tables(str(tmp_path), 7, 2.6, point)        # appends, like POSITION='APPEND'
+    for name in ("tlr", "trl", "right", "left", "mdl", "mdr"):
+        lines = open(tmp_path / ("%s_specular_param_000007.dat" % name)).read().splitlines()
+        assert len(lines) == 6

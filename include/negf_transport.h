@@ -388,8 +388,10 @@ int nt_hk_assemble_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const
                        long long kstride);
 
 /* ------------------------------------------------ autotuning
- * The batched inverse runs blocked Gauss-Jordan with a panel width of 8, 16 or
- * 32 columns; 16 is the default where it fits (else 8).  nt_autotune_inverse
+ * The batched inverse runs blocked Gauss-Jordan in one workgroup per matrix
+ * with a panel width of 8, 16 or 32 columns, or, as "width" 1 (n >= 64), with
+ * one panel kernel per panel and the trailing updates done by the batched GEMM.
+ * Default: width 16 where it fits, width 1 from n = 320.  nt_autotune_inverse
  * times every supported width on `batch` n x n matrices and keeps the fastest
  * for this context (precision NT_PREC_FP64: complex double, NT_PREC_MIXED:
  * complex float); best_nb / best_ms may be NULL.  nt_set_inverse_panel sets a

@@ -1964,7 +1964,7 @@ void tune_inverse(nt_ctx* c, int n, int batch, int* best_nb, double* best_ms)
     reg_region(c, X.p, sizeof(T) * 2 * nn * batch);
     reg_region(c, info.p, sizeof(int) * batch);
     reg_region(c, z.p, sizeof(T) * 2 * batch);
-    int widths[3];
+    int widths[INV_MAX_WIDTHS];
     const int k = inv_panel_widths<T>(n, widths);
     hipEvent_t e0, e1;
     hipcheck(c, hipEventCreate(&e0), "event");
@@ -2805,7 +2805,7 @@ int nt_set_inverse_panel(nt_ctx* c, int n, int precision, int nb)
         c->inv_nb.erase({bytes, n});
         return NT_OK;
     }
-    int w[3];
+    int w[INV_MAX_WIDTHS];
     const int k = precision == NT_PREC_FP64 ? inv_panel_widths<double>(n, w) : inv_panel_widths<float>(n, w);
     for (int i = 0; i < k; ++i)
         if (w[i] == nb) {

@@ -393,12 +393,347 @@ hipError_t launch_inv_nb(const InvArgs<T>& a, hipStream_t s)
     return hipErrorInvalidValue;
 }
 
+
+// ------------------------------------------------------------------------
+// Blocked path: the same Gauss-Jordan steps with the trailing update of every
+// panel done by the batched GEMM (nt_gemm.hip) instead of inside the
+// workgroup.  Per panel J (NB columns from j0):
+//   k_inv_panel  sweeps the n x NB panel in registers as above, writes the
+//                swept panel W to the panel columns and to Wb (n x NB), applies
+//                the panel's row exchanges to every other column (explicitly,
+//                in pivot order), copies the pivot rows of the other columns
+//                to Pb (NB x n, zero in the panel columns) and zeroes them;
+//   GEMM         C = C + Wb Pb  (in place), i.e. new[:, q] = old[:, q] with
+//                rows J zeroed + W old[J, q] for every column outside J.
+// k_inv_final undoes the column order (sigma) into X.  Wb, Pb and the pivot
+// record live in X, which is only written by k_inv_final: per item, complex
+// elements [0, n NB) hold Wb, [n NB, 2 n NB) Pb, then n ints ipiv and two
+// status words (NaN seen, first zero pivot).
+
+template <typename T>
+struct InvScratch {
+    __device__ static T* wb(T* X, int n, int nb) { return X; }
+    __device__ static T* pb(T* X, int n, int nb) { return X + 2L * n * nb; }
+    __device__ static int* ipiv(T* X, int n, int nb) { return reinterpret_cast<int*>(X + 4L * n * nb); }
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void k_inv_prologue(InvArgs<T> a)
+{
+    const int p = blockIdx.x;
+    if (a.active && !a.active[p]) return;
+    const int n = a.n;
+    const long e = (long)blockIdx.y * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int r = (int)(e % n), c = (int)(e / n);
+    const T* __restrict__ S = bptr(a.src, p, 0);
+    T* __restrict__ W = bptr(a.W, p, 0);
+    const T zr = a.z[2 * p], zi = a.z[2 * p + 1];
+    typedef typename vec2<T>::type V2;
+    V2 v = *reinterpret_cast<const V2*>(S + 2 * ((long)r + (long)c * a.src.ld));
+    V2 o;
+    o.x = (r == c ? zr : (T)0) - v.x;
+    o.y = (r == c ? zi : (T)0) - v.y;
+    *reinterpret_cast<V2*>(W + 2 * ((long)r + (long)c * a.W.ld)) = o;
+}
+
+template <typename T, int NB, int RPT>
+__global__ __launch_bounds__(NT) void k_inv_panel(InvArgs<T> a, int j0)
+{
+    typedef typename vec2<T>::type V2;
+    const int p = blockIdx.x;
+    if (a.active && !a.active[p]) return;
+    const int n = a.n;
+    const int nbe = min(NB, n - j0);
+    const int t = threadIdx.x;
+    const int lane = lane_id();
+    const int wave = wave_of();
+    __shared__ InvSmemFixed<T, NB> s;
+    __shared__ int piv_s[NB];
+
+    T* __restrict__ C = bptr(a.W, p, 0);
+    const int ld = a.W.ld;
+    T* __restrict__ X = bptr(a.X, p, 0);
+    T* __restrict__ Wb = InvScratch<T>::wb(X, n, NB);
+    T* __restrict__ Pb = InvScratch<T>::pb(X, n, NB);
+    int* __restrict__ ipiv = InvScratch<T>::ipiv(X, n, NB);
+    int* __restrict__ stat = ipiv + n;     // [0] NaN seen, [1] first zero pivot (1-based)
+
+    if (t == 0) {
+        s.nan_flag = j0 == 0 ? 0 : stat[0];
+        s.info = j0 == 0 ? 0 : stat[1];
+    }
+    __syncthreads();
+    Cx<T> x[RPT][NB];
+#pragma unroll
+    for (int sl = 0; sl < RPT; ++sl) {
+        const int r = t + sl * NT;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            Cx<T> v = {0, 0};
+            if (r < n && c < nbe) {
+                V2 q = *reinterpret_cast<const V2*>(C + 2 * ((long)r + (long)(j0 + c) * ld));
+                v.r = q.x;
+                v.i = q.y;
+                if (q.x != q.x || q.y != q.y) s.nan_flag = 1;
+            }
+            x[sl][c] = v;
+        }
+    }
+    __syncthreads();
+    int step = 0;
+    for (int cl = 0; cl < nbe; ++cl) {
+        const int c = j0 + cl;
+        T bv = (T)-1;
+        int bi = n;
+#pragma unroll
+        for (int sl = 0; sl < RPT; ++sl) {
+            const int r = t + sl * NT;
+            T v = (T)-1;
+#pragma unroll
+            for (int cc = 0; cc < NB; ++cc)
+                if (cc == cl) v = cabs1(x[sl][cc].r, x[sl][cc].i);
+            if (r < n && r >= c && v > bv) {
+                bv = v;
+                bi = r;
+            }
+        }
+        double dv = (double)bv;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(dv, off);
+            const int oi = __shfl_xor(bi, off);
+            if (ov > dv || (ov == dv && oi < bi)) {
+                dv = ov;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            s.redv[step][wave] = dv;
+            s.redi[step][wave] = bi;
+        }
+        __syncthreads();
+        double best = s.redv[step][0];
+        int piv = s.redi[step][0];
+#pragma unroll
+        for (int w = 1; w < NT / 64; ++w) {
+            const double ov = s.redv[step][w];
+            const int oi = s.redi[step][w];
+            if (ov > best || (ov == best && oi < piv)) {
+                best = ov;
+                piv = oi;
+            }
+        }
+        if (piv >= n) piv = c;
+#pragma unroll
+        for (int sl = 0; sl < RPT; ++sl) {
+            const int r = t + sl * NT;
+            if (r == piv) {
+                T pr = 0, pim = 0;
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc)
+                    if (cc == cl) {
+                        pr = x[sl][cc].r;
+                        pim = x[sl][cc].i;
+                    }
+                const T den = pr * pr + pim * pim;
+                T dr = 0, di = 0;
+                if (den != (T)0) {
+                    dr = pr / den;
+                    di = -pim / den;
+                } else if (s.info == 0) {
+                    s.info = c + 1;
+                }
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc) {
+                    const T vr = x[sl][cc].r, vi = x[sl][cc].i;
+                    s.xr[step][0][cc] = (cc == cl) ? dr : vr * dr - vi * di;
+                    s.xi[step][0][cc] = (cc == cl) ? di : vr * di + vi * dr;
+                }
+            }
+            if (r == c) {
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc) {
+                    s.xr[step][1][cc] = x[sl][cc].r;
+                    s.xi[step][1][cc] = x[sl][cc].i;
+                }
+            }
+        }
+        if (t == 0) piv_s[cl] = piv;
+        __syncthreads();
+#pragma unroll
+        for (int sl = 0; sl < RPT; ++sl) {
+            const int r = t + sl * NT;
+            if (r >= n) continue;
+            if (r == c) {
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc) {
+                    x[sl][cc].r = s.xr[step][0][cc];
+                    x[sl][cc].i = s.xi[step][0][cc];
+                }
+            } else {
+                if (r == piv) {
+#pragma unroll
+                    for (int cc = 0; cc < NB; ++cc) {
+                        x[sl][cc].r = s.xr[step][1][cc];
+                        x[sl][cc].i = s.xi[step][1][cc];
+                    }
+                }
+                T fr = 0, fi = 0;
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc)
+                    if (cc == cl) {
+                        fr = x[sl][cc].r;
+                        fi = x[sl][cc].i;
+                    }
+#pragma unroll
+                for (int cc = 0; cc < NB; ++cc) {
+                    const T sr = s.xr[step][0][cc], si = s.xi[step][0][cc];
+                    const T br = (cc == cl) ? (T)0 : x[sl][cc].r;
+                    const T bim = (cc == cl) ? (T)0 : x[sl][cc].i;
+                    x[sl][cc].r = br - (fr * sr - fi * si);
+                    x[sl][cc].i = bim - (fr * si + fi * sr);
+                }
+            }
+        }
+        step ^= 1;
+    }
+    // swept panel -> panel columns of C and Wb (zero columns beyond nbe)
+#pragma unroll
+    for (int sl = 0; sl < RPT; ++sl) {
+        const int r = t + sl * NT;
+        if (r < n) {
+#pragma unroll
+            for (int cc = 0; cc < NB; ++cc) {
+                V2 q;
+                q.x = cc < nbe ? x[sl][cc].r : (T)0;
+                q.y = cc < nbe ? x[sl][cc].i : (T)0;
+                if (cc < nbe) *reinterpret_cast<V2*>(C + 2 * ((long)r + (long)(j0 + cc) * ld)) = q;
+                *reinterpret_cast<V2*>(Wb + 2 * ((long)r + (long)cc * n)) = q;
+            }
+        }
+    }
+    __threadfence_block();
+    // row exchanges of the panel, in pivot order, on every other column
+    for (int cl = 0; cl < nbe; ++cl) {
+        const int c = j0 + cl, q = piv_s[cl];
+        if (q != c) {
+            for (int col = t; col < n; col += NT) {
+                if (col >= j0 && col < j0 + nbe) continue;
+                V2* a0 = reinterpret_cast<V2*>(C + 2 * ((long)c + (long)col * ld));
+                V2* a1 = reinterpret_cast<V2*>(C + 2 * ((long)q + (long)col * ld));
+                const V2 u = *a0, v = *a1;
+                *a0 = v;
+                *a1 = u;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+    // pivot rows of the other columns -> Pb (NB x n), then zeroed in C
+    for (long e = t; e < (long)NB * n; e += NT) {
+        const int k = (int)(e % NB), col = (int)(e / NB);
+        V2 q;
+        q.x = 0;
+        q.y = 0;
+        if (k < nbe && !(col >= j0 && col < j0 + nbe)) {
+            V2* src = reinterpret_cast<V2*>(C + 2 * ((long)(j0 + k) + (long)col * ld));
+            q = *src;
+            V2 z;
+            z.x = 0;
+            z.y = 0;
+            *src = z;
+        }
+        *reinterpret_cast<V2*>(Pb + 2 * e) = q;
+    }
+    if (t < nbe) ipiv[j0 + t] = piv_s[t];
+    if (t == 0) {
+        stat[0] = s.nan_flag;
+        stat[1] = s.info;
+    }
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(NT) void k_inv_final(InvArgs<T> a)
+{
+    typedef typename vec2<T>::type V2;
+    const int p = blockIdx.x;
+    if (a.active && !a.active[p]) return;
+    const int n = a.n;
+    __shared__ int sigma[1024];
+    T* __restrict__ C = bptr(a.W, p, 0);
+    T* __restrict__ X = bptr(a.X, p, 0);
+    const int* ipiv = InvScratch<T>::ipiv(X, n, NB);
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < n; ++j) sigma[j] = j;
+        for (int c = n - 1; c >= 0; --c) {
+            const int q = ipiv[c];
+            if (q != c) {
+                const int tmp = sigma[c];
+                sigma[c] = sigma[q];
+                sigma[q] = tmp;
+            }
+        }
+        const int* stat = ipiv + n;
+        const int v = stat[0] ? -1 : stat[1];
+        if (a.info && v != 0 && a.info[p] == 0) a.info[p] = v;
+    }
+    __syncthreads();
+    for (long e = threadIdx.x; e < (long)n * n; e += NT) {
+        const int r = (int)(e % n), j = (int)(e / n);
+        V2 q = *reinterpret_cast<const V2*>(C + 2 * ((long)r + (long)sigma[j] * a.W.ld));
+        *reinterpret_cast<V2*>(X + 2 * ((long)r + (long)j * a.X.ld)) = q;
+    }
+}
+
+template <typename T, int NB, int RPT>
+hipError_t launch_inv_blocked_t(const InvArgs<T>& a, hipStream_t s)
+{
+    const int n = a.n;
+    if (a.src.base) {
+        hipLaunchKernelGGL(k_inv_prologue<T>, dim3(a.batch, ((long)n * n + NT - 1) / NT), dim3(NT), 0, s, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    GemmArgs<T> g;
+    g.M = n;
+    g.N = n;
+    g.K = NB;
+    g.opA = OP_N;
+    g.opB = OP_N;
+    g.A = BMat<T>{a.X.base, a.X.stride, 0, nullptr, n, 0};
+    g.B = BMat<T>{a.X.base + 2L * n * NB, a.X.stride, 0, nullptr, NB, 0};
+    g.Cin = BMat<T>{a.W.base, a.W.stride, 0, a.W.idx, a.W.ld, a.W.idx_n};
+    g.C = a.W;
+    g.C2 = BMatW<T>{nullptr, 0, 0, nullptr, 0, 0};
+    g.alpha_re = 1;
+    g.alpha_im = 0;
+    g.beta_re = 1;
+    g.beta_im = 0;
+    g.diag = nullptr;
+    g.active = a.active;
+    g.maxabs = nullptr;
+    g.batch = a.batch;
+    g.groups = 1;
+    g.m3 = 0;
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        hipLaunchKernelGGL((k_inv_panel<T, NB, RPT>), dim3(a.batch), dim3(NT), 0, s, a, j0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        e = launch_gemm<T>(g, s);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_inv_final<T, NB>), dim3(a.batch), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 int inv_max_n() { return 1024; }
 
 // Largest n each panel width supports (LDS copy of the panel <= ~130 KB and
-// rows per thread within the kernel's instantiation).
+// rows per thread within the kernel's instantiation); width INV_BLOCKED (1)
+// is the panel-kernel + batched-GEMM path, for 64 <= n <= 1024.
 template <typename T>
 int inv_panel_widths(int n, int* out)
 {
@@ -407,25 +742,54 @@ int inv_panel_widths(int n, int* out)
     if (n <= lim32) out[k++] = 32;
     if (n <= lim16) out[k++] = 16;
     if (n <= inv_max_n()) out[k++] = 8;
+    if (n >= 64 && n <= inv_max_n()) out[k++] = INV_BLOCKED;
     return k;
 }
 template int inv_panel_widths<double>(int, int*);
 template int inv_panel_widths<float>(int, int*);
+
+// The blocked path keeps its scratch in X: X must be compact (ld = n, item
+// stride >= n^2, no item map) and n large enough for 2 n NB complex + n + 2
+// ints.  Panel width 32 for n <= 256 (one row per thread), 16 up to 512, else 8.
+template <typename T>
+bool inv_blocked_ok(const InvArgs<T>& a)
+{
+    const int nb = a.n <= 256 ? 32 : (a.n <= 512 ? 16 : 8);
+    const double need = 2.0 * a.n * nb * 2 * sizeof(T) + sizeof(int) * (a.n + 2.0);
+    return a.X.ld == a.n && !a.X.idx && a.X.stride >= (long)a.n * a.n && a.n >= 64 && a.n <= 1024 &&
+           need <= 2.0 * sizeof(T) * a.n * a.n;
+}
+
+template <typename T>
+hipError_t launch_inv_blocked(const InvArgs<T>& a, hipStream_t s)
+{
+    const int rpt = (a.n + NT - 1) / NT;
+    if (rpt == 1) return launch_inv_blocked_t<T, 32, 1>(a, s);
+    if (rpt == 2) return launch_inv_blocked_t<T, 16, 2>(a, s);
+    if (rpt == 3) return launch_inv_blocked_t<T, 8, 3>(a, s);
+    return launch_inv_blocked_t<T, 8, 4>(a, s);
+}
 
 template <typename T>
 hipError_t launch_inv_any(const InvArgs<T>& a, hipStream_t s)
 {
     if (a.batch <= 0 || a.n <= 0) return hipSuccess;
     if (a.n > inv_max_n()) return hipErrorInvalidValue;
-    int w[3];
+    int w[INV_MAX_WIDTHS];
     const int k = inv_panel_widths<T>(a.n, w);
     // default: 16 columns where it fits (measured fastest at b = 192: two
-    // workgroups per CU against one with 32), else the widest that fits
+    // workgroups per CU against one with 32), else the widest that fits; the
+    // panel + GEMM path from n = 320 (1.6x the 16-column kernel at n = 384)
     int nb = w[0];
     for (int i = 0; i < k; ++i)
         if (w[i] == 16) nb = 16;
+    if (a.n >= 320 && inv_blocked_ok(a)) nb = INV_BLOCKED;
     for (int i = 0; i < k; ++i)
         if (a.nb == w[i]) nb = a.nb;
+    if (nb == INV_BLOCKED) {
+        if (inv_blocked_ok(a)) return launch_inv_blocked<T>(a, s);
+        nb = w[0] == INV_BLOCKED ? 8 : w[0];    // scratch does not fit in X: one-workgroup path
+    }
     if (nb == 32) return launch_inv_nb<T, 32, 1>(a, s);
     if (nb == 16) return launch_inv_nb<T, 16, 2>(a, s);
     return launch_inv_nb<T, 8, 4>(a, s);

@@ -45,7 +45,10 @@ template <typename T> struct InvArgs {
 
 template <typename T> hipError_t launch_inv(const InvArgs<T>& a, hipStream_t s);
 int inv_max_n();
-// panel widths launch_inv accepts for an n x n matrix of type T (largest first)
+// panel widths launch_inv accepts for an n x n matrix of type T (largest first);
+// INV_BLOCKED selects the panel-kernel + batched-GEMM path (last in the list)
+constexpr int INV_BLOCKED = 1;
+constexpr int INV_MAX_WIDTHS = 4;
 template <typename T> int inv_panel_widths(int n, int* out);
 
 // Batched complex Schur / eigen / square-root kernels (nt_eig.hip), FP64.

@@ -27,7 +27,7 @@ REF_EXAMPLE = "/root/reference/examples/negf/si-si"
 def make_fixture():
     """Copy the example's input and output data files into tests/golden/si_si."""
     os.makedirs(FIXTURE, exist_ok=True)
-    for f in ("Si444.fc", "mass_profile.dat", "config"):
+    for f in ("Si444.fc", "mass_profile.dat", "config", "pc1.fdf"):
         shutil.copy(os.path.join(REF_EXAMPLE, f), os.path.join(FIXTURE, f))
     for f in ("transl_tot.dat", "transr_tot.dat"):
         shutil.copy(os.path.join(REF_EXAMPLE, "output", f), os.path.join(FIXTURE, f))
@@ -51,6 +51,9 @@ def make_fixture():
 # k-points (0-based) whose per-mode tables are kept as fixtures
 MODE_KPOINTS = (0, 10, 24, 40)
 MD_KPOINTS = (10, 24)
+# species of the example's structure files: 1 Ge, 2 Si (element_mass, negf/input.f90:410-440;
+# REAL(4) literals, as the Fortran compiler rounds them)
+SPECIES_MASS = {1: float(np.float32(72.64)), 2: float(np.float32(28.0855))}
 SPLIT_TABLES = ("transl_s", "transl_ns", "transr_s", "transr_ns", "refll_s", "refll_ns", "reflr_s", "reflr_ns")
 
 

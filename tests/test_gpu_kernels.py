@@ -137,6 +137,8 @@ def test_zinv_every_panel_width(engine, torch_dev, n):
     A = [crand(rng, n, n) + 0.5 * n ** 0.5 * np.eye(n) for _ in range(batch)]
     A[0][0, 0] = 0.0
     widths = [w for w, lim in ((32, 192), (16, 384), (8, 1024)) if n <= lim]
+    if n >= 64:
+        widths.append(1)        # panel kernels + batched GEMM
     print(f"zinv panels n={n} widths={widths}", flush=True)
     for w in widths:
         engine.set_inverse_panel(n, w)
@@ -213,3 +215,29 @@ def test_zgemm_batch_multiple_of_xcds(engine, torch_dev, batch, groups_like):
         for i in range(batch):
             ref = A[i] @ B[i] + C0[i]
             assert np.abs(got[i] - ref).max() < 2e-12 * np.abs(ref).max() * k ** 0.5, (m3, i)
+
+
+@pytest.mark.parametrize("width", [16, 1])
+def test_zinv_singular_and_nan_n96(engine, torch_dev, width):
+    """status codes of the one-workgroup and the panel + GEMM inverse: the
+    first exact zero pivot (1-based column), -1 for NaN input"""
+    from phinfortran_amd import _native as N
+    torch, dev = torch_dev
+    n = 96
+    rng = np.random.default_rng(1)
+    S = crand(rng, n, n)
+    S[:, 40] = 0.0                 # exact zero column -> zero pivot at column 41
+    Nn = crand(rng, n, n)
+    Nn[70, 50] = np.nan
+    dA = to_dev(torch, dev, [S, Nn])
+    dX = torch.zeros_like(dA)
+    info = torch.zeros(2, dtype=torch.int32, device=dev)
+    engine.set_inverse_panel(n, width)
+    st = N.lib().nt_zinv_batched_dev(engine.ctx, n, C.c_void_p(dA.data_ptr()), n * n, C.c_void_p(dX.data_ptr()),
+                                     n * n, C.c_void_p(info.data_ptr()), 2)
+    engine.set_inverse_panel(n, 0)
+    assert st == 0
+    engine.sync()
+    inf = info.cpu().numpy()
+    assert inf[0] == 41
+    assert inf[1] == -1

@@ -1,0 +1,109 @@
+"""The primitive-cell unfolding oracle (oracle/kpc.py, find_k_pc) against the
+reference's committed per-mode tables (columns kp_b and v_b of trl / mdl /
+mdr) and against what the mathematics fixes: in_ws on a cubic lattice, the
+acoustic sum rule at Gamma, the velocity as the slope of the band."""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import kpc as K
+from oracle import modes as M
+from oracle import unfold as UF
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import si_si as S  # noqa: E402
+
+SETS = ("L_adv", "L_ret", "R_ret", "R_adv")
+
+
+def test_in_ws_cubic():
+    a = np.eye(3) * 2.0
+    assert K.in_ws(a, np.array([0.3, -0.2, 0.1])) == 1          # nearest lattice point: origin
+    assert K.in_ws(a, np.array([1.0, 0.2, 0.0])) == 2           # on the zone boundary
+    assert K.in_ws(a, np.array([1.7, 0.0, 0.0])) == 0           # nearer to (2, 0, 0)
+
+
+@pytest.fixture(scope="module")
+def si():
+    cfg = S.read_config()
+    qe = S.read_qe_fc(cfg)
+    pc = K.read_pc_fdf(os.path.join(S.FIXTURE, "pc1.fdf"), S.SPECIES_MASS)
+    dyn = K.PcDynamics(qe, pc["natoms_pc"], pc["mass_uc"])
+    return dict(cfg=cfg, qe=qe, pc=pc, dyn=dyn)
+
+
+def test_primitive_cell_reader(si):
+    pc = si["pc"]
+    assert pc["natoms_pc"] == 2 and len(pc["mass_uc"]) == 4
+    # fcc primitive vectors of a = 5.46616 A, rotated by pi/4 about z: |a_i| = a / sqrt(2)
+    assert np.allclose(np.linalg.norm(pc["latvec_pc"], axis=1), 2.73308195797 * math.sqrt(2))
+    assert np.allclose(pc["recivec_pc"] @ pc["latvec_pc"].T, 2 * np.pi * np.eye(3))
+
+
+def test_dynamical_matrix_sum_rule_and_slope(si):
+    dyn = si["dyn"]
+    e0, _, _ = dyn.group_velocity(np.zeros(3))
+    assert np.abs(e0[:3]).max() < 1e-6 * np.abs(e0).max()      # three acoustic zeros at Gamma
+    k = np.array([0.21, -0.13, 0.37])
+    e, _, v = dyn.group_velocity(k)
+    h = 1e-4
+    for i in range(3):
+        dk = np.zeros(3)
+        dk[i] = h
+        ep = np.linalg.eigvalsh(dyn.dyn(k + dk))
+        em = np.linalg.eigvalsh(dyn.dyn(k - dk))
+        slope = (np.sqrt(ep) - np.sqrt(em)) / (2 * h)           # d omega / dk of every (non-degenerate) band
+        assert np.abs(slope - v[:, i]).max() < 1e-5 * max(1.0, np.abs(v).max())
+
+
+def rows_key(a):
+    return a[np.lexsort((np.round(a[:, 1], 6), np.round(a[:, 0], 6), np.round(a[:, 2], 6)))]
+
+
+@pytest.mark.parametrize("ik", [10, 24])
+def test_si_si_primitive_cell_columns(si, ik):
+    """kp_b and v_b of every row of the committed trl and mdr tables."""
+    cfg, qe, pc, dyn = si["cfg"], si["qe"], si["pc"], si["dyn"]
+    asm = S.TdbAssembler(qe, cfg)
+    ks, E = S.k_grid(cfg), S.e_grid(cfg)
+    az = qe["cell"][2, 2]
+    pos = qe["pos_all"]
+    N, per, buf = len(pos), cfg["period"], cfg["buffer"]
+    nbx, nby = cfg["n_bloch_x"], cfg["n_bloch_y"]
+    setl = UF.transverse_uc_setup(pos, qe["cell"], buf + per, per, nbx, nby, 1e-3)
+    setr = UF.transverse_uc_setup(pos, qe["cell"], N - buf - 2 * per, per, nbx, nby, 1e-5)
+    recivec = 2 * np.pi * np.linalg.inv(qe["cell_all"]).T
+    reci_uc = 2 * np.pi * np.linalg.inv(qe["cell"]).T
+    p = O.OrParams(cfg["eta"], cfg["eta0"], cfg["convergence"], 10.0, O.ETA_RELATIVE, 500)
+    gold = {"trl": S.golden_rows("trl", ik), "mdr": S.golden_rows("mdr", ik)}
+    blk = asm.blocks(ks[ik])
+    nrow = nv = 0
+    for ie in range(1, len(E), 4):
+        w = round(math.sqrt(E[ie]), 8)
+        if w not in gold["trl"]:
+            continue
+        m = M.solve_point_modes(blk, E[ie], p, 5e-3, az, az)
+        kz = [m["k_z_" + k] for k in SETS]
+        if any(len(z[z < 99.9]) > 1 and (np.diff(np.sort(z[z < 99.9])) < 1e-6).any() for z in kz):
+            continue                      # degenerate modes: labels follow the eigenvector basis
+        U = {k: m["U_" + k] for k in SETS}
+        lam = {k: m["lam_" + k] for k in SETS}
+        bz, kx, sp = UF.unfold_point(m, U, lam, ks[ik], recivec, reci_uc, reci_uc, nbx, nby, pos, setl, setr)
+        for name, key in (("trl", "L_adv"), ("mdr", "R_adv")):
+            ours = []
+            for i in np.where(kx[key][:, 2] < 99.9)[0]:
+                kp, v = K.mode_pc(E[ie], kx[key][i], reci_uc, pc["recivec_pc"], nbx, nby, dyn)
+                ours.append(list(kx[key][i]) + list(kp) + list(v))
+            a, g = rows_key(np.array(ours)), rows_key(gold[name][w][:, 1:10])
+            assert a.shape == g.shape, (ik, ie, name)
+            assert np.abs(a[:, :6] - g[:, :6]).max() < 1e-7, (ik, ie, name)
+            assert np.abs(a[:, 6:] - g[:, 6:]).max() < 1e-6 * max(1.0, np.abs(g[:, 6:]).max()), (ik, ie, name)
+            nrow += len(a)
+            nv += int((np.abs(g[:, 6:]).sum(axis=1) > 0).sum())
+    print(f"k-point {ik}: {nrow} rows, {nv} with a velocity")
+    assert nrow > 40

@@ -161,6 +161,18 @@ module negf_transport
             type(c_ptr), value :: status
         end function
 
+        ! find_k_pc (negf/dyn.f90:770-889) for nmodes modes; reci_uc / recivec_pc
+        ! row-major (reshape(transpose(reci_uc_l), [9])); other pointers device memory
+        integer(c_int) function nt_unfold_pc_dev(ctx, nmodes, E, k_xyz, n_bloch_max, reci_uc, recivec_pc, n_pc, &
+                nseg, seg_off, seg_rc, nent, coef, dr, kp, v, ncand) bind(C, name="nt_unfold_pc_dev")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nmodes, n_bloch_max, n_pc, nseg, nent
+            type(c_ptr), value :: E, k_xyz
+            real(c_double), intent(in) :: reci_uc(9), recivec_pc(9)
+            type(c_ptr), value :: seg_off, seg_rc, coef, dr, kp, v, ncand
+        end function
+
         ! vel_mat_tdb with the Brillouin-zone unfolding (velocity_mat.f90:2020-2130, :3032-3099)
         integer(c_int) function nt_modes_unfold_tdb_batched_dev(ctx, p, tb, nl, nr, npts, E, kidx, nk, Haml, &
                 Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, &

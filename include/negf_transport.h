@@ -373,6 +373,29 @@ int nt_modes_unfold_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int tb, int
                                     const nt_modes_out* out, const nt_unfold_desc* uf,
                                     const nt_unfold_out* uo, int* status);
 
+/* find_k_pc (negf/dyn.f90:770-889) for nmodes modes, as vel_mat_tdb calls it
+ * for every propagating mode (negf/velocity_mat.f90:2073-2089 and mirrors):
+ *   E        nmodes: the point's energy (omega^2) of each mode
+ *   k_xyz    nmodes x 3 (row-major): the mode's unfolded k (nt_unfold_out
+ *            k_xyz_*, k_z third); k_z >= 99.9 marks an evanescent mode
+ *   n_bloch_max  max(n_bloch_x, n_bloch_y): candidates k + n reci_uc, n in [-l, l]^3
+ *   reci_uc, recivec_pc  host, 9 doubles each, rows = reciprocal vectors of the
+ *            lead unit cell and of the primitive cell (read_pc_qe,
+ *            negf/input.f90:688-720)
+ *   n_pc     3 x atoms of the primitive cell (<= 48)
+ *   nseg ... dr  the primitive cell's dynamical matrix as a phase-sum table in
+ *            the format of nt_hk_assemble_dev (gen_dyn_pc_qe_k,
+ *            negf/dyn.f90:2537-2644: one segment per 3x3 block (i, j mod
+ *            natoms_pc)); the matrix is used as (D + D^H)/2
+ * Outputs (device): kp (nmodes x 3): the matching candidate of smallest |k|
+ * (k_xyz_*_pc), 0 when none matches, 100 for evanescent modes; v (nmodes x 3):
+ * the first matching candidate's group velocity (vel_xyz_*_pc); ncand: the
+ * number of matching candidates.  All other pointers are device memory. */
+int nt_unfold_pc_dev(nt_ctx* ctx, int nmodes, const double* E, const double* k_xyz, int n_bloch_max,
+                     const double* reci_uc, const double* recivec_pc, int n_pc, int nseg, const int* seg_off,
+                     const int* seg_rc, int nent, const double* coef, const double* dr, double* kp, double* v,
+                     int* ncand);
+
 /* ------------------------------------------------ H(k) assembly
  * gen_dyn_qe_tdb_k (negf/dyn.f90:1482-2080) on the GPU: one block for nk
  * k-points.  The host tabulates the k-independent part as nseg segments (one

@@ -3013,6 +3013,135 @@ int nt_modes_unfold_tdb_batched_dev(nt_ctx* c, const nt_params* p, int tb, int n
                        uf, uo, status);
 }
 
+int nt_unfold_pc_dev(nt_ctx* c, int nmodes, const double* E, const double* k_xyz, int n_bloch_max,
+                     const double* reci_uc, const double* recivec_pc, int n_pc, int nseg, const int* seg_off,
+                     const int* seg_rc, int nent, const double* coef, const double* dr, double* kp, double* v,
+                     int* ncand)
+{
+    if (!c || nmodes < 0 || n_bloch_max < 0 || n_bloch_max > 4 || !reci_uc || !recivec_pc || n_pc <= 0 ||
+        n_pc % 3 || n_pc > KPC_MAX_N || nseg < 0 || nent < 0 || (nseg > 0 && (!seg_off || !seg_rc || !coef || !dr)))
+        return NT_ERR_ARG;
+    if (nmodes == 0) return NT_OK;
+    if (!E || !k_xyz || !kp || !v || !ncand) return NT_ERR_ARG;
+    return guarded(c, [&] {
+        if (nseg > 0) {
+            std::vector<int> off(nseg + 1), rc(2 * (size_t)nseg);
+            hipcheck(c, hipMemcpy(off.data(), seg_off, sizeof(int) * (nseg + 1), hipMemcpyDeviceToHost), "D2H");
+            hipcheck(c, hipMemcpy(rc.data(), seg_rc, sizeof(int) * 2 * nseg, hipMemcpyDeviceToHost), "D2H");
+            if (off[0] != 0 || off[nseg] != nent) {
+                c->err = "unfold_pc: segment offsets must run from 0 to nent";
+                return (int)NT_ERR_ARG;
+            }
+            for (int q = 0; q < nseg; ++q)
+                if (off[q + 1] < off[q] || rc[2 * q] < 0 || 3 * rc[2 * q] + 3 > n_pc || rc[2 * q + 1] < 0 ||
+                    3 * rc[2 * q + 1] + 3 > n_pc) {
+                    c->err = "unfold_pc: segment outside the matrix";
+                    return (int)NT_ERR_ARG;
+                }
+        }
+        reg_region(c, E, sizeof(double) * nmodes);
+        reg_region(c, k_xyz, sizeof(double) * 3 * (size_t)nmodes);
+        reg_region(c, kp, sizeof(double) * 3 * (size_t)nmodes);
+        reg_region(c, v, sizeof(double) * 3 * (size_t)nmodes);
+        reg_region(c, ncand, sizeof(int) * (size_t)nmodes);
+        reg_region(c, seg_off, sizeof(int) * (nseg + 1));
+        reg_region(c, seg_rc, sizeof(int) * 2 * nseg);
+        reg_region(c, coef, sizeof(double) * 9 * (size_t)nent);
+        reg_region(c, dr, sizeof(double) * 3 * (size_t)nent);
+        KpcArgs a;
+        a.n = n_pc;
+        a.E = E;
+        a.k = k_xyz;
+        for (int q = 0; q < 9; ++q) {
+            a.reci_uc[q] = reci_uc[q];
+            a.recivec_pc[q] = recivec_pc[q];
+        }
+        a.l = n_bloch_max;
+        a.maxc = (2 * a.l + 1) * (2 * a.l + 1) * (2 * a.l + 1);
+        a.kp = kp;
+        a.v = v;
+        a.ncand = ncand;
+        // modes per chunk; candidates per matrix batch (7 matrices each, kept under
+        // 512 MB and 65536 phase-sum launches rows)
+        const int Mmax = 8192;
+        const size_t per_cand = 7 * (size_t)n_pc * n_pc * 2 * sizeof(double) + 7 * 3 * sizeof(double);
+        const int NCsub = (int)std::max<size_t>(1, std::min<size_t>(9000, (size_t)(512u << 20) / per_cand));
+        Arena pr;
+        const size_t o_cnt = pr.take(sizeof(int) * Mmax);
+        const size_t o_cand = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
+        const size_t o_off = pr.take(sizeof(int) * Mmax);
+        const size_t o_cm = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
+        const size_t o_cs = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
+        const size_t o_kc = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
+        const size_t o_found = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
+        const size_t o_vc = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
+        const size_t o_kl = pr.take(sizeof(double) * 21 * (size_t)NCsub);
+        const size_t o_D = pr.take(sizeof(double) * 14 * (size_t)NCsub * n_pc * n_pc);
+        ensure_ws(c, pr.off);
+        Arena ar;
+        ar.base = static_cast<char*>(c->ws);
+        a.count = ar.ptr<int>(o_cnt);
+        a.cand = ar.ptr<double>(o_cand);
+        int* doff = ar.ptr<int>(o_off);
+        int* dcm = ar.ptr<int>(o_cm);
+        int* dcs = ar.ptr<int>(o_cs);
+        double* dkc = ar.ptr<double>(o_kc);
+        int* dfound = ar.ptr<int>(o_found);
+        double* dvc = ar.ptr<double>(o_vc);
+        double* dkl = ar.ptr<double>(o_kl);
+        double* dD = ar.ptr<double>(o_D);
+        std::vector<int> cnt, off, cm, cs;
+        for (int m0 = 0; m0 < nmodes; m0 += Mmax) {
+            const int M = std::min(Mmax, nmodes - m0);
+            check_range(c, k_xyz + 3L * m0, sizeof(double) * 3 * (size_t)M, "unfold_pc k");
+            launch(c, NT_K_ELEM, 0.0, "kpc candidates", [&] { return launch_kpc_candidates(a, m0, M, c->stream); });
+            cnt.resize(M);
+            hipcheck(c, hipMemcpyAsync(cnt.data(), a.count, sizeof(int) * M, hipMemcpyDeviceToHost, c->stream),
+                     "D2H");
+            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+            off.assign(M, 0);
+            cm.clear();
+            cs.clear();
+            for (int i = 0; i < M; ++i) {
+                off[i] = (int)cm.size();
+                const int nci = cnt[i] == -2 ? 1 : std::max(0, cnt[i]);
+                if (nci > a.maxc) {
+                    c->err = "unfold_pc: candidate count out of range";
+                    return (int)NT_ERR_HIP;
+                }
+                for (int q = 0; q < nci; ++q) {
+                    cm.push_back(i);
+                    cs.push_back(q);
+                }
+            }
+            const int nc = (int)cm.size();
+            hipcheck(c, hipMemcpyAsync(doff, off.data(), sizeof(int) * M, hipMemcpyHostToDevice, c->stream), "H2D");
+            if (nc > 0) {
+                hipcheck(c, hipMemcpyAsync(dcm, cm.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->stream),
+                         "H2D");
+                hipcheck(c, hipMemcpyAsync(dcs, cs.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->stream),
+                         "H2D");
+            }
+            for (int c0 = 0; c0 < nc; c0 += NCsub) {
+                const int ncs = std::min(NCsub, nc - c0);
+                launch(c, NT_K_ELEM, 0.0, "kpc klist",
+                       [&] { return launch_kpc_klist(a, dcm, dcs, c0, ncs, dkl, dkc, c->stream); });
+                launch(c, NT_K_ELEM, 0.0, "kpc dynamical matrices", [&] {
+                    return launch_hk_assemble(7 * ncs, dkl, nseg, seg_off, seg_rc, coef, dr, n_pc, n_pc, dD, n_pc,
+                                              (long)n_pc * n_pc, c->stream);
+                });
+                launch(c, NT_K_EIG, 0.0, "kpc eigenpairs",
+                       [&] { return launch_kpc_eig(a, dcm, m0, c0, ncs, dD, dfound, dvc, c->stream); });
+            }
+            launch(c, NT_K_ELEM, 0.0, "kpc select",
+                   [&] { return launch_kpc_select(a, m0, M, doff, dfound, dvc, dkc, c->stream); });
+            // the host vectors are reused by the next chunk: its copies must have landed
+            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+        }
+        return 0;
+    });
+}
+
 int nt_hk_assemble_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const int* seg_off, const int* seg_rc,
                        int nent, const double* coef, const double* dr, int rows, int cols, nt_c128* out, int ld,
                        long long kstride)

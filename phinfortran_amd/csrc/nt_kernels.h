@@ -122,6 +122,33 @@ struct UnfoldSplitArgs {
 hipError_t launch_unfold_weights(const UnfoldDirArgs& a, hipStream_t s);
 hipError_t launch_unfold_split(const UnfoldSplitArgs& a, hipStream_t s);
 
+// Primitive-cell unfolding (nt_kpc.hip).  Mode m: E[m], k[3m..3m+2]; chunk
+// arrays (count, cand) are indexed from the chunk's first mode; outputs kp, v
+// (3 per mode) and ncand by mode.
+constexpr double KPC_DEVI = 1.0e-5;      // central-difference step (dyn.f90:548)
+constexpr int KPC_MAX_SWEEPS = 30;
+constexpr int KPC_MAX_N = 48;           // orbitals of the primitive cell
+struct KpcArgs {
+    int n;                    // 3 x atoms of the primitive cell
+    const double* E;
+    const double* k;
+    double reci_uc[9], recivec_pc[9];
+    int l, maxc;              // candidates k + n . reci_uc, n in [-l, l]^3; maxc = (2l+1)^3
+    int* count;               // per chunk mode: candidates, -1 evanescent, -2 the mode's own k
+    double* cand;             // per chunk mode: maxc x 3
+    double* kp;
+    double* v;
+    int* ncand;
+};
+hipError_t launch_kpc_candidates(const KpcArgs& a, int m0, int M, hipStream_t s);
+hipError_t launch_kpc_klist(const KpcArgs& a, const int* cm, const int* cs, int c0, int nc, double* kl, double* kc,
+                            hipStream_t s);
+size_t kpc_eig_smem(int n);
+hipError_t launch_kpc_eig(const KpcArgs& a, const int* cm, int m0, int c0, int nc, const double* D, int* found,
+                          double* vout, hipStream_t s);
+hipError_t launch_kpc_select(const KpcArgs& a, int m0, int M, const int* off, const int* found, const double* vc,
+                             const double* kc, hipStream_t s);
+
 // H(k) phase sums of the QE force-constant path (nt_hk.hip): out[k] (rows x cols,
 // leading dimension ld, k stride kstride complex elements) = sum over the
 // entries of each 3x3 segment of coef * exp(i dr . k); zero where no segment.

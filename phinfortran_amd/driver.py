@@ -23,10 +23,13 @@ from . import qe as Q
 
 
 def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None, engine=None,
-               mass_profile: Optional[str] = None, precision: str = "fp64", modes: bool = False):
+               mass_profile: Optional[str] = None, precision: str = "fp64", modes: bool = False,
+               pc_fdf: Optional[str] = None):
     """Returns dict(E (ne), k (nk x 3), transl, transr (ne x nk), status);
     with modes=True also the eight split tables (ne x nk each, names of
-    unfold.SPLIT_NAMES) and the mode-resolved status."""
+    unfold.SPLIT_NAMES) and the mode-resolved status.  pc_fdf (the
+    primitive-cell file, left_primitive_cell_input) adds the primitive-cell k
+    and group velocity columns of the per-mode files (find_k_pc)."""
     import torch
     from . import transport as T
     cfg = Q.read_config(config_path)
@@ -43,19 +46,22 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
                  precision=precision)
     out = {}
     if modes:
-        from .unfold import SPLIT_NAMES, UnfoldGeometry
+        from .unfold import SPLIT_NAMES, PrimitiveCell, UnfoldGeometry
         p.lambda_eta = cfg["lambda_eta"]
         geom = UnfoldGeometry.from_qe(qe, cfg)
+        pc = PrimitiveCell.from_qe(qe, cfg, pc_fdf) if pc_fdf else None
         az = qe["cell"][2, 2]
         r = eng.solve_modes(d, np.tile(E, nk), p, az, az, kidx=kidx, first_point_mask=mask,
-                            want=(), unfold=geom, ksc=ks)
+                            want=(), unfold=geom, ksc=ks, pc=pc)
         split = r["split"].cpu().numpy().reshape(nk, ne, 8)
         for i, name in enumerate(SPLIT_NAMES):
             out[name] = split[:, :, i].T
         out["modes_status"] = r["modes_status"].cpu().numpy().reshape(nk, ne).T
         if output_dir:
-            host = {k: r[k].cpu().numpy() for k in ("k_xyz_L_adv", "k_xyz_R_adv", "spec_trl", "spec_tlr",
-                                                    "spec_rrr", "spec_rll")}
+            names = ["k_xyz_L_adv", "k_xyz_R_adv", "spec_trl", "spec_tlr", "spec_rrr", "spec_rll"]
+            if pc is not None:
+                names += ["kp_L_adv", "v_L_adv", "kp_R_adv", "v_R_adv"]
+            host = {k: r[k].cpu().numpy() for k in names}
             for ik in range(nk):
                 for ie in range(ne):
                     q = ik * ne + ie

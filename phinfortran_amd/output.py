@@ -117,17 +117,21 @@ def write_mode_tables(output_dir: str, nn: int, omega: float, point: dict):
     (negf/velocity_mat.f90:3101-3130): {tlr,trl,right,left,mdl,mdr}_specular_param_NNNNNN.dat,
     NNNNNN the 1-based k index nn.  point: host arrays of one point from
     Engine.solve_modes(..., unfold=...): k_xyz_L_adv / k_xyz_R_adv (n x 3)
-    and spec_trl, spec_tlr, spec_rrr, spec_rll (n x 2: total, specular)."""
+    and spec_trl, spec_tlr, spec_rrr, spec_rll (n x 2: total, specular);
+    optional kp_L_adv, v_L_adv, kp_R_adv, v_R_adv (n x 3, solve_modes with
+    pc=...) fill the primitive-cell columns, zero otherwise."""
     os.makedirs(output_dir, exist_ok=True)
     kl, kr = point["k_xyz_L_adv"], point["k_xyz_R_adv"]
+    pl = dict(kp_b=point.get("kp_L_adv"), v_b=point.get("v_L_adv"))
+    pr = dict(kp_b=point.get("kp_R_adv"), v_b=point.get("v_R_adv"))
     s = {k: np.asarray(point["spec_" + k]) for k in ("trl", "tlr", "rrr", "rll")}
     tables = {
-        "tlr": specular_rows(omega, kr, s["tlr"][:, 0], s["tlr"][:, 1]),
-        "trl": specular_rows(omega, kl, s["trl"][:, 0], s["trl"][:, 1]),
-        "right": specular_rows(omega, kr, s["rrr"][:, 0], s["rrr"][:, 1]),
-        "left": specular_rows(omega, kl, s["rll"][:, 0], s["rll"][:, 1]),
-        "mdl": md_rows(omega, kl, s["rll"][:, 0], s["rll"][:, 1], s["trl"][:, 0], s["trl"][:, 1]),
-        "mdr": md_rows(omega, kr, s["rrr"][:, 0], s["rrr"][:, 1], s["tlr"][:, 0], s["tlr"][:, 1]),
+        "tlr": specular_rows(omega, kr, s["tlr"][:, 0], s["tlr"][:, 1], **pr),
+        "trl": specular_rows(omega, kl, s["trl"][:, 0], s["trl"][:, 1], **pl),
+        "right": specular_rows(omega, kr, s["rrr"][:, 0], s["rrr"][:, 1], **pr),
+        "left": specular_rows(omega, kl, s["rll"][:, 0], s["rll"][:, 1], **pl),
+        "mdl": md_rows(omega, kl, s["rll"][:, 0], s["rll"][:, 1], s["trl"][:, 0], s["trl"][:, 1], **pl),
+        "mdr": md_rows(omega, kr, s["rrr"][:, 0], s["rrr"][:, 1], s["tlr"][:, 0], s["tlr"][:, 1], **pr),
     }
     for name, rows in tables.items():
         with open(os.path.join(output_dir, "%s_specular_param_%06d.dat" % (name, nn)), "a") as f:

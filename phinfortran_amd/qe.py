@@ -30,6 +30,12 @@ MASS_PROTON = 1.6726219e-27
 
 
 
+# mass of each species of the structure files (ChemicalSpeciesLabel of the
+# reference's trans_input.fdf: 1 Ge, 2 Si) through element_mass
+# (negf/input.f90:410-440; REAL(4) literals, rounded as the compiler does)
+SPECIES_MASS = {1: F32(72.64), 2: F32(28.0855)}
+
+
 def read_config(path):
     """The namelist values the path uses (negf/config.f90)."""
     vals = {}
@@ -119,9 +125,7 @@ def read_qe_fc(cfg, path, mass_profile=None):
                     idx_all.append(i)
     mass_profile = mass_profile or os.path.join(os.path.dirname(path), "mass_profile.dat")
     mass_id = [int(x) for x in open(mass_profile).read().split()[1:]]
-    # species 2 -> atomic number 14 -> element_mass = 28.0855 (a REAL(4) literal, input.f90:419-420)
-    masses = {1: F32(72.64), 2: F32(28.0855)}
-    mass_all = np.array([masses[m] for m in mass_id[:len(pos_all)]])
+    mass_all = np.array([SPECIES_MASS[m] for m in mass_id[:len(pos_all)]])
     return dict(cell=cell, pos=pos, nat=nat, nx=nx, ny=ny, nz=nz, fc=fc, rws=rws, rd=rd, cell_all=cell_all,
                 pos_all=np.array(pos_all), idx_all=np.array(idx_all), mass_all=mass_all)
 
@@ -273,6 +277,75 @@ class TdbAssembler:
         b["Hamdv"] = (np.stack([self._block(f"Hamdv{i}", kcart) for i in range(self.nlayer - 1)])
                       if self.nlayer > 1 else np.zeros((0, self.nl, self.nl), np.complex128))
         return b
+
+
+# ------------------------------------------------ primitive cell (find_k_pc)
+def read_pc_fdf(path, species_mass=None):
+    """read_pc_qe (negf/input.f90:580-747) for one lead: the lattice vectors
+    (scaled by LatticeConstant, rotated by pi/4 about z), the atoms
+    (Cartesian), their masses, the number of atoms inside the cell
+    (fractional coordinates in (-0.001, 0.999)) and 2 pi inv(latvec)^T."""
+    species_mass = species_mass or SPECIES_MASS
+    txt = open(path).read().splitlines()
+    nat, scale, lat, atoms = None, 1.0, None, []
+    for i, line in enumerate(txt):
+        words = line.split()
+        if not words:
+            continue
+        if words[0] == "NumberOfAtoms":
+            nat = int(words[1])
+        elif words[0] == "LatticeConstant":
+            scale = float(words[1])
+        elif line.strip().startswith("%block LatticeVectors"):
+            lat = np.array([[float(x) for x in txt[i + r].split()[:3]] for r in (1, 2, 3)])
+        elif line.strip().startswith("%block AtomicScaledCoordinatesAndAtomicSpecies"):
+            atoms = [txt[i + 1 + r].split() for r in range(nat)]
+    c, s = np.cos(PI / 4.0), np.sin(PI / 4.0)
+    rot = lambda v: np.stack([v[:, 0] * c - v[:, 1] * s, v[:, 0] * s + v[:, 1] * c, v[:, 2]], axis=1)
+    latvec = rot(lat * scale)
+    frac = np.array([[float(x) for x in a[:3]] for a in atoms])
+    pos = frac @ latvec
+    mass = np.array([species_mass[int(a[3])] for a in atoms])
+    inv = np.linalg.inv(latvec)
+    cr = pos @ inv
+    npc = int(((cr < 0.999) & (cr > -0.001)).all(axis=1).sum())
+    return dict(latvec=latvec, positions=pos, mass=mass, natoms_pc=npc, recivec=2 * PI * inv.T)
+
+
+def pc_dynamics_table(qe, natoms_pc, mass):
+    """gen_dyn_pc_qe_k (negf/dyn.f90:2537-2644) as a phase-sum table in the
+    format of nt_hk_assemble_dev: one segment per 3x3 block (primitive atom i,
+    QE atom j mod natoms_pc), entries in the reference's summation order (j,
+    then the images a, b, c in [-2 n, 2 n]), coefficient w Phi_ij / sqrt(m_i m_j)
+    with w = 1 / (number of Wigner-Seitz faces the image lies on + 1), zero
+    outside the Wigner-Seitz cell of the force-constant supercell.
+    Returns (seg_off, seg_rc, coef (nent x 9), dr (nent x 3))."""
+    pos, cell, fc = qe["pos"], qe["cell"], qe["fc"]
+    nx, ny, nz = qe["nx"], qe["ny"], qe["nz"]
+    grid = np.stack(np.meshgrid(np.arange(-2 * nx, 2 * nx + 1), np.arange(-2 * ny, 2 * ny + 1),
+                                np.arange(-2 * nz, 2 * nz + 1), indexing="ij"), axis=-1).reshape(-1, 3)
+    shift = grid @ cell
+    blocks = {}
+    for i in range(natoms_pc):
+        for j in range(len(pos)):
+            dr = pos[j] - pos[i] - shift
+            df = dr @ qe["rws"].T - qe["rd"]
+            inside = (df <= 1.0e-5).all(axis=1)
+            faces = (np.abs(df) <= 1.0e-5).sum(axis=1)
+            # fc index of the image: -nint of its lattice offset, periodic in the supercell
+            t = (grid % np.array([nx, ny, nz]))
+            for q in np.nonzero(inside)[0]:
+                f = fc[:, :, i, j, t[q, 0], t[q, 1], t[q, 2]] / (faces[q] + 1.0) / np.sqrt(mass[i] * mass[j])
+                blocks.setdefault((i, j % natoms_pc), []).append((f.reshape(9), dr[q]))
+    seg_off, seg_rc, coef, drs = [0], [], [], []
+    for (i, jm) in sorted(blocks):
+        for f, d in blocks[(i, jm)]:
+            coef.append(f)
+            drs.append(d)
+        seg_rc += [i, jm]
+        seg_off.append(len(coef))
+    return (np.array(seg_off, np.int32), np.array(seg_rc, np.int32), np.array(coef, float).reshape(-1, 9),
+            np.array(drs, float).reshape(-1, 3))
 
 
 def k_grid(cfg):

@@ -16,6 +16,10 @@ it in the reference:
     UnfoldGeometry.from_qe   the QE force-constant path's structure
                        (negf/phinfortran.f90:120-160: positions = the device
                        supercell, latvec_uc = the QE cell)
+    PrimitiveCell      what the primitive-cell step (find_k_pc,
+                       negf/dyn.f90:770-889; nt_unfold_pc_dev) needs: the
+                       primitive cell's dynamical-matrix table and the
+                       reciprocal lattices
 """
 from __future__ import annotations
 
@@ -111,3 +115,31 @@ class UnfoldGeometry:
         for name, v in keep.items():
             setattr(d, name, v.data_ptr())
         return d, keep
+
+
+class PrimitiveCell:
+    """What find_k_pc needs (negf/dyn.f90:770-889): the primitive cell's
+    dynamical-matrix table (phinfortran_amd.qe.pc_dynamics_table), its
+    reciprocal vectors, the lead unit cell's, and l = max(n_bloch_x, n_bloch_y)."""
+
+    def __init__(self, table, n_pc, recivec_pc, reci_uc, n_bloch_max):
+        self.table = table
+        self.n_pc = int(n_pc)
+        self.recivec_pc = np.asarray(recivec_pc, float)
+        self.reci_uc = np.asarray(reci_uc, float)
+        self.l = int(n_bloch_max)
+        self._dev = {}
+
+    @classmethod
+    def from_qe(cls, qe, cfg, pc_fdf_path, species_mass=None):
+        from . import qe as Q
+        pc = Q.read_pc_fdf(pc_fdf_path, species_mass)
+        table = Q.pc_dynamics_table(qe, pc["natoms_pc"], pc["mass"])
+        reci_uc = 2 * np.pi * np.linalg.inv(qe["cell"]).T
+        return cls(table, 3 * pc["natoms_pc"], pc["recivec"], reci_uc, max(cfg["n_bloch_x"], cfg["n_bloch_y"]))
+
+    def device_table(self, torch, dev):
+        key = str(dev)
+        if key not in self._dev:
+            self._dev[key] = tuple(torch.as_tensor(np.ascontiguousarray(x), device=dev) for x in self.table)
+        return self._dev[key]

@@ -162,7 +162,7 @@ def test_batched_mass_defect_closed_form(engine, T):
     assert np.abs(r["T_LR"].cpu().numpy() - ref).max() < 1e-6
 
 
-@pytest.mark.parametrize("nl,b,L,nk", [(4, 4, 3, 1), (24, 24, 5, 3), (40, 40, 2, 2)])
+@pytest.mark.parametrize("nl,b,L,nk", [(4, 4, 3, 1), (24, 24, 5, 3), (40, 40, 2, 2), (320, 320, 2, 1)])
 def test_batched_random_multi_k(engine, T, nl, b, L, nk):
     blocks = [lt.random_blocks(nl, b, L, seed=10 + i) for i in range(nk)]
     p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)

@@ -57,7 +57,7 @@ def test_driver_mode_resolved_run(engine, tmp_path):
     from phinfortran_amd import driver
     print("driver mode-resolved", flush=True)
     out = driver.run_qe_tdb(os.path.join(S.FIXTURE, "config"), os.path.join(S.FIXTURE, "Si444.fc"),
-                            str(tmp_path), engine=engine, modes=True)
+                            str(tmp_path), engine=engine, modes=True, pc_fdf=os.path.join(S.FIXTURE, "pc1.fdf"))
     assert (out["status"] == 0).all() and (out["modes_status"] == 0).all()
     Eg, split = S.golden_split()
     for base in ("transl", "transr", "refll", "reflr"):
@@ -77,5 +77,11 @@ def test_driver_mode_resolved_run(engine, tmp_path):
         for r in rows:
             counts[round(r[1], 8)] = counts.get(round(r[1], 8), 0) + 1
         assert counts == {w: len(v) for w, v in gold.items()}, ik
+        # the primitive-cell columns are filled (find_k_pc): rows with a
+        # primitive-cell k, as many as in the committed table
+        g = np.concatenate(list(gold.values()))
+        n_gold = int((np.abs(g[:, 4:7]).sum(axis=1) > 0).sum())
+        n_mine = int((np.abs(rows[:, 5:8]).sum(axis=1) > 0).sum())
+        assert abs(n_mine - n_gold) <= max(2, 0.05 * n_gold), (ik, n_mine, n_gold)
         for name in ("tlr", "right", "left", "mdl", "mdr"):
             assert os.path.getsize(os.path.join(tmp_path, "%s_specular_param_%06d.dat" % (name, ik + 1))) > 0

@@ -351,7 +351,7 @@ typedef struct {
  *              columns are spec_rll + spec_trl ('mdl') and spec_rrr + spec_tlr
  *              ('mdr').
  * The primitive-cell columns of those tables (find_k_pc, negf/dyn.f90:770-889)
- * are not produced. */
+ * come from nt_unfold_pc_dev on k_xyz_*. */
 typedef struct {
     int* bz_L_adv; int* bz_L_ret; int* bz_R_ret; int* bz_R_adv;
     double* k_xyz_L_adv; double* k_xyz_L_ret; double* k_xyz_R_ret; double* k_xyz_R_adv;

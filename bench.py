@@ -72,6 +72,11 @@ def main():
     ap.add_argument("--c3-devices", type=int, default=4,
                     help="uneven-batching sample: Si/Ge superlattice devices (one per block size 24/96/216/384 "
                          "in turn), 64 frequencies x 2x2 k each, solved in one multi-device call (0 = skip)")
+    ap.add_argument("--no-modes", dest="modes", action="store_false",
+                    help="skip the mode-resolved sample (the si-si example's full mode-resolved run: solve, "
+                         "modes, Brillouin-zone and primitive-cell unfolding of 49 k x 61 omega)")
+    ap.add_argument("--modes-cpu-sample", type=int, default=6,
+                    help="points of the mode-resolved CPU baseline sample (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the (omega,k) points shard across GPUs with no "
@@ -197,6 +202,68 @@ def main():
                 "mixed_max_rel_dT_per_device": errs, "status_ok": res["fp64"][2] and res["mixed"][2]}
 
     c3 = run_c3() if args.c3_devices > 0 else None
+
+    def run_modes():
+        """the si-si example's mode-resolved run (its committed inputs) on the GPU;
+        the CPU restatement of the same chain on a few points beside it"""
+        from phinfortran_amd import driver
+        fix = os.path.join(ROOT, "tests", "golden", "si_si")
+        cfgp, fcp, pcp = (os.path.join(fix, f) for f in ("config", "Si444.fc", "pc1.fdf"))
+        driver.run_qe_tdb(cfgp, fcp, engine=eng, modes=True, pc_fdf=pcp)            # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        o = driver.run_qe_tdb(cfgp, fcp, engine=eng, modes=True, pc_fdf=pcp)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        npts = o["transl"].size
+        ok = bool((o["status"] == 0).all() and (o["modes_status"] == 0).all())
+        log(f"mode-resolved si-si: {npts} points in {el:.2f}s")
+        res = {"points": npts, "points_per_s": npts / el, "seconds": el, "status_ok": ok,
+               "what": "si-si example (b=48, L=2, 49 k x 61 omega): H(k) assembly, decimation, RGF, four "
+                       "mode sets, t/r, Brillouin-zone unfolding and find_k_pc of every propagating mode, FP64"}
+        if args.modes_cpu_sample > 0:
+            import oracle as O
+            from oracle import kpc as K
+            from oracle import modes as OM
+            from oracle import unfold as OU
+            from phinfortran_amd import qe as Q
+            cfg = Q.read_config(cfgp)
+            qe = Q.read_qe_fc(cfg, fcp)
+            asm = Q.TdbAssembler(qe, cfg)
+            ks, E = Q.k_grid(cfg), Q.e_grid(cfg)
+            pos = qe["pos_all"]
+            N, per, buf = len(pos), cfg["period"], cfg["buffer"]
+            nbx, nby = cfg["n_bloch_x"], cfg["n_bloch_y"]
+            setl = OU.transverse_uc_setup(pos, qe["cell"], buf + per, per, nbx, nby, 1e-3)
+            setr = OU.transverse_uc_setup(pos, qe["cell"], N - buf - 2 * per, per, nbx, nby, 1e-5)
+            recivec = 2 * np.pi * np.linalg.inv(qe["cell_all"]).T
+            reci_uc = 2 * np.pi * np.linalg.inv(qe["cell"]).T
+            pcf = K.read_pc_fdf(pcp, Q.SPECIES_MASS)
+            dyn = K.PcDynamics(qe, pcf["natoms_pc"], pcf["mass_uc"])
+            op = O.OrParams(cfg["eta"], cfg["eta0"], cfg["convergence"], 10.0, O.ETA_RELATIVE, 500)
+            az = qe["cell"][2, 2]
+            ik = 24
+            blk = asm.blocks(ks[ik])
+            pts = np.linspace(1, len(E) - 1, args.modes_cpu_sample).astype(int)
+            t0 = time.perf_counter()
+            for ie in pts:
+                m = OM.solve_point_modes(blk, E[ie], op, cfg["lambda_eta"], az, az)
+                U = {k: m["U_" + k] for k in ("L_adv", "L_ret", "R_ret", "R_adv")}
+                lam = {k: m["lam_" + k] for k in ("L_adv", "L_ret", "R_ret", "R_adv")}
+                bz, kx, sp = OU.unfold_point(m, U, lam, ks[ik], recivec, reci_uc, reci_uc, nbx, nby, pos,
+                                             setl, setr)
+                for key in ("L_adv", "L_ret", "R_ret", "R_adv"):
+                    for i in np.where(kx[key][:, 2] < 99.9)[0]:
+                        K.mode_pc(E[ie], kx[key][i], reci_uc, pcf["recivec_pc"], nbx, nby, dyn)
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": len(pts) / dt, "unit": "(omega,k)-points/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"{len(pts)} points of the Gamma k-point, the same chain in numpy "
+                                             f"(one core), {dt:.1f} s"}
+            log(f"mode-resolved cpu sample: {len(pts)} points in {dt:.1f}s")
+        return res
+
+    modes = run_modes() if args.modes else None
     t64 = np.concatenate(T64)
     tmx = np.concatenate(Tmx)
     sig = t64 > 1e-3 * max(1e-300, t64.max())
@@ -269,6 +336,8 @@ def main():
     }
     if c3 is not None:
         out["uneven_batching"] = c3
+    if modes is not None:
+        out["mode_resolved"] = modes
     if eladj is not None:
         out["adjoint"] = {"grads_per_s": solves / eladj, "unit": "(omega,k)-gradients/s",
                           "what": "T_LR and dW/dH of the 3 device layers around the interface (bonds owned by "

@@ -1,0 +1,129 @@
+"""The Buttiker-probe oracle (oracle/buttiker.py) against what the physics and
+the numerics fix: the probe-free limit equals the C oracle's Caroli solve,
+zero probe and contact currents in equilibrium, conservation of the energy
+current over probes and contacts, the Jacobian equal to central differences
+of the probe currents, Simpson's rule exact on cubics, Newton's update."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import buttiker as BT
+from phinfortran_amd import lattice as lt
+
+B_PROBE = 5.0e-20 * (2.0 * 3.1415926 * 1.0e12)     # negf/config.f90:97 default
+C_PROBE = 430.0
+
+
+def leads(blk, E, p):
+    sg = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, E, p, want_plus_minus=False)
+    return dict(gl=sg["gl_adv"].conj().T, gr=sg["gr_ret"], gml=-sg["gml_adv"], gmr=sg["gmr_ret"],
+                sl=sg["sl_adv"].conj().T, sr=sg["sr_ret"])
+
+
+@pytest.fixture(scope="module")
+def sys4():
+    blk = lt.random_blocks(5, 6, 4, seed=7, scale=0.6)
+    p = O.OrParams(1e-9, 1e-10, 1e-13, 10.0, O.ETA_RELATIVE, 4000)
+    return blk, p
+
+
+def test_simpson_exact_on_cubics():
+    x = np.linspace(0.3, 2.1, 11)
+    y = 2 * x ** 3 - x ** 2 + 5 * x - 1
+    exact = lambda t: 0.5 * t ** 4 - t ** 3 / 3 + 2.5 * t ** 2 - t
+    assert abs(BT.simpson(x, y + 0j) - (exact(2.1) - exact(0.3))) < 1e-12
+
+
+@pytest.mark.parametrize("E", [0.15, 0.45, 0.8])
+def test_probe_free_limit_is_the_caroli_solve(sys4, E):
+    blk, p = sys4
+    d = blk.as_dict()
+    g = leads(blk, E, p)
+    f = BT.full_tdb(d, E, p.eta, p.eta0, sig=np.zeros(4), **g)
+    tl, tr, _ = O.solve_tdb(d, E, p)
+    assert abs(f["T_LR"] - tl) < 1e-10 * max(1, tl) and abs(f["T_RL"] - tr) < 1e-10 * max(1, tr)
+    # G_full's diagonal blocks against the dense inverse of the device with lead self-energies
+    Hd, Hdv = d["Hamd"], d["Hamdv"]
+    L, b = len(Hd), Hd[0].shape[0]
+    z = E + 10.0 * (E * p.eta + p.eta0) * 1j
+    M = np.zeros((L * b, L * b), complex)
+    for i in range(L):
+        M[i * b:(i + 1) * b, i * b:(i + 1) * b] = z * np.eye(b) - Hd[i]
+        if i + 1 < L:
+            M[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = -Hdv[i]
+            M[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = -Hdv[i].conj().T
+    Hld, Hrd = d["Hamld"], d["Hamrd"]
+    M[:b, :b] -= Hld.conj().T @ g["gl"] @ Hld
+    M[-b:, -b:] -= Hrd.conj().T @ g["gr"] @ Hrd
+    G = np.linalg.inv(M)
+    for i in range(L):
+        blkG = G[i * b:(i + 1) * b, i * b:(i + 1) * b]
+        assert np.abs(f["G_full"][i] - blkG).max() < 1e-9 * np.abs(blkG).max()
+
+
+def run_point(blk, p, E, T_layers, T_left, T_right, B=B_PROBE):
+    d = blk.as_dict()
+    g = leads(blk, E, p)
+    sig, dsig = BT.sigma_bp(np.array([E]), T_layers, B, C_PROBE)
+    f = BT.full_tdb(d, E, p.eta, p.eta0, sig=sig[:, 0], **g)
+    out = BT.lesser_tdb(d, E, f, g["gl"], g["gr"], sig[:, 0], dsig[:, 0], T_layers, T_left, T_right)
+    return d, g, f, sig[:, 0], dsig[:, 0], out
+
+
+@pytest.mark.parametrize("E", [0.2, 0.6])
+def test_equilibrium_has_no_current(sys4, E):
+    blk, p = sys4
+    T = np.full(4, 300.0)
+    *_, o = run_point(blk, p, E, T, 300.0, 300.0, B=B_PROBE * 1e4)
+    scale = max(1e-30, np.abs(o["A_sum"]).max() * BT.bose(np.sqrt(E), 300.0))
+    assert np.abs(o["flux"]).max() < 1e-6 * scale
+    assert abs(o["flux_left"]) < 1e-6 * scale * np.sqrt(E) and abs(o["flux_right"]) < 1e-6 * scale * np.sqrt(E)
+
+
+@pytest.mark.parametrize("E", [0.2, 0.6])
+def test_energy_current_is_conserved(sys4, E):
+    blk, p = sys4
+    T = np.array([310.0, 290.0, 305.0, 280.0])
+    *_, o = run_point(blk, p, E, T, 330.0, 270.0, B=B_PROBE * 1e4)
+    w = np.sqrt(E)
+    total = o["flux_left"] / w + o["flux_right"] / w + o["flux"].sum()
+    scale = abs(o["flux_left"] / w) + abs(o["flux_right"] / w) + np.abs(o["flux"]).sum()
+    assert scale > 0 and abs(total) < 1e-6 * scale
+    assert abs(o["flux_left"]) > 0 and np.abs(o["flux"]).max() > 0
+
+
+def test_jacobian_is_the_derivative_of_the_probe_currents(sys4):
+    """at fixed retarded functions, d flux_i / d T_j (central differences),
+    except that the reference's diagonal leaves out the term
+    -Tr(d gamma_i/dT_i Gn_i) of d flux_i / d T_i (negf/lesser.f90:1478-1489
+    differentiates gamma_i only in its A_i term); the oracle keeps it out."""
+    blk, p = sys4
+    E = 0.5
+    T = np.array([310.0, 290.0, 305.0, 280.0])
+    d, g, f, sig, dsig, o = run_point(blk, p, E, T, 330.0, 270.0, B=B_PROBE * 1e4)
+    h = 1e-3
+    for j in range(4):
+        fl = []
+        for s in (-1, 1):
+            Tj = T.copy()
+            Tj[j] += s * h
+            sj, dsj = BT.sigma_bp(np.array([E]), Tj, B_PROBE * 1e4, C_PROBE)
+            fl.append(BT.lesser_tdb(d, E, f, g["gl"], g["gr"], sj[:, 0], dsj[:, 0], Tj, 330.0, 270.0)["flux"])
+        fd = (fl[1] - fl[0]) / (2 * h)
+        dgam = 1j * (dsig[j] - np.conj(dsig[j]))
+        fd[j] += dgam * o["Gn_sum"][j]               # the term the reference's diagonal omits
+        assert np.abs(fd - o["jac"][:, j]).max() < 1e-6 * np.abs(o["jac"]).max(), j
+
+
+def test_newton_update_and_phonon_temperature():
+    jac = np.array([[2.0, 0.5], [0.25, 1.0]]) + 0j
+    flux = np.array([1.0, -2.0]) + 0j
+    Tn, dt = BT.update_temperature([300.0, 310.0], jac, flux)
+    assert np.allclose(Tn, np.array([300.0, 310.0]) - np.linalg.solve(jac.real, flux.real))
+    assert abs(dt - np.sqrt(((Tn - [300, 310]) ** 2).mean())) < 1e-12
+    # phonon temperature of a distribution that is exactly Bose at 250 K
+    E = np.linspace(0.01, 4.0, 41) ** 2
+    A = np.ones((41, 1)) + 0j
+    Gn = np.array([[BT.bose(np.sqrt(e), 250.0)] for e in E]) + 0j
+    T = BT.get_ph_temperature(A, Gn, E, [300.0])
+    assert abs(T[0] - 250.0) < 1e-3

@@ -62,6 +62,23 @@ module negf_transport
         type(c_ptr) :: spec_trl = c_null_ptr, spec_tlr = c_null_ptr, spec_rrr = c_null_ptr, spec_rll = c_null_ptr
     end type nt_unfold_out
 
+    ! Buttiker probes (nt_buttiker_tdb_batched_dev): temperature points to the
+    ! L probe temperatures in device memory
+    type, bind(C) :: nt_probe_desc
+        type(c_ptr) :: temperature = c_null_ptr
+        real(c_double) :: t_left = 300.0d0, t_right = 300.0d0
+        real(c_double) :: B_probe = 5.0d-20*(2.0d0*3.1415926d0*1.0d12), T_probe = 430.0d0
+        integer(c_int) :: lifetime = 0
+    end type nt_probe_desc
+
+    ! per point p: flux(L), jacobian(L,L) (jacobian(i,j) = d flux_i / d T_j),
+    ! flux_left, flux_right, a_sum(L), gn_sum(L); device memory
+    type, bind(C) :: nt_probe_out
+        type(c_ptr) :: flux = c_null_ptr, jacobian = c_null_ptr
+        type(c_ptr) :: flux_left = c_null_ptr, flux_right = c_null_ptr
+        type(c_ptr) :: a_sum = c_null_ptr, gn_sum = c_null_ptr
+    end type nt_probe_out
+
     interface
         subroutine nt_params_default(p) bind(C, name="nt_params_default")
             import :: nt_params
@@ -190,6 +207,19 @@ module negf_transport
             type(nt_unfold_desc), intent(in) :: uf
             type(nt_unfold_out), intent(in) :: uo
             type(c_ptr), value :: status
+        end function
+
+        integer(c_int) function nt_buttiker_tdb_batched_dev(ctx, p, probe, nl, nr, b, L, npts, E, kidx, nk, &
+                Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd, first_point_mask, &
+                T_LR, T_RL, status, iters, out) bind(C, name="nt_buttiker_tdb_batched_dev")
+            import :: c_int, c_ptr, nt_params, nt_probe_desc, nt_probe_out
+            type(c_ptr), value :: ctx
+            type(nt_params), intent(in) :: p
+            type(nt_probe_desc), intent(in) :: probe
+            integer(c_int), value :: nl, nr, b, L, npts, nk
+            type(c_ptr), value :: E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd
+            type(c_ptr), value :: first_point_mask, T_LR, T_RL, status, iters
+            type(nt_probe_out), intent(in) :: out
         end function
 
         integer(c_int) function nt_solve_tdb_batched(ctx, p, nl, nr, b, L, npts, E, kidx, nk, &

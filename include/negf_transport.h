@@ -265,6 +265,53 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, in
                                      double* T_LR, double* T_RL, int* status, int* iters,
                                      nt_c128* dW_dHamd, nt_c128* dW_dHamdv);
 
+/* ------------------------------------------------ Buttiker probes
+ * One probe per device layer with self-energy sigma_w = -2 i sqrt(E) / tau,
+ * 1/tau = B E T_i exp(-C / T_i) (set_sigma_bp_w, negf/lesser.f90:12-45;
+ * zeroed when lifetime == 0, its T derivative is not), in every device layer
+ * of both sweeps (get_G_full_tdb, negf/lesser.f90:475-809).  Replaces, for
+ * npts points, the pair get_G_full_tdb_mpi + get_G_lesser_tdb_mpi of the
+ * driver's Buttiker loop (negf/phinfortran.f90:1213-1237): T_LR, T_RL and
+ * status as nt_solve_tdb_batched_dev, and per point p (row-major by point):
+ *   flux[p L + i]            flux_i(i)       probe current of layer i
+ *   jacobian[p L^2 + i + j L] jacobian(nn,ie,i,j) = d flux_i / d T_j at fixed
+ *                            retarded functions; the diagonal differentiates
+ *                            gamma_i only in its A_i term, as the reference
+ *   flux_left[p], flux_right[p]  contact currents (times frequency sqrt(E))
+ *   a_sum[p L + i], gn_sum[p L + i]  Tr A_i, Tr Gn_i
+ * (negf/lesser.f90:1199-1503).  The reference keeps these as complex numbers
+ * whose imaginary parts vanish in exact arithmetic; they are returned real.
+ * Points with a nonzero first_point_mask get zero probe currents, Jacobian
+ * and traces, but their contact currents (negf/lesser.f90:1467-1472).
+ * Runs in FP64 whatever p->precision says; needs L >= 2 and nl == nr.
+ * temperature: L probe temperatures in device memory. */
+typedef struct {
+    const double* temperature;   /* temperature_i (L) */
+    double t_left, t_right;      /* contact temperatures t_left, t_right */
+    double B_probe, T_probe;     /* B and C of the lifetime (config B_probe, T_probe) */
+    int lifetime;                /* config lifetime: 0 zeroes sigma_w */
+} nt_probe_desc;
+
+typedef struct {
+    double* flux;        /* npts x L */
+    double* jacobian;    /* npts x L x L (column-major per point) */
+    double* flux_left;   /* npts */
+    double* flux_right;  /* npts */
+    double* a_sum;       /* npts x L */
+    double* gn_sum;      /* npts x L */
+} nt_probe_out;
+
+int nt_buttiker_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, const nt_probe_desc* probe,
+                                int nl, int nr, int b, int L, int npts, const double* E,
+                                const int* kidx, int nk,
+                                const nt_c128* Haml, const nt_c128* Hamlv,
+                                const nt_c128* Hamr, const nt_c128* Hamrv,
+                                const nt_c128* Hamd, const nt_c128* Hamdv,
+                                const nt_c128* Hamld, const nt_c128* Hamrd,
+                                const int* first_point_mask,
+                                double* T_LR, double* T_RL, int* status, int* iters,
+                                const nt_probe_out* out);
+
 /* ------------------------------------------------ mode-resolved transmission
  * vel_mat_tdb (negf/velocity_mat.f90:1692-3150; tb/velocity_mat.f90:222-1560)
  * for npts points (the Brillouin-zone unfolding that splits specular and

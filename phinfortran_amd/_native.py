@@ -33,7 +33,7 @@ EXPORTS = [
     "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
-    "nt_solve_tdb_adjoint_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
+    "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
     "nt_set_overlap", "nt_set_gemm_3m",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
@@ -61,6 +61,18 @@ UNFOLD_OUT_FIELDS = ("bz_L_adv", "bz_L_ret", "bz_R_ret", "bz_R_adv", "k_xyz_L_ad
 
 class NtUnfoldOut(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in UNFOLD_OUT_FIELDS]
+
+
+class NtProbeDesc(C.Structure):
+    _fields_ = [("temperature", C.c_void_p), ("t_left", C.c_double), ("t_right", C.c_double),
+                ("B_probe", C.c_double), ("T_probe", C.c_double), ("lifetime", C.c_int)]
+
+
+PROBE_OUT_FIELDS = ("flux", "jacobian", "flux_left", "flux_right", "a_sum", "gn_sum")
+
+
+class NtProbeOut(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in PROBE_OUT_FIELDS]
 
 
 class NtDeviceDesc(C.Structure):
@@ -135,6 +147,8 @@ def lib():
     L.nt_solve_tdb_batched_dev.argtypes = solve_args
     L.nt_solve_tdb_batched.argtypes = solve_args
     L.nt_solve_tdb_adjoint_batched_dev.argtypes = solve_args[:19] + [P, I, I, P, P, P, P, P, P]
+    L.nt_buttiker_tdb_batched_dev.argtypes = solve_args[:2] + [C.POINTER(NtProbeDesc)] + solve_args[2:23] + [
+        C.POINTER(NtProbeOut)]
     L.nt_solve_tdb_multi_dev.argtypes = [P, pp, I, P]
     L.nt_hk_assemble_dev.argtypes = [P, I, P, I, P, P, I, P, P, I, I, P, I, C.c_longlong]
     L.nt_get_ldos.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, I, P, P]

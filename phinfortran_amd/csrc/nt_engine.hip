@@ -752,6 +752,12 @@ struct RgfInput {
     BMat<T> Hld, Hrd, Hrd_fwd, Haml, Hamr;     // Hrd_fwd.base may be null
     // retarded lead quantities (per point)
     BMat<T> gl, gr, gml, gmr, sl, sr;
+    // Buttiker probes: per-layer device shifts z - sigma_w(i) (L x P complex,
+    // layer-major; null: every device layer uses the common shift), and L slots
+    // that receive the left-connected G of layers 0..L-2 and the full G of
+    // layer L-1 (null: not kept)
+    const T* zl = nullptr;
+    T* keep = nullptr;
 };
 
 template <typename T>
@@ -993,6 +999,9 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     const int L = in.L;
     auto W = [&](T* p, int ld) { return bw(p, mm, ld); };
     auto R = [&](T* p, int ld) { return bm<T>(p, mm, ld); };
+    // diagonal shift of device layer i
+    auto zdev = [&](int i) -> const T* { return in.zl ? in.zl + 2 * (long)i * P : zd; };
+    auto keep = [&](int i) -> T* { return in.keep ? in.keep + 2 * (long)i * mm * P : nullptr; };
     BMatW<T> gdlr = out.G_D_LR.base ? out.G_D_LR : W(GLR, nl);
     BMatW<T> gdrl = out.G_D_RL.base ? out.G_D_RL : W(GRL, nr);
     BMatW<T> gLo = out.G_L.base ? out.G_L : W(Gn, nl);
@@ -1021,18 +1030,26 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         const int b = in.bsz[0];
         if (T* sl = slotL(0)) G = sl;
         if (T* s0 = slot0(0)) G0 = s0;
+        if (L > 1 && keep(0)) G = keep(0);
         gemm<T>(c, b, nl, nl, OP_C, in.Hld, OP_N, in.gl, W(T1, b), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, in.Hld, W(Aw, b), -1.0, in.Hd[0], -1.0, zd, P);
+        gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, in.Hld, W(Aw, b), -1.0, in.Hd[0], -1.0, zdev(0), P);
         inverse<T>(c, b, W(Aw, b), none<T>(), nullptr, W(G, b), st, P);
         gemm<T>(c, nl, b, nl, OP_N, in.gl, OP_N, in.Hld, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, nl, b, b, OP_N, R(T2, nl), OP_N, R(G, b), W(G0, nl), 1.0, none<T>(), 0.0, nullptr, P);
         for (int i = 1; i < L; ++i) {
             const int bp = in.bsz[i - 1], bi = in.bsz[i];
             const BMat<T>& V = in.Hdv[i - 1];
-            T* Gi = slotL(i) ? slotL(i) : other(gbufs, G);
+            T* Gi = slotL(i) ? slotL(i) : (keep(i) && i < L - 1 ? keep(i) : other(gbufs, G));
             T* G0i = slot0(i) ? slot0(i) : other(g0bufs, G0);
             gemm<T>(c, bi, bp, bp, OP_C, V, OP_N, R(G, bp), W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P);
-            gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, V, W(Aw, bi), -1.0, in.Hd[i], -1.0, zd, P);
+            gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, V, W(Aw, bi), -1.0, in.Hd[i], -1.0, zdev(i), P);
+            if (i == L - 1 && keep(i)) {
+                // full G of the last layer, closed by the right lead's surface
+                // function: inv(Aw - Hrd^+ gr Hrd)  (negf/lesser.f90:587-600)
+                gemm<T>(c, bi, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P);
+                gemm<T>(c, bi, bi, nr, OP_N, R(T1, bi), OP_N, in.Hrd, W(X, bi), -1.0, R(Aw, bi), 1.0, nullptr, P);
+                inverse<T>(c, bi, W(X, bi), none<T>(), nullptr, W(keep(i), bi), st, P);
+            }
             inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gi, bi), st, P);
             gemm<T>(c, nl, bi, bp, OP_N, R(G0, nl), OP_N, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
             gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gi, bi), W(G0i, nl), 1.0, none<T>(), 0.0, nullptr, P);
@@ -1085,7 +1102,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     {
         const int bN = in.bsz[L - 1];
         gemm<T>(c, bN, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, bN, bN, nr, OP_N, R(T1, bN), OP_N, in.Hrd, W(Aw, bN), -1.0, in.Hd[L - 1], -1.0, zd, P);
+        gemm<T>(c, bN, bN, nr, OP_N, R(T1, bN), OP_N, in.Hrd, W(Aw, bN), -1.0, in.Hd[L - 1], -1.0, zdev(L - 1), P);
         inverse<T>(c, bN, W(Aw, bN), none<T>(), nullptr, W(G, bN), st, P);
         gemm<T>(c, nr, bN, nr, OP_N, in.gr, OP_N, in.Hrd, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, nr, bN, bN, OP_N, R(T2, nr), OP_N, R(G, bN), W(G0, nr), 1.0, none<T>(), 0.0, nullptr, P);
@@ -1097,7 +1114,8 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
             const int opV = back ? OP_N : OP_C;    // gives V   (bi x bm_)
             const int opVH = back ? OP_C : OP_N;   // gives V^+ (bm_ x bi)
             gemm<T>(c, bm_, bi, bi, opVH, Vb, OP_N, R(G, bi), W(T1, bm_), 1.0, none<T>(), 0.0, nullptr, P);
-            gemm<T>(c, bm_, bm_, bi, OP_N, R(T1, bm_), opV, Vb, W(Aw, bm_), -1.0, in.Hd[i - 1], -1.0, zd, P);
+            gemm<T>(c, bm_, bm_, bi, OP_N, R(T1, bm_), opV, Vb, W(Aw, bm_), -1.0, in.Hd[i - 1], -1.0, zdev(i - 1),
+                    P);
             inverse<T>(c, bm_, W(Aw, bm_), none<T>(), nullptr, W(Gn, bm_), st, P);
             gemm<T>(c, nr, bm_, bi, OP_N, R(G0, nr), opV, Vb, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P);
             gemm<T>(c, nr, bm_, bm_, OP_N, R(T2, nr), OP_N, R(Gn, bm_), W(G0n, nr), 1.0, none<T>(), 0.0, nullptr,
@@ -1130,6 +1148,167 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     (void)gLo;
 }
 
+// ------------------------------------------------------- Buttiker probes
+// Probe outputs of a batched solve (negf/lesser.f90:1199-1503).
+struct ProbeSpec {
+    const double* temp;       // L probe temperatures (device)
+    double t_left, t_right, B, C;
+    int lifetime;
+    double *flux, *jac, *flux_left, *flux_right, *a_sum, *gn_sum;
+};
+
+// complex traces kept per point by buttiker_run
+inline long probe_acc_n(int L) { return (long)L * L + 5 * (long)L + 6; }
+// slots (of the RGF slot size, per point) buttiker_run uses after the sweep
+inline long probe_slots(int L) { return 7 * (long)L + 2; }
+
+// From the kept blocks of the left-to-right sweep (left-connected G^L_i of
+// layers 0..L-2, full G_{N,N} of the last layer) build every block of the full
+// retarded function and reduce it to the traces the probe currents need:
+//   G_{i,i}   = G^L_i + P_i G_{i+1,i},  P_i = G^L_i V_i,  G_{i+1,i} = G_{i+1,i+1} Q_i,
+//               Q_i = V_i^+ G^L_i         (negf/lesser.f90:724-749)
+//   G_{i,j}   = P_i G_{i+1,j}   (i < j),   G_{i,j} = G_{i,j+1} Q_j   (i > j),
+// each distance t = |i - j| one grouped GEMM over all pairs and points.
+// acc (per point, complex): |G_ij|_F^2 at i L + j; Tr(G_i0 gf G_i0^+),
+// Tr(G_iN gl G_iN^+), Tr(gf G_0j G_0j^+), Tr(gl G_Nj G_Nj^+), Tr G_ii (L each);
+// Tr(gf G_00), Tr(gl G_NN), Tr(gf G_00 gf G_00^+), Tr(gf G_0N gl G_0N^+),
+// Tr(gl G_N0 gf G_N0^+), Tr(gl G_NN gl G_NN^+), with gf, gl the broadenings
+// of the two contacts at the end layers.
+void buttiker_run(nt_ctx* c, int nl, int nr, const RgfInput<double>& in, int P, long mm, double* work,
+                  double* acc)
+{
+    typedef double T;
+    const int L = in.L, b = in.bsz[0];
+    const long slot = mm * P, bb = (long)b * b;
+    const long as = probe_acc_n(L);
+    long next = 0;
+    auto take = [&](long n) {
+        T* p = work + 2 * next * slot;
+        next += n;
+        return p;
+    };
+    T* keep = in.keep;
+    T* Pm = take(L - 1);
+    T* Qm = take(L - 1);
+    T* U[2] = {take(L - 1), take(L - 1)};
+    T* D[2] = {take(L - 1), take(L - 1)};
+    T* gf = take(1);
+    T* gl = take(1);
+    T* X = take(1);
+    T* Xp = take(1);
+    T* Y = take(1);
+    T* Z = take(1);
+    T* S = take(1);
+    (void)take(1);
+    auto at = [&](T* base, long k) { return base + 2 * k * slot; };
+    auto R = [&](T* p) { return bm<T>(p, mm, b); };
+    auto W = [&](T* p) { return bw(p, mm, b); };
+    auto RG = [&](T* p) { return bm<T>(p, mm, b, nullptr, slot); };      // grouped over slots
+    auto WG = [&](T* p) { return bw(p, mm, b, nullptr, slot); };
+    BMat<T> Vg = in.Hdv[0];
+    Vg.gstride = bb;                                                        // grouped over layers
+    hipStream_t s = c->stream;
+    auto frob = [&](BMat<T> A, BMat<T> B2, long off, long ogs, int groups) {
+        audit<T>(c, A, b, b, P, groups, "frob X");
+        audit<T>(c, B2, b, b, P, groups, "frob Y");
+        check_range(c, acc + 2 * off, sizeof(T) * 2 * ((P - 1) * as + (long)(groups - 1) * ogs + 1), "frob out");
+        launch(c, NT_K_ELEM, 0, "frob",
+               [&] { return launch_frob_dot<T>(b, b, A, B2, acc + 2 * off, as, ogs, P, groups, s); });
+    };
+    auto trace = [&](BMat<T> A, long off, long ogs, int groups) {
+        audit<T>(c, A, b, b, P, groups, "trace X");
+        check_range(c, acc + 2 * off, sizeof(T) * 2 * ((P - 1) * as + (long)(groups - 1) * ogs + 1), "trace out");
+        launch(c, NT_K_ELEM, 0, "trace", [&] { return launch_trace_diag<T>(b, A, acc + 2 * off, as, ogs, P, groups, s); });
+    };
+    const long oN = (long)L * L, ocL = oN, ocR = oN + L, om = oN + 2 * L, orr = oN + 3 * L, otr = oN + 4 * L,
+               omisc = oN + 5 * L;
+    // contact broadenings at the end layers: gf = i(S - S^+), S = Hld^+ gl Hld;
+    // gl likewise from Hrd^+ gr Hrd (negf/lesser.f90:1288-1292, 1385-1390)
+    gemm<T>(c, b, nl, nl, OP_C, in.Hld, OP_N, in.gl, bw(X, mm, b), 1.0, none<T>(), 0.0, nullptr, P);
+    gemm<T>(c, b, b, nl, OP_N, R(X), OP_N, in.Hld, W(S), 1.0, none<T>(), 0.0, nullptr, P);
+    gamma_op<T>(c, b, R(S), W(gf), P);
+    gemm<T>(c, b, nr, nr, OP_C, in.Hrd, OP_N, in.gr, bw(X, mm, b), 1.0, none<T>(), 0.0, nullptr, P);
+    gemm<T>(c, b, b, nr, OP_N, R(X), OP_N, in.Hrd, W(S), 1.0, none<T>(), 0.0, nullptr, P);
+    gamma_op<T>(c, b, R(S), W(gl), P);
+    // P_i = G^L_i V_i and Q_i = V_i^+ G^L_i, all layers at once
+    gemm<T>(c, b, b, b, OP_N, RG(keep), OP_N, Vg, WG(Pm), 1.0, none<T>(), 0.0, nullptr, P, -1, nullptr, L - 1);
+    gemm<T>(c, b, b, b, OP_C, Vg, OP_N, RG(keep), WG(Qm), 1.0, none<T>(), 0.0, nullptr, P, -1, nullptr, L - 1);
+    // backward: G_{i+1,i} (distance-1 lower blocks) and G_{i,i}
+    for (int i = L - 2; i >= 0; --i) {
+        T* Dn = at(D[1], i);
+        gemm<T>(c, b, b, b, OP_N, R(at(keep, i + 1)), OP_N, R(at(Qm, i)), W(Dn), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, b, b, b, OP_N, R(at(Pm, i)), OP_N, R(Dn), W(at(keep, i)), 1.0, R(at(keep, i)), 1.0, nullptr, P);
+    }
+    // distance 0: |G_ii|^2, Tr G_ii and the end-layer contact products
+    frob(RG(keep), RG(keep), 0, L + 1, L);
+    trace(RG(keep), otr, 1, L);
+    const int N = L - 1;
+    auto right_gf = [&](T* G, T* out) {   // out = G gf
+        gemm<T>(c, b, b, b, OP_N, R(G), OP_N, R(gf), W(out), 1.0, none<T>(), 0.0, nullptr, P);
+    };
+    auto right_gl = [&](T* G, T* out) {
+        gemm<T>(c, b, b, b, OP_N, R(G), OP_N, R(gl), W(out), 1.0, none<T>(), 0.0, nullptr, P);
+    };
+    auto left_gf = [&](T* G, T* out) {
+        gemm<T>(c, b, b, b, OP_N, R(gf), OP_N, R(G), W(out), 1.0, none<T>(), 0.0, nullptr, P);
+    };
+    auto left_gl = [&](T* G, T* out) {
+        gemm<T>(c, b, b, b, OP_N, R(gl), OP_N, R(G), W(out), 1.0, none<T>(), 0.0, nullptr, P);
+    };
+    {
+        T* G00 = at(keep, 0);
+        T* GNN = at(keep, N);
+        right_gf(G00, X);
+        left_gf(G00, Y);
+        right_gl(GNN, Xp);
+        left_gl(GNN, Z);
+        frob(R(G00), R(X), ocL, 0, 1);
+        frob(R(G00), R(Y), om, 0, 1);
+        frob(R(GNN), R(Xp), ocR + N, 0, 1);
+        frob(R(GNN), R(Z), orr + N, 0, 1);
+        trace(R(Y), omisc + 0, 0, 1);
+        trace(R(Z), omisc + 1, 0, 1);
+        frob(R(Y), R(X), omisc + 2, 0, 1);
+        frob(R(Z), R(Xp), omisc + 5, 0, 1);
+    }
+    int cur = 1;
+    for (int t = 1; t <= N; ++t) {
+        const int ng = L - t;
+        if (t == 1) {
+            // upper distance 1: G_{i,i+1} = P_i G_{i+1,i+1}
+            gemm<T>(c, b, b, b, OP_N, RG(Pm), OP_N, RG(at(keep, 1)), WG(U[1]), 1.0, none<T>(), 0.0, nullptr, P, -1,
+                    nullptr, ng);
+        } else {
+            const int prev = cur;
+            cur ^= 1;
+            gemm<T>(c, b, b, b, OP_N, RG(Pm), OP_N, RG(at(U[prev], 1)), WG(U[cur]), 1.0, none<T>(), 0.0, nullptr, P,
+                    -1, nullptr, ng);
+            gemm<T>(c, b, b, b, OP_N, RG(at(D[prev], 1)), OP_N, RG(Qm), WG(D[cur]), 1.0, none<T>(), 0.0, nullptr, P,
+                    -1, nullptr, ng);
+        }
+        T* Ut = U[cur];
+        T* Dt = D[cur];
+        frob(RG(Ut), RG(Ut), t, L + 1, ng);
+        frob(RG(Dt), RG(Dt), (long)t * L, L + 1, ng);
+        T* G0t = at(Ut, 0);           // G_{0,t}
+        T* GcN = at(Ut, N - t);       // G_{N-t,N}
+        T* Gt0 = at(Dt, 0);           // G_{t,0}
+        T* GNc = at(Dt, N - t);       // G_{N,N-t}
+        left_gf(G0t, Y);
+        right_gl(GcN, Xp);
+        right_gf(Gt0, X);
+        left_gl(GNc, Z);
+        frob(R(G0t), R(Y), om + t, 0, 1);
+        frob(R(GcN), R(Xp), ocR + N - t, 0, 1);
+        frob(R(Gt0), R(X), ocL + t, 0, 1);
+        frob(R(GNc), R(Z), orr + N - t, 0, 1);
+        if (t == N) {
+            frob(R(Y), R(Xp), omisc + 3, 0, 1);   // Tr(gf G_0N gl G_0N^+)
+            frob(R(Z), R(X), omisc + 4, 0, 1);    // Tr(gl G_N0 gf G_N0^+)
+        }
+    }
+}
+
 // ------------------------------------------------------- tdb batched solve
 struct TdbArgs {
     int nl, nr, b, L, npts, nk;
@@ -1145,6 +1324,7 @@ struct TdbArgs {
     int adj_lo = 0, adj_hi = -1;
     nt_c128* dHd = nullptr;    // (nk, hi - lo + 1, b, b), accumulated
     nt_c128* dHdv = nullptr;   // (nk, min(hi, L-2) - lo + 1, b, b), accumulated
+    const ProbeSpec* probe = nullptr;
 };
 
 // adjoint storage is capped so that a chunk of points fits this budget
@@ -1183,6 +1363,11 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         const double per_pt = (double)adj_slots(a.adj_lo, a.adj_hi, L) * mmr * sizeof(nt_c128);
         Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
     }
+    const ProbeSpec* probe = a.probe;
+    if (probe) {
+        const double per_pt = (double)(probe_slots(L) * mmr + probe_acc_n(L) + 2 * L) * sizeof(nt_c128);
+        Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
+    }
 
     // ---- plan the workspace for one chunk of Pmax points
     Arena ar;
@@ -1206,7 +1391,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const size_t o_gmlr = ar.take(sizeof(nt_c128) * nll * Pmax);       // gml = -gml_adv
     // mixed precision: FP32 copies of the H(k) blocks (once per call) and of
     // the lead quantities and device shifts (per chunk)
-    const bool mixed = prm->precision == NT_PREC_MIXED;
+    const bool mixed = prm->precision == NT_PREC_MIXED && !probe;   // the probe solve runs in FP64
     const size_t c32 = 2 * sizeof(float);
     const long Ldv = std::max(L - 1, 1);
     size_t o_h32[6] = {0, 0, 0, 0, 0, 0};
@@ -1240,7 +1425,20 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         (void)adj_carve(probe, a.adj_lo, a.adj_hi, L, mmr, Pmax, nullptr);
         need_adj = probe.off;
     }
-    ensure_ws(c, o_adj + need_adj);
+    // probe region: per-layer shifts and broadenings, traces, kept blocks and
+    // the work slots of buttiker_run
+    Arena pa;
+    pa.off = o_adj + need_adj;
+    size_t o_pzl = 0, o_pgam = 0, o_pdgam = 0, o_pacc = 0, o_pkeep = 0, o_pwork = 0;
+    if (probe) {
+        o_pzl = pa.take(sizeof(double) * 2 * L * Pmax);
+        o_pgam = pa.take(sizeof(double) * L * Pmax);
+        o_pdgam = pa.take(sizeof(double) * L * Pmax);
+        o_pacc = pa.take(sizeof(double) * 2 * probe_acc_n(L) * Pmax);
+        o_pkeep = pa.take(sizeof(nt_c128) * mmr * L * Pmax);
+        o_pwork = pa.take(sizeof(nt_c128) * mmr * (probe_slots(L) - L) * Pmax);
+    }
+    ensure_ws(c, pa.off);
     ar.base = static_cast<char*>(c->ws);
     hipStream_t s = c->stream;
 
@@ -1313,6 +1511,13 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         hipLaunchKernelGGL(k_lead_setup, dim3(nblk(P)), dim3(256), 0, s, E, kidx, a.nk, P, same ? 1 : 0, prm->eta,
                            prm->eta0, prm->eta_mode, prm->device_eta_factor, idx, zdec, pm ? zpm : nullptr, zdev);
         hipcheck(c, hipGetLastError(), "lead setup");
+        if (probe) {
+            check_range(c, probe->temp, sizeof(double) * L, "probe temperatures");
+            launch(c, NT_K_ELEM, 0, "probe shifts", [&] {
+                return launch_probe_shifts(E, zdev, probe->temp, L, P, probe->B, probe->C, probe->lifetime,
+                                           ar.ptr<T>(o_pzl), ar.ptr<T>(o_pgam), ar.ptr<T>(o_pdgam), s);
+            });
+        }
         hipcheck(c, hipMemsetAsync(stdec, 0, sizeof(int) * 2 * P, s), "memset");
         hipcheck(c, hipMemsetAsync(strgf, 0, sizeof(int) * P, s), "memset");
         hipcheck(c, hipMemsetAsync(stpm, 0, sizeof(int) * 2 * P, s), "memset");
@@ -1367,6 +1572,10 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         in.gr = bm<T>(gbuf + 2 * nrr, 2 * nrr, nr);
         in.sr = bm<T>(sigbuf + 2 * nrr, 2 * nrr, nr);
         in.gmr = bm<T>(gambuf + 2 * nrr, 2 * nrr, nr);
+        if (probe) {
+            in.zl = ar.ptr<T>(o_pzl);
+            in.keep = ar.ptr<T>(o_pkeep);
+        }
         RgfOutput<T> out;
         out.T_LR = a.T_LR + p0;
         out.T_RL = a.T_RL + p0;
@@ -1386,6 +1595,35 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                 adj = adj_carve(aa, a.adj_lo, a.adj_hi, L, mmr, P, a.adj_w + p0);
             }
             rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr, adjoint ? &adj : nullptr);
+            if (probe) {
+                buttiker_run(c, nl, nr, in, P, mmr, ar.ptr<T>(o_pwork), ar.ptr<T>(o_pacc));
+                ProbeCombineArgs pc;
+                pc.L = L;
+                pc.P = P;
+                pc.p0 = p0;
+                pc.E = E;
+                pc.temp = probe->temp;
+                pc.t_left = probe->t_left;
+                pc.t_right = probe->t_right;
+                pc.gam = ar.ptr<T>(o_pgam);
+                pc.dgam = ar.ptr<T>(o_pdgam);
+                pc.acc = ar.ptr<T>(o_pacc);
+                pc.acc_stride = probe_acc_n(L);
+                pc.first_mask = a.first_mask ? a.first_mask + p0 : nullptr;
+                pc.flux = probe->flux;
+                pc.jac = probe->jac;
+                pc.flux_left = probe->flux_left;
+                pc.flux_right = probe->flux_right;
+                pc.a_sum = probe->a_sum;
+                pc.gn_sum = probe->gn_sum;
+                check_range(c, probe->flux + (long)p0 * L, sizeof(double) * L * P, "probe flux");
+                check_range(c, probe->jac + (long)p0 * L * L, sizeof(double) * L * L * P, "probe jacobian");
+                check_range(c, probe->a_sum + (long)p0 * L, sizeof(double) * L * P, "probe A");
+                check_range(c, probe->gn_sum + (long)p0 * L, sizeof(double) * L * P, "probe Gn");
+                check_range(c, probe->flux_left + p0, sizeof(double) * P, "flux_left");
+                check_range(c, probe->flux_right + p0, sizeof(double) * P, "flux_right");
+                launch(c, NT_K_ELEM, 0, "probe combine", [&] { return launch_probe_combine(pc, s); });
+            }
             if (adjoint) {
                 const long ne = bb;
                 const int nw = a.adj_hi - a.adj_lo + 1, nc = adj_hc(a.adj_hi, L) - a.adj_lo + 1;
@@ -2138,11 +2376,11 @@ int nt_profile_get(nt_ctx* c, int kind, long long* launches, double* total_ms, d
 
 double nt_last_mean_iters(nt_ctx* c) { return c ? c->last_mean_iters : 0.0; }
 
-int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,
-                             const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
-                             const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
-                             const nt_c128* Hamrd, const int* first_point_mask, double* T_LR, double* T_RL,
-                             int* status, int* iters, const nt_solve_extras* extras)
+static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,
+                     const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
+                     const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
+                     const nt_c128* Hamrd, const int* first_point_mask, double* T_LR, double* T_RL, int* status,
+                     int* iters, const nt_solve_extras* extras, const nt_probe_desc* pd, const nt_probe_out* po)
 {
     if (!c) return NT_ERR_ARG;
     if (check_params(p) || check_circular(p, nl, nr)) return NT_ERR_ARG;
@@ -2198,9 +2436,48 @@ int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int 
         }
         TdbArgs a{nl, nr, b, L, npts, nk, E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
                   first_point_mask, T_LR, T_RL, status, iters, ex};
+        ProbeSpec ps{};
+        if (pd) {
+            reg_region(c, pd->temperature, sizeof(double) * L);
+            reg_region(c, po->flux, sizeof(double) * npts * L);
+            reg_region(c, po->jacobian, sizeof(double) * npts * L * L);
+            reg_region(c, po->flux_left, sizeof(double) * npts);
+            reg_region(c, po->flux_right, sizeof(double) * npts);
+            reg_region(c, po->a_sum, sizeof(double) * npts * L);
+            reg_region(c, po->gn_sum, sizeof(double) * npts * L);
+            ps = ProbeSpec{pd->temperature, pd->t_left,  pd->t_right,    pd->B_probe,     pd->T_probe, pd->lifetime,
+                           po->flux,        po->jacobian, po->flux_left, po->flux_right, po->a_sum,   po->gn_sum};
+            a.probe = &ps;
+        }
         solve_tdb_dev(c, p, a);
         return 0;
     });
+}
+
+int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,
+                             const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
+                             const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
+                             const nt_c128* Hamrd, const int* first_point_mask, double* T_LR, double* T_RL,
+                             int* status, int* iters, const nt_solve_extras* extras)
+{
+    return tdb_entry(c, p, nl, nr, b, L, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
+                     first_point_mask, T_LR, T_RL, status, iters, extras, nullptr, nullptr);
+}
+
+int nt_buttiker_tdb_batched_dev(nt_ctx* c, const nt_params* p, const nt_probe_desc* probe, int nl, int nr, int b,
+                                int L, int npts, const double* E, const int* kidx, int nk, const nt_c128* Haml,
+                                const nt_c128* Hamlv, const nt_c128* Hamr, const nt_c128* Hamrv, const nt_c128* Hamd,
+                                const nt_c128* Hamdv, const nt_c128* Hamld, const nt_c128* Hamrd,
+                                const int* first_point_mask, double* T_LR, double* T_RL, int* status, int* iters,
+                                const nt_probe_out* out)
+{
+    if (!c) return NT_ERR_ARG;
+    if (!probe || !out || !probe->temperature || L < 2) return NT_ERR_ARG;
+    if (!out->flux || !out->jacobian || !out->flux_left || !out->flux_right || !out->a_sum || !out->gn_sum)
+        return NT_ERR_ARG;
+    if (!(probe->B_probe > 0.0) || !(probe->t_left > 0.0) || !(probe->t_right > 0.0)) return NT_ERR_ARG;
+    return tdb_entry(c, p, nl, nr, b, L, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
+                     first_point_mask, T_LR, T_RL, status, iters, nullptr, probe, out);
 }
 
 int nt_solve_tdb_adjoint_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts,

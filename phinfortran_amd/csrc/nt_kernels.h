@@ -156,6 +156,33 @@ hipError_t launch_hk_assemble(int nk, const double* kc, int nseg, const int* seg
                               const double* coef, const double* dr, int rows, int cols, double* out, int ld,
                               long kstride, hipStream_t s);
 
+// Buttiker probes (nt_probe.hip).
+// zl[i][p] = zd[p] - sigma_w(i, E[p]) (zeroed sigma when !lifetime), gam / dgam
+// = -2 Im sigma_w, -2 Im dsigma_w/dT; layer-major L x P.
+hipError_t launch_probe_shifts(const double* E, const double* zd, const double* temp, int L, int P, double B,
+                               double C, int lifetime, double* zl, double* gam, double* dgam, hipStream_t s);
+// out + 2 (p ostride + g ogstride) = Tr(X^+ Y) (complex) of m x n blocks
+template <typename T>
+hipError_t launch_frob_dot(int m, int n, BMat<T> X, BMat<T> Y, double* out, long ostride, long ogstride, int batch,
+                           int groups, hipStream_t s);
+// out + 2 (p ostride + g ogstride) = Tr X (complex) of n x n blocks
+template <typename T>
+hipError_t launch_trace_diag(int n, BMat<T> X, double* out, long ostride, long ogstride, int batch, int groups,
+                             hipStream_t s);
+struct ProbeCombineArgs {
+    int L, P, p0;             // layers, points in the chunk, chunk offset of the outputs
+    const double* E;          // chunk energies
+    const double* temp;       // L probe temperatures
+    double t_left, t_right;
+    const double* gam;        // L x P
+    const double* dgam;       // L x P
+    const double* acc;        // per point: L^2 + 5L + 6 complex traces
+    long acc_stride;          // complex entries per point
+    const int* first_mask;    // chunk mask, or null
+    double *flux, *jac, *flux_left, *flux_right, *a_sum, *gn_sum;   // indexed from point p0 + p
+};
+hipError_t launch_probe_combine(const ProbeCombineArgs& a, hipStream_t s);
+
 // Elementwise helpers (nt_elem.hip).
 // out[p] = sign * (dagger ? M^+ : M) for an n x n matrix
 template <typename T>

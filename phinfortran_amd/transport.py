@@ -9,6 +9,8 @@ behaviour (a nonzero status raises, where the reference would `stop`):
     Engine.G_block             negf/device.f90:289-560    (tb/device.f90:291-575)
     Engine.solve_tdb           the driver's ie loop, negf/phinfortran.f90:1196-1251,
                                batched over energies and k-points on the GPU
+    Engine.solve_buttiker      the same loop with Buttiker probes: get_G_full_tdb +
+                               get_G_lesser_tdb (negf/lesser.f90:475-809, 1199-1503)
 
 Matrices are numpy / torch complex128 in the natural (row-major) orientation;
 the conversion to the column-major C ABI happens here.
@@ -21,6 +23,9 @@ from typing import Optional, Sequence
 import numpy as np
 
 from . import _native as N
+
+# default lifetime prefactor B of the probes (negf/config.f90:97)
+B_PROBE = 5.0e-20 * (2.0 * 3.1415926 * 1.0e12)
 
 
 def _f(a):
@@ -307,6 +312,52 @@ class Engine:
             torch.cuda.synchronize(dev)
         return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters,
                     dHamd=gd.transpose(-1, -2), dHamdv=gv.transpose(-1, -2))
+
+    def solve_buttiker(self, dblocks, E, p: N.NtParams, temperature, t_left: float, t_right: float,
+                       B_probe: float = B_PROBE, T_probe: float = 430.0, lifetime: int = 1, kidx=None,
+                       first_point_mask=None, stream_sync=True):
+        """Batched solve with a Buttiker probe in every device layer
+        (get_G_full_tdb + get_G_lesser_tdb, negf/lesser.f90:475-809, 1199-1503).
+
+        temperature: the L probe temperatures.  Returns dict(T_LR, T_RL, status,
+        iters, flux (npts, L), jacobian (npts, L, L) with [p, i, j] =
+        d flux_i / d T_j, flux_left, flux_right (npts), a_sum, gn_sum (npts, L))."""
+        import torch
+        dev = torch.device("cuda", self.device)
+        E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
+        npts = E.numel()
+        nl, nr, b, L, nk = dblocks["nl"], dblocks["nr"], dblocks["b"], dblocks["L"], dblocks["nk"]
+        temp = torch.as_tensor(temperature, dtype=torch.float64, device=dev).contiguous()
+        if temp.numel() != L:
+            raise ValueError(f"{temp.numel()} probe temperatures for {L} layers")
+        if kidx is not None:
+            kidx = torch.as_tensor(kidx, dtype=torch.int32, device=dev).contiguous()
+        if first_point_mask is not None:
+            first_point_mask = torch.as_tensor(first_point_mask, dtype=torch.int32, device=dev).contiguous()
+        f64 = dict(dtype=torch.float64, device=dev)
+        T_LR = torch.empty(npts, **f64)
+        T_RL = torch.empty(npts, **f64)
+        status = torch.zeros(npts, dtype=torch.int32, device=dev)
+        iters = torch.zeros(npts, dtype=torch.int32, device=dev)
+        res = dict(flux=torch.zeros(npts, L, **f64), jacobian=torch.zeros(npts, L, L, **f64),
+                   flux_left=torch.zeros(npts, **f64), flux_right=torch.zeros(npts, **f64),
+                   a_sum=torch.zeros(npts, L, **f64), gn_sum=torch.zeros(npts, L, **f64))
+        po = N.NtProbeOut(**{k: res[k].data_ptr() for k in N.PROBE_OUT_FIELDS})
+        pd = N.NtProbeDesc(temp.data_ptr(), float(t_left), float(t_right), float(B_probe), float(T_probe),
+                           int(lifetime))
+        if self._torch_stream:
+            self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        st = self._L.nt_buttiker_tdb_batched_dev(
+            self.ctx, C.byref(p), C.byref(pd), nl, nr, b, L, npts, ptr(E), ptr(kidx), nk,
+            ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
+            ptr(dblocks["Hamd"]), ptr(dblocks["Hamdv"]), ptr(dblocks["Hamld"]), ptr(dblocks["Hamrd"]),
+            ptr(first_point_mask), ptr(T_LR), ptr(T_RL), ptr(status), ptr(iters), C.byref(po))
+        self._check(st, "solve_buttiker")
+        if stream_sync:
+            torch.cuda.synchronize(dev)
+        res["jacobian"] = res["jacobian"].transpose(-1, -2)     # column-major per point -> [p, i, j]
+        return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters, **res)
 
     def solve_tdb_multi(self, devices, p: N.NtParams):
         """Several devices (uneven b, L, leads) in one call.  devices: list of

@@ -1,0 +1,142 @@
+"""GPU parity of the Buttiker-probe solve (nt_buttiker_tdb_batched_dev) against
+the probe oracle (oracle/buttiker.py: get_G_full_tdb + get_G_lesser_tdb), and
+the physics at larger sizes: no probe or contact current in equilibrium,
+energy-current conservation over probes and contacts.
+
+FP64 tolerance: T to 1e-9 relative; probe currents, traces and contact
+currents to 1e-9 of the size of the terms they are differences of; the
+Jacobian to 1e-9 of its largest entry."""
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import buttiker as BT
+from phinfortran_amd import lattice as lt
+
+pytestmark = pytest.mark.gpu
+
+C_PROBE = 430.0
+
+
+@pytest.fixture(scope="module")
+def T():
+    from phinfortran_amd import transport
+    return transport
+
+
+def oparams(p):
+    return O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
+
+
+def oracle_point(blk, E, op, temp, t_left, t_right, B, lifetime=True, first=False):
+    d = blk.as_dict()
+    sg = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, E, op, want_plus_minus=False)
+    g = dict(gl=sg["gl_adv"].conj().T, gr=sg["gr_ret"], gml=-sg["gml_adv"], gmr=sg["gmr_ret"],
+             sl=sg["sl_adv"].conj().T, sr=sg["sr_ret"])
+    sig, dsig = BT.sigma_bp(np.array([E]), temp, B, C_PROBE, lifetime=lifetime)
+    f = BT.full_tdb(d, E, op.eta, op.eta0, sig=sig[:, 0], first_point=first, **g)
+    o = BT.lesser_tdb(d, E, f, g["gl"], g["gr"], sig[:, 0], dsig[:, 0], temp, t_left, t_right, first_point=first)
+    o.update(T_LR=f["T_LR"], T_RL=f["T_RL"], gam=(1j * (sig[:, 0] - sig[:, 0].conj())).real)
+    return o
+
+
+def check_point(r, i, o, temp, E):
+    """GPU point i against the oracle's point o."""
+    w = np.sqrt(E)
+    assert abs(r["T_LR"][i] - o["T_LR"]) <= 1e-9 * max(1.0, o["T_LR"]), (i, r["T_LR"][i], o["T_LR"])
+    assert abs(r["T_RL"][i] - o["T_RL"]) <= 1e-9 * max(1.0, o["T_RL"]), (i, r["T_RL"][i], o["T_RL"])
+    # traces: complex in the reference, imaginary parts at rounding level
+    for key in ("a_sum", "gn_sum"):
+        ref = o[{"a_sum": "A_sum", "gn_sum": "Gn_sum"}[key]]
+        sc = np.abs(ref).max()
+        assert np.abs(ref.imag).max() <= 1e-9 * max(sc, 1e-300)
+        assert np.abs(r[key][i] - ref.real).max() <= 1e-9 * max(sc, 1e-300), (key, i)
+    bose = np.array([BT.bose(w, t) for t in temp])
+    scale = np.abs(o["gam"]) * (np.abs(o["Gn_sum"]) + np.abs(o["A_sum"]) * bose)
+    assert np.all(np.abs(r["flux"][i] - o["flux"].real) <= 1e-9 * scale.max() + 1e-300), (i, r["flux"][i], o["flux"])
+    js = np.abs(o["jac"]).max()
+    assert np.abs(r["jacobian"][i] - o["jac"].real).max() <= 1e-9 * js + 1e-300, i
+    for key in ("flux_left", "flux_right"):
+        ref = o[key]
+        assert abs(r[key][i] - ref.real) <= 1e-9 * max(abs(ref), np.abs(o["A_sum"]).max() * w * bose.max()), key
+
+
+def run(engine, T, blocks, E, p, temp, t_left, t_right, B, **kw):
+    d = engine.upload_blocks(blocks)
+    r = engine.solve_buttiker(d, E, p, temp, t_left, t_right, B_probe=B, T_probe=C_PROBE, **kw)
+    return {k: v.cpu().numpy() for k, v in r.items()}
+
+
+@pytest.mark.parametrize("nl,b,L,nk,scale", [(5, 6, 4, 1, 1e4), (4, 4, 2, 2, 1e4), (24, 24, 5, 2, 1e4),
+                                            (12, 40, 7, 1, 1.0), (48, 48, 10, 1, 1e4)])
+def test_buttiker_against_oracle(engine, T, nl, b, L, nk, scale):
+    blocks = [lt.random_blocks(nl, b, L, seed=30 + i, scale=0.6) for i in range(nk)]
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
+    op = oparams(p)
+    rng = np.random.default_rng(L)
+    npts = 7
+    E = rng.uniform(0.05, 0.9, npts)
+    kidx = rng.integers(0, nk, npts).astype(np.int32)
+    mask = np.zeros(npts, np.int32)
+    mask[0] = 1
+    temp = rng.uniform(280.0, 320.0, L)
+    B = T.B_PROBE * scale
+    print(f"buttiker nl={nl} b={b} L={L} nk={nk} B x {scale}", flush=True)
+    r = run(engine, T, blocks, E, p, temp, 330.0, 270.0, B, kidx=kidx if nk > 1 else None, first_point_mask=mask)
+    assert (r["status"] == 0).all()
+    for i in range(npts):
+        blk = blocks[kidx[i]] if nk > 1 else blocks[0]
+        o = oracle_point(blk, E[i], op, temp, 330.0, 270.0, B, first=bool(mask[i]))
+        check_point(r, i, o, temp, E[i])
+    # the masked point: zero probe currents, Jacobian and traces; contact currents kept
+    assert r["T_LR"][0] == 0.0 and not r["flux"][0].any() and not r["jacobian"][0].any()
+    assert not r["a_sum"][0].any() and not r["gn_sum"][0].any() and r["flux_left"][0] != 0.0
+
+
+def test_buttiker_without_lifetime(engine, T):
+    """lifetime = 0 zeroes sigma_w but not its derivative: no probe current,
+    T equal to the probe-free solve, a diagonal Jacobian"""
+    blk = lt.random_blocks(8, 8, 4, seed=3, scale=0.6)
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
+    op = oparams(p)
+    E = np.array([0.2, 0.45, 0.7])
+    temp = np.array([300.0, 310.0, 290.0, 305.0])
+    r = run(engine, T, blk, E, p, temp, 320.0, 280.0, T.B_PROBE * 1e4, lifetime=0)
+    for i in range(3):
+        tl, tr, _ = O.solve_tdb(blk.as_dict(), E[i], op)
+        assert abs(r["T_LR"][i] - tl) < 1e-9 * max(1, tl) and abs(r["T_RL"][i] - tr) < 1e-9 * max(1, tr)
+        o = oracle_point(blk, E[i], op, temp, 320.0, 280.0, T.B_PROBE * 1e4, lifetime=False)
+        check_point(r, i, o, temp, E[i])
+    assert not r["flux"].any()
+    off = r["jacobian"] - np.einsum("pii->pi", r["jacobian"])[:, :, None] * np.eye(4)
+    assert not off.any() and np.abs(np.einsum("pii->pi", r["jacobian"])).min() > 0
+
+
+@pytest.mark.parametrize("b,L", [(96, 12), (192, 6)])
+def test_buttiker_equilibrium_and_conservation(engine, T, b, L):
+    """At sizes past the oracle: every current vanishes when probes and
+    contacts share one temperature, and the probe currents balance the
+    contact currents otherwise (energy-current conservation).  Both hold up
+    to the device broadening 10 eta(E), which removes current without a
+    source term, hence a vanishing eta here (the currents it leaves are
+    O(eta / gamma) of the terms, gamma the probe broadening)."""
+    blk = lt.random_blocks(b, b, L, seed=11, scale=0.6)
+    p = T.params("phonon", eta=1e-10, eta0=1e-11, convergence=1e-12, max_iter=4000)
+    E = np.linspace(0.1, 0.9, 16)
+    B = T.B_PROBE * 1e4
+    print(f"buttiker properties b={b} L={L}", flush=True)
+    r = run(engine, T, blk, E, p, np.full(L, 300.0), 300.0, 300.0, B)
+    assert (r["status"] == 0).all()
+    w = np.sqrt(E)
+    bose = np.array([BT.bose(x, 300.0) for x in w])
+    sig, _ = BT.sigma_bp(E, np.full(L, 300.0), B, C_PROBE)
+    gam = (1j * (sig - sig.conj())).real.max(axis=0)          # per point
+    size = np.abs(r["a_sum"]).max(axis=1) * bose
+    assert np.all(np.abs(r["flux"]).max(axis=1) < 1e-6 * gam * size), (np.abs(r["flux"]).max(axis=1) / (gam * size))
+    assert np.all(np.abs(r["flux_left"]) < 1e-6 * size * w) and np.all(np.abs(r["flux_right"]) < 1e-6 * size * w)
+    rng = np.random.default_rng(5)
+    temp = rng.uniform(280.0, 320.0, L)
+    r = run(engine, T, blk, E, p, temp, 330.0, 270.0, B)
+    total = r["flux_left"] / w + r["flux_right"] / w + r["flux"].sum(axis=1)
+    scale = np.abs(r["flux_left"] / w) + np.abs(r["flux_right"] / w) + np.abs(r["flux"]).sum(axis=1)
+    assert np.all(scale > 0) and np.all(np.abs(total) < 1e-6 * scale), np.abs(total / scale).max()

@@ -75,6 +75,10 @@ def main():
     ap.add_argument("--no-modes", dest="modes", action="store_false",
                     help="skip the mode-resolved sample (the si-si example's full mode-resolved run: solve, "
                          "modes, Brillouin-zone and primitive-cell unfolding of 49 k x 61 omega)")
+    ap.add_argument("--no-buttiker", dest="buttiker", action="store_false",
+                    help="skip the Buttiker-probe measurement")
+    ap.add_argument("--buttiker-points", type=int, default=128, help="frequencies of the probe solve")
+    ap.add_argument("--buttiker-cpu-sample", type=int, default=1, help="points of the probe CPU sample (0 = skip)")
     ap.add_argument("--modes-cpu-sample", type=int, default=6,
                     help="points of the mode-resolved CPU baseline sample (0 = skip)")
     args = ap.parse_args()
@@ -264,6 +268,63 @@ def main():
         return res
 
     modes = run_modes() if args.modes else None
+
+    def run_buttiker():
+        """probe solve (both sweeps with a probe in every layer, every block of
+        the full G, probe currents and Jacobian) on the headline k-point blocks;
+        the oracle's get_G_full_tdb + get_G_lesser_tdb on a sample beside it"""
+        npb = args.buttiker_points
+        Eb = E[:: max(1, args.points // npb)][:npb].contiguous()
+        p = T.params("phonon")
+        temp = np.linspace(310.0, 290.0, L)
+        eng.solve_buttiker(dblocks[0], Eb, p, temp, 320.0, 280.0)          # warm-up
+        eng.profile_reset()
+        eng.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nrep = max(1, args.steps // 2)
+        for s in range(nrep):
+            r = eng.solve_buttiker(dblocks[args.warmup + s], Eb, p, temp, 320.0, 280.0, stream_sync=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        eng.profile(False)
+        g = eng.profile_get(N.NT_K_GEMM)
+        npt = nrep * Eb.numel()
+        res = {"points_per_s": npt / el, "unit": "(omega,k)-probe-solves/s", "points": npt,
+               "status_ok": bool((r["status"] == 0).all()),
+               "what": f"b={blocks[0].b}, L={L}, {Eb.numel()} frequencies per k-point: decimation, both sweeps "
+                       f"with a probe self-energy in every layer, all L^2 blocks of the full G, probe currents, "
+                       f"Jacobian, contact currents, FP64",
+               "gemm": {"launches": g["launches"], "ms": g["ms"],
+                        "tflops": g["flops"] / max(1e-9, g["ms"] * 1e-3) / 1e12,
+                        "share_of_wall": g["ms"] * 1e-3 / el}}
+        log(f"buttiker: {npt / el:.1f} probe solves/s, gemm {res['gemm']['tflops']:.1f} TF/s")
+        if args.buttiker_cpu_sample > 0:
+            import oracle as O
+            from oracle import buttiker as OB
+            blk = blocks[args.warmup].as_dict()
+            op = O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
+            Es = Eb.cpu().numpy()[:: max(1, npb // args.buttiker_cpu_sample)][: args.buttiker_cpu_sample]
+            t0 = time.perf_counter()
+            for e in Es:
+                sg = O.surface_green(blk["Haml"], blk["Hamlv"], blk["Hamr"], blk["Hamrv"], e, op,
+                                     want_plus_minus=False)
+                gg = dict(gl=sg["gl_adv"].conj().T, gr=sg["gr_ret"], gml=-sg["gml_adv"], gmr=sg["gmr_ret"],
+                          sl=sg["sl_adv"].conj().T, sr=sg["sr_ret"])
+                sig, dsig = OB.sigma_bp(np.array([e]), temp, T.B_PROBE, 430.0)
+                f = OB.full_tdb(blk, e, op.eta, op.eta0, sig=sig[:, 0], **gg)
+                OB.lesser_tdb(blk, e, f, gg["gl"], gg["gr"], sig[:, 0], dsig[:, 0], temp, 320.0, 280.0)
+            dt = time.perf_counter() - t0
+            nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+            res["cpu_baseline"] = {"value": len(Es) / dt, "unit": "(omega,k)-probe-solves/s", "cores": nth,
+                                   "kind": "port",
+                                   "sample": f"{len(Es)} point(s) at b={blocks[0].b}, L={L}: the reference's "
+                                             f"recursion (dGn for every layer pair) in numpy, threaded BLAS, "
+                                             f"{dt:.1f} s"}
+            log(f"buttiker cpu sample: {len(Es)} points in {dt:.1f}s")
+        return res
+
+    probes = run_buttiker() if args.buttiker else None
     t64 = np.concatenate(T64)
     tmx = np.concatenate(Tmx)
     sig = t64 > 1e-3 * max(1e-300, t64.max())
@@ -338,6 +399,8 @@ def main():
         out["uneven_batching"] = c3
     if modes is not None:
         out["mode_resolved"] = modes
+    if probes is not None:
+        out["buttiker"] = probes
     if eladj is not None:
         out["adjoint"] = {"grads_per_s": solves / eladj, "unit": "(omega,k)-gradients/s",
                           "what": "T_LR and dW/dH of the 3 device layers around the interface (bonds owned by "

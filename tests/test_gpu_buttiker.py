@@ -140,3 +140,41 @@ def test_buttiker_equilibrium_and_conservation(engine, T, b, L):
     total = r["flux_left"] / w + r["flux_right"] / w + r["flux"].sum(axis=1)
     scale = np.abs(r["flux_left"] / w) + np.abs(r["flux_right"] / w) + np.abs(r["flux"]).sum(axis=1)
     assert np.all(scale > 0) and np.all(np.abs(total) < 1e-6 * scale), np.abs(total / scale).max()
+
+
+def test_probe_temperature_loop_against_oracle(engine, T, tmp_path):
+    """Three iterations of the self-consistent loop (phinfortran_amd.buttiker.run)
+    against the same loop driven by the oracle: probe and phonon temperatures
+    after the last iteration, and the files of the last iteration."""
+    from phinfortran_amd import buttiker as PB
+    nk, L, ne = 2, 3, 9
+    blocks = [lt.random_blocks(6, 6, L, seed=50 + i, scale=0.6) for i in range(nk)]
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
+    op = oparams(p)
+    E = np.linspace(0.0, 0.95, ne) ** 2
+    E[0] = 0.001 ** 2
+    B = T.B_PROBE * 1e4
+    print("probe loop", flush=True)
+    r = PB.run(engine, engine.upload_blocks(blocks), E, p, 330.0, 270.0, B_probe=B, T_probe=C_PROBE, max_outer=3,
+               out_dir=str(tmp_path))
+    assert r["iterations"] == 3
+    temp = PB.initial_temperatures(L, 330.0, 270.0)
+    t_ph = temp.copy()
+    for it in range(3):
+        shape = (nk, ne, L)
+        fl, gn, A = np.zeros(shape, complex), np.zeros(shape, complex), np.zeros(shape, complex)
+        jac = np.zeros((nk, ne, L, L), complex)
+        for k in range(nk):
+            for e in range(ne):
+                o = oracle_point(blocks[k], E[e], op, temp, 330.0, 270.0, B, first=e == 0)
+                fl[k, e], gn[k, e], A[k, e], jac[k, e] = o["flux"], o["Gn_sum"], o["A_sum"], o["jac"]
+        J, F = BT.integrate_jacobian(jac, fl, E)
+        t_ph = BT.get_ph_temperature(A.sum(axis=0) / nk, gn.sum(axis=0) / nk, E, t_ph)
+        temp, td = BT.update_temperature(temp, J, F)
+        assert abs(r["tdiff"][it] - td) <= 1e-8 * max(td, 1e-3), (it, r["tdiff"][it], td)
+    assert np.allclose(r["temperature_i"], temp, rtol=1e-9, atol=0)
+    assert np.allclose(r["temperature_ph"], t_ph, rtol=1e-9, atol=0)
+    rec = PB.read_fortran_records(str(tmp_path / "temperature_i.dat"))[0]
+    assert np.array_equal(rec, r["temperature_i"])
+    flux = PB.read_fortran_records(str(tmp_path / "flux.dat"))[0].reshape((nk, ne, L), order="F")
+    assert np.array_equal(flux, r["flux"])

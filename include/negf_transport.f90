@@ -211,14 +211,15 @@ module negf_transport
 
         integer(c_int) function nt_buttiker_tdb_batched_dev(ctx, p, probe, nl, nr, b, L, npts, E, kidx, nk, &
                 Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd, first_point_mask, &
-                T_LR, T_RL, status, iters, out) bind(C, name="nt_buttiker_tdb_batched_dev")
-            import :: c_int, c_ptr, nt_params, nt_probe_desc, nt_probe_out
+                T_LR, T_RL, status, iters, extras, out) bind(C, name="nt_buttiker_tdb_batched_dev")
+            import :: c_int, c_ptr, nt_params, nt_probe_desc, nt_probe_out, nt_solve_extras
             type(c_ptr), value :: ctx
             type(nt_params), intent(in) :: p
             type(nt_probe_desc), intent(in) :: probe
             integer(c_int), value :: nl, nr, b, L, npts, nk
             type(c_ptr), value :: E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd
             type(c_ptr), value :: first_point_mask, T_LR, T_RL, status, iters
+            type(nt_solve_extras), intent(in) :: extras
             type(nt_probe_out), intent(in) :: out
         end function
 

@@ -127,7 +127,8 @@ int nt_surface_green_tdb(nt_ctx* ctx, const nt_params* p, int nl, int nr, double
  *   (lesser.f90:802-807).
  * Outputs: transl (T_LR), transr (T_RL); G_D_RL (nr x nl), G_D_LR (nl x nr),
  * G_L (nl x nl), G_R (nr x nr) may be NULL.  The Buttiker-probe bookkeeping
- * (GLeft, G_full, G_q_qp, ...) is not produced. */
+ * (GLeft, G_full, G_q_qp, ...) is not produced here; the probe path is
+ * nt_buttiker_tdb_batched_dev. */
 int nt_get_G_full_tdb(nt_ctx* ctx, const nt_params* p, int first_point,
                       int nl, int nr, int b, int L, double E,
                       const nt_c128* gl, const nt_c128* gr, const nt_c128* gml, const nt_c128* gmr,
@@ -283,6 +284,10 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, in
  * whose imaginary parts vanish in exact arithmetic; they are returned real.
  * Points with a nonzero first_point_mask get zero probe currents, Jacobian
  * and traces, but their contact currents (negf/lesser.f90:1467-1472).
+ * extras (may be NULL): the lead and RGF outputs of nt_solve_tdb_batched_dev,
+ * here with the probes in the sweeps -- what vel_mat_tdb receives in the
+ * Buttiker loop (negf/phinfortran.f90:1239-1250), so nt_modes_tdb_batched_dev
+ * on them gives the mode-resolved transmission with probes.
  * Runs in FP64 whatever p->precision says; needs L >= 2 and nl == nr.
  * temperature: L probe temperatures in device memory. */
 typedef struct {
@@ -310,7 +315,7 @@ int nt_buttiker_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, const nt_probe_
                                 const nt_c128* Hamld, const nt_c128* Hamrd,
                                 const int* first_point_mask,
                                 double* T_LR, double* T_RL, int* status, int* iters,
-                                const nt_probe_out* out);
+                                const nt_solve_extras* extras, const nt_probe_out* out);
 
 /* ------------------------------------------------ mode-resolved transmission
  * vel_mat_tdb (negf/velocity_mat.f90:1692-3150; tb/velocity_mat.f90:222-1560)

@@ -77,7 +77,8 @@ def full_tdb(d, E, eta, eta0, gl, gr, gml, gmr, sl, sr, sig, first_point=False):
     """get_G_full_tdb with probes: d = tdb blocks (Hamd, Hamdv, Hamld, Hamrd,
     Haml, Hamr), gl..sr the retarded lead quantities as the batched solve
     passes them (gl = gl_adv^+, gml = -gml_adv, sl = sl_adv^+), sig (L): the
-    probe self-energy of every layer at this point."""
+    probe self-energy of every layer at this point.  Also returns what
+    vel_mat_tdb receives: G_D_LR, G_D_RL and the closing-layer G_L, G_R."""
     Hd, Hdv, Hld, Hrd = d["Hamd"], d["Hamdv"], d["Hamld"], d["Hamrd"]
     L, b = len(Hd), Hd[0].shape[0]
     nl, nr = gl.shape[0], gr.shape[0]
@@ -125,7 +126,8 @@ def full_tdb(d, E, eta, eta0, gl, gr, gml, gmr, sl, sr, sig, first_point=False):
     TRL = np.real(np.trace(G_DRL @ gml @ G_DRL.conj().T @ gmr))
     TLR = 0.0 if TLR < 1e-60 or first_point else TLR
     TRL = 0.0 if TRL < 1e-60 or first_point else TRL
-    return dict(GLeft=GL, G_full=G_full, G_q_qp=Gq, T_LR=TLR, T_RL=TRL)
+    return dict(GLeft=GL, G_full=G_full, G_q_qp=Gq, T_LR=TLR, T_RL=TRL, G_D_LR=G_DLR, G_D_RL=G_DRL, G_L=GLl,
+                G_R=GR)
 
 
 def lesser_tdb(d, E, f, gl, gr, sig, dsig, T_layers, T_left, T_right, first_point=False):

@@ -147,7 +147,7 @@ def lib():
     L.nt_solve_tdb_batched_dev.argtypes = solve_args
     L.nt_solve_tdb_batched.argtypes = solve_args
     L.nt_solve_tdb_adjoint_batched_dev.argtypes = solve_args[:19] + [P, I, I, P, P, P, P, P, P]
-    L.nt_buttiker_tdb_batched_dev.argtypes = solve_args[:2] + [C.POINTER(NtProbeDesc)] + solve_args[2:23] + [
+    L.nt_buttiker_tdb_batched_dev.argtypes = solve_args[:2] + [C.POINTER(NtProbeDesc)] + solve_args[2:] + [
         C.POINTER(NtProbeOut)]
     L.nt_solve_tdb_multi_dev.argtypes = [P, pp, I, P]
     L.nt_hk_assemble_dev.argtypes = [P, I, P, I, P, P, I, P, P, I, I, P, I, C.c_longlong]

@@ -162,9 +162,10 @@ def run(engine, dblocks, E, p, t_left: float, t_right: float, B_probe: float = B
     dblocks (Engine.upload_blocks) and the energy grid E (ne odd; point ie = 0
     is the reference's ie = 1 and gets its zero-frequency treatment).
 
-    Returns dict(temperature_i, temperature_ph, tdiff (per iteration), T_LR,
-    T_RL (nk, ne) of the last iteration, flux, gn, A (nk, ne, L), flux_left,
-    flux_right (nk, ne), iterations)."""
+    Returns dict(temperature_i, temperature_ph, tdiff (per iteration),
+    temperature_used (the probe temperatures of the last iteration's solve),
+    T_LR, T_RL (nk, ne) of the last iteration, flux, gn, A (nk, ne, L),
+    flux_left, flux_right (nk, ne), iterations)."""
     import torch
     E = np.asarray(E, float)
     ne, nk, L = len(E), dblocks["nk"], dblocks["L"]
@@ -177,7 +178,9 @@ def run(engine, dblocks, E, p, t_left: float, t_right: float, B_probe: float = B
     mask[::ne] = 1
     hist = []
     res = None
+    used = temp
     for it in range(max_outer):
+        used = temp
         r = engine.solve_buttiker(dblocks, Ep, p, temp, t_left, t_right, B_probe=B_probe, T_probe=T_probe,
                                   lifetime=lifetime, kidx=kidx if nk > 1 else None, first_point_mask=mask)
         bad = int(torch.count_nonzero(r["status"]).item())
@@ -198,7 +201,7 @@ def run(engine, dblocks, E, p, t_left: float, t_right: float, B_probe: float = B
             log(f"probe iteration {it + 1}: |T_old - T_new| = {tdiff:.3e}")
         if tdiff < TDIFF_STOP:
             break
-    return dict(temperature_i=temp, temperature_ph=t_ph, tdiff=hist, iterations=len(hist),
+    return dict(temperature_i=temp, temperature_ph=t_ph, tdiff=hist, iterations=len(hist), temperature_used=used,
                 T_LR=res["T_LR"].reshape(nk, ne), T_RL=res["T_RL"].reshape(nk, ne),
                 flux=res["flux"].reshape(nk, ne, L), gn=res["gn_sum"].reshape(nk, ne, L),
                 A=res["a_sum"].reshape(nk, ne, L), flux_left=res["flux_left"].reshape(nk, ne),

@@ -2469,7 +2469,7 @@ int nt_buttiker_tdb_batched_dev(nt_ctx* c, const nt_params* p, const nt_probe_de
                                 const nt_c128* Hamlv, const nt_c128* Hamr, const nt_c128* Hamrv, const nt_c128* Hamd,
                                 const nt_c128* Hamdv, const nt_c128* Hamld, const nt_c128* Hamrd,
                                 const int* first_point_mask, double* T_LR, double* T_RL, int* status, int* iters,
-                                const nt_probe_out* out)
+                                const nt_solve_extras* extras, const nt_probe_out* out)
 {
     if (!c) return NT_ERR_ARG;
     if (!probe || !out || !probe->temperature || L < 2) return NT_ERR_ARG;
@@ -2477,7 +2477,7 @@ int nt_buttiker_tdb_batched_dev(nt_ctx* c, const nt_params* p, const nt_probe_de
         return NT_ERR_ARG;
     if (!(probe->B_probe > 0.0) || !(probe->t_left > 0.0) || !(probe->t_right > 0.0)) return NT_ERR_ARG;
     return tdb_entry(c, p, nl, nr, b, L, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
-                     first_point_mask, T_LR, T_RL, status, iters, nullptr, probe, out);
+                     first_point_mask, T_LR, T_RL, status, iters, extras, probe, out);
 }
 
 int nt_solve_tdb_adjoint_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts,

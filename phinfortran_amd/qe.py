@@ -50,7 +50,13 @@ def read_config(path):
                 eta=g("eta"), eta0=g("eta0"), convergence=g("convergence"), r_cutoff=g("r_cutoff"),
                 nz=int(vals["nz"]), n_bloch_x=int(vals["n_bloch_x"]), n_bloch_y=int(vals["n_bloch_y"]),
                 period=int(vals["period_left"]), buffer=int(vals["buffer_left"]),
-                lambda_eta=g("lambda_eta") if "lambda_eta" in vals else 5.0e-3)
+                lambda_eta=g("lambda_eta") if "lambda_eta" in vals else 5.0e-3,
+                # Buttiker probes (negf/config.f90:28-30, defaults :91-98)
+                buttiker=int(vals.get("buttiker", 0)), lifetime=int(vals.get("lifetime", 0)),
+                t_left=g("t_left") if "t_left" in vals else 300.0,
+                t_right=g("t_right") if "t_right" in vals else 300.0,
+                B_probe=g("B_probe") if "B_probe" in vals else 5.0e-20 * (2.0 * 3.1415926 * 1.0e12),
+                T_probe=g("T_probe") if "T_probe" in vals else 430.0)
 
 
 def read_qe_fc(cfg, path, mass_profile=None):

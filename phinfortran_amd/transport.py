@@ -239,19 +239,7 @@ class Engine:
         T_RL = torch.empty(npts, dtype=torch.float64, device=dev)
         status = torch.zeros(npts, dtype=torch.int32, device=dev)
         iters = torch.zeros(npts, dtype=torch.int32, device=dev)
-        shapes = dict(gl_adv=(nl, nl), gr_ret=(nr, nr), gl_plus=(nl, nl), gr_minus=(nr, nr), gml_adv=(nl, nl),
-                      gmr_ret=(nr, nr), sl_adv=(nl, nl), sr_ret=(nr, nr), G_D_LR=(nr, nl), G_D_RL=(nl, nr),
-                      G_L=(nl, nl), G_R=(nr, nr))
-        ex = N.NtSolveExtras()
-        res = {}
-        for name in extras:
-            if name in ("sdos_l", "sdos_r"):
-                t = torch.zeros(npts, dtype=torch.float64, device=dev)
-            else:
-                # stored column-major: tensor [p, col, row]
-                t = torch.zeros((npts,) + shapes[name], dtype=torch.complex128, device=dev)
-            res[name] = t
-            setattr(ex, name, t.data_ptr())
+        ex, res = self._extras(extras, npts, nl, nr, dev)
         if self._torch_stream:
             self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
         ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
@@ -267,6 +255,25 @@ class Engine:
         for name, t in res.items():
             out[name] = t if t.dim() == 1 else t.transpose(-1, -2)
         return out
+
+    @staticmethod
+    def _extras(names, npts, nl, nr, dev):
+        """device buffers of the requested nt_solve_extras fields (complex
+        matrices stored column-major: tensor [p, col, row])"""
+        import torch
+        shapes = dict(gl_adv=(nl, nl), gr_ret=(nr, nr), gl_plus=(nl, nl), gr_minus=(nr, nr), gml_adv=(nl, nl),
+                      gmr_ret=(nr, nr), sl_adv=(nl, nl), sr_ret=(nr, nr), G_D_LR=(nr, nl), G_D_RL=(nl, nr),
+                      G_L=(nl, nl), G_R=(nr, nr))
+        ex = N.NtSolveExtras()
+        res = {}
+        for name in names:
+            if name in ("sdos_l", "sdos_r"):
+                t = torch.zeros(npts, dtype=torch.float64, device=dev)
+            else:
+                t = torch.zeros((npts,) + shapes[name], dtype=torch.complex128, device=dev)
+            res[name] = t
+            setattr(ex, name, t.data_ptr())
+        return ex, res
 
     def solve_tdb_adjoint(self, dblocks, E, p: N.NtParams, weight, lo: int, hi: int, kidx=None,
                           first_point_mask=None, stream_sync=True):
@@ -315,13 +322,15 @@ class Engine:
 
     def solve_buttiker(self, dblocks, E, p: N.NtParams, temperature, t_left: float, t_right: float,
                        B_probe: float = B_PROBE, T_probe: float = 430.0, lifetime: int = 1, kidx=None,
-                       first_point_mask=None, stream_sync=True):
+                       first_point_mask=None, stream_sync=True, extras=()):
         """Batched solve with a Buttiker probe in every device layer
         (get_G_full_tdb + get_G_lesser_tdb, negf/lesser.f90:475-809, 1199-1503).
 
         temperature: the L probe temperatures.  Returns dict(T_LR, T_RL, status,
         iters, flux (npts, L), jacobian (npts, L, L) with [p, i, j] =
-        d flux_i / d T_j, flux_left, flux_right (npts), a_sum, gn_sum (npts, L))."""
+        d flux_i / d T_j, flux_left, flux_right (npts), a_sum, gn_sum (npts, L)),
+        plus the requested extras as solve_tdb returns them (with the probes
+        in the sweeps)."""
         import torch
         dev = torch.device("cuda", self.device)
         E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
@@ -343,6 +352,7 @@ class Engine:
                    flux_left=torch.zeros(npts, **f64), flux_right=torch.zeros(npts, **f64),
                    a_sum=torch.zeros(npts, L, **f64), gn_sum=torch.zeros(npts, L, **f64))
         po = N.NtProbeOut(**{k: res[k].data_ptr() for k in N.PROBE_OUT_FIELDS})
+        ex, exres = self._extras(extras, npts, nl, nr, dev)
         pd = N.NtProbeDesc(temp.data_ptr(), float(t_left), float(t_right), float(B_probe), float(T_probe),
                            int(lifetime))
         if self._torch_stream:
@@ -352,11 +362,13 @@ class Engine:
             self.ctx, C.byref(p), C.byref(pd), nl, nr, b, L, npts, ptr(E), ptr(kidx), nk,
             ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
             ptr(dblocks["Hamd"]), ptr(dblocks["Hamdv"]), ptr(dblocks["Hamld"]), ptr(dblocks["Hamrd"]),
-            ptr(first_point_mask), ptr(T_LR), ptr(T_RL), ptr(status), ptr(iters), C.byref(po))
+            ptr(first_point_mask), ptr(T_LR), ptr(T_RL), ptr(status), ptr(iters), C.byref(ex), C.byref(po))
         self._check(st, "solve_buttiker")
         if stream_sync:
             torch.cuda.synchronize(dev)
         res["jacobian"] = res["jacobian"].transpose(-1, -2)     # column-major per point -> [p, i, j]
+        for name, t in exres.items():
+            res[name] = t if t.dim() == 1 else t.transpose(-1, -2)
         return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters, **res)
 
     def solve_tdb_multi(self, devices, p: N.NtParams):
@@ -397,7 +409,7 @@ class Engine:
     MODE_FIELDS = ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "G_D_RL", "G_D_LR", "G_L", "G_R")
 
     def solve_modes(self, dblocks, E, p: N.NtParams, a_z_L: float, a_z_R: float, kidx=None, first_point_mask=None,
-                    tb: bool = False, want=("trl", "tlr", "rrr", "rll"), unfold=None, ksc=None, pc=None):
+                    tb: bool = False, want=("trl", "tlr", "rrr", "rll"), unfold=None, ksc=None, pc=None, probe=None):
         """Batched solve followed by the mode-resolved transmission (vel_mat_tdb).
 
         Returns the solve's dict plus k_z_* (npts, n), n_prop (npts, 4),
@@ -412,10 +424,19 @@ class Engine:
         spec_tlr, spec_rrr, spec_rll (npts, n_incident, 2): (total, specular)
         per incident mode.  pc (unfold.PrimitiveCell, with unfold) adds the
         primitive-cell k and group velocity of every mode: kp_* and v_*
-        (npts, n, 3), ncand_* (npts, n) (find_k_pc, see unfold_pc)."""
+        (npts, n, 3), ncand_* (npts, n) (find_k_pc, see unfold_pc).
+
+        probe (dict of solve_buttiker's keyword arguments: temperature, t_left,
+        t_right[, B_probe, T_probe, lifetime]) solves with Buttiker probes in
+        the sweeps, as vel_mat_tdb sees them in the reference's Buttiker loop,
+        and adds the probe outputs."""
         import torch
         dev = torch.device("cuda", self.device)
-        r = self.solve_tdb(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=self.MODE_FIELDS)
+        if probe is None:
+            r = self.solve_tdb(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=self.MODE_FIELDS)
+        else:
+            r = self.solve_buttiker(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask,
+                                    extras=self.MODE_FIELDS, **probe)
         E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
         npts = E.numel()
         nl, nr, nk = dblocks["nl"], dblocks["nr"], dblocks["nk"]

@@ -42,6 +42,10 @@ def test_probe_free_limit_is_the_caroli_solve(sys4, E):
     f = BT.full_tdb(d, E, p.eta, p.eta0, sig=np.zeros(4), **g)
     tl, tr, _ = O.solve_tdb(d, E, p)
     assert abs(f["T_LR"] - tl) < 1e-10 * max(1, tl) and abs(f["T_RL"] - tr) < 1e-10 * max(1, tr)
+    r = O.rgf(list(d["Hamd"]), list(d["Hamdv"]), d["Hamld"], d["Hamrd"], d["Haml"], d["Hamr"], g["gl"], g["gr"],
+              g["gml"], g["gmr"], g["sl"], g["sr"], E, p)
+    for key in ("G_D_LR", "G_D_RL", "G_L", "G_R"):
+        assert np.abs(f[key] - r[key]).max() < 1e-10 * np.abs(r[key]).max(), key
     # G_full's diagonal blocks against the dense inverse of the device with lead self-energies
     Hd, Hdv = d["Hamd"], d["Hamdv"]
     L, b = len(Hd), Hd[0].shape[0]

@@ -178,3 +178,41 @@ def test_probe_temperature_loop_against_oracle(engine, T, tmp_path):
     assert np.array_equal(rec, r["temperature_i"])
     flux = PB.read_fortran_records(str(tmp_path / "flux.dat"))[0].reshape((nk, ne, L), order="F")
     assert np.array_equal(flux, r["flux"])
+
+
+def test_mode_resolved_with_probes(engine, T):
+    """vel_mat_tdb on the probe sweep's outputs (the reference's Buttiker loop
+    calls it every iteration): mode sums against the oracle's vel_mat_tdb on
+    the probe oracle's G_D_LR, G_D_RL, G_L, G_R."""
+    from oracle import modes as M
+    n, L = 6, 3
+    blk = lt.random_blocks(n, n, L, seed=71, scale=0.6)
+    blk.Hamr[...] = blk.Haml
+    blk.Hamrv[...] = blk.Hamlv.conj().T
+    p = T.params("phonon", eta=1e-7, eta0=1e-8, convergence=1e-13, max_iter=3000)
+    op = oparams(p)
+    E = np.array([0.12, 0.3, 0.45, 0.58])
+    temp = np.array([305.0, 295.0, 300.0])
+    B = T.B_PROBE * 1e4
+    az = 1.3
+    print("modes with probes", flush=True)
+    r = engine.solve_modes(engine.upload_blocks(blk), E, p, az, az,
+                           probe=dict(temperature=temp, t_left=320.0, t_right=280.0, B_probe=B, T_probe=C_PROBE))
+    assert (r["modes_status"].cpu().numpy() == 0).all() and (r["status"].cpu().numpy() == 0).all()
+    sums = r["sums"].cpu().numpy()
+    d = blk.as_dict()
+    for i, e in enumerate(E):
+        sg = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, e, op, want_plus_minus=True)
+        sig, _ = BT.sigma_bp(np.array([e]), temp, B, C_PROBE)
+        f = BT.full_tdb(d, e, op.eta, op.eta0, gl=sg["gl_adv"].conj().T, gr=sg["gr_ret"], gml=-sg["gml_adv"],
+                        gmr=sg["gmr_ret"], sl=sg["sl_adv"].conj().T, sr=sg["sr_ret"], sig=sig[:, 0])
+        m = M.vel_mat_tdb(e, sg["gl_adv"], sg["gr_ret"], sg["gl_plus"], sg["gr_minus"], sg["gml_adv"], sg["gmr_ret"],
+                          f["G_D_RL"], f["G_D_LR"], f["G_L"], f["G_R"], d["Haml"], d["Hamr"], d["Hamlv"],
+                          d["Hamrv"], op.eta, op.eta0, p.lambda_eta, az, az)
+        inc, out = m["k_z_L_adv"] < 99, m["k_z_R_ret"] < 99
+        ref = [(np.abs(m["trl"][np.ix_(out, inc)]) ** 2).sum(),
+               (np.abs(m["tlr"][np.ix_(m["k_z_L_ret"] < 99, m["k_z_R_adv"] < 99)]) ** 2).sum(),
+               (np.abs(m["rrr"][np.ix_(out, m["k_z_R_adv"] < 99)]) ** 2).sum(),
+               (np.abs(m["rll"][np.ix_(m["k_z_L_ret"] < 99, inc)]) ** 2).sum()]
+        assert np.abs(sums[i] - ref).max() < 1e-6 * max(1.0, max(ref)), (i, sums[i], ref)
+        assert abs(r["T_LR"][i].item() - f["T_LR"]) < 1e-9 * max(1.0, f["T_LR"])

@@ -85,3 +85,40 @@ def test_driver_mode_resolved_run(engine, tmp_path):
         assert abs(n_mine - n_gold) <= max(2, 0.05 * n_gold), (ik, n_mine, n_gold)
         for name in ("tlr", "right", "left", "mdl", "mdr"):
             assert os.path.getsize(os.path.join(tmp_path, "%s_specular_param_%06d.dat" % (name, ik + 1))) > 0
+
+
+def test_driver_buttiker_run(engine, tmp_path):
+    """The si-si example with buttiker = 1: two iterations of the probe loop,
+    its unformatted files, and transmission tables equal to a direct probe
+    solve at the temperatures of the last iteration (and different from the
+    probe-free tables)."""
+    from phinfortran_amd import buttiker as PB
+    from phinfortran_amd import driver
+    from phinfortran_amd import transport as T
+    txt = open(os.path.join(S.FIXTURE, "config")).read()
+    txt = txt.replace("buttiker = 0", "buttiker = 1").replace("lifetime = 0", "lifetime = 1")
+    txt = txt.replace("t_left = 300.0d0", "t_left = 310.0d0").replace("t_right = 300.0d0", "t_right = 290.0d0")
+    cfgp = os.path.join(tmp_path, "config")
+    open(cfgp, "w").write(txt)
+    print("driver buttiker", flush=True)
+    out = driver.run_qe_tdb(cfgp, os.path.join(S.FIXTURE, "Si444.fc"), str(tmp_path), engine=engine, max_outer=2)
+    loop = out["buttiker"]
+    assert loop["iterations"] == 2 and len(loop["tdiff"]) == 2
+    nk, ne = out["transl"].shape[1], out["transl"].shape[0]
+    L = out["blocks"]["L"]
+    flux = PB.read_fortran_records(os.path.join(tmp_path, "flux.dat"))[0]
+    assert flux.size == nk * ne * L
+    assert np.array_equal(PB.read_fortran_records(os.path.join(tmp_path, "temperature_i.dat"))[0],
+                          loop["temperature_i"])
+    cfg = S.read_config()
+    p = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500)
+    kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)
+    mask = np.tile((np.arange(ne) == 0).astype(np.int32), nk)
+    r = engine.solve_buttiker(out["blocks"], np.tile(out["E"], nk), p, loop["temperature_used"], 310.0, 290.0,
+                              lifetime=1, kidx=kidx, first_point_mask=mask)
+    tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
+    assert np.array_equal(tl, out["transl"])
+    mine = np.loadtxt(os.path.join(tmp_path, "transl_tot.dat"))
+    gold = np.loadtxt(os.path.join(S.FIXTURE, "transl_tot.dat"))
+    assert mine.shape == gold.shape and np.isfinite(mine).all()
+    assert np.abs(mine[:, 1:] - gold[:, 1:]).max() > 1e-6       # the probes scatter

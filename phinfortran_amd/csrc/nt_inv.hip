@@ -11,7 +11,11 @@
 //      written back and kept in LDS; the row exchanges of the panel are folded
 //      into a gather permutation pi.
 //   3. every other column strip of 16 is updated with MFMA:
-//         new[:, q] = old[pi(:), q] (rows J zeroed) + W * old[pi(J), q].
+//         new[:, q] = old[pi(:), q] (rows J zeroed) + W * old[pi(J), q];
+//      software-pipelined (n <= 512): the next strip's pivot rows and row
+//      gathers are loaded into registers while the current strip's MFMAs run.
+//      Each wave owns MRT row tiles of 16 (MRT sized to n keeps two
+//      workgroups per CU at n = 192).
 // At the end, X[:, j] = W[:, sigma(j)] undoes the column order (sigma is the
 // reverse composition of the row exchanges).
 //
@@ -48,7 +52,7 @@ __device__ __forceinline__ int wave_of() { return threadIdx.x >> 6; }
 template <typename T>
 __device__ __forceinline__ T cabs1(T r, T i) { return fabs(r) + fabs(i); }
 
-template <typename T, int NB, int RPT>
+template <typename T, int NB, int RPT, int MRT>
 __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
 {
     typedef typename acc4<T>::type A4;
@@ -266,34 +270,38 @@ __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
             }
         }
         // ---- trailing update of all other columns, strip by strip
-        for (int q0 = 0; q0 < n; q0 += QW) {
-            if (q0 >= j0 && q0 + QW <= j0 + nbe) continue;   // strip is inside J
-            __syncthreads();
-            // old pivot rows of this strip -> LDS (zero beyond nbe / n)
-            for (int e = t; e < NB * QW; e += NT) {
+        const int nrt = npad / 16;
+        constexpr int BPT = (NB * QW + NT - 1) / NT;
+        auto next_strip = [&](int q) {      // next strip not inside J, or n
+            for (q += QW; q < n; q += QW)
+                if (!(q >= j0 && q + QW <= j0 + nbe)) return q;
+            return n;
+        };
+        T bpr[BPT], bpi[BPT];               // old pivot rows of the strip (zero beyond nbe / n)
+        T cr[MRT][4], ci[MRT][4];           // accumulator init: old[pi(row)][col], rows outside J
+        auto load_strip = [&](int qs) {
+#pragma unroll
+            for (int v = 0; v < BPT; ++v) {
+                const int e = t + v * NT;
                 const int k = e / QW, qq = e % QW;
-                const int col = q0 + qq;
+                const int col = qs + qq;
                 T vr = 0, vi = 0;
-                if (k < nbe && col < n && !(col >= j0 && col < j0 + nbe)) {
+                if (e < NB * QW && k < nbe && col < n && !(col >= j0 && col < j0 + nbe)) {
                     const int row = pi[j0 + k];
                     V2 q = *reinterpret_cast<const V2*>(W + 2 * ((long)row + (long)col * ld));
                     vr = q.x;
                     vi = q.y;
                 }
-                s.btr[k][qq] = vr;
-                s.bti[k][qq] = vi;
+                bpr[v] = vr;
+                bpi[v] = vi;
             }
-            // accumulator init: old[pi(row)][col] for rows outside J
-            constexpr int MAXRT = (RPT * NT / 16 + 3) / 4;
-            const int nrt = npad / 16;
-            A4 accR[MAXRT], accI[MAXRT];
 #pragma unroll
-            for (int u = 0; u < MAXRT; ++u) {
+            for (int u = 0; u < MRT; ++u) {
                 const int rt = wave + 4 * u;
                 const int row = rt * 16 + (lane & 15);
 #pragma unroll
                 for (int r4 = 0; r4 < 4; ++r4) {
-                    const int col = q0 + mfma_out_row<T>(lane, r4);
+                    const int col = qs + mfma_out_row<T>(lane, r4);
                     T vr = 0, vi = 0;
                     if (rt < nrt && row < n && col < n && !(row >= j0 && row < j0 + nbe)) {
                         const int src = pi[row];
@@ -301,13 +309,41 @@ __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
                         vr = q.x;
                         vi = q.y;
                     }
-                    accR[u][r4] = vr;
-                    accI[u][r4] = vi;
+                    cr[u][r4] = vr;
+                    ci[u][r4] = vi;
                 }
             }
+        };
+        // (with RPT > 2 the prefetch registers would spill: load in place)
+        constexpr bool PIPE = RPT <= 2;
+        int q0 = next_strip(-QW);
+        if (PIPE && q0 < n) load_strip(q0);
+        while (q0 < n) {
+            if (!PIPE) load_strip(q0);
+            __syncthreads();                // the previous strip's readers of btr are done
+#pragma unroll
+            for (int v = 0; v < BPT; ++v) {
+                const int e = t + v * NT;
+                if (e < NB * QW) {
+                    s.btr[e / QW][e % QW] = bpr[v];
+                    s.bti[e / QW][e % QW] = bpi[v];
+                }
+            }
+            A4 accR[MRT], accI[MRT];
+#pragma unroll
+            for (int u = 0; u < MRT; ++u)
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    accR[u][r4] = cr[u][r4];
+                    accI[u][r4] = ci[u][r4];
+                }
+            // strips are disjoint column ranges: the next strip's loads never
+            // touch the columns this one stores
+            const int qn = next_strip(q0);
+            if (PIPE && qn < n) load_strip(qn);
             __syncthreads();
 #pragma unroll
-            for (int u = 0; u < MAXRT; ++u) {
+            for (int u = 0; u < MRT; ++u) {
                 const int rt = wave + 4 * u;
                 if (rt >= nrt) continue;
                 const int row = rt * 16 + (lane & 15);
@@ -332,6 +368,7 @@ __global__ __launch_bounds__(NT) void k_inv_gj(InvArgs<T> a)
                     }
                 }
             }
+            q0 = qn;
         }
         __syncthreads();
     }
@@ -366,30 +403,37 @@ size_t inv_smem_bytes(int n)
     return sizeof(InvSmemFixed<T, NB>) + 2 * sizeof(T) * (size_t)NB * npad + 3 * sizeof(int) * (size_t)n;
 }
 
-template <typename T, int NB, int RPT>
+template <typename T, int NB, int RPT, int MRT>
 hipError_t launch_inv_t(const InvArgs<T>& a, hipStream_t s)
 {
     const size_t sm = inv_smem_bytes<T, NB>(a.n);
     static bool configured = false;
     if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inv_gj<T, NB, RPT>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inv_gj<T, NB, RPT, MRT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         configured = true;
     }
-    hipLaunchKernelGGL((k_inv_gj<T, NB, RPT>), dim3(a.batch), dim3(NT), sm, s, a);
+    hipLaunchKernelGGL((k_inv_gj<T, NB, RPT, MRT>), dim3(a.batch), dim3(NT), sm, s, a);
     return hipGetLastError();
 }
 
 // Panel width NB is chosen so the LDS copy of the panel stays under ~100 KB;
 // RPT (rows per thread) = ceil(n / 256) must not exceed MAXRPT for that NB.
+// MRT (row tiles of 16 per wave in the trailing update) = ceil(n / 64).
 template <typename T, int NB, int MAXRPT>
 hipError_t launch_inv_nb(const InvArgs<T>& a, hipStream_t s)
 {
     const int rpt = (a.n + NT - 1) / NT;
-    if (rpt <= 1) return launch_inv_t<T, NB, 1>(a, s);
-    if constexpr (MAXRPT >= 2) if (rpt == 2) return launch_inv_t<T, NB, 2>(a, s);
-    if constexpr (MAXRPT >= 3) if (rpt == 3) return launch_inv_t<T, NB, 3>(a, s);
-    if constexpr (MAXRPT >= 4) if (rpt == 4) return launch_inv_t<T, NB, 4>(a, s);
+    const int mrt = (a.n + 63) / 64;
+    if (rpt <= 1) {
+        if (mrt <= 1) return launch_inv_t<T, NB, 1, 1>(a, s);
+        if (mrt == 2) return launch_inv_t<T, NB, 1, 2>(a, s);
+        if (mrt == 3) return launch_inv_t<T, NB, 1, 3>(a, s);
+        return launch_inv_t<T, NB, 1, 4>(a, s);
+    }
+    if constexpr (MAXRPT >= 2) if (rpt == 2) return launch_inv_t<T, NB, 2, 8>(a, s);
+    if constexpr (MAXRPT >= 3) if (rpt == 3) return launch_inv_t<T, NB, 3, 12>(a, s);
+    if constexpr (MAXRPT >= 4) if (rpt == 4) return launch_inv_t<T, NB, 4, 16>(a, s);
     return hipErrorInvalidValue;
 }
 

@@ -23,7 +23,7 @@ module negf_transport
 
     type, bind(C) :: nt_params
         real(c_double) :: eta, eta0, convergence, device_eta_factor, lambda_eta
-        integer(c_int) :: eta_mode, max_iter, precision, circular, n_bloch_x, n_bloch_y
+        integer(c_int) :: eta_mode, max_iter, precision, circular, n_bloch_x, n_bloch_y, path_mode
     end type nt_params
 
     type, bind(C) :: nt_solve_extras

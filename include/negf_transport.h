@@ -65,11 +65,14 @@ typedef struct {
     int circular;             /* &unfolding circular_flag: block-circulant lead decimation */
     int n_bloch_x;            /* &unfolding n_bloch_x: lead copies along x (inner) */
     int n_bloch_y;            /* &unfolding n_bloch_y: lead copies along y (outer) */
+    int path_mode;            /* &kpath path_mode: 1 adds the degenerate-subspace rotation of
+                                 the mode-resolved solve (negf/velocity_mat.f90:1884-1978);
+                                 any other value runs as the example's path_mode = 2 */
 } nt_params;
 
 /* Fills the negf defaults: eta = eta0 = 1e-6 (config.f90:88-89), device x10,
  * convergence 1e-12, lambda_eta 5e-3, max_iter 300, FP64, direct decimation
- * (circular = 0, n_bloch_x = n_bloch_y = 1).
+ * (circular = 0, n_bloch_x = n_bloch_y = 1), path_mode 2.
  * With circular = 1 the lead blocks (n x n) must be block-circulant over
  * n_bloch_y x n_bloch_x transverse copies (copy index y outer, x inner); each
  * lead GF is then computed as in circular_g (negf/circular.f90:43-114): a

@@ -1,9 +1,12 @@
 """CPU restatement of the mode-resolved transmission (test infrastructure only).
 
-Follows vel_mat_tdb (negf/velocity_mat.f90:1692-3150) for path_mode != 1 (the
-branch the reference's example runs: Gram-Schmidt on the propagating modes,
-no degenerate-subspace rotation), without the Brillouin-zone unfolding that
-labels modes as specular or not (:2020-2130 and its three mirrors).  The
+Follows vel_mat_tdb (negf/velocity_mat.f90:1692-3150), without the
+Brillouin-zone unfolding that labels modes as specular or not (:2020-2130 and
+its three mirrors).  path_mode != 1 is the branch the reference's example
+runs (Gram-Schmidt on the propagating modes); path_mode = 1 adds the
+degenerate-subspace rotation (:1884-1972 and mirrors): find_degenerate
+(negf/util.f90:513-587), eigenH (ZHEEV on the upper triangle, numpy.linalg.eigh
+with UPLO='U' here) and lstsq (ZGELSY, numpy.linalg.lstsq here).  The
 LAPACK calls of the reference are the LAPACK calls here: ZGEEV through
 numpy.linalg.eig (eigen, negf/util.f90:226-262) and ZGEES through
 scipy.linalg.schur (sqrtm, negf/util.f90:359-454); the triangular square root,
@@ -60,9 +63,71 @@ def sqrtm(A):
     return Q @ U @ Q.conj().T
 
 
-def _modes(F, lambda_eta, az, sign):
+def find_degenerate(lam, start, deg_all):
+    """negf/util.f90:513-587 with its 1-based indices: the first pair i < j
+    (i from start) with |lam_i - lam_j| < 1e-4 and ||lam_j| - 1| < 1e-3 gives
+    the group of every index within 1e-4 of lam_j, unless one of them was
+    already processed (then [0]).  Returns (group, next_start, found)."""
+    ne = len(lam)
+    found, itemp, edeg = 0, 0, None
+    for i in range(start, ne):
+        for j in range(i + 1, ne + 1):
+            if abs(lam[i - 1] - lam[j - 1]) < 1.0e-4 and abs(abs(lam[j - 1]) - 1.0) < 1.0e-3:
+                found, itemp, edeg = 1, i, lam[j - 1]
+            if found:
+                break
+        if found:
+            break
+    group = [0]
+    if found:
+        group = [i for i in range(1, ne + 1) if abs(lam[i - 1] - edeg) < 1.0e-4]
+        if any(g in deg_all for g in group):
+            group = [0]
+    return group, itemp + 1, found
+
+
+def degenerate_rotation(E, lam, U, prop, H00, V):
+    """The path_mode = 1 rotation of one mode set (velocity_mat.f90:1884-1978):
+    each degenerate group is re-expressed, by least squares, in the
+    eigenvectors of H(lam) = H00 + V / lam + V^+ lam (upper triangle) whose
+    eigenvalue lies within 1e-4 of E, divided by lam; then every propagating
+    column is normalised.  B has one column per group member: eigenvectors
+    within 1e-4 beyond that count are dropped, and missing columns are taken
+    from the eigenvectors next closest to E (the reference writes past, or
+    leaves uninitialised, the columns of its B in those two cases; a member
+    whose lam differs from the group's first by up to 1e-4 can sit that far
+    from E in H(lam))."""
+    n = len(lam)
+    tf = np.eye(n, dtype=complex)
+    start, found, deg_all = 1, 1, [0]
+    while found == 1:
+        group, nxt, found = find_degenerate(lam, start, deg_all)
+        if len(group) > 1:
+            deg_all = group[:] if deg_all == [0] else deg_all + group
+            cols = [g - 1 for g in group]
+            ld = lam[cols[0]]
+            Hl = H00 + V / ld + V.conj().T * ld
+            e, ev = np.linalg.eigh(Hl, UPLO="U")
+            A = U[:, cols]
+            sel = [i for i in range(n) if abs(E - e[i]) < 1.0e-4][:len(cols)]
+            rest = sorted((i for i in range(n) if i not in sel), key=lambda i: (abs(E - e[i]), i))
+            sel += rest[:len(cols) - len(sel)]
+            B = np.zeros((n, len(cols)), complex)
+            for c, i in enumerate(sel):
+                B[:, c] = ev[:, i] / ld
+            X = np.linalg.lstsq(A, B, rcond=None)[0]
+            tf[np.ix_(cols, cols)] = X
+        start = nxt
+    U = U @ tf
+    for i in prop:
+        U[:, i] = U[:, i] / np.sqrt(np.vdot(U[:, i], U[:, i]))
+    return U
+
+
+def _modes(F, lambda_eta, az, sign, rot=None):
     """eigen + propagating selection + Gram-Schmidt of one lead direction
-    (velocity_mat.f90:1839-1882 and mirrors): (lambda, U, k_z, n_prop)."""
+    (velocity_mat.f90:1839-1882 and mirrors): (lambda, U, k_z, n_prop).
+    rot = (E, H00, V) applies the path_mode = 1 rotation."""
     lam, U = np.linalg.eig(F)
     kz = np.full(lam.shape[0], NOT_PROP)
     prop = [i for i in range(lam.shape[0]) if abs(abs(lam[i]) - 1.0) < lambda_eta]
@@ -72,11 +137,14 @@ def _modes(F, lambda_eta, az, sign):
         UL = gram_schmidt(U[:, prop])
         U = U.copy()
         U[:, prop] = UL
+    if rot is not None:
+        U = degenerate_rotation(rot[0], lam, U, prop, rot[1], rot[2])
     return lam, U, kz, len(prop)
 
 
 def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, G_R,
-                Haml, Hamr, Hamlv, Hamrv, eta, eta0, lambda_eta, a_z_L, a_z_R, first_point=False, tb=False):
+                Haml, Hamr, Hamlv, Hamrv, eta, eta0, lambda_eta, a_z_L, a_z_R, first_point=False, tb=False,
+                path_mode=2):
     """Mode-resolved transmission and reflection of one (omega, k) point.
 
     tb=True follows tb/velocity_mat.f90: velocity matrices without the
@@ -94,22 +162,24 @@ def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, 
     f = (lambda a, b: 1j) if tb else (lambda a, b: 1j * 2.0 * np.sqrt(E / a / b))
 
     # left advanced, incident from the left (:1835-2012)
-    lam_al, U_al, kz_al, n_al = _modes(gl_adv @ Hamlv, lambda_eta, a_z_L, -1.0)
+    rl = (E, Haml, Hamlv) if path_mode == 1 else None
+    rr = (E, Hamr, Hamrv) if path_mode == 1 else None
+    lam_al, U_al, kz_al, n_al = _modes(gl_adv @ Hamlv, lambda_eta, a_z_L, -1.0, rl)
     V_al = -(U_al.conj().T @ gml_adv @ U_al) * sL
     inv_U_al = np.linalg.inv(U_al.conj().T)
     Vh_al = sqrtm(V_al)
     # left retarded, leaving to the left (:2131-2303)
-    lam_rl, U_rl, kz_rl, n_rl = _modes(gl_adv.conj().T @ Hamlv, lambda_eta, a_z_L, -1.0)
+    lam_rl, U_rl, kz_rl, n_rl = _modes(gl_adv.conj().T @ Hamlv, lambda_eta, a_z_L, -1.0, rl)
     V_rl = -(U_rl.conj().T @ gml_adv.conj().T @ U_rl) * sL
     inv_U_rl = np.linalg.inv(U_rl)
     Vh_rl = sqrtm(V_rl)
     # right retarded, leaving to the right (:2394-2576)
-    lam_rr, U_rr, kz_rr, n_rr = _modes(gr_ret @ Hamrv, lambda_eta, a_z_R, 1.0)
+    lam_rr, U_rr, kz_rr, n_rr = _modes(gr_ret @ Hamrv, lambda_eta, a_z_R, 1.0, rr)
     V_rr = (U_rr.conj().T @ gmr_ret @ U_rr) * sR
     inv_U_rr = np.linalg.inv(U_rr)
     Vh_rr = sqrtm(V_rr)
     # right advanced, incident from the right (:2657-2824)
-    lam_ar, U_ar, kz_ar, n_ar = _modes(gr_ret.conj().T @ Hamrv, lambda_eta, a_z_R, 1.0)
+    lam_ar, U_ar, kz_ar, n_ar = _modes(gr_ret.conj().T @ Hamrv, lambda_eta, a_z_R, 1.0, rr)
     V_ar = (U_ar.conj().T @ gmr_ret.conj().T @ U_ar) * sR
     inv_U_ar = np.linalg.inv(U_ar.conj().T)
     Vh_ar = sqrtm(V_ar)
@@ -136,7 +206,7 @@ def vel_mat_tdb(E, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, 
                 R_modes_R=float(a2(rrr).sum()), R_modes_L=float(a2(rll).sum()))
 
 
-def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=False, tb=False):
+def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=False, tb=False, path_mode=2):
     """Full oracle chain for one point: surface GFs and get_G_full_tdb of the
     C oracle, then vel_mat_tdb with the driver's argument mapping
     (negf/phinfortran.f90:1238-1250)."""
@@ -148,6 +218,6 @@ def solve_point_modes(blk, E, params, lambda_eta, a_z_L, a_z_R, first_point=Fals
               sg["sr_ret"], E, params)
     m = vel_mat_tdb(E, sg["gl_adv"], sg["gr_ret"], sg["gl_plus"], sg["gr_minus"], sg["gml_adv"], sg["gmr_ret"],
                     r["G_D_RL"], r["G_D_LR"], r["G_L"], r["G_R"], d["Haml"], d["Hamr"], d["Hamlv"], d["Hamrv"],
-                    params.eta, params.eta0, lambda_eta, a_z_L, a_z_R, first_point, tb)
+                    params.eta, params.eta0, lambda_eta, a_z_L, a_z_R, first_point, tb, path_mode)
     m["T_LR"], m["T_RL"] = r["T_LR"], r["T_RL"]
     return m

@@ -94,6 +94,7 @@ class NtParams(C.Structure):
         ("circular", C.c_int),
         ("n_bloch_x", C.c_int),
         ("n_bloch_y", C.c_int),
+        ("path_mode", C.c_int),
     ]
 
 

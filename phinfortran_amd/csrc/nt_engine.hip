@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -64,6 +65,9 @@ struct nt_ctx {
     std::vector<CircU> circ_u;
     // autotuned inverse panel width per (element bytes, n); absent: size rule
     std::map<std::pair<int, int>, int> inv_nb;
+    // degenerate-group descriptors and statuses of the path_mode = 1 rotation
+    void* deg_buf = nullptr;
+    size_t deg_bytes = 0;
 };
 
 namespace {
@@ -1956,6 +1960,151 @@ size_t modes_scratch(int nl, int nr, int P)
 
 // Unfolding request of one chunk of points (outputs already offset to it;
 // null outputs use scratch).
+// find_degenerate (negf/util.f90:513-587) with its 1-based indices: from
+// start, the first pair i < j with |l_i - l_j| < 1e-4 and ||l_j| - 1| < 1e-3
+// gives the group of every index within 1e-4 of l_j, or {0} if one of them
+// was processed before.  Returns whether a pair was found.
+bool find_degenerate(const std::vector<std::complex<double>>& lam, int start, const std::vector<int>& deg_all,
+                     std::vector<int>& group, int& next_start)
+{
+    const int ne = (int)lam.size();
+    int found = 0, itemp = 0;
+    std::complex<double> edeg;
+    for (int i = start; i <= ne - 1 && !found; ++i)
+        for (int j = i + 1; j <= ne; ++j) {
+            if (std::abs(lam[i - 1] - lam[j - 1]) < 1.0e-4 && std::abs(std::abs(lam[j - 1]) - 1.0) < 1.0e-3) {
+                found = 1;
+                itemp = i;
+                edeg = lam[j - 1];
+            }
+            if (found) break;
+        }
+    next_start = itemp + 1;
+    group.assign(1, 0);
+    if (found) {
+        group.clear();
+        for (int i = 1; i <= ne; ++i)
+            if (std::abs(lam[i - 1] - edeg) < 1.0e-4) group.push_back(i);
+        for (int g : group)
+            for (int d : deg_all)
+                if (g == d) {
+                    group.assign(1, 0);
+                    return true;
+                }
+    }
+    return found != 0;
+}
+
+// The path_mode = 1 rotation of one mode set (negf/velocity_mat.f90:1884-1978):
+// groups found on the host from the eigenvalues w (m per point), then per
+// group the Hermitian H = H00 + V / lam + V^+ lam (upper triangle), its Schur
+// (eigen) decomposition, the least-squares coefficients and the rotation of
+// U's group columns (nt_degen.hip); finally every propagating column of U is
+// normalised.  Uses three scratch slots of mm x P complex from off.
+void degenerate_rotation(nt_ctx* c, int m, int P, long mm, const double* E, BMat<double> H00, BMat<double> V,
+                         const double* w, double* U, const double* kz, int* status, Arena& ar, size_t off)
+{
+    typedef double T;
+    hipStream_t s = c->stream;
+    std::vector<double> hw(2 * (size_t)m * P);
+    check_range(c, w, sizeof(T) * hw.size(), "rotation eigenvalues");
+    hipcheck(c, hipMemcpyAsync(hw.data(), w, sizeof(T) * hw.size(), hipMemcpyDeviceToHost, s), "D2H");
+    hipcheck(c, hipStreamSynchronize(s), "sync");
+    std::vector<DegGroup> groups;
+    std::vector<std::complex<double>> lam(m);
+    std::vector<int> group, deg_all;
+    for (int p = 0; p < P; ++p) {
+        for (int i = 0; i < m; ++i) lam[i] = {hw[2 * ((size_t)p * m + i)], hw[2 * ((size_t)p * m + i) + 1]};
+        deg_all.assign(1, 0);
+        bool first = true;
+        int start = 1, next = 1;
+        while (find_degenerate(lam, start, deg_all, group, next)) {
+            if (group.size() > 1) {
+                if (group.size() > (size_t)DEG_MAX) {
+                    c->err = "more than 16 degenerate modes in one group (path_mode = 1)";
+                    throw Fail{NT_ERR_ARG};
+                }
+                if (first) deg_all.clear();
+                first = false;
+                deg_all.insert(deg_all.end(), group.begin(), group.end());
+                DegGroup g{};
+                g.p = p;
+                g.ndeg = (int)group.size();
+                for (size_t k = 0; k < group.size(); ++k) g.idx[k] = group[k] - 1;
+                g.lr = lam[group[0] - 1].real();
+                g.li = lam[group[0] - 1].imag();
+                groups.push_back(g);
+            }
+            start = next;
+        }
+    }
+    const int G = (int)groups.size();
+    if (G > 0) {
+        const size_t need = (sizeof(DegGroup) + sizeof(int)) * (size_t)std::min(G, P);
+        if (c->deg_bytes < need) {
+            if (c->deg_buf) {
+                hipcheck(c, hipStreamSynchronize(s), "sync");
+                unreg_region(c, c->deg_buf);
+                hipcheck(c, hipFree(c->deg_buf), "hipFree");
+                c->deg_buf = nullptr;
+                c->deg_bytes = 0;
+            }
+            hipcheck(c, hipMalloc(&c->deg_buf, need), "hipMalloc rotation groups");
+            c->deg_bytes = need;
+            reg_region(c, c->deg_buf, need);
+        }
+        Arena sa = ar;
+        sa.off = off;
+        T* Hb = sa.ptr<T>(sa.take(sizeof(T) * 2 * mm * P));
+        T* Zb = sa.ptr<T>(sa.take(sizeof(T) * 2 * mm * P));
+        T* Xb = sa.ptr<T>(sa.take(sizeof(T) * 2 * mm * P));
+        DegGroup* gd = static_cast<DegGroup*>(c->deg_buf);
+        int* gst = reinterpret_cast<int*>(gd + std::min(G, P));
+        for (int g0 = 0; g0 < G; g0 += P) {
+            const int Gc = std::min(P, G - g0);
+            hipcheck(c, hipMemcpyAsync(gd, groups.data() + g0, sizeof(DegGroup) * Gc, hipMemcpyHostToDevice, s),
+                     "H2D groups");
+            hipcheck(c, launch_fill_int(gst, 0, Gc, s), "zero group status");
+            DegArgs da;
+            da.m = m;
+            da.G = Gc;
+            da.H00 = H00;
+            da.V = V;
+            da.g = gd;
+            da.E = E;
+            da.H = Hb;
+            da.Z = Zb;
+            da.hstride = mm;
+            da.U = U;
+            da.ustride = mm;
+            da.gstatus = gst;
+            da.status = status;
+            audit<T>(c, H00, m, m, P, 1, "rotation H00");
+            audit<T>(c, V, m, m, P, 1, "rotation V");
+            audit<T>(c, bw(Hb, mm, m), m, m, Gc, 1, "rotation H");
+            audit<T>(c, bw(U, mm, m), m, m, P, 1, "rotation U");
+            check_range(c, E, sizeof(T) * P, "rotation E");
+            check_range(c, status, sizeof(int) * P, "rotation status");
+            launch(c, NT_K_EIG, 0.0, "rotation build", [&] { return launch_deg_build(da, s); });
+            EigArgs e;
+            e.n = m;
+            e.A = bw(Hb, mm, m);
+            e.Z = bw(Zb, mm, m);
+            e.X = bw(Xb, mm, m);
+            e.V = nonew<T>();
+            e.w = nullptr;
+            e.status = gst;
+            e.batch = Gc;
+            launch(c, NT_K_EIG, 0.0, "rotation schur", [&] { return launch_schur(e, s); });
+            launch(c, NT_K_EIG, 0.0, "rotation solve", [&] { return launch_deg_solve(da, s); });
+            hipcheck(c, hipStreamSynchronize(s), "sync");     // groups vector reused by the next chunk
+        }
+    }
+    audit<T>(c, bw(U, mm, m), m, m, P, 1, "rotation U");
+    check_range(c, kz, sizeof(T) * (size_t)m * P, "rotation kz");
+    launch(c, NT_K_ELEM, 0.0, "rotation normalize", [&] { return launch_deg_normalize(m, U, mm, kz, P, s); });
+}
+
 struct UnfoldRun {
     const nt_unfold_desc* d;
     int* bz[4];          // L_adv, L_ret, R_ret, R_adv
@@ -2045,6 +2194,8 @@ void modes_run(nt_ctx* c, const nt_params* prm, int tb, int nl, int nr, int P, c
         check_range(c, kz, sizeof(double) * (size_t)m * P, "modes kz");
         check_range(c, o.nprop[d], sizeof(int) * P, "modes nprop");
         launch(c, NT_K_EIG, 0.0, "modes select", [&] { return launch_modes_select(ma, s); });
+        if (prm->path_mode == 1)
+            degenerate_rotation(c, m, P, mm, E, d < 2 ? Haml : Hamr, D.V, w[d], U[d], kz, status, ar, eig_off);
         // V = s (U^+ Gamma' U)
         gemm<T>(c, m, m, m, OP_C, R(U[d], m), D.opgam, D.gam, W(T1, m), 1.0, none<T>(), 0.0, nullptr, P);
         gemm<T>(c, m, m, m, OP_N, R(T1, m), OP_N, R(U[d], m), W(Vw, m), 1.0, none<T>(), 0.0, nullptr, P);
@@ -2260,6 +2411,7 @@ void nt_params_default(nt_params* p)
     p->circular = 0;
     p->n_bloch_x = 1;
     p->n_bloch_y = 1;
+    p->path_mode = 2;
 }
 
 int nt_create(nt_ctx** out, int device_ordinal)
@@ -2296,6 +2448,7 @@ int nt_destroy(nt_ctx* c)
     for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->ws) (void)hipFree(c->ws);
     for (auto& u : c->circ_u) (void)hipFree(u.d);
+    if (c->deg_buf) (void)hipFree(c->deg_buf);
     if (c->host_flag) (void)hipHostFree(c->host_flag);
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->aux) (void)hipStreamDestroy(c->aux);

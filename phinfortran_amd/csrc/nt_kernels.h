@@ -156,6 +156,31 @@ hipError_t launch_hk_assemble(int nk, const double* kc, int nseg, const int* seg
                               const double* coef, const double* dr, int rows, int cols, double* out, int ld,
                               long kstride, hipStream_t s);
 
+// Degenerate-subspace rotation of the mode-resolved solve (nt_degen.hip).
+constexpr int DEG_MAX = 16;       // modes in one degenerate group
+struct DegGroup {
+    int p, ndeg;                  // point, group size
+    int idx[DEG_MAX];             // 0-based mode indices, ascending
+    double lr, li;                // eigenvalue of the group's first mode
+};
+struct DegArgs {
+    int m, G;                     // mode-set size, groups
+    BMat<double> H00, V;          // lead blocks of each point
+    const DegGroup* g;
+    const double* E;              // per point
+    double* H;                    // per group m x m (stride hstride): built matrix, then its Schur form
+    const double* Z;              // per group Schur vectors (stride hstride)
+    long hstride;
+    double* U;                    // per point m x m mode eigenvectors (stride ustride), rotated in place
+    long ustride;
+    const int* gstatus;           // per group Schur status
+    int* status;                  // per point
+};
+hipError_t launch_deg_build(const DegArgs& a, hipStream_t s);
+hipError_t launch_deg_solve(const DegArgs& a, hipStream_t s);
+// every column with kz < 99.9 normalised (per point m x m, stride ustride, kz m per point)
+hipError_t launch_deg_normalize(int m, double* U, long ustride, const double* kz, int P, hipStream_t s);
+
 // Buttiker probes (nt_probe.hip).
 // zl[i][p] = zd[p] - sigma_w(i, E[p]) (zeroed sigma when !lifetime), gam / dgam
 // = -2 Im sigma_w, -2 Im dsigma_w/dT; layer-major L x P.

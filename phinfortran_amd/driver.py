@@ -39,13 +39,19 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
     solve (probes at the temperatures it used), as the reference writes them
     after its loop.  The per-mode files, which the reference appends in every
     iteration, are written once, for that last solve.  The result adds
-    'buttiker' (phinfortran_amd.buttiker.run's dict)."""
+    'buttiker' (phinfortran_amd.buttiker.run's dict).
+
+    path_mode = 1 (eigenvalue mode) takes the k-points of kpath_file, runs
+    the mode-resolved solve with the degenerate-subspace rotation and writes
+    the per-mode files only, as the reference (phinfortran.f90:1524)."""
     import torch
     from . import transport as T
     cfg = Q.read_config(config_path)
     qe = Q.read_qe_fc(cfg, fc_path, mass_profile)
     asm = Q.TdbAssembler(qe, cfg)
-    ks = Q.k_grid(cfg)
+    ks = Q.k_grid(cfg, os.path.dirname(os.path.abspath(config_path)))
+    if cfg["path_mode"] == 1:
+        modes = True          # eigenvalue mode: the per-mode files are its output
     E = Q.e_grid(cfg)
     eng = engine or T.Engine(0)
     d = asm.upload(eng, ks)
@@ -53,7 +59,7 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
     kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)
     mask = np.tile((np.arange(ne) == 0).astype(np.int32), nk)
     p = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500,
-                 precision=precision)
+                 precision=precision, path_mode=cfg["path_mode"])
     out = {}
     probe = None
     if cfg["buttiker"] == 1:
@@ -93,7 +99,7 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
     tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
     tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
     st = r["status"].cpu().numpy().reshape(nk, ne).T
-    if output_dir:
+    if output_dir and cfg["path_mode"] != 1:     # phinfortran.f90:1524
         OUT.write_trans(output_dir, np.repeat(E[:, None], nk, axis=1), tl, tr,
                         **{k: out[k] for k in out if k not in ("modes_status", "buttiker")})
     if engine is None:

@@ -56,7 +56,9 @@ def read_config(path):
                 t_left=g("t_left") if "t_left" in vals else 300.0,
                 t_right=g("t_right") if "t_right" in vals else 300.0,
                 B_probe=g("B_probe") if "B_probe" in vals else 5.0e-20 * (2.0 * 3.1415926 * 1.0e12),
-                T_probe=g("T_probe") if "T_probe" in vals else 430.0)
+                T_probe=g("T_probe") if "T_probe" in vals else 430.0,
+                # &kpath (negf/config.f90:64-68): path_mode 2 = full k mesh
+                path_mode=int(vals.get("path_mode", 2)), kpath_file=vals.get("kpath_file"))
 
 
 def read_qe_fc(cfg, path, mass_profile=None):
@@ -354,8 +356,20 @@ def pc_dynamics_table(qe, natoms_pc, mass):
             np.array(drs, float).reshape(-1, 3))
 
 
-def k_grid(cfg):
-    """negf/config.f90:156-175 (path_mode = 2)."""
+def k_grid(cfg, base_dir=None):
+    """negf/config.f90:156-190: the centred mesh for path_mode = 2, else the
+    points of kpath_file (first line: their number; then one point per line,
+    crystal coordinates) with the transport component set to 0
+    (config.f90:118-127, 186-187); base_dir resolves a relative kpath_file."""
+    if cfg.get("path_mode", 2) != 2:
+        path = cfg["kpath_file"]
+        if base_dir and not os.path.isabs(path):
+            path = os.path.join(base_dir, path)
+        lines = [ln.split() for ln in open(path) if ln.strip()]
+        n = int(lines[0][0])
+        ks = np.array([[float(x) for x in lines[1 + i][:3]] for i in range(n)])
+        ks[:, 2] = 0.0
+        return ks
     n1, n2 = cfg["nk1"], cfg["nk2"]
     ks = []
     for i in range(n1):

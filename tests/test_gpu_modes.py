@@ -145,3 +145,66 @@ def test_tb_ribbon_vs_oracle(engine, T):
         ref = (np.abs(m["trl"][np.ix_(out, inc)]) ** 2).sum()
         assert abs(sums[i][0] - ref) < 1e-6 * max(1.0, ref), (i, sums[i][0], ref)
         assert abs(sums[i][0] - r["T_RL"][i].item()) < 1e-4 * max(1.0, ref)
+
+
+def _set_sums(kz, tm, tol=1e-4):
+    """(k_z, summed per-mode transmission) over sets of incident modes whose
+    k_z agree to tol (a degenerate set: eigenvalues within 1e-4 -- or a
+    single mode), sorted by k_z"""
+    inc = np.where(kz < 99.9)[0]
+    order = inc[np.argsort(kz[inc])]
+    out, cur = [], None
+    for i in order:
+        if cur is not None and abs(kz[i] - cur[0]) < tol:
+            cur[1] += tm[i]
+            cur[2] += 1
+        else:
+            if cur is not None:
+                out.append(cur)
+            cur = [kz[i], tm[i], 1]
+    if cur is not None:
+        out.append(cur)
+    return out
+
+
+def test_degenerate_rotation_vs_oracle(engine, T):
+    """path_mode = 1 (degenerate-subspace rotation) at the si-si Gamma point,
+    whose leads carry degenerate transverse modes: k_z, the transmission and
+    reflection totals and the per-mode transmission summed over every set of
+    incident modes sharing one k_z against the oracle's path_mode = 1 (the
+    basis inside a degenerate set follows the eigensolver, here and in the
+    reference), and the same against the GPU's own path_mode = 2."""
+    cfg = S.read_config()
+    qe = S.read_qe_fc(cfg)
+    asm = S.TdbAssembler(qe, cfg)
+    ks, E = S.k_grid(cfg), S.e_grid(cfg)
+    az = qe["cell"][2, 2]
+    blk = asm.blocks(ks[24])
+    d = engine.upload_blocks(blk)
+    ies = np.arange(1, len(E), 3)
+    p1 = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500,
+                  lambda_eta=5e-3, path_mode=1)
+    p2 = T.params("phonon", eta=cfg["eta"], eta0=cfg["eta0"], convergence=cfg["convergence"], max_iter=500,
+                  lambda_eta=5e-3)
+    op = oparams(p1)
+    print("modes path_mode 1", flush=True)
+    r1 = engine.solve_modes(d, E[ies], p1, az, az, want=())
+    r2 = engine.solve_modes(d, E[ies], p2, az, az, want=())
+    assert (r1["modes_status"].cpu().numpy() == 0).all()
+    kz1, tm1, s1 = (r1[k].cpu().numpy() for k in ("k_z_L_adv", "t_mode_L", "sums"))
+    kz2, tm2, s2 = (r2[k].cpu().numpy() for k in ("k_z_L_adv", "t_mode_L", "sums"))
+    ndeg = 0
+    for q, ie in enumerate(ies):
+        m = M.solve_point_modes(blk, E[ie], op, 5e-3, az, az, path_mode=1)
+        ref = [m["T_modes_RL"], m["T_modes_LR"], m["R_modes_R"], m["R_modes_L"]]
+        assert np.abs(s1[q] - ref).max() < 1e-6 * max(1.0, max(ref)), (ie, s1[q], ref)
+        assert np.abs(s1[q] - s2[q]).max() < 1e-8 * max(1.0, s2[q].max()), ie
+        a = _set_sums(kz1[q], tm1[q])
+        b = _set_sums(m["k_z_L_adv"], m["t_mode_L"])
+        c = _set_sums(kz2[q], tm2[q])
+        assert [x[2] for x in a] == [x[2] for x in b] == [x[2] for x in c], ie
+        for x, y, z in zip(a, b, c):
+            assert abs(x[0] - y[0]) < 1e-7 and abs(x[1] - y[1]) < 2e-6 and abs(x[1] - z[1]) < 1e-8, (ie, x, y, z)
+        ndeg += sum(1 for x in a if x[2] > 1)
+    print(f"path_mode 1: {ndeg} degenerate incident sets over {len(ies)} energies")
+    assert ndeg >= 1

@@ -122,3 +122,24 @@ def test_driver_buttiker_run(engine, tmp_path):
     gold = np.loadtxt(os.path.join(S.FIXTURE, "transl_tot.dat"))
     assert mine.shape == gold.shape and np.isfinite(mine).all()
     assert np.abs(mine[:, 1:] - gold[:, 1:]).max() > 1e-6       # the probes scatter
+
+
+def test_driver_eigenvalue_mode(engine, tmp_path):
+    """path_mode = 1 on the si-si inputs along a two-point k path: the per-mode
+    files of both k-points and no transmission tables (phinfortran.f90:1524);
+    the mode-resolved totals equal the path_mode = 2 run's at the same k."""
+    from phinfortran_amd import driver
+    txt = open(os.path.join(S.FIXTURE, "config")).read().replace("path_mode = 2", "path_mode = 1")
+    txt = txt.replace('kpath_file = "kpoint.dat_LXKnew"', 'kpath_file = "kp.dat"')
+    cfgp = os.path.join(tmp_path, "config")
+    open(cfgp, "w").write(txt)
+    (tmp_path / "kp.dat").write_text("2\n0.0 0.0 0.0\n0.142857142857 0.0 0.0\n")
+    print("driver eigenvalue mode", flush=True)
+    out = driver.run_qe_tdb(cfgp, os.path.join(S.FIXTURE, "Si444.fc"), str(tmp_path), engine=engine)
+    assert (out["modes_status"] == 0).all() and out["k"].shape == (2, 3)
+    assert not os.path.exists(os.path.join(tmp_path, "transl_tot.dat"))
+    assert os.path.exists(os.path.join(tmp_path, "trl_specular_param_000001.dat"))
+    assert os.path.exists(os.path.join(tmp_path, "trl_specular_param_000002.dat"))
+    # the total transmission of the Gamma point equals the mesh run's (k index 24)
+    ref = driver.run_qe_tdb(os.path.join(S.FIXTURE, "config"), os.path.join(S.FIXTURE, "Si444.fc"), engine=engine)
+    assert np.abs(out["transl"][:, 0] - ref["transl"][:, 24]).max() < 1e-12

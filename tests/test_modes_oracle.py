@@ -138,3 +138,76 @@ def test_tb_chain_one_mode(E):
     kz = m["k_z_L_adv"][m["k_z_L_adv"] < 99]
     assert abs(abs(kz[0]) - math.acos(-E / 2)) < 1e-8
     assert abs(abs(m["trl"][0, 0]) ** 2 - 1) < 1e-6 and abs(m["T_RL"] - 1) < 1e-6
+
+
+def test_find_degenerate_groups():
+    """find_degenerate's scan (negf/util.f90:513-587): the first qualifying
+    pair from start, every index within 1e-4 of its second member, groups
+    already processed skipped, and the driver loop of vel_mat_tdb over it."""
+    z = np.exp(0.3j)
+    lam = np.array([1.0 + 0.5j, 0.5, z, z + 5e-5, z + 2e-5, 2.0, np.exp(1.0j), np.exp(1.0j) + 1e-6])
+    g, nxt, found = M.find_degenerate(lam, 1, [0])
+    assert found and g == [3, 4, 5] and nxt == 4
+    g, nxt, found = M.find_degenerate(lam, 4, [3, 4, 5])
+    assert found and g == [0] and nxt == 5
+    g, nxt, found = M.find_degenerate(lam, 5, [3, 4, 5])
+    assert found and g == [7, 8] and nxt == 8
+    g, nxt, found = M.find_degenerate(lam, 8, [3, 4, 5, 7, 8])
+    assert not found and g == [0]
+    # a pair off the unit circle is not degenerate in this sense
+    g, _, found = M.find_degenerate(np.array([0.5, 0.5 + 1e-6]), 1, [0])
+    assert not found
+
+
+def _groups(lam):
+    out, start, deg_all = [], 1, [0]
+    while True:
+        g, nxt, found = M.find_degenerate(lam, start, deg_all)
+        if not found:
+            return out
+        if len(g) > 1:
+            deg_all = g[:] if deg_all == [0] else deg_all + g
+            out.append([i - 1 for i in g])
+        start = nxt
+
+
+def test_degenerate_rotation_si_si_gamma(si):
+    """path_mode = 1 at the Gamma point of the si-si example, where the leads'
+    transverse symmetry makes propagating modes degenerate: every rotated
+    column of a degenerate group is a unit eigenvector of H(lam) = H00 +
+    V / lam + V^+ lam with eigenvalue within 1e-4 of E, the group stays
+    orthonormal, and the basis-independent results (k_z, counts, the
+    transmission summed over each group, totals) equal path_mode = 2's."""
+    cfg, asm, ks, E, az = si
+    blk = asm.blocks(ks[24])
+    d = blk
+    p = O.OrParams(cfg["eta"], cfg["eta0"], cfg["convergence"], 10.0, O.ETA_RELATIVE, 500)
+    ngroups = 0
+    for ie in range(1, len(E), 3):
+        m2 = M.solve_point_modes(blk, E[ie], p, 5e-3, az, az, path_mode=2)
+        m1 = M.solve_point_modes(blk, E[ie], p, 5e-3, az, az, path_mode=1)
+        for key in ("k_z_L_adv", "k_z_L_ret", "k_z_R_ret", "k_z_R_adv"):
+            assert np.array_equal(m1[key], m2[key])
+        for key in ("T_modes_RL", "T_modes_LR", "R_modes_R", "R_modes_L"):
+            assert abs(m1[key] - m2[key]) <= 1e-8 * max(1.0, abs(m2[key])), (ie, key)
+        for groups, U, lam, H0, V in ((_groups(m1["lam_L_adv"]), m1["U_L_adv"], m1["lam_L_adv"], d["Haml"],
+                                       d["Hamlv"]),
+                                      (_groups(m1["lam_R_ret"]), m1["U_R_ret"], m1["lam_R_ret"], d["Hamr"],
+                                       d["Hamrv"])):
+            for g in groups:
+                ngroups += 1
+                ld = lam[g[0]]
+                H = H0 + V / ld + V.conj().T * ld
+                H = np.triu(H) + np.triu(H, 1).conj().T
+                Ug = U[:, g]
+                assert np.abs(Ug.conj().T @ Ug - np.eye(len(g))).max() < 1e-6, (ie, g)
+                assert np.abs(H @ Ug - E[ie] * Ug).max() < 2e-4 * max(1.0, np.abs(H).max()), (ie, g)
+        # per-mode transmission summed over each incident group is basis independent
+        inc = m1["k_z_L_adv"] < 99.9
+        out = m1["k_z_R_ret"] < 99.9
+        t1 = (np.abs(m1["trl"][out]) ** 2).sum(axis=0)
+        t2 = (np.abs(m2["trl"][out]) ** 2).sum(axis=0)
+        for g in _groups(m1["lam_L_adv"]):
+            if inc[g].all():
+                assert abs(t1[g].sum() - t2[g].sum()) < 1e-6 * max(1.0, t2[g].sum()), (ie, g)
+    assert ngroups >= 4

@@ -34,3 +34,15 @@ def test_segment_tables_match_host_blocks():
         assert np.abs(emulate(tabs[name], kc) - ref[name]).max() < 1e-12 * np.abs(ref[name]).max(), name
     for il in range(asm.nlayer):
         assert np.abs(emulate(tabs[f"Hamd{il}"], kc) - ref["Hamd"][il]).max() < 1e-12 * np.abs(ref["Hamd"]).max()
+
+
+def test_k_path_points(tmp_path):
+    """path_mode != 2 reads kpath_file (count, then crystal coordinates) and
+    zeroes the transport component (negf/config.f90:118-127, 186-187)"""
+    from phinfortran_amd import qe as Q
+    (tmp_path / "kp.dat").write_text("3\n0.0 0.0 0.3\n0.25 0.0 0.0\n0.1 -0.2 0.5\n")
+    cfg = dict(path_mode=1, kpath_file="kp.dat", nk1=7, nk2=7)
+    ks = Q.k_grid(cfg, str(tmp_path))
+    assert ks.tolist() == [[0.0, 0.0, 0.0], [0.25, 0.0, 0.0], [0.1, -0.2, 0.0]]
+    cfg["path_mode"] = 2
+    assert Q.k_grid(cfg).shape == (49, 3)

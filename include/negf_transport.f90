@@ -235,6 +235,17 @@ module negf_transport
             type(nt_solve_extras), intent(in) :: extras
         end function
 
+        ! gen_ham (tb/hamiltonian.f90:444-519): H(k) of nk k-points from a
+        ! phase-sum table of norb_i x norb_j sub-blocks; all pointers device memory
+        integer(c_int) function nt_hk_assemble_orb_dev(ctx, nk, kcart, nseg, seg_off, seg_geom, nent, cstride, &
+                coef, dr, rows, cols, out, ld, kstride, hermitize) bind(C, name="nt_hk_assemble_orb_dev")
+            import :: c_int, c_ptr, c_long_long
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nk, nseg, nent, cstride, rows, cols, ld, hermitize
+            integer(c_long_long), value :: kstride
+            type(c_ptr), value :: kcart, seg_off, seg_geom, coef, dr, out
+        end function
+
         function nt_status_string(status) bind(C, name="nt_status_string")
             import :: c_int, c_ptr
             integer(c_int), value :: status

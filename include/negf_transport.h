@@ -465,6 +465,21 @@ int nt_hk_assemble_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const
                        int nent, const double* coef, const double* dr, int rows, int cols, nt_c128* out, int ld,
                        long long kstride);
 
+/* gen_ham (tb/hamiltonian.f90:444-519) on the GPU: the same phase sum with
+ * sub-blocks of any size (norb_i x norb_j orbitals per atom pair) for nk
+ * k-points.  Segment s is the sub-block at orbital row seg_geom[4s], column
+ * seg_geom[4s+1] of seg_geom[4s+2] x seg_geom[4s+3] elements; its entries
+ * seg_off[s] .. seg_off[s+1]-1 (one per cell image within r_cutoff, in the
+ * reference's image order) hold the real coefficients row-major at
+ * coef + q * cstride (onsite or V_sk, tb/hamiltonian.f90:57-417) and the
+ * bond vector dr[3 q] (Angstrom).  For every k (kcart: nk x 3, 1/Angstrom,
+ * = kpoint . recivec) out[k] = sum coef exp(i k.dr); hermitize = 1 then
+ * replaces it by (H + H^+)/2 (hermitian, tb/util.f90:709-718; rows = cols).
+ * All pointers are device memory. */
+int nt_hk_assemble_orb_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const int* seg_off,
+                           const int* seg_geom, int nent, int cstride, const double* coef, const double* dr, int rows,
+                           int cols, nt_c128* out, int ld, long long kstride, int hermitize);
+
 /* ------------------------------------------------ autotuning
  * The batched inverse runs blocked Gauss-Jordan in one workgroup per matrix
  * with a panel width of 8, 16 or 32 columns, or, as "width" 1 (n >= 64), with

@@ -33,7 +33,7 @@ EXPORTS = [
     "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
-    "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
+    "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_hk_assemble_orb_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
     "nt_set_overlap", "nt_set_gemm_3m",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
@@ -152,6 +152,7 @@ def lib():
         C.POINTER(NtProbeOut)]
     L.nt_solve_tdb_multi_dev.argtypes = [P, pp, I, P]
     L.nt_hk_assemble_dev.argtypes = [P, I, P, I, P, P, I, P, P, I, I, P, I, C.c_longlong]
+    L.nt_hk_assemble_orb_dev.argtypes = [P, I, P, I, P, P, I, I, P, P, I, I, P, I, C.c_longlong, I]
     L.nt_get_ldos.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, I, P, P]
     L.nt_modes_tdb_batched_dev.argtypes = [P, pp, I, I, I, I, P, P, I] + [P] * 14 + [D, D, P,
                                                                               C.POINTER(NtModesOut), P]

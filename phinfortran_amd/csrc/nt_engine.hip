@@ -3616,6 +3616,52 @@ int nt_hk_assemble_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const i
     });
 }
 
+int nt_hk_assemble_orb_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const int* seg_off,
+                           const int* seg_geom, int nent, int cstride, const double* coef, const double* dr, int rows,
+                           int cols, nt_c128* out, int ld, long long kstride, int hermitize)
+{
+    if (!c || nk < 0 || nseg < 0 || nent < 0 || cstride <= 0 || rows <= 0 || cols <= 0 || ld < rows || !out ||
+        (hermitize && rows != cols) || (nk > 1 && kstride < (long long)ld * cols) ||
+        (nseg > 0 && (!seg_off || !seg_geom || !coef || !dr)) || (nk > 0 && !kcart))
+        return NT_ERR_ARG;
+    if (nk == 0) return NT_OK;
+    return guarded(c, [&] {
+        // host check of the segment table: offsets monotone within [0, nent],
+        // every sub-block inside the block and no larger than cstride
+        if (nseg > 0) {
+            std::vector<int> off(nseg + 1), g(4 * (size_t)nseg);
+            hipcheck(c, hipMemcpy(off.data(), seg_off, sizeof(int) * (nseg + 1), hipMemcpyDeviceToHost), "D2H");
+            hipcheck(c, hipMemcpy(g.data(), seg_geom, sizeof(int) * 4 * nseg, hipMemcpyDeviceToHost), "D2H");
+            if (off[0] != 0 || off[nseg] != nent) {
+                c->err = "hk: segment offsets must run from 0 to nent";
+                return (int)NT_ERR_ARG;
+            }
+            for (int q = 0; q < nseg; ++q) {
+                const int r0 = g[4 * q], c0 = g[4 * q + 1], ni = g[4 * q + 2], nj = g[4 * q + 3];
+                if (off[q + 1] < off[q] || r0 < 0 || c0 < 0 || ni < 1 || nj < 1 || (long)ni * nj > cstride ||
+                    r0 + ni > rows || c0 + nj > cols) {
+                    c->err = "hk: segment outside the block";
+                    return (int)NT_ERR_ARG;
+                }
+            }
+        }
+        const size_t z = sizeof(nt_c128);
+        reg_region(c, kcart, sizeof(double) * 3 * nk);
+        reg_region(c, seg_off, sizeof(int) * (nseg + 1));
+        reg_region(c, seg_geom, sizeof(int) * 4 * nseg);
+        reg_region(c, coef, sizeof(double) * (size_t)cstride * nent);
+        reg_region(c, dr, sizeof(double) * 3 * (size_t)nent);
+        const size_t span = z * ((size_t)(nk - 1) * kstride + (size_t)ld * (cols - 1) + rows);
+        reg_region(c, out, span);
+        check_range(c, kcart, sizeof(double) * 3 * nk, "hk k");
+        launch(c, NT_K_ELEM, 0.0, "hk assemble orb", [&] {
+            return launch_hk_assemble_orb(nk, kcart, nseg, seg_off, seg_geom, cstride, coef, dr, rows, cols,
+                                          reinterpret_cast<double*>(out), ld, (long)kstride, hermitize, c->stream);
+        });
+        return 0;
+    });
+}
+
 int nt_zgeev_batched_dev(nt_ctx* c, int n, const nt_c128* A, long long strideA, nt_c128* w, nt_c128* V,
                          long long strideV, int* status, int batch)
 {

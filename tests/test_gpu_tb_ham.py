@@ -1,0 +1,80 @@
+"""gen_ham (tb/hamiltonian.f90:444-519) on the GPU: H(k) assembled by
+nt_hk_assemble_orb_dev from the host table equals the oracle's gen_ham on
+bulk Si, the Si/Ge superlattice of the reference's example at its full size
+(128 atoms, 1280 orbitals) and a cell whose atom sees its own images; the
+unhermitised sum equals the table's phase sum; malformed tables are refused
+on the host."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from oracle import tb_ham as TB
+from phinfortran_amd import tb
+from tests import tb_structures as S
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+PARAM = os.path.join(HERE, "golden", "tb", "param")
+
+
+@pytest.fixture(scope="module")
+def P():
+    return tb.read_param(PARAM)
+
+
+def rowmajor(H):
+    return np.swapaxes(H.cpu().numpy(), -1, -2)
+
+
+@pytest.mark.parametrize("name", ["bulk", "chain", "superlattice"])
+def test_gpu_gen_ham_matches_oracle(engine, P, name):
+    lat, pos, sp = {"bulk": S.si_bulk, "chain": S.short_chain, "superlattice": S.si_ge_superlattice}[name]()
+    A = tb.TbAssembler(P, lat, pos, sp, 2.6)
+    ks = np.array([[0.0, 0.0, 0.0], [0.13, -0.27, 0.41], [0.5, 0.5, 0.0]])
+    print("tb hk", name, A.norb, flush=True)
+    H = rowmajor(A.assemble(engine, ks))
+    for j in ([0, 1] if name == "superlattice" else range(len(ks))):
+        want = TB.gen_ham(P, pos, sp, lat, ks[j], 2.6)
+        assert np.abs(H[j] - want).max() <= 1e-13 * np.abs(want).max(), (name, j)
+    assert np.abs(H - np.swapaxes(H, -1, -2).conj()).max() == 0.0
+
+
+def test_gpu_unhermitised_sum(engine, P):
+    lat, pos, sp = S.si_ge_superlattice(5.5, 1, 1, 4)
+    A = tb.TbAssembler(P, lat, pos, sp, 2.6)
+    k = np.array([[0.21, 0.05, -0.33]])
+    H = rowmajor(A.assemble(engine, k, hermitize=False))[0]
+    kc = A.kcart(k)[0]
+    want = np.zeros_like(H)
+    g = A.seg_geom.reshape(-1, 4)
+    for s in range(A.nseg):
+        r0, c0, ni, nj = g[s]
+        for q in range(A.seg_off[s], A.seg_off[s + 1]):
+            want[r0:r0 + ni, c0:c0 + nj] += A.coef[q, :ni * nj].reshape(ni, nj) * np.exp(1j * (A.dr[q] @ kc))
+    assert np.abs(H - want).max() <= 1e-13 * np.abs(want).max()
+
+
+def test_gpu_malformed_tables_refused(engine, P):
+    import torch
+    lat, pos, sp = S.si_bulk()
+    A = tb.TbAssembler(P, lat, pos, sp, 2.6)
+    dev = torch.device("cuda", engine.device)
+    seg_off, seg_geom, coef, dr = A._device_tables(torch, dev)
+    kc = torch.zeros((1, 3), dtype=torch.float64, device=dev)
+    N = A.norb
+    out = torch.zeros((1, N, N), dtype=torch.complex128, device=dev)
+    p = lambda x: C.c_void_p(x.data_ptr())
+    call = lambda geom, rows, cols, herm: engine._L.nt_hk_assemble_orb_dev(
+        engine.ctx, 1, p(kc), A.nseg, p(seg_off), p(geom), A.nent, A.cstride, p(coef), p(dr), rows, cols, p(out),
+        rows, rows * cols, herm)
+    assert call(seg_geom, N, N, 1) == 0
+    assert call(seg_geom, N, N - 10, 1) == -3              # hermitise needs a square block
+    bad = seg_geom.clone()
+    bad[-4] = N - 5                                          # last sub-block runs past the last row
+    assert call(bad, N, N, 1) == -3
+    bad = seg_geom.clone()
+    bad[2] = 11                                              # 11 x 10 > cstride
+    assert call(bad, N, N, 1) == -3
+    torch.cuda.synchronize()

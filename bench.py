@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--buttiker-cpu-sample", type=int, default=1, help="points of the probe CPU sample (0 = skip)")
     ap.add_argument("--no-tb-hk", dest="tb_hk", action="store_false",
                     help="skip the tight-binding H(k) assembly measurement")
+    ap.add_argument("--tb-hk-only", action="store_true",
+                    help="measure only the tight-binding H(k) assembly (for kernel profiles)")
     ap.add_argument("--tb-hk-k", type=int, default=64, help="k-points per tight-binding H(k) launch")
     ap.add_argument("--modes-cpu-sample", type=int, default=6,
                     help="points of the mode-resolved CPU baseline sample (0 = skip)")
@@ -116,6 +118,70 @@ def main():
         torch.cuda.synchronize()
         log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
         del src, dst
+    def run_tb_hk():
+        """gen_ham of the reference's tight-binding example (Si/Ge superlattice
+        2 x 2 x 8 four-atom cells: 128 atoms, sp3d5s*, 1280 orbitals) for
+        tb_hk_k k-points per launch; the oracle's gen_ham on one k-point
+        beside it"""
+        from phinfortran_amd import tb
+        P = tb.read_param(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "tb", "param"))
+        st = tb.diamond_superlattice(5.5, 2, 2, 8)
+        A = tb.TbAssembler(P, st["latvec"], st["positions"], st["species"], 2.6)
+        n = int(round(np.sqrt(args.tb_hk_k)))
+        kk = np.array([[(i + 0.5) / n - 0.5, (j + 0.5) / n - 0.5, 0.0] for i in range(n) for j in range(n)])
+        H = A.assemble(eng, kk)                                           # warm-up, allocates the output
+        eng.profile_reset()
+        eng.profile(True)
+        torch.cuda.synchronize()
+        nrep = 10
+        t0 = time.perf_counter()
+        for s in range(nrep):
+            A.assemble(eng, kk, out=H)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        eng.profile(False)
+        e = eng.profile_get(N.NT_K_ELEM)
+        avg_ms = e["ms"] / max(1, e["launches"])
+        nk, N_ = len(kk), A.norb
+        alg = 16.0 * N_ * N_ * nk                                          # H(k) written once
+        import re
+        prof = os.path.join(ROOT, "profiles", f"tb_hk_sige_N{N_}_k{nk}_kernel_trace.md")
+        traffic = None
+        if os.path.exists(prof):
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+-]+)", open(prof).read())
+            traffic = float(m.group(1)) if m else None
+        res = {"k_points_per_s": nrep * nk / el, "unit": "H(k)/s", "norb": N_, "k_per_launch": nk,
+               "segments": A.nseg, "entries": A.nent, "avg_launch_ms": avg_ms, "launches": e["launches"],
+               "roofline": {"bound": "hbm", "achieved": alg / (avg_ms * 1e-3) / 1e9, "peak": 8000.0,
+                            "unit": "GB/s", "frac": alg / (avg_ms * 1e-3) / 1e9 / 8000.0,
+                            "algorithmic_bytes": alg, "traffic": traffic}}
+        # write-only reference: torch's fill of the same buffer, timed on torch's stream
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        H.zero_()
+        ev0.record()
+        for s in range(nrep):
+            H.zero_()
+        ev1.record()
+        torch.cuda.synchronize()
+        fill_ms = ev0.elapsed_time(ev1) / nrep
+        res["write_reference"] = {"what": "torch zero_() of the same output buffer (write only)",
+                                  "ms": fill_ms, "GB_per_s": alg / (fill_ms * 1e-3) / 1e9}
+        log(f"tb hk: {nk} k of N={N_} in {avg_ms:.3f} ms per launch ({res['roofline']['frac']:.2f} of HBM); "
+            f"fill of the same buffer {fill_ms:.3f} ms")
+        if args.cpu_sample > 0:
+            from oracle import tb_ham as OT
+            t0 = time.perf_counter()
+            OT.gen_ham(P, st["positions"], st["species"], st["latvec"], kk[1], 2.6)
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": 1.0 / dt, "unit": "H(k)/s", "cores": 1, "kind": "port",
+                                   "sample": f"one k-point of the same structure, the oracle's gen_ham (pure "
+                                             f"Python loops), {dt:.1f} s"}
+        return res
+
+    if args.tb_hk_only:
+        print(json.dumps({"tb_hk_assembly": run_tb_hk()}))
+        eng.close()
+        return
     tuned = {}
     if args.autotune:
         for prec in ("fp64", "mixed"):
@@ -328,48 +394,6 @@ def main():
         return res
 
     probes = run_buttiker() if args.buttiker else None
-
-    def run_tb_hk():
-        """gen_ham of the reference's tight-binding example (Si/Ge superlattice
-        2 x 2 x 8 four-atom cells: 128 atoms, sp3d5s*, 1280 orbitals) for
-        tb_hk_k k-points per launch; the oracle's gen_ham on one k-point
-        beside it"""
-        from phinfortran_amd import tb
-        P = tb.read_param(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "tb", "param"))
-        st = tb.diamond_superlattice(5.5, 2, 2, 8)
-        A = tb.TbAssembler(P, st["latvec"], st["positions"], st["species"], 2.6)
-        n = int(round(np.sqrt(args.tb_hk_k)))
-        kk = np.array([[(i + 0.5) / n - 0.5, (j + 0.5) / n - 0.5, 0.0] for i in range(n) for j in range(n)])
-        H = A.assemble(eng, kk)                                           # warm-up, allocates the output
-        eng.profile_reset()
-        eng.profile(True)
-        torch.cuda.synchronize()
-        nrep = 10
-        t0 = time.perf_counter()
-        for s in range(nrep):
-            A.assemble(eng, kk, out=H)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        eng.profile(False)
-        e = eng.profile_get(N.NT_K_ELEM)
-        avg_ms = e["ms"] / max(1, e["launches"])
-        nk, N_ = len(kk), A.norb
-        alg = 16.0 * N_ * N_ * nk                                          # H(k) written once
-        res = {"k_points_per_s": nrep * nk / el, "unit": "H(k)/s", "norb": N_, "k_per_launch": nk,
-               "segments": A.nseg, "entries": A.nent, "avg_launch_ms": avg_ms, "launches": e["launches"],
-               "roofline": {"bound": "hbm", "achieved": alg / (avg_ms * 1e-3) / 1e9, "peak": 8000.0,
-                            "unit": "GB/s", "frac": alg / (avg_ms * 1e-3) / 1e9 / 8000.0,
-                            "algorithmic_bytes": alg, "traffic": None}}
-        log(f"tb hk: {nk} k of N={N_} in {avg_ms:.3f} ms per launch ({res['roofline']['frac']:.2f} of HBM)")
-        if args.cpu_sample > 0:
-            from oracle import tb_ham as OT
-            t0 = time.perf_counter()
-            OT.gen_ham(P, st["positions"], st["species"], st["latvec"], kk[1], 2.6)
-            dt = time.perf_counter() - t0
-            res["cpu_baseline"] = {"value": 1.0 / dt, "unit": "H(k)/s", "cores": 1, "kind": "port",
-                                   "sample": f"one k-point of the same structure, the oracle's gen_ham (pure "
-                                             f"Python loops), {dt:.1f} s"}
-        return res
 
     tbhk = run_tb_hk() if args.tb_hk else None
     t64 = np.concatenate(T64)

@@ -475,7 +475,11 @@ int nt_hk_assemble_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const
  * bond vector dr[3 q] (Angstrom).  For every k (kcart: nk x 3, 1/Angstrom,
  * = kpoint . recivec) out[k] = sum coef exp(i k.dr); hermitize = 1 then
  * replaces it by (H + H^+)/2 (hermitian, tb/util.f90:709-718; rows = cols).
- * All pointers are device memory. */
+ * The sub-blocks must tile the block on a grid (as atom-pair blocks do):
+ * sub-blocks starting on one row have the same row count, row ranges of
+ * different starts do not overlap, likewise for columns, and no two
+ * sub-blocks share both ranges; NT_ERR_ARG otherwise.  Every element of
+ * out is written once.  All pointers are device memory. */
 int nt_hk_assemble_orb_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, const int* seg_off,
                            const int* seg_geom, int nent, int cstride, const double* coef, const double* dr, int rows,
                            int cols, nt_c128* out, int ld, long long kstride, int hermitize);

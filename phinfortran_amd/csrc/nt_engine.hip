@@ -3616,6 +3616,32 @@ int nt_hk_assemble_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const i
     });
 }
 
+// Interval partition of one axis from the segments' (start, length) pairs:
+// segments with the same start must have the same length and intervals of
+// different starts must not overlap.  Fills blk / loc (per row or column, -1
+// outside every interval) and seg_blk (each segment's interval index);
+// returns the number of intervals or -1.
+static int hk_axis(const std::vector<int>& g, int nseg, int off, int n, std::vector<int>& blk, std::vector<int>& loc,
+                   std::vector<int>& seg_blk)
+{
+    std::vector<std::pair<int, int>> iv;                 // (start, length)
+    for (int q = 0; q < nseg; ++q) iv.push_back({g[4 * q + off], g[4 * q + off + 2]});
+    std::sort(iv.begin(), iv.end());
+    iv.erase(std::unique(iv.begin(), iv.end()), iv.end());
+    for (size_t t = 1; t < iv.size(); ++t)
+        if (iv[t].first < iv[t - 1].first + iv[t - 1].second) return -1;     // overlap or two lengths
+    blk.assign(n, -1);
+    loc.assign(n, 0);
+    for (size_t t = 0; t < iv.size(); ++t)
+        for (int x = 0; x < iv[t].second; ++x) {
+            blk[iv[t].first + x] = (int)t;
+            loc[iv[t].first + x] = x;
+        }
+    seg_blk.resize(nseg);
+    for (int q = 0; q < nseg; ++q) seg_blk[q] = blk[g[4 * q + off]];
+    return (int)iv.size();
+}
+
 int nt_hk_assemble_orb_dev(nt_ctx* c, int nk, const double* kcart, int nseg, const int* seg_off,
                            const int* seg_geom, int nent, int cstride, const double* coef, const double* dr, int rows,
                            int cols, nt_c128* out, int ld, long long kstride, int hermitize)
@@ -3626,25 +3652,74 @@ int nt_hk_assemble_orb_dev(nt_ctx* c, int nk, const double* kcart, int nseg, con
         return NT_ERR_ARG;
     if (nk == 0) return NT_OK;
     return guarded(c, [&] {
-        // host check of the segment table: offsets monotone within [0, nent],
-        // every sub-block inside the block and no larger than cstride
+        // host check of the segment table (offsets monotone within [0, nent],
+        // every sub-block inside the block and no larger than cstride, the
+        // sub-blocks on a grid with no two in one cell) and the element ->
+        // segment maps the kernel reads
+        std::vector<int> off(nseg + 1, 0), g(4 * (size_t)nseg);
         if (nseg > 0) {
-            std::vector<int> off(nseg + 1), g(4 * (size_t)nseg);
             hipcheck(c, hipMemcpy(off.data(), seg_off, sizeof(int) * (nseg + 1), hipMemcpyDeviceToHost), "D2H");
             hipcheck(c, hipMemcpy(g.data(), seg_geom, sizeof(int) * 4 * nseg, hipMemcpyDeviceToHost), "D2H");
-            if (off[0] != 0 || off[nseg] != nent) {
-                c->err = "hk: segment offsets must run from 0 to nent";
+        }
+        if (off[0] != 0 || off[nseg] != nent) {
+            c->err = "hk: segment offsets must run from 0 to nent";
+            return (int)NT_ERR_ARG;
+        }
+        for (int q = 0; q < nseg; ++q) {
+            const int r0 = g[4 * q], c0 = g[4 * q + 1], ni = g[4 * q + 2], nj = g[4 * q + 3];
+            if (off[q + 1] < off[q] || r0 < 0 || c0 < 0 || ni < 1 || nj < 1 || (long)ni * nj > cstride ||
+                r0 > rows - ni || c0 > cols - nj) {
+                c->err = "hk: segment outside the block";
                 return (int)NT_ERR_ARG;
             }
-            for (int q = 0; q < nseg; ++q) {
-                const int r0 = g[4 * q], c0 = g[4 * q + 1], ni = g[4 * q + 2], nj = g[4 * q + 3];
-                if (off[q + 1] < off[q] || r0 < 0 || c0 < 0 || ni < 1 || nj < 1 || (long)ni * nj > cstride ||
-                    r0 + ni > rows || c0 + nj > cols) {
-                    c->err = "hk: segment outside the block";
-                    return (int)NT_ERR_ARG;
-                }
-            }
         }
+        std::vector<int> rblk, rloc, cblk, cloc, srb, scb;
+        const int nrb = hk_axis(g, nseg, 0, rows, rblk, rloc, srb);
+        const int ncb = hk_axis(g, nseg, 1, cols, cblk, cloc, scb);
+        if (nrb < 0 || ncb < 0) {
+            c->err = "hk: segments must tile the block on a grid of row and column intervals";
+            return (int)NT_ERR_ARG;
+        }
+        std::vector<int> order(nseg);
+        for (int q = 0; q < nseg; ++q) order[q] = q;
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return srb[a] != srb[b] ? srb[a] < srb[b] : scb[a] < scb[b];
+        });
+        std::vector<int> rptr(nrb + 1, 0), ckey(nseg), sid(nseg);
+        for (int t = 0; t < nseg; ++t) {
+            const int q = order[t];
+            if (t > 0 && srb[q] == srb[order[t - 1]] && scb[q] == scb[order[t - 1]]) {
+                c->err = "hk: two segments in one sub-block";
+                return (int)NT_ERR_ARG;
+            }
+            ckey[t] = scb[q];
+            sid[t] = q;
+            ++rptr[srb[q] + 1];
+        }
+        for (int b = 0; b < nrb; ++b) rptr[b + 1] += rptr[b];
+        // maps packed into the workspace: rblk rloc cblk cloc rptr ckey sid
+        std::vector<int> pack;
+        pack.reserve(2 * (size_t)rows + 2 * (size_t)cols + nrb + 1 + 2 * (size_t)nseg + 1);
+        size_t o_rblk = pack.size();
+        pack.insert(pack.end(), rblk.begin(), rblk.end());
+        size_t o_rloc = pack.size();
+        pack.insert(pack.end(), rloc.begin(), rloc.end());
+        size_t o_cblk = pack.size();
+        pack.insert(pack.end(), cblk.begin(), cblk.end());
+        size_t o_cloc = pack.size();
+        pack.insert(pack.end(), cloc.begin(), cloc.end());
+        size_t o_rptr = pack.size();
+        pack.insert(pack.end(), rptr.begin(), rptr.end());
+        size_t o_ckey = pack.size();
+        pack.insert(pack.end(), ckey.begin(), ckey.end());
+        size_t o_sid = pack.size();
+        pack.insert(pack.end(), sid.begin(), sid.end());
+        pack.push_back(0);                                  // keeps the last array non-empty
+        ensure_ws(c, sizeof(int) * pack.size());
+        int* w = static_cast<int*>(c->ws);
+        hipcheck(c, hipMemcpyAsync(w, pack.data(), sizeof(int) * pack.size(), hipMemcpyHostToDevice, c->stream),
+                 "H2D");
+        const HkGrid hg{w + o_rblk, w + o_rloc, w + o_cblk, w + o_cloc, w + o_rptr, w + o_ckey, w + o_sid};
         const size_t z = sizeof(nt_c128);
         reg_region(c, kcart, sizeof(double) * 3 * nk);
         reg_region(c, seg_off, sizeof(int) * (nseg + 1));
@@ -3655,9 +3730,11 @@ int nt_hk_assemble_orb_dev(nt_ctx* c, int nk, const double* kcart, int nseg, con
         reg_region(c, out, span);
         check_range(c, kcart, sizeof(double) * 3 * nk, "hk k");
         launch(c, NT_K_ELEM, 0.0, "hk assemble orb", [&] {
-            return launch_hk_assemble_orb(nk, kcart, nseg, seg_off, seg_geom, cstride, coef, dr, rows, cols,
+            return launch_hk_assemble_orb(nk, kcart, seg_off, seg_geom, cstride, coef, dr, hg, rows, cols,
                                           reinterpret_cast<double*>(out), ld, (long)kstride, hermitize, c->stream);
         });
+        // the maps were copied from pageable host memory that ends with this call
+        hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         return 0;
     });
 }

@@ -155,8 +155,12 @@ hipError_t launch_kpc_select(const KpcArgs& a, int m0, int M, const int* off, co
 hipError_t launch_hk_assemble(int nk, const double* kc, int nseg, const int* seg_off, const int* seg_rc,
                               const double* coef, const double* dr, int rows, int cols, double* out, int ld,
                               long kstride, hipStream_t s);
-hipError_t launch_hk_assemble_orb(int nk, const double* kc, int nseg, const int* seg_off, const int* seg_geom,
-                                  int cstride, const double* coef, const double* dr, int rows, int cols,
+// element -> segment maps of a grid-tiled sub-block table (nt_hk.hip)
+struct HkGrid {
+    const int *rblk, *rloc, *cblk, *cloc, *rptr, *ckey, *sid;
+};
+hipError_t launch_hk_assemble_orb(int nk, const double* kc, const int* seg_off, const int* seg_geom, int cstride,
+                                  const double* coef, const double* dr, const HkGrid& g, int rows, int cols,
                                   double* out, int ld, long kstride, int hermitize, hipStream_t s);
 
 // Degenerate-subspace rotation of the mode-resolved solve (nt_degen.hip).

@@ -77,4 +77,11 @@ def test_gpu_malformed_tables_refused(engine, P):
     bad = seg_geom.clone()
     bad[2] = 11                                              # 11 x 10 > cstride
     assert call(bad, N, N, 1) == -3
+    bad = seg_geom.clone()
+    bad[4] = 5                                               # second sub-block's rows overlap the first's
+    assert call(bad, N, N, 1) == -3
+    bad = seg_geom.clone()
+    bad[4], bad[5] = 0, 0                                    # two sub-blocks in one cell of the grid
+    assert call(bad, N, N, 1) == -3
+    assert call(seg_geom, N, N, 0) == 0                      # the table still assembles after the refusals
     torch.cuda.synchronize()

@@ -374,7 +374,7 @@ int or_rgf(int L, const int* bsz, const ocplx* const* Hd, const ocplx* const* Hd
         mm('N', 'N', nr, nr, bN, T1, Vt, M);
         dyson_lhs(nr, zd, Hamr, sr, M, A);
         if ((st = or_inv(nr, A, Gn))) goto out;
-        mm('N', 'N', nl, bN, bN, G0, Vt, T1);        /* nl x nr */
+        mm('N', 'N', nl, nr, bN, G0, Vt, T1);        /* G_{0,N} V_{N,N+1}: nl x nr */
         mm('N', 'N', nl, nr, nr, T1, Gn, GLR);
         if (G_R) memcpy(G_R, Gn, sizeof(ocplx) * nr * nr);
     }

@@ -106,3 +106,104 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
         eng.close()
     out.update(E=E, k=ks, transl=tl, transr=tr, status=st, blocks=d)
     return out
+
+
+def _tb_device_blocks(H, layers, nl: int, nr: int, nbl: int, nbr: int):
+    """tdb block arrays of every k from the dense H(k) on the device (nk x N x N
+    tensor of column-major matrices): the lead and device slices of
+    tb/surface.f90:190-206 and tb/device.f90:291-575 for equal layer sizes,
+    in the layout of Engine.upload_blocks."""
+    import torch
+    nk, N = H.shape[0], H.shape[-1]
+    b, L = layers[0], len(layers)
+    li, lo = slice(nbl, nbl + nl), slice(nbl + nl, nbl + 2 * nl)
+    ro, ri = slice(N - nbr - 2 * nr, N - nbr - nr), slice(N - nbr - nr, N - nbr)
+    d0 = nbl + 2 * nl
+    sl = [slice(d0 + i * b, d0 + (i + 1) * b) for i in range(L)]
+    blk = lambda R, Cc: H[:, Cc, R].contiguous()           # storage of the matrix H(R, Cc)
+    out = dict(Haml=blk(li, li), Hamlv=blk(li, lo), Hamr=blk(ri, ri), Hamrv=blk(ri, ro),
+               Hamd=torch.stack([blk(s, s) for s in sl], dim=1),
+               Hamld=blk(lo, sl[0]), Hamrd=blk(ro, sl[-1]))
+    out["Hamdv"] = (torch.stack([blk(sl[i], sl[i + 1]) for i in range(L - 1)], dim=1) if L > 1 else
+                    torch.zeros((nk, 1, b, b), dtype=H.dtype, device=H.device))
+    out.update(nl=nl, nr=nr, b=b, L=L, nk=nk)
+    return out
+
+
+def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, param_path: Optional[str] = None,
+           batched: Optional[bool] = None):
+    """The tight-binding driver's transport run (tb/infortran.f90:77-630,
+    path_mode 2 and 3, Caroli transmission) on one GPU.
+
+    Reads confige (tb.read_confige), the structure file it names and the
+    Slater-Koster parameters (param_path, default "param" beside the
+    configuration, as the reference opens it); H(k) of every k-point (moved
+    into the Wigner-Seitz cell by k_grid_in_ws) is assembled on the GPU
+    (gen_ham); the layers come from blockwise_setup on the first k-point's
+    H, as the reference's rank 0 does (tb/infortran.f90:398).  With equal
+    layer sizes every (E, k) point is solved in one batched call on blocks
+    sliced on the device (batched=None or True); otherwise, or with
+    batched=False, point by point (surface_green + G_block, the reference's
+    call sequence, tb/infortran.f90:492-504).  Writes transl_tot.dat /
+    transr_tot.dat in the tight-binding writer's format and positions_all.dat,
+    nkpoints.dat when output_dir is given (defaults to the config's
+    output_dir, relative to the configuration's folder).
+
+    Returns dict(E (ne), k (nk x 3, as used), transl, transr (ne x nk),
+    status, layer_list, norb, nl, nr)."""
+    import torch
+    from . import dense as D
+    from . import tb as TBM
+    from . import transport as T
+    base = os.path.dirname(os.path.abspath(config_path))
+    cfg = TBM.read_confige(config_path)
+    if cfg["path_mode"] not in (2, 3):
+        raise ValueError(f"path_mode {cfg['path_mode']}: only the transmission modes 2 and 3 are provided")
+    if cfg["circular_flag"] == 1:
+        raise NotImplementedError("the tight-binding run solves the leads by the direct decimation; "
+                                  "circular_flag = 1 is not provided here")
+    fdf = cfg["filename_input"]
+    st = TBM.read_fdf(fdf if os.path.isabs(fdf) else os.path.join(base, fdf))
+    P = TBM.read_param(param_path or os.path.join(base, "param"))
+    asm = TBM.TbAssembler(P, st["latvec"], st["positions"], st["species"], cfg["r_cutoff"])
+    ks = TBM.k_grid_in_ws(TBM.k_grid(cfg, base), asm.recivec)
+    E = TBM.e_grid(cfg)
+    lo = TBM.lead_orbitals(P, st["species"], cfg)
+    nl, nr, nbl, nbr = lo["nl"], lo["nr"], lo["n_buffer_l"], lo["n_buffer_r"]
+    eng = engine or T.Engine(0)
+    nk, ne = len(ks), len(E)
+    H = asm.assemble(eng, ks)
+    layers = D.blockwise_setup(H[0].transpose(0, 1).cpu().numpy(), nl, nr, nbl, nbr)
+    p = T.params("tb", eta=cfg["eta"], convergence=cfg["convergence"])
+    uniform = len(set(layers)) == 1
+    if batched is None:
+        batched = uniform
+    if batched and not uniform:
+        raise ValueError(f"batched solve needs equal layers, blockwise_setup gave {layers}")
+    if batched:
+        d = _tb_device_blocks(H, layers, nl, nr, nbl, nbr)
+        kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)
+        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx)
+        tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
+        tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
+        status = r["status"].cpu().numpy().reshape(nk, ne).T
+    else:
+        tl, tr = np.zeros((ne, nk)), np.zeros((ne, nk))
+        status = np.zeros((ne, nk), np.int32)
+        for ik in range(nk):
+            s = D.slices(H[ik].transpose(0, 1).cpu().numpy(), layers, nl, nr, nbl, nbr, variant="tb")
+            for ie in range(ne):
+                g = eng.surface_green_tdb(s.Haml, s.Hamr, s.Hamlv, s.Hamrv, E[ie], p, want_plus_minus=False)
+                o = eng.G_block(g["gl_adv"].conj().T, g["gr_ret"], -g["gml_adv"], g["gmr_ret"],
+                                g["sl_adv"].conj().T, g["sr_ret"], E[ie], s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml,
+                                s.Hamr, p, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
+                tl[ie, ik], tr[ie, ik] = o["transl"], o["transr"]
+    out_dir = output_dir
+    if out_dir is None and cfg["output_dir"]:
+        out_dir = os.path.join(base, cfg["output_dir"])
+    if out_dir:
+        TBM.write_positions(out_dir, st["positions"], st["species"], P["orb_num"], layers, nbl, nl, nk)
+        OUT.write_trans(out_dir, np.repeat(E[:, None], nk, axis=1), tl, tr, tb=True)
+    if engine is None:
+        eng.close()
+    return dict(E=E, k=ks, transl=tl, transr=tr, status=status, layer_list=layers, norb=asm.norb, nl=nl, nr=nr)

@@ -34,33 +34,36 @@ def fortran_e(x: float, w: int = 24, d: int = 16) -> str:
     return body.rjust(w)
 
 
-def _table(path: str, E1, T, tail: str):
+def _table(path: str, E1, T, tail: str, w: int = 24, d: int = 16, clamp: bool = True):
     T = np.asarray(T, float)
     with open(path, "w") as f:
         for i in range(T.shape[0]):
-            row = [fortran_e(E1[i])] + [fortran_e(v if v >= 1e-60 else 0.0) for v in T[i]]
+            row = [fortran_e(E1[i], w, d)] + [fortran_e(v if (v >= 1e-60 or not clamp) else 0.0, w, d) for v in T[i]]
             f.write("".join(row) + tail + "\n")
 
 
 def write_trans(output_dir: str, E, transl, transr, transl_s: Optional[np.ndarray] = None,
                 transl_ns=None, transr_s=None, transr_ns=None, refll_s=None, refll_ns=None, reflr_s=None,
-                reflr_ns=None):
+                reflr_ns=None, tb: bool = False):
     """Write the tables of write_trans.  E: (ne, nk) energy grid (or the
     first column); the T arrays are (ne, nk).  Tables passed as None are
-    not written (the Caroli-only run produces only *_tot)."""
+    not written (the Caroli-only run produces only *_tot).  tb=True: the
+    tight-binding driver's writer (tb/write_data.f90:362-470), edit
+    descriptor 1e20.10 and values written as they are."""
+    fmt = dict(w=20, d=10, clamp=False) if tb else {}
     os.makedirs(output_dir, exist_ok=True)
     E = np.asarray(E, float)
     E1 = E[:, 0] if E.ndim == 2 else E
     for name, T in (("transl_s", transl_s), ("transl_ns", transl_ns), ("refll_s", refll_s),
                     ("refll_ns", refll_ns)):
         if T is not None:
-            _table(os.path.join(output_dir, name + ".dat"), E1, T, " ")
+            _table(os.path.join(output_dir, name + ".dat"), E1, T, " ", **fmt)
     for name, T in (("transr_s", transr_s), ("transr_ns", transr_ns), ("reflr_s", reflr_s),
                     ("reflr_ns", reflr_ns)):
         if T is not None:
-            _table(os.path.join(output_dir, name + ".dat"), E1, T, "")
-    _table(os.path.join(output_dir, "transl_tot.dat"), E1, transl, "")
-    _table(os.path.join(output_dir, "transr_tot.dat"), E1, transr, "")
+            _table(os.path.join(output_dir, name + ".dat"), E1, T, "", **fmt)
+    _table(os.path.join(output_dir, "transl_tot.dat"), E1, transl, "", **fmt)
+    _table(os.path.join(output_dir, "transr_tot.dat"), E1, transr, "", **fmt)
 
 
 # ------------------------------------------------------------ per-mode tables

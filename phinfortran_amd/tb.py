@@ -26,7 +26,8 @@ the reference's sqrt(3.0).
 from __future__ import annotations
 
 import ctypes as C
-from typing import Dict, List
+import os
+from typing import Dict, List, Optional
 
 import numpy as np
 
@@ -143,6 +144,119 @@ def diamond_superlattice(a: float, nx: int, ny: int, nz: int, split: bool = True
     if split:
         sp[: nat // 2] = 2
     return dict(latvec=np.diag([nx, ny, nz]).astype(float) @ cell, positions=pos, species=sp)
+
+
+def read_confige(path: str) -> Dict:
+    """The namelists of the tight-binding driver's configuration file
+    (`confige`, tb/config.f90:19-80: configlist, setup, convergelist,
+    unfolding, kpath, saving, misori).  Keys are lower-cased; array keys may
+    be written with or without "(:)"; several assignments may share a line."""
+    import re
+    vals: Dict[str, str] = {}
+    pat = re.compile(r"([A-Za-z_]\w*)(?:\(:\))?\s*=\s*(.*?)(?=,?\s*[A-Za-z_]\w*(?:\(:\))?\s*=|$)")
+    with open(path) as f:
+        for line in f:
+            line = line.split("!")[0].strip()
+            if not line or line.startswith("&") or line == "/":
+                continue
+            for k, v in pat.findall(line):
+                vals[k.lower()] = v.strip().rstrip(",").strip().strip('"').strip("'")
+    num = lambda k, d=None: float(vals[k].replace("d", "e").replace("D", "e")) if k in vals else d
+    ints = lambda k, d=None: [int(x) for x in vals[k].replace(",", " ").split()] if k in vals else d
+    mesh = ints("k_trans_mesh", [1, 1, 1])
+    return dict(nk1=mesh[0], nk2=mesh[1], ne=int(vals.get("ne", 1)), emin=num("emin"), emax=num("emax"),
+                r_cutoff=num("r_cutoff"), filename_input=vals.get("filename_input"),
+                left_primitive_cell_input=vals.get("left_primitive_cell_input"),
+                right_primitive_cell_input=vals.get("right_primitive_cell_input"),
+                period_left=int(vals.get("period_left", 0)), period_right=int(vals.get("period_right", 0)),
+                buffer_left=int(vals.get("buffer_left", 0)), buffer_right=int(vals.get("buffer_right", 0)),
+                eta=num("eta"), convergence=num("convergence"),
+                n_bloch_x=int(vals.get("n_bloch_x", 1)), n_bloch_y=int(vals.get("n_bloch_y", 1)),
+                circular_flag=int(vals.get("circular_flag", 0)),
+                path_mode=int(vals.get("path_mode", 2)), kpath_file=vals.get("kpath_file"),
+                output_dir=vals.get("output_dir"), verbosity=int(vals.get("verbosity", 0)))
+
+
+def k_grid(cfg: Dict, base_dir: Optional[str] = None) -> np.ndarray:
+    """tb/config.f90:83-186: the centred k_trans_mesh for path_mode = 2
+    (the single point of a 1 x 1 mesh at 0), otherwise the points of
+    kpath_file (first line: their number) with the third component 0."""
+    if cfg["path_mode"] != 2:
+        path = cfg["kpath_file"]
+        if base_dir and not os.path.isabs(path):
+            path = os.path.join(base_dir, path)
+        rows = [ln.split() for ln in open(path) if ln.strip()]
+        n = int(rows[0][0])
+        ks = np.array([[float(x) for x in rows[1 + i][:3]] for i in range(n)])
+        ks[:, 2] = 0.0
+        return ks
+    n1, n2 = cfg["nk1"], cfg["nk2"]
+    ks = np.array([[(i - 1) / n1 - 0.5 * (n1 - 1) / n1, (j - 1) / n2 - 0.5 * (n2 - 1) / n2, 0.0]
+                   for i in range(1, n1 + 1) for j in range(1, n2 + 1)])
+    if n1 * n2 == 1:
+        ks[0, :2] = 0.0
+    return ks
+
+
+def k_grid_in_ws(ks: np.ndarray, recivec: np.ndarray) -> np.ndarray:
+    """k_grid_in_ws (tb/input.f90:340-382): each k moved by the in-plane
+    reciprocal vector n1 b1 + n2 b2 (n in [-2, 2]^2, n1 slowest, first
+    minimum) that brings its in-plane part closest to the origin; the
+    shift is applied to all three crystal components."""
+    a = np.array(recivec, float)
+    a[2, :] = 0.0
+    shifts = np.array([(i1, i2, 0.0) for i1 in range(-2, 3) for i2 in range(-2, 3)], float)
+    out = np.array(ks, float).copy()
+    for i in range(len(out)):
+        kp = out[i] @ a
+        d = kp[None, :] - shifts @ a
+        dist = np.einsum("ij,ij->i", d, d)
+        out[i] = out[i] - shifts[int(np.argmin(dist))]
+    return out
+
+
+def e_grid(cfg: Dict) -> np.ndarray:
+    """tb/config.f90:197-211: ne energies from emin to emax (emax alone for ne = 1)."""
+    ne, emin, emax = cfg["ne"], cfg["emin"], cfg["emax"]
+    if not emax > emin:
+        raise ValueError("emax must exceed emin")
+    if ne == 1:
+        return np.array([emax])
+    return np.array([emin + (i - 1) / (ne - 1) * (emax - emin) for i in range(1, ne + 1)])
+
+
+def lead_orbitals(P: Dict, species, cfg: Dict) -> Dict:
+    """get_periodic_lead_orb_num (tb/surface.f90:11-91): orbitals of the left
+    buffer (the first buffer_left atoms), of the right buffer (the last
+    buffer_left atoms, as the reference counts them), of one left period
+    and one right period of the leads."""
+    onum = [P["orb_num"][s - 1] for s in species]
+    nat = len(onum)
+    bl, br, pl, pr = cfg["buffer_left"], cfg["buffer_right"], cfg["period_left"], cfg["period_right"]
+    return dict(n_buffer_l=sum(onum[:bl]), n_buffer_r=sum(onum[nat - bl:]) if bl else 0,
+                nl=sum(onum[bl:bl + pl]), nr=sum(onum[nat - br - pr:nat - br]))
+
+
+def write_positions(output_dir: str, positions, species, orb_num, layer_list, n_buffer_l: int, nl: int, nk: int):
+    """write_positions (tb/write_data.f90:340-360) as blockwise_setup calls it
+    (tb/device.f90:232-247): nkpoints.dat (nk, f20.10) and positions_all.dat,
+    one row per orbital: the atom's x, y, z, species and the 1-based device
+    layer (0 outside the device), format 5f20.10."""
+    rows = []
+    for a in range(len(positions)):
+        for _ in range(orb_num[species[a] - 1]):
+            rows.append([*positions[a], float(species[a]), 0.0])
+    rows = np.array(rows)
+    start = n_buffer_l + 2 * nl
+    for i, n in enumerate(layer_list):
+        rows[start:start + n, 4] = i + 1
+        start += n
+    os.makedirs(output_dir, exist_ok=True)
+    with open(os.path.join(output_dir, "nkpoints.dat"), "w") as f:
+        f.write("%20.10f\n" % float(nk))
+    with open(os.path.join(output_dir, "positions_all.dat"), "w") as f:
+        for r in rows:
+            f.write("".join("%20.10f" % x for x in r) + "\n")
 
 
 # ------------------------------------------------------------------ Slater-Koster

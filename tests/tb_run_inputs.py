@@ -1,0 +1,54 @@
+"""Input decks of a tight-binding transport run for the driver tests: a
+Si/Ge superlattice along z (tetragonal four-atom diamond cells, one cell per
+lead period), the example's parameters, confige with every namelist the
+driver reads."""
+import os
+import shutil
+
+import numpy as np
+
+from tests import tb_structures as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PARAM = os.path.join(HERE, "golden", "tb", "param")
+
+CONFIGE = """&configlist
+ k_trans_mesh = {nk1} {nk2} 1, emin = {emin}, emax = {emax}, ne = {ne}
+ r_cutoff = 2.6d0, filename_input = 'input.fdf'
+ left_primitive_cell_input = 'none', right_primitive_cell_input = 'none'
+/
+&setup
+ period_left = 4, period_right = 4, buffer_left = 0, buffer_right = 0
+/
+&convergelist
+ eta = {eta}, convergence = 1.0d-10
+/
+&unfolding
+ n_bloch_x = 1, n_bloch_y = 1, circular_flag = 0
+/
+&kpath
+ kpath_file = 'kpath', path_mode = {path_mode}, side = 'left', az1 = 5.5
+/
+&saving
+ output_dir = 'out', verbosity = 0
+/
+&misori
+ uc_misori_x(:) = 1.0 0.0 0.0, uc_misori_y(:) = 0.0 1.0 0.0, uc_misori_z(:) = 0.0 0.0 1.0
+ n_uc_misori1 = 1, n_uc_misori2 = 1, n_uc_misori3 = 1, theta = 0.0
+/
+"""
+
+
+def write_run(folder, nz=8, split=True, nk=(2, 2), emin=-1.5, emax=2.2, ne=5, eta="1.0d-5", path_mode=2,
+              kpath=None):
+    lat, pos, sp = S.si_ge_superlattice(5.5, 1, 1, nz)
+    if not split:
+        sp = np.full_like(sp, 2)
+    S.write_fdf(os.path.join(folder, "input.fdf"), lat, pos, sp)
+    shutil.copy(PARAM, os.path.join(folder, "param"))
+    with open(os.path.join(folder, "confige"), "w") as f:
+        f.write(CONFIGE.format(nk1=nk[0], nk2=nk[1], emin=emin, emax=emax, ne=ne, eta=eta, path_mode=path_mode))
+    if kpath is not None:
+        with open(os.path.join(folder, "kpath"), "w") as f:
+            f.write(f"{len(kpath)}\n" + "".join(" ".join(str(x) for x in k) + "\n" for k in kpath))
+    return os.path.join(folder, "confige"), lat, pos, sp

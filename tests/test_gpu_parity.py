@@ -98,9 +98,10 @@ def test_get_G_full_tdb(engine, T, nl, b, L, seed):
     assert z["transl"] == 0.0 and z["transr"] == 0.0
 
 
-def test_G_block_variable_layers(engine, T):
+@pytest.mark.parametrize("sizes", [[3, 17, 2, 40, 9], [2, 9, 4, 3]])
+def test_G_block_variable_layers(engine, T, sizes):
+    """end layers wider and narrower than the 6-orbital lead layers"""
     rng = np.random.default_rng(4)
-    sizes = [3, 17, 2, 40, 9]
     nl = nr = 6
     blk = lt.random_blocks(nl, 3, 1, seed=4)
     c = lambda m, n: (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))) * 0.4 / np.sqrt(max(m, n))

@@ -1,0 +1,76 @@
+"""The tight-binding transport run (driver.run_tb: tb/infortran.f90:77-630)
+on the GPU against the oracle chain at every (E, k) point of a Si/Ge
+superlattice run: oracle gen_ham at the Wigner-Seitz-shifted k, the
+reference's layer partition and slices, the C oracle's decimation and RGF.
+The batched and the point-by-point solves agree; a pristine Si run gives
+integer channel counts; the output files have the reference's shape."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import tb_ham as TB
+from phinfortran_amd import dense as D
+from phinfortran_amd import driver
+from phinfortran_amd import tb
+from tests import tb_run_inputs as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def Tm():
+    from phinfortran_amd import transport
+    return transport
+
+
+def oracle_T(Tm, P, lat, pos, sp, k, E, eta, layers, nl, nr):
+    H = TB.gen_ham(P, pos, sp, lat, k, 2.6)
+    s = D.slices(H, layers, nl, nr, 0, 0, variant="tb")
+    p = Tm.params("tb", eta=eta, convergence=1e-10)
+    op = O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
+    r = O.surface_green(s.Haml, s.Hamlv, s.Hamr, s.Hamrv, E, op, want_plus_minus=False)
+    o = O.rgf(s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml, s.Hamr, r["gl_adv"].conj().T, r["gr_ret"], -r["gml_adv"],
+              r["gmr_ret"], r["sl_adv"].conj().T, r["sr_ret"], E, op, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
+    return o["T_LR"], o["T_RL"]
+
+
+def test_run_tb_matches_oracle(engine, Tm, tmp_path):
+    cfg, lat, pos, sp = R.write_run(str(tmp_path))
+    print("run_tb batched", flush=True)
+    r = driver.run_tb(cfg, engine=engine)
+    assert r["norb"] == 320 and r["nl"] == r["nr"] == 40 and len(set(r["layer_list"])) == 1
+    assert (r["status"] == 0).all()
+    P = tb.read_param(R.PARAM)
+    for ik, k in enumerate(r["k"]):
+        for ie, E in enumerate(r["E"]):
+            tl, tr = oracle_T(Tm, P, lat, pos, sp, k, E, 1e-5, r["layer_list"], 40, 40)
+            assert abs(r["transl"][ie, ik] - tl) <= 1e-8 * max(1.0, tl), (ik, E)
+            assert abs(r["transr"][ie, ik] - tr) <= 1e-8 * max(1.0, tr), (ik, E)
+    assert r["transl"].max() > 0.5 and r["transl"][2].max() < 1e-10            # E = 0.35 eV lies in the gap
+    out = os.path.join(str(tmp_path), "out")
+    rows = open(os.path.join(out, "transl_tot.dat")).read().splitlines()
+    assert len(rows) == 5 and all(len(x) == 20 * 5 for x in rows)
+    assert float(rows[3][20:40]) == pytest.approx(r["transl"][3, 0], rel=1e-9)
+    assert len(open(os.path.join(out, "positions_all.dat")).read().splitlines()) == 320
+    assert open(os.path.join(out, "nkpoints.dat")).read() == "        4.0000000000\n"
+
+
+def test_run_tb_pointwise_equals_batched(engine, tmp_path):
+    cfg, *_ = R.write_run(str(tmp_path), nk=(1, 2), ne=3)
+    print("run_tb batched vs point by point", flush=True)
+    a = driver.run_tb(cfg, engine=engine, output_dir=str(tmp_path / "a"))
+    b = driver.run_tb(cfg, engine=engine, output_dir=str(tmp_path / "b"), batched=False)
+    assert np.abs(a["transl"] - b["transl"]).max() <= 1e-9 * max(1.0, np.abs(a["transl"]).max())
+    assert np.abs(a["transr"] - b["transr"]).max() <= 1e-9 * max(1.0, np.abs(a["transr"]).max())
+
+
+def test_run_tb_pristine_si_integer_channels(engine, tmp_path):
+    cfg, *_ = R.write_run(str(tmp_path), split=False, nk=(1, 1), emin=-1.2, emax=2.4, ne=4, eta="1.0d-7")
+    print("run_tb pristine Si", flush=True)
+    r = driver.run_tb(cfg, engine=engine)
+    t = r["transl"][:, 0]
+    assert np.abs(t - np.round(t)).max() < 2e-3, t
+    assert np.abs(r["transl"] - r["transr"]).max() < 1e-6
+    assert t[0] >= 1 and t[-1] >= 1

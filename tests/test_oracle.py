@@ -224,12 +224,16 @@ def test_rgf_device_eta_factor_used():
     assert np.abs(G1 - G2).max() > 1e-4 * sc
 
 
-def test_rgf_variable_blocks_and_back_couplings():
-    """G_block semantics on a dense Hamiltonian cut into uneven layers."""
+@pytest.mark.parametrize("sizes,nl,nr", [([3, 5, 2, 4], 3, 3), ([2, 5, 3, 1], 4, 4), ([1, 2, 2], 3, 5)])
+def test_rgf_variable_blocks_and_back_couplings(sizes, nl, nr):
+    """G_block semantics on a dense Hamiltonian cut into uneven layers,
+    including end layers narrower than the lead layers (the closing product
+    G_{0,N} V_{N,N+1} is nl x nr whatever the last layer's width); the
+    Caroli traces of both sweeps against the dense inverse."""
     rng = np.random.default_rng(4)
-    sizes = [3, 5, 2, 4]
-    nl = nr = 3
     blk = lt.random_blocks(nl, 3, 1, seed=4)
+    if nr != nl:
+        blk.Hamr, blk.Hamrv = lt.random_blocks(nr, 3, 1, seed=5).Hamr, lt.random_blocks(nr, 3, 1, seed=5).Hamrv
     Hd = []
     for s in sizes:
         x = crand(rng, s, s) * 0.5
@@ -264,6 +268,10 @@ def test_rgf_variable_blocks_and_back_couplings():
     G = np.linalg.inv((E + 1e-3j) * np.eye(n) - H - Sig)
     assert np.abs(r["G_D_LR"] - G[:nl, n - nr:]).max() < 1e-10 * np.abs(G).max()
     assert np.abs(r["G_D_RL"] - G[n - nr:, :nl]).max() < 1e-10 * np.abs(G).max()
+    g0n, gn0 = G[:nl, n - nr:], G[n - nr:, :nl]
+    t_lr = np.trace(g0n @ lq["gmr"] @ g0n.conj().T @ lq["gml"]).real
+    t_rl = np.trace(gn0 @ lq["gml"] @ gn0.conj().T @ lq["gmr"]).real
+    assert abs(r["T_LR"] - t_lr) < 1e-9 * max(1.0, abs(t_lr)) and abs(r["T_RL"] - t_rl) < 1e-9 * max(1.0, abs(t_rl))
 
 
 def test_chain_transmission_is_one_in_band():

@@ -111,22 +111,34 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
 def _tb_device_blocks(H, layers, nl: int, nr: int, nbl: int, nbr: int):
     """tdb block arrays of every k from the dense H(k) on the device (nk x N x N
     tensor of column-major matrices): the lead and device slices of
-    tb/surface.f90:190-206 and tb/device.f90:291-575 for equal layer sizes,
-    in the layout of Engine.upload_blocks."""
+    tb/surface.f90:190-206 and tb/device.f90:291-575, in the layout of
+    Engine.upload_blocks.  Layers narrower than the widest, B, are padded to
+    B x B with orbitals that couple to nothing and have zero on-site energy:
+    each padded layer matrix is block-diagonal, [[A, 0], [0, z]], so the
+    inverse of its real part, every product and the transmission are those
+    of the unpadded layers (partial pivoting never takes a padded row for a
+    real column, and the extra GEMM terms are exact zeros)."""
     import torch
     nk, N = H.shape[0], H.shape[-1]
-    b, L = layers[0], len(layers)
+    B, L = max(layers), len(layers)
     li, lo = slice(nbl, nbl + nl), slice(nbl + nl, nbl + 2 * nl)
     ro, ri = slice(N - nbr - 2 * nr, N - nbr - nr), slice(N - nbr - nr, N - nbr)
-    d0 = nbl + 2 * nl
-    sl = [slice(d0 + i * b, d0 + (i + 1) * b) for i in range(L)]
-    blk = lambda R, Cc: H[:, Cc, R].contiguous()           # storage of the matrix H(R, Cc)
-    out = dict(Haml=blk(li, li), Hamlv=blk(li, lo), Hamr=blk(ri, ri), Hamrv=blk(ri, ro),
-               Hamd=torch.stack([blk(s, s) for s in sl], dim=1),
-               Hamld=blk(lo, sl[0]), Hamrd=blk(ro, sl[-1]))
-    out["Hamdv"] = (torch.stack([blk(sl[i], sl[i + 1]) for i in range(L - 1)], dim=1) if L > 1 else
-                    torch.zeros((nk, 1, b, b), dtype=H.dtype, device=H.device))
-    out.update(nl=nl, nr=nr, b=b, L=L, nk=nk)
+    starts = np.concatenate([[nbl + 2 * nl], nbl + 2 * nl + np.cumsum(layers)[:-1]]).astype(int)
+    sl = [slice(int(a), int(a) + int(n)) for a, n in zip(starts, layers)]
+    blk = lambda R, Cc: H[:, Cc, R]                       # storage of the matrix H(R, Cc), a view
+
+    def padded(R, Cc, rows, cols):
+        t = torch.zeros((nk, cols, rows), dtype=H.dtype, device=H.device)
+        v = blk(R, Cc)
+        t[:, : v.shape[1], : v.shape[2]] = v
+        return t
+    out = dict(Haml=blk(li, li).contiguous(), Hamlv=blk(li, lo).contiguous(), Hamr=blk(ri, ri).contiguous(),
+               Hamrv=blk(ri, ro).contiguous(),
+               Hamd=torch.stack([padded(x, x, B, B) for x in sl], dim=1),
+               Hamld=padded(lo, sl[0], nl, B), Hamrd=padded(ro, sl[-1], nr, B))
+    out["Hamdv"] = (torch.stack([padded(sl[i], sl[i + 1], B, B) for i in range(L - 1)], dim=1) if L > 1 else
+                    torch.zeros((nk, 1, B, B), dtype=H.dtype, device=H.device))
+    out.update(nl=nl, nr=nr, b=B, L=L, nk=nk)
     return out
 
 
@@ -140,11 +152,11 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     configuration, as the reference opens it); H(k) of every k-point (moved
     into the Wigner-Seitz cell by k_grid_in_ws) is assembled on the GPU
     (gen_ham); the layers come from blockwise_setup on the first k-point's
-    H, as the reference's rank 0 does (tb/infortran.f90:398).  With equal
-    layer sizes every (E, k) point is solved in one batched call on blocks
-    sliced on the device (batched=None or True); otherwise, or with
-    batched=False, point by point (surface_green + G_block, the reference's
-    call sequence, tb/infortran.f90:492-504).  Writes transl_tot.dat /
+    H, as the reference's rank 0 does (tb/infortran.f90:398).  By default
+    every (E, k) point is solved in one batched call on blocks sliced on the
+    device (layers padded to the widest one, which leaves the transmission
+    unchanged); batched=False runs the reference's call sequence point by
+    point (surface_green + G_block, tb/infortran.f90:492-504).  Writes transl_tot.dat /
     transr_tot.dat in the tight-binding writer's format and positions_all.dat,
     nkpoints.dat when output_dir is given (defaults to the config's
     output_dir, relative to the configuration's folder).
@@ -175,11 +187,8 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     H = asm.assemble(eng, ks)
     layers = D.blockwise_setup(H[0].transpose(0, 1).cpu().numpy(), nl, nr, nbl, nbr)
     p = T.params("tb", eta=cfg["eta"], convergence=cfg["convergence"])
-    uniform = len(set(layers)) == 1
     if batched is None:
-        batched = uniform
-    if batched and not uniform:
-        raise ValueError(f"batched solve needs equal layers, blockwise_setup gave {layers}")
+        batched = True
     if batched:
         d = _tb_device_blocks(H, layers, nl, nr, nbl, nbr)
         kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)

@@ -18,7 +18,7 @@ CONFIGE = """&configlist
  left_primitive_cell_input = 'none', right_primitive_cell_input = 'none'
 /
 &setup
- period_left = 4, period_right = 4, buffer_left = 0, buffer_right = 0
+ period_left = {period}, period_right = {period}, buffer_left = 0, buffer_right = 0
 /
 &convergelist
  eta = {eta}, convergence = 1.0d-10
@@ -39,15 +39,17 @@ CONFIGE = """&configlist
 """
 
 
-def write_run(folder, nz=8, split=True, nk=(2, 2), emin=-1.5, emax=2.2, ne=5, eta="1.0d-5", path_mode=2,
+def write_run(folder, cells=(1, 1, 8), split=True, nk=(2, 2), emin=-1.5, emax=2.2, ne=5, eta="1.0d-5", path_mode=2,
               kpath=None):
-    lat, pos, sp = S.si_ge_superlattice(5.5, 1, 1, nz)
+    lat, pos, sp = S.si_ge_superlattice(5.5, *cells)
+    period = 4 * cells[0] * cells[1]
     if not split:
         sp = np.full_like(sp, 2)
     S.write_fdf(os.path.join(folder, "input.fdf"), lat, pos, sp)
     shutil.copy(PARAM, os.path.join(folder, "param"))
     with open(os.path.join(folder, "confige"), "w") as f:
-        f.write(CONFIGE.format(nk1=nk[0], nk2=nk[1], emin=emin, emax=emax, ne=ne, eta=eta, path_mode=path_mode))
+        f.write(CONFIGE.format(nk1=nk[0], nk2=nk[1], emin=emin, emax=emax, ne=ne, eta=eta, path_mode=path_mode,
+                               period=period))
     if kpath is not None:
         with open(os.path.join(folder, "kpath"), "w") as f:
             f.write(f"{len(kpath)}\n" + "".join(" ".join(str(x) for x in k) + "\n" for k in kpath))

@@ -18,7 +18,7 @@ CONFIGE = """&configlist
  left_primitive_cell_input = 'none', right_primitive_cell_input = 'none'
 /
 &setup
- period_left = {period}, period_right = {period}, buffer_left = 0, buffer_right = 0
+ period_left = {period}, period_right = {period}, buffer_left = {buffer}, buffer_right = {buffer}
 /
 &convergelist
  eta = {eta}, convergence = 1.0d-10
@@ -40,7 +40,7 @@ CONFIGE = """&configlist
 
 
 def write_run(folder, cells=(1, 1, 8), split=True, nk=(2, 2), emin=-1.5, emax=2.2, ne=5, eta="1.0d-5", path_mode=2,
-              kpath=None):
+              kpath=None, buffer=0):
     lat, pos, sp = S.si_ge_superlattice(5.5, *cells)
     period = 4 * cells[0] * cells[1]
     if not split:
@@ -49,7 +49,7 @@ def write_run(folder, cells=(1, 1, 8), split=True, nk=(2, 2), emin=-1.5, emax=2.
     shutil.copy(PARAM, os.path.join(folder, "param"))
     with open(os.path.join(folder, "confige"), "w") as f:
         f.write(CONFIGE.format(nk1=nk[0], nk2=nk[1], emin=emin, emax=emax, ne=ne, eta=eta, path_mode=path_mode,
-                               period=period))
+                               period=period, buffer=buffer))
     if kpath is not None:
         with open(os.path.join(folder, "kpath"), "w") as f:
             f.write(f"{len(kpath)}\n" + "".join(" ".join(str(x) for x in k) + "\n" for k in kpath))

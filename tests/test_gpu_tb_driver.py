@@ -57,16 +57,18 @@ def test_run_tb_matches_oracle(engine, Tm, tmp_path):
     assert open(os.path.join(out, "nkpoints.dat")).read() == "        4.0000000000\n"
 
 
-@pytest.mark.parametrize("cells", [(1, 1, 8), (2, 1, 8)])
-def test_run_tb_pointwise_equals_batched(engine, tmp_path, cells):
-    """equal one-atom layers, and unequal layers [90, 40, 80, 110] padded to 110
-    in the batched solve"""
-    cfg, *_ = R.write_run(str(tmp_path), nk=(1, 2), ne=3, cells=cells)
+@pytest.mark.parametrize("cells,buffer", [((1, 1, 8), 0), ((2, 1, 8), 0), ((1, 1, 10), 4)])
+def test_run_tb_pointwise_equals_batched(engine, tmp_path, cells, buffer):
+    """equal one-atom layers, unequal layers [90, 40, 80, 110] padded to 110
+    in the batched solve, and one lead cell of buffer on each side"""
+    cfg, *_ = R.write_run(str(tmp_path), nk=(1, 2), ne=3, cells=cells, buffer=buffer)
     print("run_tb batched vs point by point", cells, flush=True)
     a = driver.run_tb(cfg, engine=engine, output_dir=str(tmp_path / "a"))
     b = driver.run_tb(cfg, engine=engine, output_dir=str(tmp_path / "b"), batched=False)
     if cells == (2, 1, 8):
         assert a["layer_list"] == [90, 40, 80, 110]
+    if buffer:
+        assert sum(a["layer_list"]) == a["norb"] - 2 * 40 - 2 * 2 * 40
     assert (a["status"] == 0).all() and a["transl"].max() > 0.5
     assert np.abs(a["transl"] - b["transl"]).max() <= 1e-9 * max(1.0, np.abs(a["transl"]).max())
     assert np.abs(a["transr"] - b["transr"]).max() <= 1e-9 * max(1.0, np.abs(a["transr"]).max())

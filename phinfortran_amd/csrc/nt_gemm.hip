@@ -1,8 +1,8 @@
 // nt_gemm.hip -- batched complex GEMM on CDNA4 MFMA (FP64 and FP32).
 //
 // One workgroup (4 wave64) computes a 64x64 complex tile of one batch item.
-// Operand tiles of depth 16 are staged through LDS as separate real and
-// imaginary planes; each wave owns a 32x32 quadrant as 2x2 MFMA 16x16 tiles
+// Operand tiles of depth 8 are staged through two LDS buffers as separate
+// real and imaginary planes (one barrier per stage); each wave owns a 32x32 quadrant as 2x2 MFMA 16x16 tiles
 // and issues four real MFMAs per complex 16x16x4 step.
 //
 // The product is formed transposed (C^T = B^T A^T): the MFMA output layout
@@ -16,28 +16,34 @@ namespace {
 
 constexpr int BM = 64;
 constexpr int BN = 64;
-constexpr int BK = 16;
+constexpr int BK = 8;         // operand depth per LDS stage
 constexpr int NT = 256;
+constexpr int EPT = BM * BK / NT;   // operand elements each thread stages per tile and stage
 constexpr int NXCD = 8;       // accelerator complex dies of the MI355X
 
+// two stages of operand tiles: the MFMAs of one read the buffer the other's
+// loads are not writing, so each K-stage needs a single barrier
 template <typename T>
 struct GemmSmem {
-    T ar[BK][BM];
-    T ai[BK][BM];
-    T br[BK][BN];
-    T bi[BK][BN];
+    T ar[2][BK][BM];
+    T ai[2][BK][BM];
+    T br[2][BK][BN];
+    T bi[2][BK][BN];
 };
 
+// Element j of thread t.  op N (A) / op C (B): row r = t % 64 (contiguous in
+// memory), depth k = t / 64 + 4 j.  op C (A) / op N (B): depth k = t % BK
+// (contiguous), row r = t / BK + (NT / BK) j.
 template <typename T>
 __device__ __forceinline__ void load_a_regs(const T* __restrict__ A, int lda, int opA, int M, int K,
-                                            int m0, int k0, T (&r)[4], T (&i)[4])
+                                            int m0, int k0, T (&r)[EPT], T (&i)[EPT])
 {
     const int t = threadIdx.x;
     typedef typename vec2<T>::type V2;
     if (opA == OP_N) {
         const int m = t & 63, kq = t >> 6;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < EPT; ++j) {
             const int k = kq + 4 * j;
             const int gm = m0 + m, gk = k0 + k;
             V2 v = {0, 0};
@@ -46,10 +52,10 @@ __device__ __forceinline__ void load_a_regs(const T* __restrict__ A, int lda, in
             i[j] = v.y;
         }
     } else {
-        const int k = t & 15, mq = t >> 4;
+        const int k = t % BK, mq = t / BK;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int m = mq + 16 * j;
+        for (int j = 0; j < EPT; ++j) {
+            const int m = mq + (NT / BK) * j;
             const int gm = m0 + m, gk = k0 + k;
             V2 v = {0, 0};
             if (gm < M && gk < K) v = *reinterpret_cast<const V2*>(A + 2 * ((long)gk + (long)gm * lda));
@@ -60,37 +66,38 @@ __device__ __forceinline__ void load_a_regs(const T* __restrict__ A, int lda, in
 }
 
 template <typename T>
-__device__ __forceinline__ void store_a_smem(GemmSmem<T>& s, int opA, const T (&r)[4], const T (&i)[4])
+__device__ __forceinline__ void store_a_smem(GemmSmem<T>& s, int buf, int opA, const T (&r)[EPT],
+                                             const T (&i)[EPT])
 {
     const int t = threadIdx.x;
     if (opA == OP_N) {
         const int m = t & 63, kq = t >> 6;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            s.ar[kq + 4 * j][m] = r[j];
-            s.ai[kq + 4 * j][m] = i[j];
+        for (int j = 0; j < EPT; ++j) {
+            s.ar[buf][kq + 4 * j][m] = r[j];
+            s.ai[buf][kq + 4 * j][m] = i[j];
         }
     } else {
-        const int k = t & 15, mq = t >> 4;
+        const int k = t % BK, mq = t / BK;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            s.ar[k][mq + 16 * j] = r[j];
-            s.ai[k][mq + 16 * j] = i[j];
+        for (int j = 0; j < EPT; ++j) {
+            s.ar[buf][k][mq + (NT / BK) * j] = r[j];
+            s.ai[buf][k][mq + (NT / BK) * j] = i[j];
         }
     }
 }
 
 template <typename T>
 __device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, int opB, int K, int N,
-                                            int k0, int n0, T (&r)[4], T (&i)[4])
+                                            int k0, int n0, T (&r)[EPT], T (&i)[EPT])
 {
     const int t = threadIdx.x;
     typedef typename vec2<T>::type V2;
     if (opB == OP_N) {
-        const int k = t & 15, nq = t >> 4;
+        const int k = t % BK, nq = t / BK;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = nq + 16 * j;
+        for (int j = 0; j < EPT; ++j) {
+            const int n = nq + (NT / BK) * j;
             const int gk = k0 + k, gn = n0 + n;
             V2 v = {0, 0};
             if (gk < K && gn < N) v = *reinterpret_cast<const V2*>(B + 2 * ((long)gk + (long)gn * ldb));
@@ -100,7 +107,7 @@ __device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, in
     } else {
         const int n = t & 63, kq = t >> 6;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < EPT; ++j) {
             const int k = kq + 4 * j;
             const int gk = k0 + k, gn = n0 + n;
             V2 v = {0, 0};
@@ -112,22 +119,23 @@ __device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, in
 }
 
 template <typename T>
-__device__ __forceinline__ void store_b_smem(GemmSmem<T>& s, int opB, const T (&r)[4], const T (&i)[4])
+__device__ __forceinline__ void store_b_smem(GemmSmem<T>& s, int buf, int opB, const T (&r)[EPT],
+                                             const T (&i)[EPT])
 {
     const int t = threadIdx.x;
     if (opB == OP_N) {
-        const int k = t & 15, nq = t >> 4;
+        const int k = t % BK, nq = t / BK;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            s.br[k][nq + 16 * j] = r[j];
-            s.bi[k][nq + 16 * j] = i[j];
+        for (int j = 0; j < EPT; ++j) {
+            s.br[buf][k][nq + (NT / BK) * j] = r[j];
+            s.bi[buf][k][nq + (NT / BK) * j] = i[j];
         }
     } else {
         const int n = t & 63, kq = t >> 6;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            s.br[kq + 4 * j][n] = r[j];
-            s.bi[kq + 4 * j][n] = i[j];
+        for (int j = 0; j < EPT; ++j) {
+            s.br[buf][kq + 4 * j][n] = r[j];
+            s.bi[buf][kq + 4 * j][n] = i[j];
         }
     }
 }
@@ -182,14 +190,15 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
             accS[x][y] = A4{0, 0, 0, 0};
         }
 
-    T ra[4], ia[4], rb[4], ib[4];
+    T ra[EPT], ia[EPT], rb[EPT], ib[EPT];
     const int nst = (a.K + BK - 1) / BK;
     load_a_regs(A, a.A.ld, a.opA, a.M, a.K, m0, 0, ra, ia);
     load_b_regs(B, a.B.ld, a.opB, a.K, a.N, 0, n0, rb, ib);
-    store_a_smem(s, a.opA, ra, ia);
-    store_b_smem(s, a.opB, rb, ib);
+    store_a_smem(s, 0, a.opA, ra, ia);
+    store_b_smem(s, 0, a.opB, rb, ib);
     __syncthreads();
     for (int st = 0; st < nst; ++st) {
+        const int cur = st & 1;
         const bool more = st + 1 < nst;
         if (more) {
             load_a_regs(A, a.A.ld, a.opA, a.M, a.K, m0, (st + 1) * BK, ra, ia);
@@ -201,13 +210,13 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
             T xr[2], xi[2], yr[2], yi[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
-                xr[x] = s.ar[k][wm + 16 * x + lr];
-                xi[x] = s.ai[k][wm + 16 * x + lr];
+                xr[x] = s.ar[cur][k][wm + 16 * x + lr];
+                xi[x] = s.ai[cur][k][wm + 16 * x + lr];
             }
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
-                yr[y] = s.br[k][wn + 16 * y + lr];
-                yi[y] = s.bi[k][wn + 16 * y + lr];
+                yr[y] = s.br[cur][k][wn + 16 * y + lr];
+                yi[y] = s.bi[cur][k][wn + 16 * y + lr];
             }
             if constexpr (M3) {
                 T xs[2], ys[2];
@@ -237,12 +246,12 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
                     }
             }
         }
-        __syncthreads();
+        // the other buffer was last read in the previous stage, before its barrier
         if (more) {
-            store_a_smem(s, a.opA, ra, ia);
-            store_b_smem(s, a.opB, rb, ib);
-            __syncthreads();
+            store_a_smem(s, cur ^ 1, a.opA, ra, ia);
+            store_b_smem(s, cur ^ 1, a.opB, rb, ib);
         }
+        __syncthreads();
     }
 
     // Epilogue.  Lane holds D[n = mfma_out_row(lane, r)][m = lr] of each sub-tile.

@@ -82,3 +82,20 @@ def test_run_tb_pristine_si_integer_channels(engine, tmp_path):
     assert np.abs(t - np.round(t)).max() < 2e-3, t
     assert np.abs(r["transl"] - r["transr"]).max() < 1e-6
     assert t[0] >= 1 and t[-1] >= 1
+
+
+def test_run_tb_kpath_mode(engine, Tm, tmp_path):
+    """path_mode = 3: the k-points of kpath_file (third component set to 0,
+    then moved into the Wigner-Seitz cell) against the oracle chain"""
+    kp = [[0.1, 0.2, 0.7], [0.45, -0.3, 0.0], [0.8, 0.6, 0.1]]
+    cfg, lat, pos, sp = R.write_run(str(tmp_path), path_mode=3, kpath=kp, ne=3)
+    print("run_tb kpath", flush=True)
+    r = driver.run_tb(cfg, engine=engine)
+    assert r["k"].shape == (3, 3) and np.all(r["k"][:, 2] == 0.0)
+    assert np.allclose(r["k"][2, :2], [-0.2, -0.4])         # (0.8, 0.6) moved by (1, 1)
+    P = tb.read_param(R.PARAM)
+    for ik, k in enumerate(r["k"]):
+        for ie, E in enumerate(r["E"]):
+            tl, tr = oracle_T(Tm, P, lat, pos, sp, k, E, 1e-5, r["layer_list"], 40, 40)
+            assert abs(r["transl"][ie, ik] - tl) <= 1e-8 * max(1.0, tl), (ik, E)
+            assert abs(r["transr"][ie, ik] - tr) <= 1e-8 * max(1.0, tr), (ik, E)

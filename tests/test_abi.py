@@ -116,3 +116,16 @@ def test_fortran_interfaces_match_header():
     for m in found:
         args = [a for a in re.sub(r"&\s*\n", " ", m.group(1)).replace("&", " ").split(",") if a.strip()]
         assert len(args) == counts[m.group(2)], (m.group(2), len(args), counts[m.group(2)])
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """without the built HIP library the product path raises instead of
+    falling back to anything on the CPU"""
+    from phinfortran_amd import _native as N
+    from phinfortran_amd import transport as T
+    monkeypatch.setattr(N, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(N, "_lib", None)
+    with pytest.raises(ImportError, match="no CPU fallback"):
+        N.lib()
+    with pytest.raises(ImportError):
+        T.params("tb")

@@ -195,7 +195,8 @@ typedef struct {
     nt_c128* gl_adv;  nt_c128* gr_ret;  nt_c128* gl_plus; nt_c128* gr_minus;
     nt_c128* gml_adv; nt_c128* gmr_ret; nt_c128* sl_adv;  nt_c128* sr_ret;
     nt_c128* G_D_LR;  nt_c128* G_D_RL;  nt_c128* G_L;     nt_c128* G_R;
-    double* sdos_l;   double* sdos_r;   /* surface_ldos_mpi (negf/surface.f90:608-630) */
+    double* sdos_l;   double* sdos_r;   /* surface_ldos_mpi: +-2 sqrt(E) Im Tr g / pi (negf/surface.f90:608-630);
+                                           with eta_mode = NT_ETA_ABSOLUTE (tb) +-Im Tr g / pi (tb/surface.f90:345-366) */
 } nt_solve_extras;
 
 /* Device-pointer variant: every array is device memory (e.g. torch tensors). */

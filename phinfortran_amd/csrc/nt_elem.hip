@@ -163,6 +163,7 @@ __global__ __launch_bounds__(NT) void k_trace_prod(int n1, int n2, BMat<T> X, BM
 }
 
 template <typename T>
+// out[p] = coef sqrt(E[p]) Im Tr g[p] / pi; without E (null), coef Im Tr g[p] / pi
 __global__ __launch_bounds__(NT) void k_trace_im(int n, BMat<T> g, const double* E, double coef, double* out, int batch)
 {
     const int p = blockIdx.x;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(NT) void k_trace_im(int n, BMat<T> g, const double*
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < NT / 64; ++w) s += red[w];
-        out[p] = coef * sqrt(E[p]) * s / 3.14159265358979323846;
+        out[p] = coef * (E ? sqrt(E[p]) : 1.0) * s / 3.14159265358979323846;
     }
 }
 

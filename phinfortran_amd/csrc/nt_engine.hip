@@ -1715,13 +1715,20 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         }
         if (a.ex.sdos_l) check_range(c, a.ex.sdos_l + p0, sizeof(double) * P, "sdos_l");
         if (a.ex.sdos_r) check_range(c, a.ex.sdos_r + p0, sizeof(double) * P, "sdos_r");
+        // surface DOS of the driver family the eta convention belongs to:
+        // negf +-2 sqrt(E) Im Tr g / pi (negf/surface.f90:608-630), tb
+        // +-Im Tr g / pi (tb/surface.f90:345-366)
+        const bool tb_dos = prm->eta_mode == NT_ETA_ABSOLUTE;
+        const double* Edos = tb_dos ? nullptr : E;
+        const double cdos = tb_dos ? 1.0 : 2.0;
         if (a.ex.sdos_l)
             launch(c, NT_K_ELEM, 0, "sdos", [&] {
-                return launch_trace_im<T>(nl, bm<T>(gbuf, 2 * nll, nl), E, 2.0, a.ex.sdos_l + p0, P, s);
+                return launch_trace_im<T>(nl, bm<T>(gbuf, 2 * nll, nl), Edos, cdos, a.ex.sdos_l + p0, P, s);
             });
         if (a.ex.sdos_r)
             launch(c, NT_K_ELEM, 0, "sdos", [&] {
-                return launch_trace_im<T>(nr, bm<T>(gbuf + 2 * nrr, 2 * nrr, nr), E, -2.0, a.ex.sdos_r + p0, P, s);
+                return launch_trace_im<T>(nr, bm<T>(gbuf + 2 * nrr, 2 * nrr, nr), Edos, -cdos, a.ex.sdos_r + p0, P,
+                                          s);
             });
         // ---- status, iterations, clamps
         hipLaunchKernelGGL(k_point_status, dim3(nblk(P)), dim3(256), 0, s, stdec, itdec, strgf, P,

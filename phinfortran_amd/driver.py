@@ -161,8 +161,8 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     nkpoints.dat when output_dir is given (defaults to the config's
     output_dir, relative to the configuration's folder).
 
-    Returns dict(E (ne), k (nk x 3, as used), transl, transr (ne x nk),
-    status, layer_list, norb, nl, nr)."""
+    Returns dict(E (ne), k (nk x 3, as used), transl, transr, sdos_l, sdos_r
+    (ne x nk), status, layer_list, norb, nl, nr)."""
     import torch
     from . import dense as D
     from . import tb as TBM
@@ -192,17 +192,22 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     if batched:
         d = _tb_device_blocks(H, layers, nl, nr, nbl, nbr)
         kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)
-        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx)
+        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, extras=("sdos_l", "sdos_r"))
         tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
         tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
         status = r["status"].cpu().numpy().reshape(nk, ne).T
+        sdl = r["sdos_l"].cpu().numpy().reshape(nk, ne).T
+        sdr = r["sdos_r"].cpu().numpy().reshape(nk, ne).T
     else:
         tl, tr = np.zeros((ne, nk)), np.zeros((ne, nk))
+        sdl, sdr = np.zeros((ne, nk)), np.zeros((ne, nk))
         status = np.zeros((ne, nk), np.int32)
         for ik in range(nk):
             s = D.slices(H[ik].transpose(0, 1).cpu().numpy(), layers, nl, nr, nbl, nbr, variant="tb")
             for ie in range(ne):
                 g = eng.surface_green_tdb(s.Haml, s.Hamr, s.Hamlv, s.Hamrv, E[ie], p, want_plus_minus=False)
+                sdl[ie, ik] = np.trace(g["gl_adv"]).imag / np.pi          # tb/surface.f90:363-364
+                sdr[ie, ik] = -np.trace(g["gr_ret"]).imag / np.pi
                 o = eng.G_block(g["gl_adv"].conj().T, g["gr_ret"], -g["gml_adv"], g["gmr_ret"],
                                 g["sl_adv"].conj().T, g["sr_ret"], E[ie], s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml,
                                 s.Hamr, p, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
@@ -212,7 +217,9 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
         out_dir = os.path.join(base, cfg["output_dir"])
     if out_dir:
         TBM.write_positions(out_dir, st["positions"], st["species"], P["orb_num"], layers, nbl, nl, nk)
+        TBM.write_surface_ldos(out_dir, sdl, sdr)
         OUT.write_trans(out_dir, np.repeat(E[:, None], nk, axis=1), tl, tr, tb=True)
     if engine is None:
         eng.close()
-    return dict(E=E, k=ks, transl=tl, transr=tr, status=status, layer_list=layers, norb=asm.norb, nl=nl, nr=nr)
+    return dict(E=E, k=ks, transl=tl, transr=tr, sdos_l=sdl, sdos_r=sdr, status=status, layer_list=layers,
+                norb=asm.norb, nl=nl, nr=nr)

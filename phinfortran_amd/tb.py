@@ -259,6 +259,17 @@ def write_positions(output_dir: str, positions, species, orb_num, layer_list, n_
             f.write("".join("%20.10f" % x for x in r) + "\n")
 
 
+def write_surface_ldos(output_dir: str, sdosl, sdosr):
+    """write_surface_ldos (tb/write_data.f90:278-308): sdosl.dat / sdosr.dat,
+    one row per energy of f20.10 values (one per k-point) ending in a blank,
+    appended to existing files as the reference opens them."""
+    os.makedirs(output_dir, exist_ok=True)
+    for name, a in (("sdosl.dat", sdosl), ("sdosr.dat", sdosr)):
+        with open(os.path.join(output_dir, name), "a") as f:
+            for row in np.asarray(a, float):
+                f.write("".join("%20.10f" % v for v in row) + " \n")
+
+
 # ------------------------------------------------------------------ Slater-Koster
 def _angular(A: int, B: int, l, m, n):
     """f_t for the canonical shell pair (A, B): dict t -> array

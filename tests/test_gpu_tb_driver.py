@@ -33,6 +33,7 @@ def oracle_T(Tm, P, lat, pos, sp, k, E, eta, layers, nl, nr):
     r = O.surface_green(s.Haml, s.Hamlv, s.Hamr, s.Hamrv, E, op, want_plus_minus=False)
     o = O.rgf(s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml, s.Hamr, r["gl_adv"].conj().T, r["gr_ret"], -r["gml_adv"],
               r["gmr_ret"], r["sl_adv"].conj().T, r["sr_ret"], E, op, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
+    oracle_T.sdos = (np.trace(r["gl_adv"]).imag / np.pi, -np.trace(r["gr_ret"]).imag / np.pi)
     return o["T_LR"], o["T_RL"]
 
 
@@ -48,12 +49,19 @@ def test_run_tb_matches_oracle(engine, Tm, tmp_path):
             tl, tr = oracle_T(Tm, P, lat, pos, sp, k, E, 1e-5, r["layer_list"], 40, 40)
             assert abs(r["transl"][ie, ik] - tl) <= 1e-8 * max(1.0, tl), (ik, E)
             assert abs(r["transr"][ie, ik] - tr) <= 1e-8 * max(1.0, tr), (ik, E)
+            sl_, sr_ = oracle_T.sdos                     # surface DOS: +-Im Tr g / pi, no sqrt(E)
+            assert abs(r["sdos_l"][ie, ik] - sl_) <= 1e-8 * max(1.0, abs(sl_)), (ik, E)
+            assert abs(r["sdos_r"][ie, ik] - sr_) <= 1e-8 * max(1.0, abs(sr_)), (ik, E)
+    assert (r["sdos_l"] > 0).any() and (r["sdos_l"] >= -1e-8).all()
     assert r["transl"].max() > 0.5 and r["transl"][2].max() < 1e-10            # E = 0.35 eV lies in the gap
     out = os.path.join(str(tmp_path), "out")
     rows = open(os.path.join(out, "transl_tot.dat")).read().splitlines()
     assert len(rows) == 5 and all(len(x) == 20 * 5 for x in rows)
     assert float(rows[3][20:40]) == pytest.approx(r["transl"][3, 0], rel=1e-9)
     assert len(open(os.path.join(out, "positions_all.dat")).read().splitlines()) == 320
+    srows = open(os.path.join(out, "sdosl.dat")).read().splitlines()
+    assert len(srows) == 5 and all(len(x) == 20 * 4 + 1 for x in srows)
+    assert float(srows[0][:20]) == pytest.approx(r["sdos_l"][0, 0], abs=1e-9)
     assert open(os.path.join(out, "nkpoints.dat")).read() == "        4.0000000000\n"
 
 
@@ -72,6 +80,8 @@ def test_run_tb_pointwise_equals_batched(engine, tmp_path, cells, buffer):
     assert (a["status"] == 0).all() and a["transl"].max() > 0.5
     assert np.abs(a["transl"] - b["transl"]).max() <= 1e-9 * max(1.0, np.abs(a["transl"]).max())
     assert np.abs(a["transr"] - b["transr"]).max() <= 1e-9 * max(1.0, np.abs(a["transr"]).max())
+    assert np.abs(a["sdos_l"] - b["sdos_l"]).max() <= 1e-9 * max(1.0, np.abs(a["sdos_l"]).max())
+    assert np.abs(a["sdos_r"] - b["sdos_r"]).max() <= 1e-9 * max(1.0, np.abs(a["sdos_r"]).max())
 
 
 def test_run_tb_pristine_si_integer_channels(engine, tmp_path):

@@ -85,6 +85,8 @@ def main():
                     help="measure only the tight-binding H(k) assembly (for kernel profiles)")
     ap.add_argument("--no-tb-run", dest="tb_run", action="store_false",
                     help="skip the tight-binding transport run measurement")
+    ap.add_argument("--tb-run-only", action="store_true",
+                    help="measure only the tight-binding transport run (for kernel profiles)")
     ap.add_argument("--tb-hk-k", type=int, default=64, help="k-points per tight-binding H(k) launch")
     ap.add_argument("--modes-cpu-sample", type=int, default=6,
                     help="points of the mode-resolved CPU baseline sample (0 = skip)")
@@ -180,6 +182,72 @@ def main():
                                              f"Python loops), {dt:.1f} s"}
         return res
 
+    def run_tb_transport():
+        """The tight-binding driver (driver.run_tb) on the reference's example
+        structure: Si/Ge superlattice 2 x 2 x 8 cells (1280 orbitals), one
+        16-atom cell layer per lead period, 4 x 4 k-points x 64 energies; the
+        oracle chain (surface GFs + RGF on the same slices) on a sample"""
+        import tempfile
+        from phinfortran_amd import driver as DRV
+        from phinfortran_amd import tb
+        from phinfortran_amd import dense as D
+        here = os.path.dirname(os.path.abspath(__file__))
+        with tempfile.TemporaryDirectory() as tmp:
+            st = tb.diamond_superlattice(5.5, 2, 2, 8)
+            with open(os.path.join(tmp, "input.fdf"), "w") as f:
+                f.write(f"NumberOfAtoms {len(st['positions'])}\nNumberOfSpecies 2\nLatticeConstant 1.0 Ang\n"
+                        "%block LatticeVectors\n")
+                for r in st["latvec"]:
+                    f.write(" ".join(f"{x:.13f}" for x in r) + "\n")
+                f.write("%endblock LatticeVectors\n%block AtomicCoordinatesAndAtomicSpecies\n")
+                for x, sp in zip(st["positions"], st["species"]):
+                    f.write(" ".join(f"{v:.13f}" for v in x) + f" {sp}\n")
+                f.write("%endblock AtomicCoordinatesAndAtomicSpecies\n")
+            with open(os.path.join(tmp, "confige"), "w") as f:
+                f.write("&configlist\n k_trans_mesh = 4 4 1, emin = -1.5, emax = 2.2, ne = 64\n"
+                        " r_cutoff = 2.6, filename_input = 'input.fdf'\n/\n"
+                        "&setup\n period_left = 16, period_right = 16, buffer_left = 0, buffer_right = 0\n/\n"
+                        "&convergelist\n eta = 1.0d-5, convergence = 1.0d-10\n/\n"
+                        "&unfolding\n n_bloch_x = 1, n_bloch_y = 1, circular_flag = 0\n/\n"
+                        "&kpath\n path_mode = 2\n/\n&saving\n output_dir = '', verbosity = 0\n/\n")
+            prm = os.path.join(here, "tests", "golden", "tb", "param")
+            DRV.run_tb(os.path.join(tmp, "confige"), engine=eng, param_path=prm)        # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = DRV.run_tb(os.path.join(tmp, "confige"), engine=eng, param_path=prm)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+        npt = r["transl"].size
+        res = {"points_per_s": npt / el, "unit": "(E,k)-points/s", "points": npt, "seconds": el,
+               "norb": r["norb"], "nl": r["nl"], "layer_list": r["layer_list"],
+               "status_ok": bool((r["status"] == 0).all()),
+               "what": "whole run: table, H(k) of 16 k-points on the GPU, blockwise_setup (host), device "
+                       "slicing, one batched solve (layers padded to the widest), FP64"}
+        log(f"tb run: {npt} points in {el:.2f}s ({npt / el:.1f} points/s), layers {r['layer_list']}")
+        if args.cpu_sample > 0:
+            import oracle as O
+            from oracle import tb_ham as OT
+            P = tb.read_param(prm)
+            p = T.params("tb", eta=1e-5, convergence=1e-10)
+            op = O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
+            t0 = time.perf_counter()
+            Hk = OT.gen_ham(P, st["positions"], st["species"], st["latvec"], r["k"][5], 2.6)
+            sl = D.slices(Hk, r["layer_list"], r["nl"], r["nr"], 0, 0, variant="tb")
+            for e in r["E"][::16]:
+                g = O.surface_green(sl.Haml, sl.Hamlv, sl.Hamr, sl.Hamrv, e, op, want_plus_minus=False)
+                O.rgf(sl.Hd, sl.Hdv, sl.Hld, sl.Hrd, sl.Haml, sl.Hamr, g["gl_adv"].conj().T, g["gr_ret"],
+                      -g["gml_adv"], g["gmr_ret"], g["sl_adv"].conj().T, g["sr_ret"], e, op,
+                      Hdv_back=sl.Hdv_back, Hrd_fwd=sl.Hrd_fwd)
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": 4 / dt, "unit": "(E,k)-points/s", "cores": 1, "kind": "port",
+                                   "sample": f"4 energies of one k-point: the oracle's gen_ham (Python) and the C "
+                                             f"oracle's decimation + RGF on the same layers, {dt:.1f} s"}
+        return res
+
+    if args.tb_run_only:
+        print(json.dumps({"tb_transport_run": run_tb_transport()}))
+        eng.close()
+        return
     if args.tb_hk_only:
         print(json.dumps({"tb_hk_assembly": run_tb_hk()}))
         eng.close()
@@ -398,68 +466,6 @@ def main():
     probes = run_buttiker() if args.buttiker else None
 
     tbhk = run_tb_hk() if args.tb_hk else None
-
-    def run_tb_transport():
-        """The tight-binding driver (driver.run_tb) on the reference's example
-        structure: Si/Ge superlattice 2 x 2 x 8 cells (1280 orbitals), one
-        16-atom cell layer per lead period, 4 x 4 k-points x 64 energies; the
-        oracle chain (surface GFs + RGF on the same slices) on a sample"""
-        import tempfile
-        from phinfortran_amd import driver as DRV
-        from phinfortran_amd import tb
-        from phinfortran_amd import dense as D
-        here = os.path.dirname(os.path.abspath(__file__))
-        with tempfile.TemporaryDirectory() as tmp:
-            st = tb.diamond_superlattice(5.5, 2, 2, 8)
-            with open(os.path.join(tmp, "input.fdf"), "w") as f:
-                f.write(f"NumberOfAtoms {len(st['positions'])}\nNumberOfSpecies 2\nLatticeConstant 1.0 Ang\n"
-                        "%block LatticeVectors\n")
-                for r in st["latvec"]:
-                    f.write(" ".join(f"{x:.13f}" for x in r) + "\n")
-                f.write("%endblock LatticeVectors\n%block AtomicCoordinatesAndAtomicSpecies\n")
-                for x, sp in zip(st["positions"], st["species"]):
-                    f.write(" ".join(f"{v:.13f}" for v in x) + f" {sp}\n")
-                f.write("%endblock AtomicCoordinatesAndAtomicSpecies\n")
-            with open(os.path.join(tmp, "confige"), "w") as f:
-                f.write("&configlist\n k_trans_mesh = 4 4 1, emin = -1.5, emax = 2.2, ne = 64\n"
-                        " r_cutoff = 2.6, filename_input = 'input.fdf'\n/\n"
-                        "&setup\n period_left = 16, period_right = 16, buffer_left = 0, buffer_right = 0\n/\n"
-                        "&convergelist\n eta = 1.0d-5, convergence = 1.0d-10\n/\n"
-                        "&unfolding\n n_bloch_x = 1, n_bloch_y = 1, circular_flag = 0\n/\n"
-                        "&kpath\n path_mode = 2\n/\n&saving\n output_dir = '', verbosity = 0\n/\n")
-            prm = os.path.join(here, "tests", "golden", "tb", "param")
-            DRV.run_tb(os.path.join(tmp, "confige"), engine=eng, param_path=prm)        # warm-up
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            r = DRV.run_tb(os.path.join(tmp, "confige"), engine=eng, param_path=prm)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - t0
-        npt = r["transl"].size
-        res = {"points_per_s": npt / el, "unit": "(E,k)-points/s", "points": npt, "seconds": el,
-               "norb": r["norb"], "nl": r["nl"], "layer_list": r["layer_list"],
-               "status_ok": bool((r["status"] == 0).all()),
-               "what": "whole run: table, H(k) of 16 k-points on the GPU, blockwise_setup (host), device "
-                       "slicing, one batched solve (layers padded to the widest), FP64"}
-        log(f"tb run: {npt} points in {el:.2f}s ({npt / el:.1f} points/s), layers {r['layer_list']}")
-        if args.cpu_sample > 0:
-            import oracle as O
-            from oracle import tb_ham as OT
-            P = tb.read_param(prm)
-            p = T.params("tb", eta=1e-5, convergence=1e-10)
-            op = O.OrParams(p.eta, p.eta0, p.convergence, p.device_eta_factor, p.eta_mode, p.max_iter)
-            t0 = time.perf_counter()
-            Hk = OT.gen_ham(P, st["positions"], st["species"], st["latvec"], r["k"][5], 2.6)
-            sl = D.slices(Hk, r["layer_list"], r["nl"], r["nr"], 0, 0, variant="tb")
-            for e in r["E"][::16]:
-                g = O.surface_green(sl.Haml, sl.Hamlv, sl.Hamr, sl.Hamrv, e, op, want_plus_minus=False)
-                O.rgf(sl.Hd, sl.Hdv, sl.Hld, sl.Hrd, sl.Haml, sl.Hamr, g["gl_adv"].conj().T, g["gr_ret"],
-                      -g["gml_adv"], g["gmr_ret"], g["sl_adv"].conj().T, g["sr_ret"], e, op,
-                      Hdv_back=sl.Hdv_back, Hrd_fwd=sl.Hrd_fwd)
-            dt = time.perf_counter() - t0
-            res["cpu_baseline"] = {"value": 4 / dt, "unit": "(E,k)-points/s", "cores": 1, "kind": "port",
-                                   "sample": f"4 energies of one k-point: the oracle's gen_ham (Python) and the C "
-                                             f"oracle's decimation + RGF on the same layers, {dt:.1f} s"}
-        return res
 
     tbrun = run_tb_transport() if args.tb_run else None
     t64 = np.concatenate(T64)

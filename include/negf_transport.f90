@@ -20,10 +20,12 @@ module negf_transport
                                  NT_ERR_ARG = -3, NT_ERR_HIP = -4, NT_ERR_NOMEM = -5
     integer(c_int), parameter :: NT_ETA_RELATIVE = 0, NT_ETA_ABSOLUTE = 1
     integer(c_int), parameter :: NT_PREC_FP64 = 0, NT_PREC_MIXED = 1
+    integer(c_int), parameter :: NT_DOS_FROM_ETA = 0, NT_DOS_NEGF = 1, NT_DOS_TB = 2
 
     type, bind(C) :: nt_params
         real(c_double) :: eta, eta0, convergence, device_eta_factor, lambda_eta
         integer(c_int) :: eta_mode, max_iter, precision, circular, n_bloch_x, n_bloch_y, path_mode
+        integer(c_int) :: dos_mode
     end type nt_params
 
     type, bind(C) :: nt_solve_extras

@@ -68,11 +68,21 @@ typedef struct {
     int path_mode;            /* &kpath path_mode: 1 adds the degenerate-subspace rotation of
                                  the mode-resolved solve (negf/velocity_mat.f90:1884-1978);
                                  any other value runs as the example's path_mode = 2 */
+    int dos_mode;             /* surface DOS convention of the sdos_l / sdos_r outputs:
+                                 NT_DOS_FROM_ETA (default) follows eta_mode (relative: negf,
+                                 absolute: tb), NT_DOS_NEGF and NT_DOS_TB choose explicitly */
 } nt_params;
+
+/* Surface DOS conventions (nt_params.dos_mode). */
+enum {
+    NT_DOS_FROM_ETA = 0,   /* NT_DOS_NEGF when eta_mode is relative, NT_DOS_TB when absolute */
+    NT_DOS_NEGF = 1,       /* +-2 sqrt(E) Im Tr g / pi, E = omega^2   (negf/surface.f90:608-630) */
+    NT_DOS_TB = 2          /* +-Im Tr g / pi                          (tb/surface.f90:345-366)   */
+};
 
 /* Fills the negf defaults: eta = eta0 = 1e-6 (config.f90:88-89), device x10,
  * convergence 1e-12, lambda_eta 5e-3, max_iter 300, FP64, direct decimation
- * (circular = 0, n_bloch_x = n_bloch_y = 1), path_mode 2.
+ * (circular = 0, n_bloch_x = n_bloch_y = 1), path_mode 2, dos_mode NT_DOS_FROM_ETA.
  * With circular = 1 the lead blocks (n x n) must be block-circulant over
  * n_bloch_y x n_bloch_x transverse copies (copy index y outer, x inner); each
  * lead GF is then computed as in circular_g (negf/circular.f90:43-114): a

@@ -64,26 +64,35 @@ void or_gemm(char opa, char opb, int m, int n, int k, ocplx alpha,
     for (int j = 0; j < n; ++j)
         for (int p = 0; p < k; ++p)
             bb[p + (size_t)j * k] = (opb == 'N') ? B[p + (size_t)j * ldb] : conj(B[j + (size_t)p * ldb]);
-    double* accr = (double*)malloc(sizeof(double) * (m ? m : 1));
-    double* acci = (double*)malloc(sizeof(double) * (m ? m : 1));
-    for (int j = 0; j < n; ++j) {
-        for (int i = 0; i < m; ++i) { accr[i] = 0.0; acci[i] = 0.0; }
-        for (int p = 0; p < k; ++p) {
-            double br = RE(bb[p + (size_t)j * k]), bi = IM(bb[p + (size_t)j * k]);
-            const ocplx* ac = a + (size_t)p * m;
+    /* Columns of C are independent: large products (the b = 768 cases) run
+     * them over the OpenMP team; inside the per-point parallel loop of
+     * or_solve_tdb_batch the region is inactive and runs on one thread. */
+    const int par = (double)m * n * k >= 2.0e7 && !omp_in_parallel();
+#pragma omp parallel if (par)
+    {
+        double* accr = (double*)malloc(sizeof(double) * (m ? m : 1));
+        double* acci = (double*)malloc(sizeof(double) * (m ? m : 1));
+#pragma omp for schedule(static)
+        for (int j = 0; j < n; ++j) {
+            for (int i = 0; i < m; ++i) { accr[i] = 0.0; acci[i] = 0.0; }
+            for (int p = 0; p < k; ++p) {
+                double br = RE(bb[p + (size_t)j * k]), bi = IM(bb[p + (size_t)j * k]);
+                const ocplx* ac = a + (size_t)p * m;
+                for (int i = 0; i < m; ++i) {
+                    double ar = RE(ac[i]), ai = IM(ac[i]);
+                    accr[i] += ar * br - ai * bi;
+                    acci[i] += ar * bi + ai * br;
+                }
+            }
             for (int i = 0; i < m; ++i) {
-                double ar = RE(ac[i]), ai = IM(ac[i]);
-                accr[i] += ar * br - ai * bi;
-                acci[i] += ar * bi + ai * br;
+                ocplx prod = MKC(accr[i], acci[i]);
+                ocplx c = (RE(beta) == 0.0 && IM(beta) == 0.0) ? 0.0 : cmul(beta, C[i + (size_t)j * ldc]);
+                C[i + (size_t)j * ldc] = cmul(alpha, prod) + c;
             }
         }
-        for (int i = 0; i < m; ++i) {
-            ocplx prod = MKC(accr[i], acci[i]);
-            ocplx c = (RE(beta) == 0.0 && IM(beta) == 0.0) ? 0.0 : cmul(beta, C[i + (size_t)j * ldc]);
-            C[i + (size_t)j * ldc] = cmul(alpha, prod) + c;
-        }
+        free(accr); free(acci);
     }
-    free(accr); free(acci); free(a); free(bb);
+    free(a); free(bb);
 }
 
 /* C = op(A) op(B) (fresh output) */
@@ -107,6 +116,8 @@ int or_inv(int n, const ocplx* A, ocplx* Ainv)
     int* piv = (int*)malloc(sizeof(int) * (n ? n : 1));
     memcpy(LU, A, sizeof(ocplx) * nn);
     int info = 0;
+    /* columns in parallel for the large blocks only (same arithmetic per entry) */
+    const int par = n >= 320 && !omp_in_parallel();
     for (int c = 0; c < n; ++c) {
         int p = c;
         double best = cabs1(LU[c + (size_t)c * n]);
@@ -127,6 +138,7 @@ int or_inv(int n, const ocplx* A, ocplx* Ainv)
             }
         ocplx d = LU[c + (size_t)c * n];
         for (int r = c + 1; r < n; ++r) LU[r + (size_t)c * n] = cdiv(LU[r + (size_t)c * n], d);
+#pragma omp parallel for schedule(static) if (par)
         for (int j = c + 1; j < n; ++j) {
             ocplx u = LU[c + (size_t)j * n];
             if (RE(u) == 0.0 && IM(u) == 0.0) continue;
@@ -145,6 +157,7 @@ int or_inv(int n, const ocplx* A, ocplx* Ainv)
                 Ainv[c + (size_t)j * n] = Ainv[piv[c] + (size_t)j * n];
                 Ainv[piv[c] + (size_t)j * n] = t;
             }
+#pragma omp parallel for schedule(static) if (par)
     for (int j = 0; j < n; ++j) {
         ocplx* x = Ainv + (size_t)j * n;
         for (int c = 0; c < n; ++c) {           /* unit lower */

@@ -22,6 +22,9 @@ NT_ETA_RELATIVE = 0
 NT_ETA_ABSOLUTE = 1
 NT_PREC_FP64 = 0
 NT_PREC_MIXED = 1
+NT_DOS_FROM_ETA = 0
+NT_DOS_NEGF = 1
+NT_DOS_TB = 2
 
 NT_K_GEMM = 0
 NT_K_INV = 1
@@ -95,6 +98,7 @@ class NtParams(C.Structure):
         ("n_bloch_x", C.c_int),
         ("n_bloch_y", C.c_int),
         ("path_mode", C.c_int),
+        ("dos_mode", C.c_int),
     ]
 
 

@@ -380,6 +380,36 @@ __global__ void k_lead_setup(const double* E, const int* kidx, int nk, int P, in
     z_dev[2 * p + 1] = f * et;
 }
 
+// Leads of different sizes (nl != nr) decimate in two batches, one per lead,
+// with contiguous per-side shifts; the interleaved [left, right] shifts and
+// k indices of k_lead_setup are split for them (side s of point p), and the
+// per-side statuses and iteration counts merged back afterwards.
+__global__ void k_lead_split(const double* z, const double* zpm, const int* idx2, int P, double* zs, double* zpms,
+                             int* idx1)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    for (int side = 0; side < 2; ++side) {
+        zs[2 * (side * P + p) + 0] = z[4 * p + 2 * side + 0];
+        zs[2 * (side * P + p) + 1] = z[4 * p + 2 * side + 1];
+        if (zpm) {
+            zpms[2 * (side * P + p) + 0] = zpm[4 * p + 2 * side + 0];
+            zpms[2 * (side * P + p) + 1] = zpm[4 * p + 2 * side + 1];
+        }
+    }
+    idx1[p] = idx2[2 * p];
+}
+
+__global__ void k_lead_merge(const int* st_s, const int* it_s, int P, int* st2, int* it2)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    for (int side = 0; side < 2; ++side) {
+        st2[2 * p + side] = st_s[side * P + p];
+        it2[2 * p + side] = it_s[side * P + p];
+    }
+}
+
 __global__ void k_dec_finish(const int* active, int* status, int Q)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1339,6 +1369,7 @@ int check_params(const nt_params* p)
     if (!p || p->max_iter < 1 || !(p->convergence > 0.0) || (p->eta_mode != 0 && p->eta_mode != 1)) return NT_ERR_ARG;
     if (p->precision != NT_PREC_FP64 && p->precision != NT_PREC_MIXED) return NT_ERR_ARG;
     if (p->circular && (p->n_bloch_x < 1 || p->n_bloch_y < 1)) return NT_ERR_ARG;
+    if (p->dos_mode < NT_DOS_FROM_ETA || p->dos_mode > NT_DOS_TB) return NT_ERR_ARG;
     return 0;
 }
 
@@ -1408,6 +1439,12 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         o_h32[5] = ar.take(c32 * nrr * a.nk);
     }
     const size_t o_zdev32 = ar.take(c32 * Pmax);
+    // per-side shifts, k indices, statuses and iterations (nl != nr only)
+    const size_t o_zs = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
+    const size_t o_zpms = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
+    const size_t o_idx1 = same ? 0 : ar.take(sizeof(int) * Pmax);
+    const size_t o_sts = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
+    const size_t o_its = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
     const size_t o_scr = ar.off;
     // per-chunk FP32 lead quantities + optional FP32 outputs, then the FP32 RGF scratch
     Arena l32;
@@ -1512,7 +1549,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         const int P = std::min(Pmax, a.npts - p0);
         const double* E = a.E + p0;
         const int* kidx = a.kidx ? a.kidx + p0 : nullptr;
-        hipLaunchKernelGGL(k_lead_setup, dim3(nblk(P)), dim3(256), 0, s, E, kidx, a.nk, P, same ? 1 : 0, prm->eta,
+        hipLaunchKernelGGL(k_lead_setup, dim3(nblk(P)), dim3(256), 0, s, E, kidx, a.nk, P, 1, prm->eta,
                            prm->eta0, prm->eta_mode, prm->device_eta_factor, idx, zdec, pm ? zpm : nullptr, zdev);
         hipcheck(c, hipGetLastError(), "lead setup");
         if (probe) {
@@ -1536,24 +1573,35 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                 lead_gfs(c, prm, nl, 2 * P, 2 * a.nk, lh, lv, idx, 0, zpm, o2, stpm, itpm, ar, o_scr);
             }
         } else {
-            // separate batches per lead: item strides 2*nmm keep [left, right] pairs
-            DecimationOut<T> ol{bw(gbuf, 2 * nmm, nl), bw(sigbuf, 2 * nmm, nl), bw(gambuf, 2 * nmm, nl)};
-            DecimationOut<T> orr{bw(gbuf + 2 * nmm, 2 * nmm, nr), bw(sigbuf + 2 * nmm, 2 * nmm, nr),
-                                 bw(gambuf + 2 * nmm, 2 * nmm, nr)};
-            // z and status are interleaved per point [left, right]: use stride-2 views
-            // by running each side with its own contiguous copies
-            std::vector<int> dummy;
-            T* zl = zpm;   // reuse: zl = left adv, zr = right ret (contiguous)
-            (void)zl;
-            c->err = "nl != nr lead blocks are not supported by the batched tdb solve";
-            throw Fail{NT_ERR_ARG};
-            (void)ol;
-            (void)orr;
+            // one batch per lead (nl != nr): per-point slots of nmm keep the
+            // [left, right] pairs; the left and right matrices of k-point k
+            // are Haml/Hamlv[k] and Hamr/Hamrv[k]
+            double* zs = ar.ptr<T>(o_zs);
+            double* zpms = ar.ptr<T>(o_zpms);
+            int* idx1 = ar.ptr<int>(o_idx1);
+            int* sts = ar.ptr<int>(o_sts);
+            int* its = ar.ptr<int>(o_its);
+            hipLaunchKernelGGL(k_lead_split, dim3(nblk(P)), dim3(256), 0, s, zdec, pm ? zpm : nullptr, idx, P, zs,
+                               pm ? zpms : nullptr, idx1);
+            hipcheck(c, hipGetLastError(), "lead split");
+            auto sides = [&](T* g, T* sg, T* gm, const T* zz, int adag, int* st2, int* it2) {
+                hipcheck(c, hipMemsetAsync(sts, 0, sizeof(int) * 2 * P, s), "memset");
+                DecimationOut<T> ol{bw(g, 2 * nmm, nl), sg ? bw(sg, 2 * nmm, nl) : nonew<T>(),
+                                    gm ? bw(gm, 2 * nmm, nl) : nonew<T>()};
+                lead_gfs(c, prm, nl, P, a.nk, Haml, Hamlv, idx1, adag, zz, ol, sts, its, ar, o_scr);
+                DecimationOut<T> orr{bw(g + 2 * nmm, 2 * nmm, nr), sg ? bw(sg + 2 * nmm, 2 * nmm, nr) : nonew<T>(),
+                                     gm ? bw(gm + 2 * nmm, 2 * nmm, nr) : nonew<T>()};
+                lead_gfs(c, prm, nr, P, a.nk, Hamr, Hamrv, idx1, adag, zz + 2 * P, orr, sts + P, its + P, ar, o_scr);
+                hipLaunchKernelGGL(k_lead_merge, dim3(nblk(P)), dim3(256), 0, s, sts, its, P, st2, it2);
+                hipcheck(c, hipGetLastError(), "lead merge");
+            };
+            sides(gbuf, sigbuf, gambuf, zs, 1, stdec, itdec);
+            if (pm) sides(gpmbuf, nullptr, nullptr, zpms, 0, stpm, itpm);
         }
         // ---- adv -> ret (lesser.f90:141-144): gl = gl_adv^+, sl = sl_adv^+, gml = -gml_adv
-        copy_op<T>(c, nl, nl, bm<T>(gbuf, 2 * nll, nl), bw(glr, nll, nl), 1, (T)1, P);
-        copy_op<T>(c, nl, nl, bm<T>(sigbuf, 2 * nll, nl), bw(slr, nll, nl), 1, (T)1, P);
-        copy_op<T>(c, nl, nl, bm<T>(gambuf, 2 * nll, nl), bw(gmlr, nll, nl), 0, (T)-1, P);
+        copy_op<T>(c, nl, nl, bm<T>(gbuf, 2 * nmm, nl), bw(glr, nll, nl), 1, (T)1, P);
+        copy_op<T>(c, nl, nl, bm<T>(sigbuf, 2 * nmm, nl), bw(slr, nll, nl), 1, (T)1, P);
+        copy_op<T>(c, nl, nl, bm<T>(gambuf, 2 * nmm, nl), bw(gmlr, nll, nl), 0, (T)-1, P);
         // ---- RGF + Caroli
         // Per-k operands: item p reads k-point kidx[p]; without a k map there is
         // one k-point and every item shares it (stride 0).
@@ -1573,9 +1621,9 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         in.gl = bm<T>(glr, nll, nl);
         in.sl = bm<T>(slr, nll, nl);
         in.gml = bm<T>(gmlr, nll, nl);
-        in.gr = bm<T>(gbuf + 2 * nrr, 2 * nrr, nr);
-        in.sr = bm<T>(sigbuf + 2 * nrr, 2 * nrr, nr);
-        in.gmr = bm<T>(gambuf + 2 * nrr, 2 * nrr, nr);
+        in.gr = bm<T>(gbuf + 2 * nmm, 2 * nmm, nr);
+        in.sr = bm<T>(sigbuf + 2 * nmm, 2 * nmm, nr);
+        in.gmr = bm<T>(gambuf + 2 * nmm, 2 * nmm, nr);
         if (probe) {
             in.zl = ar.ptr<T>(o_pzl);
             in.keep = ar.ptr<T>(o_pkeep);
@@ -1653,13 +1701,13 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             float* L32[10];
             for (int i = 0; i < 10; ++i) L32[i] = ar.ptr<float>(o_scr + o_l32[i]);
             const float one = 1.0f;
-            convert_op<T, float>(c, nl, nl, bm<T>(gbuf, 2 * nll, nl), bw<float>(L32[0], nll, nl), 1, one, P);
-            convert_op<T, float>(c, nl, nl, bm<T>(sigbuf, 2 * nll, nl), bw<float>(L32[1], nll, nl), 1, one, P);
-            convert_op<T, float>(c, nl, nl, bm<T>(gambuf, 2 * nll, nl), bw<float>(L32[2], nll, nl), 0, -one, P);
-            convert_op<T, float>(c, nr, nr, bm<T>(gbuf + 2 * nrr, 2 * nrr, nr), bw<float>(L32[3], nrr, nr), 0, one, P);
-            convert_op<T, float>(c, nr, nr, bm<T>(sigbuf + 2 * nrr, 2 * nrr, nr), bw<float>(L32[4], nrr, nr), 0, one,
+            convert_op<T, float>(c, nl, nl, bm<T>(gbuf, 2 * nmm, nl), bw<float>(L32[0], nll, nl), 1, one, P);
+            convert_op<T, float>(c, nl, nl, bm<T>(sigbuf, 2 * nmm, nl), bw<float>(L32[1], nll, nl), 1, one, P);
+            convert_op<T, float>(c, nl, nl, bm<T>(gambuf, 2 * nmm, nl), bw<float>(L32[2], nll, nl), 0, -one, P);
+            convert_op<T, float>(c, nr, nr, bm<T>(gbuf + 2 * nmm, 2 * nmm, nr), bw<float>(L32[3], nrr, nr), 0, one, P);
+            convert_op<T, float>(c, nr, nr, bm<T>(sigbuf + 2 * nmm, 2 * nmm, nr), bw<float>(L32[4], nrr, nr), 0, one,
                                  P);
-            convert_op<T, float>(c, nr, nr, bm<T>(gambuf + 2 * nrr, 2 * nrr, nr), bw<float>(L32[5], nrr, nr), 0, one,
+            convert_op<T, float>(c, nr, nr, bm<T>(gambuf + 2 * nmm, 2 * nmm, nr), bw<float>(L32[5], nrr, nr), 0, one,
                                  P);
             convert_op<T, float>(c, 1, 1, bm<T>(zdev, 1, 1), bw<float>(zdev32, 1, 1), 0, one, P);
             RgfInput<float> i32;
@@ -1703,31 +1751,32 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             copy_op<T>(c, n, n, bm<T>(srcbase, stride, n),
                        bw(reinterpret_cast<T*>(dst) + 2 * (long)p0 * n * n, (long)n * n, n), 0, (T)1, P);
         };
-        put(a.ex.gl_adv, gbuf, 2 * nll, nl);
-        put(a.ex.gr_ret, gbuf + 2 * nrr, 2 * nrr, nr);
-        put(a.ex.sl_adv, sigbuf, 2 * nll, nl);
-        put(a.ex.sr_ret, sigbuf + 2 * nrr, 2 * nrr, nr);
-        put(a.ex.gml_adv, gambuf, 2 * nll, nl);
-        put(a.ex.gmr_ret, gambuf + 2 * nrr, 2 * nrr, nr);
+        put(a.ex.gl_adv, gbuf, 2 * nmm, nl);
+        put(a.ex.gr_ret, gbuf + 2 * nmm, 2 * nmm, nr);
+        put(a.ex.sl_adv, sigbuf, 2 * nmm, nl);
+        put(a.ex.sr_ret, sigbuf + 2 * nmm, 2 * nmm, nr);
+        put(a.ex.gml_adv, gambuf, 2 * nmm, nl);
+        put(a.ex.gmr_ret, gambuf + 2 * nmm, 2 * nmm, nr);
         if (pm) {
-            put(a.ex.gl_plus, gpmbuf, 2 * nll, nl);
-            put(a.ex.gr_minus, gpmbuf + 2 * nrr, 2 * nrr, nr);
+            put(a.ex.gl_plus, gpmbuf, 2 * nmm, nl);
+            put(a.ex.gr_minus, gpmbuf + 2 * nmm, 2 * nmm, nr);
         }
         if (a.ex.sdos_l) check_range(c, a.ex.sdos_l + p0, sizeof(double) * P, "sdos_l");
         if (a.ex.sdos_r) check_range(c, a.ex.sdos_r + p0, sizeof(double) * P, "sdos_r");
-        // surface DOS of the driver family the eta convention belongs to:
-        // negf +-2 sqrt(E) Im Tr g / pi (negf/surface.f90:608-630), tb
-        // +-Im Tr g / pi (tb/surface.f90:345-366)
-        const bool tb_dos = prm->eta_mode == NT_ETA_ABSOLUTE;
+        // surface DOS: negf +-2 sqrt(E) Im Tr g / pi (negf/surface.f90:608-630),
+        // tb +-Im Tr g / pi (tb/surface.f90:345-366); by default the driver
+        // family the eta convention belongs to
+        const bool tb_dos = prm->dos_mode == NT_DOS_TB ||
+                            (prm->dos_mode == NT_DOS_FROM_ETA && prm->eta_mode == NT_ETA_ABSOLUTE);
         const double* Edos = tb_dos ? nullptr : E;
         const double cdos = tb_dos ? 1.0 : 2.0;
         if (a.ex.sdos_l)
             launch(c, NT_K_ELEM, 0, "sdos", [&] {
-                return launch_trace_im<T>(nl, bm<T>(gbuf, 2 * nll, nl), Edos, cdos, a.ex.sdos_l + p0, P, s);
+                return launch_trace_im<T>(nl, bm<T>(gbuf, 2 * nmm, nl), Edos, cdos, a.ex.sdos_l + p0, P, s);
             });
         if (a.ex.sdos_r)
             launch(c, NT_K_ELEM, 0, "sdos", [&] {
-                return launch_trace_im<T>(nr, bm<T>(gbuf + 2 * nrr, 2 * nrr, nr), Edos, -cdos, a.ex.sdos_r + p0, P,
+                return launch_trace_im<T>(nr, bm<T>(gbuf + 2 * nmm, 2 * nmm, nr), Edos, -cdos, a.ex.sdos_r + p0, P,
                                           s);
             });
         // ---- status, iterations, clamps
@@ -2419,6 +2468,7 @@ void nt_params_default(nt_params* p)
     p->n_bloch_x = 1;
     p->n_bloch_y = 1;
     p->path_mode = 2;
+    p->dos_mode = NT_DOS_FROM_ETA;
 }
 
 int nt_create(nt_ctx** out, int device_ordinal)
@@ -2632,7 +2682,7 @@ int nt_buttiker_tdb_batched_dev(nt_ctx* c, const nt_params* p, const nt_probe_de
                                 const nt_solve_extras* extras, const nt_probe_out* out)
 {
     if (!c) return NT_ERR_ARG;
-    if (!probe || !out || !probe->temperature || L < 2) return NT_ERR_ARG;
+    if (!probe || !out || !probe->temperature || L < 2 || nl != nr) return NT_ERR_ARG;
     if (!out->flux || !out->jacobian || !out->flux_left || !out->flux_right || !out->a_sum || !out->gn_sum)
         return NT_ERR_ARG;
     if (!(probe->B_probe > 0.0) || !(probe->t_left > 0.0) || !(probe->t_right > 0.0)) return NT_ERR_ARG;

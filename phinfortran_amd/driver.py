@@ -99,6 +99,7 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
     tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
     tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
     st = r["status"].cpu().numpy().reshape(nk, ne).T
+    _check_points(st, np.sqrt(E), ks, "run_qe_tdb")
     if output_dir and cfg["path_mode"] != 1:     # phinfortran.f90:1524
         OUT.write_trans(output_dir, np.repeat(E[:, None], nk, axis=1), tl, tr,
                         **{k: out[k] for k in out if k not in ("modes_status", "buttiker")})
@@ -106,6 +107,19 @@ def run_qe_tdb(config_path: str, fc_path: str, output_dir: Optional[str] = None,
         eng.close()
     out.update(E=E, k=ks, transl=tl, transr=tr, status=st, blocks=d)
     return out
+
+
+def _check_points(status, E, ks, what):
+    """Raise TransportError naming the (E, k) points whose solve failed
+    (status (ne, nk)), as the per-point path does for a single point; the
+    tables are only written when every point succeeded."""
+    status = np.asarray(status)
+    bad = np.argwhere(status != 0)
+    if bad.size:
+        from . import _native as N
+        pts = ", ".join(f"(E={E[i]:.6g}, k={j + 1}: status {int(status[i, j])})" for i, j in bad[:8])
+        more = f" and {len(bad) - 8} more" if len(bad) > 8 else ""
+        raise N.TransportError(int(status[bad[0][0], bad[0][1]]), f"{what}: {len(bad)} point(s) failed: {pts}{more}")
 
 
 def _tb_device_blocks(H, layers, nl: int, nr: int, nbl: int, nbr: int):
@@ -212,6 +226,7 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
                                 g["sl_adv"].conj().T, g["sr_ret"], E[ie], s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml,
                                 s.Hamr, p, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
                 tl[ie, ik], tr[ie, ik] = o["transl"], o["transr"]
+    _check_points(status, E, ks, "run_tb")
     out_dir = output_dir
     if out_dir is None and cfg["output_dir"]:
         out_dir = os.path.join(base, cfg["output_dir"])

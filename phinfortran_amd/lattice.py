@@ -266,6 +266,35 @@ def si_ge_interface_film(nx=4, ny=2, L=40, kfrac=(0.0, 0.0), seed=0, iface_sprin
     return phonon_blocks(layer, masses, springs, MASS_SI, MASS_GE, kfrac, alpha, beta)
 
 
+def rough_interface_masses(natom, L=100, rough_layers=3, seed=0):
+    """Masses (L, natom) of a Si|Ge film with a rough interface: layers below
+    the interface Si, above it Ge, and `rough_layers` layers centred on it
+    with each site Ge with probability (j + 1) / (rough_layers + 1) in the
+    j-th of them (a graded, seeded alloy)."""
+    rng = np.random.default_rng(seed)
+    lo = L // 2 - rough_layers // 2
+    masses = np.full((L, natom), MASS_SI)
+    masses[lo + rough_layers:] = MASS_GE
+    for j in range(rough_layers):
+        f = (j + 1) / (rough_layers + 1)
+        masses[lo + j] = np.where(rng.random(natom) < f, MASS_GE, MASS_SI)
+    return masses
+
+
+def rough_interface_film(nx=8, ny=4, L=100, rough_layers=3, kfrac=(0.0, 0.0), seed=0, masses=None,
+                         alpha=48.5, beta=13.8):
+    """Si thin film (nx x ny conventional cells in-plane, periodic) with a
+    seeded rough Si/Ge interface of `rough_layers` layers in the middle of
+    the device; Si left lead, Ge right lead.  masses (L, natom) overrides the
+    seeded mass profile (the interface masses are the large mode-resolved
+    workload's trainable parameters)."""
+    layer = _layer_cache(nx, ny)
+    if masses is None:
+        masses = rough_interface_masses(layer.natom, L, rough_layers, seed)
+    springs = np.ones((L, layer.nbond))
+    return phonon_blocks(layer, masses, springs, MASS_SI, MASS_GE, kfrac, alpha, beta)
+
+
 def chain_blocks(L=4, k=1.0, m=1.0, masses=None):
     """1-D monatomic chain (b=1), the closed-form case: T = 1 for 0 < E < 4k/m."""
     masses = np.full(L, m) if masses is None else np.asarray(masses, float)

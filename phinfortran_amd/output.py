@@ -8,6 +8,7 @@ with nothing (the reference writes "(A)" of " " and of "" respectively).
 """
 from __future__ import annotations
 
+import math
 import os
 from typing import Optional
 
@@ -18,6 +19,11 @@ def fortran_e(x: float, w: int = 24, d: int = 16) -> str:
     """Fortran Ew.d editing: 0.DDDD...E+XX right-justified in w columns
     (three-digit exponents drop the E as Fortran does)."""
     x = float(x)
+    if math.isnan(x):                   # gfortran writes NaN / Infinity right-justified
+        return "NaN".rjust(w)
+    if math.isinf(x):
+        body = "Infinity" if x > 0 else "-Infinity"
+        return body.rjust(w) if len(body) <= w else "*" * w
     if x == 0.0:
         body = "0." + "0" * d + "E+00"
     else:
@@ -38,7 +44,7 @@ def _table(path: str, E1, T, tail: str, w: int = 24, d: int = 16, clamp: bool = 
     T = np.asarray(T, float)
     with open(path, "w") as f:
         for i in range(T.shape[0]):
-            row = [fortran_e(E1[i], w, d)] + [fortran_e(v if (v >= 1e-60 or not clamp) else 0.0, w, d) for v in T[i]]
+            row = [fortran_e(E1[i], w, d)] + [fortran_e(v if (v >= 1e-60 or not clamp or math.isnan(v)) else 0.0, w, d) for v in T[i]]
             f.write("".join(row) + tail + "\n")
 
 

@@ -54,7 +54,10 @@ def params(kind: str = "phonon", **kw) -> N.NtParams:
         p.eta_mode = N.NT_ETA_ABSOLUTE
         p.device_eta_factor = 1.0
         p.eta0 = 0.0
-    elif kind != "phonon":
+        p.dos_mode = N.NT_DOS_TB
+    elif kind == "phonon":
+        p.dos_mode = N.NT_DOS_NEGF
+    else:
         raise ValueError(kind)
     for k, v in kw.items():
         if k == "precision" and isinstance(v, str):

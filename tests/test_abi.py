@@ -129,3 +129,25 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
         N.lib()
     with pytest.raises(ImportError):
         T.params("tb")
+
+
+def test_dos_mode_default_and_validation_without_gpu():
+    """The surface-DOS convention is an explicit parameter: the default follows
+    eta_mode, out-of-range values are rejected, and the Python parameter
+    builder names the convention of its driver family."""
+    from phinfortran_amd import _native as N
+    from phinfortran_amd import transport as T
+    L = N.lib()
+    p = N.NtParams()
+    L.nt_params_default(ctypes.byref(p))
+    assert p.dos_mode == N.NT_DOS_FROM_ETA
+    for m in (N.NT_DOS_FROM_ETA, N.NT_DOS_NEGF, N.NT_DOS_TB):
+        p.dos_mode = m
+        assert L.nt_params_validate(ctypes.byref(p), 4, 4) == N.NT_OK
+    for m in (-1, 3):
+        p.dos_mode = m
+        assert L.nt_params_validate(ctypes.byref(p), 4, 4) == N.NT_ERR_ARG
+    assert T.params("phonon").dos_mode == N.NT_DOS_NEGF
+    assert T.params("tb").dos_mode == N.NT_DOS_TB
+    # layout of the struct seen from Python equals the C one
+    assert ctypes.sizeof(N.NtParams) == 5 * 8 + 8 * 4

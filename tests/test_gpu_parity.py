@@ -259,3 +259,29 @@ def test_status_codes(engine, T):
         engine.get_G_full_tdb(*(np.eye(2),) * 6, 0.1, np.zeros((1, 2, 2)), np.zeros((0, 2, 2)), np.eye(2),
                               np.eye(2), np.eye(2), np.eye(2), T.params("phonon", max_iter=0))
     assert e.value.status == N.NT_ERR_ARG
+
+
+@pytest.mark.parametrize("nl,nr,b", [(12, 20, 16), (24, 8, 10)])
+def test_batched_unequal_leads(engine, T, nl, nr, b):
+    """Leads of different sizes (a heterojunction whose lead periods carry
+    different orbital counts, as tb/surface.f90:11-91 allows): each lead
+    decimates in its own batch.  T and the lead outputs against the oracle,
+    two k-points."""
+    blocks = [lt.random_blocks(nl, b, 4, nr=nr, seed=90 + i, scale=0.7) for i in range(2)]
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
+    op = oparams(p)
+    E = np.array([0.21, 0.52, 0.77, 1.3])
+    kidx = np.array([0, 1, 1, 0], np.int32)
+    print(f"unequal leads nl={nl} nr={nr} b={b}", flush=True)
+    r = _batched(engine, T, blocks, E, p, kidx=kidx, extras=("gl_adv", "gr_ret", "sdos_l", "sdos_r"))
+    assert (r["status"].cpu().numpy() == 0).all()
+    for i, e in enumerate(E):
+        blk = blocks[kidx[i]]
+        otl, otr, _ = O.solve_tdb(blk.as_dict(), e, op)
+        assert abs(r["T_LR"][i].item() - otl) < 1e-9 * max(1, otl), (e, r["T_LR"][i].item(), otl)
+        assert abs(r["T_RL"][i].item() - otr) < 1e-9 * max(1, otr)
+        sg = O.surface_green(blk.Haml, blk.Hamlv, blk.Hamr, blk.Hamrv, e, op)
+        assert rel(r["gl_adv"][i].cpu().numpy(), sg["gl_adv"]) < 1e-8
+        assert rel(r["gr_ret"][i].cpu().numpy(), sg["gr_ret"]) < 1e-8
+        sdos = -2.0 * np.sqrt(e) * np.trace(sg["gr_ret"]).imag / np.pi
+        assert abs(r["sdos_r"][i].item() - sdos) < 1e-9 * max(1, abs(sdos))

@@ -69,3 +69,28 @@ def test_specular_and_md_row_rules(tmp_path):
     for name in ("tlr", "trl", "right", "left", "mdl", "mdr"):
         lines = open(tmp_path / ("%s_specular_param_000007.dat" % name)).read().splitlines()
         assert len(lines) == 6
+
+
+def test_fortran_e_nan_and_infinity():
+    """gfortran writes NaN and Infinity right-justified in the field; the
+    tables keep a NaN instead of clamping it to zero."""
+    assert OUT.fortran_e(float("nan")) == " " * 21 + "NaN"
+    assert OUT.fortran_e(float("inf")) == " " * 16 + "Infinity"
+    assert OUT.fortran_e(float("-inf")) == " " * 15 + "-Infinity"
+    assert OUT.fortran_e(float("inf"), w=6) == "******"
+
+
+def test_driver_refuses_failed_points():
+    """A failed (E, k) point stops the run with its location instead of
+    writing it into the tables (the per-point path raises the same way)."""
+    import numpy as np
+    import pytest
+    from phinfortran_amd import _native as N
+    from phinfortran_amd import driver
+    st = np.zeros((5, 3), np.int32)
+    driver._check_points(st, np.arange(5.0), None, "run")
+    st[3, 2] = N.NT_ERR_NOT_CONVERGED
+    with pytest.raises(N.TransportError) as e:
+        driver._check_points(st, np.arange(5.0), None, "run")
+    assert e.value.status == N.NT_ERR_NOT_CONVERGED
+    assert "E=3" in str(e.value) and "k=3" in str(e.value)

@@ -168,8 +168,11 @@ def test_large_b768_vs_committed_oracle(engine, T):
     committed by tests/golden/make_c5_oracle.py."""
     ref = json.load(open(os.path.join(HERE, "golden", "c5_oracle.json")))
     p = T.params("phonon")
-    for key in ("eta", "eta0", "convergence", "device_eta_factor", "max_iter"):
+    # max_iter differs (oracle default 200, engine 300) but is not reached:
+    # the fixture's decimations converge in about 24 iterations
+    for key in ("eta", "eta0", "convergence", "device_eta_factor"):
         assert getattr(p, key) == ref["params"][key]
+    assert max(pt["iters"] for pt in ref["points"]) < min(p.max_iter, ref["params"]["max_iter"])
     for pt in ref["points"]:
         blk = lt.rough_interface_film(nx=8, ny=4, L=100, rough_layers=3, seed=0, kfrac=tuple(pt["kfrac"]))
         assert (blk.b, blk.L) == (768, 100)

@@ -258,6 +258,8 @@ def main():
             tuned[prec] = eng.autotune_inverse(blocks[0].b, args.points, prec)
             log(f"inverse panel width for b={blocks[0].b} {prec}: {tuned[prec][0]} ({tuned[prec][1]:.2f} ms)")
 
+    mixed_counts = {}
+
     def run(prec):
         """warmup + timed steps for one precision; returns (s, per-step T, kernel stats, mean iters)"""
         p = T.params("phonon", precision=prec)
@@ -270,6 +272,7 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         outs, iters = [], []
+        counts = None
         for s in range(args.steps):
             r = eng.solve_tdb(dblocks[args.warmup + s], E, p, stream_sync=False)
             outs.append(r)
@@ -283,6 +286,10 @@ def main():
             f"{k}: {v['launches']} launches {v['ms']:.1f} ms {v['flops'] / max(1e-9, v['ms'] * 1e-3) / 1e12:.2f} TF/s"
             for k, v in kern.items()))
         ok = all(bool((r["status"] == 0).all()) for r in outs)
+        if prec == "mixed":
+            counts = eng.mixed_counts()
+            log(f"mixed: points {counts[0]}, second refinement steps {counts[1]}, FP64 recomputes {counts[2]}")
+            mixed_counts.update(points=counts[0], second_steps=counts[1], fp64_recomputes=counts[2])
         return el, [r["T_LR"].cpu().numpy() for r in outs], kern, float(np.mean(iters)), ok
 
     el64, T64, k64, nbar, ok64 = run("fp64")
@@ -543,6 +550,7 @@ def main():
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,
                               "mixed_speedup": vmx / v64, "mixed_max_rel_dT": rel_err,
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",
+                              "refinement": mixed_counts,
                               "status_ok": bool(ok64 and okmx)} if args.mixed else None,
         "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps_on_two_streams": True},
         "solve_model": {"mean_decimation_iters": nbar, "flop_per_solve": flop_solve,

@@ -48,7 +48,9 @@ enum {
 /* Arithmetic of the RGF sweep. */
 enum {
     NT_PREC_FP64 = 0,      /* complex FP64 throughout (the reference's arithmetic) */
-    NT_PREC_MIXED = 1      /* complex FP32 MFMA sweep + FP64 refinement; FP64 decimation */
+    NT_PREC_MIXED = 1      /* FP64 decimation; each sweep direction factorised in complex FP32 on
+                              MFMA, its block column refined against FP64 residuals (one or two
+                              steps); points failing the error estimate recomputed in FP64 */
 };
 
 /* Scalar parameters that the reference keeps as module globals
@@ -556,6 +558,12 @@ int nt_profile_get(nt_ctx* ctx, int kind, long long* launches, double* total_ms,
 int nt_profile_reset(nt_ctx* ctx);
 /* Mean decimation iterations of the last batched solve. */
 double nt_last_mean_iters(nt_ctx* ctx);
+/* Mixed-precision bookkeeping since the last nt_profile_reset: counts[0]
+ * points solved with NT_PREC_MIXED, counts[1] second refinement steps (both
+ * sweep directions), counts[2] points whose refined result failed the error
+ * estimate and were recomputed by the FP64 sweep.  No reference counterpart
+ * (the reference computes in FP64 only). */
+int nt_mixed_counts(nt_ctx* ctx, long long* counts);
 
 #ifdef __cplusplus
 }

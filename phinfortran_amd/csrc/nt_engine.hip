@@ -11,344 +11,11 @@
 //   Caroli       T = Re Tr[G Gamma G^+ Gamma]  (negf/lesser.f90:787-807)
 // Shared operands (H(k) blocks) are addressed with stride 0 or a k-index map,
 // so one H(k) copy serves every energy of that k-point.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cmath>
-#include <complex>
-#include <cstdio>
-#include <cstring>
 #include <mutex>
-#include <string>
-#include <map>
-#include <vector>
 
-#include "../../include/negf_transport.h"
-#include "nt_kernels.h"
-
-using namespace nt;
-
-struct nt_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t aux = nullptr;     // second stream: the right-to-left sweep runs beside the left-to-right one
-    hipEvent_t fork = nullptr, join = nullptr;
-    bool overlap = true;
-    bool gemm_m3 = false;          // FP64 GEMMs with three real products per complex product
-    hipStream_t stream = nullptr;
-    std::string err;
-    int batch_limit = 512;
-    void* ws = nullptr;
-    size_t ws_bytes = 0;
-    int* host_flag = nullptr;      // pinned
-    // profiling
-    bool prof = false;
-    struct Rec {
-        hipEvent_t a, b;
-        int kind;
-        double flops;
-    };
-    std::vector<Rec> recs;
-    std::vector<hipEvent_t> pool;
-    long long launches[NT_K_COUNT] = {};
-    double ms[NT_K_COUNT] = {};
-    double flops[NT_K_COUNT] = {};
-    double last_mean_iters = 0.0;
-    // device address ranges a launch may touch: the workspace and the caller's
-    // buffers of the current call (sizes derived from the call's arguments)
-    std::vector<std::pair<uintptr_t, uintptr_t>> regions;
-    // unitary block DFT U = P1 (1 (x) P2) of circular decimation, per lead size
-    struct CircU {
-        int n, nbx, nby;
-        double* d;
-    };
-    std::vector<CircU> circ_u;
-    // autotuned inverse panel width per (element bytes, n); absent: size rule
-    std::map<std::pair<int, int>, int> inv_nb;
-    // degenerate-group descriptors and statuses of the path_mode = 1 rotation
-    void* deg_buf = nullptr;
-    size_t deg_bytes = 0;
-};
+#include "nt_host.h"
 
 namespace {
-
-// ------------------------------------------------------------- helpers
-struct Fail {
-    int code;
-};
-
-inline void hipcheck(nt_ctx* c, hipError_t e, const char* what)
-{
-    if (e != hipSuccess) {
-        c->err = std::string(what) + ": " + hipGetErrorString(e);
-        throw Fail{e == hipErrorOutOfMemory ? NT_ERR_NOMEM : NT_ERR_HIP};
-    }
-}
-
-hipEvent_t take_event(nt_ctx* c)
-{
-    if (!c->pool.empty()) {
-        hipEvent_t e = c->pool.back();
-        c->pool.pop_back();
-        return e;
-    }
-    hipEvent_t e;
-    hipcheck(c, hipEventCreate(&e), "hipEventCreate");
-    return e;
-}
-
-template <class F>
-void launch(nt_ctx* c, int kind, double flops, const char* what, F&& f)
-{
-    hipEvent_t a = nullptr, b = nullptr;
-    if (c->prof) {
-        a = take_event(c);
-        hipcheck(c, hipEventRecord(a, c->stream), "hipEventRecord");
-    }
-    hipcheck(c, f(), what);
-    if (c->prof) {
-        b = take_event(c);
-        hipcheck(c, hipEventRecord(b, c->stream), "hipEventRecord");
-        c->recs.push_back({a, b, kind, flops});
-    }
-}
-
-void harvest_profile(nt_ctx* c)
-{
-    if (c->recs.empty()) return;
-    hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    for (auto& r : c->recs) {
-        float ms = 0.f;
-        hipcheck(c, hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
-        c->launches[r.kind] += 1;
-        c->ms[r.kind] += ms;
-        c->flops[r.kind] += r.flops;
-        c->pool.push_back(r.a);
-        c->pool.push_back(r.b);
-    }
-    c->recs.clear();
-}
-
-// Bump allocator over the context workspace.
-struct Arena {
-    size_t off = 0;
-    char* base = nullptr;
-    size_t take(size_t bytes)
-    {
-        size_t o = off;
-        off += (bytes + 255) & ~(size_t)255;
-        return o;
-    }
-    template <typename P>
-    P* ptr(size_t o) const
-    {
-        return reinterpret_cast<P*>(base + o);
-    }
-};
-
-
-
-template <typename T>
-BMat<T> bm(const T* base, long stride, int ld, const int* idx = nullptr, long gstride = 0, int idx_n = 0)
-{
-    return BMat<T>{base, stride, gstride, idx, ld, idx_n};
-}
-template <typename T>
-BMatW<T> bw(T* base, long stride, int ld, const int* idx = nullptr, long gstride = 0, int idx_n = 0)
-{
-    return BMatW<T>{base, stride, gstride, idx, ld, idx_n};
-}
-template <typename T>
-BMat<T> ro(const BMatW<T>& w)
-{
-    return BMat<T>{w.base, w.stride, w.gstride, w.idx, w.ld, w.idx_n};
-}
-template <typename T>
-BMat<T> none()
-{
-    return BMat<T>{nullptr, 0, 0, nullptr, 0, 0};
-}
-template <typename T>
-BMatW<T> nonew()
-{
-    return BMatW<T>{nullptr, 0, 0, nullptr, 0, 0};
-}
-
-// ---------------------------------------------------------- bounds audit
-// Every launch checks, on the host, that each operand's whole address range
-// (all items, groups, rows and columns) lies inside one registered region.
-// An addressing mistake then becomes NT_ERR_ARG instead of a GPU fault.
-void reg_region(nt_ctx* c, const void* p, size_t bytes)
-{
-    if (p && bytes) c->regions.push_back({(uintptr_t)p, (uintptr_t)p + bytes});
-}
-
-void unreg_region(nt_ctx* c, const void* p)
-{
-    for (size_t i = 0; i < c->regions.size(); ++i)
-        if (c->regions[i].first == (uintptr_t)p) {
-            c->regions.erase(c->regions.begin() + i);
-            return;
-        }
-}
-
-void check_range(nt_ctx* c, const void* p, size_t bytes, const char* what)
-{
-    if (!p || !bytes) return;
-    const uintptr_t b = (uintptr_t)p, e = b + bytes;
-    for (auto& r : c->regions)
-        if (b >= r.first && e <= r.second) return;
-    char buf[256];
-    snprintf(buf, sizeof(buf), "bounds audit: %s touches [%p, +%zu) outside every registered buffer", what, p,
-             bytes);
-    c->err = buf;
-    throw Fail{NT_ERR_ARG};
-}
-
-void ensure_ws(nt_ctx* c, size_t bytes)
-{
-    if (c->ws_bytes >= bytes) return;
-    if (c->ws) {
-        hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        unreg_region(c, c->ws);
-        hipcheck(c, hipFree(c->ws), "hipFree");
-        c->ws = nullptr;
-        c->ws_bytes = 0;
-    }
-    size_t want = bytes + bytes / 8;
-    hipcheck(c, hipMalloc(&c->ws, want), "hipMalloc workspace");
-    c->ws_bytes = want;
-    reg_region(c, c->ws, c->ws_bytes);
-}
-
-template <typename T, typename M>
-void audit(nt_ctx* c, const M& m, int rows, int cols, int batch, int groups, const char* what)
-{
-    if (!m.base || rows <= 0 || cols <= 0 || batch <= 0) return;
-    if (m.stride < 0 || m.gstride < 0 || m.ld < rows || (m.idx && m.idx_n <= 0)) {
-        c->err = std::string("bounds audit: bad descriptor for ") + what;
-        throw Fail{NT_ERR_ARG};
-    }
-    const long items = m.idx ? m.idx_n : batch;
-    const long last = (items - 1) * m.stride + (long)(groups - 1) * m.gstride + (rows - 1) + (long)(cols - 1) * m.ld;
-    check_range(c, m.base, (size_t)(last + 1) * 2 * sizeof(T), what);
-}
-
-// C = alpha op(A) op(B) + beta Cin + diag
-template <typename T>
-void gemm(nt_ctx* c, int M, int N, int K, int opA, BMat<T> A, int opB, BMat<T> B, BMatW<T> C, double alpha,
-          BMat<T> Cin, double beta, const T* diag, int batch, double nitems = -1, const int* active = nullptr,
-          int groups = 1, BMatW<T> C2 = BMatW<T>{nullptr, 0, 0, nullptr, 0},
-          unsigned long long* maxabs = nullptr)
-{
-    GemmArgs<T> a;
-    a.M = M;
-    a.N = N;
-    a.K = K;
-    a.opA = opA;
-    a.opB = opB;
-    a.A = A;
-    a.B = B;
-    a.Cin = Cin;
-    a.C = C;
-    a.C2 = C2;
-    a.alpha_re = (T)alpha;
-    a.alpha_im = 0;
-    a.beta_re = (T)beta;
-    a.beta_im = 0;
-    a.diag = diag;
-    a.active = active;
-    a.maxabs = maxabs;
-    a.batch = batch;
-    a.groups = groups;
-    a.m3 = (sizeof(T) == 8 && c->gemm_m3) ? 1 : 0;
-    const double items = (nitems < 0 ? (double)batch : nitems) * groups;
-    audit<T>(c, A, opA == OP_N ? M : K, opA == OP_N ? K : M, batch, groups, "gemm A");
-    audit<T>(c, B, opB == OP_N ? K : N, opB == OP_N ? N : K, batch, groups, "gemm B");
-    audit<T>(c, C, M, N, batch, groups, "gemm C");
-    audit<T>(c, Cin, M, N, batch, groups, "gemm Cin");
-    audit<T>(c, C2, M, N, batch, groups, "gemm C2");
-    check_range(c, diag, sizeof(T) * 2 * batch, "gemm diag");
-    check_range(c, active, sizeof(int) * batch, "gemm active");
-    check_range(c, maxabs, sizeof(unsigned long long) * batch, "gemm maxabs");
-    launch(c, NT_K_GEMM, 8.0 * M * N * K * items, "gemm", [&] { return launch_gemm<T>(a, c->stream); });
-}
-
-template <typename T>
-void inverse(nt_ctx* c, int n, BMatW<T> W, BMat<T> src, const T* z, BMatW<T> X, int* info, int batch,
-             double nitems = -1, const int* active = nullptr)
-{
-    InvArgs<T> a;
-    a.n = n;
-    a.W = W;
-    a.src = src;
-    a.z = z;
-    a.X = X;
-    a.active = active;
-    a.info = info;
-    a.batch = batch;
-    auto tuned = c->inv_nb.find({(int)sizeof(T), n});
-    a.nb = tuned == c->inv_nb.end() ? 0 : tuned->second;
-    const double items = nitems < 0 ? (double)batch : nitems;
-    audit<T>(c, W, n, n, batch, 1, "inverse W");
-    audit<T>(c, src, n, n, batch, 1, "inverse src");
-    audit<T>(c, X, n, n, batch, 1, "inverse X");
-    if (src.base) check_range(c, z, sizeof(T) * 2 * batch, "inverse z");
-    check_range(c, info, sizeof(int) * batch, "inverse info");
-    check_range(c, active, sizeof(int) * batch, "inverse active");
-    if (W.base == X.base) {
-        c->err = "inverse: work and output alias";
-        throw Fail{NT_ERR_ARG};
-    }
-    launch(c, NT_K_INV, 8.0 * n * (double)n * n * items, "inverse", [&] { return launch_inv<T>(a, c->stream); });
-}
-
-template <typename T>
-void copy_op(nt_ctx* c, int m, int n, BMat<T> in, BMatW<T> out, int dagger, T sign, int batch)
-{
-    audit<T>(c, in, m, n, batch, 1, "copy in");
-    audit<T>(c, out, dagger ? n : m, dagger ? m : n, batch, 1, "copy out");
-    launch(c, NT_K_ELEM, 0, "copy",
-           [&] { return launch_copy_op<T>(m, n, in, out, dagger, sign, nullptr, batch, c->stream); });
-}
-
-template <typename Ti, typename To>
-void convert_op(nt_ctx* c, int m, int n, BMat<Ti> in, BMatW<To> out, int dagger, To sign, int batch)
-{
-    audit<Ti>(c, in, m, n, batch, 1, "convert in");
-    audit<To>(c, out, dagger ? n : m, dagger ? m : n, batch, 1, "convert out");
-    launch(c, NT_K_ELEM, 0, "convert",
-           [&] { return launch_convert<Ti, To>(m, n, in, out, dagger, sign, batch, c->stream); });
-}
-
-template <typename T>
-void gamma_op(nt_ctx* c, int n, BMat<T> S, BMatW<T> G, int batch)
-{
-    audit<T>(c, S, n, n, batch, 1, "gamma in");
-    audit<T>(c, G, n, n, batch, 1, "gamma out");
-    launch(c, NT_K_ELEM, 0, "gamma", [&] { return launch_gamma<T>(n, S, G, (T)1, nullptr, batch, c->stream); });
-}
-
-template <typename T>
-void shift_sub(nt_ctx* c, int n, const T* z, BMat<T> A, BMat<T> B, BMatW<T> out, int batch)
-{
-    check_range(c, z, sizeof(T) * 2 * batch, "shift z");
-    audit<T>(c, A, n, n, batch, 1, "shift A");
-    audit<T>(c, B, n, n, batch, 1, "shift B");
-    audit<T>(c, out, n, n, batch, 1, "shift out");
-    launch(c, NT_K_ELEM, 0, "shift_sub",
-           [&] { return launch_shift_sub<T>(n, z, A, B, out, nullptr, batch, c->stream); });
-}
-
-template <typename T>
-void trace_prod(nt_ctx* c, int n1, int n2, BMat<T> X, BMat<T> Y, double* out, int batch)
-{
-    audit<T>(c, X, n1, n2, batch, 1, "trace X");
-    audit<T>(c, Y, n2, n1, batch, 1, "trace Y");
-    check_range(c, out, sizeof(double) * batch, "trace out");
-    launch(c, NT_K_ELEM, 0, "trace",
-           [&] { return launch_trace_prod<T>(n1, n2, X, Y, out, 1, nullptr, batch, c->stream); });
-}
 
 // ------------------------------------------------------- engine kernels
 __global__ void k_lead_setup(const double* E, const int* kidx, int nk, int P, int two_sided, double eta, double eta0,
@@ -408,6 +75,15 @@ __global__ void k_lead_merge(const int* st_s, const int* it_s, int P, int* st2, 
         st2[2 * p + side] = st_s[side * P + p];
         it2[2 * p + side] = it_s[side * P + p];
     }
+}
+
+// fb[p] = !accept[p]; *count += number of rejected points
+__global__ void k_reject_mask(const int* accept, int* fb, int* count, int P)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    fb[p] = accept[p] ? 0 : 1;
+    if (!accept[p]) atomicAdd(count, 1);
 }
 
 __global__ void k_dec_finish(const int* active, int* status, int Q)
@@ -777,31 +453,6 @@ size_t lead_gfs_scratch(const nt_params* prm, int n, int Q, int nm)
     return NB > 1 ? decimate_circular_scratch(n, Q, nm, NB) : decimate_scratch<double>(n, Q);
 }
 
-// ------------------------------------------------------------------ RGF
-template <typename T>
-struct RgfInput {
-    int L;
-    std::vector<int> bsz;
-    std::vector<BMat<T>> Hd, Hdv, Hdv_back;   // Hdv_back[i].base may be null
-    BMat<T> Hld, Hrd, Hrd_fwd, Haml, Hamr;     // Hrd_fwd.base may be null
-    // retarded lead quantities (per point)
-    BMat<T> gl, gr, gml, gmr, sl, sr;
-    // Buttiker probes: per-layer device shifts z - sigma_w(i) (L x P complex,
-    // layer-major; null: every device layer uses the common shift), and L slots
-    // that receive the left-connected G of layers 0..L-2 and the full G of
-    // layer L-1 (null: not kept)
-    const T* zl = nullptr;
-    T* keep = nullptr;
-};
-
-template <typename T>
-struct RgfOutput {
-    double* T_LR;
-    double* T_RL;
-    int* status;
-    BMatW<T> G_D_LR, G_D_RL, G_L, G_R;   // may be null (then scratch)
-};
-
 // Adjoint of T_LR (FP64 only).  Layers [lo, hi] of the device (0-based) form
 // the window whose blocks get gradients.  All buffers hold P points of
 // bmax^2 complex entries per slot (slot-major, then point).
@@ -1065,28 +716,28 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         if (T* sl = slotL(0)) G = sl;
         if (T* s0 = slot0(0)) G0 = s0;
         if (L > 1 && keep(0)) G = keep(0);
-        gemm<T>(c, b, nl, nl, OP_C, in.Hld, OP_N, in.gl, W(T1, b), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, in.Hld, W(Aw, b), -1.0, in.Hd[0], -1.0, zdev(0), P);
-        inverse<T>(c, b, W(Aw, b), none<T>(), nullptr, W(G, b), st, P);
-        gemm<T>(c, nl, b, nl, OP_N, in.gl, OP_N, in.Hld, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nl, b, b, OP_N, R(T2, nl), OP_N, R(G, b), W(G0, nl), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, b, nl, nl, OP_C, in.Hld, OP_N, in.gl, W(T1, b), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, in.Hld, W(Aw, b), -1.0, in.Hd[0], -1.0, zdev(0), P, -1, in.act);
+        inverse<T>(c, b, W(Aw, b), none<T>(), nullptr, W(G, b), st, P, -1, in.act);
+        gemm<T>(c, nl, b, nl, OP_N, in.gl, OP_N, in.Hld, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, nl, b, b, OP_N, R(T2, nl), OP_N, R(G, b), W(G0, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         for (int i = 1; i < L; ++i) {
             const int bp = in.bsz[i - 1], bi = in.bsz[i];
             const BMat<T>& V = in.Hdv[i - 1];
             T* Gi = slotL(i) ? slotL(i) : (keep(i) && i < L - 1 ? keep(i) : other(gbufs, G));
             T* G0i = slot0(i) ? slot0(i) : other(g0bufs, G0);
-            gemm<T>(c, bi, bp, bp, OP_C, V, OP_N, R(G, bp), W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P);
-            gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, V, W(Aw, bi), -1.0, in.Hd[i], -1.0, zdev(i), P);
+            gemm<T>(c, bi, bp, bp, OP_C, V, OP_N, R(G, bp), W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, V, W(Aw, bi), -1.0, in.Hd[i], -1.0, zdev(i), P, -1, in.act);
             if (i == L - 1 && keep(i)) {
                 // full G of the last layer, closed by the right lead's surface
                 // function: inv(Aw - Hrd^+ gr Hrd)  (negf/lesser.f90:587-600)
-                gemm<T>(c, bi, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P);
-                gemm<T>(c, bi, bi, nr, OP_N, R(T1, bi), OP_N, in.Hrd, W(X, bi), -1.0, R(Aw, bi), 1.0, nullptr, P);
-                inverse<T>(c, bi, W(X, bi), none<T>(), nullptr, W(keep(i), bi), st, P);
+                gemm<T>(c, bi, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+                gemm<T>(c, bi, bi, nr, OP_N, R(T1, bi), OP_N, in.Hrd, W(X, bi), -1.0, R(Aw, bi), 1.0, nullptr, P, -1, in.act);
+                inverse<T>(c, bi, W(X, bi), none<T>(), nullptr, W(keep(i), bi), st, P, -1, in.act);
             }
-            inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gi, bi), st, P);
-            gemm<T>(c, nl, bi, bp, OP_N, R(G0, nl), OP_N, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
-            gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gi, bi), W(G0i, nl), 1.0, none<T>(), 0.0, nullptr, P);
+            inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gi, bi), st, P, -1, in.act);
+            gemm<T>(c, nl, bi, bp, OP_N, R(G0, nl), OP_N, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gi, bi), W(G0i, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
             G = Gi;
             G0 = G0i;
         }
@@ -1097,17 +748,17 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         const BMat<T> Vc = fwd ? in.Hrd_fwd : in.Hrd;
         const int opVc = fwd ? OP_N : OP_C;      // op(.) giving Vc      (bN x nr)
         const int opVcH = fwd ? OP_C : OP_N;     // op(.) giving Vc^+    (nr x bN)
-        gemm<T>(c, nr, bN, bN, opVcH, Vc, OP_N, R(G, bN), W(T1, nr), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, nr, bN, bN, opVcH, Vc, OP_N, R(G, bN), W(T1, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         // Aw = (z - Hamr - sr) - T1 Vc   (lesser.f90:616-633)
-        shift_sub<T>(c, nr, zd, in.Hamr, in.sr, W(X, nr), P);
-        gemm<T>(c, nr, nr, bN, OP_N, R(T1, nr), opVc, Vc, W(Aw, nr), -1.0, R(X, nr), 1.0, nullptr, P);
-        inverse<T>(c, nr, W(Aw, nr), none<T>(), nullptr, gRo, st, P);
-        gemm<T>(c, nl, nr, bN, OP_N, R(G0, nl), opVc, Vc, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nl, nr, nr, OP_N, R(T2, nl), OP_N, ro(gRo), gdlr, 1.0, none<T>(), 0.0, nullptr, P);
+        shift_sub<T>(c, nr, zd, in.Hamr, in.sr, W(X, nr), P, in.act);
+        gemm<T>(c, nr, nr, bN, OP_N, R(T1, nr), opVc, Vc, W(Aw, nr), -1.0, R(X, nr), 1.0, nullptr, P, -1, in.act);
+        inverse<T>(c, nr, W(Aw, nr), none<T>(), nullptr, gRo, st, P, -1, in.act);
+        gemm<T>(c, nl, nr, bN, OP_N, R(G0, nl), opVc, Vc, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, nl, nr, nr, OP_N, R(T2, nl), OP_N, ro(gRo), gdlr, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         // Caroli T_LR = Re Tr[(G_LR gmr)(G_LR^+ gml)]
-        gemm<T>(c, nl, nr, nr, OP_N, ro(gdlr), OP_N, in.gmr, W(X, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nr, nl, nl, OP_C, ro(gdlr), OP_N, in.gml, W(Y, nr), 1.0, none<T>(), 0.0, nullptr, P);
-        trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P);
+        gemm<T>(c, nl, nr, nr, OP_N, ro(gdlr), OP_N, in.gmr, W(X, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, nr, nl, nl, OP_C, ro(gdlr), OP_N, in.gml, W(Y, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P, in.act);
         if (adj) adjoint_backward<T>(c, nl, nr, in, P, mm, gdlr, gRo, Vc, opVc, opVcH, *adj, T1, T2);
     }
     // ---------------- right to left, on the auxiliary stream when enabled: it
@@ -1135,11 +786,11 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     gdrl = out.G_D_RL.base ? out.G_D_RL : W(GRL_b, nr);
     {
         const int bN = in.bsz[L - 1];
-        gemm<T>(c, bN, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, bN, bN, nr, OP_N, R(T1, bN), OP_N, in.Hrd, W(Aw, bN), -1.0, in.Hd[L - 1], -1.0, zdev(L - 1), P);
-        inverse<T>(c, bN, W(Aw, bN), none<T>(), nullptr, W(G, bN), st, P);
-        gemm<T>(c, nr, bN, nr, OP_N, in.gr, OP_N, in.Hrd, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nr, bN, bN, OP_N, R(T2, nr), OP_N, R(G, bN), W(G0, nr), 1.0, none<T>(), 0.0, nullptr, P);
+        gemm<T>(c, bN, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, bN), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, bN, bN, nr, OP_N, R(T1, bN), OP_N, in.Hrd, W(Aw, bN), -1.0, in.Hd[L - 1], -1.0, zdev(L - 1), P, -1, in.act);
+        inverse<T>(c, bN, W(Aw, bN), none<T>(), nullptr, W(G, bN), st, P, -1, in.act);
+        gemm<T>(c, nr, bN, nr, OP_N, in.gr, OP_N, in.Hrd, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, nr, bN, bN, OP_N, R(T2, nr), OP_N, R(G, bN), W(G0, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         for (int i = L - 1; i >= 1; --i) {
             const int bi = in.bsz[i], bm_ = in.bsz[i - 1];
             // V = V_{i,i-1}: Hdv_back[i-1] or Hdv[i-1]^+ (bi x bm_)
@@ -1147,29 +798,29 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
             const BMat<T> Vb = back ? in.Hdv_back[i - 1] : in.Hdv[i - 1];
             const int opV = back ? OP_N : OP_C;    // gives V   (bi x bm_)
             const int opVH = back ? OP_C : OP_N;   // gives V^+ (bm_ x bi)
-            gemm<T>(c, bm_, bi, bi, opVH, Vb, OP_N, R(G, bi), W(T1, bm_), 1.0, none<T>(), 0.0, nullptr, P);
+            gemm<T>(c, bm_, bi, bi, opVH, Vb, OP_N, R(G, bi), W(T1, bm_), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
             gemm<T>(c, bm_, bm_, bi, OP_N, R(T1, bm_), opV, Vb, W(Aw, bm_), -1.0, in.Hd[i - 1], -1.0, zdev(i - 1),
-                    P);
-            inverse<T>(c, bm_, W(Aw, bm_), none<T>(), nullptr, W(Gn, bm_), st, P);
-            gemm<T>(c, nr, bm_, bi, OP_N, R(G0, nr), opV, Vb, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P);
+                    P, -1, in.act);
+            inverse<T>(c, bm_, W(Aw, bm_), none<T>(), nullptr, W(Gn, bm_), st, P, -1, in.act);
+            gemm<T>(c, nr, bm_, bi, OP_N, R(G0, nr), opV, Vb, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
             gemm<T>(c, nr, bm_, bm_, OP_N, R(T2, nr), OP_N, R(Gn, bm_), W(G0n, nr), 1.0, none<T>(), 0.0, nullptr,
-                    P);
+                    P, -1, in.act);
             std::swap(G, Gn);
             std::swap(G0, G0n);
         }
         const int b1 = in.bsz[0];
         // closing lead layer 0: V = Hld (nl x b1); (V G) V^+  (lesser.f90:757-771)
-        gemm<T>(c, nl, b1, b1, OP_N, in.Hld, OP_N, R(G, b1), W(T1, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        shift_sub<T>(c, nl, zd, in.Haml, in.sl, W(X, nl), P);
-        gemm<T>(c, nl, nl, b1, OP_N, R(T1, nl), OP_C, in.Hld, W(Aw, nl), -1.0, R(X, nl), 1.0, nullptr, P);
+        gemm<T>(c, nl, b1, b1, OP_N, in.Hld, OP_N, R(G, b1), W(T1, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        shift_sub<T>(c, nl, zd, in.Haml, in.sl, W(X, nl), P, in.act);
+        gemm<T>(c, nl, nl, b1, OP_N, R(T1, nl), OP_C, in.Hld, W(Aw, nl), -1.0, R(X, nl), 1.0, nullptr, P, -1, in.act);
         BMatW<T> gL = out.G_L.base ? out.G_L : W(Gn, nl);
-        inverse<T>(c, nl, W(Aw, nl), none<T>(), nullptr, gL, st, P);
-        gemm<T>(c, nr, nl, b1, OP_N, R(G0, nr), OP_C, in.Hld, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nr, nl, nl, OP_N, R(T2, nr), OP_N, ro(gL), gdrl, 1.0, none<T>(), 0.0, nullptr, P);
+        inverse<T>(c, nl, W(Aw, nl), none<T>(), nullptr, gL, st, P, -1, in.act);
+        gemm<T>(c, nr, nl, b1, OP_N, R(G0, nr), OP_C, in.Hld, W(T2, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, nr, nl, nl, OP_N, R(T2, nr), OP_N, ro(gL), gdrl, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         // Caroli T_RL = Re Tr[(G_RL gml)(G_RL^+ gmr)]
-        gemm<T>(c, nr, nl, nl, OP_N, ro(gdrl), OP_N, in.gml, W(X, nr), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nl, nr, nr, OP_C, ro(gdrl), OP_N, in.gmr, W(Y, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        trace_prod<T>(c, nr, nl, R(X, nr), R(Y, nl), out.T_RL, P);
+        gemm<T>(c, nr, nl, nl, OP_N, ro(gdrl), OP_N, in.gml, W(X, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        gemm<T>(c, nl, nr, nr, OP_C, ro(gdrl), OP_N, in.gmr, W(Y, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        trace_prod<T>(c, nr, nl, R(X, nr), R(Y, nl), out.T_RL, P, in.act);
     }
     if (two) {
         hipcheck(c, hipEventRecord(c->join, c->aux), "event join");
@@ -1363,6 +1014,8 @@ struct TdbArgs {
 
 // adjoint storage is capped so that a chunk of points fits this budget
 constexpr double ADJ_BUDGET_BYTES = 24.0 * 1024 * 1024 * 1024;
+// so is the mixed-precision solve's (FP32 factors and FP64 columns of both directions)
+constexpr double MIXED_BUDGET_BYTES = 64.0 * 1024 * 1024 * 1024;
 
 int check_params(const nt_params* p)
 {
@@ -1394,6 +1047,11 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const long nmm = (long)nmax * nmax;
     const long mmr = (long)std::max(nmax, b) * std::max(nmax, b);   // RGF slot size
     int Pmax = std::min(a.npts, c->batch_limit);
+    if (prm->precision == NT_PREC_MIXED && !adjoint && !a.probe) {
+        // the mixed solve keeps both directions' FP32 factors and FP64 columns
+        const double per_pt = (double)mixed_caroli_scratch(nl, nr, b, L, 1);
+        Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, MIXED_BUDGET_BYTES / per_pt)));
+    }
     if (adjoint) {
         const double per_pt = (double)adj_slots(a.adj_lo, a.adj_hi, L) * mmr * sizeof(nt_c128);
         Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
@@ -1439,6 +1097,10 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         o_h32[5] = ar.take(c32 * nrr * a.nk);
     }
     const size_t o_zdev32 = ar.take(c32 * Pmax);
+    // mixed precision: per-point acceptance, FP64-recompute mask, counters
+    const size_t o_acc = ar.take(sizeof(int) * Pmax);
+    const size_t o_fb = ar.take(sizeof(int) * Pmax);
+    const size_t o_cnt = ar.take(sizeof(int) * 8);
     // per-side shifts, k indices, statuses and iterations (nl != nr only)
     const size_t o_zs = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
     const size_t o_zpms = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
@@ -1446,19 +1108,11 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const size_t o_sts = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
     const size_t o_its = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
     const size_t o_scr = ar.off;
-    // per-chunk FP32 lead quantities + optional FP32 outputs, then the FP32 RGF scratch
-    Arena l32;
-    size_t o_l32[10];
-    for (int i = 0; i < 3; ++i) o_l32[i] = l32.take(c32 * nll * Pmax);       // gl, sl, gml
-    for (int i = 3; i < 6; ++i) o_l32[i] = l32.take(c32 * nrr * Pmax);       // gr, sr, gmr
-    o_l32[6] = l32.take(c32 * nl * nr * Pmax);                               // G_D_LR
-    o_l32[7] = l32.take(c32 * nl * nr * Pmax);                               // G_D_RL
-    o_l32[8] = l32.take(c32 * nll * Pmax);                                   // G_L
-    o_l32[9] = l32.take(c32 * nrr * Pmax);                                   // G_R
     const size_t need_dec = lead_gfs_scratch(prm, nmax, 2 * Pmax, 2 * a.nk);
     const bool sweep32 = mixed && !adjoint;   // the adjoint runs the FP64 sweep
-    const size_t need_rgf =
-        sweep32 ? l32.off + rgf_scratch<float>(nl, nr, b, Pmax) : rgf_scratch<T>(nl, nr, b, Pmax);
+    // the mixed solve, then (for rejected points) the FP64 sweep, share the region
+    const size_t need_rgf = sweep32 ? std::max(mixed_caroli_scratch(nl, nr, b, L, Pmax), rgf_scratch<T>(nl, nr, b, Pmax))
+                                    : rgf_scratch<T>(nl, nr, b, Pmax);
     const size_t o_adj = o_scr + std::max(need_dec, need_rgf);
     size_t need_adj = 0;
     if (adjoint) {
@@ -1693,22 +1347,12 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                 hipcheck(c, hipGetLastError(), "adjoint reduce");
             }
         } else {
-            // Mixed precision: the surface GFs above stay FP64 (decimation); the
-            // sweep runs in complex FP32 on MFMA with FP32 accumulation, and the
-            // Caroli traces accumulate in FP64.
-            Arena la = ar;
-            la.off = o_scr;
-            float* L32[10];
-            for (int i = 0; i < 10; ++i) L32[i] = ar.ptr<float>(o_scr + o_l32[i]);
+            // Mixed precision: the surface GFs above stay FP64 (decimation);
+            // each sweep direction is factorised in complex FP32 on MFMA and
+            // its block column refined against FP64 residuals (nt_mixed.hip).
+            // Points whose refinement does not pass the error estimate are
+            // recomputed by the FP64 sweep, masked to those points.
             const float one = 1.0f;
-            convert_op<T, float>(c, nl, nl, bm<T>(gbuf, 2 * nmm, nl), bw<float>(L32[0], nll, nl), 1, one, P);
-            convert_op<T, float>(c, nl, nl, bm<T>(sigbuf, 2 * nmm, nl), bw<float>(L32[1], nll, nl), 1, one, P);
-            convert_op<T, float>(c, nl, nl, bm<T>(gambuf, 2 * nmm, nl), bw<float>(L32[2], nll, nl), 0, -one, P);
-            convert_op<T, float>(c, nr, nr, bm<T>(gbuf + 2 * nmm, 2 * nmm, nr), bw<float>(L32[3], nrr, nr), 0, one, P);
-            convert_op<T, float>(c, nr, nr, bm<T>(sigbuf + 2 * nmm, 2 * nmm, nr), bw<float>(L32[4], nrr, nr), 0, one,
-                                 P);
-            convert_op<T, float>(c, nr, nr, bm<T>(gambuf + 2 * nmm, 2 * nmm, nr), bw<float>(L32[5], nrr, nr), 0, one,
-                                 P);
             convert_op<T, float>(c, 1, 1, bm<T>(zdev, 1, 1), bw<float>(zdev32, 1, 1), 0, one, P);
             RgfInput<float> i32;
             i32.L = L;
@@ -1720,29 +1364,24 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             i32.Hld = bm<float>(H32[2], kst((long)nl * b), nl, kidx, 0, kn);
             i32.Hrd = bm<float>(H32[3], kst((long)nr * b), nr, kidx, 0, kn);
             i32.Hrd_fwd = none<float>();
-            i32.Haml = bm<float>(H32[4], kst(nll), nl, kidx, 0, kn);
-            i32.Hamr = bm<float>(H32[5], kst(nrr), nr, kidx, 0, kn);
-            i32.gl = bm<float>(L32[0], nll, nl);
-            i32.sl = bm<float>(L32[1], nll, nl);
-            i32.gml = bm<float>(L32[2], nll, nl);
-            i32.gr = bm<float>(L32[3], nrr, nr);
-            i32.sr = bm<float>(L32[4], nrr, nr);
-            i32.gmr = bm<float>(L32[5], nrr, nr);
-            RgfOutput<float> o32;
-            o32.T_LR = a.T_LR + p0;
-            o32.T_RL = a.T_RL + p0;
-            o32.status = strgf;
-            const bool gx = a.ex.G_D_LR || a.ex.G_D_RL || a.ex.G_L || a.ex.G_R;
-            o32.G_D_LR = gx ? bw<float>(L32[6], (long)nl * nr, nl) : nonew<float>();
-            o32.G_D_RL = gx ? bw<float>(L32[7], (long)nl * nr, nr) : nonew<float>();
-            o32.G_L = gx ? bw<float>(L32[8], nll, nl) : nonew<float>();
-            o32.G_R = gx ? bw<float>(L32[9], nrr, nr) : nonew<float>();
-            rgf_run<float>(c, nl, nr, i32, zdev32, P, o32, ar, o_scr + l32.off);
-            const double d1 = 1.0;
-            if (out.G_D_LR.base) convert_op<float, T>(c, nl, nr, ro(o32.G_D_LR), out.G_D_LR, 0, d1, P);
-            if (out.G_D_RL.base) convert_op<float, T>(c, nr, nl, ro(o32.G_D_RL), out.G_D_RL, 0, d1, P);
-            if (out.G_L.base) convert_op<float, T>(c, nl, nl, ro(o32.G_L), out.G_L, 0, d1, P);
-            if (out.G_R.base) convert_op<float, T>(c, nr, nr, ro(o32.G_R), out.G_R, 0, d1, P);
+            int* acc = ar.ptr<int>(o_acc);
+            int* fb = ar.ptr<int>(o_fb);
+            int* cnt = ar.ptr<int>(o_cnt);
+            mixed_caroli(c, nl, nr, in, i32, zdev, zdev32, P, out, ar, o_scr, acc, cnt);
+            hipcheck(c, hipMemsetAsync(cnt + 6, 0, sizeof(int) * 2, s), "memset");
+            hipLaunchKernelGGL(k_reject_mask, dim3(nblk(P)), dim3(256), 0, s, acc, fb, cnt + 6, P);
+            hipcheck(c, hipGetLastError(), "reject mask");
+            int hc[8];
+            hipcheck(c, hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s), "memcpy");
+            hipcheck(c, hipStreamSynchronize(s), "sync");
+            c->mixed_counts[0] += P;
+            c->mixed_counts[1] += hc[1] + hc[4];
+            c->mixed_counts[2] += hc[6];
+            if (hc[6] > 0) {
+                RgfInput<T> inf = in;
+                inf.act = fb;
+                rgf_run<T>(c, nl, nr, inf, zdev, P, out, ar, o_scr);
+            }
         }
 
         // ---- optional lead outputs
@@ -2569,6 +2208,14 @@ int nt_profile_reset(nt_ctx* c)
         c->ms[k] = 0;
         c->flops[k] = 0;
     }
+    for (int k = 0; k < 3; ++k) c->mixed_counts[k] = 0;
+    return NT_OK;
+}
+
+int nt_mixed_counts(nt_ctx* c, long long* counts)
+{
+    if (!c || !counts) return NT_ERR_ARG;
+    for (int k = 0; k < 3; ++k) counts[k] = c->mixed_counts[k];
     return NT_OK;
 }
 

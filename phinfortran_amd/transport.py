@@ -567,5 +567,12 @@ class Engine:
         self._check(self._L.nt_profile_get(self.ctx, int(kind), C.byref(n), C.byref(ms), C.byref(fl)), "nt_profile_get")
         return dict(launches=n.value, ms=ms.value, flops=fl.value)
 
+    def mixed_counts(self):
+        """(points, second refinement steps, FP64 recomputes) of mixed-precision
+        solves since the last profile_reset."""
+        out = (C.c_longlong * 3)()
+        self._check(self._L.nt_mixed_counts(self.ctx, C.cast(out, C.c_void_p)), "mixed_counts")
+        return tuple(int(v) for v in out)
+
     def last_mean_iters(self) -> float:
         return float(self._L.nt_last_mean_iters(self.ctx))

@@ -72,10 +72,11 @@ def check_vs_oracle(name, blk, E, r, op, tol=TOL):
 
 
 def test_wire_b96_L20(engine, T):
-    """Si/Ge alloy wire at its full shape; eta = 1e-6 omega_max^2 as eta0."""
+    """Si/Ge alloy wire at its full shape; eta(E) = 1e-6 E + 1e-6 omega_max
+    (eta0 = 1.6e-5 THz^2)."""
     blk = lt.si_ge_wire(nx=2, ny=2, L=20, seed=0)
     assert (blk.b, blk.L) == (96, 20)
-    p = T.params("phonon", eta=1e-6, eta0=1e-6 * 16.0 ** 2)
+    p = T.params("phonon", eta=1e-6, eta0=1e-6 * 16.0)
     E = omega_grid(512)[7::16] ** 2
     print("wire b=96 L=20: solve", flush=True)
     d = engine.upload_blocks(blk)

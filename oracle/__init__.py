@@ -75,6 +75,8 @@ def lib():
                                       C.c_double, C.POINTER(OrParams), C.c_int, P, P, P]
         _lib.or_solve_tdb_batch.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P,
                                             C.c_int, P, C.POINTER(OrParams), C.c_int, P, P, P]
+        _lib.or_solve_tdb_batch_kappa.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P,
+                                                  C.c_int, P, C.POINTER(OrParams), C.c_int, P, P, P, P]
         _lib.or_circular_g.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, CDouble, C.c_double, C.c_int,
                                        P, C.POINTER(C.c_int)]
         _lib.or_get_ldos.argtypes = [C.c_int, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_double,
@@ -212,6 +214,20 @@ def solve_tdb_batch(blocks, E, params, nthreads=0):
     lib().or_solve_tdb_batch(nl, nr, b, L, *[_p(a) for a in arrs], E.size, _p(E), C.byref(params),
                              int(nthreads), _p(tl), _p(tr), _p(status))
     return tl, tr, status
+
+
+def solve_tdb_batch_kappa(blocks, E, params, nthreads=0):
+    """solve_tdb_batch plus, per point, the largest 1-norm condition number of
+    the blocks the solve inverted: two correct FP64 algorithms may differ by
+    about eps times it."""
+    nl, b, L, nr = _dims(blocks)
+    arrs = _pack(blocks)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    tl, tr, kappa = np.zeros(E.size), np.zeros(E.size), np.zeros(E.size)
+    status = np.zeros(E.size, np.int32)
+    lib().or_solve_tdb_batch_kappa(nl, nr, b, L, *[_p(a) for a in arrs], E.size, _p(E), C.byref(params),
+                                   int(nthreads), _p(tl), _p(tr), _p(status), _p(kappa))
+    return tl, tr, status, kappa
 
 
 def _dims(blocks):

@@ -103,6 +103,22 @@ static void mm(char oa, char ob, int m, int n, int k, const ocplx* A, const ocpl
     or_gemm(oa, ob, m, n, k, 1.0, A, lda, B, ldb, 0.0, C, m);
 }
 
+/* Largest 1-norm condition number ||A||_1 ||A^-1||_1 of the blocks inverted
+ * on this thread since the last reset: the rounding scale by which two
+ * correct FP64 algorithms may differ (used by the tests' tolerances). */
+static _Thread_local double kappa_max = 0.0;
+
+static double norm1(int n, const ocplx* A)
+{
+    double best = 0.0;
+    for (int j = 0; j < n; ++j) {
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) s += cabs(A[i + (size_t)j * n]);
+        if (s > best) best = s;
+    }
+    return best;
+}
+
 /* ------------------------------------------------------------- inverse */
 /* negf/util.f90:16-67: NaN scan, ZGETRF (partial pivoting on |re|+|im|, first
  * maximum wins as IZAMAX does), ZGETRI.  The inverse is formed by solving
@@ -172,6 +188,10 @@ int or_inv(int n, const ocplx* A, ocplx* Ainv)
         }
     }
     free(LU); free(piv);
+    {
+        const double k = norm1(n, A) * norm1(n, Ainv);
+        if (k > kappa_max || k != k) kappa_max = k;
+    }
     return 0;
 }
 
@@ -490,6 +510,32 @@ int or_solve_tdb_batch(int nl, int nr, int b, int L,
     for (int i = 0; i < npts; ++i) {
         int st = or_solve_tdb(nl, nr, b, L, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
                               E[i], p, 0, &T_LR[i], &T_RL[i], NULL);
+        if (status) status[i] = st;
+        if (st < worst) worst = st;
+        if (st > 0 && worst == 0) worst = st;
+    }
+    (void)nthreads;
+    return worst;
+}
+
+/* or_solve_tdb_batch plus, per point, the largest condition number of the
+ * blocks its solve inverted (kappa_max above). */
+int or_solve_tdb_batch_kappa(int nl, int nr, int b, int L,
+                             const ocplx* Haml, const ocplx* Hamlv, const ocplx* Hamr, const ocplx* Hamrv,
+                             const ocplx* Hamd, const ocplx* Hamdv, const ocplx* Hamld, const ocplx* Hamrd,
+                             int npts, const double* E, const or_params* p, int nthreads,
+                             double* T_LR, double* T_RL, int* status, double* kappa)
+{
+    int worst = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : worst)
+#endif
+    for (int i = 0; i < npts; ++i) {
+        kappa_max = 0.0;
+        int st = or_solve_tdb(nl, nr, b, L, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
+                              E[i], p, 0, &T_LR[i], &T_RL[i], NULL);
+        kappa[i] = kappa_max;
         if (status) status[i] = st;
         if (st < worst) worst = st;
         if (st > 0 && worst == 0) worst = st;

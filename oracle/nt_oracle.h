@@ -119,6 +119,13 @@ int or_solve_tdb_batch(int nl, int nr, int b, int L,
                        const ocplx* Hamd, const ocplx* Hamdv, const ocplx* Hamld, const ocplx* Hamrd,
                        int npts, const double* E, const or_params* p, int nthreads,
                        double* T_LR, double* T_RL, int* status);
+/* or_solve_tdb_batch plus kappa[i]: the largest 1-norm condition number of
+ * the blocks point i's solve inverted (the rounding scale of the result). */
+int or_solve_tdb_batch_kappa(int nl, int nr, int b, int L,
+                             const ocplx* Haml, const ocplx* Hamlv, const ocplx* Hamr, const ocplx* Hamrv,
+                             const ocplx* Hamd, const ocplx* Hamdv, const ocplx* Hamld, const ocplx* Hamrd,
+                             int npts, const double* E, const or_params* p, int nthreads,
+                             double* T_LR, double* T_RL, int* status, double* kappa);
 
 /* Block-circulant decimation (negf/circular.f90:43-114): the lead supercell is
  * nbx*nby transverse copies; P1/P2 are unitary block DFTs.  Same result as

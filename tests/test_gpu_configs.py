@@ -21,10 +21,14 @@ get_G_full_tdb, negf/surface.f90:385-605, negf/lesser.f90:475-809):
             the CPU), a short device of the same cross-section live, and
             full-size invariants
 
-FP64 tolerance: |T_gpu - T_oracle| <= 1e-9 max(1, T).  Both are FP64 but the
-engine inverts by blocked Gauss-Jordan with MFMA sums and the oracle by LU
-with sequential sums, so they agree to eps times the conditioning of the
-blocks the sweep inverts; the printed maxima show the achieved agreement.
+FP64 tolerance, per point: |T_gpu - T_oracle| <= max(1e-10, 256 eps kappa)
+max(1, T), kappa the largest 1-norm condition number of the blocks the
+oracle's solve inverted.  Both are FP64, but the engine inverts by blocked
+Gauss-Jordan with MFMA sums and the oracle by LU with sequential sums, so
+they agree to eps times the conditioning of what is inverted: 1e-10 where
+that is small, and the rounding scale where it is not (the AGNR at
+eta = 1e-8 eV has kappa up to ~1e7).  The printed lines give the achieved
+agreement and how many points the rounding scale relaxed.
 """
 import json
 import os
@@ -39,7 +43,8 @@ from phinfortran_amd import lattice as lt
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-TOL = 1e-9
+TOL = 1e-9          # committed b = 768 fixture (no per-point kappa stored)
+EPS = 2.220446049250313e-16
 NTHREADS = min(16, os.cpu_count() or 1)
 
 
@@ -57,17 +62,20 @@ def omega_grid(n, wmax=16.0):
     return wmax * np.arange(1, n + 1) / n
 
 
-def check_vs_oracle(name, blk, E, r, op, tol=TOL):
+def check_vs_oracle(name, blk, E, r, op):
     t0 = time.time()
-    otl, otr, ost = O.solve_tdb_batch(blk.as_dict(), E, op, nthreads=NTHREADS)
+    otl, otr, ost, kappa = O.solve_tdb_batch_kappa(blk.as_dict(), E, op, nthreads=NTHREADS)
     assert (ost == 0).all()
+    tol = np.maximum(1e-10, 256 * EPS * kappa)
     tl, tr = r["T_LR"].cpu().numpy(), r["T_RL"].cpu().numpy()
-    scale = np.maximum(1.0, np.abs(otl))
-    dl, dr = np.abs(tl - otl) / scale, np.abs(tr - otr) / np.maximum(1.0, np.abs(otr))
+    dl = np.abs(tl - otl) / np.maximum(1.0, np.abs(otl))
+    dr = np.abs(tr - otr) / np.maximum(1.0, np.abs(otr))
     print(f"{name}: {E.size} points, max |dT|/max(1,T) LR {dl.max():.2e} RL {dr.max():.2e}, "
+          f"max |dT|/tol {max((dl / tol).max(), (dr / tol).max()):.2f}, "
+          f"{int((tol > 1e-10).sum())} point(s) at the rounding scale (kappa up to {kappa.max():.1e}), "
           f"T in [{otl.min():.3g}, {otl.max():.3g}], oracle {time.time() - t0:.1f}s", flush=True)
-    assert dl.max() < tol, (name, int(dl.argmax()), tl[dl.argmax()], otl[dl.argmax()])
-    assert dr.max() < tol, (name, int(dr.argmax()), tr[dr.argmax()], otr[dr.argmax()])
+    assert (dl <= tol).all(), (name, int((dl / tol).argmax()), tl, otl, kappa)
+    assert (dr <= tol).all(), (name, int((dr / tol).argmax()), tr, otr, kappa)
     return otl, otr
 
 
@@ -109,7 +117,9 @@ def test_ribbon_b80_L60(engine, T):
 
 def test_pristine_ribbon_channels_b80_L60(engine, T):
     """Pristine ribbon (device = lead): T is the number of right-moving
-    propagating lead modes, an integer, at every energy."""
+    propagating lead modes, an integer up to the eta damping of slow modes
+    over the 60 layers (|T - round(T)| ~ 1e-5 at eta = 1e-8 eV), at every
+    energy; the same T as the oracle."""
     blk = lt.agnr_blocks(n_dimer=40, L=60, hop_noise=0.0, onsite_w=0.0, seed=0)
     p = T.params("tb", eta=1e-8)
     E = np.linspace(-3.0, 3.0, 2048)[5::32]
@@ -122,10 +132,12 @@ def test_pristine_ribbon_channels_b80_L60(engine, T):
     frac = np.abs(tl - np.round(tl))
     print(f"pristine ribbon: max |T - round(T)| {frac.max():.2e}, |T_LR - T_RL| {np.abs(tl - tr).max():.2e}, "
           f"channels {sorted(set(np.round(tl).astype(int)))}", flush=True)
-    assert frac.max() < 1e-6
+    assert frac.max() < 1e-3
     assert np.abs(tl - tr).max() < 1e-6
     assert (np.round(tl).astype(int) == nprop[:, 0]).all()
+    assert (np.round(tl).astype(int) == nprop[:, 2]).all()
     assert len(set(np.round(tl).astype(int))) >= 5
+    check_vs_oracle("pristine ribbon", blk, E, m, oparams(p))
 
 
 UNEVEN = lt.superlattice_set(256, seed=0)

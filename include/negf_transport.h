@@ -222,6 +222,27 @@ int nt_solve_tdb_batched_dev(nt_ctx* ctx, const nt_params* p, int nl, int nr, in
                              double* T_LR, double* T_RL, int* status, int* iters,
                              const nt_solve_extras* extras);
 
+/* nt_solve_tdb_batched_dev without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results; the workspace is a static device
+ * array of 256 MiB per GPU (allocated when the library is loaded, held until
+ * the process ends) instead of the context's growing workspace, and the
+ * points are solved in chunks as large as fits in it (b = 192, L = 40, FP64:
+ * about 12 points per chunk).  NT_ERR_ARG when a single point does not fit
+ * (nt_last_error gives the size) and for circular decimation (p->circular),
+ * whose transforms are allocated.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams or threads) race. */
+int nt_solve_tdb_batched_noalloc_dev(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b, int L,
+                                     int npts, const double* E, const int* kidx, int nk,
+                                     const nt_c128* Haml, const nt_c128* Hamlv,
+                                     const nt_c128* Hamr, const nt_c128* Hamrv,
+                                     const nt_c128* Hamd, const nt_c128* Hamdv,
+                                     const nt_c128* Hamld, const nt_c128* Hamrd,
+                                     const int* first_point_mask,
+                                     double* T_LR, double* T_RL, int* status, int* iters,
+                                     const nt_solve_extras* extras);
+
 /* Several devices of uneven block size b, lead sizes and layer count L in
  * one call (every pointer of a descriptor is device memory, laid out as for
  * nt_solve_tdb_batched_dev).  Devices are solved largest-first (cost

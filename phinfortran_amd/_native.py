@@ -36,6 +36,7 @@ EXPORTS = [
     "nt_params_default", "nt_params_validate", "nt_create", "nt_destroy", "nt_sync", "nt_stream", "nt_set_stream",
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
+    "nt_solve_tdb_batched_noalloc_dev",
     "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_hk_assemble_orb_dev", "nt_autotune_inverse", "nt_set_inverse_panel",
     "nt_set_overlap", "nt_set_gemm_3m",
     "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters", "nt_mixed_counts",
@@ -150,6 +151,7 @@ def lib():
     L.nt_G_block.argtypes = [P, pp, I, I, I, P, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]
     solve_args = [P, pp, I, I, I, I, I, P, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, C.POINTER(NtSolveExtras)]
     L.nt_solve_tdb_batched_dev.argtypes = solve_args
+    L.nt_solve_tdb_batched_noalloc_dev.argtypes = solve_args
     L.nt_solve_tdb_batched.argtypes = solve_args
     L.nt_solve_tdb_adjoint_batched_dev.argtypes = solve_args[:19] + [P, I, I, P, P, P, P, P, P]
     L.nt_buttiker_tdb_batched_dev.argtypes = solve_args[:2] + [C.POINTER(NtProbeDesc)] + solve_args[2:] + [

@@ -15,7 +15,30 @@
 
 #include "nt_host.h"
 
+// Static device scratch of the allocation-free batched solve
+// (nt_solve_tdb_batched_noalloc_dev).  Each GPU the library runs on holds its
+// own copy until the process ends.  Host code never uses the array's name as
+// an address: nt_create looks up each device's copy with hipGetSymbolAddress
+// and every use goes through that pointer (scratch_ptr).
+constexpr size_t NT_SCRATCH_BYTES = 256ull << 20;
+static __device__ __align__(256) char scratch[NT_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int NT_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[NT_MAX_DEVICES] = {};
+
+// device address of this context's device's scratch (set by nt_create)
+void* scratch_ptr(nt_ctx* c)
+{
+    void* p = c->device >= 0 && c->device < NT_MAX_DEVICES ? g_scratch_dev[c->device] : nullptr;
+    if (!p) {
+        c->err = "static scratch of this device is not set up";
+        throw Fail{NT_ERR_HIP};
+    }
+    return p;
+}
 
 // ------------------------------------------------------- engine kernels
 __global__ void k_lead_setup(const double* E, const int* kidx, int nk, int P, int two_sided, double eta, double eta0,
@@ -1010,6 +1033,9 @@ struct TdbArgs {
     nt_c128* dHd = nullptr;    // (nk, hi - lo + 1, b, b), accumulated
     nt_c128* dHdv = nullptr;   // (nk, min(hi, L-2) - lo + 1, b, b), accumulated
     const ProbeSpec* probe = nullptr;
+    // workspace in the static scratch (no device allocation); the chunk of
+    // points shrinks until the plan fits
+    bool noalloc = false;
 };
 
 // adjoint storage is capped so that a chunk of points fits this budget
@@ -1062,79 +1088,97 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
     }
 
-    // ---- plan the workspace for one chunk of Pmax points
-    Arena ar;
-    const size_t o_idx = ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_zdec = ar.take(sizeof(double) * 4 * Pmax);
-    const size_t o_zpm = ar.take(sizeof(double) * 4 * Pmax);
-    const size_t o_zdev = ar.take(sizeof(double) * 2 * Pmax);
-    const size_t o_stdec = ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_itdec = ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_strgf = ar.take(sizeof(int) * Pmax);
-    const size_t o_stpm = ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_itpm = ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_lh = ar.take(sizeof(nt_c128) * 2 * nmm * a.nk);
-    const size_t o_lv = ar.take(sizeof(nt_c128) * 2 * nmm * a.nk);
-    const size_t o_g = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);      // [gl_adv, gr_ret] per point
-    const size_t o_sig = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);
-    const size_t o_gam = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);
-    const size_t o_gpm = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);    // [gl_plus, gr_minus]
-    const size_t o_glr = ar.take(sizeof(nt_c128) * nll * Pmax);        // gl = gl_adv^+
-    const size_t o_slr = ar.take(sizeof(nt_c128) * nll * Pmax);        // sl = sl_adv^+
-    const size_t o_gmlr = ar.take(sizeof(nt_c128) * nll * Pmax);       // gml = -gml_adv
-    // mixed precision: FP32 copies of the H(k) blocks (once per call) and of
-    // the lead quantities and device shifts (per chunk)
     const bool mixed = prm->precision == NT_PREC_MIXED && !probe;   // the probe solve runs in FP64
     const size_t c32 = 2 * sizeof(float);
     const long Ldv = std::max(L - 1, 1);
-    size_t o_h32[6] = {0, 0, 0, 0, 0, 0};
-    if (mixed) {
-        o_h32[0] = ar.take(c32 * bb * L * a.nk);
-        o_h32[1] = ar.take(c32 * bb * Ldv * a.nk);
-        o_h32[2] = ar.take(c32 * nl * b * a.nk);
-        o_h32[3] = ar.take(c32 * nr * b * a.nk);
-        o_h32[4] = ar.take(c32 * nll * a.nk);
-        o_h32[5] = ar.take(c32 * nrr * a.nk);
-    }
-    const size_t o_zdev32 = ar.take(c32 * Pmax);
-    // mixed precision: per-point acceptance, FP64-recompute mask, counters
-    const size_t o_acc = ar.take(sizeof(int) * Pmax);
-    const size_t o_fb = ar.take(sizeof(int) * Pmax);
-    const size_t o_cnt = ar.take(sizeof(int) * 8);
-    // per-side shifts, k indices, statuses and iterations (nl != nr only)
-    const size_t o_zs = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
-    const size_t o_zpms = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
-    const size_t o_idx1 = same ? 0 : ar.take(sizeof(int) * Pmax);
-    const size_t o_sts = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_its = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
-    const size_t o_scr = ar.off;
-    const size_t need_dec = lead_gfs_scratch(prm, nmax, 2 * Pmax, 2 * a.nk);
     const bool sweep32 = mixed && !adjoint;   // the adjoint runs the FP64 sweep
-    // the mixed solve, then (for rejected points) the FP64 sweep, share the region
-    const size_t need_rgf = sweep32 ? std::max(mixed_caroli_scratch(nl, nr, b, L, Pmax), rgf_scratch<T>(nl, nr, b, Pmax))
-                                    : rgf_scratch<T>(nl, nr, b, Pmax);
-    const size_t o_adj = o_scr + std::max(need_dec, need_rgf);
-    size_t need_adj = 0;
-    if (adjoint) {
-        Arena probe;
-        (void)adj_carve(probe, a.adj_lo, a.adj_hi, L, mmr, Pmax, nullptr);
-        need_adj = probe.off;
-    }
-    // probe region: per-layer shifts and broadenings, traces, kept blocks and
-    // the work slots of buttiker_run
-    Arena pa;
-    pa.off = o_adj + need_adj;
+    // ---- plan the workspace for one chunk of Pmax points.  Without
+    // allocation (the static scratch) the chunk shrinks until the plan fits.
+    Arena ar, pa;
+    size_t o_idx, o_zdec, o_zpm, o_zdev, o_stdec, o_itdec, o_strgf, o_stpm, o_itpm, o_lh, o_lv, o_g, o_sig, o_gam, o_gpm, o_glr, o_slr, o_gmlr, o_zdev32, o_acc, o_fb, o_cnt, o_zs, o_zpms, o_idx1, o_sts, o_its, o_scr, o_adj, need_dec, need_rgf, need_adj = 0;
+    size_t o_h32[6] = {0, 0, 0, 0, 0, 0};
     size_t o_pzl = 0, o_pgam = 0, o_pdgam = 0, o_pacc = 0, o_pkeep = 0, o_pwork = 0;
-    if (probe) {
-        o_pzl = pa.take(sizeof(double) * 2 * L * Pmax);
-        o_pgam = pa.take(sizeof(double) * L * Pmax);
-        o_pdgam = pa.take(sizeof(double) * L * Pmax);
-        o_pacc = pa.take(sizeof(double) * 2 * probe_acc_n(L) * Pmax);
-        o_pkeep = pa.take(sizeof(nt_c128) * mmr * L * Pmax);
-        o_pwork = pa.take(sizeof(nt_c128) * mmr * (probe_slots(L) - L) * Pmax);
+    for (;;) {
+        ar = Arena();
+        o_idx = ar.take(sizeof(int) * 2 * Pmax);
+        o_zdec = ar.take(sizeof(double) * 4 * Pmax);
+        o_zpm = ar.take(sizeof(double) * 4 * Pmax);
+        o_zdev = ar.take(sizeof(double) * 2 * Pmax);
+        o_stdec = ar.take(sizeof(int) * 2 * Pmax);
+        o_itdec = ar.take(sizeof(int) * 2 * Pmax);
+        o_strgf = ar.take(sizeof(int) * Pmax);
+        o_stpm = ar.take(sizeof(int) * 2 * Pmax);
+        o_itpm = ar.take(sizeof(int) * 2 * Pmax);
+        o_lh = ar.take(sizeof(nt_c128) * 2 * nmm * a.nk);
+        o_lv = ar.take(sizeof(nt_c128) * 2 * nmm * a.nk);
+        o_g = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);      // [gl_adv, gr_ret] per point
+        o_sig = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);
+        o_gam = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);
+        o_gpm = ar.take(sizeof(nt_c128) * 2 * nmm * Pmax);    // [gl_plus, gr_minus]
+        o_glr = ar.take(sizeof(nt_c128) * nll * Pmax);        // gl = gl_adv^+
+        o_slr = ar.take(sizeof(nt_c128) * nll * Pmax);        // sl = sl_adv^+
+        o_gmlr = ar.take(sizeof(nt_c128) * nll * Pmax);       // gml = -gml_adv
+        // mixed precision: FP32 copies of the H(k) blocks (once per call) and of
+        // the lead quantities and device shifts (per chunk)
+        if (mixed) {
+            o_h32[0] = ar.take(c32 * bb * L * a.nk);
+            o_h32[1] = ar.take(c32 * bb * Ldv * a.nk);
+            o_h32[2] = ar.take(c32 * nl * b * a.nk);
+            o_h32[3] = ar.take(c32 * nr * b * a.nk);
+            o_h32[4] = ar.take(c32 * nll * a.nk);
+            o_h32[5] = ar.take(c32 * nrr * a.nk);
+        }
+        o_zdev32 = ar.take(c32 * Pmax);
+        // mixed precision: per-point acceptance, FP64-recompute mask, counters
+        o_acc = ar.take(sizeof(int) * Pmax);
+        o_fb = ar.take(sizeof(int) * Pmax);
+        o_cnt = ar.take(sizeof(int) * 8);
+        // per-side shifts, k indices, statuses and iterations (nl != nr only)
+        o_zs = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
+        o_zpms = same ? 0 : ar.take(sizeof(double) * 4 * Pmax);
+        o_idx1 = same ? 0 : ar.take(sizeof(int) * Pmax);
+        o_sts = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
+        o_its = same ? 0 : ar.take(sizeof(int) * 2 * Pmax);
+        o_scr = ar.off;
+        need_dec = lead_gfs_scratch(prm, nmax, 2 * Pmax, 2 * a.nk);
+        // the mixed solve, then (for rejected points) the FP64 sweep, share the region
+        need_rgf = sweep32 ? std::max(mixed_caroli_scratch(nl, nr, b, L, Pmax), rgf_scratch<T>(nl, nr, b, Pmax))
+                                        : rgf_scratch<T>(nl, nr, b, Pmax);
+        o_adj = o_scr + std::max(need_dec, need_rgf);
+        need_adj = 0;
+        if (adjoint) {
+            Arena probe;
+            (void)adj_carve(probe, a.adj_lo, a.adj_hi, L, mmr, Pmax, nullptr);
+            need_adj = probe.off;
+        }
+        // probe region: per-layer shifts and broadenings, traces, kept blocks and
+        // the work slots of buttiker_run
+        pa = Arena();
+        pa.off = o_adj + need_adj;
+        if (probe) {
+            o_pzl = pa.take(sizeof(double) * 2 * L * Pmax);
+            o_pgam = pa.take(sizeof(double) * L * Pmax);
+            o_pdgam = pa.take(sizeof(double) * L * Pmax);
+            o_pacc = pa.take(sizeof(double) * 2 * probe_acc_n(L) * Pmax);
+            o_pkeep = pa.take(sizeof(nt_c128) * mmr * L * Pmax);
+            o_pwork = pa.take(sizeof(nt_c128) * mmr * (probe_slots(L) - L) * Pmax);
+        }
+        if (!a.noalloc || pa.off <= NT_SCRATCH_BYTES) break;
+        if (Pmax == 1) {
+            char buf[200];
+            snprintf(buf, sizeof(buf), "one point needs %zu bytes of workspace, more than the %zu-byte static scratch",
+                     pa.off, (size_t)NT_SCRATCH_BYTES);
+            c->err = buf;
+            throw Fail{NT_ERR_ARG};
+        }
+        Pmax = std::max(1, std::min(Pmax - 1, (int)((double)Pmax * NT_SCRATCH_BYTES / pa.off)));
     }
-    ensure_ws(c, pa.off);
-    ar.base = static_cast<char*>(c->ws);
+    if (a.noalloc) {
+        ar.base = static_cast<char*>(scratch_ptr(c));
+    } else {
+        ensure_ws(c, pa.off);
+        ar.base = static_cast<char*>(c->ws);
+    }
     hipStream_t s = c->stream;
 
     int* idx = ar.ptr<int>(o_idx);
@@ -1447,6 +1491,8 @@ int guarded(nt_ctx* c, F&& f)
         hipcheck(c, hipSetDevice(c->device), "hipSetDevice");
         c->regions.clear();
         reg_region(c, c->ws, c->ws_bytes);
+        if (c->device >= 0 && c->device < NT_MAX_DEVICES && g_scratch_dev[c->device])
+            reg_region(c, g_scratch_dev[c->device], NT_SCRATCH_BYTES);
         int r = f();
         if (c->prof) harvest_profile(c);
         return r;
@@ -2128,6 +2174,18 @@ int nt_create(nt_ctx** out, int device_ordinal)
         return NT_ERR_HIP;
     }
     c->stream = c->own;
+    {
+        // this device's copy of the static scratch, looked up once per device
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (device_ordinal < NT_MAX_DEVICES && !g_scratch_dev[device_ordinal]) {
+            void* ptr = nullptr;
+            if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+                nt_destroy(c);
+                return NT_ERR_HIP;
+            }
+            g_scratch_dev[device_ordinal] = ptr;
+        }
+    }
     *out = c;
     return NT_OK;
 }
@@ -2237,9 +2295,11 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
                      const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
                      const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
                      const nt_c128* Hamrd, const int* first_point_mask, double* T_LR, double* T_RL, int* status,
-                     int* iters, const nt_solve_extras* extras, const nt_probe_desc* pd, const nt_probe_out* po)
+                     int* iters, const nt_solve_extras* extras, const nt_probe_desc* pd, const nt_probe_out* po,
+                     bool noalloc = false)
 {
     if (!c) return NT_ERR_ARG;
+    if (noalloc && (p && p->circular)) return NT_ERR_ARG;   // circular decimation allocates its transforms
     if (check_params(p) || check_circular(p, nl, nr)) return NT_ERR_ARG;
     if (nl <= 0 || nr <= 0 || b <= 0 || L <= 0 || npts < 0 || nk <= 0 || (nk > 1 && !kidx)) return NT_ERR_ARG;
     if (std::max(std::max(nl, nr), b) > inv_max_n()) return NT_ERR_ARG;
@@ -2293,6 +2353,7 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
         }
         TdbArgs a{nl, nr, b, L, npts, nk, E, kidx, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
                   first_point_mask, T_LR, T_RL, status, iters, ex};
+        a.noalloc = noalloc;
         ProbeSpec ps{};
         if (pd) {
             reg_region(c, pd->temperature, sizeof(double) * L);
@@ -2319,6 +2380,17 @@ int nt_solve_tdb_batched_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int 
 {
     return tdb_entry(c, p, nl, nr, b, L, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
                      first_point_mask, T_LR, T_RL, status, iters, extras, nullptr, nullptr);
+}
+
+int nt_solve_tdb_batched_noalloc_dev(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts,
+                                     const double* E, const int* kidx, int nk, const nt_c128* Haml,
+                                     const nt_c128* Hamlv, const nt_c128* Hamr, const nt_c128* Hamrv,
+                                     const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
+                                     const nt_c128* Hamrd, const int* first_point_mask, double* T_LR, double* T_RL,
+                                     int* status, int* iters, const nt_solve_extras* extras)
+{
+    return tdb_entry(c, p, nl, nr, b, L, npts, E, kidx, nk, Haml, Hamlv, Hamr, Hamrv, Hamd, Hamdv, Hamld, Hamrd,
+                     first_point_mask, T_LR, T_RL, status, iters, extras, nullptr, nullptr, true);
 }
 
 int nt_buttiker_tdb_batched_dev(nt_ctx* c, const nt_params* p, const nt_probe_desc* probe, int nl, int nr, int b,

@@ -225,9 +225,12 @@ class Engine:
         out["nk"] = len(blocks_per_k)
         return out
 
-    def solve_tdb(self, dblocks, E, p: N.NtParams, kidx=None, first_point_mask=None, extras=(), stream_sync=True):
+    def solve_tdb(self, dblocks, E, p: N.NtParams, kidx=None, first_point_mask=None, extras=(), stream_sync=True,
+                  noalloc=False):
         """Batched (omega,k)-solve on device-resident blocks (see upload_blocks).
         E, kidx: torch tensors (float64 / int32) on the GPU or array-likes.
+        noalloc: solve through nt_solve_tdb_batched_noalloc_dev (workspace in the
+        library's static scratch; the outputs are still allocated here).
         Returns dict(T_LR, T_RL, status, iters[, extras...]) of torch tensors."""
         import torch
         dev = torch.device("cuda", self.device)
@@ -246,7 +249,8 @@ class Engine:
         if self._torch_stream:
             self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
         ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
-        st = self._L.nt_solve_tdb_batched_dev(
+        entry = self._L.nt_solve_tdb_batched_noalloc_dev if noalloc else self._L.nt_solve_tdb_batched_dev
+        st = entry(
             self.ctx, C.byref(p), nl, nr, b, L, npts, ptr(E), ptr(kidx), nk,
             ptr(dblocks["Haml"]), ptr(dblocks["Hamlv"]), ptr(dblocks["Hamr"]), ptr(dblocks["Hamrv"]),
             ptr(dblocks["Hamd"]), ptr(dblocks["Hamdv"]), ptr(dblocks["Hamld"]), ptr(dblocks["Hamrd"]),

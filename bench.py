@@ -69,6 +69,8 @@ def main():
                     help="before timing, run one device-to-device copy of this many MiB (HBM-counter calibration "
                          "under rocprofv3 --pmc)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--serial-sweeps", action="store_true",
+                    help="run the two RGF sweeps on one stream in every step (for per-kernel profiles)")
     ap.add_argument("--c3-devices", type=int, default=4,
                     help="uneven-batching sample: Si/Ge superlattice devices (one per block size 24/96/216/384 "
                          "in turn), 64 frequencies x 2x2 k each, solved in one multi-device call (0 = skip)")
@@ -114,6 +116,8 @@ def main():
     E = torch.as_tensor(omega ** 2, dtype=torch.float64, device=dev)
 
     eng = T.Engine(0, batch_limit=args.points)
+    if args.serial_sweeps:
+        eng.set_overlap(False)
     dblocks = [eng.upload_blocks(b) for b in blocks]
     torch.cuda.synchronize()
     if args.copy_probe_mib:
@@ -293,6 +297,20 @@ def main():
         return el, [r["T_LR"].cpu().numpy() for r in outs], kern, float(np.mean(iters)), ok
 
     el64, T64, k64, nbar, ok64 = run("fp64")
+
+    def isolated_kernels(prec):
+        """one more step with the two sweeps on one stream: per-launch kernel
+        times that do not include time shared with the other sweep"""
+        p = T.params("phonon", precision=prec)
+        eng.set_overlap(False)
+        eng.profile_reset()
+        eng.profile(True)
+        eng.solve_tdb(dblocks[args.warmup], E, p)
+        eng.profile(False)
+        eng.set_overlap(not args.serial_sweeps)
+        return {"gemm": eng.profile_get(N.NT_K_GEMM), "inverse": eng.profile_get(N.NT_K_INV)}
+
+    iso64 = isolated_kernels("fp64")
     elmx, Tmx, kmx, _, okmx = run("mixed") if args.mixed else (el64, T64, k64, nbar, ok64)
 
     def run_adjoint():
@@ -489,9 +507,17 @@ def main():
     b = blocks[0].b
     flop_solve = b ** 3 * (112 * nbar + 40 * L + 104)
     dom_name, dom = max(((k, v) for k, v in kern.items() if k != "elementwise"), key=lambda kv: kv[1]["ms"])
-    avg_ms = dom["ms"] / max(1, dom["launches"])
+    avg_ms_conc = dom["ms"] / max(1, dom["launches"])
     avg_flops = dom["flops"] / max(1, dom["launches"])
-    achieved = avg_flops / (avg_ms * 1e-3) / 1e12
+    achieved_conc = avg_flops / (avg_ms_conc * 1e-3) / 1e12
+    # the roofline's per-launch time comes from the serialised step (same
+    # kernels and shapes, no second stream sharing the GPU)
+    iso = iso64[dom_name] if prec_used == "fp64" else None
+    if iso and iso["launches"]:
+        avg_ms = iso["ms"] / iso["launches"]
+        achieved = iso["flops"] / iso["launches"] / (avg_ms * 1e-3) / 1e12
+    else:
+        avg_ms, achieved = avg_ms_conc, achieved_conc
     peak = PEAK[prec_used] if dom_name == "gemm" else PEAK["fp64"]
     if prec_used == "mixed" and dom_name == "inverse":
         peak = PEAK["mixed"]
@@ -543,16 +569,20 @@ def main():
         "roofline": {"bound": "mfma", "kernel": dom_name, "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_per_launch(dom_name, prec_used, b, args.points),
-                     "launches": dom["launches"], "avg_launch_ms": avg_ms,
-                     "timing": "HIP events around every launch on its own stream; the two RGF sweeps run "
-                               "concurrently on two streams, so a launch's duration includes the time it "
-                               "shares the GPU with the other sweep's kernels"},
+                     "launches": (iso or dom)["launches"], "avg_launch_ms": avg_ms,
+                     "timing": "HIP events around every launch on the launching stream, in one extra step with "
+                               "the two RGF sweeps on one stream (kernels do not share the GPU)",
+                     "concurrent": {"avg_launch_ms": avg_ms_conc, "achieved": achieved_conc,
+                                    "what": "the same kernel in the timed steps, where the two sweeps run on two "
+                                            "streams and a launch shares the GPU with the other sweep's kernels"},
+                     "inverse_isolated": {"avg_launch_ms": iso64["inverse"]["ms"] / max(1, iso64["inverse"]["launches"]),
+                                          "tflops": iso64["inverse"]["flops"] / max(1e-9, iso64["inverse"]["ms"] * 1e-3) / 1e12}},
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,
                               "mixed_speedup": vmx / v64, "mixed_max_rel_dT": rel_err,
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",
                               "refinement": mixed_counts,
                               "status_ok": bool(ok64 and okmx)} if args.mixed else None,
-        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps_on_two_streams": True},
+        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps_on_two_streams": not args.serial_sweeps},
         "solve_model": {"mean_decimation_iters": nbar, "flop_per_solve": flop_solve,
                         "solve_tflops": flop_solve * value / 1e12},
         "cpu_baseline": cpu,

@@ -784,6 +784,8 @@ int inv_panel_widths(int n, int* out)
     const int lim32 = sizeof(T) == 8 ? 192 : 256, lim16 = sizeof(T) == 8 ? 384 : 512;
     int k = 0;
     if (n <= lim32) out[k++] = 32;
+    if (n <= 256) out[k++] = 24;
+    if (n <= 256) out[k++] = 20;
     if (n <= lim16) out[k++] = 16;
     if (n <= inv_max_n()) out[k++] = 8;
     if (n >= 64 && n <= inv_max_n()) out[k++] = INV_BLOCKED;
@@ -835,6 +837,8 @@ hipError_t launch_inv_any(const InvArgs<T>& a, hipStream_t s)
         nb = w[0] == INV_BLOCKED ? 8 : w[0];    // scratch does not fit in X: one-workgroup path
     }
     if (nb == 32) return launch_inv_nb<T, 32, 1>(a, s);
+    if (nb == 24) return launch_inv_nb<T, 24, 1>(a, s);
+    if (nb == 20) return launch_inv_nb<T, 20, 1>(a, s);
     if (nb == 16) return launch_inv_nb<T, 16, 2>(a, s);
     return launch_inv_nb<T, 8, 4>(a, s);
 }

@@ -48,7 +48,7 @@ int inv_max_n();
 // panel widths launch_inv accepts for an n x n matrix of type T (largest first);
 // INV_BLOCKED selects the panel-kernel + batched-GEMM path (last in the list)
 constexpr int INV_BLOCKED = 1;
-constexpr int INV_MAX_WIDTHS = 4;
+constexpr int INV_MAX_WIDTHS = 6;
 template <typename T> int inv_panel_widths(int n, int* out);
 
 // Batched complex Schur / eigen / square-root kernels (nt_eig.hip), FP64.

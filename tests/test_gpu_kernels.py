@@ -136,7 +136,7 @@ def test_zinv_every_panel_width(engine, torch_dev, n):
     batch = 2
     A = [crand(rng, n, n) + 0.5 * n ** 0.5 * np.eye(n) for _ in range(batch)]
     A[0][0, 0] = 0.0
-    widths = [w for w, lim in ((32, 192), (16, 384), (8, 1024)) if n <= lim]
+    widths = [w for w, lim in ((32, 192), (24, 256), (20, 256), (16, 384), (8, 1024)) if n <= lim]
     if n >= 64:
         widths.append(1)        # panel kernels + batched GEMM
     print(f"zinv panels n={n} widths={widths}", flush=True)

@@ -65,6 +65,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", dest="adjoint", action="store_false", help="skip the adjoint measurement")
     ap.add_argument("--no-mixed", dest="mixed", action="store_false", help="skip the mixed-precision run")
+    ap.add_argument("--adam-steps", type=int, default=50,
+                    help="Adam steps maximising G over the 2048 bonds on a sample of the workload (0 = skip)")
+    ap.add_argument("--adam-freqs", type=int, default=32, help="frequencies of the Adam sample (per k-point)")
+    ap.add_argument("--mass-points", type=int, default=16,
+                    help="frequencies of the interface-mass gradient at b = 768, L = 100 (0 = skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="before timing, run one device-to-device copy of this many MiB (HBM-counter calibration "
                          "under rocprofv3 --pmc)")
@@ -314,24 +319,89 @@ def main():
     elmx, Tmx, kmx, _, okmx = run("mixed") if args.mixed else (el64, T64, k64, nbar, ok64)
 
     def run_adjoint():
-        """forward + adjoint of the conductance weights over the interface window"""
+        """forward + adjoint of the conductance weights over the window of the
+        2048 trainable bonds (the bonds owned by the 16 layers around the
+        interface: device layers 13..28, 1-based, and their couplings), against
+        the full forward (both sweeps) and a single left-to-right sweep (the
+        adjoint entry with a one-layer window at the last layer)"""
         from phinfortran_amd.adjoint import conductance_weights
         p = T.params("phonon")
         w = torch.as_tensor(conductance_weights(np.sqrt(omega ** 2), 300.0, (4 * 2) * (lt.A_SI * 1e-10) ** 2),
                             dtype=torch.float64, device=dev)
-        lo, hi = L // 2 - 1, L // 2 + 1
-        for s in range(args.warmup):
-            eng.solve_tdb_adjoint(dblocks[s], E, p, w, lo, hi)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for s in range(args.steps):
-            r = eng.solve_tdb_adjoint(dblocks[args.warmup + s], E, p, w, lo, hi, stream_sync=False)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        log(f"adjoint: {args.steps * args.points / el:.1f} (omega,k)-gradients/s")
-        return el, bool((r["status"] == 0).all())
+        lo, hi = L // 2 - 8, L // 2 + 8
 
-    eladj, okadj = run_adjoint() if args.adjoint else (None, True)
+        def timed(f):
+            for s in range(args.warmup):
+                f(dblocks[s])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s in range(args.steps):
+                r = f(dblocks[args.warmup + s])
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, bool((r["status"] == 0).all())
+        el, ok = timed(lambda d: eng.solve_tdb_adjoint(d, E, p, w, lo, hi, stream_sync=False))
+        el1, _ = timed(lambda d: eng.solve_tdb_adjoint(d, E, p, w, L - 1, L - 1, stream_sync=False))
+        log(f"adjoint (window {lo}..{hi}): {args.steps * args.points / el:.1f} (omega,k)-gradients/s; "
+            f"single-sweep forward {args.steps * args.points / el1:.1f} solves/s")
+        res = {"grads_per_s": args.steps * args.points / el, "unit": "(omega,k)-gradients/s",
+               "what": f"T_LR and dW/dH of device layers {lo}..{hi} (0-based), the blocks the 2048 trainable bonds "
+                       f"of the 16 interfacial layers enter; W = conductance-weighted sum of T_LR, FP64",
+               "window_layers": [lo, hi], "bytes_per_point": eng.adjoint_bytes_per_point(dblocks[0], lo, hi),
+               "single_sweep_forward_solves_per_s": args.steps * args.points / el1,
+               "cost_vs_single_sweep_forward": el / el1, "status_ok": ok}
+        if args.adam_steps > 0:
+            from phinfortran_amd.adjoint import InterfaceFilm, maximize_conductance
+            ks = kgrid(8)
+            f = 16.0 * np.arange(1, args.points + 1) / args.points
+            f = f[:: max(1, args.points // args.adam_freqs)]
+            t0 = time.perf_counter()
+            film = InterfaceFilm(eng, 4, 2, L, [ks[9], ks[36]], f, n_iface=16, params=p)
+            tb = time.perf_counter() - t0
+            rng = np.random.default_rng(0)
+            beta = torch.as_tensor(13.8 + 0.5 * rng.standard_normal(film.nparam), device=film.dev)
+            t0 = time.perf_counter()
+            _, hist = maximize_conductance(film, beta, steps=args.adam_steps, lr=0.2)
+            ta = time.perf_counter() - t0
+            log(f"adam: {args.adam_steps} steps, G {hist[0]:.4e} -> {hist[-1]:.4e} in {ta:.1f}s")
+            res["adam"] = {"steps": args.adam_steps, "parameters": film.nparam, "G_first": hist[0],
+                           "G_last": hist[-1], "seconds": ta, "setup_seconds": tb,
+                           "sample": f"2 k-points of the 8x8 grid x {len(f)} of the {args.points} frequencies",
+                           "unit": "W m^-2 K^-1"}
+        return el, ok, res
+
+    def run_mass_adjoint():
+        """gradient of G(300 K) with respect to the 768 interface masses of the
+        Si film with a rough Si/Ge interface (8x4 cells, b = 768, L = 100),
+        with checkpointed recomputation of the layer Green's functions"""
+        from phinfortran_amd.adjoint import MassFilm
+        f = 16.0 * np.arange(1, args.mass_points + 1) / args.mass_points
+        t0 = time.perf_counter()
+        mf = MassFilm(eng, 8, 4, 100, [kgrid(8)[27]], f, rough_layers=3, seed=0, params=T.params("phonon"))
+        tb = time.perf_counter() - t0
+        m = mf.initial()
+        out = {}
+        for every in (0, -1):
+            eng.set_adjoint_checkpoint(every)
+            nbytes = eng.adjoint_bytes_per_point(mf.dblocks, mf.w_lo, mf.w_hi)
+            mf.conductance(m)                      # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            G, g, _ = mf.conductance(m)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            out["checkpointed" if every else "full_storage"] = {
+                "grads_per_s": len(f) / el, "adjoint_bytes_per_point": nbytes, "G": float(G),
+                "max_abs_dG_dm": float(g.abs().max())}
+            log(f"mass adjoint b=768 L=100 ({'checkpointed' if every else 'full storage'}): "
+                f"{len(f) / el:.1f} gradients/s, {nbytes / 2 ** 20:.0f} MiB per point")
+        eng.set_adjoint_checkpoint(0)
+        out.update(unit="(omega,k)-gradients/s", points=len(f), setup_seconds=tb,
+                   what="Si film 8x4 cells (b = 768), L = 100, 3-layer rough Si/Ge interface: T_LR and "
+                        "dG/dm of the 768 interface masses, one k-point, FP64")
+        return out
+
+    eladj, okadj, adjres = run_adjoint() if args.adjoint else (None, True, None)
+    massadj = run_mass_adjoint() if args.mass_points > 0 else None
 
     def run_c3():
         """uneven devices in one call, FP64 and mixed; per-device mixed error"""
@@ -599,10 +669,10 @@ def main():
     if tbrun is not None:
         out["tb_transport_run"] = tbrun
     if eladj is not None:
-        out["adjoint"] = {"grads_per_s": solves / eladj, "unit": "(omega,k)-gradients/s",
-                          "what": "T_LR and dW/dH of the 3 device layers around the interface (bonds owned by "
-                                  "the 2 interfacial layers), W = conductance-weighted sum of T_LR, FP64",
-                          "cost_vs_fp64_forward": eladj / el64, "status_ok": okadj}
+        adjres["cost_vs_fp64_forward"] = eladj / el64
+        out["adjoint"] = adjres
+    if massadj is not None:
+        out["mass_adjoint"] = massadj
     print(json.dumps(out))
     eng.close()
 

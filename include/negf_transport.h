@@ -532,6 +532,19 @@ int nt_set_inverse_panel(nt_ctx* ctx, int n, int precision, int nb);
 /* on = 1 (default): the right-to-left RGF sweep runs on a second stream of
  * the context beside the left-to-right one; on = 0: both on the main stream. */
 int nt_set_overlap(nt_ctx* ctx, int on);
+/* Storage of nt_solve_tdb_adjoint_batched_dev's backward pass: the left-
+ * connected blocks G^L_i, G^L_{0,i} of the layers from the window to the end
+ * are kept for every layer (every = 0, default), for every n-th layer
+ * (every = n > 0) or for about every sqrt(L - lo)-th layer (every < 0); the
+ * others are recomputed from the nearest kept layer during the backward pass
+ * by the same launches as the forward sweep (same values, one more
+ * left-to-right pass over the recomputed layers).  No reference counterpart
+ * (the reference has no derivatives). */
+int nt_set_adjoint_checkpoint(nt_ctx* ctx, int every);
+/* Device bytes per (omega,k) point that the adjoint solve's stored blocks take
+ * with the current checkpoint setting (the chunk of points is capped so that
+ * they stay within 24 GiB). */
+int nt_adjoint_bytes_per_point(nt_ctx* ctx, int nl, int nr, int b, int L, int lo, int hi, long long* bytes);
 /* on = 1: FP64 complex GEMMs form each complex product from three real MFMA
  * products ((Ar+Ai)(Br+Bi) - ArBr - AiBi for the imaginary part) instead of
  * four; on = 0 (default): four. */

@@ -260,3 +260,125 @@ def maximize_conductance(film: InterfaceFilm, beta, steps: int = 50, lr: float =
         if callback:
             callback(it, float(G))
     return x.detach(), hist
+
+
+class MassFilm:
+    """Si thin film with a rough Si/Ge interface (lattice.rough_interface_film)
+    whose interface masses are the trainable parameters.
+
+    The dynamical matrix scales with the masses as D_xy = Phi_xy / sqrt(m_x m_y),
+    so the window blocks at masses m follow from the blocks at the initial
+    masses m0 by the diagonal rescaling D(m) = S D(m0) S, S = diag(sqrt(m0 / m))
+    per degree of freedom, and
+
+        dG/dm_a = -1/(2 m_a) sum over the window blocks of the entries in the
+                  rows or columns of atom a of Re(conj(dG/dD) D),
+
+    (twice for an entry in both, the on-site block of a), with dG/dD from the
+    adjoint solve over device layers lo - 1 .. hi: the on-site blocks of the
+    interface layers lo..hi, the couplings into them (lo - 1 -> lo) and out of
+    them.  Masses of the layers lo..hi are trainable (rough_layers x natom).
+    """
+
+    def __init__(self, engine, nx: int, ny: int, L: int, kpts: Sequence, f_thz, temperature: float = 300.0,
+                 rough_layers: int = 3, seed: int = 0, params=None):
+        import torch
+        from . import transport as T
+        self.engine = engine
+        self.dev = torch.device("cuda", engine.device)
+        self.layer = lt._layer_cache(nx, ny)
+        self.L = L
+        self.kpts = [tuple(k) for k in kpts]
+        self.nk = len(self.kpts)
+        self.masses0 = lt.rough_interface_masses(self.layer.natom, L, rough_layers, seed)
+        self.lo = L // 2 - rough_layers // 2             # 0-based first interface layer
+        self.hi = self.lo + rough_layers - 1
+        if self.lo < 1 or self.hi > L - 2:
+            raise ValueError("the interface layers must not touch the leads")
+        self.nparam = rough_layers * self.layer.natom
+        self.p = params if params is not None else T.params("phonon")
+        self.f = np.asarray(f_thz, float)
+        self.nf = len(self.f)
+        a = lt.A_SI * 1e-10
+        self.area = nx * a * ny * a
+        w = conductance_weights(self.f, temperature, self.area, self.nk)
+        self.E = torch.as_tensor(np.tile(self.f ** 2, self.nk), dtype=torch.float64, device=self.dev)
+        self.kidx = torch.as_tensor(np.repeat(np.arange(self.nk), self.nf), dtype=torch.int32, device=self.dev)
+        mask = np.zeros(self.nf, np.int32)
+        mask[0] = 1                                  # the reference's ie = 1 point (lesser.f90:802-807)
+        self.mask = torch.as_tensor(np.tile(mask, self.nk), device=self.dev)
+        self.weight = torch.as_tensor(np.tile(w, self.nk), dtype=torch.float64, device=self.dev)
+        blocks = [lt.rough_interface_film(nx, ny, L, rough_layers, kf, masses=self.masses0) for kf in self.kpts]
+        self.b = blocks[0].b
+        self.dblocks = engine.upload_blocks(blocks)
+        self.w_lo, self.w_hi = self.lo - 1, self.hi        # engine window (0-based layers)
+        self.nw = self.w_hi - self.w_lo + 1
+        self.nc = min(self.w_hi, L - 2) - self.w_lo + 1
+        # row-major window blocks at the initial masses
+        self.Hd0 = self.dblocks["Hamd"][:, self.w_lo:self.w_hi + 1].transpose(-1, -2).clone()
+        self.Hdv0 = self.dblocks["Hamdv"][:, self.w_lo:self.w_lo + self.nc].transpose(-1, -2).clone()
+        self.m0 = torch.as_tensor(self.masses0, dtype=torch.float64, device=self.dev)
+
+    def initial(self):
+        """the trainable masses at their seeded values (rough layers, flat)"""
+        return self.m0[self.lo:self.hi + 1].reshape(-1).clone()
+
+    def _layer_masses(self, m):
+        full = self.m0.clone()
+        full[self.lo:self.hi + 1] = m.reshape(self.hi - self.lo + 1, -1)
+        return full
+
+    def window_blocks(self, m):
+        """row-major window blocks at trainable masses m"""
+        full = self._layer_masses(m)
+        r = torch_sqrt_ratio(self.m0, full)                           # (L, natom) sqrt(m0 / m)
+        rd = r.repeat_interleave(3, dim=1)                            # per dof
+        ls = slice(self.w_lo, self.w_hi + 1)
+        hd = self.Hd0 * rd[ls][None, :, :, None] * rd[ls][None, :, None, :]
+        lc = slice(self.w_lo, self.w_lo + self.nc)
+        lc1 = slice(self.w_lo + 1, self.w_lo + 1 + self.nc)
+        hv = self.Hdv0 * rd[lc][None, :, :, None] * rd[lc1][None, :, None, :]
+        return hd, hv
+
+    def write_window(self, hd, hv):
+        self.dblocks["Hamd"][:, self.w_lo:self.w_hi + 1] = hd.transpose(-1, -2)
+        if self.nc > 0:
+            self.dblocks["Hamdv"][:, self.w_lo:self.w_lo + self.nc] = hv.transpose(-1, -2)
+
+    def mass_grad(self, m, hd, hv, gd, gv):
+        """dG/dm of the trainable masses from dG/dD of the window blocks"""
+        import torch
+        na = self.layer.natom
+        full = self._layer_masses(m)
+        acc = torch.zeros_like(full)                                   # (L, natom)
+        Ad = (torch.conj(gd) * hd).real.sum(0)                         # (nw, b, b), summed over k
+        rows = Ad.reshape(self.nw, na, 3, -1).sum(dim=(2, 3))
+        cols = Ad.reshape(self.nw, -1, na, 3).sum(dim=(1, 3))
+        acc[self.w_lo:self.w_hi + 1] += rows + cols
+        if self.nc > 0:
+            Av = (torch.conj(gv) * hv).real.sum(0)                     # (nc, b, b)
+            acc[self.w_lo:self.w_lo + self.nc] += Av.reshape(self.nc, na, 3, -1).sum(dim=(2, 3))
+            acc[self.w_lo + 1:self.w_lo + 1 + self.nc] += Av.reshape(self.nc, -1, na, 3).sum(dim=(1, 3))
+        g = -acc / (2.0 * full)
+        return g[self.lo:self.hi + 1].reshape(-1)
+
+    def conductance(self, m, with_grad: bool = True):
+        """(G, dG/dm or None, per-point T_LR) at trainable masses m (torch float64)"""
+        hd, hv = self.window_blocks(m.detach())
+        self.write_window(hd, hv)
+        if with_grad:
+            r = self.engine.solve_tdb_adjoint(self.dblocks, self.E, self.p, self.weight, self.w_lo, self.w_hi,
+                                              kidx=self.kidx, first_point_mask=self.mask)
+            g = self.mass_grad(m.detach(), hd, hv, r["dHamd"], r["dHamdv"])
+        else:
+            r = self.engine.solve_tdb(self.dblocks, self.E, self.p, kidx=self.kidx, first_point_mask=self.mask)
+            g = None
+        if not bool((r["status"] == 0).all()):
+            from ._native import TransportError
+            raise TransportError(int(r["status"][r["status"] != 0][0]), "mass film: points failed")
+        return (self.weight * r["T_LR"]).sum(), g, r["T_LR"]
+
+
+def torch_sqrt_ratio(m0, m):
+    import torch
+    return torch.sqrt(m0 / m)

@@ -481,9 +481,14 @@ size_t lead_gfs_scratch(const nt_params* prm, int n, int Q, int nm)
 // bmax^2 complex entries per slot (slot-major, then point).
 struct AdjSpec {
     int lo, hi;
+    int K;               // checkpoint interval: G^L of layers lo + n K kept, the rest recomputed
     const double* w;     // per-point weight of T_LR in the objective
-    double* SL;          // left-connected G^L_i, i = lo..L-1           (L - lo slots)
-    double* S0;          // left-connected corner G^L_{0,i}, i = lo..L-1 (L - lo slots)
+    double* SL;          // left-connected G^L_i, i = lo, lo + K, ...   (ceil((L - lo) / K) slots)
+    double* S0;          // left-connected corner G^L_{0,i}, same layers
+    double* SegL;        // K > 1: one recomputed segment of G^L_i        (K - 1 slots)
+    double* Seg0;        //        and of G^L_{0,i}                       (K - 1 slots)
+    double* Wk;          // K > 1: inverse work slot of the recomputation
+    int* st;             // K > 1: statuses of the recomputed inverses (P)
     double* CW;          // column G_{j,N+1}, j = lo..hi1              (hi1 - lo + 1 slots)
     double* RW;          // row G_{0,j}, j = lo..hi1                   (hi1 - lo + 1 slots)
     double* CA;          // G_{j,N+1} A                                 (hi1 - lo + 1 slots)
@@ -497,23 +502,41 @@ struct AdjSpec {
 
 inline int adj_hi1(int hi, int L) { return std::min(hi + 1, L - 1); }
 inline int adj_hc(int hi, int L) { return std::min(hi, L - 2); }
-inline int adj_slots(int lo, int hi, int L)
+// checkpoint interval for a setting (0: every layer kept, < 0: about sqrt(L - lo))
+inline int adj_interval(int setting, int lo, int L)
+{
+    if (setting == 0) return 1;
+    if (setting > 0) return std::min(setting, std::max(1, L - lo));
+    return std::max(1, (int)std::ceil(std::sqrt((double)(L - lo))));
+}
+inline int adj_nchk(int lo, int L, int K) { return (L - lo + K - 1) / K; }
+inline int adj_slots(int lo, int hi, int L, int K)
 {
     const int nw1 = adj_hi1(hi, L) - lo + 1;
-    return 2 * (L - lo) + 3 * nw1 + (hi - lo + 1) + std::max(0, adj_hc(hi, L) - lo + 1) + 6;
+    const int keep = 2 * adj_nchk(lo, L, K) + (K > 1 ? 2 * (K - 1) + 1 : 0);
+    return keep + 3 * nw1 + (hi - lo + 1) + std::max(0, adj_hc(hi, L) - lo + 1) + 6;
 }
 
 // carve the adjoint buffers (P points, slot size mm complex) from an arena
-inline AdjSpec adj_carve(Arena& a, int lo, int hi, int L, long mm, int P, const double* w)
+inline AdjSpec adj_carve(Arena& a, int lo, int hi, int L, int K, long mm, int P, const double* w)
 {
     const size_t sb = sizeof(double) * 2 * (size_t)mm * P;
     const int nw1 = adj_hi1(hi, L) - lo + 1, nc = std::max(0, adj_hc(hi, L) - lo + 1);
     AdjSpec s;
     s.lo = lo;
     s.hi = hi;
+    s.K = K;
     s.w = w;
-    s.SL = a.ptr<double>(a.take(sb * (L - lo)));
-    s.S0 = a.ptr<double>(a.take(sb * (L - lo)));
+    s.SL = a.ptr<double>(a.take(sb * adj_nchk(lo, L, K)));
+    s.S0 = a.ptr<double>(a.take(sb * adj_nchk(lo, L, K)));
+    s.SegL = s.Seg0 = s.Wk = nullptr;
+    s.st = nullptr;
+    if (K > 1) {
+        s.SegL = a.ptr<double>(a.take(sb * (K - 1)));
+        s.Seg0 = a.ptr<double>(a.take(sb * (K - 1)));
+        s.Wk = a.ptr<double>(a.take(sb));
+        s.st = a.ptr<int>(a.take(sizeof(int) * (size_t)P));
+    }
     s.CW = a.ptr<double>(a.take(sb * nw1));
     s.RW = a.ptr<double>(a.take(sb * nw1));
     s.CA = a.ptr<double>(a.take(sb * nw1));
@@ -606,14 +629,56 @@ size_t rgf_scratch(int nl, int nr, int bmax, int P)
 // 2 X_{l,l}^+ for Hamd_l and 2 (X_{l+1,l}^+ + X_{l,l+1}) for Hamdv_l, whose
 // dagger is the lower block.  The weight w multiplies A.
 template <typename T>
-void adjoint_backward(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, int P, long mm, BMatW<T> gdlr, BMatW<T> gRo,
-                      const BMat<T>& Vc, int opVc, int opVcH, const AdjSpec& adj, T* T1, T* T2)
+void adjoint_backward(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int P, long mm,
+                      BMatW<T> gdlr, BMatW<T> gRo, const BMat<T>& Vc, int opVc, int opVcH, const AdjSpec& adj,
+                      T* T1, T* T2)
 {
     const int L = in.L;
     const long slot = mm * P;
     auto W = [&](T* p, int ld) { return bw(p, mm, ld); };
     auto R = [&](T* p, int ld) { return bm<T>(p, mm, ld); };
     auto at = [&](double* base, int k) { return reinterpret_cast<T*>(base) + 2 * (long)k * slot; };
+    // G^L_j and G^L_{0,j} of a layer: a checkpoint, or its segment recomputed
+    // from the checkpoint by the left-to-right recursion (the same launches
+    // as the forward sweep, so the same values)
+    const int K = adj.K;
+    int seg_have = -1;
+    auto seg_ensure = [&](int j) {
+        const int sg = (j - adj.lo) / K;
+        if (K == 1 || sg == seg_have) return;
+        const int c0 = adj.lo + sg * K, c1 = std::min(c0 + K - 1, L - 1);
+        T* Gp = at(adj.SL, sg);
+        T* G0p = at(adj.S0, sg);
+        T* Aw = reinterpret_cast<T*>(adj.Wk);
+        for (int i = c0 + 1; i <= c1; ++i) {
+            const int bp = in.bsz[i - 1], bi = in.bsz[i];
+            const BMat<T>& V = in.Hdv[i - 1];
+            T* Gi = at(adj.SegL, i - c0 - 1);
+            T* G0i = at(adj.Seg0, i - c0 - 1);
+            const T* zi = in.zl ? in.zl + 2 * (long)i * P : zd;
+            gemm<T>(c, bi, bp, bp, OP_C, V, OP_N, R(Gp, bp), W(T1, bi), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, bi, bi, bp, OP_N, R(T1, bi), OP_N, V, W(Aw, bi), -1.0, in.Hd[i], -1.0, zi, P, -1, in.act);
+            inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gi, bi), adj.st, P, -1, in.act);
+            gemm<T>(c, nl, bi, bp, OP_N, R(G0p, nl), OP_N, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1,
+                    in.act);
+            gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gi, bi), W(G0i, nl), 1.0, none<T>(), 0.0, nullptr, P,
+                    -1, in.act);
+            Gp = Gi;
+            G0p = G0i;
+        }
+        seg_have = sg;
+    };
+    auto GLj = [&](int j) -> T* {
+        seg_ensure(j);
+        const int sg = (j - adj.lo) / K, off = (j - adj.lo) % K;
+        return off == 0 ? at(adj.SL, sg) : at(adj.SegL, off - 1);
+    };
+    auto G0j = [&](int j) -> T* {
+        seg_ensure(j);
+        const int sg = (j - adj.lo) / K, off = (j - adj.lo) % K;
+        return off == 0 ? at(adj.S0, sg) : at(adj.Seg0, off - 1);
+    };
+    if (K > 1) hipcheck(c, hipMemsetAsync(adj.st, 0, sizeof(int) * P, c->stream), "memset");
     const int lo = adj.lo, hi = adj.hi, hi1 = adj_hi1(hi, L), hc = adj_hc(hi, L);
     T* A = reinterpret_cast<T*>(adj.A);
     hipStream_t s = c->stream;
@@ -637,14 +702,15 @@ void adjoint_backward(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, int P, l
         T* Cn = j <= hi1 ? at(adj.CW, j - lo) : reinterpret_cast<T*>(adj.Cb[pp]);
         T* Rn = j <= hi1 ? at(adj.RW, j - lo) : reinterpret_cast<T*>(adj.Rb[pp]);
         pp ^= 1;
+        // (the recomputation of a segment uses T1 / T2: it runs first)
+        T* glj = GLj(j);
+        T* g0j = G0j(j);
         // G_{j,N+1} = G^L_j (V_{j,j+1} G_{j+1,N+1})
         gemm<T>(c, bj, nr, bn, opV, V, OP_N, Ccur, W(T1, bj), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, bj, nr, bj, OP_N, R(at(adj.SL, j - lo), bj), OP_N, R(T1, bj), W(Cn, bj), 1.0, none<T>(), 0.0,
-                nullptr, P);
+        gemm<T>(c, bj, nr, bj, OP_N, R(glj, bj), OP_N, R(T1, bj), W(Cn, bj), 1.0, none<T>(), 0.0, nullptr, P);
         // G_{0,j} = G^L_{0,j} + (G_{0,j+1} V_{j+1,j}) G^L_j
         gemm<T>(c, nl, bj, bn, OP_N, Rcur, opVH, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P);
-        gemm<T>(c, nl, bj, bj, OP_N, R(T2, nl), OP_N, R(at(adj.SL, j - lo), bj), W(Rn, nl), 1.0,
-                R(at(adj.S0, j - lo), nl), 1.0, nullptr, P);
+        gemm<T>(c, nl, bj, bj, OP_N, R(T2, nl), OP_N, R(glj, bj), W(Rn, nl), 1.0, R(g0j, nl), 1.0, nullptr, P);
         Ccur = R(Cn, bj);
         Rcur = R(Rn, nl);
     }
@@ -722,11 +788,13 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
 
     // adjoint storage slots of device layer i (null when layer i is not stored)
     const long slot = mm * P;
+    // adjoint storage: the checkpoint layers lo + n K (every layer when K = 1)
+    auto kept = [&](int i) { return adj && i >= adj->lo && (i - adj->lo) % adj->K == 0; };
     auto slotL = [&](int i) -> T* {
-        return adj && i >= adj->lo ? reinterpret_cast<T*>(adj->SL) + 2 * (long)(i - adj->lo) * slot : nullptr;
+        return kept(i) ? reinterpret_cast<T*>(adj->SL) + 2 * (long)((i - adj->lo) / adj->K) * slot : nullptr;
     };
     auto slot0 = [&](int i) -> T* {
-        return adj && i >= adj->lo ? reinterpret_cast<T*>(adj->S0) + 2 * (long)(i - adj->lo) * slot : nullptr;
+        return kept(i) ? reinterpret_cast<T*>(adj->S0) + 2 * (long)((i - adj->lo) / adj->K) * slot : nullptr;
     };
     T* const gbufs[2] = {G, Gn};
     T* const g0bufs[2] = {G0, G0n};
@@ -782,7 +850,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         gemm<T>(c, nl, nr, nr, OP_N, ro(gdlr), OP_N, in.gmr, W(X, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         gemm<T>(c, nr, nl, nl, OP_C, ro(gdlr), OP_N, in.gml, W(Y, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P, in.act);
-        if (adj) adjoint_backward<T>(c, nl, nr, in, P, mm, gdlr, gRo, Vc, opVc, opVcH, *adj, T1, T2);
+        if (adj) adjoint_backward<T>(c, nl, nr, in, zd, P, mm, gdlr, gRo, Vc, opVc, opVcH, *adj, T1, T2);
     }
     // ---------------- right to left, on the auxiliary stream when enabled: it
     // needs only the inputs, so it runs beside the left-to-right sweep and
@@ -1073,13 +1141,14 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const long nmm = (long)nmax * nmax;
     const long mmr = (long)std::max(nmax, b) * std::max(nmax, b);   // RGF slot size
     int Pmax = std::min(a.npts, c->batch_limit);
+    const int adjK = adjoint ? adj_interval(c->adj_ckpt, a.adj_lo, L) : 1;
     if (prm->precision == NT_PREC_MIXED && !adjoint && !a.probe) {
         // the mixed solve keeps both directions' FP32 factors and FP64 columns
         const double per_pt = (double)mixed_caroli_scratch(nl, nr, b, L, 1);
         Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, MIXED_BUDGET_BYTES / per_pt)));
     }
     if (adjoint) {
-        const double per_pt = (double)adj_slots(a.adj_lo, a.adj_hi, L) * mmr * sizeof(nt_c128);
+        const double per_pt = (double)adj_slots(a.adj_lo, a.adj_hi, L, adjK) * mmr * sizeof(nt_c128);
         Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
     }
     const ProbeSpec* probe = a.probe;
@@ -1148,7 +1217,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         need_adj = 0;
         if (adjoint) {
             Arena probe;
-            (void)adj_carve(probe, a.adj_lo, a.adj_hi, L, mmr, Pmax, nullptr);
+            (void)adj_carve(probe, a.adj_lo, a.adj_hi, L, adjK, mmr, Pmax, nullptr);
             need_adj = probe.off;
         }
         // probe region: per-layer shifts and broadenings, traces, kept blocks and
@@ -1342,7 +1411,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             if (adjoint) {
                 Arena aa = ar;
                 aa.off = o_adj;
-                adj = adj_carve(aa, a.adj_lo, a.adj_hi, L, mmr, P, a.adj_w + p0);
+                adj = adj_carve(aa, a.adj_lo, a.adj_hi, L, adjK, mmr, P, a.adj_w + p0);
             }
             rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr, adjoint ? &adj : nullptr);
             if (probe) {
@@ -2992,6 +3061,21 @@ int nt_set_gemm_3m(nt_ctx* c, int on)
 {
     if (!c || (on != 0 && on != 1)) return NT_ERR_ARG;
     c->gemm_m3 = on == 1;
+    return NT_OK;
+}
+
+int nt_set_adjoint_checkpoint(nt_ctx* c, int every)
+{
+    if (!c) return NT_ERR_ARG;
+    c->adj_ckpt = every;
+    return NT_OK;
+}
+
+int nt_adjoint_bytes_per_point(nt_ctx* c, int nl, int nr, int b, int L, int lo, int hi, long long* bytes)
+{
+    if (!c || !bytes || nl <= 0 || nr <= 0 || b <= 0 || L <= 0 || lo < 0 || hi < lo || hi >= L) return NT_ERR_ARG;
+    const long mm = (long)std::max(std::max(nl, nr), b) * std::max(std::max(nl, nr), b);
+    *bytes = (long long)adj_slots(lo, hi, L, adj_interval(c->adj_ckpt, lo, L)) * mm * (long long)sizeof(nt_c128);
     return NT_OK;
 }
 

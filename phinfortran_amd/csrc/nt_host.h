@@ -50,6 +50,9 @@ struct nt_ctx {
     // mixed precision since the last profile reset: points, second refinement
     // steps (both directions), points recomputed in FP64
     long long mixed_counts[3] = {0, 0, 0};
+    // adjoint: keep G^L of every layer (0), every n-th (n > 0) or about
+    // sqrt(L - lo)-th (< 0) from the window to the end; the rest recomputed
+    int adj_ckpt = 0;
     // device address ranges a launch may touch: the workspace and the caller's
     // buffers of the current call (sizes derived from the call's arguments)
     std::vector<std::pair<uintptr_t, uintptr_t>> regions;

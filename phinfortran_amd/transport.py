@@ -551,6 +551,17 @@ class Engine:
         """FP64 GEMMs with three real products per complex product."""
         self._check(self._L.nt_set_gemm_3m(self.ctx, int(bool(on))), "nt_set_gemm_3m")
 
+    def set_adjoint_checkpoint(self, every: int):
+        """adjoint storage: every layer (0), every n-th (n), about sqrt (-1)"""
+        self._check(self._L.nt_set_adjoint_checkpoint(self.ctx, int(every)), "nt_set_adjoint_checkpoint")
+
+    def adjoint_bytes_per_point(self, dblocks, lo: int, hi: int) -> int:
+        out = C.c_longlong()
+        self._check(self._L.nt_adjoint_bytes_per_point(self.ctx, dblocks["nl"], dblocks["nr"], dblocks["b"],
+                                                       dblocks["L"], int(lo), int(hi), C.byref(out)),
+                    "nt_adjoint_bytes_per_point")
+        return int(out.value)
+
     def set_overlap(self, on: bool):
         """Run the two RGF sweeps on two streams (default) or serially."""
         self._check(self._L.nt_set_overlap(self.ctx, int(bool(on))), "nt_set_overlap")

@@ -532,6 +532,15 @@ int nt_set_inverse_panel(nt_ctx* ctx, int n, int precision, int nb);
 /* on = 1 (default): the right-to-left RGF sweep runs on a second stream of
  * the context beside the left-to-right one; on = 0: both on the main stream. */
 int nt_set_overlap(nt_ctx* ctx, int on);
+/* Batched solves: n = 1 (default) computes T_RL, G_D_RL and G_L from the
+ * left-to-right sweep and a correction of its corner blocks for the two end
+ * blocks where the right-to-left system differs (the broadening of the lead
+ * surface layers), one (nl+nr) inverse per point instead of the second
+ * sweep; n = 2 runs the reference's second sweep (negf/lesser.f90:694-793).
+ * The per-point entries (nt_get_G_full_tdb, nt_G_block), the Buttiker-probe
+ * solve and leads given with Hrd_fwd / dense lower couplings always run both
+ * sweeps. */
+int nt_set_sweeps(nt_ctx* ctx, int n);
 /* Storage of nt_solve_tdb_adjoint_batched_dev's backward pass: the left-
  * connected blocks G^L_i, G^L_{0,i} of the layers from the window to the end
  * are kept for every layer (every = 0, default), for every n-th layer

@@ -12,6 +12,7 @@
 // Shared operands (H(k) blocks) are addressed with stride 0 or a k-index map,
 // so one H(k) copy serves every energy of that k-point.
 #include <mutex>
+#include <type_traits>
 
 #include "nt_host.h"
 
@@ -97,6 +98,41 @@ __global__ void k_lead_merge(const int* st_s, const int* it_s, int P, int* st2, 
     for (int side = 0; side < 2; ++side) {
         st2[2 * p + side] = st_s[side * P + p];
         it2[2 * p + side] = it_s[side * P + p];
+    }
+}
+
+// Corner system of the one-sweep correction, per point: with the corner
+// blocks G_00 (nl x nl), G_0c (nl x nr), G_c0 (nr x nl), G_cc (nr x nr) of the
+// left-to-right system's inverse (c = the closing layer N+1) and
+// d = z_dev - z_lead,
+//   M  = I + [G_00 G_0c; G_c0 G_cc] diag(d I_nl, -d I_nr)      (nl+nr square)
+//   C0 = [G_00; G_c0]                                          (nl+nr x nl)
+__global__ void k_corner_system(int nl, int nr, BMat<double> G00, BMat<double> G0c, BMat<double> Gc0,
+                                BMat<double> Gcc, const double* zd, const double* zl, int zls, BMatW<double> M,
+                                BMatW<double> C0, const int* active)
+{
+    const int p = blockIdx.y;
+    if (active && !active[p]) return;
+    const int n = nl + nr;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);
+    const double* g;
+    if (i < nl && j < nl) g = bptr(G00, p, 0) + 2 * (i + (long)j * G00.ld);
+    else if (i < nl) g = bptr(G0c, p, 0) + 2 * (i + (long)(j - nl) * G0c.ld);
+    else if (j < nl) g = bptr(Gc0, p, 0) + 2 * ((i - nl) + (long)j * Gc0.ld);
+    else g = bptr(Gcc, p, 0) + 2 * ((i - nl) + (long)(j - nl) * Gcc.ld);
+    const double gr = g[0], gi = g[1];
+    // d = i (Im z_dev - Im z_lead) (same real part E)
+    const double dr = zd[2 * p] - zl[(long)zls * p], di = zd[2 * p + 1] - zl[(long)zls * p + 1];
+    const double sr = j < nl ? dr : -dr, si = j < nl ? di : -di;
+    double* m = bptr(M, p, 0) + 2 * (i + (long)j * M.ld);
+    m[0] = (i == j ? 1.0 : 0.0) + gr * sr - gi * si;
+    m[1] = gr * si + gi * sr;
+    if (j < nl) {
+        double* c = bptr(C0, p, 0) + 2 * (i + (long)j * C0.ld);
+        c[0] = gr;
+        c[1] = gi;
     }
 }
 
@@ -611,6 +647,9 @@ size_t rgf_scratch(int nl, int nr, int bmax, int P)
     // the right-to-left sweep's own buffers and status (concurrent sweeps)
     for (int i = 0; i < 10; ++i) a.take(mb * P);
     a.take(sizeof(int) * P);
+    // corner system of the one-sweep correction: M, inv(M) ((nl+nr)^2 <= 4
+    // slots each), C0 and the corrected column ((nl+nr) nl <= 2 slots each)
+    a.take(12 * mb * P);
     return a.off;
 }
 
@@ -768,6 +807,10 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     T* X_b = sa.ptr<T>(sa.take(mb * P));
     T* Y_b = sa.ptr<T>(sa.take(mb * P));
     int* st_b = sa.ptr<int>(sa.take(sizeof(int) * P));
+    T* CM = sa.ptr<T>(sa.take(4 * mb * P));      // corner system M
+    T* CMi = sa.ptr<T>(sa.take(4 * mb * P));     // inv(M)
+    T* CC0 = sa.ptr<T>(sa.take(2 * mb * P));     // [G_00; G_c0]
+    T* CGp = sa.ptr<T>(sa.take(2 * mb * P));     // inv(M) [G_00; G_c0]
     (void)GRL;
     hipStream_t s = c->stream;
     const int L = in.L;
@@ -796,6 +839,14 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     auto slot0 = [&](int i) -> T* {
         return kept(i) ? reinterpret_cast<T*>(adj->S0) + 2 * (long)((i - adj->lo) / adj->K) * slot : nullptr;
     };
+    // one sweep: the right-to-left quantities follow from the left-to-right
+    // system's corner blocks (see one_sweep below); its buffers are free
+    const bool one = std::is_same<T, double>::value && c->one_sweep && !in.keep && !in.zl &&
+                     in.zlead && !in.Hrd_fwd.base && in.Hdv_back.empty() && nl + nr <= inv_max_n();
+    T* Gc = G_b;         // left-connected column G^L_{i,0}   (b x nl), ping-pong with Gcn
+    T* Gcn = Gn_b;
+    T* G00 = Aw_b;       // left-connected corner G^L_{00,i}  (nl x nl)
+    T* T3 = T1_b;        // V^+ G^L_{i-1,0}                   (b x nl)
     T* const gbufs[2] = {G, Gn};
     T* const g0bufs[2] = {G0, G0n};
     auto other = [](T* const* bufs, T* cur) { return cur == bufs[0] ? bufs[1] : bufs[0]; };
@@ -812,6 +863,13 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         inverse<T>(c, b, W(Aw, b), none<T>(), nullptr, W(G, b), st, P, -1, in.act);
         gemm<T>(c, nl, b, nl, OP_N, in.gl, OP_N, in.Hld, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         gemm<T>(c, nl, b, b, OP_N, R(T2, nl), OP_N, R(G, b), W(G0, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+        if (one) {
+            // G^L_{1,0} = G^L_1 (Hld^+ gl), G^L_{00,1} = gl + G^L_{0,1} (Hld^+ gl)
+            gemm<T>(c, b, nl, b, OP_N, R(G, b), OP_N, R(T1, b), W(Gc, b), 1.0, none<T>(), 0.0, nullptr, P, -1,
+                    in.act);
+            gemm<T>(c, nl, nl, b, OP_N, R(G0, nl), OP_N, R(T1, b), W(G00, nl), 1.0, in.gl, 1.0, nullptr, P, -1,
+                    in.act);
+        }
         for (int i = 1; i < L; ++i) {
             const int bp = in.bsz[i - 1], bi = in.bsz[i];
             const BMat<T>& V = in.Hdv[i - 1];
@@ -829,6 +887,16 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
             inverse<T>(c, bi, W(Aw, bi), none<T>(), nullptr, W(Gi, bi), st, P, -1, in.act);
             gemm<T>(c, nl, bi, bp, OP_N, R(G0, nl), OP_N, V, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
             gemm<T>(c, nl, bi, bi, OP_N, R(T2, nl), OP_N, R(Gi, bi), W(G0i, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            if (one) {
+                // T3 = V^+ G^L_{i-1,0}; G^L_{i,0} = G^L_i T3; G^L_{00,i} += G^L_{0,i} T3
+                gemm<T>(c, bi, nl, bp, OP_C, V, OP_N, R(Gc, bp), W(T3, bi), 1.0, none<T>(), 0.0, nullptr, P, -1,
+                        in.act);
+                gemm<T>(c, bi, nl, bi, OP_N, R(Gi, bi), OP_N, R(T3, bi), W(Gcn, bi), 1.0, none<T>(), 0.0, nullptr, P,
+                        -1, in.act);
+                gemm<T>(c, nl, nl, bi, OP_N, R(G0i, nl), OP_N, R(T3, bi), W(G00, nl), 1.0, R(G00, nl), 1.0, nullptr,
+                        P, -1, in.act);
+                std::swap(Gc, Gcn);
+            }
             G = Gi;
             G0 = G0i;
         }
@@ -851,6 +919,61 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         gemm<T>(c, nr, nl, nl, OP_C, ro(gdlr), OP_N, in.gml, W(Y, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         trace_prod<T>(c, nl, nr, R(X, nl), R(Y, nr), out.T_LR, P, in.act);
         if (adj) adjoint_backward<T>(c, nl, nr, in, zd, P, mm, gdlr, gRo, Vc, opVc, opVcH, *adj, T1, T2);
+        if (one) {
+            // One sweep.  The right-to-left sweep inverts the same block-
+            // tridiagonal system as this one except at its two ends: there the
+            // left lead's surface layer is an explicit block z_dev - Haml - sl
+            // (here folded in through gl, whose inverse is z_lead - Haml - sl
+            // by the decimation's fixed point) and the right lead's surface
+            // layer is folded in through gr (here the explicit closing block
+            // z_dev - Hamr - sr).  The two systems differ by
+            // Delta = diag(d I, -d I) on these end blocks, d = z_dev - z_lead,
+            // so with the corner blocks G_CC of this system's inverse
+            // (C = {left surface, closing layer}) the other's are
+            //   G'_CC = (I + G_CC Delta)^-1 G_CC,
+            // G_RL = G'_c0 and G_L = G'_00 (negf/lesser.f90:694-793 computes
+            // them by the second sweep).
+            const int n2 = nl + nr;
+            T* Tc = T2_b;
+            T* Gc0 = X_b;
+            // Tc = Vc^+ G^L_{N,0}; G_c0 = G_cc Tc; G_00 = G^L_{00,N} + G_0c Tc
+            gemm<T>(c, nr, nl, bN, opVcH, Vc, OP_N, R(Gc, bN), W(Tc, nr), 1.0, none<T>(), 0.0, nullptr, P, -1,
+                    in.act);
+            gemm<T>(c, nr, nl, nr, OP_N, ro(gRo), OP_N, R(Tc, nr), W(Gc0, nr), 1.0, none<T>(), 0.0, nullptr, P, -1,
+                    in.act);
+            gemm<T>(c, nl, nl, nr, OP_N, ro(gdlr), OP_N, R(Tc, nr), W(G00, nl), 1.0, R(G00, nl), 1.0, nullptr, P, -1,
+                    in.act);
+            const long m4 = 4 * mm, m2 = 2 * mm;
+            const BMatW<T> Mw = bw(CM, m4, n2), Miw = bw(CMi, m4, n2), C0w = bw(CC0, m2, n2), Gpw = bw(CGp, m2, n2);
+            audit<T>(c, R(G00, nl), nl, nl, P, 1, "corner G00");
+            audit<T>(c, ro(gdlr), nl, nr, P, 1, "corner G0c");
+            audit<T>(c, R(Gc0, nr), nr, nl, P, 1, "corner Gc0");
+            audit<T>(c, ro(gRo), nr, nr, P, 1, "corner Gcc");
+            audit<T>(c, Mw, n2, n2, P, 1, "corner M");
+            audit<T>(c, C0w, n2, nl, P, 1, "corner C0");
+            check_range(c, in.zlead, sizeof(T) * ((size_t)in.zlead_stride * (P - 1) + 2), "corner z_lead");
+            check_range(c, zd, sizeof(T) * 2 * P, "corner z_dev");
+            check_range(c, in.act, sizeof(int) * P, "corner active");
+            launch(c, NT_K_ELEM, 0, "corner system", [&] {
+                hipLaunchKernelGGL(k_corner_system, dim3(nblk((long)n2 * n2), P), dim3(256), 0, s, nl, nr,
+                                   R(G00, nl), ro(gdlr), R(Gc0, nr), ro(gRo), zd, in.zlead, in.zlead_stride, Mw, C0w,
+                                   in.act);
+                return hipGetLastError();
+            });
+            inverse<T>(c, n2, Mw, none<T>(), nullptr, Miw, st, P, -1, in.act);
+            gemm<T>(c, n2, nl, n2, OP_N, ro(Miw), OP_N, ro(C0w), Gpw, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            // G_RL = rows nl.. of the corrected column, G_L = rows 0..nl-1
+            const BMat<T> grl = bm<T>(CGp + 2 * (long)nl, m2, n2);
+            const BMat<T> gll = bm<T>(CGp, m2, n2);
+            if (out.G_D_RL.base) copy_op<T>(c, nr, nl, grl, out.G_D_RL, 0, (T)1, P, in.act);
+            if (out.G_L.base) copy_op<T>(c, nl, nl, gll, out.G_L, 0, (T)1, P, in.act);
+            // Caroli T_RL = Re Tr[(G_RL gml)(G_RL^+ gmr)]
+            gemm<T>(c, nr, nl, nl, OP_N, grl, OP_N, in.gml, W(X_b, nr), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, nl, nr, nr, OP_C, grl, OP_N, in.gmr, W(Y_b, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            trace_prod<T>(c, nr, nl, R(X_b, nr), R(Y_b, nl), out.T_RL, P, in.act);
+            (void)gLo;
+            return;
+        }
     }
     // ---------------- right to left, on the auxiliary stream when enabled: it
     // needs only the inputs, so it runs beside the left-to-right sweep and
@@ -1391,6 +1514,8 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         in.gr = bm<T>(gbuf + 2 * nmm, 2 * nmm, nr);
         in.sr = bm<T>(sigbuf + 2 * nmm, 2 * nmm, nr);
         in.gmr = bm<T>(gambuf + 2 * nmm, 2 * nmm, nr);
+        in.zlead = zdec + 2;          // right retarded E + i eta(E) of each point (stride 4)
+        in.zlead_stride = 4;
         if (probe) {
             in.zl = ar.ptr<T>(o_pzl);
             in.keep = ar.ptr<T>(o_pkeep);
@@ -3076,6 +3201,13 @@ int nt_adjoint_bytes_per_point(nt_ctx* c, int nl, int nr, int b, int L, int lo, 
     if (!c || !bytes || nl <= 0 || nr <= 0 || b <= 0 || L <= 0 || lo < 0 || hi < lo || hi >= L) return NT_ERR_ARG;
     const long mm = (long)std::max(std::max(nl, nr), b) * std::max(std::max(nl, nr), b);
     *bytes = (long long)adj_slots(lo, hi, L, adj_interval(c->adj_ckpt, lo, L)) * mm * (long long)sizeof(nt_c128);
+    return NT_OK;
+}
+
+int nt_set_sweeps(nt_ctx* c, int n)
+{
+    if (!c || (n != 1 && n != 2)) return NT_ERR_ARG;
+    c->one_sweep = n == 1;
     return NT_OK;
 }
 

@@ -53,6 +53,9 @@ struct nt_ctx {
     // adjoint: keep G^L of every layer (0), every n-th (n > 0) or about
     // sqrt(L - lo)-th (< 0) from the window to the end; the rest recomputed
     int adj_ckpt = 0;
+    // T_RL (and G_D_RL, G_L) from the left-to-right sweep and a corner
+    // correction instead of the right-to-left sweep
+    bool one_sweep = true;
     // device address ranges a launch may touch: the workspace and the caller's
     // buffers of the current call (sizes derived from the call's arguments)
     std::vector<std::pair<uintptr_t, uintptr_t>> regions;
@@ -372,6 +375,11 @@ struct RgfInput {
     T* keep = nullptr;
     // optional per-point mask: only points with act[p] != 0 are computed
     const int* act = nullptr;
+    // retarded lead shift z_lead = E + i eta(E) of each point (complex at
+    // zlead + zlead_stride p): with it the right-to-left result can come from
+    // the left-to-right sweep (one_sweep_correction)
+    const T* zlead = nullptr;
+    int zlead_stride = 2;
 };
 
 template <typename T>

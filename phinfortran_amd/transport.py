@@ -551,6 +551,11 @@ class Engine:
         """FP64 GEMMs with three real products per complex product."""
         self._check(self._L.nt_set_gemm_3m(self.ctx, int(bool(on))), "nt_set_gemm_3m")
 
+    def set_sweeps(self, n: int):
+        """batched solves: 1 = left-to-right sweep + corner correction for the
+        right-to-left quantities (default), 2 = both sweeps as the reference"""
+        self._check(self._L.nt_set_sweeps(self.ctx, int(n)), "nt_set_sweeps")
+
     def set_adjoint_checkpoint(self, every: int):
         """adjoint storage: every layer (0), every n-th (n), about sqrt (-1)"""
         self._check(self._L.nt_set_adjoint_checkpoint(self.ctx, int(every)), "nt_set_adjoint_checkpoint")

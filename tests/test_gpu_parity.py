@@ -285,3 +285,39 @@ def test_batched_unequal_leads(engine, T, nl, nr, b):
         assert rel(r["gr_ret"][i].cpu().numpy(), sg["gr_ret"]) < 1e-8
         sdos = -2.0 * np.sqrt(e) * np.trace(sg["gr_ret"]).imag / np.pi
         assert abs(r["sdos_r"][i].item() - sdos) < 1e-9 * max(1, abs(sdos))
+
+
+@pytest.mark.parametrize("case", ["random", "film", "unequal", "ribbon"])
+def test_one_sweep_matches_second_sweep(engine, T, case):
+    """The default batched solve takes T_RL, G_D_RL and G_L from the left-to-
+    right sweep and the corner correction of its two end blocks; the
+    reference's second (right-to-left) sweep (nt_set_sweeps(2)) gives the
+    same values to the rounding of the inverted blocks."""
+    if case == "random":
+        blk, p, E = lt.random_blocks(24, 24, 5, seed=11), T.params("phonon", eta=1e-3, eta0=1e-4), \
+            np.array([0.13, 0.4, 0.77])
+    elif case == "film":
+        blk, p, E = lt.si_ge_interface_film(nx=2, ny=1, L=8, kfrac=(0.125, 0.25)), T.params("phonon"), \
+            np.array([2.5, 6.0, 9.5, 13.0]) ** 2
+    elif case == "unequal":
+        blk, p, E = lt.random_blocks(12, 16, 4, nr=20, seed=12, scale=0.7), \
+            T.params("phonon", eta=1e-3, eta0=1e-4), np.array([0.3, 0.6])
+    else:
+        blk, p, E = lt.agnr_blocks(n_dimer=12, L=6, seed=3), T.params("tb", eta=1e-6), np.array([-1.1, 0.35, 2.2])
+    print(f"one sweep vs two sweeps: {case}", flush=True)
+    d = engine.upload_blocks(blk)
+    keys = ("G_D_RL", "G_L", "G_D_LR", "G_R")
+    try:
+        engine.set_sweeps(2)
+        two = engine.solve_tdb(d, E, p, extras=keys)
+    finally:
+        engine.set_sweeps(1)
+    one = engine.solve_tdb(d, E, p, extras=keys)
+    assert (one["status"].cpu().numpy() == 0).all()
+    for k in ("T_LR", "T_RL"):
+        a, b = one[k].cpu().numpy(), two[k].cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-10 * max(1.0, np.abs(b).max()), (case, k, a, b)
+    for k in keys:
+        a, b = one[k].cpu().numpy(), two[k].cpu().numpy()
+        for i in range(len(E)):
+            assert rel(a[i], b[i]) < ctol(b[i] if b[i].shape[0] == b[i].shape[1] else b[i] @ b[i].conj().T), (case, k, i)

@@ -1227,6 +1227,9 @@ struct TdbArgs {
     // workspace in the static scratch (no device allocation); the chunk of
     // points shrinks until the plan fits
     bool noalloc = false;
+    // devices with the same leads, points (E, k map) and lead sizes: their
+    // RGF sweeps reuse this call's lead Green's functions (FP64, no extras)
+    std::vector<const TdbArgs*> siblings;
 };
 
 // adjoint storage is capped so that a chunk of points fits this budget
@@ -1262,6 +1265,8 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     const bool adjoint = a.adj_w != nullptr;
     const int nmax = std::max(nl, nr);
     const long nmm = (long)nmax * nmax;
+    int bplan = b;       // block size the RGF scratch is planned for (largest of the siblings)
+    for (const TdbArgs* sb : a.siblings) bplan = std::max(bplan, sb->b);
     const long mmr = (long)std::max(nmax, b) * std::max(nmax, b);   // RGF slot size
     int Pmax = std::min(a.npts, c->batch_limit);
     const int adjK = adjoint ? adj_interval(c->adj_ckpt, a.adj_lo, L) : 1;
@@ -1335,7 +1340,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         need_dec = lead_gfs_scratch(prm, nmax, 2 * Pmax, 2 * a.nk);
         // the mixed solve, then (for rejected points) the FP64 sweep, share the region
         need_rgf = sweep32 ? std::max(mixed_caroli_scratch(nl, nr, b, L, Pmax), rgf_scratch<T>(nl, nr, b, Pmax))
-                                        : rgf_scratch<T>(nl, nr, b, Pmax);
+                                        : rgf_scratch<T>(nl, nr, bplan, Pmax);
         o_adj = o_scr + std::max(need_dec, need_rgf);
         need_adj = 0;
         if (adjoint) {
@@ -1661,6 +1666,38 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                            a.status ? a.status + p0 : nullptr, a.iters ? a.iters + p0 : nullptr,
                            a.first_mask ? a.first_mask + p0 : nullptr, a.T_LR + p0, a.T_RL + p0);
         hipcheck(c, hipGetLastError(), "point status");
+        // ---- devices sharing these leads: their own sweeps on the same lead GFs
+        for (const TdbArgs* sb : a.siblings) {
+            const int Ls = sb->L, bs = sb->b;
+            const long bbs = (long)bs * bs;
+            const T* Hd_s = reinterpret_cast<const T*>(sb->Hamd);
+            const T* Hdv_s = reinterpret_cast<const T*>(sb->Hamdv);
+            RgfInput<T> ins = in;
+            ins.L = Ls;
+            ins.bsz.assign(Ls, bs);
+            ins.Hd.clear();
+            ins.Hdv.clear();
+            for (int i = 0; i < Ls; ++i) ins.Hd.push_back(bm<T>(Hd_s + 2 * i * bbs, kst((long)Ls * bbs), bs, kidx, 0, kn));
+            for (int i = 0; i + 1 < Ls; ++i)
+                ins.Hdv.push_back(bm<T>(Hdv_s + 2 * i * bbs, kst((long)std::max(Ls - 1, 1) * bbs), bs, kidx, 0, kn));
+            ins.Hld = bm<T>(reinterpret_cast<const T*>(sb->Hamld), kst((long)nl * bs), nl, kidx, 0, kn);
+            ins.Hrd = bm<T>(reinterpret_cast<const T*>(sb->Hamrd), kst((long)nr * bs), nr, kidx, 0, kn);
+            RgfOutput<T> outs;
+            outs.T_LR = sb->T_LR + p0;
+            outs.T_RL = sb->T_RL + p0;
+            outs.status = strgf;
+            outs.G_D_LR = outs.G_D_RL = outs.G_L = outs.G_R = nonew<T>();
+            hipcheck(c, hipMemsetAsync(strgf, 0, sizeof(int) * P, s), "memset");
+            rgf_run<T>(c, nl, nr, ins, zdev, P, outs, ar, o_scr);
+            check_range(c, sb->T_LR + p0, sizeof(double) * P, "sibling T_LR");
+            check_range(c, sb->T_RL + p0, sizeof(double) * P, "sibling T_RL");
+            if (sb->status) check_range(c, sb->status + p0, sizeof(int) * P, "sibling status");
+            if (sb->first_mask) check_range(c, sb->first_mask + p0, sizeof(int) * P, "sibling mask");
+            hipLaunchKernelGGL(k_point_status, dim3(nblk(P)), dim3(256), 0, s, stdec, itdec, strgf, P,
+                               sb->status ? sb->status + p0 : nullptr, nullptr,
+                               sb->first_mask ? sb->first_mask + p0 : nullptr, sb->T_LR + p0, sb->T_RL + p0);
+            hipcheck(c, hipGetLastError(), "sibling point status");
+        }
         // mean iterations (host side, for the flop model)
         hit.resize(2 * P);
         hipcheck(c, hipMemcpyAsync(hit.data(), itdec, sizeof(int) * 2 * P, hipMemcpyDeviceToHost, s), "memcpy");
@@ -2685,23 +2722,83 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
         return b * b * b * (devs[d].L + 30.0) * devs[d].npts;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
-    std::vector<std::pair<int, int>> tuned;   // (bytes, n) already tuned this call
+    // Devices with the same lead buffers, lead sizes and points (E and k
+    // map) form a group whose lead Green's functions are computed once (FP64
+    // solves; a mixed-precision device is a group of its own).
+    auto same_leads = [&](const nt_device_desc& x, const nt_device_desc& y) {
+        return x.nl == y.nl && x.nr == y.nr && x.nk == y.nk && x.npts == y.npts && x.Haml == y.Haml &&
+               x.Hamlv == y.Hamlv && x.Hamr == y.Hamr && x.Hamrv == y.Hamrv && x.E == y.E && x.kidx == y.kidx;
+    };
+    std::vector<std::vector<int>> groups;
     for (int d : order) {
-        const nt_device_desc& D = devs[d];
-        if (D.npts == 0) continue;
+        if (devs[d].npts == 0) continue;
+        bool placed = false;
+        if (p->precision == NT_PREC_FP64)
+            for (auto& g : groups)
+                if (same_leads(devs[g[0]], devs[d])) {
+                    g.push_back(d);
+                    placed = true;
+                    break;
+                }
+        if (!placed) groups.push_back({d});
+    }
+    std::vector<std::pair<int, int>> tuned;   // (bytes, n) already tuned this call
+    for (const auto& g : groups) {
         const int bytes = p->precision == NT_PREC_FP64 ? 8 : 4;
-        for (int n : {D.b, std::max(D.nl, D.nr)}) {
-            if (c->inv_nb.count({bytes, n}) || std::find(tuned.begin(), tuned.end(), std::make_pair(bytes, n)) !=
-                                                    tuned.end())
-                continue;
-            const int tb = std::min(D.npts, c->batch_limit);
-            const int st = nt_autotune_inverse(c, n, std::max(1, std::min(tb, 256)), p->precision, nullptr, nullptr);
-            if (st) return st;
-            tuned.push_back({bytes, n});
+        for (int d : g) {
+            const nt_device_desc& D = devs[d];
+            for (int n : {D.b, std::max(D.nl, D.nr)}) {
+                if (c->inv_nb.count({bytes, n}) ||
+                    std::find(tuned.begin(), tuned.end(), std::make_pair(bytes, n)) != tuned.end())
+                    continue;
+                const int tb = std::min(D.npts, c->batch_limit);
+                const int st = nt_autotune_inverse(c, n, std::max(1, std::min(tb, 256)), p->precision, nullptr,
+                                                   nullptr);
+                if (st) return st;
+                tuned.push_back({bytes, n});
+            }
         }
-        const int st = nt_solve_tdb_batched_dev(c, p, D.nl, D.nr, D.b, D.L, D.npts, D.E, D.kidx, D.nk, D.Haml, D.Hamlv,
-                                                D.Hamr, D.Hamrv, D.Hamd, D.Hamdv, D.Hamld, D.Hamrd,
-                                                D.first_point_mask, D.T_LR, D.T_RL, D.status, nullptr, nullptr);
+        const nt_device_desc& D0 = devs[g[0]];
+        const int st = guarded(c, [&] {
+            const size_t z = sizeof(nt_c128);
+            std::vector<TdbArgs> args;
+            args.reserve(g.size());
+            for (int d : g) {
+                const nt_device_desc& D = devs[d];
+                const size_t nll = (size_t)D.nl * D.nl, nrr = (size_t)D.nr * D.nr, bb = (size_t)D.b * D.b;
+                reg_region(c, D.E, sizeof(double) * D.npts);
+                reg_region(c, D.kidx, sizeof(int) * D.npts);
+                reg_region(c, D.Haml, z * nll * D.nk);
+                reg_region(c, D.Hamlv, z * nll * D.nk);
+                reg_region(c, D.Hamr, z * nrr * D.nk);
+                reg_region(c, D.Hamrv, z * nrr * D.nk);
+                reg_region(c, D.Hamd, z * bb * D.L * D.nk);
+                reg_region(c, D.Hamdv, z * bb * std::max(D.L - 1, 1) * D.nk);
+                reg_region(c, D.Hamld, z * D.nl * D.b * D.nk);
+                reg_region(c, D.Hamrd, z * D.nr * D.b * D.nk);
+                reg_region(c, D.first_point_mask, sizeof(int) * D.npts);
+                reg_region(c, D.T_LR, sizeof(double) * D.npts);
+                reg_region(c, D.T_RL, sizeof(double) * D.npts);
+                reg_region(c, D.status, sizeof(int) * D.npts);
+                args.push_back(TdbArgs{D.nl, D.nr, D.b, D.L, D.npts, D.nk, D.E, D.kidx, D.Haml, D.Hamlv, D.Hamr,
+                                       D.Hamrv, D.Hamd, D.Hamdv, D.Hamld, D.Hamrd, D.first_point_mask, D.T_LR,
+                                       D.T_RL, D.status, nullptr, nt_solve_extras{}});
+            }
+            if (D0.kidx) {
+                std::vector<int> hk(D0.npts);
+                hipcheck(c, hipMemcpyAsync(hk.data(), D0.kidx, sizeof(int) * D0.npts, hipMemcpyDeviceToHost,
+                                           c->stream), "D2H");
+                hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+                for (int v : hk)
+                    if (v < 0 || v >= D0.nk) {
+                        c->err = "kidx out of range [0, nk)";
+                        return (int)NT_ERR_ARG;
+                    }
+            }
+            for (size_t i = 1; i < args.size(); ++i) args[0].siblings.push_back(&args[i]);
+            solve_tdb_dev(c, p, args[0]);
+            return 0;
+        });
         if (st < 0 && st != NT_ERR_NAN && st != NT_ERR_NOT_CONVERGED) return st;
     }
     return NT_OK;

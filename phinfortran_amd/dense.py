@@ -20,14 +20,17 @@ import numpy as np
 TOL = 1.0e-7     # sparsity threshold of blockwise_setup (negf/device.f90:50,108)
 
 
-def _mx(a):
-    """maxval(abs(.)) with Fortran's value for an empty array (-huge)."""
-    return float(np.max(a)) if a.size else -np.inf
-
-
 def blockwise_setup(Ham, nl: int, nr: int, n_buffer_l: int = 0, n_buffer_r: int = 0, tol: float = TOL) -> List[int]:
-    """layer_list of the device region, following negf/device.f90:11-249 step
-    for step (1-based indices kept in the variable names; slices converted).
+    """layer_list of the device region, as negf/device.f90:11-249 computes it
+    (1-based indices kept in the variable names).
+
+    The reference scans candidate layer pairs and takes maxval(abs(.)) of
+    sub-blocks of the device Hamiltonian for each; here every such test is
+    one lookup in a 2-D prefix count of the entries above tol, and the first
+    scan stops for a row as soon as a candidate qualifies or can no longer
+    beat the widest remaining coupling (the width only shrinks along the
+    scan): O(nd^2) array work and, for a block-banded H, about O(nd) Python
+    steps (the reference restatement of the scan took 2.8 s at nd = 640).
 
     Quirks kept: every improving candidate of the first scan appends a layer,
     the second scan continues its fixed j range with the updated start, and
@@ -36,31 +39,38 @@ def blockwise_setup(Ham, nl: int, nr: int, n_buffer_l: int = 0, n_buffer_r: int 
     A = np.abs(np.asarray(Ham))
     norb = A.shape[0]
     nd = norb - n_buffer_l - n_buffer_r - 2 * nl - 2 * nr
-    d0 = n_buffer_l + 2 * nl                       # 0-based first device orbital
+    d0 = n_buffer_l + 2 * nl
+    D = A[d0:d0 + nd, d0:d0 + nd] > tol
+    # count of entries above tol in HD(1:r, 1:c) (1-based, inclusive)
+    P = np.zeros((nd + 1, nd + 1), np.int64)
+    P[1:, 1:] = np.cumsum(np.cumsum(D, axis=0), axis=1)
 
-    def HD(r1, r2, c1, c2):                        # HD(r1:r2, c1:c2), 1-based inclusive
+    def nz(r1, r2, c1, c2):
+        """any entry of HD(r1:r2, c1:c2) above tol (empty ranges: none)"""
         r1, c1 = max(r1, 1), max(c1, 1)
+        r2, c2 = min(r2, nd), min(c2, nd)
         if r2 < r1 or c2 < c1:
-            return np.zeros((0,))
-        return A[d0 + r1 - 1:d0 + r2, d0 + c1 - 1:d0 + c2]
+            return False
+        return P[r2, c2] - P[r1 - 1, c2] - P[r2, c1 - 1] + P[r1 - 1, c1 - 1] > 0
 
-    # coupling range of the left lead into the device (:44-55)
-    VLD = A[n_buffer_l + nl:n_buffer_l + 2 * nl, d0:d0 + nd]
+    VLD = A[n_buffer_l + nl:n_buffer_l + 2 * nl, d0:d0 + nd] > tol
     maxij1 = 0
     for i in range(nl):
-        for j in range(1, nd):
-            if VLD[i, j - 1] > tol and _mx(VLD[i, j:nd]) < tol and j >= maxij1:
+        cols = np.nonzero(VLD[i])[0]
+        if cols.size:
+            j = int(cols[-1]) + 1                    # last column above tol, 1-based
+            if 1 <= j <= nd - 1 and j >= maxij1:
                 maxij1 = j
-    # coupling range of the right lead into the device (:57-70)
     r0 = norb - n_buffer_r - 2 * nr
-    VRD = A[r0:r0 + nr, r0 - nd:r0]
+    VRD = A[r0:r0 + nr, r0 - nd:r0] > tol
     maxij3 = 0
     for i in range(nr):
-        for j in range(2, nd + 1):
-            if VRD[i, j - 1] > tol and _mx(VRD[i, 0:j - 1]) < tol and nd - j + 1 > maxij3:
+        cols = np.nonzero(VRD[i])[0]
+        if cols.size:
+            j = int(cols[0]) + 1                     # first column above tol, 1-based
+            if 2 <= j <= nd and nd - j + 1 > maxij3:
                 maxij3 = nd - j + 1
-
-    layers: List[int] = []
+    layers = []
     start, layer1, layer2, cont, start0 = 1, 1, 1, 0, 1
     lay1 = lay2 = 0
     while start < nd - 1 and cont < 2 and start >= start0:
@@ -72,26 +82,27 @@ def blockwise_setup(Ham, nl: int, nr: int, n_buffer_l: int = 0, n_buffer_r: int 
                     layer1 = i + 1 - start
                     layer2 = j - i
                     c0 = start + layer1 + layer2
-                    Vt = HD(start, start + layer1 - 1, c0, nd)
-                    Vl = HD(start, start + layer1 - 1, c0 - 1, c0 - 1)
-                    Vb = HD(start + layer1, start + layer1, c0, nd)
                     width = nd - start - layer1 - layer2 + 1
-                    if _mx(Vl) > tol and _mx(Vb) > tol and _mx(Vt) < tol and width > maxij2:
+                    if width <= maxij2:
+                        break                         # width only shrinks with j
+                    r1, r2 = start, start + layer1 - 1
+                    if nz(r1, r2, c0 - 1, c0 - 1) and nz(r1 + layer1, r1 + layer1, c0, nd) and \
+                            not nz(r1, r2, c0, nd):
                         maxij2 = width
                         lay1, lay2 = layer1, layer2
                         layers.append(lay1)
+                        break                         # later j are narrower
             cont += 1
             start += lay1
         else:
             layer1 = lay2
             cont = 2
-            for j in range(start + layer1, nd):        # bounds fixed at loop entry
+            for j in range(start + layer1, nd):
                 layer2 = j + 1 - start - layer1
                 c0 = start + layer1 + layer2
-                Vt = HD(start, start + layer1 - 1, c0, nd)
-                Vl = HD(start, start + layer1 - 1, c0 - 1, c0 - 1)
                 width = nd - start - layer1 - layer2 + 1
-                if _mx(Vl) > tol and _mx(Vt) < tol and width > maxij2:
+                r1, r2 = start, start + layer1 - 1
+                if nz(r1, r2, c0 - 1, c0 - 1) and not nz(r1, r2, c0, nd) and width > maxij2:
                     maxij2 = width
                     lay1, lay2 = layer1, layer2
                     layers.append(lay1)
@@ -100,7 +111,6 @@ def blockwise_setup(Ham, nl: int, nr: int, n_buffer_l: int = 0, n_buffer_r: int 
                     if start + layer1 + layer2 == nd + 1:
                         layers.append(lay1)
     layers.append(nd - sum(layers))
-    # merge the last block while it is narrower than the right coupling (:199-212)
     while len(layers) > 1 and layers[-1] < maxij3:
         last = layers.pop()
         layers[-1] += last

@@ -541,6 +541,19 @@ int nt_set_overlap(nt_ctx* ctx, int on);
  * solve and leads given with Hrd_fwd / dense lower couplings always run both
  * sweeps. */
 int nt_set_sweeps(nt_ctx* ctx, int n);
+/* Surface decimation (negf/surface.f90:484-497, the convergence test of its
+ * loop): the host reads the number of unconverged (omega,k) items every
+ * `every` iterations (default 4) and stops when it is zero; items that have
+ * converged are masked inside every kernel, so the results do not depend on
+ * `every`.  every = 0: the host never reads it, each decimation runs max_iter
+ * iterations, and the FP64 batched solve (nt_solve_tdb_batched_dev and
+ * _noalloc_dev, direct decimation, profiling off) makes no host
+ * synchronisation, so a caller can capture it in a HIP graph and replay it
+ * (with the allocation-free entry, or after a first call that sized the
+ * workspace); set max_iter to the bound the points need (items still
+ * unconverged then get status -2, as with the same max_iter eagerly).
+ * No reference counterpart. */
+int nt_set_decimation_check(nt_ctx* ctx, int every);
 /* Storage of nt_solve_tdb_adjoint_batched_dev's backward pass: the left-
  * connected blocks G^L_i, G^L_{0,i} of the layers from the window to the end
  * are kept for every layer (every = 0, default), for every n-th layer

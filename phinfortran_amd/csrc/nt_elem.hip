@@ -197,20 +197,21 @@ __global__ void k_make_shifts(const double* E, int batch, double eta, double eta
     if (zd32) { zd32[2 * p] = (float)e; zd32[2 * p + 1] = (float)ed; }
 }
 
-__global__ void k_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv, int* n_active,
-                                  int batch)
+__global__ void k_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv, int* count,
+                                  int* clear, int batch)
 {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     int still = 0;
     if (p < batch && active[p]) {
         const double m = __longlong_as_double((long long)maxabs[p]);
-        iters[p] += 1;
+        if (iters) iters[p] += 1;
         still = (m > conv) ? 1 : 0;     // inf (NaN marker) stays active and hits the cap
         active[p] = still;
     }
     if (p < batch) maxabs[p] = 0ull;
+    if (p == 0) *clear = 0;
     for (int off = 32; off > 0; off >>= 1) still += __shfl_xor(still, off);
-    if ((threadIdx.x & 63) == 0 && still) atomicAdd(n_active, still);
+    if ((threadIdx.x & 63) == 0 && still) atomicAdd(count, still);
 }
 
 __global__ void k_fill_int(int* p, int v, int n)
@@ -308,12 +309,12 @@ hipError_t launch_make_shifts(const double* E, int batch, double eta, double eta
     return hipGetLastError();
 }
 
-hipError_t launch_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv, int* n_active,
-                                  int batch, hipStream_t s)
+hipError_t launch_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv, int* count,
+                                  int* clear, int batch, hipStream_t s)
 {
     if (batch <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dec_step_update, dim3(blocks_for(batch)), dim3(NT), 0, s, active, maxabs, iters, conv,
-                       n_active, batch);
+                       count, clear, batch);
     return hipGetLastError();
 }
 

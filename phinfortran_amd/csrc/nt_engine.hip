@@ -210,7 +210,7 @@ void decimate(nt_ctx* c, int n, int Q, BMat<T> H00, BMat<T> V, int alpha_dagger,
     T* W = sa.ptr<T>(sa.take(mb * Q));
     int* active = sa.ptr<int>(sa.take(sizeof(int) * Q));
     unsigned long long* maxabs = sa.ptr<unsigned long long>(sa.take(sizeof(unsigned long long) * Q));
-    int* nact = sa.ptr<int>(sa.take(sizeof(int)));
+    int* nact = sa.ptr<int>(sa.take(sizeof(int) * 2));    // two-slot count of unconverged items
 
     hipStream_t s = c->stream;
     audit<T>(c, H00, n, n, Q, 1, "dec H00");
@@ -222,49 +222,65 @@ void decimate(nt_ctx* c, int n, int Q, BMat<T> H00, BMat<T> V, int alpha_dagger,
     check_range(c, z, sizeof(T) * 2 * Q, "dec z");
     check_range(c, status, sizeof(int) * Q, "dec status");
     check_range(c, iters, sizeof(int) * Q, "dec iters");
-    hipcheck(c, hipMemsetAsync(iters, 0, sizeof(int) * Q, s), "memset");
-    hipcheck(c, hipMemsetAsync(maxabs, 0, sizeof(unsigned long long) * Q, s), "memset");
-    hipcheck(c, hipMemsetAsync(nact, 0, sizeof(int), s), "memset");
-    // initial maxdiff = max|alpha| (surface.f90:484): one pass through the GEMM
-    // epilogue is avoided with a direct reduction below via dec_step_update.
+    zero_dev(c, iters, sizeof(int) * Q, s);
+    zero_dev(c, maxabs, sizeof(unsigned long long) * Q, s);
+    zero_dev(c, nact, sizeof(int) * 2, s);
     launch(c, NT_K_ELEM, 0, "fill", [&] { return launch_fill_int(active, 1, Q, s); });
-    // max|alpha0|: alpha0 = 1 * alpha0 * I via copy -> use the GEMM epilogue on a
-    // K=0 product with Cin = alpha0 (pure copy + maxabs)
+    // initial maxdiff = max|alpha| (surface.f90:484): the GEMM epilogue's
+    // maxabs on a K = 0 product with Cin = alpha0 (a copy), then one update
+    // (update 0: slot 0) that does not count as an iteration
     gemm<T>(c, n, n, 0, OP_N, none<T>(), OP_N, none<T>(), bw(W, nn, n), 1.0, bm<T>(ab0, 2 * nn, n), 1.0, nullptr, Q,
             0.0, nullptr, 1, nonew<T>(), maxabs);
-    // the iteration counter must not count this probe: update with a zero
-    // increment by resetting iters afterwards
     launch(c, NT_K_ELEM, 0, "dec_update",
-           [&] { return launch_dec_step_update(active, maxabs, iters, conv, nact, Q, s); });
-    hipcheck(c, hipMemsetAsync(iters, 0, sizeof(int) * Q, s), "memset");
+           [&] { return launch_dec_step_update(active, maxabs, nullptr, conv, nact, nact + 1, Q, s); });
 
+    // Iteration it runs its launches on the items still active (converged
+    // items are masked inside every kernel, so an extra iteration changes
+    // nothing) and ends with update it + 1.  Every dec_check iterations the
+    // host reads the count of update it and stops at zero.  The flops of the
+    // profile follow from the per-item iteration counts at the end.
     T* abc = ab0;
     T* abn = ab1;
-    int n_active = Q;
-    for (int it = 0;; ++it) {
-        hipcheck(c, hipMemcpyAsync(c->host_flag, nact, sizeof(int), hipMemcpyDeviceToHost, s), "memcpy");
-        hipcheck(c, hipStreamSynchronize(s), "sync");
-        n_active = *c->host_flag;
-        if (n_active == 0 || it >= max_iter) break;
-        hipcheck(c, hipMemsetAsync(nact, 0, sizeof(int), s), "memset");
+    const int every = std::max(0, c->dec_check);
+    for (int it = 0; it < max_iter; ++it) {
+        if (every > 0 && it % every == 0) {
+            hipcheck(c, hipMemcpyAsync(c->host_flag, nact + (it & 1), sizeof(int), hipMemcpyDeviceToHost, s),
+                     "memcpy");
+            hipcheck(c, hipStreamSynchronize(s), "sync");
+            if (*c->host_flag == 0) break;
+        }
+        const double nf = 0.0;    // profiled flops: added from the iteration counts below
         // ive = (z - e1)^-1
-        inverse<T>(c, n, bw(W, nn, n), bm<T>(e1, nn, n), z, bw(ive, nn, n), status, Q, n_active, active);
+        inverse<T>(c, n, bw(W, nn, n), bm<T>(e1, nn, n), z, bw(ive, nn, n), status, Q, nf, active);
         // [t1, t2] = [alpha, beta] ive
         gemm<T>(c, n, n, n, OP_N, bm<T>(abc, 2 * nn, n, nullptr, nn), OP_N, bm<T>(ive, nn, n),
-                bw(t12, 2 * nn, n, nullptr, nn), 1.0, none<T>(), 0.0, nullptr, Q, n_active, active, 2);
+                bw(t12, 2 * nn, n, nullptr, nn), 1.0, none<T>(), 0.0, nullptr, Q, nf, active, 2);
         // e1 += t1 beta ; es += t1 beta
         gemm<T>(c, n, n, n, OP_N, bm<T>(t12, 2 * nn, n), OP_N, bm<T>(abc + 2 * nn, 2 * nn, n), bw(e1, nn, n), 1.0,
-                bm<T>(e1, nn, n), 1.0, nullptr, Q, n_active, active, 1, bw(es, nn, n));
+                bm<T>(e1, nn, n), 1.0, nullptr, Q, nf, active, 1, bw(es, nn, n));
         // e1 += t2 alpha
         gemm<T>(c, n, n, n, OP_N, bm<T>(t12 + 2 * nn, 2 * nn, n), OP_N, bm<T>(abc, 2 * nn, n), bw(e1, nn, n), 1.0,
-                bm<T>(e1, nn, n), 1.0, nullptr, Q, n_active, active);
+                bm<T>(e1, nn, n), 1.0, nullptr, Q, nf, active);
         // [alpha', beta'] = [t1 alpha, t2 beta], maxabs over alpha'
         gemm<T>(c, n, n, n, OP_N, bm<T>(t12, 2 * nn, n, nullptr, nn), OP_N, bm<T>(abc, 2 * nn, n, nullptr, nn),
-                bw(abn, 2 * nn, n, nullptr, nn), 1.0, none<T>(), 0.0, nullptr, Q, n_active, active, 2,
+                bw(abn, 2 * nn, n, nullptr, nn), 1.0, none<T>(), 0.0, nullptr, Q, nf, active, 2,
                 nonew<T>(), maxabs);
-        launch(c, NT_K_ELEM, 0, "dec_update",
-               [&] { return launch_dec_step_update(active, maxabs, iters, conv, nact, Q, s); });
+        const int u = it + 1;
+        launch(c, NT_K_ELEM, 0, "dec_update", [&] {
+            return launch_dec_step_update(active, maxabs, iters, conv, nact + (u & 1), nact + ((u + 1) & 1), Q, s);
+        });
         std::swap(abc, abn);
+    }
+    if (c->prof) {
+        // every active item-iteration: one inverse and six n^3 products
+        std::vector<int> h(Q);
+        hipcheck(c, hipMemcpyAsync(h.data(), iters, sizeof(int) * Q, hipMemcpyDeviceToHost, s), "memcpy");
+        hipcheck(c, hipStreamSynchronize(s), "sync");
+        double tot = 0.0;
+        for (int q = 0; q < Q; ++q) tot += h[q];
+        const double f = 8.0 * n * (double)n * n * tot;
+        c->flops_pending[NT_K_INV] += f;
+        c->flops_pending[NT_K_GEMM] += 6.0 * f;
     }
     hipLaunchKernelGGL(k_dec_finish, dim3(nblk(Q)), dim3(256), 0, s, active, status, Q);
     hipcheck(c, hipGetLastError(), "dec_finish");
@@ -432,7 +448,7 @@ void decimate_circular(nt_ctx* c, const nt_params* prm, int n, int Q, int nm, co
                        reinterpret_cast<const double2*>(Vs), alpha_dagger, reinterpret_cast<double2*>(As), nm);
     hipLaunchKernelGGL(k_circ_items, dim3(nblk((long)Q * NB)), dim3(256), 0, s, idx, z, Q, NB, idx_c, z_c);
     hipcheck(c, hipGetLastError(), "circ forward");
-    hipcheck(c, hipMemsetAsync(st_c, 0, sizeof(int) * Q * NB, s), "memset");
+    zero_dev(c, st_c, sizeof(int) * Q * NB, s);
     DecimationOut<T> os{bw(gs, ns, nac), nonew<T>(), nonew<T>()};
     decimate<T>(c, nac, Q * NB, bm<T>(Hs, ns, nac, idx_c, 0, nm * NB), bm<T>(As, ns, nac, idx_c, 0, nm * NB), 0, z_c,
                 prm->convergence, prm->max_iter, os, st_c, it_c, ar, dec_off);
@@ -469,7 +485,7 @@ size_t decimate_scratch(int n, int Q)
     a.take(mb * Q);
     a.take(sizeof(int) * Q);
     a.take(sizeof(unsigned long long) * Q);
-    a.take(sizeof(int));
+    a.take(sizeof(int) * 2);
     return a.off;
 }
 
@@ -717,7 +733,7 @@ void adjoint_backward(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T*
         const int sg = (j - adj.lo) / K, off = (j - adj.lo) % K;
         return off == 0 ? at(adj.S0, sg) : at(adj.Seg0, off - 1);
     };
-    if (K > 1) hipcheck(c, hipMemsetAsync(adj.st, 0, sizeof(int) * P, c->stream), "memset");
+    if (K > 1) zero_dev(c, adj.st, sizeof(int) * P, c->stream);
     const int lo = adj.lo, hi = adj.hi, hi1 = adj_hi1(hi, L), hc = adj_hc(hi, L);
     T* A = reinterpret_cast<T*>(adj.A);
     hipStream_t s = c->stream;
@@ -824,10 +840,6 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     BMatW<T> gLo = out.G_L.base ? out.G_L : W(Gn, nl);
     BMatW<T> gRo = out.G_R.base ? out.G_R : W(Gn, nr);
     int* st = out.status;
-    if (c->overlap && c->aux) {
-        hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
-        hipcheck(c, hipStreamWaitEvent(c->aux, c->fork, 0), "wait fork");
-    }
 
     // adjoint storage slots of device layer i (null when layer i is not stored)
     const long slot = mm * P;
@@ -843,6 +855,12 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     // system's corner blocks (see one_sweep below); its buffers are free
     const bool one = std::is_same<T, double>::value && c->one_sweep && !in.keep && !in.zl &&
                      in.zlead && !in.Hrd_fwd.base && in.Hdv_back.empty() && nl + nr <= inv_max_n();
+    // the right-to-left sweep (two sweeps) runs on the auxiliary stream from
+    // here; with one sweep only the main stream is used
+    if (!one && c->overlap && c->aux) {
+        hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
+        hipcheck(c, hipStreamWaitEvent(c->aux, c->fork, 0), "wait fork");
+    }
     T* Gc = G_b;         // left-connected column G^L_{i,0}   (b x nl), ping-pong with Gcn
     T* Gcn = Gn_b;
     T* G00 = Aw_b;       // left-connected corner G^L_{00,i}  (nl x nl)
@@ -995,7 +1013,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         c->stream = c->aux;
         s = c->aux;
     }
-    hipcheck(c, hipMemsetAsync(st_b, 0, sizeof(int) * P, s), "memset");
+    zero_dev(c, st_b, sizeof(int) * P, s);
     st = st_b;
     gdrl = out.G_D_RL.base ? out.G_D_RL : W(GRL_b, nr);
     {
@@ -1454,9 +1472,9 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                                            ar.ptr<T>(o_pzl), ar.ptr<T>(o_pgam), ar.ptr<T>(o_pdgam), s);
             });
         }
-        hipcheck(c, hipMemsetAsync(stdec, 0, sizeof(int) * 2 * P, s), "memset");
-        hipcheck(c, hipMemsetAsync(strgf, 0, sizeof(int) * P, s), "memset");
-        hipcheck(c, hipMemsetAsync(stpm, 0, sizeof(int) * 2 * P, s), "memset");
+        zero_dev(c, stdec, sizeof(int) * 2 * P, s);
+        zero_dev(c, strgf, sizeof(int) * P, s);
+        zero_dev(c, stpm, sizeof(int) * 2 * P, s);
 
         // ---- surface Green's functions (FP64)
         if (same) {
@@ -1480,7 +1498,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                                pm ? zpms : nullptr, idx1);
             hipcheck(c, hipGetLastError(), "lead split");
             auto sides = [&](T* g, T* sg, T* gm, const T* zz, int adag, int* st2, int* it2) {
-                hipcheck(c, hipMemsetAsync(sts, 0, sizeof(int) * 2 * P, s), "memset");
+                zero_dev(c, sts, sizeof(int) * 2 * P, s);
                 DecimationOut<T> ol{bw(g, 2 * nmm, nl), sg ? bw(sg, 2 * nmm, nl) : nonew<T>(),
                                     gm ? bw(gm, 2 * nmm, nl) : nonew<T>()};
                 lead_gfs(c, prm, nl, P, a.nk, Haml, Hamlv, idx1, adag, zz, ol, sts, its, ar, o_scr);
@@ -1611,7 +1629,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             int* fb = ar.ptr<int>(o_fb);
             int* cnt = ar.ptr<int>(o_cnt);
             mixed_caroli(c, nl, nr, in, i32, zdev, zdev32, P, out, ar, o_scr, acc, cnt);
-            hipcheck(c, hipMemsetAsync(cnt + 6, 0, sizeof(int) * 2, s), "memset");
+            zero_dev(c, cnt + 6, sizeof(int) * 2, s);
             hipLaunchKernelGGL(k_reject_mask, dim3(nblk(P)), dim3(256), 0, s, acc, fb, cnt + 6, P);
             hipcheck(c, hipGetLastError(), "reject mask");
             int hc[8];
@@ -1687,7 +1705,7 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             outs.T_RL = sb->T_RL + p0;
             outs.status = strgf;
             outs.G_D_LR = outs.G_D_RL = outs.G_L = outs.G_R = nonew<T>();
-            hipcheck(c, hipMemsetAsync(strgf, 0, sizeof(int) * P, s), "memset");
+            zero_dev(c, strgf, sizeof(int) * P, s);
             rgf_run<T>(c, nl, nr, ins, zdev, P, outs, ar, o_scr);
             check_range(c, sb->T_LR + p0, sizeof(double) * P, "sibling T_LR");
             check_range(c, sb->T_RL + p0, sizeof(double) * P, "sibling T_RL");
@@ -1698,13 +1716,16 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                                sb->first_mask ? sb->first_mask + p0 : nullptr, sb->T_LR + p0, sb->T_RL + p0);
             hipcheck(c, hipGetLastError(), "sibling point status");
         }
-        // mean iterations (host side, for the flop model)
-        hit.resize(2 * P);
-        hipcheck(c, hipMemcpyAsync(hit.data(), itdec, sizeof(int) * 2 * P, hipMemcpyDeviceToHost, s), "memcpy");
-        hipcheck(c, hipStreamSynchronize(s), "sync");
-        for (int q = 0; q < 2 * P; ++q) iters_sum += hit[q];
+        // mean iterations (host side, for the flop model); not read when the
+        // solve must run without host synchronisation
+        if (c->dec_check > 0) {
+            hit.resize(2 * P);
+            hipcheck(c, hipMemcpyAsync(hit.data(), itdec, sizeof(int) * 2 * P, hipMemcpyDeviceToHost, s), "memcpy");
+            hipcheck(c, hipStreamSynchronize(s), "sync");
+            for (int q = 0; q < 2 * P; ++q) iters_sum += hit[q];
+        }
     }
-    c->last_mean_iters = iters_sum / std::max(1, 2 * a.npts);
+    c->last_mean_iters = c->dec_check > 0 ? iters_sum / std::max(1, 2 * a.npts) : -1.0;
 }
 
 int first_status_host(const int* st, int n)
@@ -2318,7 +2339,7 @@ void tune_inverse(nt_ctx* c, int n, int batch, int* best_nb, double* best_ms)
     DevBuf A(c, sizeof(T) * 2 * nn * batch), W(c, sizeof(T) * 2 * nn * batch), X(c, sizeof(T) * 2 * nn * batch),
         info(c, sizeof(int) * batch), z(c, sizeof(T) * 2 * batch);
     hipLaunchKernelGGL(k_fill_tune<T>, dim3(nblk(nn * batch)), dim3(256), 0, c->stream, A.as<T>(), n, batch);
-    hipcheck(c, hipMemsetAsync(z.p, 0, sizeof(T) * 2 * batch, c->stream), "memset");
+    zero_dev(c, z.p, sizeof(T) * 2 * batch, c->stream);
     hipcheck(c, hipGetLastError(), "fill tune");
     reg_region(c, A.p, sizeof(T) * 2 * nn * batch);
     reg_region(c, W.p, sizeof(T) * 2 * nn * batch);
@@ -2496,6 +2517,7 @@ int nt_profile_reset(nt_ctx* c)
         c->launches[k] = 0;
         c->ms[k] = 0;
         c->flops[k] = 0;
+        c->flops_pending[k] = 0;
     }
     for (int k = 0; k < 3; ++k) c->mixed_counts[k] = 0;
     return NT_OK;
@@ -2928,7 +2950,7 @@ int nt_surface_green_tdb(nt_ctx* c, const nt_params* p, int nl, int nr, double E
         const double zh[8] = {E, -et, E, et, E, et, E, -et};
         DevBuf dzz(c, sizeof(zh));
         up(c, dzz, zh, sizeof(zh));
-        hipcheck(c, hipMemsetAsync(dst.p, 0, sizeof(int) * 4, c->stream), "memset");
+        zero_dev(c, dst.p, sizeof(int) * 4, c->stream);
         const int nmax = std::max(nl, nr);
         const long nmm = (long)nmax * nmax;
         DevBuf dg(c, z * nmm * 4), dsig(c, z * nmm * 2), dgam(c, z * nmm * 2);
@@ -3041,7 +3063,7 @@ static int ldos_host(nt_ctx* c, const nt_params* p, int nl, int nr, int L, const
         const T* zd = dz.as<T>();
         const T* one = dz.as<T>() + 2;
         int* st = dst.as<int>();
-        hipcheck(c, hipMemsetAsync(st, 0, sizeof(int), c->stream), "memset");
+        zero_dev(c, st, sizeof(int), c->stream);
         auto W = [&](T* q, int ld) { return bw(q, 0, ld); };
         auto R = [&](T* q, int ld) { return bm<T>(q, 0, ld); };
         auto GLi = [&](int i) { return GL + 2 * (long)i * mm; };
@@ -3163,7 +3185,7 @@ static int rgf_host_common(nt_ctx* c, const nt_params* p, int first_point, int n
         const double et = p->eta_mode == NT_ETA_RELATIVE ? E * p->eta + p->eta0 : p->eta;
         const double zh[2] = {E, p->device_eta_factor * et};
         up(c, dz, zh, sizeof(zh));
-        hipcheck(c, hipMemsetAsync(dst.p, 0, sizeof(int), c->stream), "memset");
+        zero_dev(c, dst.p, sizeof(int), c->stream);
         DevBuf oLR(c, z * nl * nr), oRL(c, z * nl * nr), oL(c, z * nll), oR(c, z * nrr);
         RgfOutput<T> out;
         out.T_LR = dT.as<double>();
@@ -3305,6 +3327,13 @@ int nt_set_sweeps(nt_ctx* c, int n)
 {
     if (!c || (n != 1 && n != 2)) return NT_ERR_ARG;
     c->one_sweep = n == 1;
+    return NT_OK;
+}
+
+int nt_set_decimation_check(nt_ctx* c, int every)
+{
+    if (!c || every < 0) return NT_ERR_ARG;
+    c->dec_check = every;
     return NT_OK;
 }
 
@@ -3460,7 +3489,7 @@ static int modes_entry(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, in
             mo.t_mode_L = ptd(o.t_mode_L, nl);
             mo.t_mode_R = ptd(o.t_mode_R, nr);
             for (int q = 0; q < 4; ++q) mo.sums[q] = o.sums ? sum4 + q * P : nullptr;
-            hipcheck(c, hipMemsetAsync(stc, 0, sizeof(int) * P, c->stream), "memset");
+            zero_dev(c, stc, sizeof(int) * P, c->stream);
             UnfoldRun ur;
             if (uf) {
                 auto pti = [&](int* b, long per) { return b ? b + (long)p0 * per : nullptr; };

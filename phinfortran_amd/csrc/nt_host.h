@@ -46,7 +46,14 @@ struct nt_ctx {
     long long launches[NT_K_COUNT] = {};
     double ms[NT_K_COUNT] = {};
     double flops[NT_K_COUNT] = {};
+    // flops known only after the launches (decimation, from the per-item
+    // iteration counts), added to flops[] when the profile is harvested
+    double flops_pending[NT_K_COUNT] = {};
     double last_mean_iters = 0.0;
+    // decimation: the host reads the number of unconverged items every
+    // dec_check iterations and stops at zero; 0 = never (max_iter iterations,
+    // converged items masked: no host synchronisation inside a solve)
+    int dec_check = 4;
     // mixed precision since the last profile reset: points, second refinement
     // steps (both directions), points recomputed in FP64
     long long mixed_counts[3] = {0, 0, 0};
@@ -117,8 +124,7 @@ void launch(nt_ctx* c, int kind, double flops, const char* what, F&& f)
 
 inline void harvest_profile(nt_ctx* c)
 {
-    if (c->recs.empty()) return;
-    hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (!c->recs.empty()) hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     for (auto& r : c->recs) {
         float ms = 0.f;
         hipcheck(c, hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
@@ -129,6 +135,22 @@ inline void harvest_profile(nt_ctx* c)
         c->pool.push_back(r.b);
     }
     c->recs.clear();
+    for (int k = 0; k < NT_K_COUNT; ++k) {
+        c->flops[k] += c->flops_pending[k];
+        c->flops_pending[k] = 0.0;
+    }
+}
+
+// Zero a device buffer of 4-byte words with a fill kernel (a memset node in a
+// captured graph is not reliably replayed; a kernel node is).
+inline void zero_dev(nt_ctx* c, void* p, size_t bytes, hipStream_t s)
+{
+    if (!p || !bytes) return;
+    if (bytes % 4 || reinterpret_cast<uintptr_t>(p) % 4) {
+        c->err = "zero_dev: buffer not made of 4-byte words";
+        throw Fail{NT_ERR_ARG};
+    }
+    hipcheck(c, launch_fill_int(static_cast<int*>(p), 0, (int)(bytes / 4), s), "fill");
 }
 
 // Bump allocator over the context workspace.

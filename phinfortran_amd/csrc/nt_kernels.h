@@ -250,9 +250,12 @@ hipError_t launch_trace_im(int n, BMat<T> g, const double* E, double coef, doubl
 hipError_t launch_make_shifts(const double* E, int batch, double eta, double eta0, int eta_mode,
                               double dev_factor, double* zl, double* zr, double* zd, float* zl32,
                               float* zr32, float* zd32, hipStream_t s);
-// active[p] &= (maxabs[p] > conv); iters[p] += was_active; maxabs[p] = 0; counts active into *n_active
-hipError_t launch_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv,
-                                  int* n_active, int batch, hipStream_t s);
+// active[p] &= (maxabs[p] > conv); iters[p] += was_active (iters may be null);
+// maxabs[p] = 0; adds the number still active into *count and sets *clear = 0
+// (the two words of a two-slot counter: update u adds to slot u % 2 and clears
+// the other, so after any update its slot holds that update's count)
+hipError_t launch_dec_step_update(int* active, unsigned long long* maxabs, int* iters, double conv, int* count,
+                                  int* clear, int batch, hipStream_t s);
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
 // Final transmission clean-up: clamp < 1e-60 to 0, zero when flagged, status merge.
 hipError_t launch_finish_T(double* T, int n, int zero_all, hipStream_t s);

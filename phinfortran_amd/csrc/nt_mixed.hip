@@ -200,9 +200,9 @@ void refine_direction(nt_ctx* c, const Chain<double>& c64, const Chain<float>& c
     auto D = [&](double* base, int sl, int ld) { return bw<double>(base + 2 * (long)sl * mm * P, mm, ld); };
     auto Dr = [&](double* base, int sl, int ld) { return bm<double>(base + 2 * (long)sl * mm * P, mm, ld); };
     const auto& n = c64.n;
-    hipcheck(c, hipMemsetAsync(d.st32, 0, sizeof(int) * P, s), "memset");
-    hipcheck(c, hipMemsetAsync(d.dmax, 0, sizeof(unsigned long long) * P, s), "memset");
-    hipcheck(c, hipMemsetAsync(d.xmax, 0, sizeof(unsigned long long) * P, s), "memset");
+    zero_dev(c, d.st32, sizeof(int) * P, s);
+    zero_dev(c, d.dmax, sizeof(unsigned long long) * P, s);
+    zero_dev(c, d.xmax, sizeof(unsigned long long) * P, s);
 
     // ---- 1. FP32 factorisation
     for (int l = 0; l <= S; ++l) {
@@ -436,7 +436,7 @@ void mixed_caroli(nt_ctx* c, int nl, int nr, const RgfInput<double>& in, const R
         b64.Dm.push_back(Rd(dr.DmSd, nl));
         b32.Dm.push_back(Rf(dr.DmS, nl));
     }
-    hipcheck(c, hipMemsetAsync(counters, 0, sizeof(int) * 6, c->stream), "memset");
+    zero_dev(c, counters, sizeof(int) * 6, c->stream);
 
     // ---- both directions, on two streams when enabled
     hipStream_t main_stream = c->stream;

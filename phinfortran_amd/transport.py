@@ -570,6 +570,12 @@ class Engine:
         right-to-left quantities (default), 2 = both sweeps as the reference"""
         self._check(self._L.nt_set_sweeps(self.ctx, int(n)), "nt_set_sweeps")
 
+    def set_decimation_check(self, every: int):
+        """decimation: read the unconverged count every `every` iterations
+        (default 4); 0 = never (max_iter iterations, no host synchronisation
+        inside the FP64 batched solve, which can then be captured in a graph)"""
+        self._check(self._L.nt_set_decimation_check(self.ctx, int(every)), "nt_set_decimation_check")
+
     def set_adjoint_checkpoint(self, every: int):
         """adjoint storage: every layer (0), every n-th (n), about sqrt (-1)"""
         self._check(self._L.nt_set_adjoint_checkpoint(self.ctx, int(every)), "nt_set_adjoint_checkpoint")

@@ -86,6 +86,9 @@ def main():
                     help="skip the Buttiker-probe measurement")
     ap.add_argument("--buttiker-points", type=int, default=128, help="frequencies of the probe solve")
     ap.add_argument("--buttiker-cpu-sample", type=int, default=1, help="points of the probe CPU sample (0 = skip)")
+    ap.add_argument("--no-configs", dest="configs", action="store_false",
+                    help="skip the full-size wire (b=96, L=20, 512 omega, with modes) and ribbon (b=80, L=60, "
+                         "2048 E) legs")
     ap.add_argument("--no-tb-hk", dest="tb_hk", action="store_false",
                     help="skip the tight-binding H(k) assembly measurement")
     ap.add_argument("--tb-hk-only", action="store_true",
@@ -220,19 +223,29 @@ def main():
                         "&unfolding\n n_bloch_x = 1, n_bloch_y = 1, circular_flag = 0\n/\n"
                         "&kpath\n path_mode = 2\n/\n&saving\n output_dir = '', verbosity = 0\n/\n")
             prm = os.path.join(here, "tests", "golden", "tb", "param")
-            DRV.run_tb(os.path.join(tmp, "confige"), engine=eng, param_path=prm)        # warm-up
+            cfgp = os.path.join(tmp, "confige")
+            DRV.run_tb(cfgp, engine=eng, param_path=prm, dos=False)        # warm-up
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = DRV.run_tb(os.path.join(tmp, "confige"), engine=eng, param_path=prm)
+            r = DRV.run_tb(cfgp, engine=eng, param_path=prm, dos=False)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
+            DRV.run_tb(cfgp, engine=eng, param_path=prm)                   # warm-up, with the device DOS
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rd = DRV.run_tb(cfgp, engine=eng, param_path=prm)
+            torch.cuda.synchronize()
+            eld = time.perf_counter() - t0
         npt = r["transl"].size
         res = {"points_per_s": npt / el, "unit": "(E,k)-points/s", "points": npt, "seconds": el,
+               "with_device_dos_points_per_s": npt / eld,
                "norb": r["norb"], "nl": r["nl"], "layer_list": r["layer_list"],
-               "status_ok": bool((r["status"] == 0).all()),
+               "status_ok": bool((r["status"] == 0).all() and (rd["status"] == 0).all()),
                "what": "whole run: table, H(k) of 16 k-points on the GPU, blockwise_setup (host), device "
-                       "slicing, one batched solve (layers padded to the widest), FP64"}
-        log(f"tb run: {npt} points in {el:.2f}s ({npt / el:.1f} points/s), layers {r['layer_list']}")
+                       "slicing, one batched solve (layers padded to the widest), FP64; with the device DOS "
+                       "(get_ldos of every point, as the reference's loop) beside it"}
+        log(f"tb run: {npt} points in {el:.2f}s ({npt / el:.1f} points/s), layers {r['layer_list']}; "
+            f"with the device DOS {eld:.2f}s ({npt / eld:.1f} points/s)")
         if args.cpu_sample > 0:
             import oracle as O
             from oracle import tb_ham as OT
@@ -423,10 +436,20 @@ def main():
             devs.append(dict(blocks=eng.upload_blocks(blk), E=E, kidx=kidx))
             npts += E.size
         log(f"c3 sample: {len(devs)} devices built in {time.time() - t0:.1f}s")
+        tune = {}
+        if args.autotune:
+            # per block size: the inverse's panel width and the GEMM's output
+            # tile, timed on batches of one device's points
+            for bs in sorted({sp["b"] for sp in specs}):
+                for prec in ("fp64", "mixed"):
+                    nb, ims = eng.autotune_inverse(bs, len(devs[0]["E"]), prec)
+                    tb, gms = eng.autotune_gemm(bs, len(devs[0]["E"]), prec)
+                    tune[f"{bs}/{prec}"] = dict(inverse_panel=nb, inverse_ms=ims, gemm_tile=tb, gemm_ms=gms)
+                    log(f"c3 tuning b={bs} {prec}: inverse panel {nb} ({ims:.3f} ms), gemm tile {tb} ({gms:.3f} ms)")
         res = {}
         for prec in ("fp64", "mixed"):
             p = T.params("phonon", precision=prec)
-            eng.solve_tdb_multi(devs, p)                      # warm-up (tunes the inverse per block size)
+            eng.solve_tdb_multi(devs, p)                      # warm-up
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             outs = eng.solve_tdb_multi(devs, p)
@@ -452,6 +475,7 @@ def main():
                 "points": npts, "fp64_solves_per_s": res["fp64"][0], "mixed_solves_per_s": res["mixed"][0],
                 "fp64_solves_per_s_per_block": per_b,
                 "mixed_max_rel_dT_per_device": errs, "status_ok": res["fp64"][2] and res["mixed"][2],
+                "tuning": tune,
                 "what": "Si/Ge superlattice films, 64 frequencies x 2x2 k per device, devices of one block size "
                         "sharing their leads' Green's functions"}
 
@@ -518,6 +542,49 @@ def main():
         return res
 
     modes = run_modes() if args.modes else None
+
+    def run_configs():
+        """the Si/Ge alloy wire (2x2 cells, b = 96, L = 20, 512 frequencies,
+        Caroli and mode-resolved T) and the armchair ribbon (40 dimers, b = 80,
+        L = 60, 2048 energies, eta = 1e-8 eV) at their full sizes, FP64, one
+        batched call each after a warm-up; the GEMM tile and inverse panel of
+        each block size from the autotuners"""
+        out = {}
+        wire = lt.si_ge_wire(nx=2, ny=2, L=20, seed=0)
+        rib = lt.agnr_blocks(n_dimer=40, L=60, seed=0)
+        cases = [("wire_b96_L20", wire, (16.0 * np.arange(1, 513) / 512) ** 2,
+                  T.params("phonon", eta=1e-6, eta0=1e-6 * 16.0)),
+                 ("ribbon_b80_L60", rib, np.linspace(-3.0, 3.0, 2048), T.params("tb", eta=1e-8))]
+        for name, blk, E, p in cases:
+            tune = None
+            if args.autotune:
+                nb, _ = eng.autotune_inverse(blk.b, len(E), "fp64")
+                tb, _ = eng.autotune_gemm(blk.b, len(E), "fp64")
+                tune = dict(inverse_panel=nb, gemm_tile=tb)
+            d = eng.upload_blocks(blk)
+            Ed = torch.as_tensor(E, dtype=torch.float64, device="cuda")
+            eng.solve_tdb(d, Ed, p)                              # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = eng.solve_tdb(d, Ed, p)
+            el = time.perf_counter() - t0
+            ok = bool((r["status"] == 0).all())
+            res = {"b": blk.b, "L": blk.L, "points": len(E), "solves_per_s": len(E) / el, "seconds": el,
+                   "mean_decimation_iters": eng.last_mean_iters(), "status_ok": ok, "tuning": tune}
+            log(f"{name}: {len(E)} points in {el * 1e3:.1f} ms ({len(E) / el:.1f} solves/s), tuning {tune}")
+            if name.startswith("wire"):
+                eng.solve_modes(d, Ed, p, lt.A_SI, lt.A_SI)     # warm-up
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                m = eng.solve_modes(d, Ed, p, lt.A_SI, lt.A_SI)
+                elm = time.perf_counter() - t0
+                res["modes_points_per_s"] = len(E) / elm
+                res["modes_status_ok"] = bool((m["modes_status"] == 0).all())
+                log(f"{name} mode-resolved: {len(E)} points in {elm:.2f} s ({len(E) / elm:.1f} points/s)")
+            out[name] = res
+        return out
+
+    configs = run_configs() if args.configs else None
 
     def run_buttiker():
         """probe solve (both sweeps with a probe in every layer, every block of
@@ -678,6 +745,8 @@ def main():
         out["uneven_batching"] = c3
     if modes is not None:
         out["mode_resolved"] = modes
+    if configs is not None:
+        out["workloads"] = configs
     if probes is not None:
         out["buttiker"] = probes
     if tbhk is not None:

@@ -33,6 +33,7 @@ module negf_transport
         type(c_ptr) :: gml_adv = c_null_ptr, gmr_ret = c_null_ptr, sl_adv = c_null_ptr, sr_ret = c_null_ptr
         type(c_ptr) :: G_D_LR = c_null_ptr, G_D_RL = c_null_ptr, G_L = c_null_ptr, G_R = c_null_ptr
         type(c_ptr) :: sdos_l = c_null_ptr, sdos_r = c_null_ptr
+        type(c_ptr) :: ldos = c_null_ptr, ddos = c_null_ptr
     end type nt_solve_extras
 
     type, bind(C) :: nt_modes_out

@@ -209,6 +209,12 @@ typedef struct {
     nt_c128* G_D_LR;  nt_c128* G_D_RL;  nt_c128* G_L;     nt_c128* G_R;
     double* sdos_l;   double* sdos_r;   /* surface_ldos_mpi: +-2 sqrt(E) Im Tr g / pi (negf/surface.f90:608-630);
                                            with eta_mode = NT_ETA_ABSOLUTE (tb) +-Im Tr g / pi (tb/surface.f90:345-366) */
+    /* get_ldos (tb/device.f90:610-855): ldos npts x L x b (point, layer,
+     * orbital; -Im G_ii(j,j) / pi of the full device G at the device shift),
+     * ddos npts (its sum).  Either one makes the solve keep every layer's
+     * left-connected G and run both sweeps in FP64 (also with
+     * NT_PREC_MIXED); not with the probe entry. */
+    double* ldos;     double* ddos;
 } nt_solve_extras;
 
 /* Device-pointer variant: every array is device memory (e.g. torch tensors). */
@@ -529,6 +535,16 @@ int nt_hk_assemble_orb_dev(nt_ctx* ctx, int nk, const double* kcart, int nseg, c
  * width directly (0 restores the default).  No reference counterpart. */
 int nt_autotune_inverse(nt_ctx* ctx, int n, int batch, int precision, int* best_nb, double* best_ms);
 int nt_set_inverse_panel(nt_ctx* ctx, int n, int precision, int nb);
+/* Output tile of the batched complex GEMM: 64x64 per 256-thread workgroup
+ * (default) or 32x32 per one-wave workgroup, which pads less when the matrix
+ * size is not a multiple of 64 (24, 80, 96, 216, ...).  nt_autotune_gemm
+ * times the n x n x n product of `batch` items with both and keeps the faster
+ * for every GEMM of this context whose larger output dimension is n
+ * (precision NT_PREC_FP64: complex double, NT_PREC_MIXED: complex float);
+ * best_tile / best_ms may be NULL.  nt_set_gemm_tile sets it directly (0
+ * restores the default).  No reference counterpart. */
+int nt_autotune_gemm(nt_ctx* ctx, int n, int batch, int precision, int* best_tile, double* best_ms);
+int nt_set_gemm_tile(nt_ctx* ctx, int n, int precision, int tile);
 /* on = 1 (default): the right-to-left RGF sweep runs on a second stream of
  * the context beside the left-to-right one; on = 0: both on the main stream. */
 int nt_set_overlap(nt_ctx* ctx, int on);

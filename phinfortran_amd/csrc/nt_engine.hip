@@ -878,6 +878,14 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
         if (L > 1 && keep(0)) G = keep(0);
         gemm<T>(c, b, nl, nl, OP_C, in.Hld, OP_N, in.gl, W(T1, b), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         gemm<T>(c, b, b, nl, OP_N, R(T1, b), OP_N, in.Hld, W(Aw, b), -1.0, in.Hd[0], -1.0, zdev(0), P, -1, in.act);
+        if (L == 1 && keep(0)) {
+            // one layer: its full G, closed by the right lead's surface
+            // function, inv(Aw - Hrd^+ gr Hrd) (T1 is free: no corner column
+            // is formed when blocks are kept)
+            gemm<T>(c, b, nr, nr, OP_C, in.Hrd, OP_N, in.gr, W(T1, b), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, b, b, nr, OP_N, R(T1, b), OP_N, in.Hrd, W(X, b), -1.0, R(Aw, b), 1.0, nullptr, P, -1, in.act);
+            inverse<T>(c, b, W(X, b), none<T>(), nullptr, W(keep(0), b), st, P, -1, in.act);
+        }
         inverse<T>(c, b, W(Aw, b), none<T>(), nullptr, W(G, b), st, P, -1, in.act);
         gemm<T>(c, nl, b, nl, OP_N, in.gl, OP_N, in.Hld, W(T2, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
         gemm<T>(c, nl, b, b, OP_N, R(T2, nl), OP_N, R(G, b), W(G0, nl), 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
@@ -1078,6 +1086,77 @@ struct ProbeSpec {
 inline long probe_acc_n(int L) { return (long)L * L + 5 * (long)L + 6; }
 // slots (of the RGF slot size, per point) buttiker_run uses after the sweep
 inline long probe_slots(int L) { return 7 * (long)L + 2; }
+
+// Device DOS: kept blocks (L) + P_i, Q_i (L - 1 each) + one product slot.
+inline long ldos_slots(int L) { return 3 * (long)L; }
+
+// ldos[p][i][j] = -Im G_ii(j, j) / pi of every layer i (tb/device.f90:845-850)
+__global__ void k_ldos_diag(const double2* __restrict__ G, long mm, int P, int L, int b, double* __restrict__ ldos)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)P * L * b) return;
+    const int j = (int)(e % b);
+    const long pi = e / b;
+    const int i = (int)(pi % L), p = (int)(pi / L);
+    const double2 g = G[((long)i * P + p) * mm + (long)j * b + j];
+    ldos[e] = -g.y / 3.14159265358979323846;
+}
+
+// ddos[p] = sum of the point's L b entries in index order (the reference's
+// sum(sum(ldos, dim=1)) of tb/device.f90:853 up to the order of the sum)
+__global__ void k_ldos_sum(const double* __restrict__ ldos, int P, long n, double* __restrict__ ddos)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double acc = 0.0;
+    for (long e = 0; e < n; ++e) acc += ldos[(long)p * n + e];
+    ddos[p] = acc;
+}
+
+// Layer-resolved device DOS of get_ldos (tb/device.f90:610-855) for a batch.
+// On entry keep[i] holds the left-connected G^L_i of layers i < L - 1 and the
+// full G of the last layer (rgf_run with in.keep).  The backward recursion
+// G_ii = G^L_i + (G^L_i V_i) G_{i+1,i+1} (V_i^+ G^L_i) replaces them by the
+// diagonal blocks of the full G; it equals the reference's
+// (I - G^L_i V_i G^R_{i+1} V_i^+)^-1 G^L_i with G^R from its right-to-left
+// sweep (both are the (i, i) block of the same inverse).
+void ldos_run(nt_ctx* c, const RgfInput<double>& in, int P, long mm, double* work, double* ldos, double* ddos)
+{
+    typedef double T;
+    const int L = in.L, b = in.bsz[0];
+    const long slot = mm * P, bb = (long)b * b;
+    T* keep = in.keep;
+    auto at = [&](T* base, long k) { return base + 2 * k * slot; };
+    auto R = [&](T* p) { return bm<T>(p, mm, b); };
+    auto W = [&](T* p) { return bw(p, mm, b); };
+    auto RG = [&](T* p) { return bm<T>(p, mm, b, nullptr, slot); };
+    auto WG = [&](T* p) { return bw(p, mm, b, nullptr, slot); };
+    hipStream_t s = c->stream;
+    if (L > 1) {
+        T* Pm = work;
+        T* Qm = at(work, L - 1);
+        T* Dn = at(work, 2 * (L - 1));
+        BMat<T> Vg = in.Hdv[0];
+        Vg.gstride = bb;      // grouped over layers
+        gemm<T>(c, b, b, b, OP_N, RG(keep), OP_N, Vg, WG(Pm), 1.0, none<T>(), 0.0, nullptr, P, -1, nullptr, L - 1);
+        gemm<T>(c, b, b, b, OP_C, Vg, OP_N, RG(keep), WG(Qm), 1.0, none<T>(), 0.0, nullptr, P, -1, nullptr, L - 1);
+        for (int i = L - 2; i >= 0; --i) {
+            gemm<T>(c, b, b, b, OP_N, R(at(keep, i + 1)), OP_N, R(at(Qm, i)), W(Dn), 1.0, none<T>(), 0.0, nullptr, P);
+            gemm<T>(c, b, b, b, OP_N, R(at(Pm, i)), OP_N, R(Dn), W(at(keep, i)), 1.0, R(at(keep, i)), 1.0, nullptr,
+                    P);
+        }
+    }
+    // ldos goes to the caller's buffer, or (ddos alone) to the first work slot
+    double* out = ldos ? ldos : work;
+    const long n = (long)L * b;
+    if (ldos) check_range(c, ldos, sizeof(double) * n * P, "ldos");
+    if (ddos) check_range(c, ddos, sizeof(double) * P, "ddos");
+    audit<T>(c, RG(keep), b, b, P, L, "ldos G");
+    hipLaunchKernelGGL(k_ldos_diag, dim3(nblk(n * P)), dim3(256), 0, s, reinterpret_cast<const double2*>(keep), mm, P,
+                       L, b, out);
+    if (ddos) hipLaunchKernelGGL(k_ldos_sum, dim3(nblk(P)), dim3(256), 0, s, out, P, n, ddos);
+    hipcheck(c, hipGetLastError(), "ldos");
+}
 
 // From the kept blocks of the left-to-right sweep (left-connected G^L_i of
 // layers 0..L-2, full G_{N,N} of the last layer) build every block of the full
@@ -1302,8 +1381,15 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         const double per_pt = (double)(probe_slots(L) * mmr + probe_acc_n(L) + 2 * L) * sizeof(nt_c128);
         Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
     }
+    // device DOS: every layer's G^L kept, then the diagonal blocks of G
+    const bool dos = a.ex.ldos || a.ex.ddos;
+    if (dos) {
+        const double per_pt = (double)ldos_slots(L) * mmr * sizeof(nt_c128);
+        Pmax = std::max(1, std::min(Pmax, (int)std::min(1.0e9, ADJ_BUDGET_BYTES / per_pt)));
+    }
 
-    const bool mixed = prm->precision == NT_PREC_MIXED && !probe;   // the probe solve runs in FP64
+    // the probe solve and the device DOS run in FP64
+    const bool mixed = prm->precision == NT_PREC_MIXED && !probe && !dos;
     const size_t c32 = 2 * sizeof(float);
     const long Ldv = std::max(L - 1, 1);
     const bool sweep32 = mixed && !adjoint;   // the adjoint runs the FP64 sweep
@@ -1377,6 +1463,9 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             o_pacc = pa.take(sizeof(double) * 2 * probe_acc_n(L) * Pmax);
             o_pkeep = pa.take(sizeof(nt_c128) * mmr * L * Pmax);
             o_pwork = pa.take(sizeof(nt_c128) * mmr * (probe_slots(L) - L) * Pmax);
+        } else if (dos) {
+            o_pkeep = pa.take(sizeof(nt_c128) * mmr * L * Pmax);
+            o_pwork = pa.take(sizeof(nt_c128) * mmr * (ldos_slots(L) - L) * Pmax);
         }
         if (!a.noalloc || pa.off <= NT_SCRATCH_BYTES) break;
         if (Pmax == 1) {
@@ -1542,6 +1631,8 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
         if (probe) {
             in.zl = ar.ptr<T>(o_pzl);
             in.keep = ar.ptr<T>(o_pkeep);
+        } else if (dos) {
+            in.keep = ar.ptr<T>(o_pkeep);
         }
         RgfOutput<T> out;
         out.T_LR = a.T_LR + p0;
@@ -1562,6 +1653,9 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
                 adj = adj_carve(aa, a.adj_lo, a.adj_hi, L, adjK, mmr, P, a.adj_w + p0);
             }
             rgf_run<T>(c, nl, nr, in, zdev, P, out, ar, o_scr, adjoint ? &adj : nullptr);
+            if (dos)
+                ldos_run(c, in, P, mmr, ar.ptr<T>(o_pwork), a.ex.ldos ? a.ex.ldos + (long)p0 * L * b : nullptr,
+                         a.ex.ddos ? a.ex.ddos + p0 : nullptr);
             if (probe) {
                 buttiker_run(c, nl, nr, in, P, mmr, ar.ptr<T>(o_pwork), ar.ptr<T>(o_pacc));
                 ProbeCombineArgs pc;
@@ -2382,6 +2476,54 @@ void tune_inverse(nt_ctx* c, int n, int batch, int* best_nb, double* best_ms)
     if (best_ms) *best_ms = best;
 }
 
+// Time the n x n x n GEMM of `batch` items with each output tile and keep
+// the fastest for GEMMs whose larger output dimension is n.
+template <typename T>
+void tune_gemm(nt_ctx* c, int n, int batch, int* best_tile, double* best_ms)
+{
+    const long nn = (long)n * n;
+    DevBuf A(c, sizeof(T) * 2 * nn * batch), B(c, sizeof(T) * 2 * nn * batch), X(c, sizeof(T) * 2 * nn * batch);
+    hipLaunchKernelGGL(k_fill_tune<T>, dim3(nblk(nn * batch)), dim3(256), 0, c->stream, A.as<T>(), n, batch);
+    hipLaunchKernelGGL(k_fill_tune<T>, dim3(nblk(nn * batch)), dim3(256), 0, c->stream, B.as<T>(), n, batch);
+    hipcheck(c, hipGetLastError(), "fill tune");
+    reg_region(c, A.p, sizeof(T) * 2 * nn * batch);
+    reg_region(c, B.p, sizeof(T) * 2 * nn * batch);
+    reg_region(c, X.p, sizeof(T) * 2 * nn * batch);
+    const int tiles[2] = {64, 32};
+    hipEvent_t e0, e1;
+    hipcheck(c, hipEventCreate(&e0), "event");
+    hipcheck(c, hipEventCreate(&e1), "event");
+    double best = 1e300;
+    int bt = 64;
+    const bool prof = c->prof;
+    c->prof = false;
+    for (int tb : tiles) {
+        c->gemm_tile[{(int)sizeof(T), n}] = tb;
+        auto run = [&] {
+            gemm<T>(c, n, n, n, OP_N, bm<T>(A.as<T>(), nn, n), OP_N, bm<T>(B.as<T>(), nn, n), bw(X.as<T>(), nn, n),
+                    1.0, none<T>(), 0.0, nullptr, batch);
+        };
+        run();
+        hipcheck(c, hipEventRecord(e0, c->stream), "event");
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) run();
+        hipcheck(c, hipEventRecord(e1, c->stream), "event");
+        hipcheck(c, hipEventSynchronize(e1), "event sync");
+        float ms = 0.f;
+        hipcheck(c, hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (ms / reps < best) {
+            best = ms / reps;
+            bt = tb;
+        }
+    }
+    c->prof = prof;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->gemm_tile[{(int)sizeof(T), n}] = bt;
+    if (best_tile) *best_tile = bt;
+    if (best_ms) *best_ms = best;
+}
+
 extern "C" {
 
 int nt_params_validate(const nt_params* p, int nl, int nr)
@@ -2593,6 +2735,12 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
         reg_region(c, ex.G_R, z * nrr * npts);
         reg_region(c, ex.sdos_l, sizeof(double) * npts);
         reg_region(c, ex.sdos_r, sizeof(double) * npts);
+        reg_region(c, ex.ldos, sizeof(double) * npts * L * b);
+        reg_region(c, ex.ddos, sizeof(double) * npts);
+        if ((ex.ldos || ex.ddos) && pd) {
+            c->err = "the device DOS (ldos, ddos) is not produced by the probe solve";
+            return (int)NT_ERR_ARG;
+        }
         if (kidx) {
             // every k index must name one of the nk block sets
             std::vector<int> hk(npts);
@@ -2894,6 +3042,10 @@ int nt_solve_tdb_batched(nt_ctx* c, const nt_params* p, int nl, int nr, int b, i
         DevBuf dsl(c, hx.sdos_l ? sizeof(double) * npts : 0), dsr(c, hx.sdos_r ? sizeof(double) * npts : 0);
         ex.sdos_l = hx.sdos_l ? dsl.as<double>() : nullptr;
         ex.sdos_r = hx.sdos_r ? dsr.as<double>() : nullptr;
+        const size_t nld = (size_t)L * b * npts;
+        DevBuf dld(c, hx.ldos ? sizeof(double) * nld : 0), ddd(c, hx.ddos ? sizeof(double) * npts : 0);
+        ex.ldos = hx.ldos ? dld.as<double>() : nullptr;
+        ex.ddos = hx.ddos ? ddd.as<double>() : nullptr;
         int rc = 0;
         try {
             TdbArgs a{nl, nr, b, L, npts, nk, dE.as<double>(), kidx ? dk.as<int>() : nullptr,
@@ -2913,6 +3065,8 @@ int nt_solve_tdb_batched(nt_ctx* c, const nt_params* p, int nl, int nr, int b, i
             }
             if (hx.sdos_l) down(c, hx.sdos_l, dsl, sizeof(double) * npts);
             if (hx.sdos_r) down(c, hx.sdos_r, dsr, sizeof(double) * npts);
+            if (hx.ldos) down(c, hx.ldos, dld, sizeof(double) * nld);
+            if (hx.ddos) down(c, hx.ddos, ddd, sizeof(double) * npts);
             hipcheck(c, hipStreamSynchronize(c->stream), "sync");
             if (status) std::memcpy(status, hst.data(), sizeof(int) * npts);
             rc = first_status_host(hst.data(), npts);
@@ -3299,6 +3453,31 @@ int nt_autotune_inverse(nt_ctx* c, int n, int batch, int precision, int* best_nb
             tune_inverse<float>(c, n, batch, best_nb, best_ms);
         return 0;
     });
+}
+
+int nt_autotune_gemm(nt_ctx* c, int n, int batch, int precision, int* best_tile, double* best_ms)
+{
+    if (!c || n <= 0 || batch <= 0) return NT_ERR_ARG;
+    if (precision != NT_PREC_FP64 && precision != NT_PREC_MIXED) return NT_ERR_ARG;
+    return guarded(c, [&] {
+        if (precision == NT_PREC_FP64)
+            tune_gemm<double>(c, n, batch, best_tile, best_ms);
+        else
+            tune_gemm<float>(c, n, batch, best_tile, best_ms);
+        return 0;
+    });
+}
+
+int nt_set_gemm_tile(nt_ctx* c, int n, int precision, int tile)
+{
+    if (!c || n <= 0 || (tile != 0 && tile != 32 && tile != 64)) return NT_ERR_ARG;
+    if (precision != NT_PREC_FP64 && precision != NT_PREC_MIXED) return NT_ERR_ARG;
+    const int bytes = precision == NT_PREC_FP64 ? 8 : 4;
+    if (tile == 0)
+        c->gemm_tile.erase({bytes, n});
+    else
+        c->gemm_tile[{bytes, n}] = tile;
+    return NT_OK;
 }
 
 int nt_set_gemm_3m(nt_ctx* c, int on)

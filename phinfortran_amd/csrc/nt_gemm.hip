@@ -1,9 +1,12 @@
 // nt_gemm.hip -- batched complex GEMM on CDNA4 MFMA (FP64 and FP32).
 //
-// One workgroup (4 wave64) computes a 64x64 complex tile of one batch item.
-// Operand tiles of depth 8 are staged through two LDS buffers as separate
-// real and imaginary planes (one barrier per stage); each wave owns a 32x32 quadrant as 2x2 MFMA 16x16 tiles
-// and issues four real MFMAs per complex 16x16x4 step.
+// One workgroup computes a TB x TB complex tile of one batch item: TB = 64
+// with 4 wave64 (the default), or TB = 32 with one wave64 (less padding when
+// the matrix size is not a multiple of 64: 24, 96, 216, ...; chosen per size
+// by the autotuner).  Operand tiles of depth 8 are staged through two LDS
+// buffers as separate real and imaginary planes (one barrier per stage); each
+// wave owns a 32x32 quadrant as 2x2 MFMA 16x16 tiles and issues four real
+// MFMAs per complex 16x16x4 step.
 //
 // The product is formed transposed (C^T = B^T A^T): the MFMA output layout
 // then maps 16 consecutive lanes to 16 consecutive rows of the column-major C,
@@ -14,37 +17,40 @@ namespace nt {
 
 namespace {
 
-constexpr int BM = 64;
-constexpr int BN = 64;
 constexpr int BK = 8;         // operand depth per LDS stage
-constexpr int NT = 256;
-constexpr int EPT = BM * BK / NT;   // operand elements each thread stages per tile and stage
 constexpr int NXCD = 8;       // accelerator complex dies of the MI355X
+
+// threads of a TB x TB tile's workgroup and operand elements each stages per stage
+template <int TB> struct Tile {
+    static constexpr int NT = TB == 64 ? 256 : 64;
+    static constexpr int EPT = TB * BK / NT;
+};
 
 // two stages of operand tiles: the MFMAs of one read the buffer the other's
 // loads are not writing, so each K-stage needs a single barrier
-template <typename T>
+template <typename T, int TB>
 struct GemmSmem {
-    T ar[2][BK][BM];
-    T ai[2][BK][BM];
-    T br[2][BK][BN];
-    T bi[2][BK][BN];
+    T ar[2][BK][TB];
+    T ai[2][BK][TB];
+    T br[2][BK][TB];
+    T bi[2][BK][TB];
 };
 
-// Element j of thread t.  op N (A) / op C (B): row r = t % 64 (contiguous in
-// memory), depth k = t / 64 + 4 j.  op C (A) / op N (B): depth k = t % BK
-// (contiguous), row r = t / BK + (NT / BK) j.
-template <typename T>
+// Element j of thread t.  op N (A) / op C (B): row r = t % TB (contiguous in
+// memory), depth k = t / TB + (NT / TB) j.  op C (A) / op N (B): depth
+// k = t % BK (contiguous), row r = t / BK + (NT / BK) j.
+template <typename T, int TB>
 __device__ __forceinline__ void load_a_regs(const T* __restrict__ A, int lda, int opA, int M, int K,
-                                            int m0, int k0, T (&r)[EPT], T (&i)[EPT])
+                                            int m0, int k0, T (&r)[Tile<TB>::EPT], T (&i)[Tile<TB>::EPT])
 {
+    constexpr int NT = Tile<TB>::NT, EPT = Tile<TB>::EPT;
     const int t = threadIdx.x;
     typedef typename vec2<T>::type V2;
     if (opA == OP_N) {
-        const int m = t & 63, kq = t >> 6;
+        const int m = t % TB, kq = t / TB;
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            const int k = kq + 4 * j;
+            const int k = kq + (NT / TB) * j;
             const int gm = m0 + m, gk = k0 + k;
             V2 v = {0, 0};
             if (gm < M && gk < K) v = *reinterpret_cast<const V2*>(A + 2 * ((long)gm + (long)gk * lda));
@@ -65,17 +71,18 @@ __device__ __forceinline__ void load_a_regs(const T* __restrict__ A, int lda, in
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void store_a_smem(GemmSmem<T>& s, int buf, int opA, const T (&r)[EPT],
-                                             const T (&i)[EPT])
+template <typename T, int TB>
+__device__ __forceinline__ void store_a_smem(GemmSmem<T, TB>& s, int buf, int opA, const T (&r)[Tile<TB>::EPT],
+                                             const T (&i)[Tile<TB>::EPT])
 {
+    constexpr int NT = Tile<TB>::NT, EPT = Tile<TB>::EPT;
     const int t = threadIdx.x;
     if (opA == OP_N) {
-        const int m = t & 63, kq = t >> 6;
+        const int m = t % TB, kq = t / TB;
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            s.ar[buf][kq + 4 * j][m] = r[j];
-            s.ai[buf][kq + 4 * j][m] = i[j];
+            s.ar[buf][kq + (NT / TB) * j][m] = r[j];
+            s.ai[buf][kq + (NT / TB) * j][m] = i[j];
         }
     } else {
         const int k = t % BK, mq = t / BK;
@@ -87,10 +94,11 @@ __device__ __forceinline__ void store_a_smem(GemmSmem<T>& s, int buf, int opA, c
     }
 }
 
-template <typename T>
+template <typename T, int TB>
 __device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, int opB, int K, int N,
-                                            int k0, int n0, T (&r)[EPT], T (&i)[EPT])
+                                            int k0, int n0, T (&r)[Tile<TB>::EPT], T (&i)[Tile<TB>::EPT])
 {
+    constexpr int NT = Tile<TB>::NT, EPT = Tile<TB>::EPT;
     const int t = threadIdx.x;
     typedef typename vec2<T>::type V2;
     if (opB == OP_N) {
@@ -105,10 +113,10 @@ __device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, in
             i[j] = v.y;
         }
     } else {
-        const int n = t & 63, kq = t >> 6;
+        const int n = t % TB, kq = t / TB;
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            const int k = kq + 4 * j;
+            const int k = kq + (NT / TB) * j;
             const int gk = k0 + k, gn = n0 + n;
             V2 v = {0, 0};
             if (gk < K && gn < N) v = *reinterpret_cast<const V2*>(B + 2 * ((long)gn + (long)gk * ldb));
@@ -118,10 +126,11 @@ __device__ __forceinline__ void load_b_regs(const T* __restrict__ B, int ldb, in
     }
 }
 
-template <typename T>
-__device__ __forceinline__ void store_b_smem(GemmSmem<T>& s, int buf, int opB, const T (&r)[EPT],
-                                             const T (&i)[EPT])
+template <typename T, int TB>
+__device__ __forceinline__ void store_b_smem(GemmSmem<T, TB>& s, int buf, int opB, const T (&r)[Tile<TB>::EPT],
+                                             const T (&i)[Tile<TB>::EPT])
 {
+    constexpr int NT = Tile<TB>::NT, EPT = Tile<TB>::EPT;
     const int t = threadIdx.x;
     if (opB == OP_N) {
         const int k = t % BK, nq = t / BK;
@@ -131,11 +140,11 @@ __device__ __forceinline__ void store_b_smem(GemmSmem<T>& s, int buf, int opB, c
             s.bi[buf][k][nq + (NT / BK) * j] = i[j];
         }
     } else {
-        const int n = t & 63, kq = t >> 6;
+        const int n = t % TB, kq = t / TB;
 #pragma unroll
         for (int j = 0; j < EPT; ++j) {
-            s.br[buf][kq + 4 * j][n] = r[j];
-            s.bi[buf][kq + 4 * j][n] = i[j];
+            s.br[buf][kq + (NT / TB) * j][n] = r[j];
+            s.bi[buf][kq + (NT / TB) * j][n] = i[j];
         }
     }
 }
@@ -147,9 +156,13 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, doub
 
 // M3: complex products by three real products (P1 = Ar Br, P2 = Ai Bi,
 // P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2) instead of four.
-template <typename T, bool M3>
-__global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
+// Occupancy: three (M3: two) 256-thread workgroups per CU, the same number of
+// waves per SIMD with one-wave workgroups.
+template <typename T, bool M3, int TB>
+__global__ __launch_bounds__(Tile<TB>::NT, (M3 ? 2 : 3) * (256 / Tile<TB>::NT)) void k_gemm(GemmArgs<T> a)
 {
+    constexpr int EPT = Tile<TB>::EPT;
+    constexpr int BM = TB, BN = TB;
     typedef typename acc4<T>::type A4;
     const int tiles_m = (a.M + BM - 1) / BM;
     const int tiles_n = (a.N + BN - 1) / BN;
@@ -171,12 +184,12 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
     const int m0 = (tile % tiles_m) * BM;
     const int n0 = (tile / tiles_m) * BN;
 
-    __shared__ GemmSmem<T> s;
+    __shared__ GemmSmem<T, TB> s;
     const T* __restrict__ A = bptr(a.A, p, g);
     const T* __restrict__ B = bptr(a.B, p, g);
 
     const int lane = lane_id();
-    const int wave = threadIdx.x >> 6;
+    const int wave = threadIdx.x >> 6;       // 0 when TB = 32
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
 
@@ -192,17 +205,17 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
 
     T ra[EPT], ia[EPT], rb[EPT], ib[EPT];
     const int nst = (a.K + BK - 1) / BK;
-    load_a_regs(A, a.A.ld, a.opA, a.M, a.K, m0, 0, ra, ia);
-    load_b_regs(B, a.B.ld, a.opB, a.K, a.N, 0, n0, rb, ib);
-    store_a_smem(s, 0, a.opA, ra, ia);
-    store_b_smem(s, 0, a.opB, rb, ib);
+    load_a_regs<T, TB>(A, a.A.ld, a.opA, a.M, a.K, m0, 0, ra, ia);
+    load_b_regs<T, TB>(B, a.B.ld, a.opB, a.K, a.N, 0, n0, rb, ib);
+    store_a_smem<T, TB>(s, 0, a.opA, ra, ia);
+    store_b_smem<T, TB>(s, 0, a.opB, rb, ib);
     __syncthreads();
     for (int st = 0; st < nst; ++st) {
         const int cur = st & 1;
         const bool more = st + 1 < nst;
         if (more) {
-            load_a_regs(A, a.A.ld, a.opA, a.M, a.K, m0, (st + 1) * BK, ra, ia);
-            load_b_regs(B, a.B.ld, a.opB, a.K, a.N, (st + 1) * BK, n0, rb, ib);
+            load_a_regs<T, TB>(A, a.A.ld, a.opA, a.M, a.K, m0, (st + 1) * BK, ra, ia);
+            load_b_regs<T, TB>(B, a.B.ld, a.opB, a.K, a.N, (st + 1) * BK, n0, rb, ib);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
@@ -248,8 +261,8 @@ __global__ __launch_bounds__(NT, (M3 ? 2 : 3)) void k_gemm(GemmArgs<T> a)
         }
         // the other buffer was last read in the previous stage, before its barrier
         if (more) {
-            store_a_smem(s, cur ^ 1, a.opA, ra, ia);
-            store_b_smem(s, cur ^ 1, a.opB, rb, ib);
+            store_a_smem<T, TB>(s, cur ^ 1, a.opA, ra, ia);
+            store_b_smem<T, TB>(s, cur ^ 1, a.opB, rb, ib);
         }
         __syncthreads();
     }
@@ -328,13 +341,22 @@ template <typename T>
 hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s)
 {
     if (a.M <= 0 || a.N <= 0 || a.batch <= 0 || a.groups <= 0) return hipSuccess;
-    const long tiles = (long)((a.M + BM - 1) / BM) * ((a.N + BN - 1) / BN);
+    if (a.tile != 0 && a.tile != 32 && a.tile != 64) return hipErrorInvalidValue;
+    const int tb = a.tile == 32 ? 32 : 64;
+    const long tiles = (long)((a.M + tb - 1) / tb) * ((a.N + tb - 1) / tb);
     const long blocks = tiles * (long)a.batch * a.groups;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-    if (a.m3)
-        hipLaunchKernelGGL((k_gemm<T, true>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_gemm<T, false>), dim3((unsigned)blocks), dim3(NT), 0, s, a);
+    if (tb == 32) {
+        if (a.m3)
+            hipLaunchKernelGGL((k_gemm<T, true, 32>), dim3((unsigned)blocks), dim3(Tile<32>::NT), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_gemm<T, false, 32>), dim3((unsigned)blocks), dim3(Tile<32>::NT), 0, s, a);
+    } else {
+        if (a.m3)
+            hipLaunchKernelGGL((k_gemm<T, true, 64>), dim3((unsigned)blocks), dim3(Tile<64>::NT), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_gemm<T, false, 64>), dim3((unsigned)blocks), dim3(Tile<64>::NT), 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -74,6 +74,8 @@ struct nt_ctx {
     std::vector<CircU> circ_u;
     // autotuned inverse panel width per (element bytes, n); absent: size rule
     std::map<std::pair<int, int>, int> inv_nb;
+    // autotuned GEMM output tile (64 or 32) per (element bytes, max(M, N)); absent: 64
+    std::map<std::pair<int, int>, int> gemm_tile;
     // degenerate-group descriptors and statuses of the path_mode = 1 rotation
     void* deg_buf = nullptr;
     size_t deg_bytes = 0;
@@ -286,6 +288,10 @@ void gemm(nt_ctx* c, int M, int N, int K, int opA, BMat<T> A, int opB, BMat<T> B
     a.batch = batch;
     a.groups = groups;
     a.m3 = (sizeof(T) == 8 && c->gemm_m3) ? 1 : 0;
+    if (!c->gemm_tile.empty()) {
+        auto tuned = c->gemm_tile.find({(int)sizeof(T), std::max(M, N)});
+        a.tile = tuned == c->gemm_tile.end() ? 0 : tuned->second;
+    }
     const double items = (nitems < 0 ? (double)batch : nitems) * groups;
     audit<T>(c, A, opA == OP_N ? M : K, opA == OP_N ? K : M, batch, groups, "gemm A");
     audit<T>(c, B, opB == OP_N ? K : N, opB == OP_N ? N : K, batch, groups, "gemm B");

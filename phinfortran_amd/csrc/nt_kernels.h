@@ -21,6 +21,7 @@ template <typename T> struct GemmArgs {
     unsigned long long* maxabs;  // per item p (bits of a non-negative double), or null
     int batch, groups;
     int m3 = 0;             // 1: three real products per complex product
+    int tile = 0;           // output tile edge: 64 (0 = default) or 32
 };
 
 template <typename T> hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s);

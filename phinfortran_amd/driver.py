@@ -157,7 +157,7 @@ def _tb_device_blocks(H, layers, nl: int, nr: int, nbl: int, nbr: int):
 
 
 def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, param_path: Optional[str] = None,
-           batched: Optional[bool] = None):
+           batched: Optional[bool] = None, dos: bool = True):
     """The tight-binding driver's transport run (tb/infortran.f90:77-630,
     path_mode 2 and 3, Caroli transmission) on one GPU.
 
@@ -170,13 +170,20 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     every (E, k) point is solved in one batched call on blocks sliced on the
     device (layers padded to the widest one, which leaves the transmission
     unchanged); batched=False runs the reference's call sequence point by
-    point (surface_green + G_block, tb/infortran.f90:492-504).  Writes transl_tot.dat /
-    transr_tot.dat in the tight-binding writer's format and positions_all.dat,
-    nkpoints.dat when output_dir is given (defaults to the config's
-    output_dir, relative to the configuration's folder).
+    point (surface_green + G_block, tb/infortran.f90:492-504).  With dos
+    (default) every point also gets the layer-resolved device DOS of get_ldos
+    (tb/infortran.f90:508-510, tb/device.f90:610-855): in the batched solve
+    from the diagonal blocks of the full device G (padded orbitals read 0, as
+    the reference's entries beyond a layer's size), per point through
+    nt_get_ldos otherwise.  Writes transl_tot.dat / transr_tot.dat in the
+    tight-binding writer's format, sdosl.dat / sdosr.dat, ddos.dat and
+    ldos_<k>.dat (with dos), positions_all.dat and nkpoints.dat when
+    output_dir is given (defaults to the config's output_dir, relative to the
+    configuration's folder).
 
-    Returns dict(E (ne), k (nk x 3, as used), transl, transr, sdos_l, sdos_r
-    (ne x nk), status, layer_list, norb, nl, nr)."""
+    Returns dict(E (ne), k (nk x 3, as used), transl, transr, sdos_l, sdos_r,
+    ddos (ne x nk), ldos (nk x ne x nlayer x maxlen), status, layer_list,
+    norb, nl, nr); ddos and ldos are None without dos."""
     import torch
     from . import dense as D
     from . import tb as TBM
@@ -203,15 +210,21 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     p = T.params("tb", eta=cfg["eta"], convergence=cfg["convergence"])
     if batched is None:
         batched = True
+    maxlen = max(layers)
+    ldos = np.zeros((nk, ne, len(layers), maxlen)) if dos else None
     if batched:
         d = _tb_device_blocks(H, layers, nl, nr, nbl, nbr)
         kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)
-        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, extras=("sdos_l", "sdos_r"))
+        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, extras=("sdos_l", "sdos_r") + (("ldos",) if dos else ()))
         tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
         tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
         status = r["status"].cpu().numpy().reshape(nk, ne).T
         sdl = r["sdos_l"].cpu().numpy().reshape(nk, ne).T
         sdr = r["sdos_r"].cpu().numpy().reshape(nk, ne).T
+        if dos:
+            # layers padded to maxlen orbitals: their extra orbitals read 0
+            real = np.arange(maxlen)[None, :] < np.asarray(layers)[:, None]
+            ldos = r["ldos"].cpu().numpy().reshape(nk, ne, len(layers), maxlen) * real
     else:
         tl, tr = np.zeros((ne, nk)), np.zeros((ne, nk))
         sdl, sdr = np.zeros((ne, nk)), np.zeros((ne, nk))
@@ -226,15 +239,24 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
                                 g["sl_adv"].conj().T, g["sr_ret"], E[ie], s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml,
                                 s.Hamr, p, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
                 tl[ie, ik], tr[ie, ik] = o["transl"], o["transr"]
+                if dos:
+                    ldos[ik, ie], _ = eng.get_ldos(g["gl_adv"].conj().T, g["gr_ret"], E[ie], s.Hd, s.Hdv, s.Hld,
+                                                   s.Hrd, p, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd,
+                                                   maxlen=maxlen)
+    ddos = ldos.sum(axis=(2, 3)).T if dos else None    # ddos = sum(ldos) (tb/device.f90:853)
     _check_points(status, E, ks, "run_tb")
     out_dir = output_dir
     if out_dir is None and cfg["output_dir"]:
         out_dir = os.path.join(base, cfg["output_dir"])
     if out_dir:
         TBM.write_positions(out_dir, st["positions"], st["species"], P["orb_num"], layers, nbl, nl, nk)
+        if dos:
+            for ik in range(nk):
+                TBM.write_ldos(out_dir, ik + 1, E, ldos[ik])
+            TBM.write_device_dos(out_dir, ddos)
         TBM.write_surface_ldos(out_dir, sdl, sdr)
         OUT.write_trans(out_dir, np.repeat(E[:, None], nk, axis=1), tl, tr, tb=True)
     if engine is None:
         eng.close()
-    return dict(E=E, k=ks, transl=tl, transr=tr, sdos_l=sdl, sdos_r=sdr, status=status, layer_list=layers,
-                norb=asm.norb, nl=nl, nr=nr)
+    return dict(E=E, k=ks, transl=tl, transr=tr, sdos_l=sdl, sdos_r=sdr, ddos=ddos, ldos=ldos, status=status,
+                layer_list=layers, norb=asm.norb, nl=nl, nr=nr)

@@ -270,6 +270,26 @@ def write_surface_ldos(output_dir: str, sdosl, sdosr):
                 f.write("".join("%20.10f" % v for v in row) + " \n")
 
 
+def write_ldos(output_dir: str, k_index: int, E, ldos):
+    """write_ldos (tb/write_data.f90:244-276) for every energy of one k-point:
+    ldos_<k_index as I6.6>.dat, one row per energy: E, then ldos(i, j) with the
+    layer i outer and the orbital j inner (f20.10 each), ending in a blank;
+    appended as the reference opens it.  ldos: (ne, nlayer, maxlen)."""
+    os.makedirs(output_dir, exist_ok=True)
+    with open(os.path.join(output_dir, "ldos_%06d.dat" % int(k_index)), "a") as f:
+        for e, l in zip(np.asarray(E, float), np.asarray(ldos, float)):
+            f.write("%20.10f" % e + "".join("%20.10f" % v for v in l.ravel()) + " \n")
+
+
+def write_device_dos(output_dir: str, ddos):
+    """write_device_dos (tb/write_data.f90:310-335): ddos.dat, one row per
+    energy of f20.10 values (one per k-point) ending in a blank, appended."""
+    os.makedirs(output_dir, exist_ok=True)
+    with open(os.path.join(output_dir, "ddos.dat"), "a") as f:
+        for row in np.asarray(ddos, float):
+            f.write("".join("%20.10f" % v for v in row) + " \n")
+
+
 # ------------------------------------------------------------------ Slater-Koster
 def _angular(A: int, B: int, l, m, n):
     """f_t for the canonical shell pair (A, B): dict t -> array

@@ -245,7 +245,7 @@ class Engine:
         T_RL = torch.empty(npts, dtype=torch.float64, device=dev)
         status = torch.zeros(npts, dtype=torch.int32, device=dev)
         iters = torch.zeros(npts, dtype=torch.int32, device=dev)
-        ex, res = self._extras(extras, npts, nl, nr, dev)
+        ex, res = self._extras(extras, npts, nl, nr, dev, L=L, b=b)
         if self._torch_stream:
             self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
         ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
@@ -260,13 +260,14 @@ class Engine:
             torch.cuda.synchronize(dev)
         out = dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters)
         for name, t in res.items():
-            out[name] = t if t.dim() == 1 else t.transpose(-1, -2)
+            out[name] = t if not t.is_complex() else t.transpose(-1, -2)
         return out
 
     @staticmethod
-    def _extras(names, npts, nl, nr, dev):
+    def _extras(names, npts, nl, nr, dev, L=None, b=None):
         """device buffers of the requested nt_solve_extras fields (complex
-        matrices stored column-major: tensor [p, col, row])"""
+        matrices stored column-major: tensor [p, col, row]; ldos [p, layer,
+        orbital] and ddos [p] of get_ldos, which need the device's L and b)"""
         import torch
         shapes = dict(gl_adv=(nl, nl), gr_ret=(nr, nr), gl_plus=(nl, nl), gr_minus=(nr, nr), gml_adv=(nl, nl),
                       gmr_ret=(nr, nr), sl_adv=(nl, nl), sr_ret=(nr, nr), G_D_LR=(nr, nl), G_D_RL=(nl, nr),
@@ -274,8 +275,12 @@ class Engine:
         ex = N.NtSolveExtras()
         res = {}
         for name in names:
-            if name in ("sdos_l", "sdos_r"):
+            if name in ("sdos_l", "sdos_r", "ddos"):
                 t = torch.zeros(npts, dtype=torch.float64, device=dev)
+            elif name == "ldos":
+                if L is None or b is None:
+                    raise ValueError("ldos is an output of the batched solve (solve_tdb) only")
+                t = torch.zeros((npts, L, b), dtype=torch.float64, device=dev)
             else:
                 t = torch.zeros((npts,) + shapes[name], dtype=torch.complex128, device=dev)
             res[name] = t
@@ -560,6 +565,21 @@ class Engine:
         self._check(self._L.nt_autotune_inverse(self.ctx, int(n), int(batch), prec, C.byref(nb), C.byref(ms)),
                     "nt_autotune_inverse")
         return nb.value, ms.value
+
+    def autotune_gemm(self, n: int, batch: int, precision: str = "fp64"):
+        """Time the 64- and 32-wide GEMM output tiles on n x n x n batches;
+        keeps the faster for GEMMs whose larger output dimension is n.
+        Returns (tile, ms per batched launch)."""
+        prec = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[precision]
+        tb, ms = C.c_int(), C.c_double()
+        self._check(self._L.nt_autotune_gemm(self.ctx, int(n), int(batch), prec, C.byref(tb), C.byref(ms)),
+                    "nt_autotune_gemm")
+        return tb.value, ms.value
+
+    def set_gemm_tile(self, n: int, tile: int, precision: str = "fp64"):
+        """GEMM output tile (64, 32; 0 = default) for larger output dimension n"""
+        prec = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[precision]
+        self._check(self._L.nt_set_gemm_tile(self.ctx, int(n), prec, int(tile)), "nt_set_gemm_tile")
 
     def set_gemm_3m(self, on: bool):
         """FP64 GEMMs with three real products per complex product."""

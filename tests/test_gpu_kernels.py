@@ -33,6 +33,30 @@ def crand(rng, *shape):
                                    (65, 33, 47), (100, 37, 200), (192, 192, 192), (250, 130, 70)])
 @pytest.mark.parametrize("ops", ["NN", "NC", "CN", "CC"])
 def test_zgemm_batched(engine, torch_dev, m, n, k, ops):
+    _zgemm_vs_numpy(engine, torch_dev, m, n, k, ops)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (24, 24, 24), (17, 9, 33), (33, 65, 40), (80, 80, 80), (96, 96, 96),
+                                   (100, 37, 200), (216, 216, 216)])
+@pytest.mark.parametrize("ops", ["NN", "NC", "CN", "CC"])
+def test_zgemm_tile32(engine, torch_dev, m, n, k, ops):
+    """the one-wave 32x32 output tile (selected per size) against numpy"""
+    engine.set_gemm_tile(max(m, n), 32)
+    try:
+        _zgemm_vs_numpy(engine, torch_dev, m, n, k, ops)
+    finally:
+        engine.set_gemm_tile(max(m, n), 0)
+
+
+def test_gemm_autotune(engine):
+    for n in (24, 96, 192):
+        tile, ms = engine.autotune_gemm(n, 256)
+        print(f"gemm tile for n={n}: {tile} ({ms:.3f} ms for 256 items)", flush=True)
+        assert tile in (32, 64) and ms > 0
+        engine.set_gemm_tile(n, 0)
+
+
+def _zgemm_vs_numpy(engine, torch_dev, m, n, k, ops):
     from phinfortran_amd import _native as N
     torch, dev = torch_dev
     rng = np.random.default_rng(m * 1000 + n * 10 + k)

@@ -328,3 +328,26 @@ def test_one_sweep_matches_second_sweep(engine, T, case):
         for i in range(len(E)):
             bound = max(tol[i], ctol(b[i] if b[i].shape[0] == b[i].shape[1] else b[i] @ b[i].conj().T))
             assert rel(a[i], b[i]) < bound, (case, k, i, kap[i])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gemm_tile32_in_solve(engine, T, precision):
+    """The solve with every GEMM of the film's size on the 32x32 tile equals
+    the default 64x64 tile's (each output element sums the same products in
+    the same order; the tiles differ only in which elements a workgroup
+    owns), and stays within the oracle's tolerance."""
+    blk = lt.si_ge_interface_film(nx=2, ny=1, L=8, kfrac=(0.125, 0.25))
+    p = T.params("phonon", precision=precision)
+    E = np.array([1.5, 4.0, 7.5, 11.0, 14.5]) ** 2
+    d = engine.upload_blocks(blk)
+    print(f"tile 32 vs 64 in the {precision} solve, b={d['b']}", flush=True)
+    ref = engine.solve_tdb(d, E, p, extras=("G_D_LR",))
+    engine.set_gemm_tile(d["b"], 32, precision=precision)
+    try:
+        got = engine.solve_tdb(d, E, p, extras=("G_D_LR",))
+    finally:
+        engine.set_gemm_tile(d["b"], 0, precision=precision)
+    assert (got["status"].cpu().numpy() == 0).all()
+    for k in ("T_LR", "T_RL", "G_D_LR"):
+        a, b = got[k].cpu().numpy(), ref[k].cpu().numpy()
+        assert np.abs(a - b).max() <= 1e-13 * max(1.0, np.abs(b).max()), (k, np.abs(a - b).max())

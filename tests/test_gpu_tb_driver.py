@@ -34,6 +34,8 @@ def oracle_T(Tm, P, lat, pos, sp, k, E, eta, layers, nl, nr):
     o = O.rgf(s.Hd, s.Hdv, s.Hld, s.Hrd, s.Haml, s.Hamr, r["gl_adv"].conj().T, r["gr_ret"], -r["gml_adv"],
               r["gmr_ret"], r["sl_adv"].conj().T, r["sr_ret"], E, op, Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
     oracle_T.sdos = (np.trace(r["gl_adv"]).imag / np.pi, -np.trace(r["gr_ret"]).imag / np.pi)
+    oracle_T.ldos = O.get_ldos(s.Hd, s.Hdv, s.Hld, s.Hrd, r["gl_adv"].conj().T, r["gr_ret"], E, op,
+                               Hdv_back=s.Hdv_back, Hrd_fwd=s.Hrd_fwd)
     return o["T_LR"], o["T_RL"]
 
 
@@ -52,7 +54,11 @@ def test_run_tb_matches_oracle(engine, Tm, tmp_path):
             sl_, sr_ = oracle_T.sdos                     # surface DOS: +-Im Tr g / pi, no sqrt(E)
             assert abs(r["sdos_l"][ie, ik] - sl_) <= 1e-8 * max(1.0, abs(sl_)), (ik, E)
             assert abs(r["sdos_r"][ie, ik] - sr_) <= 1e-8 * max(1.0, abs(sr_)), (ik, E)
+            lo, do = oracle_T.ldos                       # get_ldos: layer-resolved device DOS
+            assert np.abs(r["ldos"][ik, ie] - lo).max() <= 1e-8 * max(1.0, np.abs(lo).max()), (ik, E)
+            assert abs(r["ddos"][ie, ik] - do) <= 1e-8 * max(1.0, abs(do)), (ik, E)
     assert (r["sdos_l"] > 0).any() and (r["sdos_l"] >= -1e-8).all()
+    assert (r["ddos"] > 0).any() and (r["ldos"] >= -1e-8).all()
     assert r["transl"].max() > 0.5 and r["transl"][2].max() < 1e-10            # E = 0.35 eV lies in the gap
     out = os.path.join(str(tmp_path), "out")
     rows = open(os.path.join(out, "transl_tot.dat")).read().splitlines()
@@ -63,6 +69,15 @@ def test_run_tb_matches_oracle(engine, Tm, tmp_path):
     assert len(srows) == 5 and all(len(x) == 20 * 4 + 1 for x in srows)
     assert float(srows[0][:20]) == pytest.approx(r["sdos_l"][0, 0], abs=1e-9)
     assert open(os.path.join(out, "nkpoints.dat")).read() == "        4.0000000000\n"
+    drows = open(os.path.join(out, "ddos.dat")).read().splitlines()
+    assert len(drows) == 5 and all(len(x) == 20 * 4 + 1 for x in drows)
+    assert float(drows[1][20:40]) == pytest.approx(r["ddos"][1, 1], abs=1e-9)
+    nlay, maxlen = r["ldos"].shape[2:]
+    lrows = open(os.path.join(out, "ldos_000002.dat")).read().splitlines()
+    assert len(lrows) == 5 and all(len(x) == 20 * (1 + nlay * maxlen) + 1 for x in lrows)
+    assert float(lrows[3][:20]) == pytest.approx(r["E"][3], abs=1e-9)
+    assert float(lrows[3][20 * (1 + maxlen + 2):20 * (2 + maxlen + 2)]) == pytest.approx(r["ldos"][1, 3, 1, 2],
+                                                                                       abs=1e-9)
 
 
 @pytest.mark.parametrize("cells,buffer", [((1, 1, 8), 0), ((2, 1, 8), 0), ((1, 1, 10), 4)])
@@ -82,6 +97,11 @@ def test_run_tb_pointwise_equals_batched(engine, tmp_path, cells, buffer):
     assert np.abs(a["transr"] - b["transr"]).max() <= 1e-9 * max(1.0, np.abs(a["transr"]).max())
     assert np.abs(a["sdos_l"] - b["sdos_l"]).max() <= 1e-9 * max(1.0, np.abs(a["sdos_l"]).max())
     assert np.abs(a["sdos_r"] - b["sdos_r"]).max() <= 1e-9 * max(1.0, np.abs(a["sdos_r"]).max())
+    # device DOS: batched (padded layers) against the per-point get_ldos
+    assert np.abs(a["ldos"] - b["ldos"]).max() <= 1e-9 * max(1.0, np.abs(b["ldos"]).max())
+    assert np.abs(a["ddos"] - b["ddos"]).max() <= 1e-9 * max(1.0, np.abs(b["ddos"]).max())
+    for i, n in enumerate(a["layer_list"]):
+        assert (a["ldos"][:, :, i, n:] == 0).all() and (b["ldos"][:, :, i, n:] == 0).all()
 
 
 def test_run_tb_pristine_si_integer_channels(engine, tmp_path):

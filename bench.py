@@ -77,8 +77,13 @@ def main():
     ap.add_argument("--serial-sweeps", action="store_true",
                     help="run the two RGF sweeps on one stream in every step (for per-kernel profiles)")
     ap.add_argument("--c3-devices", type=int, default=16,
-                    help="uneven-batching sample: Si/Ge superlattice devices (one per block size 24/96/216/384 "
-                         "in turn), 64 frequencies x 2x2 k each, solved in one multi-device call (0 = skip)")
+                    help="uneven-batching sample: devices of the seeded 256-device Si/Ge superlattice draw (the "
+                         "first of each block size 24/96/216/384 in draw order, in equal numbers; 256 = the whole "
+                         "draw), solved in one multi-device call (0 = skip)")
+    ap.add_argument("--c3-freqs", type=int, default=64,
+                    help="frequencies per device of the uneven-batching sample (of its 256 in (0, 16] THz)")
+    ap.add_argument("--c3-k", type=int, default=2,
+                    help="transverse k-grid side of the uneven-batching sample (2: 2x2 k-points per device)")
     ap.add_argument("--no-modes", dest="modes", action="store_false",
                     help="skip the mode-resolved sample (the si-si example's full mode-resolved run: solve, "
                          "modes, Brillouin-zone and primitive-cell unfolding of 49 k x 61 omega)")
@@ -421,12 +426,15 @@ def main():
         devices of the same block size share their (pristine Si) leads, whose
         Green's functions the engine then computes once; FP64 solves/s per
         block size from one call per size"""
-        specs = lt.superlattice_set(args.c3_devices, seed=3)
-        for i, sp in enumerate(specs):           # one of each block size in turn
-            sp["n"] = (1, 2, 3, 4)[i % 4]
-            sp["b"] = 24 * sp["n"] ** 2
-        ks = lt.mp_grid(2, 2)
-        f = 16.0 * np.arange(1, 65) / 64
+        draw = lt.superlattice_set(256, seed=3)
+        if args.c3_devices >= len(draw):
+            specs = draw
+        else:
+            per = max(1, args.c3_devices // 4)
+            specs = [sp for n in (1, 2, 3, 4) for sp in [d for d in draw if d["n"] == n][:per]]
+        ks = lt.mp_grid(args.c3_k, args.c3_k)
+        # every device: an evenly spread sample of its 256 frequencies at every k-point of the grid
+        f = (16.0 * np.arange(1, 257) / 256)[np.linspace(0, 255, args.c3_freqs).astype(int)]
         devs, npts = [], 0
         t0 = time.time()
         for sp in specs:
@@ -435,15 +443,15 @@ def main():
             kidx = np.repeat(np.arange(len(ks), dtype=np.int32), len(f))
             devs.append(dict(blocks=eng.upload_blocks(blk), E=E, kidx=kidx))
             npts += E.size
-        log(f"c3 sample: {len(devs)} devices built in {time.time() - t0:.1f}s")
+        log(f"c3: {len(devs)} devices built in {time.time() - t0:.1f}s")
         tune = {}
         if args.autotune:
             # per block size: the inverse's panel width and the GEMM's output
             # tile, timed on batches of one device's points
             for bs in sorted({sp["b"] for sp in specs}):
                 for prec in ("fp64", "mixed"):
-                    nb, ims = eng.autotune_inverse(bs, len(devs[0]["E"]), prec)
-                    tb, gms = eng.autotune_gemm(bs, len(devs[0]["E"]), prec)
+                    nb, ims = eng.autotune_inverse(bs, 256, prec)
+                    tb, gms = eng.autotune_gemm(bs, 256, prec)
                     tune[f"{bs}/{prec}"] = dict(inverse_panel=nb, inverse_ms=ims, gemm_tile=tb, gemm_ms=gms)
                     log(f"c3 tuning b={bs} {prec}: inverse panel {nb} ({ims:.3f} ms), gemm tile {tb} ({gms:.3f} ms)")
         res = {}
@@ -476,8 +484,10 @@ def main():
                 "fp64_solves_per_s_per_block": per_b,
                 "mixed_max_rel_dT_per_device": errs, "status_ok": res["fp64"][2] and res["mixed"][2],
                 "tuning": tune,
-                "what": "Si/Ge superlattice films, 64 frequencies x 2x2 k per device, devices of one block size "
-                        "sharing their leads' Green's functions"}
+                "what": f"{len(specs)} devices of the seeded 256-device Si/Ge superlattice draw (block size, "
+                        f"period and length from the draw), {f.size} of the 256 frequencies x {len(ks)} k-points "
+                        "each, all in one multi-device call; devices with the same leads and points share their "
+                        "lead Green's functions"}
 
     c3 = run_c3() if args.c3_devices > 0 else None
 

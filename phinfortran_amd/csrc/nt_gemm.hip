@@ -168,13 +168,20 @@ __global__ __launch_bounds__(Tile<TB>::NT, (M3 ? 2 : 3) * (256 / Tile<TB>::NT)) 
     const int tiles_n = (a.N + BN - 1) / BN;
     const int tiles = tiles_m * tiles_n;
     // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
-    // L2.  When the item count divides by 8, renumber so that XCD x runs a
-    // contiguous range of logical blocks: all tiles of one item then share
-    // that XCD's L2 and its A and B panels are fetched from HBM once.
+    // L2.  From 8 points on, renumber so that point p (all its groups and
+    // tiles) runs on XCD p mod 8, its blocks consecutive in that XCD's
+    // sequence: the tiles of one item then share that XCD's L2 (its A and B
+    // panels are fetched from HBM once), and points masked out by
+    // convergence, which come in contiguous ranges of energy, leave the
+    // remaining work spread over all XCDs.  The grid then covers the point
+    // count rounded up to 8; the blocks of the padding points exit.
     long bid = blockIdx.x;
-    if (((long)a.batch * a.groups) % NXCD == 0) {
-        const long per = (long)gridDim.x / NXCD;
-        bid = (bid % NXCD) * per + bid / NXCD;
+    if (a.batch >= NXCD) {
+        const long per_point = (long)tiles * a.groups;
+        const long x = bid % NXCD, l = bid / NXCD;
+        const long pp = x + NXCD * (l / per_point);
+        if (pp >= a.batch) return;
+        bid = pp * per_point + l % per_point;
     }
     const int tile = (int)(bid % tiles);
     const long item = bid / tiles;
@@ -344,7 +351,10 @@ hipError_t launch_gemm(const GemmArgs<T>& a, hipStream_t s)
     if (a.tile != 0 && a.tile != 32 && a.tile != 64) return hipErrorInvalidValue;
     const int tb = a.tile == 32 ? 32 : 64;
     const long tiles = (long)((a.M + tb - 1) / tb) * ((a.N + tb - 1) / tb);
-    const long blocks = tiles * (long)a.batch * a.groups;
+    // from 8 points on the kernel interleaves points over the XCDs and the
+    // grid covers the point count rounded up to a multiple of 8
+    const long points = a.batch >= NXCD ? ((long)a.batch + NXCD - 1) / NXCD * NXCD : a.batch;
+    const long blocks = tiles * points * a.groups;
     if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
     if (tb == 32) {
         if (a.m3)

@@ -70,6 +70,13 @@ class Engine:
     """One GPU context (HIP stream + device workspace)."""
 
     def __init__(self, device: int = 0, batch_limit: Optional[int] = None, use_torch_stream: bool = True):
+        try:
+            import torch
+            # torch sets up the HIP runtime before the library's first HIP call
+            # (the other order leaves torch without devices in this process)
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         self._L = N.lib()
         ctx = C.c_void_p()
         st = self._L.nt_create(C.byref(ctx), int(device))

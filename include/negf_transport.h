@@ -551,8 +551,9 @@ int nt_set_overlap(nt_ctx* ctx, int on);
 /* Batched solves: n = 1 (default) computes T_RL, G_D_RL and G_L from the
  * left-to-right sweep and a correction of its corner blocks for the two end
  * blocks where the right-to-left system differs (the broadening of the lead
- * surface layers), one (nl+nr) inverse per point instead of the second
- * sweep; n = 2 runs the reference's second sweep (negf/lesser.f90:694-793).
+ * surface layers), two inverses of nl and nr and six GEMMs per point instead
+ * of the second sweep; n = 2 runs the reference's second sweep
+ * (negf/lesser.f90:694-793).
  * The per-point entries (nt_get_G_full_tdb, nt_G_block), the Buttiker-probe
  * solve and leads given with Hrd_fwd / dense lower couplings always run both
  * sweeps. */

@@ -101,39 +101,32 @@ __global__ void k_lead_merge(const int* st_s, const int* it_s, int P, int* st2, 
     }
 }
 
-// Corner system of the one-sweep correction, per point: with the corner
-// blocks G_00 (nl x nl), G_0c (nl x nr), G_c0 (nr x nl), G_cc (nr x nr) of the
-// left-to-right system's inverse (c = the closing layer N+1) and
-// d = z_dev - z_lead,
-//   M  = I + [G_00 G_0c; G_c0 G_cc] diag(d I_nl, -d I_nr)      (nl+nr square)
-//   C0 = [G_00; G_c0]                                          (nl+nr x nl)
-__global__ void k_corner_system(int nl, int nr, BMat<double> G00, BMat<double> G0c, BMat<double> Gc0,
-                                BMat<double> Gcc, const double* zd, const double* zl, int zls, BMatW<double> M,
-                                BMatW<double> C0, const int* active)
+// The blocks of the one-sweep corner system for block elimination:
+//   M11 = I + d G_00 (nl x nl), M22 = I - d G_cc (nr x nr),
+//   dGc0 = d G_c0 (nr x nl), dG0c = d G_0c (nl x nr), d = z_dev - z_lead.
+// blockIdx.z selects the block, blockIdx.y the point.
+__global__ void k_corner_blocks(int nl, int nr, BMat<double> G00, BMat<double> G0c, BMat<double> Gc0,
+                                BMat<double> Gcc, const double* zd, const double* zl, int zls, BMatW<double> M11,
+                                BMatW<double> M22, BMatW<double> dGc0, BMatW<double> dG0c, const int* active)
 {
-    const int p = blockIdx.y;
+    const int p = blockIdx.y, which = blockIdx.z;
     if (active && !active[p]) return;
-    const int n = nl + nr;
+    const int rows = (which == 0 || which == 3) ? nl : nr;
+    const int cols = (which == 0 || which == 2) ? nl : nr;
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long)n * n) return;
-    const int i = (int)(e % n), j = (int)(e / n);
-    const double* g;
-    if (i < nl && j < nl) g = bptr(G00, p, 0) + 2 * (i + (long)j * G00.ld);
-    else if (i < nl) g = bptr(G0c, p, 0) + 2 * (i + (long)(j - nl) * G0c.ld);
-    else if (j < nl) g = bptr(Gc0, p, 0) + 2 * ((i - nl) + (long)j * Gc0.ld);
-    else g = bptr(Gcc, p, 0) + 2 * ((i - nl) + (long)(j - nl) * Gcc.ld);
+    if (e >= (long)rows * cols) return;
+    const int i = (int)(e % rows), j = (int)(e / rows);
+    const BMat<double>& G = which == 0 ? G00 : which == 1 ? Gcc : which == 2 ? Gc0 : G0c;
+    const BMatW<double>& O = which == 0 ? M11 : which == 1 ? M22 : which == 2 ? dGc0 : dG0c;
+    const double* g = bptr(G, p, 0) + 2 * (i + (long)j * G.ld);
     const double gr = g[0], gi = g[1];
-    // d = i (Im z_dev - Im z_lead) (same real part E)
+    // d = i (Im z_dev - Im z_lead) (same real part E); M22 carries -d
     const double dr = zd[2 * p] - zl[(long)zls * p], di = zd[2 * p + 1] - zl[(long)zls * p + 1];
-    const double sr = j < nl ? dr : -dr, si = j < nl ? di : -di;
-    double* m = bptr(M, p, 0) + 2 * (i + (long)j * M.ld);
-    m[0] = (i == j ? 1.0 : 0.0) + gr * sr - gi * si;
-    m[1] = gr * si + gi * sr;
-    if (j < nl) {
-        double* c = bptr(C0, p, 0) + 2 * (i + (long)j * C0.ld);
-        c[0] = gr;
-        c[1] = gi;
-    }
+    const double sg = which == 1 ? -1.0 : 1.0;
+    const double one = (which < 2 && i == j) ? 1.0 : 0.0;
+    double* o = bptr(O, p, 0) + 2 * (i + (long)j * O.ld);
+    o[0] = one + sg * (gr * dr - gi * di);
+    o[1] = sg * (gr * di + gi * dr);
 }
 
 // fb[p] = !accept[p]; *count += number of rejected points
@@ -663,8 +656,8 @@ size_t rgf_scratch(int nl, int nr, int bmax, int P)
     // the right-to-left sweep's own buffers and status (concurrent sweeps)
     for (int i = 0; i < 10; ++i) a.take(mb * P);
     a.take(sizeof(int) * P);
-    // corner system of the one-sweep correction: M, inv(M) ((nl+nr)^2 <= 4
-    // slots each), C0 and the corrected column ((nl+nr) nl <= 2 slots each)
+    // one-sweep corner correction: M11, M22, d G_c0, d G_0c; Y, S, S^-1, T;
+    // R, Q (one slot each) and the corrected column ((nl+nr) nl <= 2 slots)
     a.take(12 * mb * P);
     return a.off;
 }
@@ -823,10 +816,10 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     T* X_b = sa.ptr<T>(sa.take(mb * P));
     T* Y_b = sa.ptr<T>(sa.take(mb * P));
     int* st_b = sa.ptr<int>(sa.take(sizeof(int) * P));
-    T* CM = sa.ptr<T>(sa.take(4 * mb * P));      // corner system M
-    T* CMi = sa.ptr<T>(sa.take(4 * mb * P));     // inv(M)
-    T* CC0 = sa.ptr<T>(sa.take(2 * mb * P));     // [G_00; G_c0]
-    T* CGp = sa.ptr<T>(sa.take(2 * mb * P));     // inv(M) [G_00; G_c0]
+    T* CM = sa.ptr<T>(sa.take(4 * mb * P));      // corner blocks M11, M22, d G_c0, d G_0c
+    T* CMi = sa.ptr<T>(sa.take(4 * mb * P));     // Y = M11^-1, S, S^-1, T
+    T* CC0 = sa.ptr<T>(sa.take(2 * mb * P));     // R, Q
+    T* CGp = sa.ptr<T>(sa.take(2 * mb * P));     // corrected column [G_L; G_RL]
     (void)GRL;
     hipStream_t s = c->stream;
     const int L = in.L;
@@ -854,7 +847,7 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
     // one sweep: the right-to-left quantities follow from the left-to-right
     // system's corner blocks (see one_sweep below); its buffers are free
     const bool one = std::is_same<T, double>::value && c->one_sweep && !in.keep && !in.zl &&
-                     in.zlead && !in.Hrd_fwd.base && in.Hdv_back.empty() && nl + nr <= inv_max_n();
+                     in.zlead && !in.Hrd_fwd.base && in.Hdv_back.empty();
     // the right-to-left sweep (two sweeps) runs on the auxiliary stream from
     // here; with one sweep only the main stream is used
     if (!one && c->overlap && c->aux) {
@@ -969,25 +962,48 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
                     in.act);
             gemm<T>(c, nl, nl, nr, OP_N, ro(gdlr), OP_N, R(Tc, nr), W(G00, nl), 1.0, R(G00, nl), 1.0, nullptr, P, -1,
                     in.act);
+            // G'_CC [I; 0] = M^-1 [G_00; G_c0] with M = I + G_CC Delta, by block
+            // elimination over the nl and nr blocks (M11 = I + d G_00 is I up
+            // to d G_00, d of the order of the broadening):
+            //   Y = M11^-1, T = d G_c0 Y, S = (I - d G_cc) + T d G_0c,
+            //   X2 = S^-1 (G_c0 - T G_00), X1 = Y (G_00 + d G_0c X2);
+            // G_RL = X2, G_L = X1 (two inverses of nl and nr and six GEMMs
+            // instead of an inverse of nl + nr)
             const long m4 = 4 * mm, m2 = 2 * mm;
-            const BMatW<T> Mw = bw(CM, m4, n2), Miw = bw(CMi, m4, n2), C0w = bw(CC0, m2, n2), Gpw = bw(CGp, m2, n2);
+            auto sl = [&](T* base, long stride, int k, int ld) { return bw(base + 2 * (long)k * mm, stride, ld); };
+            const BMatW<T> M11 = sl(CM, m4, 0, nl), M22 = sl(CM, m4, 1, nr), dGc0 = sl(CM, m4, 2, nr),
+                           dG0c = sl(CM, m4, 3, nl);
+            const BMatW<T> Yw = sl(CMi, m4, 0, nl), Sw = sl(CMi, m4, 1, nr), Siw = sl(CMi, m4, 2, nr),
+                           Tw = sl(CMi, m4, 3, nr);
+            const BMatW<T> Rw = sl(CC0, m2, 0, nr), Qw = sl(CC0, m2, 1, nl);
+            const BMatW<T> X1w = bw(CGp, m2, n2), X2w = bw(CGp + 2 * (long)nl, m2, n2);
             audit<T>(c, R(G00, nl), nl, nl, P, 1, "corner G00");
             audit<T>(c, ro(gdlr), nl, nr, P, 1, "corner G0c");
             audit<T>(c, R(Gc0, nr), nr, nl, P, 1, "corner Gc0");
             audit<T>(c, ro(gRo), nr, nr, P, 1, "corner Gcc");
-            audit<T>(c, Mw, n2, n2, P, 1, "corner M");
-            audit<T>(c, C0w, n2, nl, P, 1, "corner C0");
+            audit<T>(c, M11, nl, nl, P, 1, "corner M11");
+            audit<T>(c, M22, nr, nr, P, 1, "corner M22");
+            audit<T>(c, dGc0, nr, nl, P, 1, "corner dGc0");
+            audit<T>(c, dG0c, nl, nr, P, 1, "corner dG0c");
+            audit<T>(c, X1w, n2, nl, P, 1, "corner X");
             check_range(c, in.zlead, sizeof(T) * ((size_t)in.zlead_stride * (P - 1) + 2), "corner z_lead");
             check_range(c, zd, sizeof(T) * 2 * P, "corner z_dev");
             check_range(c, in.act, sizeof(int) * P, "corner active");
-            launch(c, NT_K_ELEM, 0, "corner system", [&] {
-                hipLaunchKernelGGL(k_corner_system, dim3(nblk((long)n2 * n2), P), dim3(256), 0, s, nl, nr,
-                                   R(G00, nl), ro(gdlr), R(Gc0, nr), ro(gRo), zd, in.zlead, in.zlead_stride, Mw, C0w,
-                                   in.act);
+            const int nx = std::max(nl, nr);
+            launch(c, NT_K_ELEM, 0, "corner blocks", [&] {
+                hipLaunchKernelGGL(k_corner_blocks, dim3(nblk((long)nx * nx), P, 4), dim3(256), 0, s, nl, nr,
+                                   R(G00, nl), ro(gdlr), R(Gc0, nr), ro(gRo), zd, in.zlead, in.zlead_stride, M11, M22,
+                                   dGc0, dG0c, in.act);
                 return hipGetLastError();
             });
-            inverse<T>(c, n2, Mw, none<T>(), nullptr, Miw, st, P, -1, in.act);
-            gemm<T>(c, n2, nl, n2, OP_N, ro(Miw), OP_N, ro(C0w), Gpw, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            inverse<T>(c, nl, M11, none<T>(), nullptr, Yw, st, P, -1, in.act);
+            gemm<T>(c, nr, nl, nl, OP_N, ro(dGc0), OP_N, ro(Yw), Tw, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, nr, nr, nl, OP_N, ro(Tw), OP_N, ro(dG0c), Sw, 1.0, ro(M22), 1.0, nullptr, P, -1, in.act);
+            gemm<T>(c, nr, nl, nl, OP_N, ro(Tw), OP_N, R(G00, nl), Rw, -1.0, R(Gc0, nr), 1.0, nullptr, P, -1, in.act);
+            inverse<T>(c, nr, Sw, none<T>(), nullptr, Siw, st, P, -1, in.act);
+            gemm<T>(c, nr, nl, nr, OP_N, ro(Siw), OP_N, ro(Rw), X2w, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
+            gemm<T>(c, nl, nl, nr, OP_N, ro(dG0c), OP_N, ro(X2w), Qw, 1.0, R(G00, nl), 1.0, nullptr, P, -1, in.act);
+            gemm<T>(c, nl, nl, nl, OP_N, ro(Yw), OP_N, ro(Qw), X1w, 1.0, none<T>(), 0.0, nullptr, P, -1, in.act);
             // G_RL = rows nl.. of the corrected column, G_L = rows 0..nl-1
             const BMat<T> grl = bm<T>(CGp + 2 * (long)nl, m2, n2);
             const BMat<T> gll = bm<T>(CGp, m2, n2);

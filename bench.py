@@ -322,8 +322,8 @@ def main():
     el64, T64, k64, nbar, ok64 = run("fp64")
 
     def isolated_kernels(prec):
-        """one more step with the two sweeps on one stream: per-launch kernel
-        times that do not include time shared with the other sweep"""
+        """one more step with every launch on one stream: per-launch kernel
+        times that include no time shared with another stream's kernels"""
         p = T.params("phonon", precision=prec)
         eng.set_overlap(False)
         eng.profile_reset()
@@ -734,10 +734,11 @@ def main():
                      "traffic": traffic_per_launch(dom_name, prec_used, b, args.points),
                      "launches": (iso or dom)["launches"], "avg_launch_ms": avg_ms,
                      "timing": "HIP events around every launch on the launching stream, in one extra step with "
-                               "the two RGF sweeps on one stream (kernels do not share the GPU)",
+                               "every launch on one stream (no kernel shares the GPU with another)",
                      "concurrent": {"avg_launch_ms": avg_ms_conc, "achieved": achieved_conc,
-                                    "what": "the same kernel in the timed steps, where the two sweeps run on two "
-                                            "streams and a launch shares the GPU with the other sweep's kernels"},
+                                    "what": "the same kernel in the timed steps (the default one-sweep solve uses "
+                                            "one stream too; with two sweeps the second runs on a second stream "
+                                            "and launches can share the GPU)"},
                      "inverse_isolated": {"avg_launch_ms": iso64["inverse"]["ms"] / max(1, iso64["inverse"]["launches"]),
                                           "tflops": iso64["inverse"]["flops"] / max(1e-9, iso64["inverse"]["ms"] * 1e-3) / 1e12}},
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,
@@ -745,7 +746,8 @@ def main():
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",
                               "refinement": mixed_counts,
                               "status_ok": bool(ok64 and okmx)} if args.mixed else None,
-        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps_on_two_streams": not args.serial_sweeps},
+        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps": 1,
+                   "decimation_check_every": 4},
         "solve_model": {"mean_decimation_iters": nbar, "flop_per_solve": flop_solve,
                         "solve_tflops": flop_solve * value / 1e12},
         "cpu_baseline": cpu,

@@ -424,6 +424,92 @@ void mixed_caroli(nt_ctx* c, int nl, int nr, const RgfInput<double>& in, const R
                   const double* zd, const float* zd32, int P, RgfOutput<double> out, Arena& ar, size_t off,
                   int* accept, int* counters);
 
+
+// device address and size of the allocation-free solve's static scratch of
+// this device (null before nt_create has looked it up on that device)
+void* scratch_region(int device, size_t* bytes);
+
+inline int check_params(const nt_params* p)
+{
+    if (!p || p->max_iter < 1 || !(p->convergence > 0.0) || (p->eta_mode != 0 && p->eta_mode != 1)) return NT_ERR_ARG;
+    if (p->precision != NT_PREC_FP64 && p->precision != NT_PREC_MIXED) return NT_ERR_ARG;
+    if (p->circular && (p->n_bloch_x < 1 || p->n_bloch_y < 1)) return NT_ERR_ARG;
+    if (p->dos_mode < NT_DOS_FROM_ETA || p->dos_mode > NT_DOS_TB) return NT_ERR_ARG;
+    return 0;
+}
+
+// circular decimation needs both lead sizes divisible by the number of copies
+inline int check_circular(const nt_params* p, int nl, int nr)
+{
+    if (!p->circular) return 0;
+    const int nb = p->n_bloch_x * p->n_bloch_y;
+    if (nl % nb || nr % nb || (nl / p->n_bloch_y) % p->n_bloch_x) return NT_ERR_ARG;
+    return 0;
+}
+
+inline unsigned nblk(long n, int t = 256) { return (unsigned)std::max<long>(1, (n + t - 1) / t); }
+
+inline int first_status_host(const int* st, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (st[i]) return st[i];
+    return 0;
+}
+
+template <class F>
+inline int guarded(nt_ctx* c, F&& f)
+{
+    if (!c) return NT_ERR_ARG;
+    try {
+        hipcheck(c, hipSetDevice(c->device), "hipSetDevice");
+        c->regions.clear();
+        reg_region(c, c->ws, c->ws_bytes);
+        size_t sb = 0;
+        if (void* sp = scratch_region(c->device, &sb)) reg_region(c, sp, sb);
+        int r = f();
+        if (c->prof) harvest_profile(c);
+        return r;
+    } catch (const Fail& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        c->err = "host allocation failed";
+        return NT_ERR_NOMEM;
+    }
+}
+
+// device buffer RAII for the host-pointer entry points
+struct DevBuf {
+    nt_ctx* c;
+    void* p = nullptr;
+    DevBuf(nt_ctx* c_, size_t bytes) : c(c_)
+    {
+        if (bytes) hipcheck(c, hipMalloc(&p, bytes), "hipMalloc");
+        reg_region(c, p, bytes);
+    }
+    ~DevBuf()
+    {
+        if (p) {
+            unreg_region(c, p);
+            (void)hipFree(p);
+        }
+    }
+    template <typename P>
+    P* as()
+    {
+        return static_cast<P*>(p);
+    }
+};
+
+inline void* up(nt_ctx* c, DevBuf& d, const void* h, size_t bytes)
+{
+    if (h && bytes) hipcheck(c, hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+    return d.p;
+}
+inline void down(nt_ctx* c, void* h, const DevBuf& d, size_t bytes)
+{
+    if (h && bytes) hipcheck(c, hipMemcpyAsync(h, d.p, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+}
+
 }  // namespace nth
 
 using namespace nth;

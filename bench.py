@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--no-configs", dest="configs", action="store_false",
                     help="skip the full-size wire (b=96, L=20, 512 omega, with modes) and ribbon (b=80, L=60, "
                          "2048 E) legs")
+    ap.add_argument("--no-graph", dest="graph", action="store_false",
+                    help="skip the graph-replay leg (a small device solved eagerly and as a captured HIP graph)")
     ap.add_argument("--no-tb-hk", dest="tb_hk", action="store_false",
                     help="skip the tight-binding H(k) assembly measurement")
     ap.add_argument("--tb-hk-only", action="store_true",
@@ -279,11 +281,13 @@ def main():
         print(json.dumps({"tb_hk_assembly": run_tb_hk()}))
         eng.close()
         return
-    tuned = {}
+    tuned, gtuned = {}, {}
     if args.autotune:
         for prec in ("fp64", "mixed"):
             tuned[prec] = eng.autotune_inverse(blocks[0].b, args.points, prec)
-            log(f"inverse panel width for b={blocks[0].b} {prec}: {tuned[prec][0]} ({tuned[prec][1]:.2f} ms)")
+            gtuned[prec] = eng.autotune_gemm(blocks[0].b, args.points, prec)
+            log(f"inverse panel width for b={blocks[0].b} {prec}: {tuned[prec][0]} ({tuned[prec][1]:.2f} ms); "
+                f"gemm tile {gtuned[prec][0]} ({gtuned[prec][1]:.3f} ms)")
 
     mixed_counts = {}
 
@@ -653,6 +657,52 @@ def main():
 
     probes = run_buttiker() if args.buttiker else None
 
+    def run_graph():
+        """the allocation-free FP64 solve of a small device (b = 24, L = 40,
+        64 points: the launch-bound end of the uneven-batching workload)
+        eagerly and as a captured HIP graph (decimation check off, max_iter
+        64); replays are checked against the eager result"""
+        blk = lt.si_ge_superlattice_film(1, 4, 40, (0.125, 0.25), seed=7)
+        d = eng.upload_blocks(blk)
+        p = T.params("phonon", max_iter=64)
+        Ed = torch.as_tensor((16.0 * np.arange(1, 65) / 64) ** 2, dtype=torch.float64, device="cuda")
+        reps = 20
+        eng.set_decimation_check(4)
+        ref = eng.solve_tdb(d, Ed, p, noalloc=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            eng.solve_tdb(d, Ed, p, noalloc=True, stream_sync=False)
+        torch.cuda.synchronize()
+        t_eager = (time.perf_counter() - t0) / reps
+        eng.set_decimation_check(0)
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            eng.solve_tdb(d, Ed, p, noalloc=True, stream_sync=False)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            out = eng.solve_tdb(d, Ed, p, noalloc=True, stream_sync=False)
+        eng.set_decimation_check(4)
+        g.replay()
+        torch.cuda.synchronize()
+        same = bool(torch.equal(out["T_LR"], ref["T_LR"]) and torch.equal(out["T_RL"], ref["T_RL"]))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.replay()
+        torch.cuda.synchronize()
+        t_graph = (time.perf_counter() - t0) / reps
+        log(f"graph: b=24 L=40 64 points, eager {t_eager * 1e3:.2f} ms, graph replay {t_graph * 1e3:.2f} ms, "
+            f"equal {same}")
+        return {"what": "b=24, L=40 superlattice device, 64 frequencies, allocation-free FP64 solve",
+                "eager_ms": t_eager * 1e3, "graph_replay_ms": t_graph * 1e3,
+                "eager_solves_per_s": 64 / t_eager, "graph_solves_per_s": 64 / t_graph,
+                "replay_equals_eager": same, "graph_decimation_iterations": 64}
+
+
+    graph = run_graph() if args.graph else None
     tbhk = run_tb_hk() if args.tb_hk else None
 
     tbrun = run_tb_transport() if args.tb_run else None
@@ -746,7 +796,8 @@ def main():
                               "criterion": "max |T_mixed - T_fp64| / T_fp64 over T > 1e-3 max T, all timed steps",
                               "refinement": mixed_counts,
                               "status_ok": bool(ok64 and okmx)} if args.mixed else None,
-        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()}, "sweeps": 1,
+        "tuning": {"inverse_panel": {k: v[0] for k, v in tuned.items()},
+                   "gemm_tile": {k: v[0] for k, v in gtuned.items()}, "sweeps": 1,
                    "decimation_check_every": 4},
         "solve_model": {"mean_decimation_iters": nbar, "flop_per_solve": flop_solve,
                         "solve_tflops": flop_solve * value / 1e12},
@@ -759,6 +810,8 @@ def main():
         out["mode_resolved"] = modes
     if configs is not None:
         out["workloads"] = configs
+    if graph is not None:
+        out["graph_replay"] = graph
     if probes is not None:
         out["buttiker"] = probes
     if tbhk is not None:

@@ -417,6 +417,23 @@ def main():
             log(f"mass adjoint b=768 L=100 ({'checkpointed' if every else 'full storage'}): "
                 f"{len(f) / el:.1f} gradients/s, {nbytes / 2 ** 20:.0f} MiB per point")
         eng.set_adjoint_checkpoint(0)
+        # the forward solve of the same points in FP64 and in mixed precision
+        fwd = {}
+        for prec in ("fp64", "mixed"):
+            pp = T.params("phonon", precision=prec)
+            eng.solve_tdb(mf.dblocks, mf.E, pp, kidx=mf.kidx)                # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = eng.solve_tdb(mf.dblocks, mf.E, pp, kidx=mf.kidx)
+            el = time.perf_counter() - t0
+            fwd[prec] = (len(f) / el, r["T_LR"].cpu().numpy(), bool((r["status"] == 0).all()))
+        t64, tmx = fwd["fp64"][1], fwd["mixed"][1]
+        sig = t64 > 1e-3 * t64.max()
+        mx_err = float(np.max(np.abs(tmx[sig] - t64[sig]) / t64[sig])) if sig.any() else 0.0
+        out["forward"] = {"fp64_solves_per_s": fwd["fp64"][0], "mixed_solves_per_s": fwd["mixed"][0],
+                          "mixed_max_rel_dT": mx_err, "status_ok": fwd["fp64"][2] and fwd["mixed"][2]}
+        log(f"b=768 L=100 forward: fp64 {fwd['fp64'][0]:.1f}, mixed {fwd['mixed'][0]:.1f} solves/s, "
+            f"mixed max rel dT {mx_err:.2e}")
         out.update(unit="(omega,k)-gradients/s", points=len(f), setup_seconds=tb,
                    what="Si film 8x4 cells (b = 768), L = 100, 3-layer rough Si/Ge interface: T_LR and "
                         "dG/dm of the 768 interface masses, one k-point, FP64")
@@ -469,6 +486,15 @@ def main():
             res[prec] = (npts / el, [o["T_LR"].cpu().numpy() for o in outs],
                          all(bool((o["status"] == 0).all()) for o in outs))
             log(f"c3 {prec}: {npts / el:.1f} solves/s")
+        # the same FP64 call with the groups one after the other on the context
+        p = T.params("phonon")
+        eng.set_multi_workers(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.solve_tdb_multi(devs, p)
+        one_worker = npts / (time.perf_counter() - t0)
+        eng.set_multi_workers(4)
+        log(f"c3 fp64, groups one after the other: {one_worker:.1f} solves/s")
         errs = []
         for a, b in zip(res["fp64"][1], res["mixed"][1]):
             sig = a > 1e-3 * max(1e-300, a.max())
@@ -485,7 +511,8 @@ def main():
             log(f"c3 fp64 b={bs}: {per_b[bs]:.1f} solves/s ({len(sub)} devices)")
         return {"devices": [dict(b=sp["b"], L=sp["L"], period=sp["period"]) for sp in specs],
                 "points": npts, "fp64_solves_per_s": res["fp64"][0], "mixed_solves_per_s": res["mixed"][0],
-                "fp64_solves_per_s_per_block": per_b,
+                "fp64_solves_per_s_per_block": per_b, "workers": 4,
+                "fp64_solves_per_s_one_worker": one_worker,
                 "mixed_max_rel_dT_per_device": errs, "status_ok": res["fp64"][2] and res["mixed"][2],
                 "tuning": tune,
                 "what": f"{len(specs)} devices of the seeded 256-device Si/Ge superlattice draw (block size, "

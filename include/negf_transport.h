@@ -251,12 +251,17 @@ int nt_solve_tdb_batched_noalloc_dev(nt_ctx* ctx, const nt_params* p, int nl, in
 
 /* Several devices of uneven block size b, lead sizes and layer count L in
  * one call (every pointer of a descriptor is device memory, laid out as for
- * nt_solve_tdb_batched_dev).  Devices are solved largest-first (cost
- * b^3 (L + 2 N_dec) per point), each as one batched solve over its points;
- * the inverse panel width is autotuned once per distinct block size.  Each
- * descriptor's status array (may be NULL) receives its points' statuses;
- * returns NT_OK or the first argument/HIP error.  No reference counterpart:
- * the reference solves one device per run. */
+ * nt_solve_tdb_batched_dev).  Devices whose lead buffers, lead sizes and
+ * points (E, k map) are the same form a group whose lead Green's functions
+ * are computed once (FP64).  Groups are taken largest first (cost
+ * b^3 (L + 2 N_dec) per point) by up to nt_set_multi_workers worker
+ * contexts, each with its own stream, workspace and host thread, so that
+ * devices whose launches do not fill the GPU run side by side; the work
+ * starts after the context's stream has reached the call and the context's
+ * stream waits for all of it.  The inverse panel width is autotuned once per
+ * distinct block size.  Each descriptor's status array (may be NULL)
+ * receives its points' statuses; returns NT_OK or the first argument/HIP
+ * error.  No reference counterpart: the reference solves one device per run. */
 typedef struct {
     int nl, nr, b, L, nk, npts;
     const nt_c128 *Haml, *Hamlv, *Hamr, *Hamrv, *Hamd, *Hamdv, *Hamld, *Hamrd;
@@ -269,6 +274,10 @@ typedef struct {
 } nt_device_desc;
 
 int nt_solve_tdb_multi_dev(nt_ctx* ctx, const nt_params* p, int ndev, const nt_device_desc* devs);
+/* Worker contexts of nt_solve_tdb_multi_dev, 1..16 (default 4; 1: every
+ * group on the context itself, one after the other).  No reference
+ * counterpart. */
+int nt_set_multi_workers(nt_ctx* ctx, int n);
 
 /* Host-pointer variant (copies in and out; same layout). */
 int nt_solve_tdb_batched(nt_ctx* ctx, const nt_params* p, int nl, int nr, int b, int L,

@@ -11,7 +11,9 @@
 //   Caroli       T = Re Tr[G Gamma G^+ Gamma]  (negf/lesser.f90:787-807)
 // Shared operands (H(k) blocks) are addressed with stride 0 or a k-index map,
 // so one H(k) copy serves every energy of that k-point.
+#include <atomic>
 #include <mutex>
+#include <thread>
 #include <type_traits>
 
 #include "nt_host.h"
@@ -1476,15 +1478,19 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
             o_pkeep = pa.take(sizeof(nt_c128) * mmr * L * Pmax);
             o_pwork = pa.take(sizeof(nt_c128) * mmr * (ldos_slots(L) - L) * Pmax);
         }
-        if (!a.noalloc || pa.off <= NT_SCRATCH_BYTES) break;
+        // the static scratch, or a workspace cap of the context (worker
+        // contexts of a multi-device call share the device's memory)
+        const size_t cap = a.noalloc ? NT_SCRATCH_BYTES : c->ws_cap;
+        if (cap == 0 || pa.off <= cap) break;
         if (Pmax == 1) {
+            if (!a.noalloc) break;     // one point over a workspace cap: solve it anyway
             char buf[200];
             snprintf(buf, sizeof(buf), "one point needs %zu bytes of workspace, more than the %zu-byte static scratch",
                      pa.off, (size_t)NT_SCRATCH_BYTES);
             c->err = buf;
             throw Fail{NT_ERR_ARG};
         }
-        Pmax = std::max(1, std::min(Pmax - 1, (int)((double)Pmax * NT_SCRATCH_BYTES / pa.off)));
+        Pmax = std::max(1, std::min(Pmax - 1, (int)((double)Pmax * cap / pa.off)));
     }
     if (a.noalloc) {
         ar.base = static_cast<char*>(scratch_ptr(c));
@@ -2028,6 +2034,7 @@ int nt_destroy(nt_ctx* c)
     if (c->ws) (void)hipFree(c->ws);
     for (auto& u : c->circ_u) (void)hipFree(u.d);
     if (c->deg_buf) (void)hipFree(c->deg_buf);
+    for (nt_ctx* s : c->subs) nt_destroy(s);
     if (c->host_flag) (void)hipHostFree(c->host_flag);
     if (c->own) (void)hipStreamDestroy(c->own);
     if (c->aux) (void)hipStreamDestroy(c->aux);
@@ -2344,8 +2351,8 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
         if (!placed) groups.push_back({d});
     }
     std::vector<std::pair<int, int>> tuned;   // (bytes, n) already tuned this call
-    for (const auto& g : groups) {
-        const int bytes = p->precision == NT_PREC_FP64 ? 8 : 4;
+    const int bytes = p->precision == NT_PREC_FP64 ? 8 : 4;
+    for (const auto& g : groups)
         for (int d : g) {
             const nt_device_desc& D = devs[d];
             for (int n : {D.b, std::max(D.nl, D.nr)}) {
@@ -2359,12 +2366,45 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
                 tuned.push_back({bytes, n});
             }
         }
-        const nt_device_desc& D0 = devs[g[0]];
-        const int st = guarded(c, [&] {
-            const size_t z = sizeof(nt_c128);
-            std::vector<TdbArgs> args;
-            args.reserve(g.size());
-            for (int d : g) {
+    const int ng = (int)groups.size();
+    const int W = std::min(std::max(1, c->multi_workers), ng);
+    if (W > 1) {
+        // worker contexts, created once per context and given its settings
+        while ((int)c->subs.size() < W) {
+            nt_ctx* s = nullptr;
+            const int st = nt_create(&s, c->device);
+            if (st) return st;
+            c->subs.push_back(s);
+        }
+        // each worker's workspace within its share of the device memory left
+        // (its own workspace from earlier calls counts as available to it)
+        size_t free_b = 0, total_b = 0;
+        if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess) return NT_ERR_HIP;
+        size_t avail = free_b;
+        for (int w = 0; w < W; ++w) avail += c->subs[w]->ws_bytes;
+        const size_t share = std::max<size_t>(size_t(2) << 30, (size_t)(0.6 * (double)avail / W));
+        for (int w = 0; w < W; ++w) {
+            nt_ctx* s = c->subs[w];
+            s->ws_cap = share;
+            s->overlap = c->overlap;
+            s->gemm_m3 = c->gemm_m3;
+            s->batch_limit = c->batch_limit;
+            s->one_sweep = c->one_sweep;
+            s->dec_check = c->dec_check;
+            s->adj_ckpt = c->adj_ckpt;
+            s->inv_nb = c->inv_nb;
+            s->gemm_tile = c->gemm_tile;
+            s->prof = c->prof;
+            s->err.clear();
+        }
+    }
+    return guarded(c, [&] {
+        const size_t z = sizeof(nt_c128);
+        std::vector<std::vector<TdbArgs>> args(ng);
+        for (int gi = 0; gi < ng; ++gi) {
+            auto& a = args[gi];
+            a.reserve(groups[gi].size());
+            for (int d : groups[gi]) {
                 const nt_device_desc& D = devs[d];
                 const size_t nll = (size_t)D.nl * D.nl, nrr = (size_t)D.nr * D.nr, bb = (size_t)D.b * D.b;
                 reg_region(c, D.E, sizeof(double) * D.npts);
@@ -2381,10 +2421,13 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
                 reg_region(c, D.T_LR, sizeof(double) * D.npts);
                 reg_region(c, D.T_RL, sizeof(double) * D.npts);
                 reg_region(c, D.status, sizeof(int) * D.npts);
-                args.push_back(TdbArgs{D.nl, D.nr, D.b, D.L, D.npts, D.nk, D.E, D.kidx, D.Haml, D.Hamlv, D.Hamr,
-                                       D.Hamrv, D.Hamd, D.Hamdv, D.Hamld, D.Hamrd, D.first_point_mask, D.T_LR,
-                                       D.T_RL, D.status, nullptr, nt_solve_extras{}});
+                a.push_back(TdbArgs{D.nl, D.nr, D.b, D.L, D.npts, D.nk, D.E, D.kidx, D.Haml, D.Hamlv, D.Hamr,
+                                    D.Hamrv, D.Hamd, D.Hamdv, D.Hamld, D.Hamrd, D.first_point_mask, D.T_LR, D.T_RL,
+                                    D.status, nullptr, nt_solve_extras{}});
             }
+            for (size_t i = 1; i < a.size(); ++i) a[0].siblings.push_back(&a[i]);
+            // every k index must name one of the group's nk block sets
+            const nt_device_desc& D0 = devs[groups[gi][0]];
             if (D0.kidx) {
                 std::vector<int> hk(D0.npts);
                 hipcheck(c, hipMemcpyAsync(hk.data(), D0.kidx, sizeof(int) * D0.npts, hipMemcpyDeviceToHost,
@@ -2396,12 +2439,92 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
                         return (int)NT_ERR_ARG;
                     }
             }
-            for (size_t i = 1; i < args.size(); ++i) args[0].siblings.push_back(&args[i]);
-            solve_tdb_dev(c, p, args[0]);
+        }
+        if (W <= 1) {
+            for (int gi = 0; gi < ng; ++gi) solve_tdb_dev(c, p, args[gi][0]);
             return 0;
-        });
-        if (st < 0 && st != NT_ERR_NAN && st != NT_ERR_NOT_CONVERGED) return st;
-    }
+        }
+        // Groups go to W worker contexts, largest first, each worker taking
+        // the next group when it is done: every worker solves on its own
+        // stream with its own workspace from its own host thread, so small
+        // devices, whose launches do not fill the GPU, run side by side.
+        // The workers start after the caller's stream has reached this call
+        // and the caller's stream waits for all of them.
+        hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
+        for (int w = 0; w < W; ++w) hipcheck(c, hipStreamWaitEvent(c->subs[w]->stream, c->fork, 0), "wait fork");
+        std::atomic<int> next{0};
+        std::mutex mu;
+        int fail = 0;
+        std::string fail_err;
+        auto work = [&](nt_ctx* s) {
+            try {
+                hipcheck(s, hipSetDevice(s->device), "hipSetDevice");
+                s->regions = c->regions;                       // the caller's buffers
+                if (s->ws) reg_region(s, s->ws, s->ws_bytes);
+                for (;;) {
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        if (fail) break;
+                    }
+                    const int gi = next.fetch_add(1);
+                    if (gi >= ng) break;
+                    solve_tdb_dev(s, p, args[gi][0]);
+                }
+                if (s->prof) harvest_profile(s);
+            } catch (const Fail& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!fail) {
+                    fail = e.code;
+                    fail_err = s->err;
+                }
+            } catch (const std::bad_alloc&) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!fail) {
+                    fail = NT_ERR_NOMEM;
+                    fail_err = "host allocation failed";
+                }
+            } catch (...) {
+                // nothing may leave a worker thread
+                std::lock_guard<std::mutex> lk(mu);
+                if (!fail) {
+                    fail = NT_ERR_HIP;
+                    fail_err = "unexpected error in a worker context";
+                }
+            }
+        };
+        std::vector<std::thread> threads;
+        threads.reserve(W);
+        for (int w = 0; w < W; ++w) threads.emplace_back(work, c->subs[w]);
+        for (auto& t : threads) t.join();
+        for (int w = 0; w < W; ++w) {
+            nt_ctx* s = c->subs[w];
+            hipcheck(c, hipEventRecord(s->join, s->stream), "event join");
+            hipcheck(c, hipStreamWaitEvent(c->stream, s->join, 0), "wait join");
+            for (int k = 0; k < NT_K_COUNT; ++k) {
+                c->launches[k] += s->launches[k];
+                c->ms[k] += s->ms[k];
+                c->flops[k] += s->flops[k];
+                s->launches[k] = 0;
+                s->ms[k] = 0;
+                s->flops[k] = 0;
+            }
+            for (int k = 0; k < 3; ++k) {
+                c->mixed_counts[k] += s->mixed_counts[k];
+                s->mixed_counts[k] = 0;
+            }
+        }
+        if (fail) {
+            c->err = fail_err;
+            throw Fail{fail};
+        }
+        return 0;
+    });
+}
+
+int nt_set_multi_workers(nt_ctx* c, int n)
+{
+    if (!c || n < 1 || n > 16) return NT_ERR_ARG;
+    c->multi_workers = n;
     return NT_OK;
 }
 

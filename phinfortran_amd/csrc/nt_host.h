@@ -33,6 +33,7 @@ struct nt_ctx {
     int batch_limit = 512;
     void* ws = nullptr;
     size_t ws_bytes = 0;
+    size_t ws_cap = 0;             // > 0: chunks of points shrink until their workspace fits
     int* host_flag = nullptr;      // pinned
     // profiling
     bool prof = false;
@@ -79,6 +80,10 @@ struct nt_ctx {
     // degenerate-group descriptors and statuses of the path_mode = 1 rotation
     void* deg_buf = nullptr;
     size_t deg_bytes = 0;
+    // uneven-device calls: worker contexts (own stream and workspace, one host
+    // thread each) that solve groups of devices concurrently
+    int multi_workers = 4;
+    std::vector<nt_ctx*> subs;
 };
 
 namespace nth {

@@ -407,12 +407,14 @@ template <typename T, int NB, int RPT, int MRT>
 hipError_t launch_inv_t(const InvArgs<T>& a, hipStream_t s)
 {
     const size_t sm = inv_smem_bytes<T, NB>(a.n);
-    static bool configured = false;
-    if (!configured) {
+    // once per instantiation (a function-local static is initialised once,
+    // also when several host threads launch concurrently)
+    static const bool configured = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_inv_gj<T, NB, RPT, MRT>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        configured = true;
-    }
+        return true;
+    }();
+    (void)configured;
     hipLaunchKernelGGL((k_inv_gj<T, NB, RPT, MRT>), dim3(a.batch), dim3(NT), sm, s, a);
     return hipGetLastError();
 }

@@ -346,12 +346,12 @@ hipError_t launch_kpc_eig(const KpcArgs& a, const int* cm, int m0, int c0, int n
                           double* vout, hipStream_t s)
 {
     if (nc <= 0) return hipSuccess;
-    static bool configured = false;
-    if (!configured) {
+    static const bool configured = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kpc_eig),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kpc_eig_smem(KPC_MAX_N));
-        configured = true;
-    }
+        return true;
+    }();
+    (void)configured;
     hipLaunchKernelGGL(k_kpc_eig, dim3(nc), dim3(NT_K), kpc_eig_smem(a.n), s, a, cm, m0, c0, nc, D, found, vout);
     return hipGetLastError();
 }

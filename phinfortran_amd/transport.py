@@ -390,10 +390,11 @@ class Engine:
             res[name] = t if t.dim() == 1 else t.transpose(-1, -2)
         return dict(T_LR=T_LR, T_RL=T_RL, status=status, iters=iters, **res)
 
-    def solve_tdb_multi(self, devices, p: N.NtParams):
+    def solve_tdb_multi(self, devices, p: N.NtParams, stream_sync=True):
         """Several devices (uneven b, L, leads) in one call.  devices: list of
         dicts with 'blocks' (from upload_blocks), 'E' and optional 'kidx',
-        'first_point_mask'.  Returns a list of dict(T_LR, T_RL, status)."""
+        'first_point_mask'.  Returns a list of dict(T_LR, T_RL, status); with
+        stream_sync=False they are ready on torch's current stream."""
         import torch
         dev = torch.device("cuda", self.device)
         descs = (N.NtDeviceDesc * max(1, len(devices)))()
@@ -436,7 +437,8 @@ class Engine:
             self._L.nt_set_stream(self.ctx, C.c_void_p(torch.cuda.current_stream(dev).cuda_stream))
         self._check(self._L.nt_solve_tdb_multi_dev(self.ctx, C.byref(p), len(devices), C.cast(descs, C.c_void_p)),
                     "solve_tdb_multi")
-        torch.cuda.synchronize(dev)
+        if stream_sync:
+            torch.cuda.synchronize(dev)
         return outs
 
     MODE_FIELDS = ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "G_D_RL", "G_D_LR", "G_L", "G_R")
@@ -602,6 +604,11 @@ class Engine:
         (default 4); 0 = never (max_iter iterations, no host synchronisation
         inside the FP64 batched solve, which can then be captured in a graph)"""
         self._check(self._L.nt_set_decimation_check(self.ctx, int(every)), "nt_set_decimation_check")
+
+    def set_multi_workers(self, n: int):
+        """uneven-device calls: groups of devices solved by n worker contexts
+        side by side (default 4; 1 = one after the other on this context)"""
+        self._check(self._L.nt_set_multi_workers(self.ctx, int(n)), "nt_set_multi_workers")
 
     def set_adjoint_checkpoint(self, every: int):
         """adjoint storage: every layer (0), every n-th (n), about sqrt (-1)"""

@@ -40,3 +40,44 @@ def test_multi_device_equals_single(engine, T):
         for j in (0, 4):
             tl, tr, _ = O.solve_tdb(blocks[i][d["kidx"][j]].as_dict(), d["E"][j], op)
             assert abs(o["T_LR"][j].item() - tl) < 1e-9 * max(1, tl)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_multi_device_workers(engine, T, precision):
+    """Twelve devices of four block sizes (groups sharing leads among them):
+    the groups solved side by side by four worker contexts give bitwise the
+    results of one after the other on the context itself, the caller's
+    stream sees them without a further synchronisation, and the profile
+    counts the workers' launches."""
+    import torch
+    from phinfortran_amd import _native as N
+    p = T.params("phonon", precision=precision)
+    ks = lt.mp_grid(2, 2)
+    f = 16.0 * np.arange(1, 17) / 16
+    devices = []
+    for i, n in enumerate([1, 2, 3, 1, 2, 3, 1, 2, 1, 4, 1, 2]):
+        blk = [lt.si_ge_superlattice_film(n, 2 + i % 3, 6 + 2 * i, kf, seed=40 + i) for kf in ks]
+        devices.append(dict(blocks=engine.upload_blocks(blk), E=np.tile(f ** 2, 4),
+                            kidx=np.repeat(np.arange(4, dtype=np.int32), f.size)))
+    print(f"multi-device workers, {precision}", flush=True)
+    try:
+        engine.set_multi_workers(1)
+        ref = engine.solve_tdb_multi(devices, p)
+        engine.set_multi_workers(4)
+        engine.profile_reset()
+        engine.profile(True)
+        got = engine.solve_tdb_multi(devices, p, stream_sync=False)
+        # read on the caller's stream: ordered after the workers by the call
+        tl = [o["T_LR"].clone() for o in got]
+        torch.cuda.synchronize()
+        engine.profile(False)
+        prof = engine.profile_get(N.NT_K_GEMM)
+    finally:
+        engine.set_multi_workers(4)
+        engine.profile(False)
+    assert prof["launches"] > 0 and prof["ms"] > 0
+    for i, (a, b, t) in enumerate(zip(ref, got, tl)):
+        assert (b["status"].cpu().numpy() == 0).all(), i
+        for k in ("T_LR", "T_RL"):
+            assert torch.equal(a[k], b[k]), (i, k)
+        assert torch.equal(t, a["T_LR"]), i

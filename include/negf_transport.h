@@ -597,6 +597,12 @@ int nt_adjoint_bytes_per_point(nt_ctx* ctx, int nl, int nr, int b, int L, int lo
  * products ((Ar+Ai)(Br+Bi) - ArBr - AiBi for the imaginary part) instead of
  * four; on = 0 (default): four. */
 int nt_set_gemm_3m(nt_ctx* ctx, int on);
+/* Debugging aid, no counterpart in the reference: on = 1 fills the
+ * workspace (and, for nt_solve_tdb_batched_noalloc_dev, the static scratch)
+ * with all-ones words, a NaN in every double, at the start of each call, so
+ * that a read of workspace the call has not written shows up as a NaN
+ * status or output; on = 0 (default): no fill. */
+int nt_set_workspace_poison(nt_ctx* ctx, int on);
 
 /* ------------------------------------------------ batched building blocks
  * The two kernels every solve is made of, on device memory (column-major,

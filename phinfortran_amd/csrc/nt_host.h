@@ -34,7 +34,9 @@ struct nt_ctx {
     void* ws = nullptr;
     size_t ws_bytes = 0;
     size_t ws_cap = 0;             // > 0: chunks of points shrink until their workspace fits
-    int* host_flag = nullptr;      // pinned
+    bool ws_poison = false;        // fill the workspace with NaN bit patterns before each use
+    int* host_flag = nullptr;      // pinned, two slots (the decimation's pipelined checks)
+    hipEvent_t chk[2] = {nullptr, nullptr};
     // profiling
     bool prof = false;
     struct Rec {
@@ -236,20 +238,34 @@ inline void check_range(nt_ctx* c, const void* p, size_t bytes, const char* what
     throw Fail{NT_ERR_ARG};
 }
 
+// all-ones words (a NaN in every double, -1 in every int) over [p, p + bytes)
+inline void poison_dev(nt_ctx* c, void* p, size_t bytes, hipStream_t s)
+{
+    int* w = static_cast<int*>(p);
+    for (size_t n = bytes / 4; n > 0;) {
+        const int k = (int)std::min<size_t>(n, size_t(1) << 30);
+        hipcheck(c, launch_fill_int(w, -1, k, s), "poison");
+        w += k;
+        n -= k;
+    }
+}
+
 inline void ensure_ws(nt_ctx* c, size_t bytes)
 {
-    if (c->ws_bytes >= bytes) return;
-    if (c->ws) {
-        hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        unreg_region(c, c->ws);
-        hipcheck(c, hipFree(c->ws), "hipFree");
-        c->ws = nullptr;
-        c->ws_bytes = 0;
+    if (c->ws_bytes < bytes) {
+        if (c->ws) {
+            hipcheck(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            unreg_region(c, c->ws);
+            hipcheck(c, hipFree(c->ws), "hipFree");
+            c->ws = nullptr;
+            c->ws_bytes = 0;
+        }
+        size_t want = bytes + bytes / 8;
+        hipcheck(c, hipMalloc(&c->ws, want), "hipMalloc workspace");
+        c->ws_bytes = want;
+        reg_region(c, c->ws, c->ws_bytes);
     }
-    size_t want = bytes + bytes / 8;
-    hipcheck(c, hipMalloc(&c->ws, want), "hipMalloc workspace");
-    c->ws_bytes = want;
-    reg_region(c, c->ws, c->ws_bytes);
+    if (c->ws_poison) poison_dev(c, c->ws, bytes, c->stream);
 }
 
 template <typename T, typename M>

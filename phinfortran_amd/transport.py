@@ -622,8 +622,15 @@ class Engine:
         return int(out.value)
 
     def set_overlap(self, on: bool):
-        """Run the two RGF sweeps on two streams (default) or serially."""
+        """With both sweeps (set_sweeps(2)): run them on two streams (default)
+        or on one."""
         self._check(self._L.nt_set_overlap(self.ctx, int(bool(on))), "nt_set_overlap")
+
+    def set_workspace_poison(self, on: bool):
+        """Debugging aid: fill the workspace with NaN bit patterns at the start
+        of every call, so that a read of workspace the call did not write
+        shows as a NaN status or output."""
+        self._check(self._L.nt_set_workspace_poison(self.ctx, int(bool(on))), "nt_set_workspace_poison")
 
     def set_inverse_panel(self, n: int, nb: int, precision: str = "fp64"):
         prec = {"fp64": N.NT_PREC_FP64, "mixed": N.NT_PREC_MIXED}[precision]

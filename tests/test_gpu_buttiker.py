@@ -119,10 +119,17 @@ def test_buttiker_equilibrium_and_conservation(engine, T, b, L):
     contact currents otherwise (energy-current conservation).  Both hold up
     to the device broadening 10 eta(E), which removes current without a
     source term, hence a vanishing eta here (the currents it leaves are
-    O(eta / gamma) of the terms, gamma the probe broadening)."""
+    O(eta / gamma) of the terms, gamma the probe broadening).
+
+    The energies avoid the few where the lead decimation of these random
+    leads meets a nearly singular z - e1 at this eta: there the reference
+    algorithm itself (the oracle's decimation) ends in NaN, and the outcome
+    turns on rounding (E = 0.1533 with b = 96 converges or not depending on
+    the inverse's panel width).  On this grid the oracle's decimation
+    converges at eta = 1e-8, 1e-10 and (b = 96) 1e-12."""
     blk = lt.random_blocks(b, b, L, seed=11, scale=0.6)
     p = T.params("phonon", eta=1e-10, eta0=1e-11, convergence=1e-12, max_iter=4000)
-    E = np.linspace(0.1, 0.9, 16)
+    E = np.linspace(0.12, 0.88, 16)
     B = T.B_PROBE * 1e4
     print(f"buttiker properties b={b} L={L}", flush=True)
     r = run(engine, T, blk, E, p, np.full(L, 300.0), 300.0, 300.0, B)

@@ -1,0 +1,125 @@
+"""No entry reads workspace it has not written: every solve path run with the
+workspace filled with NaN bit patterns first (nt_set_workspace_poison) gives
+the statuses and values of the same call without the fill.
+
+Statuses and iteration counts must be equal; values equal to 1e-12 of their
+size (the probe traces accumulate with atomics, whose order may differ)."""
+import numpy as np
+import pytest
+
+from phinfortran_amd import lattice as lt
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def T():
+    from phinfortran_amd import transport
+    return transport
+
+
+@pytest.fixture
+def poison_off(engine):
+    yield
+    engine.set_workspace_poison(False)
+    engine.set_sweeps(1)
+
+
+def _same(a, b, what):
+    import torch
+    for k in a:
+        if not torch.is_tensor(a[k]):
+            continue
+        x, y = a[k].cpu().numpy(), b[k].cpu().numpy()
+        if k in ("status", "iters") or x.dtype.kind in "iu":
+            assert np.array_equal(x, y), (what, k, x, y)
+            continue
+        assert np.isfinite(y).all() == np.isfinite(x).all(), (what, k)
+        sc = max(1e-300, np.abs(x[np.isfinite(x)]).max(initial=0.0))
+        assert np.abs(x - y).max(initial=0.0) <= 1e-12 * sc, (what, k, np.abs(x - y).max())
+
+
+def _both(engine, what, call):
+    engine.set_workspace_poison(False)
+    ref = call()
+    engine.set_workspace_poison(True)
+    print(f"poisoned workspace: {what}", flush=True)
+    got = call()
+    engine.set_workspace_poison(False)
+    _same(ref, got, what)
+    return ref
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_solve_paths(engine, T, poison_off, sweeps, precision):
+    blocks = [lt.random_blocks(12, 12, 5, seed=70 + i, scale=0.6) for i in range(2)]
+    d = engine.upload_blocks(blocks)
+    E = np.linspace(0.05, 0.9, 11)
+    kidx = (np.arange(11) % 2).astype(np.int32)
+    mask = np.zeros(11, np.int32)
+    mask[0] = 1
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, precision=precision)
+    engine.set_sweeps(sweeps)
+    ex = ("gl_adv", "gr_ret", "G_D_RL", "G_L", "sdos_l")
+    r = _both(engine, f"solve sweeps={sweeps} {precision}",
+              lambda: engine.solve_tdb(d, E, p, kidx=kidx, first_point_mask=mask, extras=ex))
+    assert (r["status"].cpu().numpy() == 0).all()
+    _both(engine, f"solve noalloc sweeps={sweeps} {precision}",
+          lambda: engine.solve_tdb(d, E, p, kidx=kidx, noalloc=True))
+
+
+def test_device_dos(engine, T, poison_off):
+    blk = lt.random_blocks(10, 10, 6, seed=3)
+    d = engine.upload_blocks(blk)
+    p = T.params("tb", eta=0.01)
+    _both(engine, "ldos", lambda: engine.solve_tdb(d, np.linspace(-0.8, 0.8, 9), p, extras=("ldos", "ddos")))
+
+
+@pytest.mark.parametrize("b,L,npts", [(8, 4, 7), (96, 12, 16)])
+def test_buttiker(engine, T, poison_off, b, L, npts):
+    blk = lt.random_blocks(b, b, L, seed=11, scale=0.6)
+    d = engine.upload_blocks(blk)
+    p = T.params("phonon", eta=1e-6, eta0=1e-7, convergence=1e-12, max_iter=4000)
+    E = np.linspace(0.12, 0.88, npts)
+    temp = np.linspace(290.0, 310.0, L)
+    r = _both(engine, f"buttiker b={b} L={L}",
+              lambda: engine.solve_buttiker(d, E, p, temp, 320.0, 280.0, B_probe=T.B_PROBE * 1e4, T_probe=430.0))
+    assert (r["status"].cpu().numpy() == 0).all()
+
+
+def test_adjoint_modes_surface(engine, T, poison_off):
+    blk = lt.random_blocks(6, 6, 5, seed=21, scale=0.6)
+    blk.Hamr[...] = blk.Haml
+    blk.Hamrv[...] = blk.Hamlv.conj().T
+    d = engine.upload_blocks(blk)
+    p = T.params("phonon", eta=1e-6, eta0=1e-7, convergence=1e-13, max_iter=3000)
+    E = np.array([0.12, 0.3, 0.45, 0.58])
+    _both(engine, "adjoint", lambda: engine.solve_tdb_adjoint(d, E, p, np.ones(4), 1, 3))
+    _both(engine, "modes", lambda: engine.solve_modes(d, E, p, 1.3, 1.3))
+
+    def surface():
+        import torch
+        out = {}
+        for i, e in enumerate(E):
+            g = engine.surface_green_tdb(blk.Haml, blk.Hamr, blk.Hamlv, blk.Hamrv, e, p)
+            for k, v in g.items():
+                if isinstance(v, np.ndarray):
+                    out[f"{k}{i}"] = torch.as_tensor(v)
+        return out
+    _both(engine, "surface green", surface)
+
+
+def test_multi_device(engine, T, poison_off):
+    ks = lt.mp_grid(2, 2)
+    devices = []
+    for i, n in enumerate([1, 2, 1]):
+        blk = [lt.si_ge_superlattice_film(n, 2, 4 + i, kf, seed=60 + i) for kf in ks]
+        devices.append(dict(blocks=engine.upload_blocks(blk), E=np.linspace(1.0, 200.0, 8),
+                            kidx=(np.arange(8) % 4).astype(np.int32)))
+    p = T.params("phonon")
+
+    def run():
+        outs = engine.solve_tdb_multi(devices, p)
+        return {f"{k}{i}": v for i, o in enumerate(outs) for k, v in o.items()}
+    _both(engine, "multi", run)

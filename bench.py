@@ -231,28 +231,41 @@ def main():
                         "&kpath\n path_mode = 2\n/\n&saving\n output_dir = '', verbosity = 0\n/\n")
             prm = os.path.join(here, "tests", "golden", "tb", "param")
             cfgp = os.path.join(tmp, "confige")
-            DRV.run_tb(cfgp, engine=eng, param_path=prm, dos=False)        # warm-up
+            DRV.run_tb(cfgp, engine=eng, param_path=prm, dos=False, modes=False)        # warm-up
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            r = DRV.run_tb(cfgp, engine=eng, param_path=prm, dos=False)
+            r = DRV.run_tb(cfgp, engine=eng, param_path=prm, dos=False, modes=False)
             torch.cuda.synchronize()
             el = time.perf_counter() - t0
-            DRV.run_tb(cfgp, engine=eng, param_path=prm)                   # warm-up, with the device DOS
+            DRV.run_tb(cfgp, engine=eng, param_path=prm, modes=False)      # warm-up, with the device DOS
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            rd = DRV.run_tb(cfgp, engine=eng, param_path=prm)
+            rd = DRV.run_tb(cfgp, engine=eng, param_path=prm, modes=False)
             torch.cuda.synchronize()
             eld = time.perf_counter() - t0
+            # the reference's whole loop: transmission, device DOS and vel_mat of every point
+            DRV.run_tb(cfgp, engine=eng, param_path=prm)                   # warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rm = DRV.run_tb(cfgp, engine=eng, param_path=prm)
+            torch.cuda.synchronize()
+            elm = time.perf_counter() - t0
         npt = r["transl"].size
         res = {"points_per_s": npt / el, "unit": "(E,k)-points/s", "points": npt, "seconds": el,
                "with_device_dos_points_per_s": npt / eld,
+               "with_dos_and_modes_points_per_s": npt / elm,
+               "modes_max_unitarity_dev": float(np.abs(rm["mode_sums"][..., 0] + rm["mode_sums"][..., 3]
+                                                       - rm["n_prop"][..., 0]).max()),
                "norb": r["norb"], "nl": r["nl"], "layer_list": r["layer_list"],
-               "status_ok": bool((r["status"] == 0).all() and (rd["status"] == 0).all()),
+               "status_ok": bool((r["status"] == 0).all() and (rd["status"] == 0).all()
+                                 and (rm["status"] == 0).all()),
                "what": "whole run: table, H(k) of 16 k-points on the GPU, blockwise_setup (host), device "
                        "slicing, one batched solve (layers padded to the widest), FP64; with the device DOS "
-                       "(get_ldos of every point, as the reference's loop) beside it"}
+                       "(get_ldos of every point) beside it, and with the device DOS and the mode-resolved "
+                       "path (vel_mat of every point: lead modes, t/r, unfolding) as the reference's loop"}
         log(f"tb run: {npt} points in {el:.2f}s ({npt / el:.1f} points/s), layers {r['layer_list']}; "
-            f"with the device DOS {eld:.2f}s ({npt / eld:.1f} points/s)")
+            f"with the device DOS {eld:.2f}s ({npt / eld:.1f} points/s); with DOS and modes {elm:.2f}s "
+            f"({npt / elm:.1f} points/s)")
         if args.cpu_sample > 0:
             import oracle as O
             from oracle import tb_ham as OT

@@ -193,6 +193,18 @@ module negf_transport
             type(c_ptr), value :: seg_off, seg_rc, coef, dr, kp, v, ncand
         end function
 
+        ! find_k_pc_l / find_k_pc_r of the tight-binding run (tb/hamiltonian.f90:666-916):
+        ! the primitive cell's H(k) from an element-wise table (el_off, coef, dr)
+        integer(c_int) function nt_unfold_pc_tb_dev(ctx, nmodes, E, k_xyz, n_bloch_max, reci_uc, recivec_pc, &
+                n_pc, el_off, nent, coef, dr, kp, v, ncand) bind(C, name="nt_unfold_pc_tb_dev")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: nmodes, n_bloch_max, n_pc, nent
+            type(c_ptr), value :: E, k_xyz
+            real(c_double), intent(in) :: reci_uc(9), recivec_pc(9)
+            type(c_ptr), value :: el_off, coef, dr, kp, v, ncand
+        end function
+
         ! vel_mat_tdb with the Brillouin-zone unfolding (velocity_mat.f90:2020-2130, :3032-3099)
         integer(c_int) function nt_modes_unfold_tdb_batched_dev(ctx, p, tb, nl, nr, npts, E, kidx, nk, Haml, &
                 Hamlv, Hamr, Hamrv, gl_adv, gr_ret, gl_plus, gr_minus, gml_adv, gmr_ret, G_D_RL, G_D_LR, G_L, &

@@ -500,6 +500,27 @@ int nt_unfold_pc_dev(nt_ctx* ctx, int nmodes, const double* E, const double* k_x
                      const int* seg_rc, int nent, const double* coef, const double* dr, double* kp, double* v,
                      int* ncand);
 
+/* find_k_pc_l / find_k_pc_r of the tight-binding run (tb/hamiltonian.f90:
+ * 666-916), as its vel_mat calls them for every propagating mode
+ * (tb/velocity_mat.f90:593-605 and mirrors): the candidates of
+ * nt_unfold_pc_dev; per candidate k the primitive cell's H(k) = hermitian(
+ * sum V exp(i k.r)) (gen_ham_pc_l, :918-989), its eigenpairs and the velocities
+ * Re(z^H HF_d z), HF_d = i antihermitian(sum V r_d exp(i k.r)) (gen_hf_pc_l,
+ * :1238-1313; group_velocity_pc_l, :1316-1378); a candidate matches when a
+ * band lies within 1e-4 of E (compare_evs_l, :1596-1686; within 1e-3 for the
+ * mode's own k on the unit cell's zone boundary).
+ *   E        nmodes: the point's energy (eV) of each mode
+ *   n_pc     orbitals of the primitive cell (<= 48)
+ *   el_off   n_pc^2 + 1 ints: the table entries of matrix element
+ *            (r, c) are el_off[r + c n_pc] .. el_off[r + c n_pc + 1] - 1
+ *   coef, dr nent values V and nent x 3 bond vectors r (Angstrom), one per
+ *            (atom pair, cell image within r_cutoff, orbital pair)
+ * Other arguments and outputs as nt_unfold_pc_dev.  All pointers except
+ * reci_uc and recivec_pc are device memory. */
+int nt_unfold_pc_tb_dev(nt_ctx* ctx, int nmodes, const double* E, const double* k_xyz, int n_bloch_max,
+                        const double* reci_uc, const double* recivec_pc, int n_pc, const int* el_off, int nent,
+                        const double* coef, const double* dr, double* kp, double* v, int* ncand);
+
 /* ------------------------------------------------ H(k) assembly
  * gen_dyn_qe_tdb_k (negf/dyn.f90:1482-2080) on the GPU: one block for nk
  * k-points.  The host tabulates the k-independent part as nseg segments (one

@@ -163,16 +163,19 @@ def specular_rows(M, k_f, k_b):
     return rows
 
 
-def unfold_point(m, U, lam, ksc, recivec, reci_uc_l, reci_uc_r, nbx, nby, positions, setup_l, setup_r):
+def unfold_point(m, U, lam, ksc, recivec, reci_uc_l, reci_uc_r, nbx, nby, positions, setup_l, setup_r, ndims=3):
     """The unfolding of one (omega, k) point: m is oracle.modes.vel_mat_tdb's
     result, U / lam the four eigenvector matrices and eigenvalues (keys
     L_adv, L_ret, R_ret, R_adv), setup_l / setup_r transverse_uc_setup's
-    results.  Returns bz, k_xyz (dicts by mode set) and the split sums."""
+    results (with the same ndims: 3 for phonons, the orbitals per atom of a
+    tight-binding structure whose atoms all carry that many, as in
+    tb/velocity_mat.f90:524-557).  Returns bz, k_xyz (dicts by mode set) and
+    the split sums."""
     kz = dict(L_adv=m["k_z_L_adv"], L_ret=m["k_z_L_ret"], R_ret=m["k_z_R_ret"], R_adv=m["k_z_R_adv"])
     bz, kxyz = {}, {}
     for key in ("L_adv", "L_ret", "R_ret", "R_adv"):
         Ll, pidx, pos = setup_l if key[0] == "L" else setup_r
         reci = reci_uc_l if key[0] == "L" else reci_uc_r
         _, bz[key], kxyz[key] = unfold_direction(U[key], lam[key], kz[key], ksc, recivec, reci, nbx, nby,
-                                                 positions, Ll, pidx, pos)
+                                                 positions, Ll, pidx, pos, ndims=ndims)
     return bz, kxyz, split_sums(m, bz)

@@ -37,7 +37,7 @@ EXPORTS = [
     "nt_set_batch_limit", "nt_last_error", "nt_status_string", "nt_surface_green_tdb",
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
     "nt_solve_tdb_batched_noalloc_dev",
-    "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_hk_assemble_orb_dev", "nt_autotune_inverse", "nt_set_inverse_panel", "nt_autotune_gemm", "nt_set_gemm_tile",
+    "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_unfold_pc_tb_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_hk_assemble_orb_dev", "nt_autotune_inverse", "nt_set_inverse_panel", "nt_autotune_gemm", "nt_set_gemm_tile",
     "nt_set_overlap", "nt_set_gemm_3m", "nt_set_workspace_poison",
     "nt_set_adjoint_checkpoint", "nt_adjoint_bytes_per_point", "nt_set_sweeps", "nt_set_decimation_check", "nt_set_multi_workers", "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters", "nt_mixed_counts",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
@@ -163,6 +163,7 @@ def lib():
     L.nt_modes_tdb_batched_dev.argtypes = [P, pp, I, I, I, I, P, P, I] + [P] * 14 + [D, D, P,
                                                                               C.POINTER(NtModesOut), P]
     L.nt_unfold_pc_dev.argtypes = [P, I, P, P, I, C.POINTER(D), C.POINTER(D), I, I, P, P, I, P, P, P, P, P]
+    L.nt_unfold_pc_tb_dev.argtypes = [P, I, P, P, I, C.POINTER(D), C.POINTER(D), I, P, I, P, P, P, P, P]
     L.nt_modes_unfold_tdb_batched_dev.argtypes = [P, pp, I, I, I, I, P, P, I] + [P] * 14 + [
         D, D, P, C.POINTER(NtModesOut), C.POINTER(NtUnfoldDesc), C.POINTER(NtUnfoldOut), P]
     L.nt_autotune_inverse.argtypes = [P, I, I, I, C.POINTER(I), C.POINTER(D)]

@@ -130,7 +130,8 @@ constexpr double KPC_DEVI = 1.0e-5;      // central-difference step (dyn.f90:548
 constexpr int KPC_MAX_SWEEPS = 30;
 constexpr int KPC_MAX_N = 48;           // orbitals of the primitive cell
 struct KpcArgs {
-    int n;                    // 3 x atoms of the primitive cell
+    int n;                    // 3 x atoms of the primitive cell (orbitals: tight binding)
+    int tb;                   // 1: tight-binding conventions (find_k_pc_l, tb/hamiltonian.f90:666-822)
     const double* E;
     const double* k;
     double reci_uc[9], recivec_pc[9];
@@ -149,6 +150,10 @@ hipError_t launch_kpc_eig(const KpcArgs& a, const int* cm, int m0, int c0, int n
                           double* vout, hipStream_t s);
 hipError_t launch_kpc_select(const KpcArgs& a, int m0, int M, const int* off, const int* found, const double* vc,
                              const double* kc, hipStream_t s);
+// tight binding: H(k) and the three velocity sums of every candidate from an
+// element-wise table (slots 0..3 of the 7-slot layout), candidate k into kc
+hipError_t launch_kpc_tb_mats(const KpcArgs& a, const int* cm, const int* cs, int c0, int nc, const int* el_off,
+                              const double* coef, const double* dr, double* D, double* kc, hipStream_t s);
 
 // H(k) phase sums of the QE force-constant path (nt_hk.hip): out[k] (rows x cols,
 // leading dimension ld, k stride kstride complex elements) = sum over the

@@ -16,6 +16,14 @@
 // matching candidate of smallest |k| (first on ties, find_min_dist), its v the
 // first matching candidate's velocity (velocity_mat.f90:2088-2089); zero when
 // nothing matches.
+//
+// Tight binding (find_k_pc_l / _r, tb/hamiltonian.f90:666-916): the same
+// candidates; per candidate the primitive cell's H(k) (gen_ham_pc_l,
+// :918-989) and HF_d(k) = sum V r_d exp(i k.r) (gen_hf_pc_l, :1238-1313)
+// from an element-wise table, hermitian(H) diagonalised, velocities
+// Re(z^H [i antihermitian(HF_d)] z) (group_velocity_pc_l, :1316-1378); a band
+// matches within 1e-4 of E (compare_evs_l, :1596-1686), within 1e-3 for the
+// unit cell's own k on its zone boundary.
 #include "nt_kernels.h"
 
 namespace nt {
@@ -103,6 +111,45 @@ __global__ void k_kpc_klist(KpcArgs a, const int* cm, const int* cs, int c0, int
         o[1] = q[1];
         o[2] = q[2];
         if (s > 0) o[(s - 1) / 2] += (s % 2 ? -KPC_DEVI : KPC_DEVI);
+    }
+}
+
+// Tight binding: thread per (element e = r + c n, candidate) sums the table
+// entries of the element: H += V e^{i k.r}, HF_d += V r_d e^{i k.r}.
+__global__ void k_kpc_tb_mats(KpcArgs a, const int* cm, const int* cs, int c0, int nc, const int* el_off,
+                              const double* coef, const double* dr, double* D, double* kc)
+{
+    const int cl = blockIdx.y;
+    const int n = a.n;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cl >= nc || e >= (long)n * n) return;
+    const int c = c0 + cl;
+    const double* q = a.cand + ((long)cm[c] * a.maxc + cs[c]) * 3;
+    if (e == 0) {
+        kc[3L * c] = q[0];
+        kc[3L * c + 1] = q[1];
+        kc[3L * c + 2] = q[2];
+    }
+    double hr = 0.0, hi = 0.0, fr[3] = {0.0, 0.0, 0.0}, fi[3] = {0.0, 0.0, 0.0};
+    for (int j = el_off[e]; j < el_off[e + 1]; ++j) {
+        const double* d = dr + 3L * j;
+        double sn, cs_;
+        sincos(d[0] * q[0] + d[1] * q[1] + d[2] * q[2], &sn, &cs_);
+        const double v = coef[j];
+        hr += v * cs_;
+        hi += v * sn;
+        for (int k = 0; k < 3; ++k) {
+            fr[k] += v * d[k] * cs_;
+            fi[k] += v * d[k] * sn;
+        }
+    }
+    double* D0 = D + 2L * 7 * cl * n * n;
+    D0[2 * e] = hr;
+    D0[2 * e + 1] = hi;
+    for (int k = 0; k < 3; ++k) {
+        double* Dk = D0 + 2L * (1 + k) * n * n;
+        Dk[2 * e] = fr[k];
+        Dk[2 * e + 1] = fi[k];
     }
 }
 
@@ -231,6 +278,50 @@ __global__ void __launch_bounds__(NT_K) k_kpc_eig(KpcArgs a, const int* cm, int 
         for (int j = 0; j < n; ++j) ev[j] = Ar[perm[j] * n + perm[j]];
     }
     __syncthreads();
+    // tight binding: Re(z^H M z), M = i antihermitian(HF_d):
+    // M(r, q) = i (HF(r, q) - conj HF(q, r)) / 2
+    if (a.tb) {
+        for (int j = t; j < n; j += NT_K) {
+            const int pj = perm[j];
+            for (int d = 0; d < 3; ++d) {
+                const double* F = D0 + 2L * (1 + d) * n * n;
+                double acc = 0.0;
+                for (int r = 0; r < n; ++r) {
+                    double wr = 0.0, wi = 0.0;
+                    for (int q = 0; q < n; ++q) {
+                        const double* x = F + 2 * ((long)r + (long)q * n);
+                        const double* y = F + 2 * ((long)q + (long)r * n);
+                        const double mr = -0.5 * (x[1] + y[1]), mi = 0.5 * (x[0] - y[0]);
+                        const double zr = Vr[q * n + pj], zi = Vi[q * n + pj];
+                        wr += mr * zr - mi * zi;
+                        wi += mr * zi + mi * zr;
+                    }
+                    const double zr = Vr[r * n + pj], zi = Vi[r * n + pj];
+                    acc += zr * wr + zi * wi;
+                }
+                vel[3 * j + d] = acc;
+            }
+        }
+        __syncthreads();
+        if (t == 0) {
+            const double tol = a.count[cm[c]] == -2 ? 1.0e-3 : 1.0e-4;
+            const double E = a.E[m0 + cm[c]];
+            int f = 0;
+            double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+            for (int j = 0; j < n; ++j)
+                if (fabs(E - ev[j]) < tol) {
+                    f = 1;
+                    v0 = vel[3 * j];
+                    v1 = vel[3 * j + 1];
+                    v2 = vel[3 * j + 2];
+                }
+            found[c] = f;
+            vout[3L * c] = v0;
+            vout[3L * c + 1] = v1;
+            vout[3L * c + 2] = v2;
+        }
+        return;
+    }
     // group velocities Re(z^H dD z) / (2 sqrt|e|), dD = (hermitian(D+) - hermitian(D-)) / (2 devi)
     for (int j = t; j < n; j += NT_K) {
         const int pj = perm[j];
@@ -353,6 +444,16 @@ hipError_t launch_kpc_eig(const KpcArgs& a, const int* cm, int m0, int c0, int n
     }();
     (void)configured;
     hipLaunchKernelGGL(k_kpc_eig, dim3(nc), dim3(NT_K), kpc_eig_smem(a.n), s, a, cm, m0, c0, nc, D, found, vout);
+    return hipGetLastError();
+}
+
+hipError_t launch_kpc_tb_mats(const KpcArgs& a, const int* cm, const int* cs, int c0, int nc, const int* el_off,
+                              const double* coef, const double* dr, double* D, double* kc, hipStream_t s)
+{
+    if (nc <= 0) return hipSuccess;
+    const long ne = (long)a.n * a.n;
+    hipLaunchKernelGGL(k_kpc_tb_mats, dim3((unsigned)((ne + 255) / 256), nc), dim3(256), 0, s, a, cm, cs, c0, nc,
+                       el_off, coef, dr, D, kc);
     return hipGetLastError();
 }
 

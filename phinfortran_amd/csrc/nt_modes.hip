@@ -526,6 +526,107 @@ void modes_run(nt_ctx* c, const nt_params* prm, int tb, int nl, int nr, int P, c
     launch(c, NT_K_ELEM, 0.0, "unfold split", [&] { return launch_unfold_split(sp, s); });
 }
 
+// find_k_pc for nmodes modes (negf/dyn.f90:770-889; tb/hamiltonian.f90:666-916):
+// candidates per mode, the matrices of each candidate (mats: the dynamical
+// matrices at k and k +- 1e-5 e_d, or the tight-binding H and HF_d), their
+// eigenpairs and the selection.  Runs inside guarded().
+template <typename Mats>
+int unfold_pc_run(nt_ctx* c, KpcArgs& a, int nmodes, const double* k_xyz, Mats mats)
+{
+    const int n_pc = a.n;
+    // modes per chunk; candidates per matrix batch (7 matrices each, kept under
+    // 512 MB and 65536 phase-sum launches rows)
+    const int Mmax = 8192;
+    const size_t per_cand = 7 * (size_t)n_pc * n_pc * 2 * sizeof(double) + 7 * 3 * sizeof(double);
+    const int NCsub = (int)std::max<size_t>(1, std::min<size_t>(9000, (size_t)(512u << 20) / per_cand));
+    Arena pr;
+    const size_t o_cnt = pr.take(sizeof(int) * Mmax);
+    const size_t o_cand = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
+    const size_t o_off = pr.take(sizeof(int) * Mmax);
+    const size_t o_cm = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
+    const size_t o_cs = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
+    const size_t o_kc = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
+    const size_t o_found = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
+    const size_t o_vc = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
+    const size_t o_kl = pr.take(sizeof(double) * 21 * (size_t)NCsub);
+    const size_t o_D = pr.take(sizeof(double) * 14 * (size_t)NCsub * n_pc * n_pc);
+    ensure_ws(c, pr.off);
+    Arena ar;
+    ar.base = static_cast<char*>(c->ws);
+    a.count = ar.ptr<int>(o_cnt);
+    a.cand = ar.ptr<double>(o_cand);
+    int* doff = ar.ptr<int>(o_off);
+    int* dcm = ar.ptr<int>(o_cm);
+    int* dcs = ar.ptr<int>(o_cs);
+    double* dkc = ar.ptr<double>(o_kc);
+    int* dfound = ar.ptr<int>(o_found);
+    double* dvc = ar.ptr<double>(o_vc);
+    double* dkl = ar.ptr<double>(o_kl);
+    double* dD = ar.ptr<double>(o_D);
+    std::vector<int> cnt, off, cm, cs;
+    for (int m0 = 0; m0 < nmodes; m0 += Mmax) {
+        const int M = std::min(Mmax, nmodes - m0);
+        check_range(c, k_xyz + 3L * m0, sizeof(double) * 3 * (size_t)M, "unfold_pc k");
+        launch(c, NT_K_ELEM, 0.0, "kpc candidates", [&] { return launch_kpc_candidates(a, m0, M, c->stream); });
+        cnt.resize(M);
+        hipcheck(c, hipMemcpyAsync(cnt.data(), a.count, sizeof(int) * M, hipMemcpyDeviceToHost, c->stream), "D2H");
+        hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+        off.assign(M, 0);
+        cm.clear();
+        cs.clear();
+        for (int i = 0; i < M; ++i) {
+            off[i] = (int)cm.size();
+            const int nci = cnt[i] == -2 ? 1 : std::max(0, cnt[i]);
+            if (nci > a.maxc) {
+                c->err = "unfold_pc: candidate count out of range";
+                return (int)NT_ERR_HIP;
+            }
+            for (int q = 0; q < nci; ++q) {
+                cm.push_back(i);
+                cs.push_back(q);
+            }
+        }
+        const int nc = (int)cm.size();
+        hipcheck(c, hipMemcpyAsync(doff, off.data(), sizeof(int) * M, hipMemcpyHostToDevice, c->stream), "H2D");
+        if (nc > 0) {
+            hipcheck(c, hipMemcpyAsync(dcm, cm.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->stream), "H2D");
+            hipcheck(c, hipMemcpyAsync(dcs, cs.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->stream), "H2D");
+        }
+        for (int c0 = 0; c0 < nc; c0 += NCsub) {
+            const int ncs = std::min(NCsub, nc - c0);
+            mats(c0, ncs, dcm, dcs, dkl, dkc, dD);
+            launch(c, NT_K_EIG, 0.0, "kpc eigenpairs",
+                   [&] { return launch_kpc_eig(a, dcm, m0, c0, ncs, dD, dfound, dvc, c->stream); });
+        }
+        launch(c, NT_K_ELEM, 0.0, "kpc select",
+               [&] { return launch_kpc_select(a, m0, M, doff, dfound, dvc, dkc, c->stream); });
+        // the host vectors are reused by the next chunk: its copies must have landed
+        hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+    }
+    return 0;
+}
+
+// the mode-level arguments every find_k_pc entry shares
+inline KpcArgs kpc_args(int tb, int n_pc, const double* E, const double* k_xyz, int n_bloch_max,
+                        const double* reci_uc, const double* recivec_pc, double* kp, double* v, int* ncand)
+{
+    KpcArgs a;
+    a.n = n_pc;
+    a.tb = tb;
+    a.E = E;
+    a.k = k_xyz;
+    for (int q = 0; q < 9; ++q) {
+        a.reci_uc[q] = reci_uc[q];
+        a.recivec_pc[q] = recivec_pc[q];
+    }
+    a.l = n_bloch_max;
+    a.maxc = (2 * a.l + 1) * (2 * a.l + 1) * (2 * a.l + 1);
+    a.kp = kp;
+    a.v = v;
+    a.ncand = ncand;
+    return a;
+}
+
 extern "C" {
 
 static int modes_entry(nt_ctx* c, const nt_params* p, int tb, int nl, int nr, int npts, const double* E,
@@ -761,97 +862,65 @@ int nt_unfold_pc_dev(nt_ctx* c, int nmodes, const double* E, const double* k_xyz
         reg_region(c, seg_rc, sizeof(int) * 2 * nseg);
         reg_region(c, coef, sizeof(double) * 9 * (size_t)nent);
         reg_region(c, dr, sizeof(double) * 3 * (size_t)nent);
-        KpcArgs a;
-        a.n = n_pc;
-        a.E = E;
-        a.k = k_xyz;
-        for (int q = 0; q < 9; ++q) {
-            a.reci_uc[q] = reci_uc[q];
-            a.recivec_pc[q] = recivec_pc[q];
+        KpcArgs a = kpc_args(0, n_pc, E, k_xyz, n_bloch_max, reci_uc, recivec_pc, kp, v, ncand);
+        return unfold_pc_run(c, a, nmodes, k_xyz,
+                             [&](int c0, int ncs, const int* dcm, const int* dcs, double* dkl, double* dkc,
+                                 double* dD) {
+                                 launch(c, NT_K_ELEM, 0.0, "kpc klist", [&] {
+                                     return launch_kpc_klist(a, dcm, dcs, c0, ncs, dkl, dkc, c->stream);
+                                 });
+                                 launch(c, NT_K_ELEM, 0.0, "kpc dynamical matrices", [&] {
+                                     return launch_hk_assemble(7 * ncs, dkl, nseg, seg_off, seg_rc, coef, dr, n_pc,
+                                                               n_pc, dD, n_pc, (long)n_pc * n_pc, c->stream);
+                                 });
+                             });
+    });
+}
+
+int nt_unfold_pc_tb_dev(nt_ctx* c, int nmodes, const double* E, const double* k_xyz, int n_bloch_max,
+                        const double* reci_uc, const double* recivec_pc, int n_pc, const int* el_off, int nent,
+                        const double* coef, const double* dr, double* kp, double* v, int* ncand)
+{
+    if (!c || nmodes < 0 || n_bloch_max < 0 || n_bloch_max > 4 || !reci_uc || !recivec_pc || n_pc <= 0 ||
+        n_pc > KPC_MAX_N || nent < 0 || !el_off || (nent > 0 && (!coef || !dr)))
+        return NT_ERR_ARG;
+    if (nmodes == 0) return NT_OK;
+    if (!E || !k_xyz || !kp || !v || !ncand) return NT_ERR_ARG;
+    return guarded(c, [&] {
+        // host check of the element table: offsets monotone from 0 to nent
+        const int ne = n_pc * n_pc;
+        std::vector<int> off(ne + 1);
+        hipcheck(c, hipMemcpy(off.data(), el_off, sizeof(int) * (ne + 1), hipMemcpyDeviceToHost), "D2H");
+        if (off[0] != 0 || off[ne] != nent) {
+            c->err = "unfold_pc_tb: element offsets must run from 0 to nent";
+            return (int)NT_ERR_ARG;
         }
-        a.l = n_bloch_max;
-        a.maxc = (2 * a.l + 1) * (2 * a.l + 1) * (2 * a.l + 1);
-        a.kp = kp;
-        a.v = v;
-        a.ncand = ncand;
-        // modes per chunk; candidates per matrix batch (7 matrices each, kept under
-        // 512 MB and 65536 phase-sum launches rows)
-        const int Mmax = 8192;
-        const size_t per_cand = 7 * (size_t)n_pc * n_pc * 2 * sizeof(double) + 7 * 3 * sizeof(double);
-        const int NCsub = (int)std::max<size_t>(1, std::min<size_t>(9000, (size_t)(512u << 20) / per_cand));
-        Arena pr;
-        const size_t o_cnt = pr.take(sizeof(int) * Mmax);
-        const size_t o_cand = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
-        const size_t o_off = pr.take(sizeof(int) * Mmax);
-        const size_t o_cm = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
-        const size_t o_cs = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
-        const size_t o_kc = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
-        const size_t o_found = pr.take(sizeof(int) * (size_t)Mmax * a.maxc);
-        const size_t o_vc = pr.take(sizeof(double) * 3 * (size_t)Mmax * a.maxc);
-        const size_t o_kl = pr.take(sizeof(double) * 21 * (size_t)NCsub);
-        const size_t o_D = pr.take(sizeof(double) * 14 * (size_t)NCsub * n_pc * n_pc);
-        ensure_ws(c, pr.off);
-        Arena ar;
-        ar.base = static_cast<char*>(c->ws);
-        a.count = ar.ptr<int>(o_cnt);
-        a.cand = ar.ptr<double>(o_cand);
-        int* doff = ar.ptr<int>(o_off);
-        int* dcm = ar.ptr<int>(o_cm);
-        int* dcs = ar.ptr<int>(o_cs);
-        double* dkc = ar.ptr<double>(o_kc);
-        int* dfound = ar.ptr<int>(o_found);
-        double* dvc = ar.ptr<double>(o_vc);
-        double* dkl = ar.ptr<double>(o_kl);
-        double* dD = ar.ptr<double>(o_D);
-        std::vector<int> cnt, off, cm, cs;
-        for (int m0 = 0; m0 < nmodes; m0 += Mmax) {
-            const int M = std::min(Mmax, nmodes - m0);
-            check_range(c, k_xyz + 3L * m0, sizeof(double) * 3 * (size_t)M, "unfold_pc k");
-            launch(c, NT_K_ELEM, 0.0, "kpc candidates", [&] { return launch_kpc_candidates(a, m0, M, c->stream); });
-            cnt.resize(M);
-            hipcheck(c, hipMemcpyAsync(cnt.data(), a.count, sizeof(int) * M, hipMemcpyDeviceToHost, c->stream),
-                     "D2H");
-            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
-            off.assign(M, 0);
-            cm.clear();
-            cs.clear();
-            for (int i = 0; i < M; ++i) {
-                off[i] = (int)cm.size();
-                const int nci = cnt[i] == -2 ? 1 : std::max(0, cnt[i]);
-                if (nci > a.maxc) {
-                    c->err = "unfold_pc: candidate count out of range";
-                    return (int)NT_ERR_HIP;
-                }
-                for (int q = 0; q < nci; ++q) {
-                    cm.push_back(i);
-                    cs.push_back(q);
-                }
+        for (int e = 0; e < ne; ++e)
+            if (off[e + 1] < off[e]) {
+                c->err = "unfold_pc_tb: element offsets must not decrease";
+                return (int)NT_ERR_ARG;
             }
-            const int nc = (int)cm.size();
-            hipcheck(c, hipMemcpyAsync(doff, off.data(), sizeof(int) * M, hipMemcpyHostToDevice, c->stream), "H2D");
-            if (nc > 0) {
-                hipcheck(c, hipMemcpyAsync(dcm, cm.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->stream),
-                         "H2D");
-                hipcheck(c, hipMemcpyAsync(dcs, cs.data(), sizeof(int) * nc, hipMemcpyHostToDevice, c->stream),
-                         "H2D");
-            }
-            for (int c0 = 0; c0 < nc; c0 += NCsub) {
-                const int ncs = std::min(NCsub, nc - c0);
-                launch(c, NT_K_ELEM, 0.0, "kpc klist",
-                       [&] { return launch_kpc_klist(a, dcm, dcs, c0, ncs, dkl, dkc, c->stream); });
-                launch(c, NT_K_ELEM, 0.0, "kpc dynamical matrices", [&] {
-                    return launch_hk_assemble(7 * ncs, dkl, nseg, seg_off, seg_rc, coef, dr, n_pc, n_pc, dD, n_pc,
-                                              (long)n_pc * n_pc, c->stream);
-                });
-                launch(c, NT_K_EIG, 0.0, "kpc eigenpairs",
-                       [&] { return launch_kpc_eig(a, dcm, m0, c0, ncs, dD, dfound, dvc, c->stream); });
-            }
-            launch(c, NT_K_ELEM, 0.0, "kpc select",
-                   [&] { return launch_kpc_select(a, m0, M, doff, dfound, dvc, dkc, c->stream); });
-            // the host vectors are reused by the next chunk: its copies must have landed
-            hipcheck(c, hipStreamSynchronize(c->stream), "sync");
+        reg_region(c, E, sizeof(double) * nmodes);
+        reg_region(c, k_xyz, sizeof(double) * 3 * (size_t)nmodes);
+        reg_region(c, kp, sizeof(double) * 3 * (size_t)nmodes);
+        reg_region(c, v, sizeof(double) * 3 * (size_t)nmodes);
+        reg_region(c, ncand, sizeof(int) * (size_t)nmodes);
+        reg_region(c, el_off, sizeof(int) * (ne + 1));
+        reg_region(c, coef, sizeof(double) * (size_t)nent);
+        reg_region(c, dr, sizeof(double) * 3 * (size_t)nent);
+        check_range(c, el_off, sizeof(int) * (ne + 1), "unfold_pc_tb offsets");
+        if (nent > 0) {
+            check_range(c, coef, sizeof(double) * (size_t)nent, "unfold_pc_tb coef");
+            check_range(c, dr, sizeof(double) * 3 * (size_t)nent, "unfold_pc_tb dr");
         }
-        return 0;
+        KpcArgs a = kpc_args(1, n_pc, E, k_xyz, n_bloch_max, reci_uc, recivec_pc, kp, v, ncand);
+        return unfold_pc_run(c, a, nmodes, k_xyz,
+                             [&](int c0, int ncs, const int* dcm, const int* dcs, double*, double* dkc, double* dD) {
+                                 launch(c, NT_K_ELEM, 0.0, "kpc tight-binding matrices", [&] {
+                                     return launch_kpc_tb_mats(a, dcm, dcs, c0, ncs, el_off, coef, dr, dD, dkc,
+                                                               c->stream);
+                                 });
+                             });
     });
 }
 

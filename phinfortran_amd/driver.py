@@ -157,7 +157,7 @@ def _tb_device_blocks(H, layers, nl: int, nr: int, nbl: int, nbr: int):
 
 
 def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, param_path: Optional[str] = None,
-           batched: Optional[bool] = None, dos: bool = True):
+           batched: Optional[bool] = None, dos: bool = True, modes: Optional[bool] = None):
     """The tight-binding driver's transport run (tb/infortran.f90:77-630,
     path_mode 2 and 3, Caroli transmission) on one GPU.
 
@@ -169,8 +169,10 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     H, as the reference's rank 0 does (tb/infortran.f90:398).  By default
     every (E, k) point is solved in one batched call on blocks sliced on the
     device (layers padded to the widest one, which leaves the transmission
-    unchanged); batched=False runs the reference's call sequence point by
-    point (surface_green + G_block, tb/infortran.f90:492-504).  With dos
+    unchanged); circular_flag = 1 computes the four lead Green's functions
+    by the block-circulant decimation (circular_g); batched=False runs the
+    reference's call sequence point by point (surface_green + G_block,
+    tb/infortran.f90:492-504).  With dos
     (default) every point also gets the layer-resolved device DOS of get_ldos
     (tb/infortran.f90:508-510, tb/device.f90:610-855): in the batched solve
     from the diagonal blocks of the full device G (padded orbitals read 0, as
@@ -181,20 +183,41 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     output_dir is given (defaults to the config's output_dir, relative to the
     configuration's folder).
 
+    modes (default: with the batched solve) adds the mode-resolved path of
+    the loop, vel_mat_mpi (tb/infortran.f90:512-523 -> tb/velocity_mat.f90:
+    222-1560): the lead Bloch modes of every point with the tight-binding
+    conventions (|lambda| within 1e-3 of 1, no velocity factors), the t/r
+    matrices, and the Brillouin-zone unfolding of the transverse supercell
+    (lead geometry from tb.unfold_geometry) that splits transmission and
+    reflection into specular and non-specular parts; write_trans then also
+    writes transl_s/ns, transr_s/ns, refll_s/ns, reflr_s/ns.dat
+    (tb/write_data.f90:364-471; reflr_ns.dat holds reflr_s, as the
+    reference passes reflr_s for both, tb/infortran.f90:615-620).  With
+    left_primitive_cell_input and right_primitive_cell_input (files beside
+    the configuration, tb/input.f90:144-334) every propagating mode is also
+    unfolded to the leads' primitive cells (find_k_pc_l / _r,
+    tb/hamiltonian.f90:666-916) and the per-mode files of the loop are
+    written ({left,right,trl,tlr,mdl,mdr}_specular_param_NNNNNN.dat, and
+    *_md2md_NNNNNN.dat with verbosity = 1; tb/velocity_mat.f90:1511-1544);
+    without them the primitive-cell columns of those files are zero.
+
     Returns dict(E (ne), k (nk x 3, as used), transl, transr, sdos_l, sdos_r,
     ddos (ne x nk), ldos (nk x ne x nlayer x maxlen), status, layer_list,
-    norb, nl, nr); ddos and ldos are None without dos."""
+    norb, nl, nr; with modes also transl_s, transl_ns, transr_s, transr_ns,
+    refll_s, refll_ns, reflr_s, reflr_ns (ne x nk), mode_sums (ne x nk x 4:
+    T_RL, T_LR, R_RR, R_LL over propagating modes), n_prop (ne x nk x 4) and
+    mode_points (per point, k outer: k_xyz_*, spec_*, and with the
+    primitive-cell inputs kp_* and v_* of the four mode sets)); ddos and
+    ldos are None without dos."""
     import torch
     from . import dense as D
     from . import tb as TBM
     from . import transport as T
+    from . import unfold as UF
     base = os.path.dirname(os.path.abspath(config_path))
     cfg = TBM.read_confige(config_path)
     if cfg["path_mode"] not in (2, 3):
         raise ValueError(f"path_mode {cfg['path_mode']}: only the transmission modes 2 and 3 are provided")
-    if cfg["circular_flag"] == 1:
-        raise NotImplementedError("the tight-binding run solves the leads by the direct decimation; "
-                                  "circular_flag = 1 is not provided here")
     fdf = cfg["filename_input"]
     st = TBM.read_fdf(fdf if os.path.isabs(fdf) else os.path.join(base, fdf))
     P = TBM.read_param(param_path or os.path.join(base, "param"))
@@ -207,15 +230,50 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
     nk, ne = len(ks), len(E)
     H = asm.assemble(eng, ks)
     layers = D.blockwise_setup(H[0].transpose(0, 1).cpu().numpy(), nl, nr, nbl, nbr)
-    p = T.params("tb", eta=cfg["eta"], convergence=cfg["convergence"])
+    # circular_flag = 1: every lead GF by the block-circulant decimation over
+    # the n_bloch_y x n_bloch_x transverse copies (tb/surface.f90:225-340 ->
+    # circular_g, tb/circular.f90:43-108); the lead orbitals must come as
+    # copies, y outer, x inner, as the structure's atom order gives them
+    circ = dict(circular=1, n_bloch_x=cfg["n_bloch_x"], n_bloch_y=cfg["n_bloch_y"]) \
+        if cfg["circular_flag"] == 1 else {}
+    p = T.params("tb", eta=cfg["eta"], convergence=cfg["convergence"], **circ)
     if batched is None:
         batched = True
+    if modes is None:
+        modes = batched
+    if modes and not batched:
+        raise ValueError("the mode-resolved path runs with the batched solve")
     maxlen = max(layers)
     ldos = np.zeros((nk, ne, len(layers), maxlen)) if dos else None
     if batched:
         d = _tb_device_blocks(H, layers, nl, nr, nbl, nbr)
         kidx = np.repeat(np.arange(nk, dtype=np.int32), ne)
-        r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, extras=("sdos_l", "sdos_r") + (("ldos",) if dos else ()))
+        ex = ("sdos_l", "sdos_r") + (("ldos",) if dos else ())
+        if modes:
+            geom, a_z_L, a_z_R = TBM.unfold_geometry(st["positions"], st["latvec"], st["species"], P, cfg)
+            pm = T.params("tb", eta=cfg["eta"], convergence=cfg["convergence"], lambda_eta=1e-3, **circ)
+            pcs = None
+            pcin = [cfg.get("left_primitive_cell_input"), cfg.get("right_primitive_cell_input")]
+            if all(x and x.lower() != "none" for x in pcin):
+                pcs = tuple(UF.TbPrimitiveCell.from_fdf(P, x if os.path.isabs(x) else os.path.join(base, x), ru, cfg,
+                                                        cfg["r_cutoff"], cfg.get("theta", 0.0))
+                            for x, ru in zip(pcin, (geom.reci_uc_l, geom.reci_uc_r)))
+            md2md = cfg.get("verbosity", 0) == 1
+            r = eng.solve_modes(d, np.tile(E, nk), pm, a_z_L, a_z_R, kidx=kidx, tb=True,
+                                want=("trl", "tlr", "rrr", "rll") if md2md else (), unfold=geom, ksc=ks, pc=pcs,
+                                extras=ex)
+            mode_pts = {key: r[key].cpu().numpy() for key in r
+                        if key.startswith(("k_xyz_", "kp_", "v_", "spec_")) or (md2md and key in
+                                                                                ("trl", "tlr", "rrr", "rll"))}
+            split = r["split"].cpu().numpy().reshape(nk, ne, 8).transpose(1, 0, 2)
+            mode_res = {name: split[:, :, i] for i, name in enumerate(UF.SPLIT_NAMES)}
+            mode_res["mode_sums"] = r["sums"].cpu().numpy().reshape(nk, ne, 4).transpose(1, 0, 2)
+            mode_res["n_prop"] = r["n_prop"].cpu().numpy().reshape(nk, ne, 4).transpose(1, 0, 2)
+            ms = r["modes_status"].cpu().numpy()
+            if ms.any():
+                raise RuntimeError(f"run_tb: mode-resolved path status {int(ms[ms != 0][0])}")
+        else:
+            r = eng.solve_tdb(d, np.tile(E, nk), p, kidx=kidx, extras=ex)
         tl = r["T_LR"].cpu().numpy().reshape(nk, ne).T
         tr = r["T_RL"].cpu().numpy().reshape(nk, ne).T
         status = r["status"].cpu().numpy().reshape(nk, ne).T
@@ -255,8 +313,24 @@ def run_tb(config_path: str, output_dir: Optional[str] = None, engine=None, para
                 TBM.write_ldos(out_dir, ik + 1, E, ldos[ik])
             TBM.write_device_dos(out_dir, ddos)
         TBM.write_surface_ldos(out_dir, sdl, sdr)
-        OUT.write_trans(out_dir, np.repeat(E[:, None], nk, axis=1), tl, tr, tb=True)
+        sp = {}
+        if modes:
+            sp = {k: mode_res[k] for k in UF.SPLIT_NAMES}
+            sp["reflr_ns"] = mode_res["reflr_s"]
+        OUT.write_trans(out_dir, np.repeat(E[:, None], nk, axis=1), tl, tr, tb=True, **sp)
+        if modes:
+            # the per-mode files, appended point by point in the loop's order
+            # (k outer, E inner), one file set per k-point
+            for ik in range(nk):
+                for ie in range(ne):
+                    q = ik * ne + ie
+                    OUT.write_mode_tables_tb(out_dir, ik + 1, E[ie], {key: a[q] for key, a in mode_pts.items()},
+                                             md2md=md2md)
     if engine is None:
         eng.close()
-    return dict(E=E, k=ks, transl=tl, transr=tr, sdos_l=sdl, sdos_r=sdr, ddos=ddos, ldos=ldos, status=status,
-                layer_list=layers, norb=asm.norb, nl=nl, nr=nr)
+    res = dict(E=E, k=ks, transl=tl, transr=tr, sdos_l=sdl, sdos_r=sdr, ddos=ddos, ldos=ldos, status=status,
+               layer_list=layers, norb=asm.norb, nl=nl, nr=nr)
+    if modes:
+        res.update(mode_res)
+        res["mode_points"] = mode_pts
+    return res

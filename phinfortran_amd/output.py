@@ -146,3 +146,92 @@ def write_mode_tables(output_dir: str, nn: int, omega: float, point: dict):
         with open(os.path.join(output_dir, "%s_specular_param_%06d.dat" % (name, nn)), "a") as f:
             for r in rows:
                 f.write(r + "\n")
+
+
+# ------------------------------------------------ tight-binding per-mode tables
+def specular_rows_tb(E, k_f, total, spec, kp_f=None, v_f=None):
+    """The tight-binding write_specular's records for one (E, k) point
+    (tb/write_data.f90:3-67): one per incident propagating mode (k_f(i,3) <
+    99.9), '(13e20.10)': E, k_f(i,1:3), kp_f(i,1:3), v_f(i,1:3), r_s, r_t,
+    r_s / r_t (0 when r_t = 0); no mode number and no 1e-60 floor."""
+    k_f = np.asarray(k_f, float)
+    n = len(k_f)
+    kp_f = np.zeros((n, 3)) if kp_f is None else np.asarray(kp_f, float)
+    v_f = np.zeros((n, 3)) if v_f is None else np.asarray(v_f, float)
+    out = []
+    for i in range(n):
+        if not k_f[i, 2] < 99.9:
+            continue
+        rt, rs = float(total[i]), float(spec[i])
+        vals = [E, *k_f[i], *kp_f[i], *v_f[i], rs, rt, rs / rt if rt != 0.0 else 0.0]
+        out.append("".join(fortran_e(v, 20, 10) for v in vals))
+    return out
+
+
+def md2md_rows_tb(E, M, k_f, k_b, kp_f=None, kp_b=None, v_f=None, v_b=None):
+    """The tight-binding write_md2md's records (tb/write_data.f90:166-239):
+    for every propagating outgoing mode i (k_f, numbered count1 from 1) and
+    propagating incident mode j (k_b, count2 running over all pairs), '(2I5,
+    21e20.10,2I5)': count1, count2, E, k_f(i), k_b(j), kp_f(i), kp_b(j),
+    v_f(i), v_b(j), ts, tns, i, j (1-based), |M(i, j)|^2 in ts when the
+    in-plane k agree (|dk|^2 < 1e-5), else in tns.  M: outgoing x incident."""
+    k_f, k_b = np.asarray(k_f, float), np.asarray(k_b, float)
+    z = lambda a, n: np.zeros((n, 3)) if a is None else np.asarray(a, float)
+    kp_f, kp_b, v_f, v_b = z(kp_f, len(k_f)), z(kp_b, len(k_b)), z(v_f, len(k_f)), z(v_b, len(k_b))
+    out, c1, c2 = [], 0, 0
+    for i in range(len(k_f)):
+        if not k_f[i, 2] < 99.9:
+            continue
+        c1 += 1
+        for j in range(len(k_b)):
+            if not k_b[j, 2] < 99.9:
+                continue
+            c2 += 1
+            d = k_f[i] - k_b[j]
+            d[2] = 0.0
+            a = abs(M[i, j]) ** 2
+            ts, tns = (a, 0.0) if d @ d < 1.0e-5 else (0.0, a)
+            vals = [E, *k_f[i], *k_b[j], *kp_f[i], *kp_b[j], *v_f[i], *v_b[j], ts, tns]
+            out.append("%5d%5d" % (c1, c2) + "".join(fortran_e(v, 20, 10) for v in vals) + "%5d%5d" % (i + 1, j + 1))
+    return out
+
+
+def write_mode_tables_tb(output_dir: str, nn: int, E: float, point: dict, md2md: bool = False):
+    """Append one (E, k) point to the per-mode files of the tight-binding
+    vel_mat (tb/velocity_mat.f90:1511-1544): {left,right,trl,tlr}_specular_
+    param_NNNNNN.dat (write_specular of r_LL, r_RR, t_RL, t_LR per incident
+    mode), {mdl,mdr}_specular_param_NNNNNN.dat (write_md, the negf format)
+    and, with md2md (verbosity = 1), {tlr,trl,rrr,rll}_md2md_NNNNNN.dat.
+    point: host arrays of one point from Engine.solve_modes(..., unfold=...,
+    pc=...): k_xyz_* and kp_* / v_* (n x 3) of the four mode sets, spec_*
+    (n x 2: total, specular), and for md2md the t/r matrices (outgoing x
+    incident)."""
+    os.makedirs(output_dir, exist_ok=True)
+    k = {s: point["k_xyz_" + s] for s in ("L_adv", "L_ret", "R_ret", "R_adv")}
+    kp = {s: point.get("kp_" + s) for s in k}
+    v = {s: point.get("v_" + s) for s in k}
+    sp = {m: np.asarray(point["spec_" + m]) for m in ("trl", "tlr", "rrr", "rll")}
+    tables = {
+        "left_specular_param": specular_rows_tb(E, k["L_adv"], sp["rll"][:, 0], sp["rll"][:, 1], kp["L_adv"], v["L_adv"]),
+        "right_specular_param": specular_rows_tb(E, k["R_adv"], sp["rrr"][:, 0], sp["rrr"][:, 1], kp["R_adv"],
+                                                 v["R_adv"]),
+        "trl_specular_param": specular_rows_tb(E, k["L_adv"], sp["trl"][:, 0], sp["trl"][:, 1], kp["L_adv"], v["L_adv"]),
+        "tlr_specular_param": specular_rows_tb(E, k["R_adv"], sp["tlr"][:, 0], sp["tlr"][:, 1], kp["R_adv"], v["R_adv"]),
+        "mdl_specular_param": md_rows(E, k["L_adv"], sp["rll"][:, 0], sp["rll"][:, 1], sp["trl"][:, 0],
+                                      sp["trl"][:, 1], kp_b=kp["L_adv"], v_b=v["L_adv"]),
+        "mdr_specular_param": md_rows(E, k["R_adv"], sp["rrr"][:, 0], sp["rrr"][:, 1], sp["tlr"][:, 0],
+                                      sp["tlr"][:, 1], kp_b=kp["R_adv"], v_b=v["R_adv"]),
+    }
+    if md2md:
+        tables["tlr_md2md"] = md2md_rows_tb(E, point["tlr"], k["L_ret"], k["R_adv"], kp["L_ret"], kp["R_adv"],
+                                            v["L_ret"], v["R_adv"])
+        tables["trl_md2md"] = md2md_rows_tb(E, point["trl"], k["R_ret"], k["L_adv"], kp["R_ret"], kp["L_adv"],
+                                            v["R_ret"], v["L_adv"])
+        tables["rrr_md2md"] = md2md_rows_tb(E, point["rrr"], k["R_ret"], k["R_adv"], kp["R_ret"], kp["R_adv"],
+                                            v["R_ret"], v["R_adv"])
+        tables["rll_md2md"] = md2md_rows_tb(E, point["rll"], k["L_ret"], k["L_adv"], kp["L_ret"], kp["L_adv"],
+                                            v["L_ret"], v["L_adv"])
+    for name, rows in tables.items():
+        with open(os.path.join(output_dir, "%s_%06d.dat" % (name, nn)), "a") as f:
+            for r in rows:
+                f.write(r + "\n")

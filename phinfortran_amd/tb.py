@@ -174,7 +174,8 @@ def read_confige(path: str) -> Dict:
                 n_bloch_x=int(vals.get("n_bloch_x", 1)), n_bloch_y=int(vals.get("n_bloch_y", 1)),
                 circular_flag=int(vals.get("circular_flag", 0)),
                 path_mode=int(vals.get("path_mode", 2)), kpath_file=vals.get("kpath_file"),
-                output_dir=vals.get("output_dir"), verbosity=int(vals.get("verbosity", 0)))
+                output_dir=vals.get("output_dir"), verbosity=int(vals.get("verbosity", 0)),
+                theta=num("theta", 0.0))
 
 
 def k_grid(cfg: Dict, base_dir: Optional[str] = None) -> np.ndarray:
@@ -235,6 +236,54 @@ def lead_orbitals(P: Dict, species, cfg: Dict) -> Dict:
     bl, br, pl, pr = cfg["buffer_left"], cfg["buffer_right"], cfg["period_left"], cfg["period_right"]
     return dict(n_buffer_l=sum(onum[:bl]), n_buffer_r=sum(onum[nat - bl:]) if bl else 0,
                 nl=sum(onum[bl:bl + pl]), nr=sum(onum[nat - br - pr:nat - br]))
+
+
+def lead_periods(positions, cfg: Dict):
+    """The leads' period lengths along z of get_periodic_lead_orb_num
+    (tb/surface.f90:59-86): a_z_L = (lowest z of the second left period) -
+    (lowest z of the first, atoms buffer_left + 1 .. buffer_left +
+    period_left); a_z_R = (highest z of the atoms period_left beyond the
+    right lead's inner period) - (highest z of that period), the reference
+    offsetting the right lead by period_left."""
+    z = np.asarray(positions, float)[:, 2]
+    nat = len(z)
+    bl, pl, br, pr = cfg["buffer_left"], cfg["period_left"], cfg["buffer_right"], cfg["period_right"]
+    i = np.arange(bl, bl + pl)
+    a_z_L = z[i + pl].min() - z[i].min()
+    j = np.arange(nat - br - 2 * pr, nat - br - pr)
+    a_z_R = z[j + pl].max() - z[j].max()
+    return float(a_z_L), float(a_z_R)
+
+
+def unfold_geometry(positions, latvec, species, P: Dict, cfg: Dict):
+    """The lead geometry the mode-resolved path's unfolding needs
+    (transverse_uc_setup, tb/velocity_mat.f90:14-174, with the lattices of
+    tb/input.f90:128-142 and tb/surface.f90:69-90): the transverse unit
+    cell latvec_uc (the first two lattice vectors over n_bloch_x and
+    n_bloch_y), each lead's cell with its z vector's z component replaced by
+    the lead period (reci_uc_l / reci_uc_r), the transverse cells of the
+    second left period and of the right lead's inner period (tolerance 1e-3
+    and 1e-5) with the orbitals of every atom.  Returns (UnfoldGeometry,
+    a_z_L, a_z_R)."""
+    from . import unfold as U
+    latvec = np.asarray(latvec, float)
+    nbx, nby = cfg["n_bloch_x"], cfg["n_bloch_y"]
+    latvec_uc = latvec.copy()
+    latvec_uc[0] /= nbx
+    latvec_uc[1] /= nby
+    a_z_L, a_z_R = lead_periods(positions, cfg)
+    reci = []
+    for a_z in (a_z_L, a_z_R):
+        lat = latvec_uc.copy()
+        lat[2, 2] = a_z
+        reci.append(2 * np.pi * np.linalg.inv(lat).T)
+    onum = np.array([P["orb_num"][s - 1] for s in species], int)
+    nat = len(onum)
+    bl, pl, br, pr = cfg["buffer_left"], cfg["period_left"], cfg["buffer_right"], cfg["period_right"]
+    left = U.transverse_cells(positions, latvec_uc, bl + pl, pl, nbx, nby, 1e-3, ndims=onum)
+    right = U.transverse_cells(positions, latvec_uc, nat - br - 2 * pr, pr, nbx, nby, 1e-5, ndims=onum)
+    recivec = 2 * np.pi * np.linalg.inv(latvec).T
+    return U.UnfoldGeometry(nbx, nby, recivec, reci[0], reci[1], left, right), a_z_L, a_z_R
 
 
 def write_positions(output_dir: str, positions, species, orb_num, layer_list, n_buffer_l: int, nl: int, nk: int):
@@ -407,6 +456,91 @@ def onsite_block(P: Dict, s: int) -> np.ndarray:
     """onsite (tb/hamiltonian.f90:57-72): diagonal V(s, s, p, p, onsite)."""
     names = P["orbitals"][s - 1]
     return np.diag([P["V"][s - 1, s - 1, SHELL[o] - 1, SHELL[o] - 1, 0] if o in SHELL else 0.0 for o in names])
+
+
+# ------------------------------------------------------------------ primitive cell
+def misorientation_rotation(theta_deg: float = 0.0) -> np.ndarray:
+    """rot_all of tb/input.f90:173-182: rot_z^T rot_misori rot_z, rot_z
+    holding the default-real sqrt(2.0) / 2.0 (so the rotation of theta = 0 is
+    the identity up to single-precision rounding, as in the reference)."""
+    c = float(np.float32(np.sqrt(np.float32(2.0))) / np.float32(2.0))
+    rz = np.array([[c, -c, 0.0], [c, c, 0.0], [0.0, 0.0, 1.0]])
+    th = np.deg2rad(theta_deg)
+    rm = np.array([[1.0, 0.0, 0.0], [0.0, np.cos(th), -np.sin(th)], [0.0, np.sin(th), np.cos(th)]])
+    return rz.T @ rm @ rz
+
+
+def read_pc_fdf(path: str, theta_deg: float = 0.0) -> Dict:
+    """The primitive-cell part of load_input (tb/input.f90:144-234) for one
+    lead: NumberOfAtoms and LatticeConstant, the lattice vectors (scaled,
+    rotated by misorientation_rotation), the listed atoms' scaled
+    coordinates and species; the atoms whose crystal coordinates lie in
+    (-0.001, 0.999) form the primitive cell.  Returns dict(latvec_pc (rows),
+    positions_uc (n x 3), species_uc, natoms_uc, positions_pc, species_pc,
+    recivec_pc = 2 pi inv(latvec_pc)^T)."""
+    lines = open(path).read().splitlines()
+    nat, scale = 0, 1.0
+    for ln in lines:
+        t = ln.split()
+        if t and t[0] == "NumberOfAtoms":
+            nat = int(t[1])
+        elif t and t[0] == "LatticeConstant":
+            scale = float(t[1])
+    R = misorientation_rotation(theta_deg)
+    lat, rows = None, []
+    for i, ln in enumerate(lines):
+        if "%block LatticeVectors" in ln:
+            lat = np.array([R @ (np.array([float(x) for x in lines[i + 1 + r].split()[:3]]) * scale)
+                            for r in range(3)])
+        elif "%block AtomicScaledCoordinatesAndAtomicSpecies" in ln:
+            rows = [[float(x) for x in lines[i + 1 + r].split()[:4]] for r in range(nat)]
+    if lat is None or len(rows) != nat:
+        raise ValueError(f"{path}: lattice vectors or the {nat} scaled coordinates are missing")
+    rows = np.array(rows)
+    pos = rows[:, :3] @ lat
+    sp = rows[:, 3].astype(int)
+    frac = pos @ np.linalg.inv(lat)
+    inside = ((frac > -0.001) & (frac < 0.999)).all(axis=1)
+    return dict(latvec_pc=lat, positions_uc=pos, species_uc=sp, natoms_uc=nat, positions_pc=pos[inside],
+                species_pc=sp[inside], recivec_pc=2 * np.pi * np.linalg.inv(lat).T)
+
+
+def pc_table(P: Dict, positions_pc, species_pc, latvec_pc, r_cutoff: float):
+    """gen_ham_pc_l (tb/hamiltonian.f90:918-989) as an element-wise table:
+    for atom pairs (i, j) and images n in [-2, 2]^3 (n1 slowest) with
+    r = x_j - x_i + n . latvec_pc shorter than r_cutoff, the block
+    onsite (i = j) or V_sk(r) contributes V exp(i k.r) to every element of
+    the pair.  Returns (el_off (n^2 + 1; element r + c n column-major),
+    coef (nent), dr (nent x 3), n): the input of nt_unfold_pc_tb_dev, which
+    also forms gen_hf_pc_l's sums V r_d exp(i k.r) from it."""
+    pos = np.asarray(positions_pc, float)
+    sp = [int(x) for x in species_pc]
+    lat = np.asarray(latvec_pc, float)
+    start = np.concatenate([[0], np.cumsum([P["orb_num"][s - 1] for s in sp])]).astype(int)
+    n = int(start[-1])
+    shifts = np.array([(i1, i2, i3) for i1 in range(-2, 3) for i2 in range(-2, 3) for i3 in range(-2, 3)], float)
+    elems: List[List] = [[] for _ in range(n * n)]
+    for ia in range(len(pos)):
+        for ja in range(len(pos)):
+            rs = pos[ja] - pos[ia] + shifts @ lat
+            keep = np.sqrt((rs * rs).sum(axis=1)) < r_cutoff
+            if not keep.any():
+                continue
+            rs = rs[keep]
+            blocks = (np.repeat(onsite_block(P, sp[ia])[None], len(rs), axis=0) if ia == ja
+                      else sk_blocks(P, sp[ia], sp[ja], rs))
+            for q in range(len(rs)):
+                for x in range(blocks.shape[1]):
+                    for y in range(blocks.shape[2]):
+                        elems[(start[ia] + x) + (start[ja] + y) * n].append((blocks[q, x, y], rs[q]))
+    el_off = np.zeros(n * n + 1, np.int32)
+    coef, dr = [], []
+    for e, lst in enumerate(elems):
+        for v, r in lst:
+            coef.append(v)
+            dr.append(r)
+        el_off[e + 1] = len(coef)
+    return el_off, np.array(coef, float), np.array(dr, float).reshape(-1, 3), n
 
 
 # ------------------------------------------------------------------ assembly

@@ -444,7 +444,8 @@ class Engine:
     MODE_FIELDS = ("gl_adv", "gr_ret", "gl_plus", "gr_minus", "gml_adv", "gmr_ret", "G_D_RL", "G_D_LR", "G_L", "G_R")
 
     def solve_modes(self, dblocks, E, p: N.NtParams, a_z_L: float, a_z_R: float, kidx=None, first_point_mask=None,
-                    tb: bool = False, want=("trl", "tlr", "rrr", "rll"), unfold=None, ksc=None, pc=None, probe=None):
+                    tb: bool = False, want=("trl", "tlr", "rrr", "rll"), unfold=None, ksc=None, pc=None, probe=None,
+                    extras=()):
         """Batched solve followed by the mode-resolved transmission (vel_mat_tdb).
 
         Returns the solve's dict plus k_z_* (npts, n), n_prop (npts, 4),
@@ -457,21 +458,24 @@ class Engine:
         unfolding: bz_* (npts, n) labels, k_xyz_* (npts, n, 3) unfolded k,
         split (npts, 8) in the order of unfold.SPLIT_NAMES, and spec_trl,
         spec_tlr, spec_rrr, spec_rll (npts, n_incident, 2): (total, specular)
-        per incident mode.  pc (unfold.PrimitiveCell, with unfold) adds the
-        primitive-cell k and group velocity of every mode: kp_* and v_*
-        (npts, n, 3), ncand_* (npts, n) (find_k_pc, see unfold_pc).
+        per incident mode.  pc (unfold.PrimitiveCell, with unfold; or a
+        (left, right) pair, as the tight-binding run's find_k_pc_l / _r with
+        unfold.TbPrimitiveCell) adds the primitive-cell k and group velocity
+        of every mode: kp_* and v_* (npts, n, 3), ncand_* (npts, n)
+        (find_k_pc, see unfold_pc).
 
         probe (dict of solve_buttiker's keyword arguments: temperature, t_left,
         t_right[, B_probe, T_probe, lifetime]) solves with Buttiker probes in
         the sweeps, as vel_mat_tdb sees them in the reference's Buttiker loop,
-        and adds the probe outputs."""
+        and adds the probe outputs.  extras: further outputs of the batched
+        solve (names as in solve_tdb: sdos_l, sdos_r, ldos, ...)."""
         import torch
         dev = torch.device("cuda", self.device)
+        ex = tuple(self.MODE_FIELDS) + tuple(x for x in extras if x not in self.MODE_FIELDS)
         if probe is None:
-            r = self.solve_tdb(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=self.MODE_FIELDS)
+            r = self.solve_tdb(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=ex)
         else:
-            r = self.solve_buttiker(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask,
-                                    extras=self.MODE_FIELDS, **probe)
+            r = self.solve_buttiker(dblocks, E, p, kidx=kidx, first_point_mask=first_point_mask, extras=ex, **probe)
         E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous()
         npts = E.numel()
         nl, nr, nk = dblocks["nl"], dblocks["nr"], dblocks["nk"]
@@ -528,11 +532,12 @@ class Engine:
             for name in [k for k in res if k.startswith("k_xyz_") or k.startswith("spec_")]:
                 res[name] = res[name].transpose(-1, -2)
             if pc is not None:
+                pcs = tuple(pc) if isinstance(pc, (tuple, list)) else (pc, pc)
                 for key in ("L_adv", "L_ret", "R_ret", "R_adv"):
                     kx = res["k_xyz_" + key]
                     n = kx.shape[1]
                     Em = E.reshape(-1, 1).expand(npts, n)
-                    u = self.unfold_pc(pc, Em.contiguous(), kx.contiguous())
+                    u = self.unfold_pc(pcs[0] if key[0] == "L" else pcs[1], Em.contiguous(), kx.contiguous())
                     res["kp_" + key] = u["kp"].reshape(npts, n, 3)
                     res["v_" + key] = u["v"].reshape(npts, n, 3)
                     res["ncand_" + key] = u["ncand"].reshape(npts, n)
@@ -543,8 +548,10 @@ class Engine:
     def unfold_pc(self, pc, E, k_xyz):
         """find_k_pc (negf/dyn.f90:770-889) for modes of energies E (nmodes)
         and unfolded k k_xyz (nmodes x 3, k_z >= 99.9 for evanescent modes) on
-        the GPU; pc: phinfortran_amd.unfold.PrimitiveCell.  Returns dict kp,
-        v (nmodes x 3) and ncand (nmodes) as torch tensors."""
+        the GPU; pc: phinfortran_amd.unfold.PrimitiveCell, or
+        unfold.TbPrimitiveCell for the tight-binding run's find_k_pc_l / _r
+        (tb/hamiltonian.f90:666-916).  Returns dict kp, v (nmodes x 3) and
+        ncand (nmodes) as torch tensors."""
         import torch
         dev = torch.device("cuda", self.device)
         E = torch.as_tensor(E, dtype=torch.float64, device=dev).contiguous().reshape(-1)
@@ -552,16 +559,21 @@ class Engine:
         n = E.numel()
         if k.shape[0] != n:
             raise ValueError("E and k_xyz disagree on the number of modes")
-        seg_off, seg_rc, coef, dr = pc.device_table(torch, dev)
         kp = torch.zeros((n, 3), dtype=torch.float64, device=dev)
         v = torch.zeros((n, 3), dtype=torch.float64, device=dev)
         nc = torch.zeros(n, dtype=torch.int32, device=dev)
         ptr = lambda t: C.c_void_p(t.data_ptr())
         ru = (C.c_double * 9)(*pc.reci_uc.reshape(-1))
         rp = (C.c_double * 9)(*pc.recivec_pc.reshape(-1))
-        nseg, nent = len(pc.table[0]) - 1, len(pc.table[2])
-        st = self._L.nt_unfold_pc_dev(self.ctx, n, ptr(E), ptr(k), pc.l, ru, rp, pc.n_pc, nseg, ptr(seg_off),
-                                      ptr(seg_rc), nent, ptr(coef), ptr(dr), ptr(kp), ptr(v), ptr(nc))
+        if getattr(pc, "tb", False):
+            el_off, coef, dr = pc.device_table(torch, dev)
+            st = self._L.nt_unfold_pc_tb_dev(self.ctx, n, ptr(E), ptr(k), pc.l, ru, rp, pc.n_pc, ptr(el_off),
+                                             int(coef.numel()), ptr(coef), ptr(dr), ptr(kp), ptr(v), ptr(nc))
+        else:
+            seg_off, seg_rc, coef, dr = pc.device_table(torch, dev)
+            nseg, nent = len(pc.table[0]) - 1, len(pc.table[2])
+            st = self._L.nt_unfold_pc_dev(self.ctx, n, ptr(E), ptr(k), pc.l, ru, rp, pc.n_pc, nseg, ptr(seg_off),
+                                          ptr(seg_rc), nent, ptr(coef), ptr(dr), ptr(kp), ptr(v), ptr(nc))
         self._check(st, "unfold_pc")
         return dict(kp=kp, v=v, ncand=nc)
 

@@ -38,13 +38,15 @@ class LeadCells(NamedTuple):
 
 
 def transverse_cells(positions, latvec_uc, first: int, period: int, nbx: int, nby: int, tol: float,
-                     ndims: int = 3) -> LeadCells:
+                     ndims=3) -> LeadCells:
     """transverse_uc_setup for one lead period starting at 0-based atom
     `first`: atoms whose in-plane fractional coordinates lie in (-0.001,
     0.999) form L_list (velocity_mat.f90:39-68); every atom maps to the L_list
     atom it equals up to k a1 + l a2 (0 <= k < nbx, 0 <= l < nby) within tol,
     the last such L_list entry winning (:121-134; the reference uses tol 1e-3
-    on the left, 1e-5 on the right)."""
+    on the left, 1e-5 on the right).  ndims: degrees of freedom per atom, 3
+    for phonons, or per-atom orbital counts of the whole structure (the
+    tight-binding run's atom_orb_num, tb/velocity_mat.f90:14-174)."""
     positions = np.asarray(positions, float)[:, :3]
     latvec_uc = np.asarray(latvec_uc, float)
     atoms = positions[first:first + period]
@@ -67,11 +69,13 @@ def transverse_cells(positions, latvec_uc, first: int, period: int, nbx: int, nb
         if last is None:
             raise ValueError(f"lead atom {first + a} matches no transverse-cell atom")
         pidx_atom[a] = last
-    pidx = np.repeat(pidx_atom, ndims)
-    pos = np.repeat(atoms, ndims, axis=0)
+    per_atom = np.full(len(positions), ndims, int) if np.isscalar(ndims) else np.asarray(ndims, int)
+    counts = per_atom[first:first + period]
+    pidx = np.repeat(pidx_atom, counts)
+    pos = np.repeat(atoms, counts, axis=0)
     nb = nbx * nby
     n = len(pos)
-    if n % nb or n // nb != ndims * len(L_list):
+    if n % nb or n // nb != per_atom[L_list].sum():
         raise ValueError(f"lead orbitals ({n}) are not {nb} images of the {len(L_list)}-atom transverse cell")
     pos_uc = positions[L_list[pidx[:n // nb]]]
     return LeadCells(L_list, pidx, pos, pos_uc)
@@ -137,6 +141,36 @@ class PrimitiveCell:
         table = Q.pc_dynamics_table(qe, pc["natoms_pc"], pc["mass"])
         reci_uc = 2 * np.pi * np.linalg.inv(qe["cell"]).T
         return cls(table, 3 * pc["natoms_pc"], pc["recivec"], reci_uc, max(cfg["n_bloch_x"], cfg["n_bloch_y"]))
+
+    def device_table(self, torch, dev):
+        key = str(dev)
+        if key not in self._dev:
+            self._dev[key] = tuple(torch.as_tensor(np.ascontiguousarray(x), device=dev) for x in self.table)
+        return self._dev[key]
+
+
+class TbPrimitiveCell:
+    """What the tight-binding run's find_k_pc_l / _r needs
+    (tb/hamiltonian.f90:666-916; nt_unfold_pc_tb_dev): the primitive cell's
+    element-wise H table (tb.pc_table), its reciprocal vectors, the lead
+    unit cell's (reci_uc_l / reci_uc_r) and l = max(n_bloch_x, n_bloch_y)."""
+
+    tb = True
+
+    def __init__(self, table, recivec_pc, reci_uc, n_bloch_max):
+        self.table = table[:3]
+        self.n_pc = int(table[3])
+        self.recivec_pc = np.asarray(recivec_pc, float)
+        self.reci_uc = np.asarray(reci_uc, float)
+        self.l = int(n_bloch_max)
+        self._dev = {}
+
+    @classmethod
+    def from_fdf(cls, P, pc_fdf_path, reci_uc, cfg, r_cutoff, theta_deg=0.0):
+        from . import tb as TBM
+        pc = TBM.read_pc_fdf(pc_fdf_path, theta_deg)
+        table = TBM.pc_table(P, pc["positions_pc"], pc["species_pc"], pc["latvec_pc"], r_cutoff)
+        return cls(table, pc["recivec_pc"], reci_uc, max(cfg["n_bloch_x"], cfg["n_bloch_y"]))
 
     def device_table(self, torch, dev):
         key = str(dev)

@@ -106,3 +106,57 @@ def test_against_oracle(run):
                 assert (r["kp_" + key][q][ev] == 100.0).all() and (r["ncand_" + key][q][ev] == 0).all()
     print(f"oracle: {checked} modes")
     assert checked > 50
+
+
+def test_tight_binding_primitive_cell_against_oracle(engine, tmp_path):
+    """nt_unfold_pc_tb_dev (find_k_pc_l, tb/hamiltonian.f90:666-916) on the Si
+    primitive cell of the tight-binding example's parameters, seen from the
+    four-atom tetragonal lead cell: energies on the primitive bands at random
+    k, the mode's k folded into the lead cell's zone, some on its boundary;
+    kp, v and the candidate count against oracle/kpc_tb.py (v where one
+    candidate matches)."""
+    import torch
+    from oracle import kpc_tb as KT
+    from phinfortran_amd import tb
+    from phinfortran_amd.unfold import TbPrimitiveCell
+    from tests import tb_run_inputs as R
+    a = 5.5
+    path = R.write_pc(str(tmp_path), "pc.fdf", 1, a)
+    P = tb.read_param(R.PARAM)
+    cell = np.array([[a / 2, -a / 2, 0.0], [a / 2, a / 2, 0.0], [0.0, 0.0, a]])
+    reci_uc = 2 * np.pi * np.linalg.inv(cell).T
+    cfg = dict(n_bloch_x=1, n_bloch_y=1)
+    pc = TbPrimitiveCell.from_fdf(P, path, reci_uc, cfg, 2.6)
+    o = KT.read_pc_fdf(path)
+    opc = KT.TbPrimitiveCell(P, o["positions_pc"], o["latvec_pc"], 2.6)
+    assert pc.n_pc == opc.n == 20
+    rng = np.random.default_rng(11)
+    Es, ks = [], []
+    for _ in range(24):
+        kpc = rng.uniform(-0.5, 0.5, 3) @ reci_uc
+        e = np.linalg.eigvalsh(opc.ham(kpc))
+        Es.append(float(e[rng.integers(4, 12)]))
+        # the mode's k: kpc moved into the lead cell's zone
+        f = np.linalg.solve(reci_uc.T, kpc)
+        ks.append((f - np.round(f)) @ reci_uc)
+    ks[3] = 0.5 * reci_uc[2]                                 # on the zone boundary: its own k
+    Es[3] = float(np.linalg.eigvalsh(opc.ham(ks[3]))[6])
+    ks.append(np.array([0.0, 0.0, 100.0]))                   # an evanescent mode
+    Es.append(0.3)
+    E = np.array(Es)
+    kx = np.array(ks)
+    print("kpc tight binding", flush=True)
+    u = engine.unfold_pc(pc, torch.as_tensor(E), torch.as_tensor(kx))
+    kp, v, nc = u["kp"].cpu().numpy(), u["v"].cpu().numpy(), u["ncand"].cpu().numpy()
+    assert np.all(kp[-1] == 100.0) and nc[-1] == 0
+    nv = 0
+    for i in range(len(E) - 1):
+        okp, ov = KT.mode_pc(E[i], kx[i], reci_uc, o["recivec_pc"], 1, 1, opc)
+        kpcs, _ = KT.find_k_pc(E[i], kx[i], reci_uc, o["recivec_pc"], 1, 1, opc)
+        assert nc[i] == len(kpcs), i
+        assert np.abs(kp[i] - okp).max() < 1e-10, (i, kp[i], okp)
+        e = np.linalg.eigvalsh(opc.ham(okp))
+        if len(kpcs) == 1 and (np.abs(e - E[i]) < 1e-4).sum() == 1:
+            assert np.abs(v[i] - ov).max() < 1e-8 * max(1.0, np.abs(ov).max()), (i, v[i], ov)
+            nv += 1
+    assert nv >= 5

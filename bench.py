@@ -75,7 +75,8 @@ def main():
                          "under rocprofv3 --pmc)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
     ap.add_argument("--serial-sweeps", action="store_true",
-                    help="run the two RGF sweeps on one stream in every step (for per-kernel profiles)")
+                    help="every launch on one stream in every step: the two RGF sweeps (with two sweeps) and the "
+                         "parts of a split solve (for per-kernel profiles)")
     ap.add_argument("--c3-devices", type=int, default=16,
                     help="uneven-batching sample: devices of the seeded 256-device Si/Ge superlattice draw (the "
                          "first of each block size 24/96/216/384 in draw order, in equal numbers; 256 = the whole "
@@ -133,6 +134,7 @@ def main():
     eng = T.Engine(0, batch_limit=args.points)
     if args.serial_sweeps:
         eng.set_overlap(False)
+        eng.set_point_split(1)
     dblocks = [eng.upload_blocks(b) for b in blocks]
     torch.cuda.synchronize()
     if args.copy_probe_mib:
@@ -343,11 +345,13 @@ def main():
         times that include no time shared with another stream's kernels"""
         p = T.params("phonon", precision=prec)
         eng.set_overlap(False)
+        eng.set_point_split(1)
         eng.profile_reset()
         eng.profile(True)
         eng.solve_tdb(dblocks[args.warmup], E, p)
         eng.profile(False)
         eng.set_overlap(not args.serial_sweeps)
+        eng.set_point_split(1 if args.serial_sweeps else 2)
         return {"gemm": eng.profile_get(N.NT_K_GEMM), "inverse": eng.profile_get(N.NT_K_INV)}
 
     iso64 = isolated_kernels("fp64")
@@ -826,9 +830,9 @@ def main():
                      "timing": "HIP events around every launch on the launching stream, in one extra step with "
                                "every launch on one stream (no kernel shares the GPU with another)",
                      "concurrent": {"avg_launch_ms": avg_ms_conc, "achieved": achieved_conc,
-                                    "what": "the same kernel in the timed steps (the default one-sweep solve uses "
-                                            "one stream too; with two sweeps the second runs on a second stream "
-                                            "and launches can share the GPU)"},
+                                    "what": "the same kernel in the timed steps, where the step's points run as "
+                                            "two parts side by side on two worker streams (nt_set_point_split) "
+                                            "and launches share the GPU"},
                      "inverse_isolated": {"avg_launch_ms": iso64["inverse"]["ms"] / max(1, iso64["inverse"]["launches"]),
                                           "tflops": iso64["inverse"]["flops"] / max(1e-9, iso64["inverse"]["ms"] * 1e-3) / 1e12}},
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,

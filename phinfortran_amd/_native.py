@@ -38,7 +38,7 @@ EXPORTS = [
     "nt_get_G_full_tdb", "nt_G_block", "nt_solve_tdb_batched_dev", "nt_solve_tdb_batched",
     "nt_solve_tdb_batched_noalloc_dev",
     "nt_solve_tdb_adjoint_batched_dev", "nt_buttiker_tdb_batched_dev", "nt_get_ldos", "nt_modes_tdb_batched_dev", "nt_modes_unfold_tdb_batched_dev", "nt_unfold_pc_dev", "nt_unfold_pc_tb_dev", "nt_solve_tdb_multi_dev", "nt_hk_assemble_dev", "nt_hk_assemble_orb_dev", "nt_autotune_inverse", "nt_set_inverse_panel", "nt_autotune_gemm", "nt_set_gemm_tile",
-    "nt_set_overlap", "nt_set_gemm_3m", "nt_set_workspace_poison",
+    "nt_set_overlap", "nt_set_gemm_3m", "nt_set_workspace_poison", "nt_set_point_split",
     "nt_set_adjoint_checkpoint", "nt_adjoint_bytes_per_point", "nt_set_sweeps", "nt_set_decimation_check", "nt_set_multi_workers", "nt_profile_enable", "nt_profile_get", "nt_profile_reset", "nt_last_mean_iters", "nt_mixed_counts",
     "nt_zgemm_batched_dev", "nt_zinv_batched_dev", "nt_zgeev_batched_dev", "nt_zsqrtm_batched_dev",
 ]
@@ -172,6 +172,7 @@ def lib():
     L.nt_set_gemm_tile.argtypes = [P, I, I, I]
     L.nt_set_overlap.argtypes = [P, I]
     L.nt_set_workspace_poison.argtypes = [P, I]
+    L.nt_set_point_split.argtypes = [P, I]
     L.nt_set_gemm_3m.argtypes = [P, I]
     L.nt_profile_enable.argtypes = [P, I]
     L.nt_profile_get.argtypes = [P, I, C.POINTER(C.c_longlong), C.POINTER(D), C.POINTER(D)]

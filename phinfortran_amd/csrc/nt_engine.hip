@@ -2139,6 +2139,161 @@ int nt_profile_get(nt_ctx* c, int kind, long long* launches, double* total_ms, d
 
 double nt_last_mean_iters(nt_ctx* c) { return c ? c->last_mean_iters : 0.0; }
 
+// Worker contexts of c (created once, kept until nt_destroy) given c's
+// settings and each a workspace cap of its share of the device memory left
+// (its own workspace from earlier calls counts as available to it).
+static int prepare_workers(nt_ctx* c, int W)
+{
+    while ((int)c->subs.size() < W) {
+        nt_ctx* s = nullptr;
+        const int st = nt_create(&s, c->device);
+        if (st) return st;
+        c->subs.push_back(s);
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess) return NT_ERR_HIP;
+    size_t avail = free_b;
+    for (int w = 0; w < W; ++w) avail += c->subs[w]->ws_bytes;
+    const size_t share = std::max<size_t>(size_t(2) << 30, (size_t)(0.6 * (double)avail / W));
+    for (int w = 0; w < W; ++w) {
+        nt_ctx* s = c->subs[w];
+        s->ws_cap = share;
+        s->overlap = c->overlap;
+        s->gemm_m3 = c->gemm_m3;
+        s->batch_limit = c->batch_limit;
+        s->one_sweep = c->one_sweep;
+        s->dec_check = c->dec_check;
+        s->adj_ckpt = c->adj_ckpt;
+        s->inv_nb = c->inv_nb;
+        s->gemm_tile = c->gemm_tile;
+        s->prof = c->prof;
+        s->ws_poison = c->ws_poison;
+        s->err.clear();
+    }
+    return NT_OK;
+}
+
+// Solve the jobs on W prepared worker contexts, each on its own stream with
+// its own workspace from its own host thread, each worker taking the next
+// job when it is done.  The workers start after c's stream has reached this
+// call and c's stream waits for all of them; their profile counts, mixed
+// counters and mean decimation iterations go to c.  Runs inside guarded().
+static void run_on_workers(nt_ctx* c, const nt_params* p, int W, const std::vector<const TdbArgs*>& jobs)
+{
+    const int nj = (int)jobs.size();
+    hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
+    for (int w = 0; w < W; ++w) hipcheck(c, hipStreamWaitEvent(c->subs[w]->stream, c->fork, 0), "wait fork");
+    std::atomic<int> next{0};
+    std::mutex mu;
+    int fail = 0;
+    std::string fail_err;
+    std::vector<double> iters_w(W, 0.0), pts_w(W, 0.0);
+    auto work = [&](int w) {
+        nt_ctx* s = c->subs[w];
+        try {
+            hipcheck(s, hipSetDevice(s->device), "hipSetDevice");
+            s->regions = c->regions;                       // the caller's buffers
+            if (s->ws) reg_region(s, s->ws, s->ws_bytes);
+            for (;;) {
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (fail) break;
+                }
+                const int j = next.fetch_add(1);
+                if (j >= nj) break;
+                solve_tdb_dev(s, p, *jobs[j]);
+                if (s->last_mean_iters >= 0.0) {
+                    iters_w[w] += s->last_mean_iters * jobs[j]->npts;
+                    pts_w[w] += jobs[j]->npts;
+                }
+            }
+            if (s->prof) harvest_profile(s);
+        } catch (const Fail& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!fail) {
+                fail = e.code;
+                fail_err = s->err;
+            }
+        } catch (const std::bad_alloc&) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!fail) {
+                fail = NT_ERR_NOMEM;
+                fail_err = "host allocation failed";
+            }
+        } catch (...) {
+            // nothing may leave a worker thread
+            std::lock_guard<std::mutex> lk(mu);
+            if (!fail) {
+                fail = NT_ERR_HIP;
+                fail_err = "unexpected error in a worker context";
+            }
+        }
+    };
+    std::vector<std::thread> threads;
+    threads.reserve(W);
+    for (int w = 0; w < W; ++w) threads.emplace_back(work, w);
+    for (auto& t : threads) t.join();
+    double it_sum = 0.0, pt_sum = 0.0;
+    for (int w = 0; w < W; ++w) {
+        nt_ctx* s = c->subs[w];
+        hipcheck(c, hipEventRecord(s->join, s->stream), "event join");
+        hipcheck(c, hipStreamWaitEvent(c->stream, s->join, 0), "wait join");
+        for (int k = 0; k < NT_K_COUNT; ++k) {
+            c->launches[k] += s->launches[k];
+            c->ms[k] += s->ms[k];
+            c->flops[k] += s->flops[k];
+            s->launches[k] = 0;
+            s->ms[k] = 0;
+            s->flops[k] = 0;
+        }
+        for (int k = 0; k < 3; ++k) {
+            c->mixed_counts[k] += s->mixed_counts[k];
+            s->mixed_counts[k] = 0;
+        }
+        it_sum += iters_w[w];
+        pt_sum += pts_w[w];
+    }
+    c->last_mean_iters = pt_sum > 0.0 ? it_sum / pt_sum : -1.0;
+    if (fail) {
+        c->err = fail_err;
+        throw Fail{fail};
+    }
+}
+
+// the part [p0, p0 + n) of a batched solve's points
+static TdbArgs point_part(const TdbArgs& a, int p0, int n)
+{
+    auto o = [&](auto* ptr, long per) { return ptr ? ptr + (long)p0 * per : ptr; };
+    const long nll = (long)a.nl * a.nl, nrr = (long)a.nr * a.nr, nlr = (long)a.nl * a.nr;
+    TdbArgs t = a;
+    t.npts = n;
+    t.E = a.E + p0;
+    t.kidx = o(a.kidx, 1);
+    t.first_mask = o(a.first_mask, 1);
+    t.T_LR = a.T_LR + p0;
+    t.T_RL = a.T_RL + p0;
+    t.status = o(a.status, 1);
+    t.iters = o(a.iters, 1);
+    nt_solve_extras& x = t.ex;
+    x.gl_adv = o(x.gl_adv, nll);
+    x.gr_ret = o(x.gr_ret, nrr);
+    x.gl_plus = o(x.gl_plus, nll);
+    x.gr_minus = o(x.gr_minus, nrr);
+    x.gml_adv = o(x.gml_adv, nll);
+    x.gmr_ret = o(x.gmr_ret, nrr);
+    x.sl_adv = o(x.sl_adv, nll);
+    x.sr_ret = o(x.sr_ret, nrr);
+    x.G_D_LR = o(x.G_D_LR, nlr);
+    x.G_D_RL = o(x.G_D_RL, nlr);
+    x.G_L = o(x.G_L, nll);
+    x.G_R = o(x.G_R, nrr);
+    x.sdos_l = o(x.sdos_l, 1);
+    x.sdos_r = o(x.sdos_r, 1);
+    x.ldos = o(x.ldos, (long)a.L * a.b);
+    x.ddos = o(x.ddos, 1);
+    return t;
+}
+
 static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,
                      const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
                      const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
@@ -2220,6 +2375,81 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
             ps = ProbeSpec{pd->temperature, pd->t_left,  pd->t_right,    pd->B_probe,     pd->T_probe, pd->lifetime,
                            po->flux,        po->jacobian, po->flux_left, po->flux_right, po->a_sum,   po->gn_sum};
             a.probe = &ps;
+        }
+        // the FP64 solve of many points in parts solved side by side by worker
+        // contexts: the latency-bound launches of one part (inverses,
+        // decimation steps of few unconverged items) leave compute units to
+        // the other's GEMMs
+        const int parts = c->point_split;
+        if (parts > 1 && !noalloc && !pd && p->precision == NT_PREC_FP64 && npts >= parts * NT_SPLIT_MIN_POINTS) {
+            if (const int st = prepare_workers(c, parts)) {
+                c->err = "could not set up the worker contexts of a split solve";
+                throw Fail{st};
+            }
+            std::vector<TdbArgs> part(parts);
+            std::vector<const TdbArgs*> jobs(parts);
+            std::vector<int> cnt(parts), off(parts);
+            for (int q = 0, p0 = 0; q < parts; ++q) {
+                cnt[q] = npts / parts + (q < npts % parts ? 1 : 0);
+                off[q] = p0;
+                p0 += cnt[q];
+            }
+            // Without extras the parts take every parts-th point (q, q + parts,
+            // ...), gathered into the context's workspace and the results
+            // scattered back: neighbouring energies need similar numbers of
+            // decimation steps, so interleaved parts take equal times.  With
+            // extras the parts are contiguous ranges of the caller's buffers.
+            const nt_solve_extras& x = a.ex;
+            const bool extras_any = x.gl_adv || x.gr_ret || x.gl_plus || x.gr_minus || x.gml_adv || x.gmr_ret ||
+                                    x.sl_adv || x.sr_ret || x.G_D_LR || x.G_D_RL || x.G_L || x.G_R || x.sdos_l ||
+                                    x.sdos_r || x.ldos || x.ddos;
+            if (!extras_any) {
+                const size_t nd = sizeof(double) * npts, ni = sizeof(int) * npts;
+                Arena pa;
+                const size_t oE = pa.take(nd), oTl = pa.take(nd), oTr = pa.take(nd), oK = pa.take(ni),
+                             oM = pa.take(ni), oS = pa.take(ni), oI = pa.take(ni);
+                ensure_ws(c, pa.off);
+                pa.base = static_cast<char*>(c->ws);
+                TdbArgs g = a;
+                g.E = pa.ptr<double>(oE);
+                g.T_LR = pa.ptr<double>(oTl);
+                g.T_RL = pa.ptr<double>(oTr);
+                g.kidx = a.kidx ? pa.ptr<int>(oK) : nullptr;
+                g.first_mask = a.first_mask ? pa.ptr<int>(oM) : nullptr;
+                g.status = a.status ? pa.ptr<int>(oS) : nullptr;
+                g.iters = a.iters ? pa.ptr<int>(oI) : nullptr;
+                // element q + parts i of the caller's array <-> element off[q] + i of the gathered one
+                auto move = [&](void* gath, const void* user, size_t w, bool in) {
+                    for (int q = 0; q < parts; ++q) {
+                        char* gq = static_cast<char*>(gath) + w * off[q];
+                        char* uq = static_cast<char*>(const_cast<void*>(user)) + w * q;
+                        hipcheck(c, in ? hipMemcpy2DAsync(gq, w, uq, w * parts, w, cnt[q], hipMemcpyDeviceToDevice,
+                                                          c->stream)
+                                       : hipMemcpy2DAsync(uq, w * parts, gq, w, w, cnt[q], hipMemcpyDeviceToDevice,
+                                                          c->stream),
+                                 "point gather");
+                    }
+                };
+                move(const_cast<double*>(g.E), a.E, sizeof(double), true);
+                if (a.kidx) move(const_cast<int*>(g.kidx), a.kidx, sizeof(int), true);
+                if (a.first_mask) move(const_cast<int*>(g.first_mask), a.first_mask, sizeof(int), true);
+                for (int q = 0; q < parts; ++q) {
+                    part[q] = point_part(g, off[q], cnt[q]);
+                    jobs[q] = &part[q];
+                }
+                run_on_workers(c, p, parts, jobs);
+                move(g.T_LR, a.T_LR, sizeof(double), false);
+                move(g.T_RL, a.T_RL, sizeof(double), false);
+                if (a.status) move(g.status, a.status, sizeof(int), false);
+                if (a.iters) move(g.iters, a.iters, sizeof(int), false);
+                return 0;
+            }
+            for (int q = 0; q < parts; ++q) {
+                part[q] = point_part(a, off[q], cnt[q]);
+                jobs[q] = &part[q];
+            }
+            run_on_workers(c, p, parts, jobs);
+            return 0;
         }
         solve_tdb_dev(c, p, a);
         return 0;
@@ -2384,35 +2614,7 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
     const int ng = (int)groups.size();
     const int W = std::min(std::max(1, c->multi_workers), ng);
     if (W > 1) {
-        // worker contexts, created once per context and given its settings
-        while ((int)c->subs.size() < W) {
-            nt_ctx* s = nullptr;
-            const int st = nt_create(&s, c->device);
-            if (st) return st;
-            c->subs.push_back(s);
-        }
-        // each worker's workspace within its share of the device memory left
-        // (its own workspace from earlier calls counts as available to it)
-        size_t free_b = 0, total_b = 0;
-        if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess) return NT_ERR_HIP;
-        size_t avail = free_b;
-        for (int w = 0; w < W; ++w) avail += c->subs[w]->ws_bytes;
-        const size_t share = std::max<size_t>(size_t(2) << 30, (size_t)(0.6 * (double)avail / W));
-        for (int w = 0; w < W; ++w) {
-            nt_ctx* s = c->subs[w];
-            s->ws_cap = share;
-            s->overlap = c->overlap;
-            s->gemm_m3 = c->gemm_m3;
-            s->batch_limit = c->batch_limit;
-            s->one_sweep = c->one_sweep;
-            s->dec_check = c->dec_check;
-            s->adj_ckpt = c->adj_ckpt;
-            s->inv_nb = c->inv_nb;
-            s->gemm_tile = c->gemm_tile;
-            s->prof = c->prof;
-            s->ws_poison = c->ws_poison;
-            s->err.clear();
-        }
+        if (const int st = prepare_workers(c, W)) return st;
     }
     return guarded(c, [&] {
         const size_t z = sizeof(nt_c128);
@@ -2460,79 +2662,11 @@ int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_dev
             for (int gi = 0; gi < ng; ++gi) solve_tdb_dev(c, p, args[gi][0]);
             return 0;
         }
-        // Groups go to W worker contexts, largest first, each worker taking
-        // the next group when it is done: every worker solves on its own
-        // stream with its own workspace from its own host thread, so small
-        // devices, whose launches do not fill the GPU, run side by side.
-        // The workers start after the caller's stream has reached this call
-        // and the caller's stream waits for all of them.
-        hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
-        for (int w = 0; w < W; ++w) hipcheck(c, hipStreamWaitEvent(c->subs[w]->stream, c->fork, 0), "wait fork");
-        std::atomic<int> next{0};
-        std::mutex mu;
-        int fail = 0;
-        std::string fail_err;
-        auto work = [&](nt_ctx* s) {
-            try {
-                hipcheck(s, hipSetDevice(s->device), "hipSetDevice");
-                s->regions = c->regions;                       // the caller's buffers
-                if (s->ws) reg_region(s, s->ws, s->ws_bytes);
-                for (;;) {
-                    {
-                        std::lock_guard<std::mutex> lk(mu);
-                        if (fail) break;
-                    }
-                    const int gi = next.fetch_add(1);
-                    if (gi >= ng) break;
-                    solve_tdb_dev(s, p, args[gi][0]);
-                }
-                if (s->prof) harvest_profile(s);
-            } catch (const Fail& e) {
-                std::lock_guard<std::mutex> lk(mu);
-                if (!fail) {
-                    fail = e.code;
-                    fail_err = s->err;
-                }
-            } catch (const std::bad_alloc&) {
-                std::lock_guard<std::mutex> lk(mu);
-                if (!fail) {
-                    fail = NT_ERR_NOMEM;
-                    fail_err = "host allocation failed";
-                }
-            } catch (...) {
-                // nothing may leave a worker thread
-                std::lock_guard<std::mutex> lk(mu);
-                if (!fail) {
-                    fail = NT_ERR_HIP;
-                    fail_err = "unexpected error in a worker context";
-                }
-            }
-        };
-        std::vector<std::thread> threads;
-        threads.reserve(W);
-        for (int w = 0; w < W; ++w) threads.emplace_back(work, c->subs[w]);
-        for (auto& t : threads) t.join();
-        for (int w = 0; w < W; ++w) {
-            nt_ctx* s = c->subs[w];
-            hipcheck(c, hipEventRecord(s->join, s->stream), "event join");
-            hipcheck(c, hipStreamWaitEvent(c->stream, s->join, 0), "wait join");
-            for (int k = 0; k < NT_K_COUNT; ++k) {
-                c->launches[k] += s->launches[k];
-                c->ms[k] += s->ms[k];
-                c->flops[k] += s->flops[k];
-                s->launches[k] = 0;
-                s->ms[k] = 0;
-                s->flops[k] = 0;
-            }
-            for (int k = 0; k < 3; ++k) {
-                c->mixed_counts[k] += s->mixed_counts[k];
-                s->mixed_counts[k] = 0;
-            }
-        }
-        if (fail) {
-            c->err = fail_err;
-            throw Fail{fail};
-        }
+        // Groups go to W worker contexts, largest first: small devices, whose
+        // launches do not fill the GPU, run side by side.
+        std::vector<const TdbArgs*> jobs(ng);
+        for (int gi = 0; gi < ng; ++gi) jobs[gi] = &args[gi][0];
+        run_on_workers(c, p, W, jobs);
         return 0;
     });
 }
@@ -2541,6 +2675,13 @@ int nt_set_multi_workers(nt_ctx* c, int n)
 {
     if (!c || n < 1 || n > 16) return NT_ERR_ARG;
     c->multi_workers = n;
+    return NT_OK;
+}
+
+int nt_set_point_split(nt_ctx* c, int n)
+{
+    if (!c || n < 1 || n > 8) return NT_ERR_ARG;
+    c->point_split = n;
     return NT_OK;
 }
 

@@ -21,6 +21,9 @@
 
 using namespace nt;
 
+// smallest part of a batched solve split over worker contexts
+constexpr int NT_SPLIT_MIN_POINTS = 256;
+
 struct nt_ctx {
     int device = 0;
     hipStream_t own = nullptr;
@@ -85,6 +88,9 @@ struct nt_ctx {
     // uneven-device calls: worker contexts (own stream and workspace, one host
     // thread each) that solve groups of devices concurrently
     int multi_workers = 4;
+    // batched FP64 solves of at least point_split * NT_SPLIT_MIN_POINTS points
+    // run as point_split parts side by side on worker contexts (1: one stream)
+    int point_split = 2;
     std::vector<nt_ctx*> subs;
 };
 

@@ -638,6 +638,11 @@ class Engine:
         or on one."""
         self._check(self._L.nt_set_overlap(self.ctx, int(bool(on))), "nt_set_overlap")
 
+    def set_point_split(self, n: int):
+        """Batched FP64 solves of at least n * 256 points run as n parts side
+        by side on worker contexts (default 2); 1: one stream."""
+        self._check(self._L.nt_set_point_split(self.ctx, int(n)), "nt_set_point_split")
+
     def set_workspace_poison(self, on: bool):
         """Debugging aid: fill the workspace with NaN bit patterns at the start
         of every call, so that a read of workspace the call did not write

@@ -81,3 +81,41 @@ def test_multi_device_workers(engine, T, precision):
         for k in ("T_LR", "T_RL"):
             assert torch.equal(a[k], b[k]), (i, k)
         assert torch.equal(t, a["T_LR"]), i
+
+
+@pytest.mark.parametrize("with_extras", [True, False])
+@pytest.mark.parametrize("parts,npts", [(2, 600), (3, 800)])
+def test_point_split_equals_one_stream(engine, T, parts, npts, with_extras):
+    """A batched FP64 solve of at least parts x 256 points runs as parts
+    solved side by side by worker contexts (nt_set_point_split): bitwise the
+    T, statuses, iteration counts and extras of the solve on one stream, and
+    the caller's stream sees them without a further synchronisation.  With
+    extras the parts are contiguous ranges; without, every parts-th point
+    (gathered and scattered back)."""
+    import torch
+    ks = lt.mp_grid(2, 2)
+    blocks = [lt.si_ge_superlattice_film(2, 2, 6, kf, seed=7) for kf in ks]
+    d = engine.upload_blocks(blocks)
+    f = 16.0 * (np.arange(npts) + 0.5) / npts
+    E = f ** 2
+    kidx = (np.arange(npts) % len(ks)).astype(np.int32)
+    mask = np.zeros(npts, np.int32)
+    mask[0] = 1
+    p = T.params("phonon")
+    ex = ("G_D_RL", "gl_adv", "G_L", "sdos_l", "ldos", "ddos") if with_extras else ()
+    print(f"split solve, {parts} parts of {npts} points, extras {with_extras}", flush=True)
+    try:
+        engine.set_point_split(1)
+        ref = engine.solve_tdb(d, E, p, kidx=kidx, first_point_mask=mask, extras=ex)
+        it1 = engine.last_mean_iters()
+        engine.set_point_split(parts)
+        got = engine.solve_tdb(d, E, p, kidx=kidx, first_point_mask=mask, extras=ex, stream_sync=False)
+        snap = {k: v.clone() for k, v in got.items()}       # on the caller's stream
+        torch.cuda.synchronize()
+        it2 = engine.last_mean_iters()
+    finally:
+        engine.set_point_split(2)
+    assert (ref["status"].cpu().numpy() == 0).all()
+    for k in ("T_LR", "T_RL", "status", "iters") + ex:
+        assert torch.equal(ref[k], snap[k]), k
+    assert abs(it1 - it2) < 1e-9 * max(1.0, it1)

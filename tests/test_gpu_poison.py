@@ -69,6 +69,18 @@ def test_solve_paths(engine, T, poison_off, sweeps, precision):
           lambda: engine.solve_tdb(d, E, p, kidx=kidx, noalloc=True))
 
 
+def test_split_solve(engine, T, poison_off):
+    """the batched solve of 600 points as two interleaved parts on worker
+    contexts (gathered inputs and outputs in the context's workspace)"""
+    blocks = [lt.random_blocks(12, 12, 4, seed=90 + i, scale=0.6) for i in range(2)]
+    d = engine.upload_blocks(blocks)
+    E = np.linspace(0.05, 0.9, 600)
+    kidx = (np.arange(600) % 2).astype(np.int32)
+    p = T.params("phonon", eta=1e-3, eta0=1e-4)
+    r = _both(engine, "split solve", lambda: engine.solve_tdb(d, E, p, kidx=kidx))
+    assert (r["status"].cpu().numpy() == 0).all()
+
+
 def test_device_dos(engine, T, poison_off):
     blk = lt.random_blocks(10, 10, 6, seed=3)
     d = engine.upload_blocks(blk)

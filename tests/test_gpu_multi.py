@@ -119,3 +119,33 @@ def test_point_split_equals_one_stream(engine, T, parts, npts, with_extras):
     for k in ("T_LR", "T_RL", "status", "iters") + ex:
         assert torch.equal(ref[k], snap[k]), k
     assert abs(it1 - it2) < 1e-9 * max(1.0, it1)
+
+
+@pytest.mark.parametrize("case", ["unequal_leads", "circular"])
+def test_point_split_other_lead_paths(engine, T, case):
+    """The two-part solve with leads of different sizes (one decimation batch
+    per lead) and with the block-circulant decimation: bitwise the
+    one-stream results, with and without extras."""
+    import torch
+    if case == "unequal_leads":
+        blocks = [lt.random_blocks(8, 12, 4, nr=16, seed=40 + i, scale=0.7) for i in range(2)]
+        p = T.params("phonon", eta=1e-3, eta0=1e-4)
+    else:
+        blocks = [lt.circulant_lead_blocks(3, 2, 2, 3, seed=s) for s in (1, 2)]
+        p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13, circular=1, n_bloch_x=2, n_bloch_y=2)
+    d = engine.upload_blocks(blocks)
+    npts = 520
+    E = np.linspace(0.05, 0.9, npts)
+    kidx = (np.arange(npts) % 2).astype(np.int32)
+    print(f"split solve, {case}", flush=True)
+    for ex in ((), ("G_D_RL", "G_L")):
+        try:
+            engine.set_point_split(1)
+            ref = engine.solve_tdb(d, E, p, kidx=kidx, extras=ex)
+            engine.set_point_split(2)
+            got = engine.solve_tdb(d, E, p, kidx=kidx, extras=ex)
+        finally:
+            engine.set_point_split(2)
+        assert (ref["status"].cpu().numpy() == 0).all(), (case, ex)
+        for k in ("T_LR", "T_RL", "status", "iters") + ex:
+            assert torch.equal(ref[k], got[k]), (case, ex, k)

@@ -287,9 +287,13 @@ int nt_set_multi_workers(nt_ctx* ctx, int n);
  * n-th point from q (gathered in the context's workspace and scattered
  * back, so the parts need similar numbers of decimation steps); with
  * extras the parts are contiguous ranges.  Results are bitwise those of one
- * part.  n = 2 (default) .. 8; n = 1: every point on the context's own
- * stream.  The allocation-free entry, probes and NT_PREC_MIXED always run on
- * one stream.  No counterpart in the reference. */
+ * part.  The adjoint solve (nt_solve_tdb_adjoint_batched_dev) splits the
+ * same way, interleaved; part 0 accumulates into the caller's gradients,
+ * the others into zeroed workspace summed into them at the end (the
+ * gradients then differ from one part's in the rounding of that order).
+ * n = 2 (default) .. 8; n = 1: every point on the context's own stream.
+ * The allocation-free entry, probes and NT_PREC_MIXED always run on one
+ * stream.  No counterpart in the reference. */
 int nt_set_point_split(nt_ctx* ctx, int n);
 
 /* Host-pointer variant (copies in and out; same layout). */

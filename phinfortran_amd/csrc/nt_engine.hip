@@ -635,6 +635,13 @@ __global__ void k_scale_points(double2* A, long n, long stride, const double* w,
     A[(long)p * stride + e] = make_double2(v.x * f, v.y * f);
 }
 
+// out[i] += in[i] (n doubles): the gradient of one part of a split adjoint solve
+__global__ void k_add_doubles(double* __restrict__ out, const double* __restrict__ in, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += in[i];
+}
+
 // out[k][blk][e] += 2 sum_{p : k(p) = k, point ok} Z[blk][p][e]
 __global__ void k_adj_reduce(const double2* Z, long mm, int P, long ne, const int* kidx, const int* stdec,
                              const int* strgf, const int* first_mask, double2* out, int nblk)
@@ -2549,6 +2556,83 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* c, const nt_params* p, int nl, int 
         a.adj_hi = hi;
         a.dHd = dW_dHamd;
         a.dHdv = nc > 0 ? dW_dHamdv : nullptr;
+        // many points: parts solved side by side by worker contexts, as the
+        // batched solve; every parts-th point (gathered in the context's
+        // workspace, results scattered back), part 0 accumulating into the
+        // caller's gradients, the others into zeroed buffers added at the end
+        const int parts = c->point_split;
+        if (parts > 1 && npts >= parts * NT_SPLIT_MIN_POINTS) {
+            if (const int st = prepare_workers(c, parts)) {
+                c->err = "could not set up the worker contexts of a split solve";
+                throw Fail{st};
+            }
+            std::vector<int> cnt(parts), off(parts);
+            for (int q = 0, p0 = 0; q < parts; ++q) {
+                cnt[q] = npts / parts + (q < npts % parts ? 1 : 0);
+                off[q] = p0;
+                p0 += cnt[q];
+            }
+            const size_t nd = sizeof(double) * npts, ni = sizeof(int) * npts;
+            const size_t gd = z * bb * (hi - lo + 1) * nk, gv = nc > 0 ? z * bb * nc * nk : 0;
+            Arena pa;
+            const size_t oE = pa.take(nd), oW = pa.take(nd), oTl = pa.take(nd), oTr = pa.take(nd),
+                         oK = pa.take(ni), oM = pa.take(ni), oS = pa.take(ni), oI = pa.take(ni),
+                         oG = pa.take(gd * (parts - 1)), oGv = pa.take(gv * (parts - 1));
+            ensure_ws(c, pa.off);
+            pa.base = static_cast<char*>(c->ws);
+            TdbArgs g = a;
+            g.E = pa.ptr<double>(oE);
+            g.adj_w = pa.ptr<double>(oW);
+            g.T_LR = pa.ptr<double>(oTl);
+            g.T_RL = pa.ptr<double>(oTr);
+            g.kidx = kidx ? pa.ptr<int>(oK) : nullptr;
+            g.first_mask = first_point_mask ? pa.ptr<int>(oM) : nullptr;
+            g.status = status ? pa.ptr<int>(oS) : nullptr;
+            g.iters = iters ? pa.ptr<int>(oI) : nullptr;
+            auto move = [&](void* gath, const void* user, size_t w, bool in) {
+                for (int q = 0; q < parts; ++q) {
+                    char* gq = static_cast<char*>(gath) + w * off[q];
+                    char* uq = static_cast<char*>(const_cast<void*>(user)) + w * q;
+                    hipcheck(c, in ? hipMemcpy2DAsync(gq, w, uq, w * parts, w, cnt[q], hipMemcpyDeviceToDevice,
+                                                      c->stream)
+                                   : hipMemcpy2DAsync(uq, w * parts, gq, w, w, cnt[q], hipMemcpyDeviceToDevice,
+                                                      c->stream),
+                             "point gather");
+                }
+            };
+            move(const_cast<double*>(g.E), E, sizeof(double), true);
+            move(const_cast<double*>(g.adj_w), weight, sizeof(double), true);
+            if (kidx) move(const_cast<int*>(g.kidx), kidx, sizeof(int), true);
+            if (first_point_mask) move(const_cast<int*>(g.first_mask), first_point_mask, sizeof(int), true);
+            zero_dev(c, pa.ptr<char>(oG), gd * (parts - 1), c->stream);
+            if (gv) zero_dev(c, pa.ptr<char>(oGv), gv * (parts - 1), c->stream);
+            std::vector<TdbArgs> part(parts);
+            std::vector<const TdbArgs*> jobs(parts);
+            for (int q = 0; q < parts; ++q) {
+                part[q] = point_part(g, off[q], cnt[q]);
+                part[q].adj_w = g.adj_w + off[q];
+                if (q > 0) {
+                    part[q].dHd = reinterpret_cast<nt_c128*>(pa.ptr<char>(oG) + gd * (q - 1));
+                    part[q].dHdv = gv ? reinterpret_cast<nt_c128*>(pa.ptr<char>(oGv) + gv * (q - 1)) : nullptr;
+                }
+                jobs[q] = &part[q];
+            }
+            run_on_workers(c, p, parts, jobs);
+            move(g.T_LR, T_LR, sizeof(double), false);
+            move(g.T_RL, T_RL, sizeof(double), false);
+            if (status) move(g.status, status, sizeof(int), false);
+            if (iters) move(g.iters, iters, sizeof(int), false);
+            for (int q = 1; q < parts; ++q) {
+                const long n1 = (long)(gd / sizeof(double)), n2 = (long)(gv / sizeof(double));
+                hipLaunchKernelGGL(k_add_doubles, dim3(nblk(n1)), dim3(256), 0, c->stream,
+                                   reinterpret_cast<double*>(dW_dHamd), pa.ptr<double>(oG + gd * (q - 1)), n1);
+                if (gv)
+                    hipLaunchKernelGGL(k_add_doubles, dim3(nblk(n2)), dim3(256), 0, c->stream,
+                                       reinterpret_cast<double*>(dW_dHamdv), pa.ptr<double>(oGv + gv * (q - 1)), n2);
+                hipcheck(c, hipGetLastError(), "gradient sum");
+            }
+            return 0;
+        }
         solve_tdb_dev(c, p, a);
         return 0;
     });

@@ -149,3 +149,36 @@ def test_point_split_other_lead_paths(engine, T, case):
         assert (ref["status"].cpu().numpy() == 0).all(), (case, ex)
         for k in ("T_LR", "T_RL", "status", "iters") + ex:
             assert torch.equal(ref[k], got[k]), (case, ex, k)
+
+
+def test_point_split_adjoint(engine, T):
+    """The adjoint solve of at least 2 x 256 points runs as two interleaved
+    parts on worker contexts, the second part's gradients summed into the
+    caller's at the end: T, statuses and iteration counts bitwise the
+    one-stream solve's, the gradients to the rounding of the summation order
+    (1e-12 of their largest element)."""
+    import torch
+    blocks = [lt.random_blocks(12, 12, 5, seed=60 + i, scale=0.7) for i in range(2)]
+    d = engine.upload_blocks(blocks)
+    p = T.params("phonon", eta=1e-3, eta0=1e-4, convergence=1e-13)
+    npts = 530
+    E = np.linspace(0.05, 0.9, npts)
+    kidx = (np.arange(npts) % 2).astype(np.int32)
+    w = np.random.default_rng(8).uniform(0.5, 2.0, npts)
+    mask = np.zeros(npts, np.int32)
+    mask[7] = 1
+    print("split adjoint solve, 2 parts of 530 points", flush=True)
+    try:
+        engine.set_point_split(1)
+        ref = engine.solve_tdb_adjoint(d, E, p, w, 1, 3, kidx=kidx, first_point_mask=mask)
+        engine.set_point_split(2)
+        got = engine.solve_tdb_adjoint(d, E, p, w, 1, 3, kidx=kidx, first_point_mask=mask)
+    finally:
+        engine.set_point_split(2)
+    assert (ref["status"].cpu().numpy() == 0).all()
+    for k in ("T_LR", "T_RL", "status", "iters"):
+        assert torch.equal(ref[k], got[k]), k
+    for k in ("dHamd", "dHamdv"):
+        a, b = ref[k].cpu().numpy(), got[k].cpu().numpy()
+        assert np.abs(a).max() > 0
+        assert np.abs(a - b).max() < 1e-12 * np.abs(a).max(), k

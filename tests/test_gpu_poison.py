@@ -79,6 +79,11 @@ def test_split_solve(engine, T, poison_off):
     p = T.params("phonon", eta=1e-3, eta0=1e-4)
     r = _both(engine, "split solve", lambda: engine.solve_tdb(d, E, p, kidx=kidx))
     assert (r["status"].cpu().numpy() == 0).all()
+    # the adjoint solve split the same way (the second part's gradients in
+    # zeroed workspace, summed into the caller's)
+    w = np.linspace(0.5, 2.0, 600)
+    r = _both(engine, "split adjoint", lambda: engine.solve_tdb_adjoint(d, E, p, w, 1, 2, kidx=kidx))
+    assert (r["status"].cpu().numpy() == 0).all() and np.abs(r["dHamd"].cpu().numpy()).max() > 0
 
 
 def test_device_dos(engine, T, poison_off):

@@ -17,6 +17,7 @@
 #include <type_traits>
 
 #include "nt_host.h"
+#include "nt_solve.h"
 
 // Static device scratch of the allocation-free batched solve
 // (nt_solve_tdb_batched_noalloc_dev).  Each GPU the library runs on holds its
@@ -635,13 +636,6 @@ __global__ void k_scale_points(double2* A, long n, long stride, const double* w,
     A[(long)p * stride + e] = make_double2(v.x * f, v.y * f);
 }
 
-// out[i] += in[i] (n doubles): the gradient of one part of a split adjoint solve
-__global__ void k_add_doubles(double* __restrict__ out, const double* __restrict__ in, long n)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += in[i];
-}
-
 // out[k][blk][e] += 2 sum_{p : k(p) = k, point ok} Z[blk][p][e]
 __global__ void k_adj_reduce(const double2* Z, long mm, int P, long ne, const int* kidx, const int* stdec,
                              const int* strgf, const int* first_mask, double2* out, int nblk)
@@ -1121,12 +1115,17 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
 
 // ------------------------------------------------------- Buttiker probes
 // Probe outputs of a batched solve (negf/lesser.f90:1199-1503).
+}  // namespace
+
+// the probe path's inputs and outputs (device pointers), carried by TdbArgs
 struct ProbeSpec {
     const double* temp;       // L probe temperatures (device)
     double t_left, t_right, B, C;
     int lifetime;
     double *flux, *jac, *flux_left, *flux_right, *a_sum, *gn_sum;
 };
+
+namespace {
 
 // complex traces kept per point by buttiker_run
 inline long probe_acc_n(int L) { return (long)L * L + 5 * (long)L + 6; }
@@ -1352,33 +1351,12 @@ void buttiker_run(nt_ctx* c, int nl, int nr, const RgfInput<double>& in, int P, 
 }
 
 // ------------------------------------------------------- tdb batched solve
-struct TdbArgs {
-    int nl, nr, b, L, npts, nk;
-    const double* E;
-    const int* kidx;
-    const nt_c128 *Haml, *Hamlv, *Hamr, *Hamrv, *Hamd, *Hamdv, *Hamld, *Hamrd;
-    const int* first_mask;
-    double *T_LR, *T_RL;
-    int *status, *iters;
-    nt_solve_extras ex;
-    // adjoint of sum_p w_p T_LR(p) (w null: no adjoint)
-    const double* adj_w = nullptr;
-    int adj_lo = 0, adj_hi = -1;
-    nt_c128* dHd = nullptr;    // (nk, hi - lo + 1, b, b), accumulated
-    nt_c128* dHdv = nullptr;   // (nk, min(hi, L-2) - lo + 1, b, b), accumulated
-    const ProbeSpec* probe = nullptr;
-    // workspace in the static scratch (no device allocation); the chunk of
-    // points shrinks until the plan fits
-    bool noalloc = false;
-    // devices with the same leads, points (E, k map) and lead sizes: their
-    // RGF sweeps reuse this call's lead Green's functions (FP64, no extras)
-    std::vector<const TdbArgs*> siblings;
-};
-
 // adjoint storage is capped so that a chunk of points fits this budget
 constexpr double ADJ_BUDGET_BYTES = 24.0 * 1024 * 1024 * 1024;
 // so is the mixed-precision solve's (FP32 factors and FP64 columns of both directions)
 constexpr double MIXED_BUDGET_BYTES = 64.0 * 1024 * 1024 * 1024;
+
+}  // namespace
 
 void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
 {
@@ -1855,8 +1833,6 @@ void solve_tdb_dev(nt_ctx* c, const nt_params* prm, const TdbArgs& a)
     c->last_mean_iters = c->dec_check > 0 ? iters_sum / std::max(1, 2 * a.npts) : -1.0;
 }
 
-}  // namespace
-
 // =================================================================== C ABI
 // ------------------------------------------------------------- autotuner
 // Deterministic diagonally dominant test matrices: entry hash in [-0.5, 0.5)
@@ -2146,161 +2122,6 @@ int nt_profile_get(nt_ctx* c, int kind, long long* launches, double* total_ms, d
 
 double nt_last_mean_iters(nt_ctx* c) { return c ? c->last_mean_iters : 0.0; }
 
-// Worker contexts of c (created once, kept until nt_destroy) given c's
-// settings and each a workspace cap of its share of the device memory left
-// (its own workspace from earlier calls counts as available to it).
-static int prepare_workers(nt_ctx* c, int W)
-{
-    while ((int)c->subs.size() < W) {
-        nt_ctx* s = nullptr;
-        const int st = nt_create(&s, c->device);
-        if (st) return st;
-        c->subs.push_back(s);
-    }
-    size_t free_b = 0, total_b = 0;
-    if (hipSetDevice(c->device) != hipSuccess || hipMemGetInfo(&free_b, &total_b) != hipSuccess) return NT_ERR_HIP;
-    size_t avail = free_b;
-    for (int w = 0; w < W; ++w) avail += c->subs[w]->ws_bytes;
-    const size_t share = std::max<size_t>(size_t(2) << 30, (size_t)(0.6 * (double)avail / W));
-    for (int w = 0; w < W; ++w) {
-        nt_ctx* s = c->subs[w];
-        s->ws_cap = share;
-        s->overlap = c->overlap;
-        s->gemm_m3 = c->gemm_m3;
-        s->batch_limit = c->batch_limit;
-        s->one_sweep = c->one_sweep;
-        s->dec_check = c->dec_check;
-        s->adj_ckpt = c->adj_ckpt;
-        s->inv_nb = c->inv_nb;
-        s->gemm_tile = c->gemm_tile;
-        s->prof = c->prof;
-        s->ws_poison = c->ws_poison;
-        s->err.clear();
-    }
-    return NT_OK;
-}
-
-// Solve the jobs on W prepared worker contexts, each on its own stream with
-// its own workspace from its own host thread, each worker taking the next
-// job when it is done.  The workers start after c's stream has reached this
-// call and c's stream waits for all of them; their profile counts, mixed
-// counters and mean decimation iterations go to c.  Runs inside guarded().
-static void run_on_workers(nt_ctx* c, const nt_params* p, int W, const std::vector<const TdbArgs*>& jobs)
-{
-    const int nj = (int)jobs.size();
-    hipcheck(c, hipEventRecord(c->fork, c->stream), "event fork");
-    for (int w = 0; w < W; ++w) hipcheck(c, hipStreamWaitEvent(c->subs[w]->stream, c->fork, 0), "wait fork");
-    std::atomic<int> next{0};
-    std::mutex mu;
-    int fail = 0;
-    std::string fail_err;
-    std::vector<double> iters_w(W, 0.0), pts_w(W, 0.0);
-    auto work = [&](int w) {
-        nt_ctx* s = c->subs[w];
-        try {
-            hipcheck(s, hipSetDevice(s->device), "hipSetDevice");
-            s->regions = c->regions;                       // the caller's buffers
-            if (s->ws) reg_region(s, s->ws, s->ws_bytes);
-            for (;;) {
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (fail) break;
-                }
-                const int j = next.fetch_add(1);
-                if (j >= nj) break;
-                solve_tdb_dev(s, p, *jobs[j]);
-                if (s->last_mean_iters >= 0.0) {
-                    iters_w[w] += s->last_mean_iters * jobs[j]->npts;
-                    pts_w[w] += jobs[j]->npts;
-                }
-            }
-            if (s->prof) harvest_profile(s);
-        } catch (const Fail& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            if (!fail) {
-                fail = e.code;
-                fail_err = s->err;
-            }
-        } catch (const std::bad_alloc&) {
-            std::lock_guard<std::mutex> lk(mu);
-            if (!fail) {
-                fail = NT_ERR_NOMEM;
-                fail_err = "host allocation failed";
-            }
-        } catch (...) {
-            // nothing may leave a worker thread
-            std::lock_guard<std::mutex> lk(mu);
-            if (!fail) {
-                fail = NT_ERR_HIP;
-                fail_err = "unexpected error in a worker context";
-            }
-        }
-    };
-    std::vector<std::thread> threads;
-    threads.reserve(W);
-    for (int w = 0; w < W; ++w) threads.emplace_back(work, w);
-    for (auto& t : threads) t.join();
-    double it_sum = 0.0, pt_sum = 0.0;
-    for (int w = 0; w < W; ++w) {
-        nt_ctx* s = c->subs[w];
-        hipcheck(c, hipEventRecord(s->join, s->stream), "event join");
-        hipcheck(c, hipStreamWaitEvent(c->stream, s->join, 0), "wait join");
-        for (int k = 0; k < NT_K_COUNT; ++k) {
-            c->launches[k] += s->launches[k];
-            c->ms[k] += s->ms[k];
-            c->flops[k] += s->flops[k];
-            s->launches[k] = 0;
-            s->ms[k] = 0;
-            s->flops[k] = 0;
-        }
-        for (int k = 0; k < 3; ++k) {
-            c->mixed_counts[k] += s->mixed_counts[k];
-            s->mixed_counts[k] = 0;
-        }
-        it_sum += iters_w[w];
-        pt_sum += pts_w[w];
-    }
-    c->last_mean_iters = pt_sum > 0.0 ? it_sum / pt_sum : -1.0;
-    if (fail) {
-        c->err = fail_err;
-        throw Fail{fail};
-    }
-}
-
-// the part [p0, p0 + n) of a batched solve's points
-static TdbArgs point_part(const TdbArgs& a, int p0, int n)
-{
-    auto o = [&](auto* ptr, long per) { return ptr ? ptr + (long)p0 * per : ptr; };
-    const long nll = (long)a.nl * a.nl, nrr = (long)a.nr * a.nr, nlr = (long)a.nl * a.nr;
-    TdbArgs t = a;
-    t.npts = n;
-    t.E = a.E + p0;
-    t.kidx = o(a.kidx, 1);
-    t.first_mask = o(a.first_mask, 1);
-    t.T_LR = a.T_LR + p0;
-    t.T_RL = a.T_RL + p0;
-    t.status = o(a.status, 1);
-    t.iters = o(a.iters, 1);
-    nt_solve_extras& x = t.ex;
-    x.gl_adv = o(x.gl_adv, nll);
-    x.gr_ret = o(x.gr_ret, nrr);
-    x.gl_plus = o(x.gl_plus, nll);
-    x.gr_minus = o(x.gr_minus, nrr);
-    x.gml_adv = o(x.gml_adv, nll);
-    x.gmr_ret = o(x.gmr_ret, nrr);
-    x.sl_adv = o(x.sl_adv, nll);
-    x.sr_ret = o(x.sr_ret, nrr);
-    x.G_D_LR = o(x.G_D_LR, nlr);
-    x.G_D_RL = o(x.G_D_RL, nlr);
-    x.G_L = o(x.G_L, nll);
-    x.G_R = o(x.G_R, nrr);
-    x.sdos_l = o(x.sdos_l, 1);
-    x.sdos_r = o(x.sdos_r, 1);
-    x.ldos = o(x.ldos, (long)a.L * a.b);
-    x.ddos = o(x.ddos, 1);
-    return t;
-}
-
 static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,
                      const int* kidx, int nk, const nt_c128* Haml, const nt_c128* Hamlv, const nt_c128* Hamr,
                      const nt_c128* Hamrv, const nt_c128* Hamd, const nt_c128* Hamdv, const nt_c128* Hamld,
@@ -2389,73 +2210,7 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
         // the other's GEMMs
         const int parts = c->point_split;
         if (parts > 1 && !noalloc && !pd && p->precision == NT_PREC_FP64 && npts >= parts * NT_SPLIT_MIN_POINTS) {
-            if (const int st = prepare_workers(c, parts)) {
-                c->err = "could not set up the worker contexts of a split solve";
-                throw Fail{st};
-            }
-            std::vector<TdbArgs> part(parts);
-            std::vector<const TdbArgs*> jobs(parts);
-            std::vector<int> cnt(parts), off(parts);
-            for (int q = 0, p0 = 0; q < parts; ++q) {
-                cnt[q] = npts / parts + (q < npts % parts ? 1 : 0);
-                off[q] = p0;
-                p0 += cnt[q];
-            }
-            // Without extras the parts take every parts-th point (q, q + parts,
-            // ...), gathered into the context's workspace and the results
-            // scattered back: neighbouring energies need similar numbers of
-            // decimation steps, so interleaved parts take equal times.  With
-            // extras the parts are contiguous ranges of the caller's buffers.
-            const nt_solve_extras& x = a.ex;
-            const bool extras_any = x.gl_adv || x.gr_ret || x.gl_plus || x.gr_minus || x.gml_adv || x.gmr_ret ||
-                                    x.sl_adv || x.sr_ret || x.G_D_LR || x.G_D_RL || x.G_L || x.G_R || x.sdos_l ||
-                                    x.sdos_r || x.ldos || x.ddos;
-            if (!extras_any) {
-                const size_t nd = sizeof(double) * npts, ni = sizeof(int) * npts;
-                Arena pa;
-                const size_t oE = pa.take(nd), oTl = pa.take(nd), oTr = pa.take(nd), oK = pa.take(ni),
-                             oM = pa.take(ni), oS = pa.take(ni), oI = pa.take(ni);
-                ensure_ws(c, pa.off);
-                pa.base = static_cast<char*>(c->ws);
-                TdbArgs g = a;
-                g.E = pa.ptr<double>(oE);
-                g.T_LR = pa.ptr<double>(oTl);
-                g.T_RL = pa.ptr<double>(oTr);
-                g.kidx = a.kidx ? pa.ptr<int>(oK) : nullptr;
-                g.first_mask = a.first_mask ? pa.ptr<int>(oM) : nullptr;
-                g.status = a.status ? pa.ptr<int>(oS) : nullptr;
-                g.iters = a.iters ? pa.ptr<int>(oI) : nullptr;
-                // element q + parts i of the caller's array <-> element off[q] + i of the gathered one
-                auto move = [&](void* gath, const void* user, size_t w, bool in) {
-                    for (int q = 0; q < parts; ++q) {
-                        char* gq = static_cast<char*>(gath) + w * off[q];
-                        char* uq = static_cast<char*>(const_cast<void*>(user)) + w * q;
-                        hipcheck(c, in ? hipMemcpy2DAsync(gq, w, uq, w * parts, w, cnt[q], hipMemcpyDeviceToDevice,
-                                                          c->stream)
-                                       : hipMemcpy2DAsync(uq, w * parts, gq, w, w, cnt[q], hipMemcpyDeviceToDevice,
-                                                          c->stream),
-                                 "point gather");
-                    }
-                };
-                move(const_cast<double*>(g.E), a.E, sizeof(double), true);
-                if (a.kidx) move(const_cast<int*>(g.kidx), a.kidx, sizeof(int), true);
-                if (a.first_mask) move(const_cast<int*>(g.first_mask), a.first_mask, sizeof(int), true);
-                for (int q = 0; q < parts; ++q) {
-                    part[q] = point_part(g, off[q], cnt[q]);
-                    jobs[q] = &part[q];
-                }
-                run_on_workers(c, p, parts, jobs);
-                move(g.T_LR, a.T_LR, sizeof(double), false);
-                move(g.T_RL, a.T_RL, sizeof(double), false);
-                if (a.status) move(g.status, a.status, sizeof(int), false);
-                if (a.iters) move(g.iters, a.iters, sizeof(int), false);
-                return 0;
-            }
-            for (int q = 0; q < parts; ++q) {
-                part[q] = point_part(a, off[q], cnt[q]);
-                jobs[q] = &part[q];
-            }
-            run_on_workers(c, p, parts, jobs);
+            solve_split(c, p, a, parts);
             return 0;
         }
         solve_tdb_dev(c, p, a);
@@ -2557,216 +2312,15 @@ int nt_solve_tdb_adjoint_batched_dev(nt_ctx* c, const nt_params* p, int nl, int 
         a.dHd = dW_dHamd;
         a.dHdv = nc > 0 ? dW_dHamdv : nullptr;
         // many points: parts solved side by side by worker contexts, as the
-        // batched solve; every parts-th point (gathered in the context's
-        // workspace, results scattered back), part 0 accumulating into the
-        // caller's gradients, the others into zeroed buffers added at the end
+        // batched solve (the adjoint sweep runs in FP64 whatever the precision)
         const int parts = c->point_split;
         if (parts > 1 && npts >= parts * NT_SPLIT_MIN_POINTS) {
-            if (const int st = prepare_workers(c, parts)) {
-                c->err = "could not set up the worker contexts of a split solve";
-                throw Fail{st};
-            }
-            std::vector<int> cnt(parts), off(parts);
-            for (int q = 0, p0 = 0; q < parts; ++q) {
-                cnt[q] = npts / parts + (q < npts % parts ? 1 : 0);
-                off[q] = p0;
-                p0 += cnt[q];
-            }
-            const size_t nd = sizeof(double) * npts, ni = sizeof(int) * npts;
-            const size_t gd = z * bb * (hi - lo + 1) * nk, gv = nc > 0 ? z * bb * nc * nk : 0;
-            Arena pa;
-            const size_t oE = pa.take(nd), oW = pa.take(nd), oTl = pa.take(nd), oTr = pa.take(nd),
-                         oK = pa.take(ni), oM = pa.take(ni), oS = pa.take(ni), oI = pa.take(ni),
-                         oG = pa.take(gd * (parts - 1)), oGv = pa.take(gv * (parts - 1));
-            ensure_ws(c, pa.off);
-            pa.base = static_cast<char*>(c->ws);
-            TdbArgs g = a;
-            g.E = pa.ptr<double>(oE);
-            g.adj_w = pa.ptr<double>(oW);
-            g.T_LR = pa.ptr<double>(oTl);
-            g.T_RL = pa.ptr<double>(oTr);
-            g.kidx = kidx ? pa.ptr<int>(oK) : nullptr;
-            g.first_mask = first_point_mask ? pa.ptr<int>(oM) : nullptr;
-            g.status = status ? pa.ptr<int>(oS) : nullptr;
-            g.iters = iters ? pa.ptr<int>(oI) : nullptr;
-            auto move = [&](void* gath, const void* user, size_t w, bool in) {
-                for (int q = 0; q < parts; ++q) {
-                    char* gq = static_cast<char*>(gath) + w * off[q];
-                    char* uq = static_cast<char*>(const_cast<void*>(user)) + w * q;
-                    hipcheck(c, in ? hipMemcpy2DAsync(gq, w, uq, w * parts, w, cnt[q], hipMemcpyDeviceToDevice,
-                                                      c->stream)
-                                   : hipMemcpy2DAsync(uq, w * parts, gq, w, w, cnt[q], hipMemcpyDeviceToDevice,
-                                                      c->stream),
-                             "point gather");
-                }
-            };
-            move(const_cast<double*>(g.E), E, sizeof(double), true);
-            move(const_cast<double*>(g.adj_w), weight, sizeof(double), true);
-            if (kidx) move(const_cast<int*>(g.kidx), kidx, sizeof(int), true);
-            if (first_point_mask) move(const_cast<int*>(g.first_mask), first_point_mask, sizeof(int), true);
-            zero_dev(c, pa.ptr<char>(oG), gd * (parts - 1), c->stream);
-            if (gv) zero_dev(c, pa.ptr<char>(oGv), gv * (parts - 1), c->stream);
-            std::vector<TdbArgs> part(parts);
-            std::vector<const TdbArgs*> jobs(parts);
-            for (int q = 0; q < parts; ++q) {
-                part[q] = point_part(g, off[q], cnt[q]);
-                part[q].adj_w = g.adj_w + off[q];
-                if (q > 0) {
-                    part[q].dHd = reinterpret_cast<nt_c128*>(pa.ptr<char>(oG) + gd * (q - 1));
-                    part[q].dHdv = gv ? reinterpret_cast<nt_c128*>(pa.ptr<char>(oGv) + gv * (q - 1)) : nullptr;
-                }
-                jobs[q] = &part[q];
-            }
-            run_on_workers(c, p, parts, jobs);
-            move(g.T_LR, T_LR, sizeof(double), false);
-            move(g.T_RL, T_RL, sizeof(double), false);
-            if (status) move(g.status, status, sizeof(int), false);
-            if (iters) move(g.iters, iters, sizeof(int), false);
-            for (int q = 1; q < parts; ++q) {
-                const long n1 = (long)(gd / sizeof(double)), n2 = (long)(gv / sizeof(double));
-                hipLaunchKernelGGL(k_add_doubles, dim3(nblk(n1)), dim3(256), 0, c->stream,
-                                   reinterpret_cast<double*>(dW_dHamd), pa.ptr<double>(oG + gd * (q - 1)), n1);
-                if (gv)
-                    hipLaunchKernelGGL(k_add_doubles, dim3(nblk(n2)), dim3(256), 0, c->stream,
-                                       reinterpret_cast<double*>(dW_dHamdv), pa.ptr<double>(oGv + gv * (q - 1)), n2);
-                hipcheck(c, hipGetLastError(), "gradient sum");
-            }
+            solve_split(c, p, a, parts);
             return 0;
         }
         solve_tdb_dev(c, p, a);
         return 0;
     });
-}
-
-int nt_solve_tdb_multi_dev(nt_ctx* c, const nt_params* p, int ndev, const nt_device_desc* devs)
-{
-    if (!c || ndev < 0 || (ndev > 0 && !devs) || check_params(p)) return NT_ERR_ARG;
-    for (int d = 0; d < ndev; ++d) {
-        const nt_device_desc& D = devs[d];
-        if (check_circular(p, D.nl, D.nr)) return NT_ERR_ARG;
-        if (D.nl <= 0 || D.nr <= 0 || D.b <= 0 || D.L <= 0 || D.npts < 0 || D.nk <= 0 || (D.nk > 1 && !D.kidx))
-            return NT_ERR_ARG;
-        if (std::max(std::max(D.nl, D.nr), D.b) > inv_max_n()) return NT_ERR_ARG;
-        if (D.npts > 0 && (!D.E || !D.Haml || !D.Hamlv || !D.Hamr || !D.Hamrv || !D.Hamd || (D.L > 1 && !D.Hamdv) ||
-                           !D.Hamld || !D.Hamrd || !D.T_LR || !D.T_RL))
-            return NT_ERR_ARG;
-    }
-    // largest first: b^3 (L + 2 N_dec) per point, N_dec ~ 15
-    std::vector<int> order(ndev);
-    for (int d = 0; d < ndev; ++d) order[d] = d;
-    auto cost = [&](int d) {
-        const double b = devs[d].b;
-        return b * b * b * (devs[d].L + 30.0) * devs[d].npts;
-    };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
-    // Devices with the same lead buffers, lead sizes and points (E and k
-    // map) form a group whose lead Green's functions are computed once (FP64
-    // solves; a mixed-precision device is a group of its own).
-    auto same_leads = [&](const nt_device_desc& x, const nt_device_desc& y) {
-        return x.nl == y.nl && x.nr == y.nr && x.nk == y.nk && x.npts == y.npts && x.Haml == y.Haml &&
-               x.Hamlv == y.Hamlv && x.Hamr == y.Hamr && x.Hamrv == y.Hamrv && x.E == y.E && x.kidx == y.kidx;
-    };
-    std::vector<std::vector<int>> groups;
-    for (int d : order) {
-        if (devs[d].npts == 0) continue;
-        bool placed = false;
-        if (p->precision == NT_PREC_FP64)
-            for (auto& g : groups)
-                if (same_leads(devs[g[0]], devs[d])) {
-                    g.push_back(d);
-                    placed = true;
-                    break;
-                }
-        if (!placed) groups.push_back({d});
-    }
-    std::vector<std::pair<int, int>> tuned;   // (bytes, n) already tuned this call
-    const int bytes = p->precision == NT_PREC_FP64 ? 8 : 4;
-    for (const auto& g : groups)
-        for (int d : g) {
-            const nt_device_desc& D = devs[d];
-            for (int n : {D.b, std::max(D.nl, D.nr)}) {
-                if (c->inv_nb.count({bytes, n}) ||
-                    std::find(tuned.begin(), tuned.end(), std::make_pair(bytes, n)) != tuned.end())
-                    continue;
-                const int tb = std::min(D.npts, c->batch_limit);
-                const int st = nt_autotune_inverse(c, n, std::max(1, std::min(tb, 256)), p->precision, nullptr,
-                                                   nullptr);
-                if (st) return st;
-                tuned.push_back({bytes, n});
-            }
-        }
-    const int ng = (int)groups.size();
-    const int W = std::min(std::max(1, c->multi_workers), ng);
-    if (W > 1) {
-        if (const int st = prepare_workers(c, W)) return st;
-    }
-    return guarded(c, [&] {
-        const size_t z = sizeof(nt_c128);
-        std::vector<std::vector<TdbArgs>> args(ng);
-        for (int gi = 0; gi < ng; ++gi) {
-            auto& a = args[gi];
-            a.reserve(groups[gi].size());
-            for (int d : groups[gi]) {
-                const nt_device_desc& D = devs[d];
-                const size_t nll = (size_t)D.nl * D.nl, nrr = (size_t)D.nr * D.nr, bb = (size_t)D.b * D.b;
-                reg_region(c, D.E, sizeof(double) * D.npts);
-                reg_region(c, D.kidx, sizeof(int) * D.npts);
-                reg_region(c, D.Haml, z * nll * D.nk);
-                reg_region(c, D.Hamlv, z * nll * D.nk);
-                reg_region(c, D.Hamr, z * nrr * D.nk);
-                reg_region(c, D.Hamrv, z * nrr * D.nk);
-                reg_region(c, D.Hamd, z * bb * D.L * D.nk);
-                reg_region(c, D.Hamdv, z * bb * std::max(D.L - 1, 1) * D.nk);
-                reg_region(c, D.Hamld, z * D.nl * D.b * D.nk);
-                reg_region(c, D.Hamrd, z * D.nr * D.b * D.nk);
-                reg_region(c, D.first_point_mask, sizeof(int) * D.npts);
-                reg_region(c, D.T_LR, sizeof(double) * D.npts);
-                reg_region(c, D.T_RL, sizeof(double) * D.npts);
-                reg_region(c, D.status, sizeof(int) * D.npts);
-                a.push_back(TdbArgs{D.nl, D.nr, D.b, D.L, D.npts, D.nk, D.E, D.kidx, D.Haml, D.Hamlv, D.Hamr,
-                                    D.Hamrv, D.Hamd, D.Hamdv, D.Hamld, D.Hamrd, D.first_point_mask, D.T_LR, D.T_RL,
-                                    D.status, nullptr, nt_solve_extras{}});
-            }
-            for (size_t i = 1; i < a.size(); ++i) a[0].siblings.push_back(&a[i]);
-            // every k index must name one of the group's nk block sets
-            const nt_device_desc& D0 = devs[groups[gi][0]];
-            if (D0.kidx) {
-                std::vector<int> hk(D0.npts);
-                hipcheck(c, hipMemcpyAsync(hk.data(), D0.kidx, sizeof(int) * D0.npts, hipMemcpyDeviceToHost,
-                                           c->stream), "D2H");
-                hipcheck(c, hipStreamSynchronize(c->stream), "sync");
-                for (int v : hk)
-                    if (v < 0 || v >= D0.nk) {
-                        c->err = "kidx out of range [0, nk)";
-                        return (int)NT_ERR_ARG;
-                    }
-            }
-        }
-        if (W <= 1) {
-            for (int gi = 0; gi < ng; ++gi) solve_tdb_dev(c, p, args[gi][0]);
-            return 0;
-        }
-        // Groups go to W worker contexts, largest first: small devices, whose
-        // launches do not fill the GPU, run side by side.
-        std::vector<const TdbArgs*> jobs(ng);
-        for (int gi = 0; gi < ng; ++gi) jobs[gi] = &args[gi][0];
-        run_on_workers(c, p, W, jobs);
-        return 0;
-    });
-}
-
-int nt_set_multi_workers(nt_ctx* c, int n)
-{
-    if (!c || n < 1 || n > 16) return NT_ERR_ARG;
-    c->multi_workers = n;
-    return NT_OK;
-}
-
-int nt_set_point_split(nt_ctx* c, int n)
-{
-    if (!c || n < 1 || n > 8) return NT_ERR_ARG;
-    c->point_split = n;
-    return NT_OK;
 }
 
 int nt_solve_tdb_batched(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L, int npts, const double* E,

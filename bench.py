@@ -74,6 +74,8 @@ def main():
                     help="before timing, run one device-to-device copy of this many MiB (HBM-counter calibration "
                          "under rocprofv3 --pmc)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false")
+    ap.add_argument("--point-split", type=int, default=2,
+                    help="parts a solve of many points runs as, side by side on worker streams (1 = one stream)")
     ap.add_argument("--serial-sweeps", action="store_true",
                     help="every launch on one stream in every step: the two RGF sweeps (with two sweeps) and the "
                          "parts of a split solve (for per-kernel profiles)")
@@ -132,9 +134,9 @@ def main():
     E = torch.as_tensor(omega ** 2, dtype=torch.float64, device=dev)
 
     eng = T.Engine(0, batch_limit=args.points)
+    eng.set_point_split(1 if args.serial_sweeps else args.point_split)
     if args.serial_sweeps:
         eng.set_overlap(False)
-        eng.set_point_split(1)
     dblocks = [eng.upload_blocks(b) for b in blocks]
     torch.cuda.synchronize()
     if args.copy_probe_mib:
@@ -351,7 +353,7 @@ def main():
         eng.solve_tdb(dblocks[args.warmup], E, p)
         eng.profile(False)
         eng.set_overlap(not args.serial_sweeps)
-        eng.set_point_split(1 if args.serial_sweeps else 2)
+        eng.set_point_split(1 if args.serial_sweeps else args.point_split)
         return {"gemm": eng.profile_get(N.NT_K_GEMM), "inverse": eng.profile_get(N.NT_K_INV)}
 
     iso64 = isolated_kernels("fp64")
@@ -833,9 +835,9 @@ def main():
                      "timing": "HIP events around every launch on the launching stream, in one extra step with "
                                "every launch on one stream (no kernel shares the GPU with another)",
                      "concurrent": {"avg_launch_ms": avg_ms_conc, "achieved": achieved_conc,
-                                    "what": "the same kernel in the timed steps, where the step's points run as "
-                                            "two parts side by side on two worker streams (nt_set_point_split) "
-                                            "and launches share the GPU"},
+                                    "what": f"the same kernel in the timed steps, where the step's points run as "
+                                            f"{args.point_split} part(s) side by side on worker streams "
+                                            f"(nt_set_point_split) and launches share the GPU"},
                      "inverse_isolated": {"avg_launch_ms": iso64["inverse"]["ms"] / max(1, iso64["inverse"]["launches"]),
                                           "tflops": iso64["inverse"]["flops"] / max(1e-9, iso64["inverse"]["ms"] * 1e-3) / 1e12}},
         "precision_compare": {"fp64_solves_per_s": v64, "mixed_solves_per_s": vmx,

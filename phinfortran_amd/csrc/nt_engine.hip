@@ -1115,17 +1115,6 @@ void rgf_run(nt_ctx* c, int nl, int nr, const RgfInput<T>& in, const T* zd, int 
 
 // ------------------------------------------------------- Buttiker probes
 // Probe outputs of a batched solve (negf/lesser.f90:1199-1503).
-}  // namespace
-
-// the probe path's inputs and outputs (device pointers), carried by TdbArgs
-struct ProbeSpec {
-    const double* temp;       // L probe temperatures (device)
-    double t_left, t_right, B, C;
-    int lifetime;
-    double *flux, *jac, *flux_left, *flux_right, *a_sum, *gn_sum;
-};
-
-namespace {
 
 // complex traces kept per point by buttiker_run
 inline long probe_acc_n(int L) { return (long)L * L + 5 * (long)L + 6; }
@@ -2207,7 +2196,9 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
         // the FP64 solve of many points in parts solved side by side by worker
         // contexts: the latency-bound launches of one part (inverses,
         // decimation steps of few unconverged items) leave compute units to
-        // the other's GEMMs
+        // the other's GEMMs.  Not the probe solve, whose grouped GEMMs fill
+        // the GPU by themselves (two parts: 287 against 297 probe solves/s at
+        // b = 192, L = 40, 128 points)
         const int parts = c->point_split;
         if (parts > 1 && !noalloc && !pd && p->precision == NT_PREC_FP64 && npts >= parts * NT_SPLIT_MIN_POINTS) {
             solve_split(c, p, a, parts);

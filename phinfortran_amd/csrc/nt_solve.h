@@ -6,7 +6,14 @@
 #include <vector>
 #include "nt_host.h"
 
-struct ProbeSpec;
+// the probe path's inputs and outputs (device pointers; outputs per point:
+// flux, a_sum, gn_sum L doubles, jac L x L, flux_left, flux_right one)
+struct ProbeSpec {
+    const double* temp;       // L probe temperatures
+    double t_left, t_right, B, C;
+    int lifetime;
+    double *flux, *jac, *flux_left, *flux_right, *a_sum, *gn_sum;
+};
 
 struct TdbArgs {
     int nl, nr, b, L, npts, nk;

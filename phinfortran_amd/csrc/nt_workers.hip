@@ -199,6 +199,10 @@ void solve_split(nt_ctx* c, const nt_params* p, const TdbArgs& a, int parts)
                             x.sr_ret || x.G_D_LR || x.G_D_RL || x.G_L || x.G_R || x.sdos_l || x.sdos_r || x.ldos ||
                             x.ddos;
     const bool adj = a.adj_w != nullptr;
+    if (a.probe || !a.siblings.empty()) {
+        c->err = "a split solve takes no probes or sibling devices";
+        throw Fail{NT_ERR_ARG};
+    }
     if (extras_any) {
         if (adj) {
             c->err = "a split adjoint solve takes no extras";

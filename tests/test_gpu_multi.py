@@ -182,3 +182,4 @@ def test_point_split_adjoint(engine, T):
         a, b = ref[k].cpu().numpy(), got[k].cpu().numpy()
         assert np.abs(a).max() > 0
         assert np.abs(a - b).max() < 1e-12 * np.abs(a).max(), k
+

@@ -278,7 +278,7 @@ int nt_solve_tdb_multi_dev(nt_ctx* ctx, const nt_params* p, int ndev, const nt_d
  * group on the context itself, one after the other).  No reference
  * counterpart. */
 int nt_set_multi_workers(nt_ctx* ctx, int n);
-/* Batched FP64 solves (nt_solve_tdb_batched[_dev]) of at least
+/* Batched solves (nt_solve_tdb_batched[_dev], FP64 or mixed) of at least
  * n * 256 points run as n parts solved side by side by worker contexts of
  * ctx (own stream, workspace and host thread each, ordered after and before
  * the caller's stream): the latency-bound launches of one part (the
@@ -292,8 +292,8 @@ int nt_set_multi_workers(nt_ctx* ctx, int n);
  * the others into zeroed workspace summed into them at the end (the
  * gradients then differ from one part's in the rounding of that order).
  * n = 2 (default) .. 8; n = 1: every point on the context's own stream.
- * The allocation-free entry, probes and NT_PREC_MIXED always run on one
- * stream.  No counterpart in the reference. */
+ * The allocation-free entry and probes always run on one stream.  No
+ * counterpart in the reference. */
 int nt_set_point_split(nt_ctx* ctx, int n);
 
 /* Host-pointer variant (copies in and out; same layout). */

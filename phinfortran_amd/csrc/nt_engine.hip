@@ -2200,7 +2200,7 @@ static int tdb_entry(nt_ctx* c, const nt_params* p, int nl, int nr, int b, int L
         // the GPU by themselves (two parts: 287 against 297 probe solves/s at
         // b = 192, L = 40, 128 points)
         const int parts = c->point_split;
-        if (parts > 1 && !noalloc && !pd && p->precision == NT_PREC_FP64 && npts >= parts * NT_SPLIT_MIN_POINTS) {
+        if (parts > 1 && !noalloc && !pd && npts >= parts * NT_SPLIT_MIN_POINTS) {
             solve_split(c, p, a, parts);
             return 0;
         }

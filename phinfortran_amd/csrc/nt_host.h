@@ -88,7 +88,7 @@ struct nt_ctx {
     // uneven-device calls: worker contexts (own stream and workspace, one host
     // thread each) that solve groups of devices concurrently
     int multi_workers = 4;
-    // batched FP64 solves of at least point_split * NT_SPLIT_MIN_POINTS points
+    // batched solves of at least point_split * NT_SPLIT_MIN_POINTS points
     // run as point_split parts side by side on worker contexts (1: one stream)
     int point_split = 2;
     std::vector<nt_ctx*> subs;

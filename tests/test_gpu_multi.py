@@ -183,3 +183,33 @@ def test_point_split_adjoint(engine, T):
         assert np.abs(a).max() > 0
         assert np.abs(a - b).max() < 1e-12 * np.abs(a).max(), k
 
+
+
+def test_point_split_mixed(engine, T):
+    """The mixed-precision solve split in two interleaved parts: T, statuses,
+    iteration counts bitwise those of one stream, and the refinement counters
+    (second steps, FP64 recomputes) summed over the parts equal."""
+    import torch
+    ks = lt.mp_grid(2, 2)
+    blocks = [lt.si_ge_superlattice_film(2, 2, 6, kf, seed=7) for kf in ks]
+    d = engine.upload_blocks(blocks)
+    npts = 600
+    f = 16.0 * (np.arange(npts) + 0.5) / npts
+    E = f ** 2
+    kidx = (np.arange(npts) % len(ks)).astype(np.int32)
+    p = T.params("phonon", precision="mixed")
+    print("split mixed solve, 2 parts of 600 points", flush=True)
+    try:
+        engine.set_point_split(1)
+        c0 = np.array(engine.mixed_counts())
+        ref = engine.solve_tdb(d, E, p, kidx=kidx)
+        c1 = np.array(engine.mixed_counts())
+        engine.set_point_split(2)
+        got = engine.solve_tdb(d, E, p, kidx=kidx)
+        c2 = np.array(engine.mixed_counts())
+    finally:
+        engine.set_point_split(2)
+    assert (ref["status"].cpu().numpy() == 0).all()
+    for k in ("T_LR", "T_RL", "status", "iters"):
+        assert torch.equal(ref[k], got[k]), k
+    assert np.array_equal(c1 - c0, c2 - c1), (c1 - c0, c2 - c1)

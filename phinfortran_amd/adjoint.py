@@ -256,9 +256,9 @@ def maximize_conductance(film: InterfaceFilm, beta, steps: int = 50, lr: float =
         G = conductance(x, film)
         (-G).backward()
         opt.step()
-        hist.append(float(G))
+        hist.append(float(G.detach()))
         if callback:
-            callback(it, float(G))
+            callback(it, hist[-1])
     return x.detach(), hist
 
 

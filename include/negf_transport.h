@@ -283,11 +283,10 @@ int nt_set_multi_workers(nt_ctx* ctx, int n);
  * ctx (own stream, workspace and host thread each, ordered after and before
  * the caller's stream): the latency-bound launches of one part (the
  * inverses, the late decimation steps of few unconverged items) leave
- * compute units to the other's GEMMs.  Without extras part q takes every
- * n-th point from q (gathered in the context's workspace and scattered
- * back, so the parts need similar numbers of decimation steps); with
- * extras the parts are contiguous ranges.  Results are bitwise those of one
- * part.  The adjoint solve (nt_solve_tdb_adjoint_batched_dev) splits the
+ * compute units to the other's GEMMs.  Part q takes every n-th point from
+ * q (inputs, T, statuses and every extra output gathered in the context's
+ * workspace and scattered back, so the parts need similar numbers of
+ * decimation steps).  Results are bitwise those of one part.  The adjoint solve (nt_solve_tdb_adjoint_batched_dev) splits the
  * same way, interleaved; part 0 accumulates into the caller's gradients,
  * the others into zeroed workspace summed into them at the end (the
  * gradients then differ from one part's in the rounding of that order).

@@ -4,6 +4,7 @@
 // block-size groups of nt_solve_tdb_multi_dev (nt_set_multi_workers).
 #include <algorithm>
 #include <atomic>
+#include <functional>
 #include <mutex>
 #include <thread>
 #include "nt_host.h"
@@ -35,6 +36,16 @@ static int prepare_workers(nt_ctx* c, int W)
     const size_t share = std::max<size_t>(size_t(2) << 30, (size_t)(0.6 * (double)avail / W));
     for (int w = 0; w < W; ++w) {
         nt_ctx* s = c->subs[w];
+        if (s->ws && s->ws_bytes > share + share / 8) {
+            // a workspace grown by an earlier call beyond this call's share
+            // (and the eighth ensure_ws adds) would leave the other workers
+            // less than theirs: release it
+            if (hipSetDevice(s->device) != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess ||
+                hipFree(s->ws) != hipSuccess)
+                return NT_ERR_HIP;
+            s->ws = nullptr;
+            s->ws_bytes = 0;
+        }
         s->ws_cap = share;
         s->overlap = c->overlap;
         s->gemm_m3 = c->gemm_m3;
@@ -172,11 +183,11 @@ static TdbArgs point_part(const TdbArgs& a, int p0, int n)
     return t;
 }
 
-// Without extras part q takes the points q, q + parts, ... (gathered in c's
+// Part q takes the points q, q + parts, ... (gathered in c's
 // workspace by strided copies, the results scattered back): neighbouring
 // energies need similar numbers of decimation steps, so interleaved parts
-// take equal times.  With extras the parts are contiguous ranges of the
-// caller's buffers.  An adjoint solve accumulates part 0's gradients into the
+// take equal times; the extra outputs are gathered and scattered the same
+// way.  An adjoint solve accumulates part 0's gradients into the
 // caller's and the other parts' into zeroed workspace, summed into the
 // caller's after the join.
 void solve_split(nt_ctx* c, const nt_params* p, const TdbArgs& a, int parts)
@@ -203,17 +214,9 @@ void solve_split(nt_ctx* c, const nt_params* p, const TdbArgs& a, int parts)
         c->err = "a split solve takes no probes or sibling devices";
         throw Fail{NT_ERR_ARG};
     }
-    if (extras_any) {
-        if (adj) {
-            c->err = "a split adjoint solve takes no extras";
-            throw Fail{NT_ERR_ARG};
-        }
-        for (int q = 0; q < parts; ++q) {
-            part[q] = point_part(a, off[q], cnt[q]);
-            jobs[q] = &part[q];
-        }
-        run_on_workers(c, p, parts, jobs);
-        return;
+    if (extras_any && adj) {
+        c->err = "a split adjoint solve takes no extras";
+        throw Fail{NT_ERR_ARG};
     }
     const size_t nd = sizeof(double) * npts, ni = sizeof(int) * npts;
     const size_t z = sizeof(nt_c128), bb = (size_t)a.b * a.b;
@@ -224,9 +227,36 @@ void solve_split(nt_ctx* c, const nt_params* p, const TdbArgs& a, int parts)
     const size_t oE = pa.take(nd), oW = pa.take(adj ? nd : 0), oTl = pa.take(nd), oTr = pa.take(nd),
                  oK = pa.take(ni), oM = pa.take(ni), oS = pa.take(ni), oI = pa.take(ni),
                  oG = pa.take(gd * (parts - 1)), oGv = pa.take(gv * (parts - 1));
+    // the extra outputs, gathered the same way: (the gathered copy's pointer
+    // in g, the caller's buffer, bytes per point, workspace offset)
+    TdbArgs g = a;
+    struct ExOut {
+        void* gath;
+        std::function<void(char*)> set;
+        const void* user;
+        size_t w, off;
+    };
+    std::vector<ExOut> exo;
+    auto ex = [&](auto*& gp, size_t w) {
+        if (!gp) return;
+        exo.push_back({nullptr, [&gp](char* b) { gp = reinterpret_cast<std::remove_reference_t<decltype(gp)>>(b); },
+                       gp, w, pa.take(w * npts)});
+    };
+    {
+        const size_t nll = (size_t)a.nl * a.nl, nrr = (size_t)a.nr * a.nr, nlr = (size_t)a.nl * a.nr;
+        nt_solve_extras& e = g.ex;
+        ex(e.gl_adv, z * nll), ex(e.gr_ret, z * nrr), ex(e.gl_plus, z * nll), ex(e.gr_minus, z * nrr);
+        ex(e.gml_adv, z * nll), ex(e.gmr_ret, z * nrr), ex(e.sl_adv, z * nll), ex(e.sr_ret, z * nrr);
+        ex(e.G_D_LR, z * nlr), ex(e.G_D_RL, z * nlr), ex(e.G_L, z * nll), ex(e.G_R, z * nrr);
+        ex(e.sdos_l, sizeof(double)), ex(e.sdos_r, sizeof(double));
+        ex(e.ldos, sizeof(double) * a.L * a.b), ex(e.ddos, sizeof(double));
+    }
     ensure_ws(c, pa.off);
     pa.base = static_cast<char*>(c->ws);
-    TdbArgs g = a;
+    for (ExOut& o : exo) {
+        o.gath = pa.ptr<char>(o.off);
+        o.set(pa.ptr<char>(o.off));
+    }
     g.E = pa.ptr<double>(oE);
     g.adj_w = adj ? pa.ptr<double>(oW) : nullptr;
     g.T_LR = pa.ptr<double>(oTl);
@@ -267,6 +297,7 @@ void solve_split(nt_ctx* c, const nt_params* p, const TdbArgs& a, int parts)
     move(g.T_RL, a.T_RL, sizeof(double), false);
     if (a.status) move(g.status, a.status, sizeof(int), false);
     if (a.iters) move(g.iters, a.iters, sizeof(int), false);
+    for (const ExOut& o : exo) move(o.gath, o.user, o.w, false);
     for (int q = 1; q < parts && adj; ++q) {
         const long n1 = (long)(gd / sizeof(double)), n2 = (long)(gv / sizeof(double));
         hipLaunchKernelGGL(k_add_doubles, dim3(nblk(n1)), dim3(256), 0, c->stream, reinterpret_cast<double*>(a.dHd),

@@ -89,9 +89,9 @@ def test_point_split_equals_one_stream(engine, T, parts, npts, with_extras):
     """A batched FP64 solve of at least parts x 256 points runs as parts
     solved side by side by worker contexts (nt_set_point_split): bitwise the
     T, statuses, iteration counts and extras of the solve on one stream, and
-    the caller's stream sees them without a further synchronisation.  With
-    extras the parts are contiguous ranges; without, every parts-th point
-    (gathered and scattered back)."""
+    the caller's stream sees them without a further synchronisation.  Part q
+    takes every parts-th point from q (inputs, results and extras gathered
+    and scattered back)."""
     import torch
     ks = lt.mp_grid(2, 2)
     blocks = [lt.si_ge_superlattice_film(2, 2, 6, kf, seed=7) for kf in ks]

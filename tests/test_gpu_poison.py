@@ -79,6 +79,8 @@ def test_split_solve(engine, T, poison_off):
     p = T.params("phonon", eta=1e-3, eta0=1e-4)
     r = _both(engine, "split solve", lambda: engine.solve_tdb(d, E, p, kidx=kidx))
     assert (r["status"].cpu().numpy() == 0).all()
+    ex = ("G_D_RL", "gl_adv", "sdos_l", "ldos")
+    _both(engine, "split solve with extras", lambda: engine.solve_tdb(d, E, p, kidx=kidx, extras=ex))
     # the adjoint solve split the same way (the second part's gradients in
     # zeroed workspace, summed into the caller's)
     w = np.linspace(0.5, 2.0, 600)
